@@ -1,0 +1,752 @@
+// gi_api.hip -- gi_context and the C ABI declared in include/gi_intersector.h.
+//
+// One context owns one HIP stream, both meshes (host copy + device state),
+// the candidate/exact/result arenas and the events used for stage timings.
+// A query runs the whole device pipeline asynchronously and synchronises
+// once, to read the counters (capacity check, grid/non-grid branch).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/gi_intersector.h"
+#include "gi_device.h"
+#include "host_exact.h"
+
+namespace {
+
+using namespace gi;
+
+struct GiError {
+    int status;
+    std::string msg;
+};
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw GiError{e_ == hipErrorOutOfMemory ? GI_ERR_OUT_OF_MEMORY : GI_ERR_HIP,               \
+                          std::string(#expr) + ": " + hipGetErrorString(e_)};                          \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* ensure(size_t n) {
+        size_t need = std::max<size_t>(n, 1) * sizeof(T);
+        if (bytes < need) {
+            release();
+            HIP_TRY(hipMalloc(&p, need));
+            bytes = need;
+        }
+        return (T*)p;
+    }
+    template <class T>
+    T* get() const { return (T*)p; }
+};
+
+struct MeshSlot {
+    std::vector<double> h_xyz;
+    std::vector<uint32_t> h_tri;
+    uint32_t nv = 0, nt = 0;
+    bool set = false;
+    bool on_device = false;  // device buffers hold the mesh
+    bool built = false;      // BVH topology is valid
+    bool dirty = true;       // vertices/boxes/BVH boxes need an update
+    Xform xf;
+    DevBuf src, tri, v64, vi, lo, hi, keys, vals, tkeys, tvals, nodes, parent, children, flags;
+    void release() {
+        DevBuf* all[] = {&src, &tri, &v64, &vi, &lo, &hi, &keys, &vals, &tkeys, &tvals, &nodes, &parent, &children, &flags};
+        for (DevBuf* b : all) b->release();
+        on_device = false;
+        built = false;
+        dirty = true;
+    }
+};
+
+Xform identity_xform() {
+    Xform x;
+    memset(&x, 0, sizeof(x));
+    x.m[0][0] = x.m[1][1] = x.m[2][2] = 1.0;
+    x.identity = 1;
+    return x;
+}
+
+int bits_for(uint32_t n) { return n <= 1 ? 1 : 32 - __builtin_clz(n - 1); }
+
+}  // namespace
+
+struct gi_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int host_threads = 0;
+    uint32_t cand_reserve = 0;
+    MeshSlot mesh[2];
+    DevBuf grid, bounds, counters, hist, scan_tmp;
+    DevBuf cand, exact, rec_key, rec_seg, seg_pool;
+    uint32_t cap_cand = 0, cap_exact = 0, cap_rec = 0, cap_seg = 0;
+    DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2;
+    hipEvent_t ev[8] = {};
+    void release_device() {
+        for (auto& m : mesh) m.release();
+        DevBuf* all[] = {&grid, &bounds, &counters, &hist, &scan_tmp, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
+                         &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2};
+        for (DevBuf* b : all) b->release();
+        cap_cand = cap_exact = cap_rec = cap_seg = 0;
+    }
+};
+
+namespace {
+
+struct HostResultStore {
+    std::vector<uint32_t> pairs, offs;
+    std::vector<uint64_t> seg;
+    std::vector<int32_t> exp2;
+    std::vector<double> f64;
+};
+
+int fail(gi_context* ctx, const GiError& e) {
+    if (ctx) ctx->err = e.msg;
+    return e.status;
+}
+
+void upload_mesh(gi_context* ctx, int slot) {
+    MeshSlot& m = ctx->mesh[slot];
+    double* src = m.src.ensure<double>(3 * (size_t)m.nv);
+    uint32_t* tri = m.tri.ensure<uint32_t>(3 * (size_t)m.nt);
+    if (m.nv) HIP_TRY(hipMemcpyAsync(src, m.h_xyz.data(), sizeof(double) * 3 * m.nv, hipMemcpyHostToDevice, ctx->stream));
+    if (m.nt) HIP_TRY(hipMemcpyAsync(tri, m.h_tri.data(), sizeof(uint32_t) * 3 * m.nt, hipMemcpyHostToDevice, ctx->stream));
+    m.on_device = true;
+    m.built = false;
+    m.dirty = true;
+}
+
+// vertices -> boxes -> (Morton sort + Karras build | leaf refresh) -> refit
+void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
+    MeshSlot& m = ctx->mesh[slot];
+    if (!m.on_device) upload_mesh(ctx, slot);
+    if (!m.dirty && m.built && !rebuild) return;
+    hipStream_t s = ctx->stream;
+    uint32_t* grid = ctx->grid.ensure<uint32_t>(2);
+    uint32_t* bounds = ctx->bounds.ensure<uint32_t>(12) + 6 * slot;
+    size_t nv = m.nv, nt = m.nt;
+    double4* v64 = m.v64.ensure<double4>(nv);
+    int4* vi = m.vi.ensure<int4>(nv);
+    float4* lo = m.lo.ensure<float4>(nt);
+    float4* hi = m.hi.ensure<float4>(nt);
+    size_t nnodes = nt ? 2 * nt - 1 : 1;
+    float4* nodes = m.nodes.ensure<float4>(2 * nnodes);
+    uint32_t* parent = m.parent.ensure<uint32_t>(nnodes);
+    uint2* children = m.children.ensure<uint2>(nt);
+    uint32_t* flags = m.flags.ensure<uint32_t>(nt);
+    launch_fill_u32(grid + slot, 1u, 1, s);
+    launch_prepare_vertices(m.src.get<double>(), m.nv, m.xf, v64, vi, grid + slot, s);
+    launch_fill_u32(bounds, 0xFFFFFFFFu, 3, s);
+    launch_fill_u32(bounds + 3, 0u, 3, s);
+    launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, lo, hi, bounds, s);
+    if (!m.built || rebuild) {
+        uint32_t* keys = m.keys.ensure<uint32_t>(nt);
+        uint32_t* vals = m.vals.ensure<uint32_t>(nt);
+        uint32_t* tk = m.tkeys.ensure<uint32_t>(nt);
+        uint32_t* tv = m.tvals.ensure<uint32_t>(nt);
+        uint32_t* hist = ctx->hist.ensure<uint32_t>(sort_hist_elems(nt));
+        uint32_t* stmp = ctx->scan_tmp.ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
+        launch_morton(lo, hi, m.nt, bounds, keys, vals, s);
+        radix_sort_u32(keys, vals, tk, tv, m.nt, 30, hist, stmp, s);
+        launch_build_bvh(keys, vals, m.nt, lo, hi, nodes, parent, children, flags, s);
+        launch_refit(m.nt, parent, children, nodes, flags, s);
+        m.built = true;
+    } else {
+        launch_refresh_leaves(m.nt, lo, hi, nodes, flags, s);
+        launch_refit(m.nt, parent, children, nodes, flags, s);
+    }
+    HIP_TRY(hipGetLastError());
+    m.dirty = false;
+}
+
+MeshDev mesh_dev(const MeshSlot& m) {
+    MeshDev d;
+    d.nv = m.nv;
+    d.nt = m.nt;
+    d.v64 = m.v64.get<double4>();
+    d.vi = m.vi.get<int4>();
+    d.tri = m.tri.get<uint32_t>();
+    d.nodes = m.nodes.get<float4>();
+    return d;
+}
+
+void ensure_arenas(gi_context* ctx, uint32_t cand, uint32_t exact, uint32_t rec, uint32_t seg) {
+    if (cand > ctx->cap_cand) { ctx->cand.ensure<uint2>(cand); ctx->cap_cand = cand; }
+    if (exact > ctx->cap_exact) { ctx->exact.ensure<uint2>(exact); ctx->cap_exact = exact; }
+    if (rec > ctx->cap_rec) {
+        ctx->rec_key.ensure<uint64_t>(rec);
+        ctx->rec_seg.ensure<uint2>(rec);
+        ctx->cap_rec = rec;
+    }
+    if (seg > ctx->cap_seg) { ctx->seg_pool.ensure<__int128>(8 * (size_t)seg); ctx->cap_seg = seg; }
+}
+
+uint32_t grow(uint64_t need, uint32_t cap) {
+    uint64_t v = std::max<uint64_t>(need + need / 4 + 1024, 2 * (uint64_t)cap);
+    return (uint32_t)std::min<uint64_t>(v, 0xFFFFFFF0ull);
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+    return ms;
+}
+
+void empty_result(gi_result* out, int on_device) {
+    HostResultStore* st = new HostResultStore();
+    st->offs.assign(1, 0);
+    out->n_pairs = 0;
+    out->n_segments = 0;
+    out->pairs = st->pairs.data();
+    out->seg_offsets = st->offs.data();
+    out->seg_exact = nullptr;
+    out->seg_exp2 = nullptr;
+    out->seg_f64 = nullptr;
+    out->exact_limbs = 2;
+    out->on_device = 0;
+    out->internal = st;
+    (void)on_device;
+}
+
+int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
+    memset(out, 0, sizeof(*out));
+    gi_options o = {0, 1, 0, 0};
+    if (opt) o = *opt;
+    MeshSlot& MA = ctx->mesh[0];
+    MeshSlot& MB = ctx->mesh[1];
+    if (!MA.set || !MB.set) throw GiError{GI_ERR_NOT_READY, "both meshes must be set before gi_intersect"};
+    if (MA.nt == 0 || MB.nt == 0) {  // reference: empty mesh -> early return, no pairs
+        empty_result(out, o.output_on_device);
+        return GI_OK;
+    }
+    hipStream_t s = ctx->stream;
+    gi_stats& st = out->stats;
+    HIP_TRY(hipEventRecord(ctx->ev[0], s));
+    prepare_mesh(ctx, 0, o.rebuild != 0);
+    prepare_mesh(ctx, 1, o.rebuild != 0);
+    HIP_TRY(hipEventRecord(ctx->ev[1], s));
+
+    if (ctx->cap_cand == 0) {
+        uint64_t c = ctx->cand_reserve ? ctx->cand_reserve : std::max<uint64_t>(1u << 20, 8ull * (MA.nt + (uint64_t)MB.nt));
+        c = std::min<uint64_t>(c, 0xFFFFFFF0ull);
+        ensure_arenas(ctx, (uint32_t)c, (uint32_t)std::max<uint64_t>(c / 4, 1024), (uint32_t)std::max<uint64_t>(c / 4, 1024),
+                      (uint32_t)std::max<uint64_t>(c / 4, 1024));
+    }
+    uint32_t* counters = ctx->counters.ensure<uint32_t>(CNT_COUNT);
+    uint32_t* grid = ctx->grid.get<uint32_t>();
+    int bitsA = bits_for(MA.nt), bitsB = bits_for(MB.nt);
+    uint32_t hc[CNT_COUNT];
+    uint32_t hgrid[2];
+    MeshDev A = mesh_dev(MA), B = mesh_dev(MB);
+    for (int attempt = 0;; ++attempt) {
+        launch_fill_u32(counters, 0u, CNT_COUNT, s);
+        launch_traverse(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, counters, s);
+        HIP_TRY(hipEventRecord(ctx->ev[2], s));
+        launch_filter(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, grid, ctx->exact.get<uint2>(), ctx->cap_exact,
+                      counters, s);
+        HIP_TRY(hipEventRecord(ctx->ev[3], s));
+        launch_exact(A, B, ctx->exact.get<uint2>(), ctx->cap_exact, grid, bitsB, ctx->rec_key.get<uint64_t>(),
+                     ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<__int128>(), ctx->cap_seg, counters, s);
+        HIP_TRY(hipEventRecord(ctx->ev[4], s));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(hgrid, grid, sizeof(hgrid), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        bool over = hc[CNT_TRAVERSAL_HITS] > ctx->cap_cand || hc[CNT_EXACT_LIST] > ctx->cap_exact ||
+                    hc[CNT_RESULTS] > ctx->cap_rec || hc[CNT_SEGMENTS] > ctx->cap_seg;
+        if (!over) break;
+        if (attempt > 8) throw GiError{GI_ERR_INTERNAL, "candidate capacity did not converge"};
+        uint32_t nc = hc[CNT_TRAVERSAL_HITS] > ctx->cap_cand ? grow(hc[CNT_TRAVERSAL_HITS], ctx->cap_cand) : ctx->cap_cand;
+        uint32_t ne = hc[CNT_EXACT_LIST] > ctx->cap_exact ? grow(hc[CNT_EXACT_LIST], ctx->cap_exact) : ctx->cap_exact;
+        uint32_t nr = hc[CNT_RESULTS] > ctx->cap_rec ? grow(hc[CNT_RESULTS], ctx->cap_rec) : ctx->cap_rec;
+        uint32_t ns = hc[CNT_SEGMENTS] > ctx->cap_seg ? grow(hc[CNT_SEGMENTS], ctx->cap_seg) : ctx->cap_seg;
+        ensure_arenas(ctx, nc, ne, nr, ns);
+        st.retries++;
+    }
+    if (hc[CNT_UNSOUND]) throw GiError{GI_ERR_INTERNAL, "FP64 filter certified a pair the exact stage rejects"};
+    st.traversal_hits = hc[CNT_TRAVERSAL_HITS];
+    st.candidates = hc[CNT_CANDIDATES];
+    st.filter_no = hc[CNT_FILTER_NO];
+    st.filter_green = hc[CNT_FILTER_GREEN];
+    st.filter_yellow = hc[CNT_FILTER_YELLOW];
+    st.grid = hgrid[0] && hgrid[1];
+    st.ms_build = elapsed(ctx->ev[0], ctx->ev[1]);
+    st.ms_traverse = elapsed(ctx->ev[1], ctx->ev[2]);
+    st.ms_filter = elapsed(ctx->ev[2], ctx->ev[3]);
+
+    if (st.grid) {
+        st.exact_pairs = hc[CNT_EXACT_LIST];
+        st.exact_yellow_yes = hc[CNT_EXACT_YELLOW_YES];
+        st.exact_yellow_no = hc[CNT_EXACT_YELLOW_NO];
+        st.coplanar = hc[CNT_COPLANAR];
+        st.degenerate = hc[CNT_DEGENERATE];
+        st.ms_exact = elapsed(ctx->ev[3], ctx->ev[4]);
+        uint32_t R = hc[CNT_RESULTS], S = o.want_segments ? hc[CNT_SEGMENTS] : 0;
+        uint64_t* keys = ctx->rec_key.get<uint64_t>();
+        uint32_t* order = ctx->order.ensure<uint32_t>(R);
+        uint64_t* tk = ctx->tkeys.ensure<uint64_t>(R);
+        uint32_t* tv = ctx->tvals.ensure<uint32_t>(R);
+        uint32_t* hist = ctx->hist.ensure<uint32_t>(sort_hist_elems(R));
+        uint32_t* stmp = ctx->scan_tmp.ensure<uint32_t>(scan_tmp_elems(std::max(sort_hist_elems(R), (size_t)R + 1)));
+        uint2* pairs = ctx->pairs.ensure<uint2>(R);
+        uint32_t* nseg = ctx->nseg.ensure<uint32_t>(R);
+        uint32_t* offs = ctx->offs.ensure<uint32_t>((size_t)R + 1);
+        __int128* segs = ctx->segs.ensure<__int128>(8 * (size_t)S);
+        double* f64 = ctx->f64.ensure<double>(6 * (size_t)S);
+        launch_iota(order, R, s);
+        radix_sort_u64(keys, order, tk, tv, R, bitsA + bitsB, hist, stmp, s);
+        launch_gather_pairs(keys, order, ctx->rec_seg.get<uint2>(), R, bitsB, pairs, nseg, s);
+        if (!o.want_segments) launch_fill_u32(nseg, 0u, R, s);
+        exclusive_scan_u32(nseg, offs, R, stmp, offs + R, s);
+        if (o.want_segments)
+            launch_gather_segments(order, ctx->rec_seg.get<uint2>(), offs, R, ctx->seg_pool.get<__int128>(), segs, f64, s);
+        HIP_TRY(hipEventRecord(ctx->ev[5], s));
+        HIP_TRY(hipGetLastError());
+        st.pairs = R;
+        st.segments = S;
+        out->n_pairs = R;
+        out->n_segments = S;
+        out->exact_limbs = 2;
+        if (o.output_on_device) {
+            HIP_TRY(hipEventSynchronize(ctx->ev[5]));
+            out->on_device = 1;
+            out->pairs = (uint32_t*)pairs;
+            out->seg_offsets = offs;
+            out->seg_exact = (uint64_t*)segs;
+            out->seg_exp2 = nullptr;
+            out->seg_f64 = f64;
+            out->internal = nullptr;
+        } else {
+            HostResultStore* hs = new HostResultStore();
+            hs->pairs.resize(2 * (size_t)R);
+            hs->offs.resize((size_t)R + 1);
+            hs->seg.resize(16 * (size_t)S);
+            hs->f64.resize(6 * (size_t)S);
+            if (R) HIP_TRY(hipMemcpyAsync(hs->pairs.data(), pairs, 8 * (size_t)R, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(hs->offs.data(), offs, 4 * ((size_t)R + 1), hipMemcpyDeviceToHost, s));
+            if (S) {
+                HIP_TRY(hipMemcpyAsync(hs->seg.data(), segs, 128 * (size_t)S, hipMemcpyDeviceToHost, s));
+                HIP_TRY(hipMemcpyAsync(hs->f64.data(), f64, 48 * (size_t)S, hipMemcpyDeviceToHost, s));
+            }
+            HIP_TRY(hipStreamSynchronize(s));
+            out->on_device = 0;
+            out->pairs = hs->pairs.data();
+            out->seg_offsets = hs->offs.data();
+            out->seg_exact = hs->seg.data();
+            out->seg_exp2 = nullptr;
+            out->seg_f64 = hs->f64.data();
+            out->internal = hs;
+        }
+        st.ms_output = elapsed(ctx->ev[4], ctx->ev[5]);
+        st.ms_total = elapsed(ctx->ev[0], ctx->ev[5]);
+        return GI_OK;
+    }
+
+    // General doubles: the GREEN/YELLOW list goes to the host residual stage.
+    auto t0 = std::chrono::steady_clock::now();
+    uint32_t n = hc[CNT_EXACT_LIST];
+    std::vector<uint32_t> list(2 * (size_t)n);
+    std::vector<double> vA(4 * (size_t)MA.nv), vB(4 * (size_t)MB.nv);
+    if (n) HIP_TRY(hipMemcpyAsync(list.data(), ctx->exact.get<uint2>(), 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(vA.data(), MA.v64.get<double4>(), 32 * (size_t)MA.nv, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(vB.data(), MB.v64.get<double4>(), 32 * (size_t)MB.nv, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<HostHit> hits;
+    uint64_t yes = 0;
+    int nth = ctx->host_threads > 0 ? ctx->host_threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    if (host_exact_batch(vA.data(), MA.h_tri.data(), vB.data(), MB.h_tri.data(), list.data(), n, nth, hits, &yes) != 0)
+        throw GiError{GI_ERR_INTERNAL, "host residual stage failed (overflow or unsound filter)"};
+    HostResultStore* hs = new HostResultStore();
+    int limbs = 1;
+    for (auto& h : hits) limbs = std::max(limbs, h.limbs);
+    hs->pairs.resize(2 * hits.size());
+    hs->offs.resize(hits.size() + 1);
+    size_t S = 0;
+    for (size_t i = 0; i < hits.size(); ++i) {
+        hs->pairs[2 * i] = hits[i].a;
+        hs->pairs[2 * i + 1] = hits[i].b;
+        hs->offs[i] = (uint32_t)S;
+        if (o.want_segments) S += (size_t)hits[i].nseg;
+        st.coplanar += hits[i].coplanar;
+        st.degenerate += hits[i].degenerate;
+    }
+    hs->offs[hits.size()] = (uint32_t)S;
+    hs->seg.assign(S * 8 * (size_t)limbs, 0);
+    hs->exp2.resize(S);
+    hs->f64.resize(6 * S);
+    if (o.want_segments) {
+        size_t si = 0;
+        for (auto& h : hits)
+            for (int sg = 0; sg < h.nseg; ++sg, ++si) {
+                hs->exp2[si] = h.exp2;
+                double wv[2];
+                for (int e = 0; e < 8; ++e) {
+                    const uint64_t* src = &h.data[((size_t)sg * 8 + e) * h.limbs];
+                    uint64_t* dst = &hs->seg[(si * 8 + e) * (size_t)limbs];
+                    uint64_t ext = (src[h.limbs - 1] >> 63) ? ~0ull : 0ull;
+                    for (int l = 0; l < limbs; ++l) dst[l] = l < h.limbs ? src[l] : ext;
+                }
+                // double rendering: value(h)/value(w) * 2^exp2
+                for (int e = 0; e < 2; ++e) {
+                    auto val = [&](int idx) {
+                        const uint64_t* x = &hs->seg[(si * 8 + 4 * e + idx) * (size_t)limbs];
+                        bool neg = x[limbs - 1] >> 63;
+                        double v = 0.0;
+                        for (int l = limbs - 1; l >= 0; --l) v = v * 18446744073709551616.0 + (double)(neg ? ~x[l] : x[l]);
+                        return neg ? -(v + 1.0) : v;
+                    };
+                    wv[e] = val(3);
+                    for (int k = 0; k < 3; ++k) hs->f64[6 * si + 3 * e + k] = ldexp(val(k) / wv[e], h.exp2);
+                }
+            }
+    }
+    st.host_pairs = n;
+    st.host_yes = yes;
+    st.pairs = hits.size();
+    st.segments = S;
+    st.ms_host = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    st.ms_total = elapsed(ctx->ev[0], ctx->ev[4]) + st.ms_host;
+    out->n_pairs = hits.size();
+    out->n_segments = S;
+    out->exact_limbs = limbs;
+    out->pairs = hs->pairs.data();
+    out->seg_offsets = hs->offs.data();
+    out->seg_exact = hs->seg.data();
+    out->seg_exp2 = hs->exp2.data();
+    out->seg_f64 = hs->f64.data();
+    out->on_device = 0;
+    out->internal = hs;
+    if (o.output_on_device) {
+        // copy into the context's device result buffers
+        uint32_t* dp = ctx->pairs.ensure<uint32_t>(hs->pairs.size());
+        uint32_t* doffs = ctx->offs.ensure<uint32_t>(hs->offs.size());
+        uint64_t* dseg = (uint64_t*)ctx->segs.ensure<uint64_t>(hs->seg.size());
+        int32_t* dexp = ctx->exp2.ensure<int32_t>(hs->exp2.size());
+        double* df = ctx->f64.ensure<double>(hs->f64.size());
+        if (!hs->pairs.empty()) HIP_TRY(hipMemcpy(dp, hs->pairs.data(), 4 * hs->pairs.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(doffs, hs->offs.data(), 4 * hs->offs.size(), hipMemcpyHostToDevice));
+        if (S) {
+            HIP_TRY(hipMemcpy(dseg, hs->seg.data(), 8 * hs->seg.size(), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(dexp, hs->exp2.data(), 4 * hs->exp2.size(), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(df, hs->f64.data(), 8 * hs->f64.size(), hipMemcpyHostToDevice));
+        }
+        delete hs;
+        out->pairs = dp;
+        out->seg_offsets = doffs;
+        out->seg_exact = dseg;
+        out->seg_exp2 = dexp;
+        out->seg_f64 = df;
+        out->on_device = 1;
+        out->internal = nullptr;
+    }
+    return GI_OK;
+}
+
+// R = Rz Ry Rx from Euler angles in degrees and T = c + t - R c, evaluated
+// in the order of src/CPU/RotationTools.h:28-104 (createRigidTransformation).
+Xform rigid_xform(const double rot[3], const double trans[3], const double c[3]) {
+    Xform x = identity_xform();
+    if (rot[0] == 0.0 && rot[1] == 0.0 && rot[2] == 0.0 && trans[0] == 0.0 && trans[1] == 0.0 && trans[2] == 0.0)
+        return x;
+    const double DEG_TO_RAD = 3.14159265358979323846 / 180.0;
+    double rx = rot[0] * DEG_TO_RAD, ry = rot[1] * DEG_TO_RAD, rz = rot[2] * DEG_TO_RAD;
+    double cx = cos(rx), sx = sin(rx), cy = cos(ry), sy = sin(ry), cz = cos(rz), sz = sin(rz);
+    double R[3][3];
+    R[0][0] = cy * cz;
+    R[0][1] = sx * sy * cz - cx * sz;
+    R[0][2] = cx * sy * cz + sx * sz;
+    R[1][0] = cy * sz;
+    R[1][1] = sx * sy * sz + cx * cz;
+    R[1][2] = cx * sy * sz - sx * cz;
+    R[2][0] = -sy;
+    R[2][1] = sx * cy;
+    R[2][2] = cx * cy;
+    for (int r = 0; r < 3; ++r) {
+        for (int k = 0; k < 3; ++k) x.m[r][k] = R[r][k];
+        x.m[r][3] = c[r] + trans[r] - (R[r][0] * c[0] + R[r][1] * c[1] + R[r][2] * c[2]);
+    }
+    x.identity = 0;
+    return x;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+
+extern "C" {
+
+const char* gi_status_string(int status) {
+    switch (status) {
+        case GI_OK: return "ok";
+        case GI_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case GI_ERR_HIP: return "HIP runtime error";
+        case GI_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case GI_ERR_NO_DEVICE: return "no HIP device";
+        case GI_ERR_NOT_READY: return "mesh not set";
+        case GI_ERR_INTERNAL: return "internal consistency check failed";
+        case GI_ERR_TOO_LARGE: return "mesh too large";
+        default: return "unknown status";
+    }
+}
+
+int gi_create(const gi_config* cfg, gi_context** out) {
+    if (!out) return GI_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    gi_context* ctx = new gi_context();
+    try {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+            delete ctx;
+            return GI_ERR_NO_DEVICE;
+        }
+        ctx->device = cfg ? cfg->device : 0;
+        if (ctx->device < 0 || ctx->device >= ndev) {
+            delete ctx;
+            return GI_ERR_INVALID_ARGUMENT;
+        }
+        ctx->host_threads = cfg ? cfg->host_threads : 0;
+        ctx->cand_reserve = cfg ? cfg->candidate_reserve : 0;
+        HIP_TRY(hipSetDevice(ctx->device));
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+        for (auto& m : ctx->mesh) m.xf = identity_xform();
+        *out = ctx;
+        return GI_OK;
+    } catch (const GiError& e) {
+        int st = e.status;
+        delete ctx;
+        return st;
+    }
+}
+
+int gi_destroy(gi_context* ctx) {
+    if (!ctx) return GI_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->release_device();
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GI_OK;
+}
+
+const char* gi_last_error(const gi_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+static int set_mesh_common(gi_context* ctx, int slot, uint32_t nv, uint32_t nt) {
+    if (!ctx || slot < 0 || slot > 1) return GI_ERR_INVALID_ARGUMENT;
+    if (nt >= (1u << 30)) return GI_ERR_TOO_LARGE;
+    (void)nv;
+    return GI_OK;
+}
+
+int gi_set_mesh(gi_context* ctx, int slot, const double* xyz, uint32_t nv, const uint32_t* tri, uint32_t nt) {
+    int st = set_mesh_common(ctx, slot, nv, nt);
+    if (st) return st;
+    if ((nv && !xyz) || (nt && !tri)) return GI_ERR_INVALID_ARGUMENT;
+    for (size_t i = 0; i < 3 * (size_t)nt; ++i)
+        if (tri[i] >= nv) {
+            ctx->err = "triangle index out of range";
+            return GI_ERR_INVALID_ARGUMENT;
+        }
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        MeshSlot& m = ctx->mesh[slot];
+        m.h_xyz.assign(xyz, xyz + 3 * (size_t)nv);
+        m.h_tri.assign(tri, tri + 3 * (size_t)nt);
+        m.nv = nv;
+        m.nt = nt;
+        m.set = true;
+        m.xf = identity_xform();
+        upload_mesh(ctx, slot);
+        return GI_OK;
+    } catch (const GiError& e) {
+        return fail(ctx, e);
+    }
+}
+
+int gi_set_mesh_device(gi_context* ctx, int slot, const double* d_xyz, uint32_t nv, const uint32_t* d_tri,
+                       uint32_t nt) {
+    int st = set_mesh_common(ctx, slot, nv, nt);
+    if (st) return st;
+    if ((nv && !d_xyz) || (nt && !d_tri)) return GI_ERR_INVALID_ARGUMENT;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        MeshSlot& m = ctx->mesh[slot];
+        m.nv = nv;
+        m.nt = nt;
+        m.set = true;
+        m.xf = identity_xform();
+        double* src = m.src.ensure<double>(3 * (size_t)nv);
+        uint32_t* tri = m.tri.ensure<uint32_t>(3 * (size_t)nt);
+        if (nv) HIP_TRY(hipMemcpyAsync(src, d_xyz, 24 * (size_t)nv, hipMemcpyDeviceToDevice, ctx->stream));
+        if (nt) HIP_TRY(hipMemcpyAsync(tri, d_tri, 12 * (size_t)nt, hipMemcpyDeviceToDevice, ctx->stream));
+        // host copies serve gi_reconstruct_device and the host residual stage
+        m.h_xyz.resize(3 * (size_t)nv);
+        m.h_tri.resize(3 * (size_t)nt);
+        if (nv) HIP_TRY(hipMemcpyAsync(m.h_xyz.data(), d_xyz, 24 * (size_t)nv, hipMemcpyDeviceToHost, ctx->stream));
+        if (nt) HIP_TRY(hipMemcpyAsync(m.h_tri.data(), d_tri, 12 * (size_t)nt, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < 3 * (size_t)nt; ++i)
+            if (m.h_tri[i] >= nv) {
+                m.set = false;
+                ctx->err = "triangle index out of range";
+                return GI_ERR_INVALID_ARGUMENT;
+            }
+        m.on_device = true;
+        m.built = false;
+        m.dirty = true;
+        return GI_OK;
+    } catch (const GiError& e) {
+        return fail(ctx, e);
+    }
+}
+
+int gi_set_transform(gi_context* ctx, int slot, const double rot_deg[3], const double trans[3], const double center[3],
+                     int snap_to_grid) {
+    if (!ctx || slot < 0 || slot > 1 || !rot_deg || !trans || !center) return GI_ERR_INVALID_ARGUMENT;
+    if (!ctx->mesh[slot].set) return GI_ERR_NOT_READY;
+    Xform x = rigid_xform(rot_deg, trans, center);
+    x.snap = snap_to_grid ? 1 : 0;
+    ctx->mesh[slot].xf = x;
+    ctx->mesh[slot].dirty = true;
+    return GI_OK;
+}
+
+int gi_set_transform_matrix(gi_context* ctx, int slot, const double m[12], int snap_to_grid) {
+    if (!ctx || slot < 0 || slot > 1 || !m) return GI_ERR_INVALID_ARGUMENT;
+    if (!ctx->mesh[slot].set) return GI_ERR_NOT_READY;
+    Xform x = identity_xform();
+    bool ident = true;
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 4; ++k) {
+            x.m[r][k] = m[4 * r + k];
+            double want = (k == r) ? 1.0 : 0.0;
+            if (m[4 * r + k] != want) ident = false;
+        }
+    x.identity = ident ? 1 : 0;
+    x.snap = snap_to_grid ? 1 : 0;
+    ctx->mesh[slot].xf = x;
+    ctx->mesh[slot].dirty = true;
+    return GI_OK;
+}
+
+int gi_build(gi_context* ctx) {
+    if (!ctx) return GI_ERR_INVALID_ARGUMENT;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        for (int sl = 0; sl < 2; ++sl)
+            if (ctx->mesh[sl].set) prepare_mesh(ctx, sl, false);
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return GI_OK;
+    } catch (const GiError& e) {
+        return fail(ctx, e);
+    }
+}
+
+int gi_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
+    if (!ctx || !out) return GI_ERR_INVALID_ARGUMENT;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        return run_intersect(ctx, opt, out);
+    } catch (const GiError& e) {
+        return fail(ctx, e);
+    }
+}
+
+int gi_result_free(gi_result* res) {
+    if (!res) return GI_ERR_INVALID_ARGUMENT;
+    delete (HostResultStore*)res->internal;
+    memset(res, 0, sizeof(*res));
+    return GI_OK;
+}
+
+int gi_intersect_oneshot(const gi_config* cfg, const double* xyzA, uint32_t nvA, const uint32_t* triA, uint32_t ntA,
+                         const double* xyzB, uint32_t nvB, const uint32_t* triB, uint32_t ntB, const gi_options* opt,
+                         gi_result* out) {
+    gi_context* ctx = nullptr;
+    int st = gi_create(cfg, &ctx);
+    if (st) return st;
+    gi_options o = {0, 1, 0, 0};
+    if (opt) o = *opt;
+    o.output_on_device = 0;  // the context dies with this call
+    st = gi_set_mesh(ctx, 0, xyzA, nvA, triA, ntA);
+    if (!st) st = gi_set_mesh(ctx, 1, xyzB, nvB, triB, ntB);
+    if (!st) st = gi_intersect(ctx, &o, out);
+    gi_destroy(ctx);
+    return st;
+}
+
+int gi_clear_device(gi_context* ctx) {
+    if (!ctx) return GI_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->release_device();
+    return GI_OK;
+}
+
+int gi_reconstruct_device(gi_context* ctx) {
+    if (!ctx) return GI_ERR_INVALID_ARGUMENT;
+    for (int sl = 0; sl < 2; ++sl)
+        if (!ctx->mesh[sl].set) {
+            ctx->err = "reconstruct needs both meshes";  // reference prints to stderr and returns
+            return GI_ERR_NOT_READY;
+        }
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        for (int sl = 0; sl < 2; ++sl) {
+            if (!ctx->mesh[sl].on_device) upload_mesh(ctx, sl);
+            prepare_mesh(ctx, sl, true);
+        }
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return GI_OK;
+    } catch (const GiError& e) {
+        return fail(ctx, e);
+    }
+}
+
+int gi_exact_pair_host(const double a[9], const double b[9], int32_t* hit, int32_t* coplanar, int32_t* nseg,
+                       int32_t* exp2, int32_t* limbs, uint64_t* seg, int32_t limbs_cap) {
+    if (!a || !b || !hit || !nseg || !exp2 || !limbs) return GI_ERR_INVALID_ARGUMENT;
+    HostHit h;
+    bool ok;
+    bool r = host_exact_pair(a, b, h, &ok);
+    if (!ok) return GI_ERR_TOO_LARGE;
+    *hit = r ? 1 : 0;
+    if (coplanar) *coplanar = r ? h.coplanar : 0;
+    *nseg = r ? h.nseg : 0;
+    *exp2 = r ? h.exp2 : 0;
+    *limbs = r ? h.limbs : 0;
+    if (r && seg) {
+        if (h.limbs > limbs_cap) return GI_ERR_TOO_LARGE;
+        for (int s = 0; s < h.nseg; ++s)
+            for (int e = 0; e < 8; ++e) {
+                const uint64_t* src = &h.data[((size_t)s * 8 + e) * h.limbs];
+                uint64_t* dst = seg + ((size_t)s * 8 + e) * limbs_cap;
+                uint64_t ext = (src[h.limbs - 1] >> 63) ? ~0ull : 0ull;
+                for (int l = 0; l < limbs_cap; ++l) dst[l] = l < h.limbs ? src[l] : ext;
+            }
+    }
+    return GI_OK;
+}
+
+}  // extern "C"

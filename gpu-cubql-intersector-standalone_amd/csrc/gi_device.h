@@ -1,0 +1,90 @@
+// gi_device.h -- launchers of the gfx950 kernels (implemented in
+// gi_device.hip). All launches are asynchronous on the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gi {
+
+constexpr uint32_t kSentinel = 0xFFFFFFFFu;
+
+// Pipeline counters (device, uint32 each), cleared by launch_fill at the
+// start of every intersection.
+enum Counter {
+    CNT_TRAVERSAL_HITS = 0,  // FP32 box overlaps emitted by the traversal
+    CNT_CANDIDATES,          // exact (double) box overlaps = candidate pairs
+    CNT_FILTER_NO,
+    CNT_FILTER_GREEN,
+    CNT_FILTER_YELLOW,
+    CNT_EXACT_LIST,          // pairs handed to the exact stage
+    CNT_RESULTS,             // intersecting pairs written by the exact stage
+    CNT_SEGMENTS,            // segments written by the exact stage
+    CNT_EXACT_YELLOW_YES,    // YELLOW pairs the exact stage found intersecting
+    CNT_EXACT_YELLOW_NO,
+    CNT_COPLANAR,
+    CNT_DEGENERATE,
+    CNT_UNSOUND,             // filter said GREEN, exact said no (must stay 0)
+    CNT_COUNT
+};
+
+// Rigid transform applied as CGAL's Aff_transformation_3 evaluates it:
+// x' = ((m00*x + m01*y) + m02*z) + t0, every operation rounded separately.
+struct Xform {
+    double m[3][4];
+    int identity;
+    int snap;  // round the result to the nearest integer (grid meshes)
+};
+
+struct MeshDev {
+    uint32_t nv, nt;
+    const double4* v64;  // vertices used by the predicates
+    const int4* vi;      // integer copy (valid when the mesh is on the grid)
+    const uint32_t* tri; // 3 * nt
+    const float4* nodes; // BVH, 2 float4 per node, 2*nt - 1 nodes
+};
+
+void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
+void launch_prepare_vertices(const double* src, uint32_t nv, Xform xf, double4* v64, int4* vi, uint32_t* grid_flag,
+                             hipStream_t s);
+void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, float4* lo, float4* hi,
+                      uint32_t* bounds6, hipStream_t s);
+void launch_morton(const float4* lo, const float4* hi, uint32_t nt, const uint32_t* bounds6, uint32_t* keys,
+                   uint32_t* vals, hipStream_t s);
+
+// LSD radix sort of (key, value) pairs. tmp_* are ping-pong buffers of n
+// elements; hist needs sort_hist_elems(n) entries, scan_tmp scan_tmp_elems().
+size_t sort_hist_elems(size_t n);
+size_t scan_tmp_elems(size_t n);
+void radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
+                    uint32_t* hist, uint32_t* scan_tmp, hipStream_t s);
+void radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
+                    uint32_t* hist, uint32_t* scan_tmp, hipStream_t s);
+// exclusive scan of n uint32 (in place allowed); writes the total to *total
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scan_tmp, uint32_t* total,
+                        hipStream_t s);
+
+void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
+                      const float4* hi, float4* nodes, uint32_t* parent, uint2* children, uint32_t* flags,
+                      hipStream_t s);
+void launch_refit(uint32_t n, const uint32_t* parent, const uint2* children, float4* nodes, uint32_t* flags,
+                  hipStream_t s);
+// Refit after a rigid motion: new prim boxes are written into the leaves
+// (leaf k holds prim nodes[2*(n-1+k)].w), then internal boxes bottom-up.
+void launch_refresh_leaves(uint32_t n, const float4* lo, const float4* hi, float4* nodes, uint32_t* flags,
+                           hipStream_t s);
+
+void launch_traverse(MeshDev A, MeshDev B, uint2* cand, uint32_t cap, uint32_t* counters, hipStream_t s);
+void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const uint32_t* grid_flags,
+                   uint2* exact_list, uint32_t exact_cap, uint32_t* counters, hipStream_t s);
+void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
+                  int bitsB,
+                  uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, __int128* seg_pool, uint32_t seg_cap,
+                  uint32_t* counters, hipStream_t s);
+void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
+void launch_gather_pairs(const uint64_t* sorted_key, const uint32_t* order, const uint2* rec_seg, uint32_t n,
+                         int bitsB, uint2* pairs, uint32_t* nseg, hipStream_t s);
+void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const uint32_t* seg_offsets, uint32_t n,
+                            const __int128* seg_pool, __int128* seg_out, double* seg_f64, hipStream_t s);
+
+}  // namespace gi

@@ -1,0 +1,861 @@
+// gi_device.hip -- gfx950 kernels of the mesh/mesh intersection path.
+//
+// Reference counterparts (files relative to the reference root):
+//   prepare_vertices  src/common/utils.cu:479-593 (point intervals, rigid
+//                     transform; here the transform is evaluated exactly as
+//                     the CGAL path evaluates it, src/CPU/RotationTools.h:76-104)
+//   tri_boxes         src/common/utils.cu:595-643 (FP64 -> FP32 boxes; here
+//                     rounded outward instead of padded)
+//   morton/sort/bvh   include/third-party/cubql/level_cut_builder.h:16-364
+//                     and cuBQL's refit (replaced by a Karras LBVH with ropes)
+//   traverse          src/single_tree/SingleTreeBatchCommon.cu:116-209 and
+//                     include/third-party/cubql/fixedBoxQueryv2.h:55-105
+//                     (count+fill with a 64-entry stack; here one stackless
+//                     pass with wave-aggregated emission)
+//   filter            src/GPU_exact_predicates/OriginShiftedFloatPredicate.cu:7-45
+//   exact             src/GPU_exact_predicates/ExactBigIntPredicate.cu:381-548
+//                     plus the host CGAL fallback and segment extraction
+//   gather/sort       thrust::copy_if / sort in
+//                     src/single_tree/SingleTreeBatchIntersector.cu:276-384
+#include <hip/hip_runtime.h>
+
+#include "exact_tritri.h"
+#include "filter.h"
+#include "gi_device.h"
+
+namespace gi {
+
+#define GI_LAUNCH_CHECK() (void)hipGetLastError()
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+static inline uint32_t blocks_for(size_t n, uint32_t threads) { return (uint32_t)((n + threads - 1) / threads); }
+
+// ------------------------------------------------------------------ fill
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    uint32_t blocks = blocks_for(n, 256);
+    if (blocks > 4096) blocks = 4096;
+    k_fill_u32<<<blocks, 256, 0, s>>>(p, value, n);
+    GI_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------- vertex preparation
+
+__device__ __forceinline__ double xform_coord(const Xform& xf, int r, double x, double y, double z) {
+    // CGAL Aff_transformation_3 on a Cartesian point, left to right, no FMA
+    return __dadd_rn(__dadd_rn(__dadd_rn(__dmul_rn(xf.m[r][0], x), __dmul_rn(xf.m[r][1], y)), __dmul_rn(xf.m[r][2], z)),
+                     xf.m[r][3]);
+}
+
+__global__ __launch_bounds__(256) void k_prepare_vertices(const double* __restrict__ src, uint32_t nv, Xform xf,
+                                                          double4* __restrict__ v64, int4* __restrict__ vi,
+                                                          uint32_t* grid_flag) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool grid = true;
+    if (i < nv) {
+        double x = src[3 * (size_t)i], y = src[3 * (size_t)i + 1], z = src[3 * (size_t)i + 2];
+        if (!xf.identity) {
+            double X = xform_coord(xf, 0, x, y, z);
+            double Y = xform_coord(xf, 1, x, y, z);
+            double Z = xform_coord(xf, 2, x, y, z);
+            x = X; y = Y; z = Z;
+        }
+        if (xf.snap) { x = rint(x); y = rint(y); z = rint(z); }
+        v64[i] = make_double4(x, y, z, 0.0);
+        const double L = 67108864.0;  // 2^26
+        grid = x == rint(x) && y == rint(y) && z == rint(z) && fabs(x) <= L && fabs(y) <= L && fabs(z) <= L;
+        vi[i] = grid ? make_int4((int)x, (int)y, (int)z, 0) : make_int4(0, 0, 0, 0);
+    }
+    uint64_t bad = __ballot(!grid);
+    if (bad && lane_id() == 0) atomicAnd(grid_flag, 0u);
+}
+
+void launch_prepare_vertices(const double* src, uint32_t nv, Xform xf, double4* v64, int4* vi, uint32_t* grid_flag,
+                             hipStream_t s) {
+    if (nv == 0) return;
+    k_prepare_vertices<<<blocks_for(nv, 256), 256, 0, s>>>(src, nv, xf, v64, vi, grid_flag);
+    GI_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------- triangle boxes
+
+__device__ __forceinline__ float f32_down(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = nextafterf(f, -INFINITY);
+    return f;
+}
+__device__ __forceinline__ float f32_up(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = nextafterf(f, INFINITY);
+    return f;
+}
+// order-preserving map float -> uint32 (for atomic min/max)
+__device__ __forceinline__ uint32_t f2key(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float key2f(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ tri, uint32_t nt,
+                                                   const double4* __restrict__ v, float4* __restrict__ lo,
+                                                   float4* __restrict__ hi, uint32_t* bounds6) {
+    __shared__ float red[6][4];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (i < nt) {
+        double4 a = v[tri[3 * (size_t)i]], b = v[tri[3 * (size_t)i + 1]], c = v[tri[3 * (size_t)i + 2]];
+        double lx = fmin(fmin(a.x, b.x), c.x), hx = fmax(fmax(a.x, b.x), c.x);
+        double ly = fmin(fmin(a.y, b.y), c.y), hy = fmax(fmax(a.y, b.y), c.y);
+        double lz = fmin(fmin(a.z, b.z), c.z), hz = fmax(fmax(a.z, b.z), c.z);
+        l[0] = f32_down(lx); l[1] = f32_down(ly); l[2] = f32_down(lz);
+        h[0] = f32_up(hx); h[1] = f32_up(hy); h[2] = f32_up(hz);
+        lo[i] = make_float4(l[0], l[1], l[2], 0.f);
+        hi[i] = make_float4(h[0], h[1], h[2], 0.f);
+    }
+    uint32_t wave = threadIdx.x >> 6;
+    for (int k = 0; k < 3; ++k) {
+        float mn = wave_min(l[k]), mx = wave_max(h[k]);
+        if (lane_id() == 0) { red[k][wave] = mn; red[3 + k][wave] = mx; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int k = threadIdx.x;
+        float r = red[k][0];
+        for (int w = 1; w < 4; ++w) r = k < 3 ? fminf(r, red[k][w]) : fmaxf(r, red[k][w]);
+        if (k < 3) atomicMin(&bounds6[k], f2key(r));
+        else atomicMax(&bounds6[k], f2key(r));
+    }
+}
+
+void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, float4* lo, float4* hi, uint32_t* bounds6,
+                      hipStream_t s) {
+    if (nt == 0) return;
+    k_tri_boxes<<<blocks_for(nt, 256), 256, 0, s>>>(tri, nt, v64, lo, hi, bounds6);
+    GI_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ morton
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ lo, const float4* __restrict__ hi,
+                                                uint32_t nt, const uint32_t* __restrict__ bounds6,
+                                                uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    float4 l = lo[i], h = hi[i];
+    float c[3] = {0.5f * l.x + 0.5f * h.x, 0.5f * l.y + 0.5f * h.y, 0.5f * l.z + 0.5f * h.z};
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        float bmin = key2f(bounds6[k]), bmax = key2f(bounds6[3 + k]);
+        float ext = bmax - bmin;
+        float u = (ext > 0.f && isfinite(ext)) ? (c[k] - bmin) / ext : 0.f;
+        u = isfinite(u) ? u : 0.f;
+        float s = fminf(fmaxf(u * 1024.f, 0.f), 1023.f);
+        q[k] = (uint32_t)s;
+    }
+    keys[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+    vals[i] = i;
+}
+
+void launch_morton(const float4* lo, const float4* hi, uint32_t nt, const uint32_t* bounds6, uint32_t* keys,
+                   uint32_t* vals, hipStream_t s) {
+    if (nt == 0) return;
+    k_morton<<<blocks_for(nt, 256), 256, 0, s>>>(lo, hi, nt, bounds6, keys, vals);
+    GI_LAUNCH_CHECK();
+}
+
+// -------------------------------------------------------------------- scan
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr uint32_t SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+size_t scan_tmp_elems(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t ws[SCAN_THREADS / 64];
+    uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) ws[wave] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (uint32_t w = 0; w < SCAN_THREADS / 64; ++w) {
+        if (w < wave) off += ws[w];
+        tot += ws[w];
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return off + x - v;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
+                                                              uint32_t* __restrict__ sums) {
+    uint32_t base = blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        uint32_t i = base + it * SCAN_THREADS + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint32_t tot;
+    block_exclusive_sum(s, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(uint32_t* sums, uint32_t nb, uint32_t* total) {
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += SCAN_TILE) {
+        uint32_t v[SCAN_ITEMS];
+        uint32_t s = 0;
+        for (int it = 0; it < SCAN_ITEMS; ++it) {
+            uint32_t i = base + threadIdx.x * SCAN_ITEMS + it;
+            v[it] = i < nb ? sums[i] : 0;
+            s += v[it];
+        }
+        uint32_t tot;
+        uint32_t off = block_exclusive_sum(s, &tot) + carry;
+        for (int it = 0; it < SCAN_ITEMS; ++it) {
+            uint32_t i = base + threadIdx.x * SCAN_ITEMS + it;
+            if (i < nb) sums[i] = off;
+            off += v[it];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                            uint32_t n, const uint32_t* __restrict__ sums) {
+    uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        uint32_t i = base + it;
+        v[it] = i < n ? in[i] : 0;
+        s += v[it];
+    }
+    uint32_t off = block_exclusive_sum(s, nullptr) + sums[blockIdx.x];
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        uint32_t i = base + it;
+        if (i < n) out[i] = off;
+        off += v[it];
+    }
+}
+
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scan_tmp, uint32_t* total,
+                        hipStream_t s) {
+    uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (n == 0) {
+        if (total) launch_fill_u32(total, 0, 1, s);
+        return;
+    }
+    k_scan_reduce<<<nb, SCAN_THREADS, 0, s>>>(in, n, scan_tmp);
+    k_scan_top<<<1, SCAN_THREADS, 0, s>>>(scan_tmp, nb, total);
+    k_scan_down<<<nb, SCAN_THREADS, 0, s>>>(in, out, n, scan_tmp);
+    GI_LAUNCH_CHECK();
+}
+
+// -------------------------------------------------------------- radix sort
+
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr uint32_t SORT_TILE = SORT_THREADS * SORT_ITEMS;
+constexpr int RADIX = 256;
+
+size_t sort_hist_elems(size_t n) { return RADIX * ((n + SORT_TILE - 1) / SORT_TILE) + 1; }
+
+template <class K>
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_upsweep(const K* __restrict__ keys, uint32_t n, int shift,
+                                                               uint32_t* __restrict__ hist, uint32_t ntiles) {
+    __shared__ uint32_t h[SORT_THREADS / 64][RADIX];
+    uint32_t wave = threadIdx.x >> 6;
+    for (int d = threadIdx.x; d < (SORT_THREADS / 64) * RADIX; d += SORT_THREADS) (&h[0][0])[d] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * SORT_TILE;
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        uint32_t i = base + it * SORT_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[wave][(uint32_t)(keys[i] >> shift) & (RADIX - 1)], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) {
+        uint32_t c = 0;
+        for (int w = 0; w < SORT_THREADS / 64; ++w) c += h[w][d];
+        hist[(size_t)d * ntiles + blockIdx.x] = c;
+    }
+}
+
+// Stable scatter: keys of a tile are ranked in (wave, item, lane) order,
+// which is their original order.
+template <class K>
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                                 K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                                 uint32_t n, int shift,
+                                                                 const uint32_t* __restrict__ hist, uint32_t ntiles) {
+    __shared__ uint32_t wc[SORT_THREADS / 64][RADIX];
+    __shared__ uint32_t gbase[RADIX];
+    uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    for (int d = threadIdx.x; d < (SORT_THREADS / 64) * RADIX; d += SORT_THREADS) (&wc[0][0])[d] = 0;
+    for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) gbase[d] = hist[(size_t)d * ntiles + blockIdx.x];
+    __syncthreads();
+    uint32_t base = blockIdx.x * SORT_TILE + wave * (64 * SORT_ITEMS);
+    uint64_t lt = lanemask_lt();
+    K k[SORT_ITEMS];
+    uint32_t v[SORT_ITEMS];
+    uint32_t rank[SORT_ITEMS];
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        uint32_t i = base + it * 64 + lane;
+        bool valid = i < n;
+        k[it] = valid ? kin[i] : (K)0;
+        v[it] = valid ? vin[i] : 0u;
+    }
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        uint32_t i = base + it * 64 + lane;
+        bool valid = i < n;
+        uint32_t d = (uint32_t)(k[it] >> shift) & (RADIX - 1);
+        uint64_t match = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            uint64_t bb = __ballot((d >> b) & 1u);
+            match &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        uint32_t peers = (uint32_t)__popcll(match & lt);
+        uint32_t cnt = (uint32_t)__popcll(match);
+        uint32_t cur = wc[wave][d];
+        rank[it] = cur + peers;
+        wave_sync_lds();
+        if (valid && peers == 0) wc[wave][d] = cur + cnt;
+        wave_sync_lds();
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) {
+        uint32_t run = 0;
+        for (int w = 0; w < SORT_THREADS / 64; ++w) {
+            uint32_t t = wc[w][d];
+            wc[w][d] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        uint32_t i = base + it * 64 + lane;
+        if (i >= n) continue;
+        uint32_t d = (uint32_t)(k[it] >> shift) & (RADIX - 1);
+        uint32_t pos = gbase[d] + wc[wave][d] + rank[it];
+        kout[pos] = k[it];
+        vout[pos] = v[it];
+    }
+}
+
+template <class K>
+static void radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32_t n, int bits, uint32_t* hist,
+                            uint32_t* scan_tmp, hipStream_t s) {
+    if (n <= 1) return;
+    uint32_t ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+    int passes = (bits + 7) / 8;
+    K* src_k = keys;
+    uint32_t* src_v = vals;
+    K* dst_k = tk;
+    uint32_t* dst_v = tv;
+    for (int p = 0; p < passes; ++p) {
+        int shift = 8 * p;
+        k_sort_upsweep<K><<<ntiles, SORT_THREADS, 0, s>>>(src_k, n, shift, hist, ntiles);
+        exclusive_scan_u32(hist, hist, RADIX * ntiles, scan_tmp, nullptr, s);
+        k_sort_downsweep<K><<<ntiles, SORT_THREADS, 0, s>>>(src_k, src_v, dst_k, dst_v, n, shift, hist, ntiles);
+        K* t = src_k; src_k = dst_k; dst_k = t;
+        uint32_t* tvv = src_v; src_v = dst_v; dst_v = tvv;
+    }
+    if (src_k != keys) {
+        (void)hipMemcpyAsync(keys, src_k, sizeof(K) * n, hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(vals, src_v, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
+    }
+    GI_LAUNCH_CHECK();
+}
+
+void radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
+                    uint32_t* hist, uint32_t* scan_tmp, hipStream_t s) {
+    radix_sort_impl<uint32_t>(keys, vals, tmp_keys, tmp_vals, n, bits, hist, scan_tmp, s);
+}
+void radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
+                    uint32_t* hist, uint32_t* scan_tmp, hipStream_t s) {
+    radix_sort_impl<uint64_t>(keys, vals, tmp_keys, tmp_vals, n, bits, hist, scan_tmp, s);
+}
+
+// --------------------------------------------------------- BVH (Karras)
+
+// common-prefix length of the composite keys (morton, index)
+__device__ __forceinline__ int delta(const uint32_t* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = k[i], b = k[j];
+    if (a != b) return __clz(a ^ b);
+    return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+}
+
+// The node visited right after every subtree whose last leaf is m - 1: the
+// right child that starts at leaf m. In Karras' numbering a right child
+// covering [m, x] is internal node m, which is the case iff internal node m
+// extends to the right; otherwise it is leaf m.
+__device__ __forceinline__ uint32_t chain_top(const uint32_t* __restrict__ k, int n, int m) {
+    if (m < n - 1 && delta(k, n, m, m + 1) > delta(k, n, m, m - 1)) return (uint32_t)m;
+    return (uint32_t)(n - 1 + m);
+}
+
+__global__ __launch_bounds__(256) void k_build_bvh(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ prims,
+                                                   int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
+                                                   float4* __restrict__ nodes, uint32_t* __restrict__ parent,
+                                                   uint2* __restrict__ children, uint32_t* __restrict__ flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    {
+        uint32_t leaf = (uint32_t)(n - 1 + i);
+        uint32_t prim = prims[i];
+        float4 l = lo[prim], h = hi[prim];
+        uint32_t rope = (i == n - 1) ? kSentinel : chain_top(keys, n, i + 1);
+        nodes[2 * (size_t)leaf] = make_float4(l.x, l.y, l.z, __uint_as_float(prim));
+        nodes[2 * (size_t)leaf + 1] = make_float4(h.x, h.y, h.z, __uint_as_float(rope));
+    }
+    if (i == 0) parent[0] = kSentinel;
+    if (i >= n - 1) return;
+    int d = delta(keys, n, i, i + 1) > delta(keys, n, i, i - 1) ? 1 : -1;
+    int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(keys, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + (d < 0 ? -1 : 0);
+    int lo_i = i < j ? i : j, hi_i = i < j ? j : i;
+    uint32_t left = lo_i == gamma ? (uint32_t)(n - 1 + gamma) : (uint32_t)gamma;
+    uint32_t right = hi_i == gamma + 1 ? (uint32_t)(n - 1 + gamma + 1) : (uint32_t)(gamma + 1);
+    children[i] = make_uint2(left, right);
+    parent[left] = (uint32_t)i;
+    parent[right] = (uint32_t)i;
+    uint32_t rope = hi_i == n - 1 ? kSentinel : chain_top(keys, n, hi_i + 1);
+    nodes[2 * (size_t)i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(left));
+    nodes[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rope));
+    flags[i] = 0;
+}
+
+void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
+                      const float4* hi, float4* nodes, uint32_t* parent, uint2* children, uint32_t* flags,
+                      hipStream_t s) {
+    if (n == 0) return;
+    k_build_bvh<<<blocks_for(n, 256), 256, 0, s>>>(sorted_keys, sorted_prims, (int)n, lo, hi, nodes, parent, children,
+                                                   flags);
+    GI_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(256) void k_refit(uint32_t n, const uint32_t* __restrict__ parent,
+                                               const uint2* __restrict__ children, float4* nodes, uint32_t* flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || n < 2) return;
+    uint32_t p = parent[n - 1 + i];
+    while (p != kSentinel) {
+        __threadfence();
+        if (atomicAdd(&flags[p], 1u) == 0) return;
+        __threadfence();
+        uint2 c = children[p];
+        float4 l0 = nodes[2 * (size_t)c.x];
+        float4 h0 = nodes[2 * (size_t)c.x + 1];
+        float4 l1 = nodes[2 * (size_t)c.y];
+        float4 h1 = nodes[2 * (size_t)c.y + 1];
+        float4 pl = nodes[2 * (size_t)p], ph = nodes[2 * (size_t)p + 1];
+        pl.x = fminf(l0.x, l1.x); pl.y = fminf(l0.y, l1.y); pl.z = fminf(l0.z, l1.z);
+        ph.x = fmaxf(h0.x, h1.x); ph.y = fmaxf(h0.y, h1.y); ph.z = fmaxf(h0.z, h1.z);
+        nodes[2 * (size_t)p] = pl;
+        nodes[2 * (size_t)p + 1] = ph;
+        p = parent[p];
+    }
+}
+
+void launch_refit(uint32_t n, const uint32_t* parent, const uint2* children, float4* nodes, uint32_t* flags,
+                  hipStream_t s) {
+    if (n < 2) return;
+    k_refit<<<blocks_for(n, 256), 256, 0, s>>>(n, parent, children, nodes, flags);
+    GI_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(256) void k_refresh_leaves(uint32_t n, const float4* __restrict__ lo,
+                                                        const float4* __restrict__ hi, float4* nodes,
+                                                        uint32_t* flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    size_t leaf = (size_t)(n - 1 + i);
+    float4 L = nodes[2 * leaf], H = nodes[2 * leaf + 1];
+    uint32_t prim = __float_as_uint(L.w);
+    float4 l = lo[prim], h = hi[prim];
+    nodes[2 * leaf] = make_float4(l.x, l.y, l.z, L.w);
+    nodes[2 * leaf + 1] = make_float4(h.x, h.y, h.z, H.w);
+    if (i + 1 < n) flags[i] = 0;
+}
+
+void launch_refresh_leaves(uint32_t n, const float4* lo, const float4* hi, float4* nodes, uint32_t* flags,
+                           hipStream_t s) {
+    if (n == 0) return;
+    k_refresh_leaves<<<blocks_for(n, 256), 256, 0, s>>>(n, lo, hi, nodes, flags);
+    GI_LAUNCH_CHECK();
+}
+
+// --------------------------------------------------------------- traversal
+
+constexpr int TRAV_THREADS = 256;
+constexpr uint32_t TRAV_STAGE = 256;  // candidate pairs staged per wave in LDS
+
+__device__ __forceinline__ void flush_stage(uint2* stage, uint32_t fill, uint2* cand, uint32_t cap,
+                                            uint32_t* counter) {
+    wave_sync_lds();
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(counter, fill);
+    base = __shfl(base, 0);
+    for (uint32_t i = lane_id(); i < fill; i += 64) {
+        uint32_t pos = base + i;
+        if (pos < cap) cand[pos] = stage[i];
+    }
+    wave_sync_lds();
+}
+
+// One thread per leaf of B (in B's Morton order, so a wave queries a compact
+// cluster), stackless traversal of A along the ropes. Overlapping leaf pairs
+// are compacted per wave (ballot + popcount) into LDS and flushed to the
+// global candidate queue with one atomic per TRAV_STAGE pairs.
+__global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restrict__ nodesA, uint32_t nA,
+                                                           const float4* __restrict__ nodesB, uint32_t nB,
+                                                           uint2* __restrict__ cand, uint32_t cap,
+                                                           uint32_t* __restrict__ counters) {
+    __shared__ uint2 stage_all[TRAV_THREADS / 64][TRAV_STAGE];
+    uint2* stage = stage_all[threadIdx.x >> 6];
+    uint32_t k = blockIdx.x * TRAV_THREADS + threadIdx.x;
+    float qlx = 0.f, qly = 0.f, qlz = 0.f, qhx = 0.f, qhy = 0.f, qhz = 0.f;
+    uint32_t b = 0;
+    uint32_t node = kSentinel;
+    if (k < nB) {
+        size_t leaf = (size_t)(nB - 1 + k);
+        float4 L = nodesB[2 * leaf], H = nodesB[2 * leaf + 1];
+        qlx = L.x; qly = L.y; qlz = L.z;
+        qhx = H.x; qhy = H.y; qhz = H.z;
+        b = __float_as_uint(L.w);
+        node = 0;
+    }
+    const uint32_t leafBase = nA - 1;
+    uint64_t lt = lanemask_lt();
+    uint32_t fill = 0;
+    while (__any(node != kSentinel)) {
+        bool hit = false;
+        uint32_t a = 0;
+        if (node != kSentinel) {
+            float4 L = nodesA[2 * (size_t)node], H = nodesA[2 * (size_t)node + 1];
+            bool ov = L.x <= qhx && qlx <= H.x && L.y <= qhy && qly <= H.y && L.z <= qhz && qlz <= H.z;
+            uint32_t w0 = __float_as_uint(L.w), w1 = __float_as_uint(H.w);
+            if (ov && node >= leafBase) { hit = true; a = w0; }
+            node = (ov && node < leafBase) ? w0 : w1;
+        }
+        uint64_t m = __ballot(hit);
+        if (m) {
+            uint32_t cnt = (uint32_t)__popcll(m);
+            if (fill + cnt > TRAV_STAGE) {
+                flush_stage(stage, fill, cand, cap, &counters[CNT_TRAVERSAL_HITS]);
+                fill = 0;
+            }
+            if (hit) stage[fill + (uint32_t)__popcll(m & lt)] = make_uint2(a, b);
+            fill += cnt;
+        }
+    }
+    if (fill) flush_stage(stage, fill, cand, cap, &counters[CNT_TRAVERSAL_HITS]);
+}
+
+void launch_traverse(MeshDev A, MeshDev B, uint2* cand, uint32_t cap, uint32_t* counters, hipStream_t s) {
+    if (A.nt == 0 || B.nt == 0) return;
+    k_traverse<<<blocks_for(B.nt, TRAV_THREADS), TRAV_THREADS, 0, s>>>(A.nodes, A.nt, B.nodes, B.nt, cand, cap,
+                                                                       counters);
+    GI_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ filter
+
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter, uint32_t& slot) {
+    uint64_t m = __ballot(pred);
+    uint32_t base = 0;
+    if (m) {
+        uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+        if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+        base = __shfl(base, (int)leader);
+    }
+    slot = base + (uint32_t)__popcll(m & lanemask_lt());
+    return (uint32_t)__popcll(m);
+}
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t v) {
+    __shared__ uint32_t red[4];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if (lane_id() == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t t = red[0] + red[1] + red[2] + red[3];
+    return t;
+}
+
+constexpr uint32_t kStatusShift = 30;
+constexpr uint32_t kIndexMask = (1u << kStatusShift) - 1u;
+
+__global__ __launch_bounds__(256) void k_filter(MeshDev A, MeshDev B, const uint2* __restrict__ cand, uint32_t cap,
+                                                const uint32_t* __restrict__ grid_flags, uint2* __restrict__ exact_list,
+                                                uint32_t exact_cap, uint32_t* __restrict__ counters) {
+    uint32_t n = counters[CNT_TRAVERSAL_HITS];
+    if (n > cap) n = cap;
+    const bool grid = grid_flags[0] && grid_flags[1];
+    uint32_t c_cand = 0, c_no = 0, c_green = 0, c_yellow = 0;
+    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        uint32_t i = base + threadIdx.x;
+        int status = PAIR_NO;
+        uint32_t a = 0, b = 0;
+        if (i < n) {
+            uint2 pr = cand[i];
+            a = pr.x;
+            b = pr.y;
+            const uint32_t* ta = A.tri + 3 * (size_t)a;
+            const uint32_t* tb = B.tri + 3 * (size_t)b;
+            d3 PA[3], PB[3];
+            if (grid) {
+                for (int v = 0; v < 3; ++v) {
+                    int4 p = A.vi[ta[v]], q = B.vi[tb[v]];
+                    PA[v] = d3{(double)p.x, (double)p.y, (double)p.z};
+                    PB[v] = d3{(double)q.x, (double)q.y, (double)q.z};
+                }
+            } else {
+                for (int v = 0; v < 3; ++v) {
+                    double4 p = A.v64[ta[v]], q = B.v64[tb[v]];
+                    PA[v] = d3{p.x, p.y, p.z};
+                    PB[v] = d3{q.x, q.y, q.z};
+                }
+            }
+            // exact closed box overlap (the CGAL box_intersection_d candidate set)
+            bool ov = fmax(fmax(PA[0].x, PA[1].x), PA[2].x) >= fmin(fmin(PB[0].x, PB[1].x), PB[2].x) &&
+                      fmax(fmax(PB[0].x, PB[1].x), PB[2].x) >= fmin(fmin(PA[0].x, PA[1].x), PA[2].x) &&
+                      fmax(fmax(PA[0].y, PA[1].y), PA[2].y) >= fmin(fmin(PB[0].y, PB[1].y), PB[2].y) &&
+                      fmax(fmax(PB[0].y, PB[1].y), PB[2].y) >= fmin(fmin(PA[0].y, PA[1].y), PA[2].y) &&
+                      fmax(fmax(PA[0].z, PA[1].z), PA[2].z) >= fmin(fmin(PB[0].z, PB[1].z), PB[2].z) &&
+                      fmax(fmax(PB[0].z, PB[1].z), PB[2].z) >= fmin(fmin(PA[0].z, PA[1].z), PA[2].z);
+            if (ov) {
+                c_cand++;
+                status = classify_pair_fp64(PA, PB);
+                c_no += status == PAIR_NO;
+                c_green += status == PAIR_GREEN;
+                c_yellow += status == PAIR_YELLOW;
+            }
+        }
+        uint32_t slot;
+        bool emit = status != PAIR_NO;
+        wave_append(emit, &counters[CNT_EXACT_LIST], slot);
+        if (emit && slot < exact_cap) exact_list[slot] = make_uint2(a, b | ((uint32_t)status << kStatusShift));
+    }
+    uint32_t t;
+    t = block_sum(c_cand); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_CANDIDATES], t);
+    t = block_sum(c_no); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_FILTER_NO], t);
+    t = block_sum(c_green); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_FILTER_GREEN], t);
+    t = block_sum(c_yellow); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_FILTER_YELLOW], t);
+}
+
+void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const uint32_t* grid_flags,
+                   uint2* exact_list, uint32_t exact_cap, uint32_t* counters, hipStream_t s) {
+    if (A.nt == 0 || B.nt == 0) return;
+    k_filter<<<2048, 256, 0, s>>>(A, B, cand, cap, grid_flags, exact_list, exact_cap, counters);
+    GI_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------- exact
+
+typedef ExactTT<GridPolicy> GridExact;
+
+__device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& total) {
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane_id() >= (uint32_t)o) x += y;
+    }
+    total = __shfl(x, 63);
+    return x - v;
+}
+
+__global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2* __restrict__ list, uint32_t cap,
+                                               const uint32_t* __restrict__ grid_flags, int bitsB,
+                                               uint64_t* __restrict__ rec_key, uint2* __restrict__ rec_seg,
+                                               uint32_t rec_cap, __int128* __restrict__ seg_pool, uint32_t seg_cap,
+                                               uint32_t* __restrict__ counters) {
+    if (!(grid_flags[0] && grid_flags[1])) return;  // general doubles: host residual path
+    uint32_t n = counters[CNT_EXACT_LIST];
+    if (n > cap) n = cap;
+    uint32_t c_yes = 0, c_no = 0, c_cop = 0, c_deg = 0, c_bad = 0;
+    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        uint32_t i = base + threadIdx.x;
+        GridExact::Result r;
+        r.hit = 0;
+        r.nseg = 0;
+        GridExact::V3 PA[3], PB[3];
+        uint32_t a = 0, b = 0;
+        if (i < n) {
+            uint2 e = list[i];
+            a = e.x;
+            b = e.y & kIndexMask;
+            uint32_t status = e.y >> kStatusShift;
+            const uint32_t* ta = A.tri + 3 * (size_t)a;
+            const uint32_t* tb = B.tri + 3 * (size_t)b;
+            for (int v = 0; v < 3; ++v) {
+                int4 p = A.vi[ta[v]], q = B.vi[tb[v]];
+                PA[v].v[0] = p.x; PA[v].v[1] = p.y; PA[v].v[2] = p.z;
+                PB[v].v[0] = q.x; PB[v].v[1] = q.y; PB[v].v[2] = q.z;
+            }
+            GridExact::intersect(PA, PB, r);
+            if (status == PAIR_YELLOW) { c_yes += r.hit; c_no += !r.hit; }
+            if (status == PAIR_GREEN && !r.hit) c_bad++;
+            c_cop += r.hit && r.coplanar;
+            c_deg += r.hit && r.degenerate;
+        }
+        uint32_t slot;
+        wave_append(r.hit != 0, &counters[CNT_RESULTS], slot);
+        uint32_t nseg = r.hit ? (uint32_t)r.nseg : 0u;
+        uint32_t wtot;
+        uint32_t soff = wave_exclusive_sum(nseg, wtot);
+        uint32_t sbase = 0;
+        if (wtot) {
+            if (lane_id() == 0) sbase = atomicAdd(&counters[CNT_SEGMENTS], wtot);
+            sbase = __shfl(sbase, 0);
+        }
+        if (r.hit && slot < rec_cap) {
+            rec_key[slot] = ((uint64_t)a << bitsB) | (uint64_t)b;
+            rec_seg[slot] = make_uint2(sbase + soff, nseg);
+            for (uint32_t sI = 0; sI < nseg; ++sI) {
+                uint32_t dst = sbase + soff + sI;
+                if (dst >= seg_cap) break;
+                GridExact::HP p, q;
+                GridExact::segment(PA, PB, r, (int)sI, p, q);
+                __int128* o = seg_pool + 8 * (size_t)dst;
+                o[0] = p.h[0]; o[1] = p.h[1]; o[2] = p.h[2]; o[3] = p.w;
+                o[4] = q.h[0]; o[5] = q.h[1]; o[6] = q.h[2]; o[7] = q.w;
+            }
+        }
+    }
+    uint32_t t;
+    t = block_sum(c_yes); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_EXACT_YELLOW_YES], t);
+    t = block_sum(c_no); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_EXACT_YELLOW_NO], t);
+    t = block_sum(c_cop); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_COPLANAR], t);
+    t = block_sum(c_deg); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_DEGENERATE], t);
+    t = block_sum(c_bad); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_UNSOUND], t);
+}
+
+void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
+                  int bitsB, uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, __int128* seg_pool,
+                  uint32_t seg_cap, uint32_t* counters, hipStream_t s) {
+    if (A.nt == 0 || B.nt == 0) return;
+    k_exact<<<1024, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap, seg_pool,
+                                 seg_cap, counters);
+    GI_LAUNCH_CHECK();
+}
+
+// ----------------------------------------------------------------- gathers
+
+__global__ void k_iota(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    k_iota<<<blocks_for(n, 256), 256, 0, s>>>(p, n);
+    GI_LAUNCH_CHECK();
+}
+
+__global__ void k_gather_pairs(const uint64_t* __restrict__ key, const uint32_t* __restrict__ order,
+                               const uint2* __restrict__ rec_seg, uint32_t n, int bitsB, uint2* __restrict__ pairs,
+                               uint32_t* __restrict__ nseg) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = key[i];
+    pairs[i] = make_uint2((uint32_t)(k >> bitsB), (uint32_t)(k & ((1ull << bitsB) - 1ull)));
+    nseg[i] = rec_seg[order[i]].y;
+}
+
+void launch_gather_pairs(const uint64_t* sorted_key, const uint32_t* order, const uint2* rec_seg, uint32_t n, int bitsB,
+                         uint2* pairs, uint32_t* nseg, hipStream_t s) {
+    if (n == 0) return;
+    k_gather_pairs<<<blocks_for(n, 256), 256, 0, s>>>(sorted_key, order, rec_seg, n, bitsB, pairs, nseg);
+    GI_LAUNCH_CHECK();
+}
+
+__device__ __forceinline__ double i128_to_double(__int128 x) {
+    bool neg = x < 0;
+    unsigned __int128 u = neg ? (unsigned __int128)0 - (unsigned __int128)x : (unsigned __int128)x;
+    double d = (double)(uint64_t)(u >> 64) * 18446744073709551616.0 + (double)(uint64_t)u;
+    return neg ? -d : d;
+}
+
+__global__ void k_gather_segments(const uint32_t* __restrict__ order, const uint2* __restrict__ rec_seg,
+                                  const uint32_t* __restrict__ seg_offsets, uint32_t n,
+                                  const __int128* __restrict__ pool, __int128* __restrict__ out,
+                                  double* __restrict__ f64) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint2 rs = rec_seg[order[i]];
+    uint32_t dst = seg_offsets[i];
+    for (uint32_t s = 0; s < rs.y; ++s) {
+        const __int128* src = pool + 8 * (size_t)(rs.x + s);
+        __int128* o = out + 8 * (size_t)(dst + s);
+        double* f = f64 + 6 * (size_t)(dst + s);
+        for (int e = 0; e < 2; ++e) {
+            __int128 w = src[4 * e + 3];
+            double wd = i128_to_double(w);
+            for (int k = 0; k < 3; ++k) {
+                o[4 * e + k] = src[4 * e + k];
+                f[3 * e + k] = i128_to_double(src[4 * e + k]) / wd;
+            }
+            o[4 * e + 3] = w;
+        }
+    }
+}
+
+void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const uint32_t* seg_offsets, uint32_t n,
+                            const __int128* seg_pool, __int128* seg_out, double* seg_f64, hipStream_t s) {
+    if (n == 0) return;
+    k_gather_segments<<<blocks_for(n, 256), 256, 0, s>>>(order, rec_seg, seg_offsets, n, seg_pool, seg_out, seg_f64);
+    GI_LAUNCH_CHECK();
+}
+
+}  // namespace gi

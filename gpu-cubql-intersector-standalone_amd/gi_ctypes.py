@@ -1,0 +1,228 @@
+"""ctypes binding of libgi.so (the C ABI in include/gi_intersector.h).
+
+`Intersector` mirrors the reference's KernelBVHController call pattern
+(src/GPUIntersector/KernelBVHController.h:36-95): set both meshes, optionally
+move them, run the intersection, read sorted pairs and exact segments.
+Loading fails loudly when libgi.so has not been built; there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from fractions import Fraction
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgi.so")
+
+
+class GiConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("candidate_reserve", C.c_uint32), ("host_threads", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class GiOptions(C.Structure):
+    _fields_ = [("output_on_device", C.c_int32), ("want_segments", C.c_int32), ("rebuild", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+_STAT_U64 = ["traversal_hits", "candidates", "filter_no", "filter_green", "filter_yellow", "exact_pairs",
+             "exact_yellow_yes", "exact_yellow_no", "host_pairs", "host_yes", "pairs", "segments", "coplanar",
+             "degenerate"]
+_STAT_F32 = ["ms_total", "ms_build", "ms_traverse", "ms_filter", "ms_exact", "ms_output", "ms_host", "pad"]
+
+
+class GiStats(C.Structure):
+    _fields_ = ([(n, C.c_uint64) for n in _STAT_U64] + [("grid", C.c_int32), ("retries", C.c_int32)]
+                + [(n, C.c_float) for n in _STAT_F32])
+
+    def as_dict(self):
+        d = {n: int(getattr(self, n)) for n in _STAT_U64}
+        d.update({n: float(getattr(self, n)) for n in _STAT_F32 if n != "pad"})
+        d["grid"] = int(self.grid)
+        d["retries"] = int(self.retries)
+        return d
+
+
+class GiResult(C.Structure):
+    _fields_ = [("n_pairs", C.c_uint64), ("pairs", C.POINTER(C.c_uint32)), ("seg_offsets", C.POINTER(C.c_uint32)),
+                ("n_segments", C.c_uint64), ("exact_limbs", C.c_int32), ("on_device", C.c_int32),
+                ("seg_exact", C.POINTER(C.c_uint64)), ("seg_exp2", C.POINTER(C.c_int32)),
+                ("seg_f64", C.POINTER(C.c_double)), ("stats", GiStats), ("internal", C.c_void_p)]
+
+
+class GiError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"gi status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+SYMBOLS = ["gi_create", "gi_destroy", "gi_last_error", "gi_status_string", "gi_set_mesh", "gi_set_mesh_device",
+           "gi_set_transform", "gi_set_transform_matrix", "gi_build", "gi_intersect", "gi_result_free",
+           "gi_intersect_oneshot", "gi_clear_device", "gi_reconstruct_device", "gi_exact_pair_host"]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, up = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint32)
+        L.gi_create.argtypes = [C.POINTER(GiConfig), C.POINTER(vp)]
+        L.gi_destroy.argtypes = [vp]
+        L.gi_last_error.argtypes = [vp]
+        L.gi_last_error.restype = C.c_char_p
+        L.gi_status_string.argtypes = [C.c_int]
+        L.gi_status_string.restype = C.c_char_p
+        L.gi_set_mesh.argtypes = [vp, C.c_int, dp, C.c_uint32, up, C.c_uint32]
+        L.gi_set_mesh_device.argtypes = [vp, C.c_int, vp, C.c_uint32, vp, C.c_uint32]
+        L.gi_set_transform.argtypes = [vp, C.c_int, dp, dp, dp, C.c_int]
+        L.gi_set_transform_matrix.argtypes = [vp, C.c_int, dp, C.c_int]
+        L.gi_build.argtypes = [vp]
+        L.gi_intersect.argtypes = [vp, C.POINTER(GiOptions), C.POINTER(GiResult)]
+        L.gi_result_free.argtypes = [C.POINTER(GiResult)]
+        L.gi_intersect_oneshot.argtypes = [C.POINTER(GiConfig), dp, C.c_uint32, up, C.c_uint32, dp, C.c_uint32, up,
+                                           C.c_uint32, C.POINTER(GiOptions), C.POINTER(GiResult)]
+        L.gi_clear_device.argtypes = [vp]
+        L.gi_reconstruct_device.argtypes = [vp]
+        ip = C.POINTER(C.c_int32)
+        L.gi_exact_pair_host.argtypes = [dp, dp, ip, ip, ip, ip, ip, C.POINTER(C.c_uint64), C.c_int32]
+        for name in SYMBOLS:
+            if name not in ("gi_last_error", "gi_status_string"):
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _limbs_to_int(words) -> int:
+    v = 0
+    for i, w in enumerate(words):
+        v |= int(w) << (64 * i)
+    n = 64 * len(words)
+    if v >> (n - 1):
+        v -= 1 << n
+    return v
+
+
+def decode_segments(exact: np.ndarray, exp2, offsets: np.ndarray):
+    """exact: (S, 8, limbs) uint64. Returns per pair a sorted list of canonical
+    ((x0,y0,z0),(x1,y1,z1)) Fraction endpoints."""
+    out = []
+    S = exact.shape[0]
+    pts = []
+    for s in range(S):
+        nums = [_limbs_to_int(exact[s, e]) for e in range(8)]
+        sc = Fraction(2) ** int(exp2[s]) if exp2 is not None else Fraction(1)
+        p = tuple(Fraction(nums[k], nums[3]) * sc for k in range(3))
+        q = tuple(Fraction(nums[4 + k], nums[7]) * sc for k in range(3))
+        pts.append((p, q) if p <= q else (q, p))
+    for i in range(len(offsets) - 1):
+        out.append(sorted(pts[offsets[i]:offsets[i + 1]]))
+    return out
+
+
+def _check(st, ctx=None):
+    if st != 0:
+        msg = lib().gi_last_error(ctx).decode() if ctx else ""
+        raise GiError(st, f"{lib().gi_status_string(st).decode()} {msg}")
+
+
+class Intersector:
+    """One gi_context: two mesh slots (A = 0, B = 1) on one GPU."""
+
+    def __init__(self, device: int = 0, candidate_reserve: int = 0, host_threads: int = 0):
+        self._ctx = C.c_void_p()
+        cfg = GiConfig(device, candidate_reserve, host_threads, 0)
+        _check(lib().gi_create(C.byref(cfg), C.byref(self._ctx)))
+
+    def close(self):
+        if self._ctx:
+            lib().gi_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mesh(self, slot: int, xyz, tri):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        tri = np.ascontiguousarray(tri, dtype=np.uint32).reshape(-1, 3)
+        _check(lib().gi_set_mesh(self._ctx, slot, xyz.ctypes.data_as(C.POINTER(C.c_double)), xyz.shape[0],
+                                 tri.ctypes.data_as(C.POINTER(C.c_uint32)), tri.shape[0]), self._ctx)
+
+    def set_mesh_device(self, slot: int, xyz_ptr: int, nv: int, tri_ptr: int, nt: int):
+        _check(lib().gi_set_mesh_device(self._ctx, slot, C.c_void_p(xyz_ptr), nv, C.c_void_p(tri_ptr), nt),
+               self._ctx)
+
+    def set_transform(self, slot: int, rot_deg, trans, center, snap: bool = False):
+        a = (C.c_double * 3)(*rot_deg)
+        b = (C.c_double * 3)(*trans)
+        c = (C.c_double * 3)(*center)
+        _check(lib().gi_set_transform(self._ctx, slot, a, b, c, int(snap)), self._ctx)
+
+    def set_transform_matrix(self, slot: int, m, snap: bool = False):
+        arr = (C.c_double * 12)(*np.asarray(m, dtype=np.float64).ravel())
+        _check(lib().gi_set_transform_matrix(self._ctx, slot, arr, int(snap)), self._ctx)
+
+    def build(self):
+        _check(lib().gi_build(self._ctx), self._ctx)
+
+    def clear_device(self):
+        _check(lib().gi_clear_device(self._ctx), self._ctx)
+
+    def reconstruct_device(self):
+        _check(lib().gi_reconstruct_device(self._ctx), self._ctx)
+
+    def intersect_raw(self, output_on_device=False, want_segments=True, rebuild=False) -> GiResult:
+        """Runs the query; the caller must gi_result_free the returned struct."""
+        res = GiResult()
+        opt = GiOptions(int(output_on_device), int(want_segments), int(rebuild), 0)
+        _check(lib().gi_intersect(self._ctx, C.byref(opt), C.byref(res)), self._ctx)
+        return res
+
+    def intersect(self, want_segments=True, rebuild=False, decode=True):
+        res = self.intersect_raw(False, want_segments, rebuild)
+        try:
+            return result_to_py(res, want_segments, decode)
+        finally:
+            lib().gi_result_free(C.byref(res))
+
+
+def result_to_py(res: GiResult, want_segments=True, decode=True):
+    n = int(res.n_pairs)
+    S = int(res.n_segments)
+    pairs = np.ctypeslib.as_array(res.pairs, shape=(max(2 * n, 1),))[: 2 * n].reshape(n, 2).copy() if n else \
+        np.zeros((0, 2), dtype=np.uint32)
+    offs = np.ctypeslib.as_array(res.seg_offsets, shape=(n + 1,)).copy()
+    out = dict(pairs=pairs, seg_offsets=offs, stats=res.stats.as_dict(), n_segments=S)
+    if want_segments and S:
+        L = int(res.exact_limbs)
+        exact = np.ctypeslib.as_array(res.seg_exact, shape=(S * 8 * L,)).reshape(S, 8, L).copy()
+        exp2 = np.ctypeslib.as_array(res.seg_exp2, shape=(S,)).copy() if res.seg_exp2 else None
+        f64 = np.ctypeslib.as_array(res.seg_f64, shape=(S * 6,)).reshape(S, 6).copy()
+        out.update(seg_exact=exact, seg_exp2=exp2, seg_f64=f64)
+        if decode:
+            out["segments"] = decode_segments(exact, exp2, offs)
+    elif want_segments:
+        out["segments"] = [[] for _ in range(n)]
+    return out
+
+
+def exact_pair_host(a, b, limbs_cap=64):
+    """Host residual stage on one pair: (hit, coplanar, canonical segments)."""
+    hit, cop, nseg, exp2, limbs = (C.c_int32() for _ in range(5))
+    buf = (C.c_uint64 * (6 * 8 * limbs_cap))()
+    da = (C.c_double * 9)(*[float(x) for x in a])
+    db = (C.c_double * 9)(*[float(x) for x in b])
+    _check(lib().gi_exact_pair_host(da, db, C.byref(hit), C.byref(cop), C.byref(nseg), C.byref(exp2), C.byref(limbs),
+                                    buf, limbs_cap))
+    raw = np.ctypeslib.as_array(buf).reshape(6, 8, limbs_cap)
+    segs = decode_segments(raw[: nseg.value], [exp2.value] * nseg.value, np.array([0, nseg.value]))[0] \
+        if nseg.value else []
+    return bool(hit.value), bool(cop.value), segs

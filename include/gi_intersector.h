@@ -1,0 +1,161 @@
+/*
+ * gi_intersector.h -- C ABI of the MI355X mesh/mesh intersection library
+ * (libgi.so). Plain C types only; every function returns a gi_status and
+ * never exits the process.
+ *
+ * The entry points replace the reference's GPUIntersector surface
+ * (paths relative to the reference root):
+ *
+ *   gi_create / gi_destroy      KernelBVHController::KernelBVHController,
+ *                               ~KernelBVHController, cleanup
+ *                               (src/GPUIntersector/KernelBVHController.h:38-39, 66)
+ *   gi_set_mesh                 the per-mesh half of KernelBVHController::construct
+ *                               (KernelBVHController.h:46-57; .cu:444-597)
+ *   gi_set_transform            KernelBVHController::setTransformBoth /
+ *                               setTranslation (KernelBVHController.h:72-80;
+ *                               .cu:289-412), one mesh per call
+ *   gi_build                    the BVH build inside construct (.cu:546-580)
+ *   gi_intersect                KernelBVHController::runIntersectionPipeline
+ *                               (KernelBVHController.h:60-63; .cu:603-699) fused
+ *                               with computeExactIntersectionCurves
+ *                               (src/CPU/MeshIntersectionCurves.h:24-29)
+ *   gi_intersect_oneshot        extern "C" runGridForestIntersectionPipeline
+ *                               (src/GPUIntersector/StandaloneBVHPipeline.h:27-42):
+ *                               everything built per call from host arrays
+ *   gi_clear_device /           KernelBVHController::clearGPU / reconstructGPU
+ *   gi_reconstruct_device       (KernelBVHController.h:67-69; .cu:731-948)
+ *   gi_result_free              the caller's std::free of the pair buffer
+ *                               (apps/mainApplication.cpp:793-796)
+ *   gi_exact_pair_host          CGAL::do_intersect + CGAL::intersection on one
+ *                               triangle pair (src/CPU/YellowFilter.h:122,
+ *                               src/CPU/MeshIntersectionCurves.cpp:63), as run by
+ *                               the host residual stage
+ *
+ * Semantics shared by all entry points:
+ *   - triangles are closed point sets; a zero-area triangle is the segment or
+ *     point it spans;
+ *   - a pair (a, b) is reported iff triangle a of mesh A and triangle b of
+ *     mesh B have a common point, decided exactly on the double coordinates
+ *     after the rigid transform;
+ *   - pairs come back sorted by (a, b) and unique;
+ *   - segments are exact: non-coplanar pairs give one segment (a point is a
+ *     segment with equal endpoints), coplanar pairs give the boundary edges of
+ *     their overlap polygon.
+ */
+#ifndef GI_INTERSECTOR_H
+#define GI_INTERSECTOR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    GI_OK = 0,
+    GI_ERR_INVALID_ARGUMENT = 1,
+    GI_ERR_HIP = 2,
+    GI_ERR_OUT_OF_MEMORY = 3,
+    GI_ERR_NO_DEVICE = 4,
+    GI_ERR_NOT_READY = 5,   /* a mesh slot has not been set */
+    GI_ERR_INTERNAL = 6,    /* an internal consistency check failed */
+    GI_ERR_TOO_LARGE = 7    /* more than 2^30 triangles in one mesh */
+} gi_status;
+
+typedef struct gi_context gi_context;
+
+typedef struct {
+    int32_t device;             /* HIP device ordinal */
+    uint32_t candidate_reserve; /* initial candidate-queue capacity, 0 = auto */
+    int32_t host_threads;       /* threads of the host residual stage, 0 = auto */
+    int32_t reserved;
+} gi_config;
+
+/* slot 0 is mesh A, slot 1 is mesh B */
+enum { GI_MESH_A = 0, GI_MESH_B = 1 };
+
+typedef struct {
+    int32_t output_on_device;   /* 1: result arrays stay in device memory, owned by
+                                   the context until its next gi_intersect */
+    int32_t want_segments;      /* 0: pairs only */
+    int32_t rebuild;            /* 1: rebuild both BVHs from scratch before the query */
+    int32_t reserved;
+} gi_options;
+
+typedef struct {
+    /* candidate pipeline: traversal (FP32 boxes) -> exact boxes -> filter -> exact */
+    uint64_t traversal_hits;    /* FP32 box overlaps found by the BVH traversal */
+    uint64_t candidates;        /* exact closed double-box overlaps */
+    uint64_t filter_no, filter_green, filter_yellow; /* FP64 filter outcomes */
+    uint64_t exact_pairs;       /* pairs run through the exact integer stage (GPU) */
+    uint64_t exact_yellow_yes, exact_yellow_no;      /* YELLOW pairs resolved exactly */
+    uint64_t host_pairs;        /* pairs run through the host residual stage */
+    uint64_t host_yes;
+    uint64_t pairs, segments, coplanar, degenerate;
+    int32_t grid;               /* 1: both meshes on the integer grid (GPU exact stage) */
+    int32_t retries;            /* capacity regrowths during this call */
+    /* milliseconds (GPU events; host stage by wall clock) */
+    float ms_total, ms_build, ms_traverse, ms_filter, ms_exact, ms_output, ms_host;
+    float pad;
+} gi_stats;
+
+typedef struct {
+    uint64_t n_pairs;
+    uint32_t* pairs;        /* 2 * n_pairs: (a, b), sorted, unique */
+    uint32_t* seg_offsets;  /* n_pairs + 1: segments of pair i are [off[i], off[i+1]) */
+    uint64_t n_segments;
+    int32_t exact_limbs;    /* 64-bit limbs per exact number (2 on the grid path) */
+    int32_t on_device;      /* arrays are device pointers */
+    uint64_t* seg_exact;    /* n_segments * 8 * exact_limbs: x0 y0 z0 w0 x1 y1 z1 w1,
+                               little-endian two's complement; point = (x/w, y/w, z/w)
+                               * 2^seg_exp2, w > 0 */
+    int32_t* seg_exp2;      /* n_segments power-of-two scales (NULL: all zero) */
+    double* seg_f64;        /* n_segments * 6: endpoints rounded to double */
+    gi_stats stats;
+    void* internal;
+} gi_result;
+
+int gi_create(const gi_config* cfg, gi_context** out);
+int gi_destroy(gi_context* ctx);
+const char* gi_last_error(const gi_context* ctx);
+const char* gi_status_string(int status);
+
+/* Host arrays: xyz = 3*nv doubles, tri = 3*nt vertex indices. Copied. */
+int gi_set_mesh(gi_context* ctx, int slot, const double* xyz, uint32_t nv, const uint32_t* tri, uint32_t nt);
+/* Same with device arrays (already resident in HBM). Copied device-to-device. */
+int gi_set_mesh_device(gi_context* ctx, int slot, const double* d_xyz, uint32_t nv, const uint32_t* d_tri,
+                       uint32_t nt);
+/* Rigid motion about `center`: p' = R (p - c) + c + t with R = Rz Ry Rx from
+ * Euler angles in degrees, applied as one affine map the way the reference's
+ * CGAL path does (src/CPU/RotationTools.h:28-104). snap_to_grid rounds the
+ * moved vertices to integers. All zero = identity. Refits the BVH. */
+int gi_set_transform(gi_context* ctx, int slot, const double rot_deg[3], const double trans[3], const double center[3],
+                     int snap_to_grid);
+/* Same with an explicit row-major 3x4 affine matrix. */
+int gi_set_transform_matrix(gi_context* ctx, int slot, const double m[12], int snap_to_grid);
+/* Builds (or refits) whatever is stale. gi_intersect calls it as needed. */
+int gi_build(gi_context* ctx);
+int gi_intersect(gi_context* ctx, const gi_options* opt, gi_result* out);
+int gi_result_free(gi_result* res);
+
+/* Per-call pipeline: copies both meshes, builds, intersects, frees device state. */
+int gi_intersect_oneshot(const gi_config* cfg, const double* xyzA, uint32_t nvA, const uint32_t* triA, uint32_t ntA,
+                         const double* xyzB, uint32_t nvB, const uint32_t* triB, uint32_t ntB,
+                         const gi_options* opt, gi_result* out);
+
+/* Frees all device memory of the context but keeps the host copies; the next
+ * call rebuilds (reference: clearGPU / reconstructGPU). */
+int gi_clear_device(gi_context* ctx);
+int gi_reconstruct_device(gi_context* ctx);
+
+/* Exact intersection of one triangle pair on the host (the residual stage).
+ * a, b: 9 doubles each. seg: up to 6 segments * 8 numbers * limbs_cap limbs.
+ * Returns GI_OK and fills *hit, *nseg, *limbs (limbs used), *exp2. */
+int gi_exact_pair_host(const double a[9], const double b[9], int32_t* hit, int32_t* coplanar, int32_t* nseg,
+                       int32_t* exp2, int32_t* limbs, uint64_t* seg, int32_t limbs_cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
