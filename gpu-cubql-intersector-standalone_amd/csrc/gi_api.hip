@@ -259,6 +259,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     MeshDev A = mesh_dev(MA), B = mesh_dev(MB);
     for (int attempt = 0;; ++attempt) {
         launch_fill_u32(counters, 0u, CNT_COUNT, s);
+        HIP_TRY(hipEventRecord(ctx->ev[6], s));
         launch_traverse(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, counters, s);
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
         launch_filter(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, grid, ctx->exact.get<uint2>(), ctx->cap_exact,
@@ -290,7 +291,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     st.filter_yellow = hc[CNT_FILTER_YELLOW];
     st.grid = hgrid[0] && hgrid[1];
     st.ms_build = elapsed(ctx->ev[0], ctx->ev[1]);
-    st.ms_traverse = elapsed(ctx->ev[1], ctx->ev[2]);
+    st.ms_traverse = elapsed(ctx->ev[6], ctx->ev[2]);
     st.ms_filter = elapsed(ctx->ev[2], ctx->ev[3]);
 
     if (st.grid) {
