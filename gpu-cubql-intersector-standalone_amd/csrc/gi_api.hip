@@ -67,9 +67,10 @@ struct MeshSlot {
     bool built = false;      // BVH topology is valid
     bool dirty = true;       // vertices/boxes/BVH boxes need an update
     Xform xf;
-    DevBuf src, tri, v64, vi, lo, hi, keys, vals, tkeys, tvals, nodes, parent, children, flags;
+    DevBuf src, tri, v64, vi, lo, hi, keys, vals, tkeys, tvals, nodes, range, blocks, parent, top, misc;
     void release() {
-        DevBuf* all[] = {&src, &tri, &v64, &vi, &lo, &hi, &keys, &vals, &tkeys, &tvals, &nodes, &parent, &children, &flags};
+        DevBuf* all[] = {&src, &tri, &v64, &vi, &lo, &hi, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks,
+                         &parent, &top, &misc};
         for (DevBuf* b : all) b->release();
         on_device = false;
         built = false;
@@ -96,14 +97,14 @@ struct gi_context {
     int host_threads = 0;
     uint32_t cand_reserve = 0;
     MeshSlot mesh[2];
-    DevBuf grid, bounds, counters, hist, scan_tmp;
+    DevBuf bounds, counters, hist, scan_tmp;
     DevBuf cand, exact, rec_key, rec_seg, seg_pool;
     uint32_t cap_cand = 0, cap_exact = 0, cap_rec = 0, cap_seg = 0;
     DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2;
     hipEvent_t ev[8] = {};
     void release_device() {
         for (auto& m : mesh) m.release();
-        DevBuf* all[] = {&grid, &bounds, &counters, &hist, &scan_tmp, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
+        DevBuf* all[] = {&bounds, &counters, &hist, &scan_tmp, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
                          &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2};
         for (DevBuf* b : all) b->release();
         cap_cand = cap_exact = cap_rec = cap_seg = 0;
@@ -141,7 +142,7 @@ void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
     if (!m.on_device) upload_mesh(ctx, slot);
     if (!m.dirty && m.built && !rebuild) return;
     hipStream_t s = ctx->stream;
-    uint32_t* grid = ctx->grid.ensure<uint32_t>(2);
+    uint32_t* grid = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2) + CNT_COUNT;
     uint32_t* bounds = ctx->bounds.ensure<uint32_t>(12) + 6 * slot;
     size_t nv = m.nv, nt = m.nt;
     double4* v64 = m.v64.ensure<double4>(nv);
@@ -150,14 +151,16 @@ void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
     float4* hi = m.hi.ensure<float4>(nt);
     size_t nnodes = nt ? 2 * nt - 1 : 1;
     float4* nodes = m.nodes.ensure<float4>(2 * nnodes);
+    uint2* range = m.range.ensure<uint2>(nt);
+    float4* blocks = m.blocks.ensure<float4>(block_levels_elems((uint32_t)nt));
     uint32_t* parent = m.parent.ensure<uint32_t>(nnodes);
-    uint2* children = m.children.ensure<uint2>(nt);
-    uint32_t* flags = m.flags.ensure<uint32_t>(nt);
+    uint32_t* top = m.top.ensure<uint32_t>(nt);
+    uint32_t* top_count = m.misc.ensure<uint32_t>(4);
     launch_fill_u32(grid + slot, 1u, 1, s);
     launch_prepare_vertices(m.src.get<double>(), m.nv, m.xf, v64, vi, grid + slot, s);
     launch_fill_u32(bounds, 0xFFFFFFFFu, 3, s);
     launch_fill_u32(bounds + 3, 0u, 3, s);
-    launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, lo, hi, bounds, s);
+    launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, vi, grid + slot, lo, hi, bounds, s);
     if (!m.built || rebuild) {
         uint32_t* keys = m.keys.ensure<uint32_t>(nt);
         uint32_t* vals = m.vals.ensure<uint32_t>(nt);
@@ -166,13 +169,13 @@ void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
         uint32_t* hist = ctx->hist.ensure<uint32_t>(sort_hist_elems(nt));
         uint32_t* stmp = ctx->scan_tmp.ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
         launch_morton(lo, hi, m.nt, bounds, keys, vals, s);
-        radix_sort_u32(keys, vals, tk, tv, m.nt, 30, hist, stmp, s);
-        launch_build_bvh(keys, vals, m.nt, lo, hi, nodes, parent, children, flags, s);
-        launch_refit(m.nt, parent, children, nodes, flags, s);
+        int in_tmp = radix_sort_u32(keys, vals, tk, tv, m.nt, 30, hist, stmp, s);
+        launch_build_bvh(in_tmp ? tk : keys, in_tmp ? tv : vals, m.nt, lo, hi, nodes, range, parent, top, top_count, s);
+        launch_refit(m.nt, parent, range, nodes, blocks, top, top_count, s);
         m.built = true;
     } else {
-        launch_refresh_leaves(m.nt, lo, hi, nodes, flags, s);
-        launch_refit(m.nt, parent, children, nodes, flags, s);
+        launch_refresh_leaves(m.nt, lo, hi, nodes, s);
+        launch_refit(m.nt, parent, range, nodes, blocks, top, top_count, s);
     }
     HIP_TRY(hipGetLastError());
     m.dirty = false;
@@ -251,11 +254,11 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         ensure_arenas(ctx, (uint32_t)c, (uint32_t)std::max<uint64_t>(c / 4, 1024), (uint32_t)std::max<uint64_t>(c / 4, 1024),
                       (uint32_t)std::max<uint64_t>(c / 4, 1024));
     }
-    uint32_t* counters = ctx->counters.ensure<uint32_t>(CNT_COUNT);
-    uint32_t* grid = ctx->grid.get<uint32_t>();
+    uint32_t* counters = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);
+    uint32_t* grid = counters + CNT_COUNT;
     int bitsA = bits_for(MA.nt), bitsB = bits_for(MB.nt);
-    uint32_t hc[CNT_COUNT];
-    uint32_t hgrid[2];
+    uint32_t hc[CNT_COUNT + 2];
+    uint32_t* hgrid = hc + CNT_COUNT;
     MeshDev A = mesh_dev(MA), B = mesh_dev(MB);
     for (int attempt = 0;; ++attempt) {
         launch_fill_u32(counters, 0u, CNT_COUNT, s);
@@ -270,7 +273,6 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         HIP_TRY(hipEventRecord(ctx->ev[4], s));
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(hgrid, grid, sizeof(hgrid), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         bool over = hc[CNT_TRAVERSAL_HITS] > ctx->cap_cand || hc[CNT_EXACT_LIST] > ctx->cap_exact ||
                     hc[CNT_RESULTS] > ctx->cap_rec || hc[CNT_SEGMENTS] > ctx->cap_seg;
@@ -314,7 +316,10 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         __int128* segs = ctx->segs.ensure<__int128>(8 * (size_t)S);
         double* f64 = ctx->f64.ensure<double>(6 * (size_t)S);
         launch_iota(order, R, s);
-        radix_sort_u64(keys, order, tk, tv, R, bitsA + bitsB, hist, stmp, s);
+        if (radix_sort_u64(keys, order, tk, tv, R, bitsA + bitsB, hist, stmp, s)) {
+            keys = tk;
+            order = tv;
+        }
         launch_gather_pairs(keys, order, ctx->rec_seg.get<uint2>(), R, bitsB, pairs, nseg, s);
         if (!o.want_segments) launch_fill_u32(nseg, 0u, R, s);
         exclusive_scan_u32(nseg, offs, R, stmp, offs + R, s);

@@ -47,32 +47,47 @@ struct MeshDev {
 void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
 void launch_prepare_vertices(const double* src, uint32_t nv, Xform xf, double4* v64, int4* vi, uint32_t* grid_flag,
                              hipStream_t s);
-void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, float4* lo, float4* hi,
-                      uint32_t* bounds6, hipStream_t s);
+void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,
+                      float4* lo, float4* hi, uint32_t* bounds6, hipStream_t s);
 void launch_morton(const float4* lo, const float4* hi, uint32_t nt, const uint32_t* bounds6, uint32_t* keys,
                    uint32_t* vals, hipStream_t s);
 
-// LSD radix sort of (key, value) pairs. tmp_* are ping-pong buffers of n
-// elements; hist needs sort_hist_elems(n) entries, scan_tmp scan_tmp_elems().
+// LSD radix sort of (key, value) pairs, 8 bits per pass. tmp_* are ping-pong
+// buffers of n elements; hist needs sort_hist_elems(n) entries, scan_tmp
+// scan_tmp_elems(sort_hist_elems(n)). Returns 1 when the sorted data ends in
+// the tmp buffers, 0 when it ends in (keys, vals).
 size_t sort_hist_elems(size_t n);
 size_t scan_tmp_elems(size_t n);
-void radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
-                    uint32_t* hist, uint32_t* scan_tmp, hipStream_t s);
-void radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
-                    uint32_t* hist, uint32_t* scan_tmp, hipStream_t s);
+int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
+                   uint32_t* hist, uint32_t* scan_tmp, hipStream_t s);
+int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
+                   uint32_t* hist, uint32_t* scan_tmp, hipStream_t s);
 // exclusive scan of n uint32 (in place allowed); writes the total to *total
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scan_tmp, uint32_t* total,
                         hipStream_t s);
 
+// Karras LBVH over the sorted Morton keys: leaf k (node n-1+k) holds prim
+// sorted_prims[k] and its box; internal node i holds its left child and its
+// rope (next node in pre-order after its subtree); range[i] = its leaf range,
+// parent[] the parent of every node (2n-1 entries). Internal nodes whose leaf
+// range crosses a kRefitChunk boundary are listed in top[0..*top_count).
+constexpr uint32_t kRefitChunk = 1024;
 void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
-                      const float4* hi, float4* nodes, uint32_t* parent, uint2* children, uint32_t* flags,
-                      hipStream_t s);
-void launch_refit(uint32_t n, const uint32_t* parent, const uint2* children, float4* nodes, uint32_t* flags,
-                  hipStream_t s);
-// Refit after a rigid motion: new prim boxes are written into the leaves
-// (leaf k holds prim nodes[2*(n-1+k)].w), then internal boxes bottom-up.
-void launch_refresh_leaves(uint32_t n, const float4* lo, const float4* hi, float4* nodes, uint32_t* flags,
-                           hipStream_t s);
+                      const float4* hi, float4* nodes, uint2* range, uint32_t* parent, uint32_t* top,
+                      uint32_t* top_count, hipStream_t s);
+
+// Internal-node boxes from the leaf boxes (see k_refit_chunks and
+// k_internal_boxes); `blocks` needs block_levels_elems(n) float4.
+constexpr int kMaxBlockLevels = 6;
+struct BlockLevels {
+    const float4* level[kMaxBlockLevels];
+};
+size_t block_levels_elems(uint32_t n);
+void launch_refit(uint32_t n, const uint32_t* parent, const uint2* range, float4* nodes, float4* blocks,
+                  const uint32_t* top, const uint32_t* top_count, hipStream_t s);
+// After a rigid motion: the new prim boxes go into the leaves (leaf k holds
+// prim nodes[2*(n-1+k)].w); then launch_refit.
+void launch_refresh_leaves(uint32_t n, const float4* lo, const float4* hi, float4* nodes, hipStream_t s);
 
 void launch_traverse(MeshDev A, MeshDev B, uint2* cand, uint32_t cap, uint32_t* counters, hipStream_t s);
 void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const uint32_t* grid_flags,

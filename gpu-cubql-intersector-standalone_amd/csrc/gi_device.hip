@@ -120,21 +120,39 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// Grid-stride over triangles with a capped grid, so the mesh bounds take one
+// atomic per block and component instead of one per 256 triangles. Grid
+// meshes gather 16-byte integer vertices instead of 32-byte doubles.
 __global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ tri, uint32_t nt,
-                                                   const double4* __restrict__ v, float4* __restrict__ lo,
+                                                   const double4* __restrict__ v, const int4* __restrict__ vi,
+                                                   const uint32_t* __restrict__ grid_flag, float4* __restrict__ lo,
                                                    float4* __restrict__ hi, uint32_t* bounds6) {
     __shared__ float red[6][4];
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool grid = *grid_flag != 0;
     float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
-    if (i < nt) {
-        double4 a = v[tri[3 * (size_t)i]], b = v[tri[3 * (size_t)i + 1]], c = v[tri[3 * (size_t)i + 2]];
-        double lx = fmin(fmin(a.x, b.x), c.x), hx = fmax(fmax(a.x, b.x), c.x);
-        double ly = fmin(fmin(a.y, b.y), c.y), hy = fmax(fmax(a.y, b.y), c.y);
-        double lz = fmin(fmin(a.z, b.z), c.z), hz = fmax(fmax(a.z, b.z), c.z);
-        l[0] = f32_down(lx); l[1] = f32_down(ly); l[2] = f32_down(lz);
-        h[0] = f32_up(hx); h[1] = f32_up(hy); h[2] = f32_up(hz);
-        lo[i] = make_float4(l[0], l[1], l[2], 0.f);
-        hi[i] = make_float4(h[0], h[1], h[2], 0.f);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x) {
+        uint32_t i0 = tri[3 * (size_t)i], i1 = tri[3 * (size_t)i + 1], i2 = tri[3 * (size_t)i + 2];
+        float bl[3], bh[3];
+        if (grid) {
+            int4 a = vi[i0], b = vi[i1], c = vi[i2];
+            bl[0] = f32_down((double)min(min(a.x, b.x), c.x));
+            bl[1] = f32_down((double)min(min(a.y, b.y), c.y));
+            bl[2] = f32_down((double)min(min(a.z, b.z), c.z));
+            bh[0] = f32_up((double)max(max(a.x, b.x), c.x));
+            bh[1] = f32_up((double)max(max(a.y, b.y), c.y));
+            bh[2] = f32_up((double)max(max(a.z, b.z), c.z));
+        } else {
+            double4 a = v[i0], b = v[i1], c = v[i2];
+            bl[0] = f32_down(fmin(fmin(a.x, b.x), c.x));
+            bl[1] = f32_down(fmin(fmin(a.y, b.y), c.y));
+            bl[2] = f32_down(fmin(fmin(a.z, b.z), c.z));
+            bh[0] = f32_up(fmax(fmax(a.x, b.x), c.x));
+            bh[1] = f32_up(fmax(fmax(a.y, b.y), c.y));
+            bh[2] = f32_up(fmax(fmax(a.z, b.z), c.z));
+        }
+        lo[i] = make_float4(bl[0], bl[1], bl[2], 0.f);
+        hi[i] = make_float4(bh[0], bh[1], bh[2], 0.f);
+        for (int k = 0; k < 3; ++k) { l[k] = fminf(l[k], bl[k]); h[k] = fmaxf(h[k], bh[k]); }
     }
     uint32_t wave = threadIdx.x >> 6;
     for (int k = 0; k < 3; ++k) {
@@ -151,10 +169,11 @@ __global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ 
     }
 }
 
-void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, float4* lo, float4* hi, uint32_t* bounds6,
-                      hipStream_t s) {
+void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,
+                      float4* lo, float4* hi, uint32_t* bounds6, hipStream_t s) {
     if (nt == 0) return;
-    k_tri_boxes<<<blocks_for(nt, 256), 256, 0, s>>>(tri, nt, v64, lo, hi, bounds6);
+    uint32_t blocks = std::min<uint32_t>(blocks_for(nt, 256), 1024);
+    k_tri_boxes<<<blocks, 256, 0, s>>>(tri, nt, v64, vi, grid_flag, lo, hi, bounds6);
     GI_LAUNCH_CHECK();
 }
 
@@ -379,10 +398,12 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(const K* __rest
     }
 }
 
+// Ping-pongs between (keys, vals) and (tk, tv); returns 1 when the sorted
+// data ends in (tk, tv), 0 when it ends in (keys, vals).
 template <class K>
-static void radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32_t n, int bits, uint32_t* hist,
-                            uint32_t* scan_tmp, hipStream_t s) {
-    if (n <= 1) return;
+static int radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32_t n, int bits, uint32_t* hist,
+                           uint32_t* scan_tmp, hipStream_t s) {
+    if (n <= 1) return 0;
     uint32_t ntiles = (n + SORT_TILE - 1) / SORT_TILE;
     int passes = (bits + 7) / 8;
     K* src_k = keys;
@@ -397,20 +418,17 @@ static void radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32
         K* t = src_k; src_k = dst_k; dst_k = t;
         uint32_t* tvv = src_v; src_v = dst_v; dst_v = tvv;
     }
-    if (src_k != keys) {
-        (void)hipMemcpyAsync(keys, src_k, sizeof(K) * n, hipMemcpyDeviceToDevice, s);
-        (void)hipMemcpyAsync(vals, src_v, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
-    }
     GI_LAUNCH_CHECK();
+    return src_k != keys ? 1 : 0;
 }
 
-void radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
-                    uint32_t* hist, uint32_t* scan_tmp, hipStream_t s) {
-    radix_sort_impl<uint32_t>(keys, vals, tmp_keys, tmp_vals, n, bits, hist, scan_tmp, s);
+int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
+                   uint32_t* hist, uint32_t* scan_tmp, hipStream_t s) {
+    return radix_sort_impl<uint32_t>(keys, vals, tmp_keys, tmp_vals, n, bits, hist, scan_tmp, s);
 }
-void radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
-                    uint32_t* hist, uint32_t* scan_tmp, hipStream_t s) {
-    radix_sort_impl<uint64_t>(keys, vals, tmp_keys, tmp_vals, n, bits, hist, scan_tmp, s);
+int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
+                   uint32_t* hist, uint32_t* scan_tmp, hipStream_t s) {
+    return radix_sort_impl<uint64_t>(keys, vals, tmp_keys, tmp_vals, n, bits, hist, scan_tmp, s);
 }
 
 // --------------------------------------------------------- BVH (Karras)
@@ -434,10 +452,12 @@ __device__ __forceinline__ uint32_t chain_top(const uint32_t* __restrict__ k, in
 
 __global__ __launch_bounds__(256) void k_build_bvh(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ prims,
                                                    int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
-                                                   float4* __restrict__ nodes, uint32_t* __restrict__ parent,
-                                                   uint2* __restrict__ children, uint32_t* __restrict__ flags) {
+                                                   float4* __restrict__ nodes, uint2* __restrict__ range,
+                                                   uint32_t* __restrict__ parent, uint32_t* __restrict__ top,
+                                                   uint32_t* __restrict__ top_count) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (i == 0) parent[0] = kSentinel;
     {
         uint32_t leaf = (uint32_t)(n - 1 + i);
         uint32_t prim = prims[i];
@@ -446,7 +466,6 @@ __global__ __launch_bounds__(256) void k_build_bvh(const uint32_t* __restrict__ 
         nodes[2 * (size_t)leaf] = make_float4(l.x, l.y, l.z, __uint_as_float(prim));
         nodes[2 * (size_t)leaf + 1] = make_float4(h.x, h.y, h.z, __uint_as_float(rope));
     }
-    if (i == 0) parent[0] = kSentinel;
     if (i >= n - 1) return;
     int d = delta(keys, n, i, i + 1) > delta(keys, n, i, i - 1) ? 1 : -1;
     int dmin = delta(keys, n, i, i - d);
@@ -466,57 +485,184 @@ __global__ __launch_bounds__(256) void k_build_bvh(const uint32_t* __restrict__ 
     int lo_i = i < j ? i : j, hi_i = i < j ? j : i;
     uint32_t left = lo_i == gamma ? (uint32_t)(n - 1 + gamma) : (uint32_t)gamma;
     uint32_t right = hi_i == gamma + 1 ? (uint32_t)(n - 1 + gamma + 1) : (uint32_t)(gamma + 1);
-    children[i] = make_uint2(left, right);
     parent[left] = (uint32_t)i;
     parent[right] = (uint32_t)i;
+    range[i] = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
     uint32_t rope = hi_i == n - 1 ? kSentinel : chain_top(keys, n, hi_i + 1);
     nodes[2 * (size_t)i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(left));
     nodes[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rope));
-    flags[i] = 0;
-}
-
-void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
-                      const float4* hi, float4* nodes, uint32_t* parent, uint2* children, uint32_t* flags,
-                      hipStream_t s) {
-    if (n == 0) return;
-    k_build_bvh<<<blocks_for(n, 256), 256, 0, s>>>(sorted_keys, sorted_prims, (int)n, lo, hi, nodes, parent, children,
-                                                   flags);
-    GI_LAUNCH_CHECK();
-}
-
-__global__ __launch_bounds__(256) void k_refit(uint32_t n, const uint32_t* __restrict__ parent,
-                                               const uint2* __restrict__ children, float4* nodes, uint32_t* flags) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || n < 2) return;
-    uint32_t p = parent[n - 1 + i];
-    while (p != kSentinel) {
-        __threadfence();
-        if (atomicAdd(&flags[p], 1u) == 0) return;
-        __threadfence();
-        uint2 c = children[p];
-        float4 l0 = nodes[2 * (size_t)c.x];
-        float4 h0 = nodes[2 * (size_t)c.x + 1];
-        float4 l1 = nodes[2 * (size_t)c.y];
-        float4 h1 = nodes[2 * (size_t)c.y + 1];
-        float4 pl = nodes[2 * (size_t)p], ph = nodes[2 * (size_t)p + 1];
-        pl.x = fminf(l0.x, l1.x); pl.y = fminf(l0.y, l1.y); pl.z = fminf(l0.z, l1.z);
-        ph.x = fmaxf(h0.x, h1.x); ph.y = fmaxf(h0.y, h1.y); ph.z = fmaxf(h0.z, h1.z);
-        nodes[2 * (size_t)p] = pl;
-        nodes[2 * (size_t)p + 1] = ph;
-        p = parent[p];
+    // wave-aggregated append to the top-node list
+    bool is_top = (uint32_t)lo_i / kRefitChunk != (uint32_t)hi_i / kRefitChunk;
+    uint64_t m = __ballot(is_top);
+    if (m) {
+        uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(top_count, (uint32_t)__popcll(m));
+        base = __shfl(base, (int)leader);
+        if (is_top) top[base + (uint32_t)__popcll(m & lanemask_lt())] = (uint32_t)i;
     }
 }
 
-void launch_refit(uint32_t n, const uint32_t* parent, const uint2* children, float4* nodes, uint32_t* flags,
-                  hipStream_t s) {
+void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
+                      const float4* hi, float4* nodes, uint2* range, uint32_t* parent, uint32_t* top,
+                      uint32_t* top_count, hipStream_t s) {
+    if (n == 0) return;
+    launch_fill_u32(top_count, 0u, 1, s);
+    k_build_bvh<<<blocks_for(n, 256), 256, 0, s>>>(sorted_keys, sorted_prims, (int)n, lo, hi, nodes, range, parent,
+                                                   top, top_count);
+    GI_LAUNCH_CHECK();
+}
+
+// Fence-free refit, in two phases. Device-scope fences (the usual bottom-up
+// refit with a global atomic per node) write back and invalidate the per-XCD
+// L2 on gfx950, which made that refit 6 ms for 2^20 triangles.
+//
+// Phase 1: internal node i of a Karras tree covers the contiguous sorted-leaf
+// range range[i] = [lo, hi], and i is lo or hi. Every node whose range lies in
+// one chunk of kRefitChunk leaves is refit bottom-up by the workgroup that owns
+// the chunk, synchronising through LDS only (workgroup-scope fences).
+//
+// Phase 2: the few "top" nodes whose range crosses a chunk boundary (listed by
+// the build kernel) take the union of their leaf range directly: a 64-ary
+// hierarchy of block boxes (level k element j covers leaves
+// [j*64^k, (j+1)*64^k)) lets one wave union any range with at most two partial
+// 64-element reads per level.
+
+__global__ __launch_bounds__(256) void k_refit_chunks(uint32_t n, const uint32_t* __restrict__ parent,
+                                                      const uint2* __restrict__ range, float4* nodes) {
+    __shared__ uint32_t flag[kRefitChunk];
+    __shared__ float side_box[kRefitChunk][2][6];
+    const uint32_t cs = blockIdx.x * kRefitChunk;
+    const uint32_t ce = min(n, cs + kRefitChunk);
+    for (uint32_t i = threadIdx.x; i < kRefitChunk; i += blockDim.x) flag[i] = 0;
+    __syncthreads();
+    for (uint32_t j = cs + threadIdx.x; j < ce; j += blockDim.x) {
+        uint32_t node = n - 1 + j;
+        float4 L = nodes[2 * (size_t)node], H = nodes[2 * (size_t)node + 1];
+        float b[6] = {L.x, L.y, L.z, H.x, H.y, H.z};
+        for (;;) {
+            uint32_t p = parent[node];
+            if (p == kSentinel) break;
+            uint2 r = range[p];
+            if (r.x < cs || r.y >= ce) break;  // top node: phase 2
+            float4 PL = nodes[2 * (size_t)p];
+            int side = __float_as_uint(PL.w) == node ? 0 : 1;
+            uint32_t slot = p - cs;
+            for (int k = 0; k < 6; ++k) side_box[slot][side][k] = b[k];
+            __threadfence_block();
+            if (atomicAdd(&flag[slot], 1u) == 0) break;  // sibling not done yet: it continues
+            __threadfence_block();
+            for (int k = 0; k < 3; ++k) b[k] = fminf(b[k], side_box[slot][1 - side][k]);
+            for (int k = 3; k < 6; ++k) b[k] = fmaxf(b[k], side_box[slot][1 - side][k]);
+            float4 PH = nodes[2 * (size_t)p + 1];
+            nodes[2 * (size_t)p] = make_float4(b[0], b[1], b[2], PL.w);
+            nodes[2 * (size_t)p + 1] = make_float4(b[3], b[4], b[5], PH.w);
+            node = p;
+        }
+    }
+}
+
+__device__ __forceinline__ void box_absorb(const float4* __restrict__ e, size_t idx, float* b) {
+    float4 l = e[2 * idx], h = e[2 * idx + 1];
+    b[0] = fminf(b[0], l.x); b[1] = fminf(b[1], l.y); b[2] = fminf(b[2], l.z);
+    b[3] = fmaxf(b[3], h.x); b[4] = fmaxf(b[4], h.y); b[5] = fmaxf(b[5], h.z);
+}
+
+__device__ __forceinline__ void wave_box_reduce(float* b) {
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int k = 0; k < 3; ++k) b[k] = fminf(b[k], __shfl_xor(b[k], o));
+        for (int k = 3; k < 6; ++k) b[k] = fmaxf(b[k], __shfl_xor(b[k], o));
+    }
+}
+
+// one wave per output block of 64 input elements
+__global__ __launch_bounds__(256) void k_block_boxes(const float4* __restrict__ in, uint32_t n_in,
+                                                     float4* __restrict__ out, uint32_t n_out) {
+    uint32_t blk = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (blk >= n_out) return;
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    uint32_t idx = blk * 64 + lane_id();
+    if (idx < n_in) box_absorb(in, idx, b);
+    wave_box_reduce(b);
+    if (lane_id() == 0) {
+        out[2 * (size_t)blk] = make_float4(b[0], b[1], b[2], 0.f);
+        out[2 * (size_t)blk + 1] = make_float4(b[3], b[4], b[5], 0.f);
+    }
+}
+
+__device__ __forceinline__ void union_range_box(uint32_t n, const uint2* __restrict__ range, float4* nodes,
+                                                const BlockLevels& lv, uint32_t node) {
+    uint2 r = range[node];
+    uint32_t L = r.x, R = r.y + 1;
+    uint32_t lane = lane_id();
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    const float4* leaves = nodes + 2 * (size_t)(n - 1);
+    for (int lvl = 0;; ++lvl) {
+        const float4* E = lvl == 0 ? leaves : lv.level[lvl];
+        if (R - L <= 64) {
+            if (lane < R - L) box_absorb(E, L + lane, b);
+            break;
+        }
+        uint32_t Lh = (L + 63u) & ~63u, Rt = R & ~63u;
+        if (lane < Lh - L) box_absorb(E, L + lane, b);
+        if (lane < R - Rt) box_absorb(E, Rt + lane, b);
+        L = Lh >> 6;
+        R = Rt >> 6;
+        if (L >= R) break;
+    }
+    wave_box_reduce(b);
+    if (lane == 0) {
+        float4 pl = nodes[2 * (size_t)node], ph = nodes[2 * (size_t)node + 1];
+        nodes[2 * (size_t)node] = make_float4(b[0], b[1], b[2], pl.w);
+        nodes[2 * (size_t)node + 1] = make_float4(b[3], b[4], b[5], ph.w);
+    }
+}
+
+// one wave per top node (grid-stride over the device-side count)
+__global__ __launch_bounds__(256) void k_internal_boxes(uint32_t n, const uint2* __restrict__ range, float4* nodes,
+                                                        BlockLevels lv, const uint32_t* __restrict__ top,
+                                                        const uint32_t* __restrict__ top_count) {
+    const uint32_t count = *top_count;
+    for (uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6); t < count; t += gridDim.x * 4)
+        union_range_box(n, range, nodes, lv, top[t]);
+}
+
+uint32_t block_level_count(uint32_t n, int level) {
+    uint64_t c = n;
+    for (int k = 0; k < level; ++k) c = (c + 63) / 64;
+    return (uint32_t)c;
+}
+
+size_t block_levels_elems(uint32_t n) {
+    size_t total = 0;
+    for (int k = 1; k < kMaxBlockLevels; ++k) total += 2 * (size_t)block_level_count(n, k);
+    return total;
+}
+
+void launch_refit(uint32_t n, const uint32_t* parent, const uint2* range, float4* nodes, float4* blocks,
+                  const uint32_t* top, const uint32_t* top_count, hipStream_t s) {
     if (n < 2) return;
-    k_refit<<<blocks_for(n, 256), 256, 0, s>>>(n, parent, children, nodes, flags);
+    k_refit_chunks<<<blocks_for(n, kRefitChunk), 256, 0, s>>>(n, parent, range, nodes);
+    BlockLevels lv;
+    lv.level[0] = nodes + 2 * (size_t)(n - 1);
+    size_t off = 0;
+    for (int k = 1; k < kMaxBlockLevels; ++k) {
+        lv.level[k] = blocks + off;
+        off += 2 * (size_t)block_level_count(n, k);
+    }
+    for (int k = 1; k < kMaxBlockLevels; ++k) {
+        uint32_t n_in = block_level_count(n, k - 1), n_out = block_level_count(n, k);
+        if (n_in <= 1) break;
+        k_block_boxes<<<blocks_for(n_out, 4), 256, 0, s>>>(lv.level[k - 1], n_in, (float4*)lv.level[k], n_out);
+    }
+    uint32_t max_top = n - 1;
+    k_internal_boxes<<<std::min<uint32_t>(blocks_for(max_top, 4), 2048), 256, 0, s>>>(n, range, nodes, lv, top,
+                                                                                       top_count);
     GI_LAUNCH_CHECK();
 }
 
 __global__ __launch_bounds__(256) void k_refresh_leaves(uint32_t n, const float4* __restrict__ lo,
-                                                        const float4* __restrict__ hi, float4* nodes,
-                                                        uint32_t* flags) {
+                                                        const float4* __restrict__ hi, float4* nodes) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     size_t leaf = (size_t)(n - 1 + i);
@@ -525,13 +671,11 @@ __global__ __launch_bounds__(256) void k_refresh_leaves(uint32_t n, const float4
     float4 l = lo[prim], h = hi[prim];
     nodes[2 * leaf] = make_float4(l.x, l.y, l.z, L.w);
     nodes[2 * leaf + 1] = make_float4(h.x, h.y, h.z, H.w);
-    if (i + 1 < n) flags[i] = 0;
 }
 
-void launch_refresh_leaves(uint32_t n, const float4* lo, const float4* hi, float4* nodes, uint32_t* flags,
-                           hipStream_t s) {
+void launch_refresh_leaves(uint32_t n, const float4* lo, const float4* hi, float4* nodes, hipStream_t s) {
     if (n == 0) return;
-    k_refresh_leaves<<<blocks_for(n, 256), 256, 0, s>>>(n, lo, hi, nodes, flags);
+    k_refresh_leaves<<<blocks_for(n, 256), 256, 0, s>>>(n, lo, hi, nodes);
     GI_LAUNCH_CHECK();
 }
 
