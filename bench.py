@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check", action="store_true", help="compare the pair set with the oracle (slow)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (3 times): the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
@@ -166,6 +169,14 @@ def main():
         ref = oracle_py.intersect(xa, ta, xb, tb, nthreads=os.cpu_count(), want_segments=False)
         line["check"] = bool(np.array_equal(r["pairs"], ref["pairs"]))
     eng.close()
+    if args.copy_calib:
+        src = torch.empty(args.copy_calib * (1 << 20) // 4, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copy calibration: {args.copy_calib} MiB read + {args.copy_calib} MiB written per dispatch",
+              file=sys.stderr)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(xa, ta, xb, tb, args.cpu_stride)
     print(json.dumps(line), flush=True)

@@ -67,10 +67,10 @@ struct MeshSlot {
     bool built = false;      // BVH topology is valid
     bool dirty = true;       // vertices/boxes/BVH boxes need an update
     Xform xf;
-    DevBuf src, tri, v64, vi, lo, hi, keys, vals, tkeys, tvals, nodes, range, blocks, parent, top, misc;
+    int sorted_in_tmp = 0;   // the sorted Morton keys live in (tkeys, tvals)
+    DevBuf src, tri, v64, vi, lo, hi, keys, vals, tkeys, tvals, nodes, range, blocks;
     void release() {
-        DevBuf* all[] = {&src, &tri, &v64, &vi, &lo, &hi, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks,
-                         &parent, &top, &misc};
+        DevBuf* all[] = {&src, &tri, &v64, &vi, &lo, &hi, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks};
         for (DevBuf* b : all) b->release();
         on_device = false;
         built = false;
@@ -153,30 +153,24 @@ void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
     float4* nodes = m.nodes.ensure<float4>(2 * nnodes);
     uint2* range = m.range.ensure<uint2>(nt);
     float4* blocks = m.blocks.ensure<float4>(block_levels_elems((uint32_t)nt));
-    uint32_t* parent = m.parent.ensure<uint32_t>(nnodes);
-    uint32_t* top = m.top.ensure<uint32_t>(nt);
-    uint32_t* top_count = m.misc.ensure<uint32_t>(4);
     launch_fill_u32(grid + slot, 1u, 1, s);
     launch_prepare_vertices(m.src.get<double>(), m.nv, m.xf, v64, vi, grid + slot, s);
     launch_fill_u32(bounds, 0xFFFFFFFFu, 3, s);
     launch_fill_u32(bounds + 3, 0u, 3, s);
     launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, vi, grid + slot, lo, hi, bounds, s);
+    uint32_t* keys = m.keys.ensure<uint32_t>(nt);
+    uint32_t* vals = m.vals.ensure<uint32_t>(nt);
+    uint32_t* tk = m.tkeys.ensure<uint32_t>(nt);
+    uint32_t* tv = m.tvals.ensure<uint32_t>(nt);
     if (!m.built || rebuild) {
-        uint32_t* keys = m.keys.ensure<uint32_t>(nt);
-        uint32_t* vals = m.vals.ensure<uint32_t>(nt);
-        uint32_t* tk = m.tkeys.ensure<uint32_t>(nt);
-        uint32_t* tv = m.tvals.ensure<uint32_t>(nt);
         uint32_t* hist = ctx->hist.ensure<uint32_t>(sort_hist_elems(nt));
         uint32_t* stmp = ctx->scan_tmp.ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
         launch_morton(lo, hi, m.nt, bounds, keys, vals, s);
-        int in_tmp = radix_sort_u32(keys, vals, tk, tv, m.nt, 30, hist, stmp, s);
-        launch_build_bvh(in_tmp ? tk : keys, in_tmp ? tv : vals, m.nt, lo, hi, nodes, range, parent, top, top_count, s);
-        launch_refit(m.nt, parent, range, nodes, blocks, top, top_count, s);
+        m.sorted_in_tmp = radix_sort_u32(keys, vals, tk, tv, m.nt, 30, hist, stmp, s);
         m.built = true;
-    } else {
-        launch_refresh_leaves(m.nt, lo, hi, nodes, s);
-        launch_refit(m.nt, parent, range, nodes, blocks, top, top_count, s);
     }
+    // build, or refit on the kept sorted keys (same topology, new boxes)
+    launch_build_bvh(m.sorted_in_tmp ? tk : keys, m.sorted_in_tmp ? tv : vals, m.nt, lo, hi, nodes, range, blocks, s);
     HIP_TRY(hipGetLastError());
     m.dirty = false;
 }

@@ -66,28 +66,20 @@ int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t*
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scan_tmp, uint32_t* total,
                         hipStream_t s);
 
-// Karras LBVH over the sorted Morton keys: leaf k (node n-1+k) holds prim
-// sorted_prims[k] and its box; internal node i holds its left child and its
-// rope (next node in pre-order after its subtree); range[i] = its leaf range,
-// parent[] the parent of every node (2n-1 entries). Internal nodes whose leaf
-// range crosses a kRefitChunk boundary are listed in top[0..*top_count).
-constexpr uint32_t kRefitChunk = 1024;
-void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
-                      const float4* hi, float4* nodes, uint2* range, uint32_t* parent, uint32_t* top,
-                      uint32_t* top_count, hipStream_t s);
-
-// Internal-node boxes from the leaf boxes (see k_refit_chunks and
-// k_internal_boxes); `blocks` needs block_levels_elems(n) float4.
+// Karras LBVH over the sorted Morton keys, boxes included: leaf k (node
+// n-1+k) holds prim sorted_prims[k] and its box; internal node i holds its
+// box, its left child and its rope (next node in pre-order after its
+// subtree); range[i] = its leaf range. `blocks` needs block_levels_elems(n)
+// float4. Running it again on the same sorted keys with new prim boxes is the
+// refit after a rigid motion (same topology, new boxes).
+constexpr uint32_t kBuildChunk = 512;
 constexpr int kMaxBlockLevels = 6;
 struct BlockLevels {
     const float4* level[kMaxBlockLevels];
 };
 size_t block_levels_elems(uint32_t n);
-void launch_refit(uint32_t n, const uint32_t* parent, const uint2* range, float4* nodes, float4* blocks,
-                  const uint32_t* top, const uint32_t* top_count, hipStream_t s);
-// After a rigid motion: the new prim boxes go into the leaves (leaf k holds
-// prim nodes[2*(n-1+k)].w); then launch_refit.
-void launch_refresh_leaves(uint32_t n, const float4* lo, const float4* hi, float4* nodes, hipStream_t s);
+void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
+                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s);
 
 void launch_traverse(MeshDev A, MeshDev B, uint2* cand, uint32_t cap, uint32_t* counters, hipStream_t s);
 void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const uint32_t* grid_flags,

@@ -220,7 +220,11 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr uint32_t SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
-size_t scan_tmp_elems(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+// block sums of a scan, and at least the 256 digit totals of a sort pass
+size_t scan_tmp_elems(size_t n) {
+    size_t b = (n + SCAN_TILE - 1) / SCAN_TILE + 1;
+    return b < 257 ? 257 : b;
+}
 
 __device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* total) {
     __shared__ uint32_t ws[SCAN_THREADS / 64];
@@ -339,16 +343,49 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_upsweep(const K* __restri
 
 // Stable scatter: keys of a tile are ranked in (wave, item, lane) order,
 // which is their original order.
+// One block per digit: exclusive scan of that digit's per-tile counts (in
+// place) and the digit total. The digit totals are scanned by each
+// downsweep block, so a pass is three launches.
+__global__ __launch_bounds__(SCAN_THREADS) void k_sort_digit_scan(uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                                  uint32_t* __restrict__ totals) {
+    uint32_t* h = hist + (size_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += SCAN_TILE) {
+        uint32_t v[SCAN_ITEMS];
+        uint32_t s = 0;
+        for (int it = 0; it < SCAN_ITEMS; ++it) {
+            uint32_t i = base + threadIdx.x * SCAN_ITEMS + it;
+            v[it] = i < ntiles ? h[i] : 0;
+            s += v[it];
+        }
+        uint32_t tot;
+        uint32_t off = block_exclusive_sum(s, &tot) + carry;
+        for (int it = 0; it < SCAN_ITEMS; ++it) {
+            uint32_t i = base + threadIdx.x * SCAN_ITEMS + it;
+            if (i < ntiles) h[i] = off;
+            off += v[it];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
 template <class K>
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                                  K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                                  uint32_t n, int shift,
-                                                                 const uint32_t* __restrict__ hist, uint32_t ntiles) {
+                                                                 const uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                                 const uint32_t* __restrict__ totals) {
     __shared__ uint32_t wc[SORT_THREADS / 64][RADIX];
     __shared__ uint32_t gbase[RADIX];
     uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     for (int d = threadIdx.x; d < (SORT_THREADS / 64) * RADIX; d += SORT_THREADS) (&wc[0][0])[d] = 0;
-    for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) gbase[d] = hist[(size_t)d * ntiles + blockIdx.x];
+    {
+        // digit base = exclusive prefix of the digit totals (RADIX == SORT_THREADS)
+        uint32_t t = totals[threadIdx.x];
+        uint32_t ex = block_exclusive_sum(t, nullptr);
+        gbase[threadIdx.x] = ex + hist[(size_t)threadIdx.x * ntiles + blockIdx.x];
+    }
     __syncthreads();
     uint32_t base = blockIdx.x * SORT_TILE + wave * (64 * SORT_ITEMS);
     uint64_t lt = lanemask_lt();
@@ -410,11 +447,13 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32_
     uint32_t* src_v = vals;
     K* dst_k = tk;
     uint32_t* dst_v = tv;
+    uint32_t* totals = scan_tmp;  // RADIX entries
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
         k_sort_upsweep<K><<<ntiles, SORT_THREADS, 0, s>>>(src_k, n, shift, hist, ntiles);
-        exclusive_scan_u32(hist, hist, RADIX * ntiles, scan_tmp, nullptr, s);
-        k_sort_downsweep<K><<<ntiles, SORT_THREADS, 0, s>>>(src_k, src_v, dst_k, dst_v, n, shift, hist, ntiles);
+        k_sort_digit_scan<<<RADIX, SCAN_THREADS, 0, s>>>(hist, ntiles, totals);
+        k_sort_downsweep<K><<<ntiles, SORT_THREADS, 0, s>>>(src_k, src_v, dst_k, dst_v, n, shift, hist, ntiles,
+                                                            totals);
         K* t = src_k; src_k = dst_k; dst_k = t;
         uint32_t* tvv = src_v; src_v = dst_v; dst_v = tvv;
     }
@@ -432,11 +471,30 @@ int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t*
 }
 
 // --------------------------------------------------------- BVH (Karras)
+//
+// Build and refit are one kernel per chunk of kBuildChunk sorted leaves:
+// the chunk's Morton keys sit in LDS for Karras' prefix searches, every
+// internal node whose leaf range lies inside the chunk is refit bottom-up
+// through LDS only (parents, merge flags and merged boxes never leave LDS,
+// so no device-scope fence is needed; on gfx950 those write back and
+// invalidate the per-XCD L2, which made a global-atomic refit cost 6 ms for
+// 2^20 triangles). The few "top" nodes whose range crosses a chunk boundary
+// take the union of their leaf range from a 64-ary hierarchy of block boxes.
+// A refit after a rigid motion reruns the same kernels on the same sorted
+// keys: the topology comes out identical, only the boxes change.
+
+// Sorted Morton keys, the chunk's slice read from LDS.
+struct KeySrc {
+    const uint32_t* __restrict__ g;
+    const uint32_t* s;
+    int cs, ce, n;
+    __device__ __forceinline__ uint32_t at(int j) const { return (j >= cs && j < ce) ? s[j - cs] : g[j]; }
+};
 
 // common-prefix length of the composite keys (morton, index)
-__device__ __forceinline__ int delta(const uint32_t* __restrict__ k, int n, int i, int j) {
-    if (j < 0 || j >= n) return -1;
-    uint32_t a = k[i], b = k[j];
+__device__ __forceinline__ int delta(const KeySrc& k, int i, int j) {
+    if (j < 0 || j >= k.n) return -1;
+    uint32_t a = k.at(i), b = k.at(j);
     if (a != b) return __clz(a ^ b);
     return 32 + __clz((uint32_t)i ^ (uint32_t)j);
 }
@@ -445,119 +503,98 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ k, int n, int 
 // right child that starts at leaf m. In Karras' numbering a right child
 // covering [m, x] is internal node m, which is the case iff internal node m
 // extends to the right; otherwise it is leaf m.
-__device__ __forceinline__ uint32_t chain_top(const uint32_t* __restrict__ k, int n, int m) {
-    if (m < n - 1 && delta(k, n, m, m + 1) > delta(k, n, m, m - 1)) return (uint32_t)m;
-    return (uint32_t)(n - 1 + m);
+__device__ __forceinline__ uint32_t chain_top(const KeySrc& k, int m) {
+    if (m < k.n - 1 && delta(k, m, m + 1) > delta(k, m, m - 1)) return (uint32_t)m;
+    return (uint32_t)(k.n - 1 + m);
 }
 
-__global__ __launch_bounds__(256) void k_build_bvh(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ prims,
-                                                   int n, const float4* __restrict__ lo, const float4* __restrict__ hi,
-                                                   float4* __restrict__ nodes, uint2* __restrict__ range,
-                                                   uint32_t* __restrict__ parent, uint32_t* __restrict__ top,
-                                                   uint32_t* __restrict__ top_count) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (i == 0) parent[0] = kSentinel;
-    {
-        uint32_t leaf = (uint32_t)(n - 1 + i);
-        uint32_t prim = prims[i];
-        float4 l = lo[prim], h = hi[prim];
-        uint32_t rope = (i == n - 1) ? kSentinel : chain_top(keys, n, i + 1);
-        nodes[2 * (size_t)leaf] = make_float4(l.x, l.y, l.z, __uint_as_float(prim));
-        nodes[2 * (size_t)leaf + 1] = make_float4(h.x, h.y, h.z, __uint_as_float(rope));
+__global__ __launch_bounds__(256) void k_build_chunks(const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ prims, int n,
+                                                      const float4* __restrict__ lo, const float4* __restrict__ hi,
+                                                      float4* __restrict__ nodes, uint2* __restrict__ range) {
+    __shared__ uint32_t skey[kBuildChunk];
+    __shared__ uint32_t par_leaf[kBuildChunk];
+    __shared__ uint32_t par_int[kBuildChunk];
+    __shared__ uint32_t left_int[kBuildChunk];
+    __shared__ uint32_t rope_int[kBuildChunk];
+    __shared__ uint32_t flag[kBuildChunk];
+    __shared__ uint32_t ubox[kBuildChunk][6];  // merged boxes as order-preserving uint keys
+    const int cs = blockIdx.x * (int)kBuildChunk;
+    const int ce = min(n, cs + (int)kBuildChunk);
+    for (int k = threadIdx.x; k < (int)kBuildChunk; k += blockDim.x) {
+        if (cs + k < ce) skey[k] = keys[cs + k];
+        par_leaf[k] = kSentinel;
+        par_int[k] = kSentinel;
+        flag[k] = 0;
+        ubox[k][0] = ubox[k][1] = ubox[k][2] = 0xFFFFFFFFu;
+        ubox[k][3] = ubox[k][4] = ubox[k][5] = 0u;
     }
-    if (i >= n - 1) return;
-    int d = delta(keys, n, i, i + 1) > delta(keys, n, i, i - 1) ? 1 : -1;
-    int dmin = delta(keys, n, i, i - d);
-    int lmax = 2;
-    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
-    int l = 0;
-    for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-    int j = i + l * d;
-    int dnode = delta(keys, n, i, j);
-    int s = 0, t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-    } while (t > 1);
-    int gamma = i + s * d + (d < 0 ? -1 : 0);
-    int lo_i = i < j ? i : j, hi_i = i < j ? j : i;
-    uint32_t left = lo_i == gamma ? (uint32_t)(n - 1 + gamma) : (uint32_t)gamma;
-    uint32_t right = hi_i == gamma + 1 ? (uint32_t)(n - 1 + gamma + 1) : (uint32_t)(gamma + 1);
-    parent[left] = (uint32_t)i;
-    parent[right] = (uint32_t)i;
-    range[i] = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
-    uint32_t rope = hi_i == n - 1 ? kSentinel : chain_top(keys, n, hi_i + 1);
-    nodes[2 * (size_t)i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(left));
-    nodes[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rope));
-    // wave-aggregated append to the top-node list
-    bool is_top = (uint32_t)lo_i / kRefitChunk != (uint32_t)hi_i / kRefitChunk;
-    uint64_t m = __ballot(is_top);
-    if (m) {
-        uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
-        uint32_t base = 0;
-        if (lane_id() == leader) base = atomicAdd(top_count, (uint32_t)__popcll(m));
-        base = __shfl(base, (int)leader);
-        if (is_top) top[base + (uint32_t)__popcll(m & lanemask_lt())] = (uint32_t)i;
-    }
-}
-
-void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
-                      const float4* hi, float4* nodes, uint2* range, uint32_t* parent, uint32_t* top,
-                      uint32_t* top_count, hipStream_t s) {
-    if (n == 0) return;
-    launch_fill_u32(top_count, 0u, 1, s);
-    k_build_bvh<<<blocks_for(n, 256), 256, 0, s>>>(sorted_keys, sorted_prims, (int)n, lo, hi, nodes, range, parent,
-                                                   top, top_count);
-    GI_LAUNCH_CHECK();
-}
-
-// Fence-free refit, in two phases. Device-scope fences (the usual bottom-up
-// refit with a global atomic per node) write back and invalidate the per-XCD
-// L2 on gfx950, which made that refit 6 ms for 2^20 triangles.
-//
-// Phase 1: internal node i of a Karras tree covers the contiguous sorted-leaf
-// range range[i] = [lo, hi], and i is lo or hi. Every node whose range lies in
-// one chunk of kRefitChunk leaves is refit bottom-up by the workgroup that owns
-// the chunk, synchronising through LDS only (workgroup-scope fences).
-//
-// Phase 2: the few "top" nodes whose range crosses a chunk boundary (listed by
-// the build kernel) take the union of their leaf range directly: a 64-ary
-// hierarchy of block boxes (level k element j covers leaves
-// [j*64^k, (j+1)*64^k)) lets one wave union any range with at most two partial
-// 64-element reads per level.
-
-__global__ __launch_bounds__(256) void k_refit_chunks(uint32_t n, const uint32_t* __restrict__ parent,
-                                                      const uint2* __restrict__ range, float4* nodes) {
-    __shared__ uint32_t flag[kRefitChunk];
-    __shared__ float side_box[kRefitChunk][2][6];
-    const uint32_t cs = blockIdx.x * kRefitChunk;
-    const uint32_t ce = min(n, cs + kRefitChunk);
-    for (uint32_t i = threadIdx.x; i < kRefitChunk; i += blockDim.x) flag[i] = 0;
     __syncthreads();
-    for (uint32_t j = cs + threadIdx.x; j < ce; j += blockDim.x) {
-        uint32_t node = n - 1 + j;
-        float4 L = nodes[2 * (size_t)node], H = nodes[2 * (size_t)node + 1];
-        float b[6] = {L.x, L.y, L.z, H.x, H.y, H.z};
-        for (;;) {
-            uint32_t p = parent[node];
-            if (p == kSentinel) break;
-            uint2 r = range[p];
-            if (r.x < cs || r.y >= ce) break;  // top node: phase 2
-            float4 PL = nodes[2 * (size_t)p];
-            int side = __float_as_uint(PL.w) == node ? 0 : 1;
-            uint32_t slot = p - cs;
-            for (int k = 0; k < 6; ++k) side_box[slot][side][k] = b[k];
+    const KeySrc K{keys, skey, cs, ce, n};
+    const int iend = min(ce, n - 1);
+    for (int i = cs + threadIdx.x; i < iend; i += blockDim.x) {
+        int d = delta(K, i, i + 1) > delta(K, i, i - 1) ? 1 : -1;
+        int dmin = delta(K, i, i - d);
+        int lmax = 2;
+        while (delta(K, i, i + lmax * d) > dmin) lmax <<= 1;
+        int l = 0;
+        for (int t = lmax >> 1; t >= 1; t >>= 1)
+            if (delta(K, i, i + (l + t) * d) > dmin) l += t;
+        int j = i + l * d;
+        int dnode = delta(K, i, j);
+        int s = 0, t = l;
+        do {
+            t = (t + 1) >> 1;
+            if (delta(K, i, i + (s + t) * d) > dnode) s += t;
+        } while (t > 1);
+        int gamma = i + s * d + (d < 0 ? -1 : 0);
+        int lo_i = i < j ? i : j, hi_i = i < j ? j : i;
+        uint32_t left = lo_i == gamma ? (uint32_t)(n - 1 + gamma) : (uint32_t)gamma;
+        uint32_t right = hi_i == gamma + 1 ? (uint32_t)(n - 1 + gamma + 1) : (uint32_t)(gamma + 1);
+        uint32_t rope = hi_i == n - 1 ? kSentinel : chain_top(K, hi_i + 1);
+        range[i] = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
+        nodes[2 * (size_t)i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(left));
+        nodes[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rope));
+        left_int[i - cs] = left;
+        rope_int[i - cs] = rope;
+        if (lo_i >= cs && hi_i < ce) {  // in-chunk node: both children are in the chunk too
+            uint32_t c[2] = {left, right};
+            for (int e = 0; e < 2; ++e) {
+                if (c[e] >= (uint32_t)(n - 1)) par_leaf[c[e] - (uint32_t)(n - 1) - (uint32_t)cs] = (uint32_t)i;
+                else par_int[c[e] - (uint32_t)cs] = (uint32_t)i;
+            }
+        }
+    }
+    float lb[kBuildChunk / 256][6];
+    for (int u = 0; u < (int)(kBuildChunk / 256); ++u) {
+        int jl = cs + threadIdx.x + 256 * u;
+        if (jl >= ce) continue;
+        uint32_t prim = prims[jl];
+        float4 l = lo[prim], h = hi[prim];
+        uint32_t rope = jl == n - 1 ? kSentinel : chain_top(K, jl + 1);
+        nodes[2 * (size_t)(n - 1 + jl)] = make_float4(l.x, l.y, l.z, __uint_as_float(prim));
+        nodes[2 * (size_t)(n - 1 + jl) + 1] = make_float4(h.x, h.y, h.z, __uint_as_float(rope));
+        lb[u][0] = l.x; lb[u][1] = l.y; lb[u][2] = l.z;
+        lb[u][3] = h.x; lb[u][4] = h.y; lb[u][5] = h.z;
+    }
+    __syncthreads();
+    for (int u = 0; u < (int)(kBuildChunk / 256); ++u) {
+        int jl = cs + threadIdx.x + 256 * u;
+        if (jl >= ce) continue;
+        float b[6];
+        for (int k = 0; k < 6; ++k) b[k] = lb[u][k];
+        uint32_t p = par_leaf[jl - cs];
+        while (p != kSentinel) {
+            uint32_t slot = p - (uint32_t)cs;
+            for (int k = 0; k < 3; ++k) atomicMin(&ubox[slot][k], f2key(b[k]));
+            for (int k = 3; k < 6; ++k) atomicMax(&ubox[slot][k], f2key(b[k]));
             __threadfence_block();
-            if (atomicAdd(&flag[slot], 1u) == 0) break;  // sibling not done yet: it continues
+            if (atomicAdd(&flag[slot], 1u) == 0) break;  // the sibling continues upward
             __threadfence_block();
-            for (int k = 0; k < 3; ++k) b[k] = fminf(b[k], side_box[slot][1 - side][k]);
-            for (int k = 3; k < 6; ++k) b[k] = fmaxf(b[k], side_box[slot][1 - side][k]);
-            float4 PH = nodes[2 * (size_t)p + 1];
-            nodes[2 * (size_t)p] = make_float4(b[0], b[1], b[2], PL.w);
-            nodes[2 * (size_t)p + 1] = make_float4(b[3], b[4], b[5], PH.w);
-            node = p;
+            for (int k = 0; k < 6; ++k) b[k] = key2f(atomicAdd(&ubox[slot][k], 0u));
+            nodes[2 * (size_t)p] = make_float4(b[0], b[1], b[2], __uint_as_float(left_int[slot]));
+            nodes[2 * (size_t)p + 1] = make_float4(b[3], b[4], b[5], __uint_as_float(rope_int[slot]));
+            p = par_int[slot];
         }
     }
 }
@@ -590,15 +627,16 @@ __global__ __launch_bounds__(256) void k_block_boxes(const float4* __restrict__ 
     }
 }
 
+// wave-cooperative union of the leaf boxes of node's range (at most two
+// partial 64-element reads per level of the block hierarchy)
 __device__ __forceinline__ void union_range_box(uint32_t n, const uint2* __restrict__ range, float4* nodes,
                                                 const BlockLevels& lv, uint32_t node) {
     uint2 r = range[node];
     uint32_t L = r.x, R = r.y + 1;
     uint32_t lane = lane_id();
     float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    const float4* leaves = nodes + 2 * (size_t)(n - 1);
     for (int lvl = 0;; ++lvl) {
-        const float4* E = lvl == 0 ? leaves : lv.level[lvl];
+        const float4* E = lv.level[lvl];
         if (R - L <= 64) {
             if (lane < R - L) box_absorb(E, L + lane, b);
             break;
@@ -618,13 +656,23 @@ __device__ __forceinline__ void union_range_box(uint32_t n, const uint2* __restr
     }
 }
 
-// one wave per top node (grid-stride over the device-side count)
-__global__ __launch_bounds__(256) void k_internal_boxes(uint32_t n, const uint2* __restrict__ range, float4* nodes,
-                                                        BlockLevels lv, const uint32_t* __restrict__ top,
-                                                        const uint32_t* __restrict__ top_count) {
-    const uint32_t count = *top_count;
-    for (uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6); t < count; t += gridDim.x * 4)
-        union_range_box(n, range, nodes, lv, top[t]);
+// one wave per 64 consecutive internal nodes; the top ones among them are
+// done one after the other by the whole wave
+__global__ __launch_bounds__(256) void k_top_boxes(uint32_t n, const uint2* __restrict__ range, float4* nodes,
+                                                   BlockLevels lv) {
+    uint32_t base = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    uint32_t i = base + lane_id();
+    bool top = false;
+    if (i + 1 < n) {
+        uint2 r = range[i];
+        top = r.x / kBuildChunk != r.y / kBuildChunk;
+    }
+    uint64_t m = __ballot(top);
+    while (m) {
+        uint32_t b = (uint32_t)__ffsll((long long)m) - 1;
+        m &= m - 1;
+        union_range_box(n, range, nodes, lv, base + b);
+    }
 }
 
 uint32_t block_level_count(uint32_t n, int level) {
@@ -639,10 +687,11 @@ size_t block_levels_elems(uint32_t n) {
     return total;
 }
 
-void launch_refit(uint32_t n, const uint32_t* parent, const uint2* range, float4* nodes, float4* blocks,
-                  const uint32_t* top, const uint32_t* top_count, hipStream_t s) {
+void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
+                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s) {
+    if (n == 0) return;
+    k_build_chunks<<<blocks_for(n, kBuildChunk), 256, 0, s>>>(sorted_keys, sorted_prims, (int)n, lo, hi, nodes, range);
     if (n < 2) return;
-    k_refit_chunks<<<blocks_for(n, kRefitChunk), 256, 0, s>>>(n, parent, range, nodes);
     BlockLevels lv;
     lv.level[0] = nodes + 2 * (size_t)(n - 1);
     size_t off = 0;
@@ -655,27 +704,7 @@ void launch_refit(uint32_t n, const uint32_t* parent, const uint2* range, float4
         if (n_in <= 1) break;
         k_block_boxes<<<blocks_for(n_out, 4), 256, 0, s>>>(lv.level[k - 1], n_in, (float4*)lv.level[k], n_out);
     }
-    uint32_t max_top = n - 1;
-    k_internal_boxes<<<std::min<uint32_t>(blocks_for(max_top, 4), 2048), 256, 0, s>>>(n, range, nodes, lv, top,
-                                                                                       top_count);
-    GI_LAUNCH_CHECK();
-}
-
-__global__ __launch_bounds__(256) void k_refresh_leaves(uint32_t n, const float4* __restrict__ lo,
-                                                        const float4* __restrict__ hi, float4* nodes) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    size_t leaf = (size_t)(n - 1 + i);
-    float4 L = nodes[2 * leaf], H = nodes[2 * leaf + 1];
-    uint32_t prim = __float_as_uint(L.w);
-    float4 l = lo[prim], h = hi[prim];
-    nodes[2 * leaf] = make_float4(l.x, l.y, l.z, L.w);
-    nodes[2 * leaf + 1] = make_float4(h.x, h.y, h.z, H.w);
-}
-
-void launch_refresh_leaves(uint32_t n, const float4* lo, const float4* hi, float4* nodes, hipStream_t s) {
-    if (n == 0) return;
-    k_refresh_leaves<<<blocks_for(n, 256), 256, 0, s>>>(n, lo, hi, nodes);
+    k_top_boxes<<<blocks_for(blocks_for(n - 1, 64), 4), 256, 0, s>>>(n, range, nodes, lv);
     GI_LAUNCH_CHECK();
 }
 
