@@ -21,7 +21,9 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgi.so")
 ROOT = os.path.dirname(HERE)
 
-SOURCES = ["gi_device.hip", "gi_api.hip", "host_exact.cpp"]
+SOURCES = ["gi_device.hip", "gi_api.hip", "host_exact.cpp", "../host/KernelBVHController.cpp"]
+FACADE_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "facade_test.cpp")
+FACADE_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "facade_test")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
@@ -31,7 +33,7 @@ FLAGS = [
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    obj = os.path.join(OBJ, os.path.splitext(os.path.basename(src))[0] + ".o")
     cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -52,6 +54,18 @@ def build() -> str:
         raise RuntimeError("link of libgi.so failed")
     os.replace(tmp, LIB)
     return LIB
+
+
+def build_facade_test() -> str:
+    """The C++ driver of the KernelBVHController facade used by the GPU tests
+    (links libgi.so from this directory)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(HERE, "host"), FACADE_TEST_SRC, "-o", FACADE_TEST_BIN,
+           "-L" + HERE, "-lgi", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../gpu-cubql-intersector-standalone_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("facade test driver failed to build")
+    return FACADE_TEST_BIN
 
 
 if __name__ == "__main__":

@@ -153,19 +153,18 @@ void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
     float4* nodes = m.nodes.ensure<float4>(2 * nnodes);
     uint2* range = m.range.ensure<uint2>(nt);
     float4* blocks = m.blocks.ensure<float4>(block_levels_elems((uint32_t)nt));
-    launch_fill_u32(grid + slot, 1u, 1, s);
-    launch_prepare_vertices(m.src.get<double>(), m.nv, m.xf, v64, vi, grid + slot, s);
-    launch_fill_u32(bounds, 0xFFFFFFFFu, 3, s);
-    launch_fill_u32(bounds + 3, 0u, 3, s);
-    launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, vi, grid + slot, lo, hi, bounds, s);
     uint32_t* keys = m.keys.ensure<uint32_t>(nt);
     uint32_t* vals = m.vals.ensure<uint32_t>(nt);
     uint32_t* tk = m.tkeys.ensure<uint32_t>(nt);
     uint32_t* tv = m.tvals.ensure<uint32_t>(nt);
-    if (!m.built || rebuild) {
+    const bool topology = !m.built || rebuild;
+    launch_init_mesh_scalars(grid + slot, bounds, s);
+    launch_prepare_vertices(m.src.get<double>(), m.nv, m.xf, v64, vi, grid + slot, bounds, s);
+    launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, vi, grid + slot, bounds, lo, hi, topology ? keys : nullptr,
+                     topology ? vals : nullptr, s);
+    if (topology) {
         uint32_t* hist = ctx->hist.ensure<uint32_t>(sort_hist_elems(nt));
         uint32_t* stmp = ctx->scan_tmp.ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
-        launch_morton(lo, hi, m.nt, bounds, keys, vals, s);
         m.sorted_in_tmp = radix_sort_u32(keys, vals, tk, tv, m.nt, 30, hist, stmp, s);
         m.built = true;
     }

@@ -45,12 +45,17 @@ struct MeshDev {
 };
 
 void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
+// grid flag = 1, bounds6 = empty box (order-preserving keys)
+void launch_init_mesh_scalars(uint32_t* grid_flag, uint32_t* bounds6, hipStream_t s);
+// bounds6 (3 min keys, 3 max keys of order-preserving float keys) must be
+// cleared to (0xFFFFFFFF x3, 0 x3) before; it receives the mesh bounds.
 void launch_prepare_vertices(const double* src, uint32_t nv, Xform xf, double4* v64, int4* vi, uint32_t* grid_flag,
-                             hipStream_t s);
+                             uint32_t* bounds6, hipStream_t s);
+// Triangle boxes; with keys/vals non-null also Morton keys (frame bounds6)
+// and the identity permutation.
 void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,
-                      float4* lo, float4* hi, uint32_t* bounds6, hipStream_t s);
-void launch_morton(const float4* lo, const float4* hi, uint32_t nt, const uint32_t* bounds6, uint32_t* keys,
-                   uint32_t* vals, hipStream_t s);
+                      const uint32_t* bounds6, float4* lo, float4* hi, uint32_t* keys, uint32_t* vals,
+                      hipStream_t s);
 
 // LSD radix sort of (key, value) pairs, 8 bits per pass. tmp_* are ping-pong
 // buffers of n elements; hist needs sort_hist_elems(n) entries, scan_tmp

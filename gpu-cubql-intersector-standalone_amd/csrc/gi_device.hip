@@ -52,6 +52,16 @@ void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s) {
     GI_LAUNCH_CHECK();
 }
 
+__global__ void k_init_mesh_scalars(uint32_t* grid_flag, uint32_t* bounds6) {
+    if (threadIdx.x == 0) *grid_flag = 1u;
+    if (threadIdx.x < 6) bounds6[threadIdx.x] = threadIdx.x < 3 ? 0xFFFFFFFFu : 0u;
+}
+
+void launch_init_mesh_scalars(uint32_t* grid_flag, uint32_t* bounds6, hipStream_t s) {
+    k_init_mesh_scalars<<<1, 64, 0, s>>>(grid_flag, bounds6);
+    GI_LAUNCH_CHECK();
+}
+
 // ------------------------------------------------------- vertex preparation
 
 __device__ __forceinline__ double xform_coord(const Xform& xf, int r, double x, double y, double z) {
@@ -59,38 +69,6 @@ __device__ __forceinline__ double xform_coord(const Xform& xf, int r, double x, 
     return __dadd_rn(__dadd_rn(__dadd_rn(__dmul_rn(xf.m[r][0], x), __dmul_rn(xf.m[r][1], y)), __dmul_rn(xf.m[r][2], z)),
                      xf.m[r][3]);
 }
-
-__global__ __launch_bounds__(256) void k_prepare_vertices(const double* __restrict__ src, uint32_t nv, Xform xf,
-                                                          double4* __restrict__ v64, int4* __restrict__ vi,
-                                                          uint32_t* grid_flag) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool grid = true;
-    if (i < nv) {
-        double x = src[3 * (size_t)i], y = src[3 * (size_t)i + 1], z = src[3 * (size_t)i + 2];
-        if (!xf.identity) {
-            double X = xform_coord(xf, 0, x, y, z);
-            double Y = xform_coord(xf, 1, x, y, z);
-            double Z = xform_coord(xf, 2, x, y, z);
-            x = X; y = Y; z = Z;
-        }
-        if (xf.snap) { x = rint(x); y = rint(y); z = rint(z); }
-        v64[i] = make_double4(x, y, z, 0.0);
-        const double L = 67108864.0;  // 2^26
-        grid = x == rint(x) && y == rint(y) && z == rint(z) && fabs(x) <= L && fabs(y) <= L && fabs(z) <= L;
-        vi[i] = grid ? make_int4((int)x, (int)y, (int)z, 0) : make_int4(0, 0, 0, 0);
-    }
-    uint64_t bad = __ballot(!grid);
-    if (bad && lane_id() == 0) atomicAnd(grid_flag, 0u);
-}
-
-void launch_prepare_vertices(const double* src, uint32_t nv, Xform xf, double4* v64, int4* vi, uint32_t* grid_flag,
-                             hipStream_t s) {
-    if (nv == 0) return;
-    k_prepare_vertices<<<blocks_for(nv, 256), 256, 0, s>>>(src, nv, xf, v64, vi, grid_flag);
-    GI_LAUNCH_CHECK();
-}
-
-// ---------------------------------------------------------- triangle boxes
 
 __device__ __forceinline__ float f32_down(double x) {
     float f = (float)x;
@@ -120,40 +98,9 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// Grid-stride over triangles with a capped grid, so the mesh bounds take one
-// atomic per block and component instead of one per 256 triangles. Grid
-// meshes gather 16-byte integer vertices instead of 32-byte doubles.
-__global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ tri, uint32_t nt,
-                                                   const double4* __restrict__ v, const int4* __restrict__ vi,
-                                                   const uint32_t* __restrict__ grid_flag, float4* __restrict__ lo,
-                                                   float4* __restrict__ hi, uint32_t* bounds6) {
+// block-reduce a box and merge it into bounds6 (3 min keys, 3 max keys)
+__device__ __forceinline__ void block_bounds_merge(const float* l, const float* h, uint32_t* bounds6) {
     __shared__ float red[6][4];
-    const bool grid = *grid_flag != 0;
-    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x) {
-        uint32_t i0 = tri[3 * (size_t)i], i1 = tri[3 * (size_t)i + 1], i2 = tri[3 * (size_t)i + 2];
-        float bl[3], bh[3];
-        if (grid) {
-            int4 a = vi[i0], b = vi[i1], c = vi[i2];
-            bl[0] = f32_down((double)min(min(a.x, b.x), c.x));
-            bl[1] = f32_down((double)min(min(a.y, b.y), c.y));
-            bl[2] = f32_down((double)min(min(a.z, b.z), c.z));
-            bh[0] = f32_up((double)max(max(a.x, b.x), c.x));
-            bh[1] = f32_up((double)max(max(a.y, b.y), c.y));
-            bh[2] = f32_up((double)max(max(a.z, b.z), c.z));
-        } else {
-            double4 a = v[i0], b = v[i1], c = v[i2];
-            bl[0] = f32_down(fmin(fmin(a.x, b.x), c.x));
-            bl[1] = f32_down(fmin(fmin(a.y, b.y), c.y));
-            bl[2] = f32_down(fmin(fmin(a.z, b.z), c.z));
-            bh[0] = f32_up(fmax(fmax(a.x, b.x), c.x));
-            bh[1] = f32_up(fmax(fmax(a.y, b.y), c.y));
-            bh[2] = f32_up(fmax(fmax(a.z, b.z), c.z));
-        }
-        lo[i] = make_float4(bl[0], bl[1], bl[2], 0.f);
-        hi[i] = make_float4(bh[0], bh[1], bh[2], 0.f);
-        for (int k = 0; k < 3; ++k) { l[k] = fminf(l[k], bl[k]); h[k] = fmaxf(h[k], bh[k]); }
-    }
     uint32_t wave = threadIdx.x >> 6;
     for (int k = 0; k < 3; ++k) {
         float mn = wave_min(l[k]), mx = wave_max(h[k]);
@@ -169,15 +116,46 @@ __global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ 
     }
 }
 
-void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,
-                      float4* lo, float4* hi, uint32_t* bounds6, hipStream_t s) {
-    if (nt == 0) return;
-    uint32_t blocks = std::min<uint32_t>(blocks_for(nt, 256), 1024);
-    k_tri_boxes<<<blocks, 256, 0, s>>>(tri, nt, v64, vi, grid_flag, lo, hi, bounds6);
+// Moves the vertices (when a transform is set), classifies the mesh as grid
+// or not, writes the integer copy and the mesh bounds (Morton frame).
+__global__ __launch_bounds__(256) void k_prepare_vertices(const double* __restrict__ src, uint32_t nv, Xform xf,
+                                                          double4* __restrict__ v64, int4* __restrict__ vi,
+                                                          uint32_t* grid_flag, uint32_t* bounds6) {
+    bool grid = true;
+    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
+        double x = src[3 * (size_t)i], y = src[3 * (size_t)i + 1], z = src[3 * (size_t)i + 2];
+        if (!xf.identity) {
+            double X = xform_coord(xf, 0, x, y, z);
+            double Y = xform_coord(xf, 1, x, y, z);
+            double Z = xform_coord(xf, 2, x, y, z);
+            x = X; y = Y; z = Z;
+        }
+        if (xf.snap) { x = rint(x); y = rint(y); z = rint(z); }
+        v64[i] = make_double4(x, y, z, 0.0);
+        const double L = 67108864.0;  // 2^26
+        bool g = x == rint(x) && y == rint(y) && z == rint(z) && fabs(x) <= L && fabs(y) <= L && fabs(z) <= L;
+        grid = grid && g;
+        vi[i] = g ? make_int4((int)x, (int)y, (int)z, 0) : make_int4(0, 0, 0, 0);
+        // the Morton frame only needs approximate bounds
+        float fx = (float)x, fy = (float)y, fz = (float)z;
+        l[0] = fminf(l[0], fx); l[1] = fminf(l[1], fy); l[2] = fminf(l[2], fz);
+        h[0] = fmaxf(h[0], fx); h[1] = fmaxf(h[1], fy); h[2] = fmaxf(h[2], fz);
+    }
+    uint64_t bad = __ballot(!grid);
+    if (bad && lane_id() == 0) atomicAnd(grid_flag, 0u);
+    block_bounds_merge(l, h, bounds6);
+}
+
+void launch_prepare_vertices(const double* src, uint32_t nv, Xform xf, double4* v64, int4* vi, uint32_t* grid_flag,
+                             uint32_t* bounds6, hipStream_t s) {
+    if (nv == 0) return;
+    uint32_t blocks = std::min<uint32_t>(blocks_for(nv, 256), 4096);
+    k_prepare_vertices<<<blocks, 256, 0, s>>>(src, nv, xf, v64, vi, grid_flag, bounds6);
     GI_LAUNCH_CHECK();
 }
 
-// ------------------------------------------------------------------ morton
+// ---------------------------------------------------------- triangle boxes
 
 __device__ __forceinline__ uint32_t expand10(uint32_t v) {
     v = (v * 0x00010001u) & 0xFF0000FFu;
@@ -187,30 +165,68 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ lo, const float4* __restrict__ hi,
-                                                uint32_t nt, const uint32_t* __restrict__ bounds6,
-                                                uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nt) return;
-    float4 l = lo[i], h = hi[i];
-    float c[3] = {0.5f * l.x + 0.5f * h.x, 0.5f * l.y + 0.5f * h.y, 0.5f * l.z + 0.5f * h.z};
+// 30-bit Morton code of a box centroid in the mesh bounds frame
+__device__ __forceinline__ uint32_t morton30(const float* bl, const float* bh, const float* bmin, const float* bext) {
     uint32_t q[3];
     for (int k = 0; k < 3; ++k) {
-        float bmin = key2f(bounds6[k]), bmax = key2f(bounds6[3 + k]);
-        float ext = bmax - bmin;
-        float u = (ext > 0.f && isfinite(ext)) ? (c[k] - bmin) / ext : 0.f;
+        float c = 0.5f * bl[k] + 0.5f * bh[k];
+        float u = (bext[k] > 0.f && isfinite(bext[k])) ? (c - bmin[k]) / bext[k] : 0.f;
         u = isfinite(u) ? u : 0.f;
-        float s = fminf(fmaxf(u * 1024.f, 0.f), 1023.f);
-        q[k] = (uint32_t)s;
+        q[k] = (uint32_t)fminf(fmaxf(u * 1024.f, 0.f), 1023.f);
     }
-    keys[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
-    vals[i] = i;
+    return (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
 }
 
-void launch_morton(const float4* lo, const float4* hi, uint32_t nt, const uint32_t* bounds6, uint32_t* keys,
-                   uint32_t* vals, hipStream_t s) {
+// Per triangle: FP32 box rounded outward from the exact vertex coordinates
+// (so it contains the exact box), and, when keys != nullptr, the Morton key of
+// its centroid with the triangle index as value. Grid meshes gather 16-byte
+// integer vertices instead of 32-byte doubles.
+__global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ tri, uint32_t nt,
+                                                   const double4* __restrict__ v, const int4* __restrict__ vi,
+                                                   const uint32_t* __restrict__ grid_flag,
+                                                   const uint32_t* __restrict__ bounds6, float4* __restrict__ lo,
+                                                   float4* __restrict__ hi, uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const bool grid = *grid_flag != 0;
+    uint32_t i0 = tri[3 * (size_t)i], i1 = tri[3 * (size_t)i + 1], i2 = tri[3 * (size_t)i + 2];
+    float bl[3], bh[3];
+    if (grid) {
+        int4 a = vi[i0], b = vi[i1], c = vi[i2];
+        bl[0] = f32_down((double)min(min(a.x, b.x), c.x));
+        bl[1] = f32_down((double)min(min(a.y, b.y), c.y));
+        bl[2] = f32_down((double)min(min(a.z, b.z), c.z));
+        bh[0] = f32_up((double)max(max(a.x, b.x), c.x));
+        bh[1] = f32_up((double)max(max(a.y, b.y), c.y));
+        bh[2] = f32_up((double)max(max(a.z, b.z), c.z));
+    } else {
+        double4 a = v[i0], b = v[i1], c = v[i2];
+        bl[0] = f32_down(fmin(fmin(a.x, b.x), c.x));
+        bl[1] = f32_down(fmin(fmin(a.y, b.y), c.y));
+        bl[2] = f32_down(fmin(fmin(a.z, b.z), c.z));
+        bh[0] = f32_up(fmax(fmax(a.x, b.x), c.x));
+        bh[1] = f32_up(fmax(fmax(a.y, b.y), c.y));
+        bh[2] = f32_up(fmax(fmax(a.z, b.z), c.z));
+    }
+    lo[i] = make_float4(bl[0], bl[1], bl[2], 0.f);
+    hi[i] = make_float4(bh[0], bh[1], bh[2], 0.f);
+    if (keys) {
+        float bmin[3], bext[3];
+        for (int k = 0; k < 3; ++k) {
+            bmin[k] = key2f(bounds6[k]);
+            bext[k] = key2f(bounds6[3 + k]) - bmin[k];
+        }
+        keys[i] = morton30(bl, bh, bmin, bext);
+        vals[i] = i;
+    }
+}
+
+void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,
+                      const uint32_t* bounds6, float4* lo, float4* hi, uint32_t* keys, uint32_t* vals,
+                      hipStream_t s) {
     if (nt == 0) return;
-    k_morton<<<blocks_for(nt, 256), 256, 0, s>>>(lo, hi, nt, bounds6, keys, vals);
+    k_tri_boxes<<<blocks_for(nt, 256), 256, 0, s>>>(tri, nt, v64, vi, grid_flag, bounds6, lo, hi, keys, vals);
     GI_LAUNCH_CHECK();
 }
 
@@ -315,7 +331,7 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t*
 // -------------------------------------------------------------- radix sort
 
 constexpr int SORT_THREADS = 256;
-constexpr int SORT_ITEMS = 16;
+constexpr int SORT_ITEMS = 4;
 constexpr uint32_t SORT_TILE = SORT_THREADS * SORT_ITEMS;
 constexpr int RADIX = 256;
 
@@ -435,12 +451,93 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(const K* __rest
     }
 }
 
+// Whole LSD radix sort in one workgroup (one launch) for small inputs, where
+// the multi-block passes are bound by launch gaps. Keys are ranked in chunks
+// of 1024 in (chunk, wave, lane) order, which keeps every pass stable.
+constexpr uint32_t kSmallSortMax = 4096;
+constexpr int SMALL_THREADS = 1024;
+
+template <class K>
+__global__ __launch_bounds__(SMALL_THREADS) void k_sort_small(K* k0, uint32_t* v0, K* k1, uint32_t* v1, uint32_t n,
+                                                              int passes) {
+    __shared__ uint32_t base[RADIX];
+    __shared__ uint32_t chunk_tot[RADIX];
+    __shared__ uint32_t wc[SMALL_THREADS / 64][RADIX];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt();
+    for (int p = 0; p < passes; ++p) {
+        const K* ks = (p & 1) ? k1 : k0;
+        const uint32_t* vs = (p & 1) ? v1 : v0;
+        K* kd = (p & 1) ? k0 : k1;
+        uint32_t* vd = (p & 1) ? v0 : v1;
+        const int shift = 8 * p;
+        if (threadIdx.x < RADIX) base[threadIdx.x] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += SMALL_THREADS)
+            atomicAdd(&base[(uint32_t)(ks[i] >> shift) & (RADIX - 1)], 1u);
+        __syncthreads();
+        if (threadIdx.x < 64) {  // exclusive scan of the 256 digit counts by one wave
+            uint32_t c[4], s = 0;
+            for (int q = 0; q < 4; ++q) { c[q] = base[4 * threadIdx.x + q]; s += c[q]; }
+            uint32_t x = s;
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t y = __shfl_up(x, o);
+                if (lane >= (uint32_t)o) x += y;
+            }
+            uint32_t off = x - s;
+            for (int q = 0; q < 4; ++q) { base[4 * threadIdx.x + q] = off; off += c[q]; }
+        }
+        __syncthreads();
+        for (uint32_t c0 = 0; c0 < n; c0 += SMALL_THREADS) {
+            for (int d = threadIdx.x; d < (SMALL_THREADS / 64) * RADIX; d += SMALL_THREADS) (&wc[0][0])[d] = 0;
+            __syncthreads();
+            uint32_t i = c0 + threadIdx.x;
+            bool valid = i < n;
+            K key = valid ? ks[i] : (K)0;
+            uint32_t val = valid ? vs[i] : 0u;
+            uint32_t d = (uint32_t)(key >> shift) & (RADIX - 1);
+            uint64_t match = __ballot(valid);
+            for (int b = 0; b < 8; ++b) {
+                uint64_t bb = __ballot((d >> b) & 1u);
+                match &= ((d >> b) & 1u) ? bb : ~bb;
+            }
+            uint32_t rank = (uint32_t)__popcll(match & lt);
+            if (valid && rank == 0) wc[wave][d] = (uint32_t)__popcll(match);
+            __syncthreads();
+            if (threadIdx.x < RADIX) {
+                uint32_t run = 0;
+                for (int w = 0; w < SMALL_THREADS / 64; ++w) {
+                    uint32_t t = wc[w][threadIdx.x];
+                    wc[w][threadIdx.x] = run;
+                    run += t;
+                }
+                chunk_tot[threadIdx.x] = run;
+            }
+            __syncthreads();
+            if (valid) {
+                uint32_t pos = base[d] + wc[wave][d] + rank;
+                kd[pos] = key;
+                vd[pos] = val;
+            }
+            __syncthreads();
+            if (threadIdx.x < RADIX) base[threadIdx.x] += chunk_tot[threadIdx.x];
+            __syncthreads();
+        }
+    }
+}
+
 // Ping-pongs between (keys, vals) and (tk, tv); returns 1 when the sorted
 // data ends in (tk, tv), 0 when it ends in (keys, vals).
 template <class K>
 static int radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32_t n, int bits, uint32_t* hist,
                            uint32_t* scan_tmp, hipStream_t s) {
     if (n <= 1) return 0;
+    if (n <= kSmallSortMax) {
+        int passes = (bits + 7) / 8;
+        k_sort_small<K><<<1, SMALL_THREADS, 0, s>>>(keys, vals, tk, tv, n, passes);
+        GI_LAUNCH_CHECK();
+        return passes & 1;
+    }
     uint32_t ntiles = (n + SORT_TILE - 1) / SORT_TILE;
     int passes = (bits + 7) / 8;
     K* src_k = keys;

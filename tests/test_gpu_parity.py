@@ -31,6 +31,14 @@ def test_sphere_torus_small_grid():
     assert len(res["pairs"]) > 100
 
 
+def test_sphere_torus_multi_block_sort_and_chunks():
+    # 131,072 triangles per mesh: the Morton sort takes the multi-block radix
+    # path (more than 32,768 keys) and the BVH spans 256 build chunks
+    (xa, ta), (xb, tb) = mg.sphere_torus(256, 256)
+    res, ref = _check(xa, ta, xb, tb, True)
+    assert len(res["pairs"]) > 1000
+
+
 def test_icosphere_pair_doubles_host_residual():
     (xa, ta), (xb, tb) = mg.icosphere_pair(level=4)
     res, ref = _check(xa, ta, xb, tb, False)
