@@ -68,6 +68,7 @@ struct MeshSlot {
     bool dirty = true;       // vertices/boxes/BVH boxes need an update
     Xform xf;
     int sorted_in_tmp = 0;   // the sorted Morton keys live in (tkeys, tvals)
+    bool chunk_timed = false;  // ev_chunk of this slot bracket its last k_build_chunks
     DevBuf src, tri, v64, vi, lo, hi, keys, vals, tkeys, tvals, nodes, range, blocks;
     void release() {
         DevBuf* all[] = {&src, &tri, &v64, &vi, &lo, &hi, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks};
@@ -97,14 +98,15 @@ struct gi_context {
     int host_threads = 0;
     uint32_t cand_reserve = 0;
     MeshSlot mesh[2];
-    DevBuf bounds, counters, hist, scan_tmp;
+    DevBuf bounds, counters, hist, scan_tmp, partials;
     DevBuf cand, exact, rec_key, rec_seg, seg_pool;
     uint32_t cap_cand = 0, cap_exact = 0, cap_rec = 0, cap_seg = 0;
     DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2;
     hipEvent_t ev[8] = {};
+    hipEvent_t ev_chunk[4] = {};  // k_build_chunks begin/end for mesh A and B
     void release_device() {
         for (auto& m : mesh) m.release();
-        DevBuf* all[] = {&bounds, &counters, &hist, &scan_tmp, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
+        DevBuf* all[] = {&bounds, &counters, &hist, &scan_tmp, &partials, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
                          &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2};
         for (DevBuf* b : all) b->release();
         cap_cand = cap_exact = cap_rec = cap_seg = 0;
@@ -159,7 +161,8 @@ void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
     uint32_t* tv = m.tvals.ensure<uint32_t>(nt);
     const bool topology = !m.built || rebuild;
     launch_init_mesh_scalars(grid + slot, bounds, s);
-    launch_prepare_vertices(m.src.get<double>(), m.nv, m.xf, v64, vi, grid + slot, bounds, s);
+    float* partials = ctx->partials.ensure<float>(6 * (size_t)kPrepareMaxBlocks);
+    launch_prepare_vertices(m.src.get<double>(), m.nv, m.xf, v64, vi, grid + slot, partials, bounds, s);
     launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, vi, grid + slot, bounds, lo, hi, topology ? keys : nullptr,
                      topology ? vals : nullptr, s);
     if (topology) {
@@ -169,7 +172,9 @@ void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
         m.built = true;
     }
     // build, or refit on the kept sorted keys (same topology, new boxes)
-    launch_build_bvh(m.sorted_in_tmp ? tk : keys, m.sorted_in_tmp ? tv : vals, m.nt, lo, hi, nodes, range, blocks, s);
+    launch_build_bvh(m.sorted_in_tmp ? tk : keys, m.sorted_in_tmp ? tv : vals, m.nt, lo, hi, nodes, range, blocks, s,
+                     ctx->ev_chunk[2 * slot], ctx->ev_chunk[2 * slot + 1]);
+    m.chunk_timed = m.nt > 0;
     HIP_TRY(hipGetLastError());
     m.dirty = false;
 }
@@ -237,6 +242,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     hipStream_t s = ctx->stream;
     gi_stats& st = out->stats;
     HIP_TRY(hipEventRecord(ctx->ev[0], s));
+    MA.chunk_timed = MB.chunk_timed = false;
     prepare_mesh(ctx, 0, o.rebuild != 0);
     prepare_mesh(ctx, 1, o.rebuild != 0);
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
@@ -286,6 +292,11 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     st.filter_yellow = hc[CNT_FILTER_YELLOW];
     st.grid = hgrid[0] && hgrid[1];
     st.ms_build = elapsed(ctx->ev[0], ctx->ev[1]);
+    for (int sl = 0; sl < 2; ++sl)
+        if (ctx->mesh[sl].chunk_timed) {
+            st.ms_bvh_chunks += elapsed(ctx->ev_chunk[2 * sl], ctx->ev_chunk[2 * sl + 1]);
+            st.bvh_chunk_launches++;
+        }
     st.ms_traverse = elapsed(ctx->ev[6], ctx->ev[2]);
     st.ms_filter = elapsed(ctx->ev[2], ctx->ev[3]);
 
@@ -527,6 +538,7 @@ int gi_create(const gi_config* cfg, gi_context** out) {
         HIP_TRY(hipSetDevice(ctx->device));
         HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+        for (auto& e : ctx->ev_chunk) HIP_TRY(hipEventCreate(&e));
         for (auto& m : ctx->mesh) m.xf = identity_xform();
         *out = ctx;
         return GI_OK;
@@ -543,6 +555,8 @@ int gi_destroy(gi_context* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->release_device();
     for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->ev_chunk)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

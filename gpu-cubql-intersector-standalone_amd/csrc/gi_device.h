@@ -47,10 +47,11 @@ struct MeshDev {
 void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
 // grid flag = 1, bounds6 = empty box (order-preserving keys)
 void launch_init_mesh_scalars(uint32_t* grid_flag, uint32_t* bounds6, hipStream_t s);
-// bounds6 (3 min keys, 3 max keys of order-preserving float keys) must be
-// cleared to (0xFFFFFFFF x3, 0 x3) before; it receives the mesh bounds.
+// bounds6 receives the mesh bounds (3 min, 3 max order-preserving float
+// keys); partials needs 6 * kPrepareMaxBlocks floats.
+constexpr uint32_t kPrepareMaxBlocks = 4096;
 void launch_prepare_vertices(const double* src, uint32_t nv, Xform xf, double4* v64, int4* vi, uint32_t* grid_flag,
-                             uint32_t* bounds6, hipStream_t s);
+                             float* partials, uint32_t* bounds6, hipStream_t s);
 // Triangle boxes; with keys/vals non-null also Morton keys (frame bounds6)
 // and the identity permutation.
 void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,
@@ -83,8 +84,10 @@ struct BlockLevels {
     const float4* level[kMaxBlockLevels];
 };
 size_t block_levels_elems(uint32_t n);
+// chunk_begin/chunk_end (may be null) bracket the k_build_chunks launch.
 void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
-                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s);
+                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s,
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end);
 
 void launch_traverse(MeshDev A, MeshDev B, uint2* cand, uint32_t cap, uint32_t* counters, hipStream_t s);
 void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const uint32_t* grid_flags,

@@ -98,8 +98,9 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// block-reduce a box and merge it into bounds6 (3 min keys, 3 max keys)
-__device__ __forceinline__ void block_bounds_merge(const float* l, const float* h, uint32_t* bounds6) {
+// block-reduce a box; thread 0 writes it as 6 floats to out (one box per
+// block: a second pass reduces them, no same-address atomics)
+__device__ __forceinline__ void block_box_partial(const float* l, const float* h, float* out) {
     __shared__ float red[6][4];
     uint32_t wave = threadIdx.x >> 6;
     for (int k = 0; k < 3; ++k) {
@@ -111,16 +112,30 @@ __device__ __forceinline__ void block_bounds_merge(const float* l, const float* 
         int k = threadIdx.x;
         float r = red[k][0];
         for (int w = 1; w < 4; ++w) r = k < 3 ? fminf(r, red[k][w]) : fmaxf(r, red[k][w]);
-        if (k < 3) atomicMin(&bounds6[k], f2key(r));
-        else atomicMax(&bounds6[k], f2key(r));
+        out[k] = r;
     }
 }
 
+// one block: the partial boxes -> bounds6 (order-preserving keys)
+__global__ __launch_bounds__(256) void k_reduce_bounds(const float* __restrict__ part, uint32_t nparts,
+                                                       uint32_t* bounds6) {
+    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x)
+        for (int k = 0; k < 3; ++k) {
+            l[k] = fminf(l[k], part[6 * i + k]);
+            h[k] = fmaxf(h[k], part[6 * i + 3 + k]);
+        }
+    __shared__ float fin[6];
+    block_box_partial(l, h, fin);
+    __syncthreads();
+    if (threadIdx.x < 6) bounds6[threadIdx.x] = f2key(fin[threadIdx.x]);
+}
+
 // Moves the vertices (when a transform is set), classifies the mesh as grid
-// or not, writes the integer copy and the mesh bounds (Morton frame).
+// or not, writes the integer copy and per-block partial bounds (Morton frame).
 __global__ __launch_bounds__(256) void k_prepare_vertices(const double* __restrict__ src, uint32_t nv, Xform xf,
                                                           double4* __restrict__ v64, int4* __restrict__ vi,
-                                                          uint32_t* grid_flag, uint32_t* bounds6) {
+                                                          uint32_t* grid_flag, float* __restrict__ part) {
     bool grid = true;
     float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
@@ -144,14 +159,15 @@ __global__ __launch_bounds__(256) void k_prepare_vertices(const double* __restri
     }
     uint64_t bad = __ballot(!grid);
     if (bad && lane_id() == 0) atomicAnd(grid_flag, 0u);
-    block_bounds_merge(l, h, bounds6);
+    block_box_partial(l, h, part + 6 * (size_t)blockIdx.x);
 }
 
 void launch_prepare_vertices(const double* src, uint32_t nv, Xform xf, double4* v64, int4* vi, uint32_t* grid_flag,
-                             uint32_t* bounds6, hipStream_t s) {
+                             float* partials, uint32_t* bounds6, hipStream_t s) {
     if (nv == 0) return;
-    uint32_t blocks = std::min<uint32_t>(blocks_for(nv, 256), 4096);
-    k_prepare_vertices<<<blocks, 256, 0, s>>>(src, nv, xf, v64, vi, grid_flag, bounds6);
+    uint32_t blocks = std::min<uint32_t>(blocks_for(nv, 256), kPrepareMaxBlocks);
+    k_prepare_vertices<<<blocks, 256, 0, s>>>(src, nv, xf, v64, vi, grid_flag, partials);
+    k_reduce_bounds<<<1, 256, 0, s>>>(partials, blocks, bounds6);
     GI_LAUNCH_CHECK();
 }
 
@@ -650,8 +666,6 @@ __global__ __launch_bounds__(256) void k_build_chunks(const uint32_t* __restrict
         uint32_t right = hi_i == gamma + 1 ? (uint32_t)(n - 1 + gamma + 1) : (uint32_t)(gamma + 1);
         uint32_t rope = hi_i == n - 1 ? kSentinel : chain_top(K, hi_i + 1);
         range[i] = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
-        nodes[2 * (size_t)i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(left));
-        nodes[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rope));
         left_int[i - cs] = left;
         rope_int[i - cs] = rope;
         if (lo_i >= cs && hi_i < ce) {  // in-chunk node: both children are in the chunk too
@@ -660,6 +674,11 @@ __global__ __launch_bounds__(256) void k_build_chunks(const uint32_t* __restrict
                 if (c[e] >= (uint32_t)(n - 1)) par_leaf[c[e] - (uint32_t)(n - 1) - (uint32_t)cs] = (uint32_t)i;
                 else par_int[c[e] - (uint32_t)cs] = (uint32_t)i;
             }
+        } else {
+            // top node: written whole here, its box comes from k_top_boxes;
+            // in-chunk nodes are written once, with their box, below
+            nodes[2 * (size_t)i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(left));
+            nodes[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rope));
         }
     }
     float lb[kBuildChunk / 256][6];
@@ -785,9 +804,12 @@ size_t block_levels_elems(uint32_t n) {
 }
 
 void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
-                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s) {
+                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s,
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end) {
     if (n == 0) return;
+    if (chunk_begin) (void)hipEventRecord(chunk_begin, s);
     k_build_chunks<<<blocks_for(n, kBuildChunk), 256, 0, s>>>(sorted_keys, sorted_prims, (int)n, lo, hi, nodes, range);
+    if (chunk_end) (void)hipEventRecord(chunk_end, s);
     if (n < 2) return;
     BlockLevels lv;
     lv.level[0] = nodes + 2 * (size_t)(n - 1);

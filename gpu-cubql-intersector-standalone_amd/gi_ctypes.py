@@ -30,18 +30,20 @@ class GiOptions(C.Structure):
 _STAT_U64 = ["traversal_hits", "candidates", "filter_no", "filter_green", "filter_yellow", "exact_pairs",
              "exact_yellow_yes", "exact_yellow_no", "host_pairs", "host_yes", "pairs", "segments", "coplanar",
              "degenerate"]
-_STAT_F32 = ["ms_total", "ms_build", "ms_traverse", "ms_filter", "ms_exact", "ms_output", "ms_host", "pad"]
+_STAT_F32 = ["ms_total", "ms_build", "ms_traverse", "ms_filter", "ms_exact", "ms_output", "ms_host",
+             "ms_bvh_chunks"]
 
 
 class GiStats(C.Structure):
     _fields_ = ([(n, C.c_uint64) for n in _STAT_U64] + [("grid", C.c_int32), ("retries", C.c_int32)]
-                + [(n, C.c_float) for n in _STAT_F32])
+                + [(n, C.c_float) for n in _STAT_F32] + [("bvh_chunk_launches", C.c_int32), ("pad", C.c_int32)])
 
     def as_dict(self):
         d = {n: int(getattr(self, n)) for n in _STAT_U64}
-        d.update({n: float(getattr(self, n)) for n in _STAT_F32 if n != "pad"})
+        d.update({n: float(getattr(self, n)) for n in _STAT_F32})
         d["grid"] = int(self.grid)
         d["retries"] = int(self.retries)
+        d["bvh_chunk_launches"] = int(self.bvh_chunk_launches)
         return d
 
 

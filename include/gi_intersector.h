@@ -96,7 +96,9 @@ typedef struct {
     int32_t retries;            /* capacity regrowths during this call */
     /* milliseconds (GPU events; host stage by wall clock) */
     float ms_total, ms_build, ms_traverse, ms_filter, ms_exact, ms_output, ms_host;
-    float pad;
+    float ms_bvh_chunks;        /* sum of the k_build_chunks launches of this call */
+    int32_t bvh_chunk_launches; /* how many (0, 1 or 2: one per mesh built or refit) */
+    int32_t pad;
 } gi_stats;
 
 typedef struct {
