@@ -10,8 +10,8 @@ exact segments, sort of the intersecting pairs and gather of their segments
 
 value = exact candidate pairs decided per second (candidates / step time).
 The CPU baseline is the oracle (C restatement of the reference CPU path:
-box sweep + filtered exact predicates + exact segments) on a strided sample
-of mesh A against all of mesh B, timed on this host's cores.
+box sweep + filtered exact predicates + exact segments) on the same meshes,
+timed on this host's cores (median of 5 runs after a warm-up).
 """
 from __future__ import annotations
 
