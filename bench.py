@@ -93,6 +93,85 @@ def cpu_baseline(xa, ta, xb, tb, stride, reps):
                       f"{dt:.2f} s per run ({sum(times):.1f} s timed)"}
 
 
+def serving_leg(gi, mg, n_requests, qps, slots):
+    """Serving workload: a resident 2,097,152-triangle displaced sphere; Poisson
+    requests (seeded) moving a 262,144-triangle torus (80 %) or a 4,096-triangle
+    probe (20 %) by a random rigid motion; each request = transform + refit of
+    the moving mesh, query against the resident BVH, exact segments, host copy.
+    Open loop at `qps`, then a closed burst of the same requests (all queued at
+    once) for the saturation throughput."""
+    import queue
+    import threading
+
+    w = mg.serving_workload(n_requests=n_requests, qps=qps)
+    t_setup = time.perf_counter()
+    srv = gi.Server(*w["static"], slots=slots, snap=True, want_segments=True)
+    for m in w["meshes"]:
+        srv.add_mesh(*m)
+    setup_s = time.perf_counter() - t_setup
+    reqs = w["requests"]
+    for mid in (0, 1):  # warm both meshes on every slot
+        first = next(m for _, k, m in reqs if k == mid)
+        for t in [srv.submit(mid, first) for _ in range(2 * slots)]:
+            srv.wait(t, decode=False)
+
+    def run(arrivals):
+        todo, out = queue.Queue(), []
+
+        def waiter():
+            while True:
+                t = todo.get()
+                if t is None:
+                    return
+                r = srv.wait(t, decode=False)
+                out.append((r["info"], len(r["pairs"]), r["n_segments"]))
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        lag = []
+        t0 = time.perf_counter()
+        for (_, mid, m), ta in zip(reqs, arrivals):
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            lag.append(time.perf_counter() - t0 - ta)
+            todo.put(srv.submit(mid, m))
+        todo.put(None)
+        th.join()
+        wall = time.perf_counter() - t0
+        return out, wall, lag
+
+    open_loop, wall_open, lag = run([a for a, _, _ in reqs])
+    burst, wall_burst, _ = run([0.0] * len(reqs))
+    srv.close()
+
+    def pct(v, q):
+        return float(np.percentile(np.asarray(v), q))
+
+    lat = [i["latency_ms"] for i, _, _ in open_loop]
+    svc = [i["service_ms"] for i, _, _ in open_loop]
+    by_mesh = {}
+    for (_, mid, _), (i, npairs, _) in zip(reqs, open_loop):
+        by_mesh.setdefault(mid, []).append(i["service_ms"])
+    return {
+        "workload": f"static {w['static'][1].shape[0]} triangles resident; {len(reqs)} Poisson requests at "
+                    f"{qps:g}/s (seed 7): torus {w['meshes'][0][1].shape[0]} / probe "
+                    f"{w['meshes'][1][1].shape[0]} triangles, random rigid motion, exact segments to host",
+        "slots": slots, "requests": len(reqs), "qps_offered": qps,
+        "qps_achieved": len(reqs) / wall_open,
+        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_ms": max(lat),
+        "service_p50_ms": pct(svc, 50), "service_p99_ms": pct(svc, 99),
+        "service_p50_torus_ms": pct(by_mesh.get(0, [0.0]), 50),
+        "service_p50_probe_ms": pct(by_mesh.get(1, [0.0]), 50),
+        "saturation_rps": len(reqs) / wall_burst,
+        "burst_p50_ms": pct([i["latency_ms"] for i, _, _ in burst], 50),
+        "burst_p99_ms": pct([i["latency_ms"] for i, _, _ in burst], 99),
+        "submit_lag_p99_ms": pct(lag, 99) * 1000.0,
+        "pairs_mean": float(np.mean([n for _, n, _ in open_loop])),
+        "setup_s": setup_s,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +181,10 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=1)
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-qps", type=float, default=200.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--check", action="store_true", help="compare the pair set with the oracle (slow)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (3 times): the reference kernel for scaling "
@@ -205,6 +288,8 @@ def main():
         torch.cuda.synchronize()
         print(f"copy calibration: {args.copy_calib} MiB read + {args.copy_calib} MiB written per dispatch",
               file=sys.stderr)
+    if not args.no_serve:
+        line["serving"] = serving_leg(gi, mg, args.serve_requests, args.serve_qps, args.serve_slots)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(xa, ta, xb, tb, args.cpu_stride, args.cpu_reps)
     print(json.dumps(line), flush=True)

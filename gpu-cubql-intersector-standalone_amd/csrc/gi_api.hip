@@ -69,10 +69,15 @@ struct MeshSlot {
     Xform xf;
     int sorted_in_tmp = 0;   // the sorted Morton keys live in (tkeys, tvals)
     bool chunk_timed = false;  // ev_chunk of this slot bracket its last k_build_chunks
+    bool borrowed = false;     // device buffers belong to another context (gi_share_mesh)
     DevBuf src, tri, v64, vi, lo, hi, keys, vals, tkeys, tvals, nodes, range, blocks;
     void release() {
         DevBuf* all[] = {&src, &tri, &v64, &vi, &lo, &hi, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks};
-        for (DevBuf* b : all) b->release();
+        for (DevBuf* b : all) {
+            if (borrowed) { b->p = nullptr; b->bytes = 0; }
+            else b->release();
+        }
+        if (borrowed) { borrowed = false; set = false; }
         on_device = false;
         built = false;
         dirty = true;
@@ -141,6 +146,7 @@ void upload_mesh(gi_context* ctx, int slot) {
 // vertices -> boxes -> (Morton sort + Karras build | leaf refresh) -> refit
 void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
     MeshSlot& m = ctx->mesh[slot];
+    if (m.borrowed) return;  // built and kept current by its owner
     if (!m.on_device) upload_mesh(ctx, slot);
     if (!m.dirty && m.built && !rebuild) return;
     hipStream_t s = ctx->stream;
@@ -638,6 +644,7 @@ int gi_set_transform(gi_context* ctx, int slot, const double rot_deg[3], const d
                      int snap_to_grid) {
     if (!ctx || slot < 0 || slot > 1 || !rot_deg || !trans || !center) return GI_ERR_INVALID_ARGUMENT;
     if (!ctx->mesh[slot].set) return GI_ERR_NOT_READY;
+    if (ctx->mesh[slot].borrowed) return GI_ERR_INVALID_ARGUMENT;  // a shared mesh is read-only
     Xform x = rigid_xform(rot_deg, trans, center);
     x.snap = snap_to_grid ? 1 : 0;
     ctx->mesh[slot].xf = x;
@@ -648,6 +655,7 @@ int gi_set_transform(gi_context* ctx, int slot, const double rot_deg[3], const d
 int gi_set_transform_matrix(gi_context* ctx, int slot, const double m[12], int snap_to_grid) {
     if (!ctx || slot < 0 || slot > 1 || !m) return GI_ERR_INVALID_ARGUMENT;
     if (!ctx->mesh[slot].set) return GI_ERR_NOT_READY;
+    if (ctx->mesh[slot].borrowed) return GI_ERR_INVALID_ARGUMENT;  // a shared mesh is read-only
     Xform x = identity_xform();
     bool ident = true;
     for (int r = 0; r < 3; ++r)
@@ -707,6 +715,47 @@ int gi_intersect_oneshot(const gi_config* cfg, const double* xyzA, uint32_t nvA,
     if (!st) st = gi_intersect(ctx, &o, out);
     gi_destroy(ctx);
     return st;
+}
+
+int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) {
+    if (!dst || !src || dst == src || dst_slot < 0 || dst_slot > 1 || src_slot < 0 || src_slot > 1)
+        return GI_ERR_INVALID_ARGUMENT;
+    if (dst->device != src->device) return GI_ERR_INVALID_ARGUMENT;
+    if (!src->mesh[src_slot].set) return GI_ERR_NOT_READY;
+    try {
+        HIP_TRY(hipSetDevice(src->device));
+        prepare_mesh(src, src_slot, false);
+        HIP_TRY(hipStreamSynchronize(src->stream));
+        MeshSlot& d = dst->mesh[dst_slot];
+        const MeshSlot& s = src->mesh[src_slot];
+        d.release();
+        d.h_xyz = s.h_xyz;
+        d.h_tri = s.h_tri;
+        d.nv = s.nv;
+        d.nt = s.nt;
+        d.set = true;
+        d.on_device = true;
+        d.built = true;
+        d.dirty = false;
+        d.xf = s.xf;
+        d.sorted_in_tmp = s.sorted_in_tmp;
+        d.borrowed = true;
+        DevBuf* dd[] = {&d.src, &d.tri, &d.v64, &d.vi, &d.lo, &d.hi, &d.keys, &d.vals, &d.tkeys, &d.tvals, &d.nodes,
+                        &d.range, &d.blocks};
+        const DevBuf* ss[] = {&s.src, &s.tri, &s.v64, &s.vi, &s.lo, &s.hi, &s.keys, &s.vals, &s.tkeys, &s.tvals,
+                              &s.nodes, &s.range, &s.blocks};
+        for (int i = 0; i < 13; ++i) { dd[i]->p = ss[i]->p; dd[i]->bytes = ss[i]->bytes; }
+        // the owner's grid classification of this mesh
+        uint32_t* dgrid = dst->counters.ensure<uint32_t>(CNT_COUNT + 2) + CNT_COUNT;
+        const uint32_t* sgrid = src->counters.get<uint32_t>() + CNT_COUNT;
+        HIP_TRY(hipSetDevice(dst->device));
+        HIP_TRY(hipMemcpyAsync(dgrid + dst_slot, sgrid + src_slot, sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                               dst->stream));
+        HIP_TRY(hipStreamSynchronize(dst->stream));
+        return GI_OK;
+    } catch (const GiError& e) {
+        return fail(dst, e);
+    }
 }
 
 int gi_clear_device(gi_context* ctx) {

@@ -54,6 +54,17 @@ class GiResult(C.Structure):
                 ("seg_f64", C.POINTER(C.c_double)), ("stats", GiStats), ("internal", C.c_void_p)]
 
 
+class GiServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("snap_to_grid", C.c_int32),
+                ("want_segments", C.c_int32), ("hbm_reserve_bytes", C.c_uint64)]
+
+
+class GiRequestInfo(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("service_ms", C.c_double), ("latency_ms", C.c_double),
+                ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double), ("slot", C.c_int32), ("status", C.c_int32),
+                ("refusals", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class GiError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"gi status {status}: {msg}")
@@ -64,7 +75,9 @@ _lib = None
 
 SYMBOLS = ["gi_create", "gi_destroy", "gi_last_error", "gi_status_string", "gi_set_mesh", "gi_set_mesh_device",
            "gi_set_transform", "gi_set_transform_matrix", "gi_build", "gi_intersect", "gi_result_free",
-           "gi_intersect_oneshot", "gi_clear_device", "gi_reconstruct_device", "gi_exact_pair_host"]
+           "gi_intersect_oneshot", "gi_share_mesh", "gi_clear_device", "gi_reconstruct_device",
+           "gi_server_create", "gi_server_add_mesh", "gi_server_submit", "gi_server_wait", "gi_server_destroy",
+           "gi_exact_pair_host"]
 
 
 def lib():
@@ -93,6 +106,12 @@ def lib():
         L.gi_reconstruct_device.argtypes = [vp]
         ip = C.POINTER(C.c_int32)
         L.gi_exact_pair_host.argtypes = [dp, dp, ip, ip, ip, ip, ip, C.POINTER(C.c_uint64), C.c_int32]
+        L.gi_share_mesh.argtypes = [vp, C.c_int, vp, C.c_int]
+        L.gi_server_create.argtypes = [C.POINTER(GiServerConfig), dp, C.c_uint32, up, C.c_uint32, C.POINTER(vp)]
+        L.gi_server_add_mesh.argtypes = [vp, dp, C.c_uint32, up, C.c_uint32, ip]
+        L.gi_server_submit.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_uint64)]
+        L.gi_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(GiResult), C.POINTER(GiRequestInfo)]
+        L.gi_server_destroy.argtypes = [vp]
         for name in SYMBOLS:
             if name not in ("gi_last_error", "gi_status_string"):
                 getattr(L, name).restype = C.c_int
@@ -214,6 +233,58 @@ def result_to_py(res: GiResult, want_segments=True, decode=True):
     elif want_segments:
         out["segments"] = [[] for _ in range(n)]
     return out
+
+
+class Server:
+    """gi_server: one resident static mesh, registered moving meshes, queued
+    rigid-transform requests processed by `slots` concurrent GPU slots."""
+
+    def __init__(self, xyz, tri, slots=4, snap=True, want_segments=True, device=0, hbm_reserve_bytes=0):
+        self._s = C.c_void_p()
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        tri = np.ascontiguousarray(tri, dtype=np.uint32).reshape(-1, 3)
+        cfg = GiServerConfig(device, slots, int(snap), int(want_segments), hbm_reserve_bytes)
+        _check(lib().gi_server_create(C.byref(cfg), xyz.ctypes.data_as(C.POINTER(C.c_double)), xyz.shape[0],
+                                      tri.ctypes.data_as(C.POINTER(C.c_uint32)), tri.shape[0], C.byref(self._s)))
+
+    def add_mesh(self, xyz, tri) -> int:
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        tri = np.ascontiguousarray(tri, dtype=np.uint32).reshape(-1, 3)
+        mid = C.c_int32()
+        _check(lib().gi_server_add_mesh(self._s, xyz.ctypes.data_as(C.POINTER(C.c_double)), xyz.shape[0],
+                                        tri.ctypes.data_as(C.POINTER(C.c_uint32)), tri.shape[0], C.byref(mid)))
+        return mid.value
+
+    def submit(self, mesh_id: int, m) -> int:
+        arr = (C.c_double * 12)(*np.asarray(m, dtype=np.float64).ravel())
+        t = C.c_uint64()
+        _check(lib().gi_server_submit(self._s, mesh_id, arr, C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, decode=True):
+        res = GiResult()
+        info = GiRequestInfo()
+        st = lib().gi_server_wait(self._s, ticket, C.byref(res), C.byref(info))
+        try:
+            _check(st)
+            out = result_to_py(res, True, decode)
+        finally:
+            lib().gi_result_free(C.byref(res))
+        out["info"] = {"queue_ms": info.queue_ms, "service_ms": info.service_ms, "latency_ms": info.latency_ms,
+                       "t_admit_ms": info.t_admit_ms, "t_done_ms": info.t_done_ms, "slot": info.slot,
+                       "refusals": info.refusals}
+        return out
+
+    def close(self):
+        if self._s:
+            lib().gi_server_destroy(self._s)
+            self._s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def exact_pair_host(a, b, limbs_cap=64):

@@ -217,3 +217,70 @@ def height_field_pairs(cells: int = 512, seed: int = 5, spacing: int = 2048):
     L = float(cells * spacing)
     rot = np.stack([L - p[:, 1], p[:, 0], p[:, 2]], axis=1)
     return (p, t), (shifted, t), (rot, t)
+
+
+# ----------------------------------------------------------------- serving
+
+
+def gaussian(seed: int, n: int, stream: int = 0) -> np.ndarray:
+    """Box-Muller normals from two SplitMix64 uniform streams."""
+    u1 = uniform01(seed, n, 2 * stream)
+    u2 = uniform01(seed, n, 2 * stream + 1)
+    return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def quat_matrix(q) -> np.ndarray:
+    """Rotation matrix of a quaternion (w, x, y, z); normalised here."""
+    w, x, y, z = np.asarray(q, dtype=np.float64) / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def affine_about(R: np.ndarray, center, trans) -> np.ndarray:
+    """Row-major 3x4 map p -> R (p - c) + c + t."""
+    c = np.asarray(center, dtype=np.float64)
+    m = np.zeros((3, 4))
+    m[:, :3] = R
+    m[:, 3] = c + np.asarray(trans, dtype=np.float64) - R @ c
+    return m
+
+
+def apply_affine(m: np.ndarray, x: np.ndarray, snap_to_grid: bool = True) -> np.ndarray:
+    """The library's vertex map: ((m0 x + m1 y) + m2 z) + m3 per row, each
+    operation rounded separately, then rint when snapping."""
+    out = np.empty_like(x)
+    for r in range(3):
+        out[:, r] = ((m[r, 0] * x[:, 0] + m[r, 1] * x[:, 1]) + m[r, 2] * x[:, 2]) + m[r, 3]
+    return np.rint(out) if snap_to_grid else out
+
+
+def serving_workload(n_requests: int = 400, qps: float = 200.0, seed: int = 7, static_res=(1024, 1024),
+                     torus_res=(512, 256), probe_res=(64, 32), torus_share: float = 0.8):
+    """Static displaced sphere (2 * nu * nv triangles, 2,097,152 at 1024x1024)
+    and two moving meshes registered around the grid centre 2^23: a torus
+    (262,144 triangles at 512x256, R=0.8, r=0.3) and a probe sphere of radius
+    0.1 (4,096 triangles at 64x32). Requests: Poisson arrivals at `qps`;
+    `torus_share` of them move the torus by a rotation uniform on SO(3)
+    (quaternion of 4 normals) about the centre plus a translation
+    U[-1/2, 1/2]^3; the rest put the probe at a uniform direction on the unit
+    sphere. All in grid units (2^22 per unit), snapped.
+    Returns dict(static=(xyz, tri), meshes=[torus, probe], requests=[(t_s, mesh_id, m)])."""
+    sp, st = uv_sphere(static_res[0], static_res[1], 3)
+    tp, tt = torus(*torus_res)
+    pp, pt = uv_sphere(probe_res[0], probe_res[1], 11, amp=0.0)
+    c = np.full(3, 2.0 * GRID_SCALE)
+    arrivals = np.cumsum(-np.log1p(-uniform01(seed, n_requests, 0)) / qps)
+    kind = uniform01(seed, n_requests, 1) >= torus_share           # True -> probe
+    q = np.stack([gaussian(seed, n_requests, 1 + k) for k in range(4)], axis=1)
+    t = np.stack([uniform01(seed, n_requests, 20 + k) for k in range(3)], axis=1) - 0.5
+    d = np.stack([gaussian(seed, n_requests, 5 + k) for k in range(3)], axis=1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    reqs = []
+    for i in range(n_requests):
+        R = quat_matrix(q[i])
+        if kind[i]:
+            reqs.append((float(arrivals[i]), 1, affine_about(R, c, d[i] * GRID_SCALE)))
+        else:
+            reqs.append((float(arrivals[i]), 0, affine_about(R, c, t[i] * GRID_SCALE)))
+    return dict(static=(snap(sp), st), meshes=[(snap(tp), tt), (snap(0.1 * pp), pt)], requests=reqs)

@@ -145,10 +145,60 @@ int gi_intersect_oneshot(const gi_config* cfg, const double* xyzA, uint32_t nvA,
                          const double* xyzB, uint32_t nvB, const uint32_t* triB, uint32_t ntB,
                          const gi_options* opt, gi_result* out);
 
+/* Makes dst's slot a read-only view of src's (built) slot: no copy; src must
+ * outlive dst's use of it. Used to serve many queries against one resident
+ * BVH (no reference counterpart: the reference keeps one controller per pair). */
+int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot);
+
 /* Frees all device memory of the context but keeps the host copies; the next
  * call rebuilds (reference: clearGPU / reconstructGPU). */
 int gi_clear_device(gi_context* ctx);
 int gi_reconstruct_device(gi_context* ctx);
+
+/* ------------------------------------------------------------- serving
+ * A request loop around one resident static mesh (mesh A, built once): each
+ * request names a registered moving mesh and a rigid transform (row-major
+ * 3x4); the moving mesh is transformed and its BVH refit (same topology),
+ * then intersected with the static one. `slots` requests run on the GPU at
+ * once, each on its own stream; requests that arrive meanwhile queue and are
+ * admitted as slots free up and while free HBM stays above the reserve.
+ * No reference counterpart (the reference runs one query at a time from its
+ * GUI loop, apps/mainApplication.cpp:1693-1722). */
+typedef struct gi_server gi_server;
+
+typedef struct {
+    int32_t device;
+    int32_t slots;              /* requests in flight on the GPU, 0 = 4 */
+    int32_t snap_to_grid;       /* round moved vertices to integers (GPU exact stage) */
+    int32_t want_segments;
+    uint64_t hbm_reserve_bytes; /* admission: free HBM to keep, 0 = 2 GiB; a request is
+                                   admitted anyway when no other request runs, so
+                                   UINT64_MAX serialises the requests */
+} gi_server_config;
+
+typedef struct {
+    double queue_ms;   /* submit -> admitted to a slot */
+    double service_ms; /* admitted -> result ready */
+    double latency_ms; /* submit -> result ready */
+    double t_admit_ms; /* admission time, ms since gi_server_create */
+    double t_done_ms;  /* completion time, ms since gi_server_create */
+    int32_t slot;
+    int32_t status;    /* gi_status of the request */
+    uint32_t refusals; /* admission checks this request failed (free HBM below the reserve) */
+    uint32_t pad;
+} gi_request_info;
+
+int gi_server_create(const gi_server_config* cfg, const double* xyz, uint32_t nv, const uint32_t* tri, uint32_t nt,
+                     gi_server** out);
+/* registers a moving mesh (uploaded once per slot); returns its id */
+int gi_server_add_mesh(gi_server* srv, const double* xyz, uint32_t nv, const uint32_t* tri, uint32_t nt,
+                       int32_t* mesh_id);
+/* non-blocking; m = row-major 3x4 affine map applied to the registered vertices */
+int gi_server_submit(gi_server* srv, int32_t mesh_id, const double m[12], uint64_t* ticket);
+/* blocks until the ticket's result is ready; out holds host arrays
+ * (gi_result_free), info may be NULL */
+int gi_server_wait(gi_server* srv, uint64_t ticket, gi_result* out, gi_request_info* info);
+int gi_server_destroy(gi_server* srv);
 
 /* Exact intersection of one triangle pair on the host (the residual stage).
  * a, b: 9 doubles each. seg: up to 6 segments * 8 numbers * limbs_cap limbs.

@@ -32,6 +32,13 @@ def test_status_strings_and_null_arguments():
     assert lib.gi_destroy(None) == 1  # GI_ERR_INVALID_ARGUMENT
     assert lib.gi_intersect(None, None, None) == 1
     assert lib.gi_result_free(None) == 1
+    # serving entry points reject missing arguments before touching a device
+    t = ctypes.c_uint64()
+    assert lib.gi_server_create(None, None, 0, None, 0, None) == 1
+    assert lib.gi_server_submit(None, 0, None, ctypes.byref(t)) == 1
+    assert lib.gi_server_wait(None, 1, None, None) == 1
+    assert lib.gi_server_destroy(None) == 1
+    assert lib.gi_share_mesh(None, 0, None, 0) == 1
 
 
 def test_host_exact_stage_matches_golden():
