@@ -172,6 +172,82 @@ def serving_leg(gi, mg, n_requests, qps, slots):
     }
 
 
+def batch_leg(gi, mg, n_pairs, steps):
+    """Batched workload: n_pairs independent mesh pairs (seed 6), each side an
+    icosphere, displaced sphere or torus with N log-uniform in [500, 200000],
+    B placed at U[0.3, 1.5] radii from A, 2^24 grid. Batched = one pass of
+    launches over the whole batch (gi_intersect_batch); plain = the
+    single-pair API once per pair (gi_set_mesh x2 + gi_intersect). Both are
+    also timed end to end from host arrays (uploads included)."""
+    from ctypes import byref
+
+    t_gen = time.perf_counter()
+    pairs = mg.batch_workload(n_pairs)
+    gen_s = time.perf_counter() - t_gen
+    ntri = int(sum(a[1].shape[0] + b[1].shape[0] for a, b in pairs))
+    free = gi.lib().gi_result_free
+
+    eng = gi.Intersector()
+    t0 = time.perf_counter()
+    eng.set_batch(pairs)
+    set_s = time.perf_counter() - t0
+    for _ in range(2):  # warm-up: arenas grow to their final sizes here
+        res, offs = eng.intersect_batch_raw(output_on_device=True, want_segments=True, rebuild=True)
+        free(byref(res))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res, offs = eng.intersect_batch_raw(output_on_device=True, want_segments=True, rebuild=True)
+        st = res.stats.as_dict()
+        free(byref(res))
+    step_s = (time.perf_counter() - t0) / steps
+    counts = np.diff(offs.astype(np.int64))
+    # end to end from host arrays: set + one query (pairs to device memory)
+    t0 = time.perf_counter()
+    eng.set_batch(pairs)
+    res, _ = eng.intersect_batch_raw(output_on_device=True, want_segments=True, rebuild=False)
+    free(byref(res))
+    e2e_batch_s = time.perf_counter() - t0
+    sample = {i: r for i, r in enumerate(eng.intersect_batch(want_segments=False)[0]) if i % 97 == 0}
+    eng.close()
+
+    plain = gi.Intersector()
+    plain_counts, plain_cands, plain_sample = [], 0, {}
+    t0 = time.perf_counter()
+    for (xa, ta), (xb, tb) in pairs:
+        plain.set_mesh(0, xa, ta)
+        plain.set_mesh(1, xb, tb)
+        res = plain.intersect_raw(output_on_device=True, want_segments=True)
+        plain_counts.append(int(res.n_pairs))
+        plain_cands += int(res.stats.candidates)
+        free(byref(res))
+    plain_s = time.perf_counter() - t0
+    for i in sample:
+        (xa, ta), (xb, tb) = pairs[i]
+        plain.set_mesh(0, xa, ta)
+        plain.set_mesh(1, xb, tb)
+        plain_sample[i] = plain.intersect(want_segments=False)["pairs"]
+    plain.close()
+    parity = bool(np.array_equal(counts, np.asarray(plain_counts)) and plain_cands == st["candidates"] and
+                  all(np.array_equal(sample[i]["pairs"], plain_sample[i]) for i in sample))
+    return {
+        "workload": f"{n_pairs} mesh pairs (seed 6), {ntri} triangles in total, N log-uniform in [500, 200000], "
+                    "icosphere / displaced sphere / torus, centre distance U[0.3, 1.5] radii, 2^24 grid",
+        "pairs": n_pairs, "triangles": ntri,
+        "batched_ms": step_s * 1e3,
+        "batched_mesh_pairs_per_s": n_pairs / step_s,
+        "batched_exact_tri_pairs_per_s": st["candidates"] / step_s,
+        "batched_stage_ms": {k: st[k] for k in ("ms_build", "ms_traverse", "ms_filter", "ms_exact", "ms_output")},
+        "plain_ms": plain_s * 1e3,
+        "plain_mesh_pairs_per_s": n_pairs / plain_s,
+        "speedup_vs_plain": plain_s / step_s,
+        "e2e_batched_ms": e2e_batch_s * 1e3,
+        "speedup_e2e": plain_s / e2e_batch_s,
+        "candidates": st["candidates"], "intersecting_pairs": st["pairs"], "segments": st["segments"],
+        "parity_with_plain": parity,
+        "generate_s": gen_s, "set_batch_s": set_s,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +258,9 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement")
+    ap.add_argument("--batch-pairs", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-qps", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -288,6 +367,8 @@ def main():
         torch.cuda.synchronize()
         print(f"copy calibration: {args.copy_calib} MiB read + {args.copy_calib} MiB written per dispatch",
               file=sys.stderr)
+    if not args.no_batch:
+        line["batched"] = batch_leg(gi, mg, args.batch_pairs, args.batch_steps)
     if not args.no_serve:
         line["serving"] = serving_leg(gi, mg, args.serve_requests, args.serve_qps, args.serve_slots)
     if not args.no_cpu_baseline:

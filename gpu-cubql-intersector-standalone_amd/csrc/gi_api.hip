@@ -70,8 +70,16 @@ struct MeshSlot {
     int sorted_in_tmp = 0;   // the sorted Morton keys live in (tkeys, tvals)
     bool chunk_timed = false;  // ev_chunk of this slot bracket its last k_build_chunks
     bool borrowed = false;     // device buffers belong to another context (gi_share_mesh)
+    // batch ("forest"): `forest` meshes stored back to back, indices global;
+    // mesh m owns vertices [h_voff[m], h_voff[m+1]) and triangles
+    // [h_toff[m], h_toff[m+1]); tiles = (mesh, first element) pairs
+    uint32_t forest = 0;
+    std::vector<uint32_t> h_voff, h_toff, h_vtiles, h_ttiles, h_vtile0;
     DevBuf src, tri, v64, vi, lo, hi, keys, vals, tkeys, tvals, nodes, range, blocks;
+    DevBuf voff, toff, vtiles, ttiles, vtile0, fbounds, roots;
     void release() {
+        DevBuf* forest_bufs[] = {&voff, &toff, &vtiles, &ttiles, &vtile0, &fbounds, &roots};
+        for (DevBuf* b : forest_bufs) b->release();
         DevBuf* all[] = {&src, &tri, &v64, &vi, &lo, &hi, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks};
         for (DevBuf* b : all) {
             if (borrowed) { b->p = nullptr; b->bytes = 0; }
@@ -106,13 +114,14 @@ struct gi_context {
     DevBuf bounds, counters, hist, scan_tmp, partials;
     DevBuf cand, exact, rec_key, rec_seg, seg_pool;
     uint32_t cap_cand = 0, cap_exact = 0, cap_rec = 0, cap_seg = 0;
-    DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2;
+    DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2, pair_off;
+    std::vector<uint32_t> h_pair_off;  // per mesh pair result offsets of the last batch query
     hipEvent_t ev[8] = {};
     hipEvent_t ev_chunk[4] = {};  // k_build_chunks begin/end for mesh A and B
     void release_device() {
         for (auto& m : mesh) m.release();
         DevBuf* all[] = {&bounds, &counters, &hist, &scan_tmp, &partials, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
-                         &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2};
+                         &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2, &pair_off};
         for (DevBuf* b : all) b->release();
         cap_cand = cap_exact = cap_rec = cap_seg = 0;
     }
@@ -138,9 +147,63 @@ void upload_mesh(gi_context* ctx, int slot) {
     uint32_t* tri = m.tri.ensure<uint32_t>(3 * (size_t)m.nt);
     if (m.nv) HIP_TRY(hipMemcpyAsync(src, m.h_xyz.data(), sizeof(double) * 3 * m.nv, hipMemcpyHostToDevice, ctx->stream));
     if (m.nt) HIP_TRY(hipMemcpyAsync(tri, m.h_tri.data(), sizeof(uint32_t) * 3 * m.nt, hipMemcpyHostToDevice, ctx->stream));
+    if (m.forest) {
+        auto put = [&](DevBuf& d, const std::vector<uint32_t>& h) {
+            uint32_t* p = d.ensure<uint32_t>(h.size());
+            if (!h.empty())
+                HIP_TRY(hipMemcpyAsync(p, h.data(), 4 * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        };
+        put(m.voff, m.h_voff);
+        put(m.toff, m.h_toff);
+        put(m.vtiles, m.h_vtiles);
+        put(m.ttiles, m.h_ttiles);
+        put(m.vtile0, m.h_vtile0);
+        // the host tables must outlive the asynchronous copies
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
     m.on_device = true;
     m.built = false;
     m.dirty = true;
+}
+
+// A batch: the same stages over all its meshes at once, one launch each.
+// Keys mesh << 30 | Morton: one sort groups by mesh, one Karras build makes
+// every mesh's BVH a subtree, whose roots are then looked up.
+void prepare_forest(gi_context* ctx, int slot) {
+    MeshSlot& m = ctx->mesh[slot];
+    hipStream_t s = ctx->stream;
+    uint32_t* grid = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2) + CNT_COUNT;
+    size_t nv = m.nv, nt = m.nt;
+    const uint32_t M = m.forest;
+    const uint32_t n_vtiles = (uint32_t)(m.h_vtiles.size() / 2), n_ttiles = (uint32_t)(m.h_ttiles.size() / 2);
+    double4* v64 = m.v64.ensure<double4>(nv);
+    int4* vi = m.vi.ensure<int4>(nv);
+    float4* lo = m.lo.ensure<float4>(nt);
+    float4* hi = m.hi.ensure<float4>(nt);
+    size_t nnodes = nt ? 2 * nt - 1 : 1;
+    float4* nodes = m.nodes.ensure<float4>(2 * nnodes);
+    uint2* range = m.range.ensure<uint2>(nt);
+    float4* blocks = m.blocks.ensure<float4>(block_levels_elems((uint32_t)nt));
+    uint64_t* keys = m.keys.ensure<uint64_t>(nt);
+    uint32_t* vals = m.vals.ensure<uint32_t>(nt);
+    uint64_t* tk = m.tkeys.ensure<uint64_t>(nt);
+    uint32_t* tv = m.tvals.ensure<uint32_t>(nt);
+    uint32_t* bounds = m.fbounds.ensure<uint32_t>(6 * (size_t)M);
+    float* partials = ctx->partials.ensure<float>(6 * (size_t)std::max(n_vtiles, kPrepareMaxBlocks));
+    launch_prepare_forest(m.src.get<double>(), m.vtiles.get<uint2>(), n_vtiles, m.voff.get<uint32_t>(),
+                          m.vtile0.get<uint32_t>(), M, v64, vi, grid + slot, partials, bounds, s);
+    launch_tri_boxes_forest(m.tri.get<uint32_t>(), m.ttiles.get<uint2>(), n_ttiles, m.toff.get<uint32_t>(), v64, vi,
+                            grid + slot, bounds, lo, hi, keys, vals, s);
+    uint32_t* hist = ctx->hist.ensure<uint32_t>(sort_hist_elems(nt));
+    uint32_t* stmp = ctx->scan_tmp.ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
+    m.sorted_in_tmp = radix_sort_u64(keys, vals, tk, tv, m.nt, 30 + bits_for(M), hist, stmp, s);
+    launch_build_bvh(m.sorted_in_tmp ? tk : keys, m.sorted_in_tmp ? tv : vals, m.nt, lo, hi, nodes, range, blocks, s,
+                     ctx->ev_chunk[2 * slot], ctx->ev_chunk[2 * slot + 1]);
+    launch_forest_roots(range, nodes, m.nt, m.toff.get<uint32_t>(), M, m.roots.ensure<uint2>(M), s);
+    m.chunk_timed = m.nt > 0;
+    HIP_TRY(hipGetLastError());
+    m.built = true;
+    m.dirty = false;
 }
 
 // vertices -> boxes -> (Morton sort + Karras build | leaf refresh) -> refit
@@ -149,6 +212,10 @@ void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
     if (m.borrowed) return;  // built and kept current by its owner
     if (!m.on_device) upload_mesh(ctx, slot);
     if (!m.dirty && m.built && !rebuild) return;
+    if (m.forest) {
+        prepare_forest(ctx, slot);
+        return;
+    }
     hipStream_t s = ctx->stream;
     uint32_t* grid = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2) + CNT_COUNT;
     uint32_t* bounds = ctx->bounds.ensure<uint32_t>(12) + 6 * slot;
@@ -241,6 +308,9 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     MeshSlot& MA = ctx->mesh[0];
     MeshSlot& MB = ctx->mesh[1];
     if (!MA.set || !MB.set) throw GiError{GI_ERR_NOT_READY, "both meshes must be set before gi_intersect"};
+    const uint32_t M = MA.forest;
+    if (MB.forest != M) throw GiError{GI_ERR_INVALID_ARGUMENT, "mesh slots hold different batches"};
+    if (M) ctx->h_pair_off.assign((size_t)M + 1, 0u);
     if (MA.nt == 0 || MB.nt == 0) {  // reference: empty mesh -> early return, no pairs
         empty_result(out, o.output_on_device);
         return GI_OK;
@@ -254,7 +324,11 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
 
     if (ctx->cap_cand == 0) {
-        uint64_t c = ctx->cand_reserve ? ctx->cand_reserve : std::max<uint64_t>(1u << 20, 8ull * (MA.nt + (uint64_t)MB.nt));
+        // first guess 8 candidates per triangle, at most 2^26 (the queues
+        // grow on overflow; a 2^26 guess is ~3 GB of arenas with segments)
+        uint64_t c = ctx->cand_reserve ? ctx->cand_reserve
+                                       : std::min<uint64_t>(std::max<uint64_t>(1u << 20, 8ull * (MA.nt + (uint64_t)MB.nt)),
+                                                            1ull << 26);
         c = std::min<uint64_t>(c, 0xFFFFFFF0ull);
         ensure_arenas(ctx, (uint32_t)c, (uint32_t)std::max<uint64_t>(c / 4, 1024), (uint32_t)std::max<uint64_t>(c / 4, 1024),
                       (uint32_t)std::max<uint64_t>(c / 4, 1024));
@@ -268,7 +342,9 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     for (int attempt = 0;; ++attempt) {
         launch_fill_u32(counters, 0u, CNT_COUNT, s);
         HIP_TRY(hipEventRecord(ctx->ev[6], s));
-        launch_traverse(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, counters, s);
+        const uint64_t* keysB = M ? (MB.sorted_in_tmp ? MB.tkeys.get<uint64_t>() : MB.keys.get<uint64_t>()) : nullptr;
+        launch_traverse(A, B, keysB, M ? MA.roots.get<uint2>() : nullptr, ctx->cand.get<uint2>(), ctx->cap_cand,
+                        counters, s);
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
         launch_filter(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, grid, ctx->exact.get<uint2>(), ctx->cap_exact,
                       counters, s);
@@ -335,6 +411,12 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         exclusive_scan_u32(nseg, offs, R, stmp, offs + R, s);
         if (o.want_segments)
             launch_gather_segments(order, ctx->rec_seg.get<uint2>(), offs, R, ctx->seg_pool.get<__int128>(), segs, f64, s);
+        if (M) {
+            uint32_t* poff = ctx->pair_off.ensure<uint32_t>((size_t)M + 1);
+            launch_forest_localize(pairs, R, MA.toff.get<uint32_t>(), MB.toff.get<uint32_t>(), M, poff, s);
+            HIP_TRY(hipMemcpyAsync(ctx->h_pair_off.data(), poff, 4 * ((size_t)M + 1), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));  // pageable destination
+        }
         HIP_TRY(hipEventRecord(ctx->ev[5], s));
         HIP_TRY(hipGetLastError());
         st.pairs = R;
@@ -391,6 +473,15 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     int nth = ctx->host_threads > 0 ? ctx->host_threads : (int)std::max(1u, std::thread::hardware_concurrency());
     if (host_exact_batch(vA.data(), MA.h_tri.data(), vB.data(), MB.h_tri.data(), list.data(), n, nth, hits, &yes) != 0)
         throw GiError{GI_ERR_INTERNAL, "host residual stage failed (overflow or unsound filter)"};
+    if (M) {  // batch: per mesh pair offsets, per-mesh indices
+        size_t m = 0;
+        for (size_t i = 0; i < hits.size(); ++i) {
+            while (m + 1 < M && MA.h_toff[m + 1] <= hits[i].a) ctx->h_pair_off[++m] = (uint32_t)i;
+            hits[i].a -= MA.h_toff[m];
+            hits[i].b -= MB.h_toff[m];
+        }
+        while (m + 1 <= M) ctx->h_pair_off[++m] = (uint32_t)hits.size();
+    }
     HostResultStore* hs = new HostResultStore();
     int limbs = 1;
     for (auto& h : hits) limbs = std::max(limbs, h.limbs);
@@ -590,6 +681,8 @@ int gi_set_mesh(gi_context* ctx, int slot, const double* xyz, uint32_t nv, const
     try {
         HIP_TRY(hipSetDevice(ctx->device));
         MeshSlot& m = ctx->mesh[slot];
+        if (m.borrowed) m.release();
+        m.forest = 0;
         m.h_xyz.assign(xyz, xyz + 3 * (size_t)nv);
         m.h_tri.assign(tri, tri + 3 * (size_t)nt);
         m.nv = nv;
@@ -611,6 +704,8 @@ int gi_set_mesh_device(gi_context* ctx, int slot, const double* d_xyz, uint32_t 
     try {
         HIP_TRY(hipSetDevice(ctx->device));
         MeshSlot& m = ctx->mesh[slot];
+        if (m.borrowed) m.release();
+        m.forest = 0;
         m.nv = nv;
         m.nt = nt;
         m.set = true;
@@ -645,6 +740,7 @@ int gi_set_transform(gi_context* ctx, int slot, const double rot_deg[3], const d
     if (!ctx || slot < 0 || slot > 1 || !rot_deg || !trans || !center) return GI_ERR_INVALID_ARGUMENT;
     if (!ctx->mesh[slot].set) return GI_ERR_NOT_READY;
     if (ctx->mesh[slot].borrowed) return GI_ERR_INVALID_ARGUMENT;  // a shared mesh is read-only
+    if (ctx->mesh[slot].forest) return GI_ERR_INVALID_ARGUMENT;    // batches take placed meshes
     Xform x = rigid_xform(rot_deg, trans, center);
     x.snap = snap_to_grid ? 1 : 0;
     ctx->mesh[slot].xf = x;
@@ -656,6 +752,7 @@ int gi_set_transform_matrix(gi_context* ctx, int slot, const double m[12], int s
     if (!ctx || slot < 0 || slot > 1 || !m) return GI_ERR_INVALID_ARGUMENT;
     if (!ctx->mesh[slot].set) return GI_ERR_NOT_READY;
     if (ctx->mesh[slot].borrowed) return GI_ERR_INVALID_ARGUMENT;  // a shared mesh is read-only
+    if (ctx->mesh[slot].forest) return GI_ERR_INVALID_ARGUMENT;    // batches take placed meshes
     Xform x = identity_xform();
     bool ident = true;
     for (int r = 0; r < 3; ++r)
@@ -686,6 +783,10 @@ int gi_build(gi_context* ctx) {
 
 int gi_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     if (!ctx || !out) return GI_ERR_INVALID_ARGUMENT;
+    if (ctx->mesh[0].forest || ctx->mesh[1].forest) {
+        ctx->err = "the context holds a batch: use gi_intersect_batch";
+        return GI_ERR_INVALID_ARGUMENT;
+    }
     try {
         HIP_TRY(hipSetDevice(ctx->device));
         return run_intersect(ctx, opt, out);
@@ -722,6 +823,7 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         return GI_ERR_INVALID_ARGUMENT;
     if (dst->device != src->device) return GI_ERR_INVALID_ARGUMENT;
     if (!src->mesh[src_slot].set) return GI_ERR_NOT_READY;
+    if (src->mesh[src_slot].forest) return GI_ERR_INVALID_ARGUMENT;
     try {
         HIP_TRY(hipSetDevice(src->device));
         prepare_mesh(src, src_slot, false);
@@ -755,6 +857,116 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         return GI_OK;
     } catch (const GiError& e) {
         return fail(dst, e);
+    }
+}
+
+namespace {
+
+// One side of a batch -> the slot's concatenated host arrays and tile tables.
+void fill_forest(MeshSlot& m, const gi_mesh_pair* pairs, uint32_t n, bool side_b) {
+    uint64_t nv = 0, nt = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        nv += side_b ? pairs[i].nvB : pairs[i].nvA;
+        nt += side_b ? pairs[i].ntB : pairs[i].ntA;
+    }
+    m.release();
+    m.forest = n;
+    m.nv = (uint32_t)nv;
+    m.nt = (uint32_t)nt;
+    m.h_xyz.resize(3 * nv);
+    m.h_tri.resize(3 * nt);
+    m.h_voff.assign((size_t)n + 1, 0u);
+    m.h_toff.assign((size_t)n + 1, 0u);
+    m.h_vtiles.clear();
+    m.h_ttiles.clear();
+    m.h_vtile0.assign((size_t)n + 1, 0u);
+    uint32_t vo = 0, to = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* xyz = side_b ? pairs[i].xyzB : pairs[i].xyzA;
+        const uint32_t* tri = side_b ? pairs[i].triB : pairs[i].triA;
+        uint32_t mv = side_b ? pairs[i].nvB : pairs[i].nvA, mt = side_b ? pairs[i].ntB : pairs[i].ntA;
+        if (mv) memcpy(&m.h_xyz[3 * (size_t)vo], xyz, 24 * (size_t)mv);
+        for (size_t k = 0; k < 3 * (size_t)mt; ++k) m.h_tri[3 * (size_t)to + k] = tri[k] + vo;
+        m.h_vtile0[i] = (uint32_t)(m.h_vtiles.size() / 2);
+        for (uint32_t f = 0; f < mv; f += kForestTile) {
+            m.h_vtiles.push_back(i);
+            m.h_vtiles.push_back(vo + f);
+        }
+        for (uint32_t f = 0; f < mt; f += kForestTile) {
+            m.h_ttiles.push_back(i);
+            m.h_ttiles.push_back(to + f);
+        }
+        vo += mv;
+        to += mt;
+        m.h_voff[i + 1] = vo;
+        m.h_toff[i + 1] = to;
+    }
+    m.h_vtile0[n] = (uint32_t)(m.h_vtiles.size() / 2);
+    m.set = true;
+    m.on_device = false;
+    m.built = false;
+    m.dirty = true;
+    m.xf = identity_xform();
+}
+
+}  // namespace
+
+int gi_set_batch(gi_context* ctx, const gi_mesh_pair* pairs, uint32_t n_pairs) {
+    if (!ctx || !pairs || n_pairs == 0) return GI_ERR_INVALID_ARGUMENT;
+    uint64_t nv[2] = {0, 0}, nt[2] = {0, 0};
+    for (uint32_t i = 0; i < n_pairs; ++i) {
+        const gi_mesh_pair& p = pairs[i];
+        const double* xyz[2] = {p.xyzA, p.xyzB};
+        const uint32_t* tri[2] = {p.triA, p.triB};
+        uint32_t mv[2] = {p.nvA, p.nvB}, mt[2] = {p.ntA, p.ntB};
+        for (int sd = 0; sd < 2; ++sd) {
+            if ((mv[sd] && !xyz[sd]) || (mt[sd] && !tri[sd])) {
+                ctx->err = "mesh pair " + std::to_string(i) + ": null array";
+                return GI_ERR_INVALID_ARGUMENT;
+            }
+            for (size_t k = 0; k < 3 * (size_t)mt[sd]; ++k)
+                if (tri[sd][k] >= mv[sd]) {
+                    ctx->err = "mesh pair " + std::to_string(i) + ": triangle index out of range";
+                    return GI_ERR_INVALID_ARGUMENT;
+                }
+            nv[sd] += mv[sd];
+            nt[sd] += mt[sd];
+        }
+    }
+    // 2^30 triangles per side: node indices and the status bits of the
+    // candidate lists; 2^32 vertices: global indices
+    for (int sd = 0; sd < 2; ++sd)
+        if (nt[sd] >= (1ull << 30) || nv[sd] >= (1ull << 32)) return GI_ERR_TOO_LARGE;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (int sd = 0; sd < 2; ++sd) {
+            if (ctx->mesh[sd].borrowed) ctx->mesh[sd].release();
+            fill_forest(ctx->mesh[sd], pairs, n_pairs, sd == 1);
+            upload_mesh(ctx, sd);
+        }
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return GI_OK;
+    } catch (const GiError& e) {
+        for (auto& m : ctx->mesh) m.set = false;
+        return fail(ctx, e);
+    }
+}
+
+int gi_intersect_batch(gi_context* ctx, const gi_options* opt, gi_result* out, uint64_t* pair_offsets) {
+    if (!ctx || !out) return GI_ERR_INVALID_ARGUMENT;
+    if (!ctx->mesh[0].forest || ctx->mesh[0].forest != ctx->mesh[1].forest) {
+        ctx->err = "no batch set (gi_set_batch)";
+        return GI_ERR_NOT_READY;
+    }
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        int st = run_intersect(ctx, opt, out);
+        if (pair_offsets)
+            for (size_t i = 0; i < ctx->h_pair_off.size(); ++i) pair_offsets[i] = ctx->h_pair_off[i];
+        return st;
+    } catch (const GiError& e) {
+        return fail(ctx, e);
     }
 }
 

@@ -58,6 +58,20 @@ void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, cons
                       const uint32_t* bounds6, float4* lo, float4* hi, uint32_t* keys, uint32_t* vals,
                       hipStream_t s);
 
+// Forest (a batch of meshes stored back to back, indices global): tiles of
+// kForestTile vertices/triangles, each inside one mesh, tile = (mesh, first).
+// mesh_vtile0[m]..mesh_vtile0[m+1] are mesh m's vertex tiles; partials needs
+// 6 floats per vertex tile; bounds 6 keys per mesh. Vertices are used as
+// given (no transform).
+constexpr uint32_t kForestTile = 1024;
+void launch_prepare_forest(const double* src, const uint2* vtiles, uint32_t n_vtiles, const uint32_t* voff,
+                           const uint32_t* mesh_vtile0, uint32_t M, double4* v64, int4* vi, uint32_t* grid_flag,
+                           float* partials, uint32_t* bounds, hipStream_t s);
+// boxes and keys mesh << 30 | Morton code (sort on 30 + bits(M) bits)
+void launch_tri_boxes_forest(const uint32_t* tri, const uint2* ttiles, uint32_t n_ttiles, const uint32_t* toff,
+                             const double4* v64, const int4* vi, const uint32_t* grid_flag, const uint32_t* bounds,
+                             float4* lo, float4* hi, uint64_t* keys, uint32_t* vals, hipStream_t s);
+
 // LSD radix sort of (key, value) pairs, 8 bits per pass. tmp_* are ping-pong
 // buffers of n elements; hist needs sort_hist_elems(n) entries, scan_tmp
 // scan_tmp_elems(sort_hist_elems(n)). Returns 1 when the sorted data ends in
@@ -88,8 +102,18 @@ size_t block_levels_elems(uint32_t n);
 void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
                       const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s,
                       hipEvent_t chunk_begin, hipEvent_t chunk_end);
+// the same over a forest's 64-bit keys: one tree whose mesh subtrees are the
+// per-mesh BVHs
+void launch_build_bvh(const uint64_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
+                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s,
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end);
+// roots[m] = (subtree root of mesh m, rope leaving it)
+void launch_forest_roots(const uint2* range, const float4* nodes, uint32_t n, const uint32_t* toff, uint32_t M,
+                         uint2* roots, hipStream_t s);
 
-void launch_traverse(MeshDev A, MeshDev B, uint2* cand, uint32_t cap, uint32_t* counters, hipStream_t s);
+// keysB/rootsA null: whole trees; else forests (sorted B keys, A mesh roots)
+void launch_traverse(MeshDev A, MeshDev B, const uint64_t* keysB, const uint2* rootsA, uint2* cand, uint32_t cap,
+                     uint32_t* counters, hipStream_t s);
 void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const uint32_t* grid_flags,
                    uint2* exact_list, uint32_t exact_cap, uint32_t* counters, hipStream_t s);
 void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
@@ -101,5 +125,10 @@ void launch_gather_pairs(const uint64_t* sorted_key, const uint32_t* order, cons
                          int bitsB, uint2* pairs, uint32_t* nseg, hipStream_t s);
 void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const uint32_t* seg_offsets, uint32_t n,
                             const __int128* seg_pool, __int128* seg_out, double* seg_f64, hipStream_t s);
+
+// forest results: pair_off[m] (M+1 entries) delimits mesh pair m's sorted
+// pairs, whose indices become per-mesh
+void launch_forest_localize(uint2* pairs, uint32_t R, const uint32_t* toffA, const uint32_t* toffB, uint32_t M,
+                            uint32_t* pair_off, hipStream_t s);
 
 }  // namespace gi

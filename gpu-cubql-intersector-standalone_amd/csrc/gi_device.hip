@@ -131,6 +131,29 @@ __global__ __launch_bounds__(256) void k_reduce_bounds(const float* __restrict__
     if (threadIdx.x < 6) bounds6[threadIdx.x] = f2key(fin[threadIdx.x]);
 }
 
+// One vertex: optional transform and rounding, the double and integer
+// copies, the grid test and the running FP32 bounds (Morton frame only, so
+// they need not be rounded outward).
+__device__ __forceinline__ bool prepare_vertex(const double* __restrict__ src, uint32_t i, const Xform& xf,
+                                               double4* __restrict__ v64, int4* __restrict__ vi, float* l, float* h) {
+    double x = src[3 * (size_t)i], y = src[3 * (size_t)i + 1], z = src[3 * (size_t)i + 2];
+    if (!xf.identity) {
+        double X = xform_coord(xf, 0, x, y, z);
+        double Y = xform_coord(xf, 1, x, y, z);
+        double Z = xform_coord(xf, 2, x, y, z);
+        x = X; y = Y; z = Z;
+    }
+    if (xf.snap) { x = rint(x); y = rint(y); z = rint(z); }
+    v64[i] = make_double4(x, y, z, 0.0);
+    const double L = 67108864.0;  // 2^26
+    bool g = x == rint(x) && y == rint(y) && z == rint(z) && fabs(x) <= L && fabs(y) <= L && fabs(z) <= L;
+    vi[i] = g ? make_int4((int)x, (int)y, (int)z, 0) : make_int4(0, 0, 0, 0);
+    float fx = (float)x, fy = (float)y, fz = (float)z;
+    l[0] = fminf(l[0], fx); l[1] = fminf(l[1], fy); l[2] = fminf(l[2], fz);
+    h[0] = fmaxf(h[0], fx); h[1] = fmaxf(h[1], fy); h[2] = fmaxf(h[2], fz);
+    return g;
+}
+
 // Moves the vertices (when a transform is set), classifies the mesh as grid
 // or not, writes the integer copy and per-block partial bounds (Morton frame).
 __global__ __launch_bounds__(256) void k_prepare_vertices(const double* __restrict__ src, uint32_t nv, Xform xf,
@@ -138,25 +161,8 @@ __global__ __launch_bounds__(256) void k_prepare_vertices(const double* __restri
                                                           uint32_t* grid_flag, float* __restrict__ part) {
     bool grid = true;
     float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
-        double x = src[3 * (size_t)i], y = src[3 * (size_t)i + 1], z = src[3 * (size_t)i + 2];
-        if (!xf.identity) {
-            double X = xform_coord(xf, 0, x, y, z);
-            double Y = xform_coord(xf, 1, x, y, z);
-            double Z = xform_coord(xf, 2, x, y, z);
-            x = X; y = Y; z = Z;
-        }
-        if (xf.snap) { x = rint(x); y = rint(y); z = rint(z); }
-        v64[i] = make_double4(x, y, z, 0.0);
-        const double L = 67108864.0;  // 2^26
-        bool g = x == rint(x) && y == rint(y) && z == rint(z) && fabs(x) <= L && fabs(y) <= L && fabs(z) <= L;
-        grid = grid && g;
-        vi[i] = g ? make_int4((int)x, (int)y, (int)z, 0) : make_int4(0, 0, 0, 0);
-        // the Morton frame only needs approximate bounds
-        float fx = (float)x, fy = (float)y, fz = (float)z;
-        l[0] = fminf(l[0], fx); l[1] = fminf(l[1], fy); l[2] = fminf(l[2], fz);
-        h[0] = fmaxf(h[0], fx); h[1] = fmaxf(h[1], fy); h[2] = fmaxf(h[2], fz);
-    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x)
+        grid = prepare_vertex(src, i, xf, v64, vi, l, h) && grid;
     uint64_t bad = __ballot(!grid);
     if (bad && lane_id() == 0) atomicAnd(grid_flag, 0u);
     block_box_partial(l, h, part + 6 * (size_t)blockIdx.x);
@@ -193,21 +199,13 @@ __device__ __forceinline__ uint32_t morton30(const float* bl, const float* bh, c
     return (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
 }
 
-// Per triangle: FP32 box rounded outward from the exact vertex coordinates
-// (so it contains the exact box), and, when keys != nullptr, the Morton key of
-// its centroid with the triangle index as value. Grid meshes gather 16-byte
-// integer vertices instead of 32-byte doubles.
-__global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ tri, uint32_t nt,
-                                                   const double4* __restrict__ v, const int4* __restrict__ vi,
-                                                   const uint32_t* __restrict__ grid_flag,
-                                                   const uint32_t* __restrict__ bounds6, float4* __restrict__ lo,
-                                                   float4* __restrict__ hi, uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nt) return;
-    const bool grid = *grid_flag != 0;
+// FP32 box of triangle i rounded outward from the exact vertex coordinates
+// (so it contains the exact box). Grid meshes gather 16-byte integer
+// vertices instead of 32-byte doubles.
+__device__ __forceinline__ void tri_box(const uint32_t* __restrict__ tri, uint32_t i, bool grid,
+                                        const double4* __restrict__ v, const int4* __restrict__ vi, float* bl,
+                                        float* bh) {
     uint32_t i0 = tri[3 * (size_t)i], i1 = tri[3 * (size_t)i + 1], i2 = tri[3 * (size_t)i + 2];
-    float bl[3], bh[3];
     if (grid) {
         int4 a = vi[i0], b = vi[i1], c = vi[i2];
         bl[0] = f32_down((double)min(min(a.x, b.x), c.x));
@@ -225,14 +223,32 @@ __global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ 
         bh[1] = f32_up(fmax(fmax(a.y, b.y), c.y));
         bh[2] = f32_up(fmax(fmax(a.z, b.z), c.z));
     }
+}
+
+__device__ __forceinline__ void morton_frame(const uint32_t* __restrict__ bounds6, float* bmin, float* bext) {
+    for (int k = 0; k < 3; ++k) {
+        bmin[k] = key2f(bounds6[k]);
+        bext[k] = key2f(bounds6[3 + k]) - bmin[k];
+    }
+}
+
+// Per triangle: its box, and, when keys != nullptr, the Morton key of its
+// centroid with the triangle index as value.
+__global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ tri, uint32_t nt,
+                                                   const double4* __restrict__ v, const int4* __restrict__ vi,
+                                                   const uint32_t* __restrict__ grid_flag,
+                                                   const uint32_t* __restrict__ bounds6, float4* __restrict__ lo,
+                                                   float4* __restrict__ hi, uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    float bl[3], bh[3];
+    tri_box(tri, i, *grid_flag != 0, v, vi, bl, bh);
     lo[i] = make_float4(bl[0], bl[1], bl[2], 0.f);
     hi[i] = make_float4(bh[0], bh[1], bh[2], 0.f);
     if (keys) {
         float bmin[3], bext[3];
-        for (int k = 0; k < 3; ++k) {
-            bmin[k] = key2f(bounds6[k]);
-            bext[k] = key2f(bounds6[3 + k]) - bmin[k];
-        }
+        morton_frame(bounds6, bmin, bext);
         keys[i] = morton30(bl, bh, bmin, bext);
         vals[i] = i;
     }
@@ -243,6 +259,106 @@ void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, cons
                       hipStream_t s) {
     if (nt == 0) return;
     k_tri_boxes<<<blocks_for(nt, 256), 256, 0, s>>>(tri, nt, v64, vi, grid_flag, bounds6, lo, hi, keys, vals);
+    GI_LAUNCH_CHECK();
+}
+
+__device__ __forceinline__ void box_absorb(const float4* __restrict__ e, size_t idx, float* b) {
+    float4 l = e[2 * idx], h = e[2 * idx + 1];
+    b[0] = fminf(b[0], l.x); b[1] = fminf(b[1], l.y); b[2] = fminf(b[2], l.z);
+    b[3] = fmaxf(b[3], h.x); b[4] = fmaxf(b[4], h.y); b[5] = fmaxf(b[5], h.z);
+}
+
+__device__ __forceinline__ void wave_box_reduce(float* b) {
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int k = 0; k < 3; ++k) b[k] = fminf(b[k], __shfl_xor(b[k], o));
+        for (int k = 3; k < 6; ++k) b[k] = fmaxf(b[k], __shfl_xor(b[k], o));
+    }
+}
+
+// ------------------------------------------------------ forests (batches)
+//
+// A batch stores its meshes back to back (mesh m's vertices at
+// [voff[m], voff[m+1]), triangles at [toff[m], toff[m+1]), indices global).
+// Work is cut into tiles of kForestTile elements that never straddle two
+// meshes: tile t = (mesh, first element). Per-mesh results (bounds) come from
+// per-tile partials, so no same-address atomics are needed.
+
+__global__ __launch_bounds__(256) void k_prepare_vertices_forest(const double* __restrict__ src,
+                                                                 const uint2* __restrict__ tiles,
+                                                                 const uint32_t* __restrict__ voff,
+                                                                 double4* __restrict__ v64, int4* __restrict__ vi,
+                                                                 uint32_t* grid_flag, float* __restrict__ part) {
+    const uint2 t = tiles[blockIdx.x];
+    const uint32_t end = min(t.y + kForestTile, voff[t.x + 1]);
+    Xform id;
+    id.identity = 1;
+    id.snap = 0;
+    bool grid = true;
+    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = t.y + threadIdx.x; i < end; i += blockDim.x) grid = prepare_vertex(src, i, id, v64, vi, l, h) && grid;
+    uint64_t bad = __ballot(!grid);
+    if (bad && lane_id() == 0) atomicAnd(grid_flag, 0u);
+    block_box_partial(l, h, part + 6 * (size_t)blockIdx.x);
+}
+
+// one wave per mesh: union of its tiles' partial boxes -> 6 keys per mesh
+__global__ __launch_bounds__(256) void k_forest_bounds(const float* __restrict__ part,
+                                                       const uint32_t* __restrict__ mesh_tile0, uint32_t M,
+                                                       uint32_t* __restrict__ bounds) {
+    uint32_t m = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (m >= M) return;
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t t = mesh_tile0[m] + lane_id(); t < mesh_tile0[m + 1]; t += 64)
+        for (int k = 0; k < 3; ++k) {
+            b[k] = fminf(b[k], part[6 * (size_t)t + k]);
+            b[3 + k] = fmaxf(b[3 + k], part[6 * (size_t)t + 3 + k]);
+        }
+    wave_box_reduce(b);
+    if (lane_id() == 0)
+        for (int k = 0; k < 6; ++k) bounds[6 * (size_t)m + k] = f2key(b[k]);
+}
+
+// Triangle boxes and 64-bit keys (mesh id << 30 | Morton code in the mesh's
+// own frame): one radix sort then groups the batch by mesh and orders each
+// mesh along its Morton curve.
+__global__ __launch_bounds__(256) void k_tri_boxes_forest(const uint32_t* __restrict__ tri,
+                                                          const uint2* __restrict__ tiles,
+                                                          const uint32_t* __restrict__ toff,
+                                                          const double4* __restrict__ v, const int4* __restrict__ vi,
+                                                          const uint32_t* __restrict__ grid_flag,
+                                                          const uint32_t* __restrict__ bounds,
+                                                          float4* __restrict__ lo, float4* __restrict__ hi,
+                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint2 t = tiles[blockIdx.x];
+    const uint32_t end = min(t.y + kForestTile, toff[t.x + 1]);
+    const bool grid = *grid_flag != 0;
+    float bmin[3], bext[3];
+    morton_frame(bounds + 6 * (size_t)t.x, bmin, bext);
+    for (uint32_t i = t.y + threadIdx.x; i < end; i += blockDim.x) {
+        float bl[3], bh[3];
+        tri_box(tri, i, grid, v, vi, bl, bh);
+        lo[i] = make_float4(bl[0], bl[1], bl[2], 0.f);
+        hi[i] = make_float4(bh[0], bh[1], bh[2], 0.f);
+        keys[i] = ((uint64_t)t.x << 30) | morton30(bl, bh, bmin, bext);
+        vals[i] = i;
+    }
+}
+
+void launch_prepare_forest(const double* src, const uint2* vtiles, uint32_t n_vtiles, const uint32_t* voff,
+                           const uint32_t* mesh_vtile0, uint32_t M, double4* v64, int4* vi, uint32_t* grid_flag,
+                           float* partials, uint32_t* bounds, hipStream_t s) {
+    launch_fill_u32(grid_flag, 1u, 1, s);
+    if (n_vtiles == 0 || M == 0) return;
+    k_prepare_vertices_forest<<<n_vtiles, 256, 0, s>>>(src, vtiles, voff, v64, vi, grid_flag, partials);
+    k_forest_bounds<<<blocks_for(M, 4), 256, 0, s>>>(partials, mesh_vtile0, M, bounds);
+    GI_LAUNCH_CHECK();
+}
+
+void launch_tri_boxes_forest(const uint32_t* tri, const uint2* ttiles, uint32_t n_ttiles, const uint32_t* toff,
+                             const double4* v64, const int4* vi, const uint32_t* grid_flag, const uint32_t* bounds,
+                             float4* lo, float4* hi, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+    if (n_ttiles == 0) return;
+    k_tri_boxes_forest<<<n_ttiles, 256, 0, s>>>(tri, ttiles, toff, v64, vi, grid_flag, bounds, lo, hi, keys, vals);
     GI_LAUNCH_CHECK();
 }
 
@@ -596,36 +712,46 @@ int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t*
 // A refit after a rigid motion reruns the same kernels on the same sorted
 // keys: the topology comes out identical, only the boxes change.
 
-// Sorted Morton keys, the chunk's slice read from LDS.
+// Sorted keys (30-bit Morton codes, or mesh id << 30 | Morton code for a
+// forest), the chunk's slice read from LDS.
+template <class K>
 struct KeySrc {
-    const uint32_t* __restrict__ g;
-    const uint32_t* s;
+    const K* __restrict__ g;
+    const K* s;
     int cs, ce, n;
-    __device__ __forceinline__ uint32_t at(int j) const { return (j >= cs && j < ce) ? s[j - cs] : g[j]; }
+    __device__ __forceinline__ K at(int j) const { return (j >= cs && j < ce) ? s[j - cs] : g[j]; }
 };
 
-// common-prefix length of the composite keys (morton, index)
-__device__ __forceinline__ int delta(const KeySrc& k, int i, int j) {
+__device__ __forceinline__ int clz_key(uint32_t x) { return __clz(x); }
+__device__ __forceinline__ int clz_key(uint64_t x) { return __clzll((long long)x); }
+
+// common-prefix length of the composite keys (key, index). In a forest the
+// mesh id bits lead, so the leaves of one mesh share a longer prefix with
+// each other than with any other leaf and form one subtree.
+template <class K>
+__device__ __forceinline__ int delta(const KeySrc<K>& k, int i, int j) {
     if (j < 0 || j >= k.n) return -1;
-    uint32_t a = k.at(i), b = k.at(j);
-    if (a != b) return __clz(a ^ b);
-    return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+    K a = k.at(i), b = k.at(j);
+    if (a != b) return clz_key(a ^ b);
+    return 8 * (int)sizeof(K) + __clz((uint32_t)i ^ (uint32_t)j);
 }
 
 // The node visited right after every subtree whose last leaf is m - 1: the
 // right child that starts at leaf m. In Karras' numbering a right child
 // covering [m, x] is internal node m, which is the case iff internal node m
 // extends to the right; otherwise it is leaf m.
-__device__ __forceinline__ uint32_t chain_top(const KeySrc& k, int m) {
+template <class K>
+__device__ __forceinline__ uint32_t chain_top(const KeySrc<K>& k, int m) {
     if (m < k.n - 1 && delta(k, m, m + 1) > delta(k, m, m - 1)) return (uint32_t)m;
     return (uint32_t)(k.n - 1 + m);
 }
 
-__global__ __launch_bounds__(256) void k_build_chunks(const uint32_t* __restrict__ keys,
-                                                      const uint32_t* __restrict__ prims, int n,
-                                                      const float4* __restrict__ lo, const float4* __restrict__ hi,
-                                                      float4* __restrict__ nodes, uint2* __restrict__ range) {
-    __shared__ uint32_t skey[kBuildChunk];
+template <class K>
+__global__ __launch_bounds__(256) void k_build_chunks(const K* __restrict__ keys, const uint32_t* __restrict__ prims,
+                                                      int n, const float4* __restrict__ lo,
+                                                      const float4* __restrict__ hi, float4* __restrict__ nodes,
+                                                      uint2* __restrict__ range) {
+    __shared__ K skey[kBuildChunk];
     __shared__ uint32_t par_leaf[kBuildChunk];
     __shared__ uint32_t par_int[kBuildChunk];
     __shared__ uint32_t left_int[kBuildChunk];
@@ -643,28 +769,28 @@ __global__ __launch_bounds__(256) void k_build_chunks(const uint32_t* __restrict
         ubox[k][3] = ubox[k][4] = ubox[k][5] = 0u;
     }
     __syncthreads();
-    const KeySrc K{keys, skey, cs, ce, n};
+    const KeySrc<K> KS{keys, skey, cs, ce, n};
     const int iend = min(ce, n - 1);
     for (int i = cs + threadIdx.x; i < iend; i += blockDim.x) {
-        int d = delta(K, i, i + 1) > delta(K, i, i - 1) ? 1 : -1;
-        int dmin = delta(K, i, i - d);
+        int d = delta(KS, i, i + 1) > delta(KS, i, i - 1) ? 1 : -1;
+        int dmin = delta(KS, i, i - d);
         int lmax = 2;
-        while (delta(K, i, i + lmax * d) > dmin) lmax <<= 1;
+        while (delta(KS, i, i + lmax * d) > dmin) lmax <<= 1;
         int l = 0;
         for (int t = lmax >> 1; t >= 1; t >>= 1)
-            if (delta(K, i, i + (l + t) * d) > dmin) l += t;
+            if (delta(KS, i, i + (l + t) * d) > dmin) l += t;
         int j = i + l * d;
-        int dnode = delta(K, i, j);
+        int dnode = delta(KS, i, j);
         int s = 0, t = l;
         do {
             t = (t + 1) >> 1;
-            if (delta(K, i, i + (s + t) * d) > dnode) s += t;
+            if (delta(KS, i, i + (s + t) * d) > dnode) s += t;
         } while (t > 1);
         int gamma = i + s * d + (d < 0 ? -1 : 0);
         int lo_i = i < j ? i : j, hi_i = i < j ? j : i;
         uint32_t left = lo_i == gamma ? (uint32_t)(n - 1 + gamma) : (uint32_t)gamma;
         uint32_t right = hi_i == gamma + 1 ? (uint32_t)(n - 1 + gamma + 1) : (uint32_t)(gamma + 1);
-        uint32_t rope = hi_i == n - 1 ? kSentinel : chain_top(K, hi_i + 1);
+        uint32_t rope = hi_i == n - 1 ? kSentinel : chain_top(KS, hi_i + 1);
         range[i] = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
         left_int[i - cs] = left;
         rope_int[i - cs] = rope;
@@ -687,7 +813,7 @@ __global__ __launch_bounds__(256) void k_build_chunks(const uint32_t* __restrict
         if (jl >= ce) continue;
         uint32_t prim = prims[jl];
         float4 l = lo[prim], h = hi[prim];
-        uint32_t rope = jl == n - 1 ? kSentinel : chain_top(K, jl + 1);
+        uint32_t rope = jl == n - 1 ? kSentinel : chain_top(KS, jl + 1);
         nodes[2 * (size_t)(n - 1 + jl)] = make_float4(l.x, l.y, l.z, __uint_as_float(prim));
         nodes[2 * (size_t)(n - 1 + jl) + 1] = make_float4(h.x, h.y, h.z, __uint_as_float(rope));
         lb[u][0] = l.x; lb[u][1] = l.y; lb[u][2] = l.z;
@@ -712,19 +838,6 @@ __global__ __launch_bounds__(256) void k_build_chunks(const uint32_t* __restrict
             nodes[2 * (size_t)p + 1] = make_float4(b[3], b[4], b[5], __uint_as_float(rope_int[slot]));
             p = par_int[slot];
         }
-    }
-}
-
-__device__ __forceinline__ void box_absorb(const float4* __restrict__ e, size_t idx, float* b) {
-    float4 l = e[2 * idx], h = e[2 * idx + 1];
-    b[0] = fminf(b[0], l.x); b[1] = fminf(b[1], l.y); b[2] = fminf(b[2], l.z);
-    b[3] = fmaxf(b[3], h.x); b[4] = fmaxf(b[4], h.y); b[5] = fmaxf(b[5], h.z);
-}
-
-__device__ __forceinline__ void wave_box_reduce(float* b) {
-    for (int o = 32; o > 0; o >>= 1) {
-        for (int k = 0; k < 3; ++k) b[k] = fminf(b[k], __shfl_xor(b[k], o));
-        for (int k = 3; k < 6; ++k) b[k] = fmaxf(b[k], __shfl_xor(b[k], o));
     }
 }
 
@@ -803,12 +916,13 @@ size_t block_levels_elems(uint32_t n) {
     return total;
 }
 
-void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
+template <class K>
+static void build_bvh_impl(const K* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
                       const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s,
                       hipEvent_t chunk_begin, hipEvent_t chunk_end) {
     if (n == 0) return;
     if (chunk_begin) (void)hipEventRecord(chunk_begin, s);
-    k_build_chunks<<<blocks_for(n, kBuildChunk), 256, 0, s>>>(sorted_keys, sorted_prims, (int)n, lo, hi, nodes, range);
+    k_build_chunks<K><<<blocks_for(n, kBuildChunk), 256, 0, s>>>(sorted_keys, sorted_prims, (int)n, lo, hi, nodes, range);
     if (chunk_end) (void)hipEventRecord(chunk_end, s);
     if (n < 2) return;
     BlockLevels lv;
@@ -824,6 +938,47 @@ void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims,
         k_block_boxes<<<blocks_for(n_out, 4), 256, 0, s>>>(lv.level[k - 1], n_in, (float4*)lv.level[k], n_out);
     }
     k_top_boxes<<<blocks_for(blocks_for(n - 1, 64), 4), 256, 0, s>>>(n, range, nodes, lv);
+    GI_LAUNCH_CHECK();
+}
+
+void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
+                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s,
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end) {
+    build_bvh_impl<uint32_t>(sorted_keys, sorted_prims, n, lo, hi, nodes, range, blocks, s, chunk_begin, chunk_end);
+}
+void launch_build_bvh(const uint64_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
+                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s,
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end) {
+    build_bvh_impl<uint64_t>(sorted_keys, sorted_prims, n, lo, hi, nodes, range, blocks, s, chunk_begin, chunk_end);
+}
+
+// Subtree of each forest mesh: the node whose leaf range is exactly the
+// mesh's [toff[m], toff[m+1]) (in Karras' numbering internal node first or
+// internal node last; a one-triangle mesh is its leaf), and the rope that
+// leaves it, which ends a traversal of that mesh. Empty mesh: both sentinel.
+__global__ void k_forest_roots(const uint2* __restrict__ range, const float4* __restrict__ nodes, uint32_t n,
+                               const uint32_t* __restrict__ toff, uint32_t M, uint2* __restrict__ roots) {
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    uint32_t f = toff[m], e = toff[m + 1];
+    if (e == f) {
+        roots[m] = make_uint2(kSentinel, kSentinel);
+        return;
+    }
+    uint32_t l = e - 1, root;
+    if (e - f == 1) {
+        root = n - 1 + f;
+    } else {
+        uint2 r = range[f];
+        root = (r.x == f && r.y == l) ? f : l;
+    }
+    roots[m] = make_uint2(root, __float_as_uint(nodes[2 * (size_t)root + 1].w));
+}
+
+void launch_forest_roots(const uint2* range, const float4* nodes, uint32_t n, const uint32_t* toff, uint32_t M,
+                         uint2* roots, hipStream_t s) {
+    if (M == 0) return;
+    k_forest_roots<<<blocks_for(M, 256), 256, 0, s>>>(range, nodes, n, toff, M, roots);
     GI_LAUNCH_CHECK();
 }
 
@@ -849,8 +1004,12 @@ __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t fill, uint2* 
 // cluster), stackless traversal of A along the ropes. Overlapping leaf pairs
 // are compacted per wave (ballot + popcount) into LDS and flushed to the
 // global candidate queue with one atomic per TRAV_STAGE pairs.
+// Forest (batch): leaf k of B belongs to mesh keysB[k] >> 30 and walks only
+// the subtree of the same mesh of A, from its root to the rope leaving it.
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restrict__ nodesA, uint32_t nA,
                                                            const float4* __restrict__ nodesB, uint32_t nB,
+                                                           const uint64_t* __restrict__ keysB,
+                                                           const uint2* __restrict__ rootsA,
                                                            uint2* __restrict__ cand, uint32_t cap,
                                                            uint32_t* __restrict__ counters) {
     __shared__ uint2 stage_all[TRAV_THREADS / 64][TRAV_STAGE];
@@ -858,7 +1017,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
     uint32_t k = blockIdx.x * TRAV_THREADS + threadIdx.x;
     float qlx = 0.f, qly = 0.f, qlz = 0.f, qhx = 0.f, qhy = 0.f, qhz = 0.f;
     uint32_t b = 0;
-    uint32_t node = kSentinel;
+    uint32_t node = kSentinel, end = kSentinel;
     if (k < nB) {
         size_t leaf = (size_t)(nB - 1 + k);
         float4 L = nodesB[2 * leaf], H = nodesB[2 * leaf + 1];
@@ -866,14 +1025,21 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
         qhx = H.x; qhy = H.y; qhz = H.z;
         b = __float_as_uint(L.w);
         node = 0;
+        if (rootsA) {
+            uint2 r = rootsA[keysB[k] >> 30];
+            node = r.x;
+            end = r.y;
+        }
     }
     const uint32_t leafBase = nA - 1;
     uint64_t lt = lanemask_lt();
     uint32_t fill = 0;
-    while (__any(node != kSentinel)) {
+    if (node == end) node = end = kSentinel;
+    // a walk ends at its end rope; the rope chain itself ends in kSentinel
+    while (__any(node != end && node != kSentinel)) {
         bool hit = false;
         uint32_t a = 0;
-        if (node != kSentinel) {
+        if (node != end && node != kSentinel) {
             float4 L = nodesA[2 * (size_t)node], H = nodesA[2 * (size_t)node + 1];
             bool ov = L.x <= qhx && qlx <= H.x && L.y <= qhy && qly <= H.y && L.z <= qhz && qlz <= H.z;
             uint32_t w0 = __float_as_uint(L.w), w1 = __float_as_uint(H.w);
@@ -894,10 +1060,11 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
     if (fill) flush_stage(stage, fill, cand, cap, &counters[CNT_TRAVERSAL_HITS]);
 }
 
-void launch_traverse(MeshDev A, MeshDev B, uint2* cand, uint32_t cap, uint32_t* counters, hipStream_t s) {
+void launch_traverse(MeshDev A, MeshDev B, const uint64_t* keysB, const uint2* rootsA, uint2* cand, uint32_t cap,
+                     uint32_t* counters, hipStream_t s) {
     if (A.nt == 0 || B.nt == 0) return;
-    k_traverse<<<blocks_for(B.nt, TRAV_THREADS), TRAV_THREADS, 0, s>>>(A.nodes, A.nt, B.nodes, B.nt, cand, cap,
-                                                                       counters);
+    k_traverse<<<blocks_for(B.nt, TRAV_THREADS), TRAV_THREADS, 0, s>>>(A.nodes, A.nt, B.nodes, B.nt, keysB, rootsA,
+                                                                       cand, cap, counters);
     GI_LAUNCH_CHECK();
 }
 
@@ -1147,6 +1314,47 @@ void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const u
                             const __int128* seg_pool, __int128* seg_out, double* seg_f64, hipStream_t s) {
     if (n == 0) return;
     k_gather_segments<<<blocks_for(n, 256), 256, 0, s>>>(order, rec_seg, seg_offsets, n, seg_pool, seg_out, seg_f64);
+    GI_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------ forest results
+//
+// The sorted pairs of a forest are ordered by global A index, so mesh m's
+// pairs start at the first pair whose A triangle is at or after toffA[m].
+
+__global__ void k_pair_offsets(const uint2* __restrict__ pairs, uint32_t R, const uint32_t* __restrict__ toffA,
+                               uint32_t M, uint32_t* __restrict__ pair_off) {
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > M) return;
+    uint32_t key = toffA[m], lo = 0, hi = R;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (pairs[mid].x < key) lo = mid + 1;
+        else hi = mid;
+    }
+    pair_off[m] = m == M ? R : lo;
+}
+
+// global -> per-mesh triangle indices: pair i belongs to the last mesh m
+// with pair_off[m] <= i
+__global__ void k_localize(uint2* __restrict__ pairs, uint32_t R, const uint32_t* __restrict__ pair_off, uint32_t M,
+                           const uint32_t* __restrict__ toffA, const uint32_t* __restrict__ toffB) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    uint32_t lo = 0, hi = M;  // invariant: pair_off[lo] <= i < pair_off[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (pair_off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    uint2 p = pairs[i];
+    pairs[i] = make_uint2(p.x - toffA[lo], p.y - toffB[lo]);
+}
+
+void launch_forest_localize(uint2* pairs, uint32_t R, const uint32_t* toffA, const uint32_t* toffB, uint32_t M,
+                            uint32_t* pair_off, hipStream_t s) {
+    k_pair_offsets<<<blocks_for((size_t)M + 1, 256), 256, 0, s>>>(pairs, R, toffA, M, pair_off);
+    if (R) k_localize<<<blocks_for(R, 256), 256, 0, s>>>(pairs, R, pair_off, M, toffA, toffB);
     GI_LAUNCH_CHECK();
 }
 

@@ -54,6 +54,12 @@ class GiResult(C.Structure):
                 ("seg_f64", C.POINTER(C.c_double)), ("stats", GiStats), ("internal", C.c_void_p)]
 
 
+class GiMeshPair(C.Structure):
+    _fields_ = [("xyzA", C.POINTER(C.c_double)), ("triA", C.POINTER(C.c_uint32)), ("nvA", C.c_uint32),
+                ("ntA", C.c_uint32), ("xyzB", C.POINTER(C.c_double)), ("triB", C.POINTER(C.c_uint32)),
+                ("nvB", C.c_uint32), ("ntB", C.c_uint32)]
+
+
 class GiServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("snap_to_grid", C.c_int32),
                 ("want_segments", C.c_int32), ("hbm_reserve_bytes", C.c_uint64)]
@@ -75,7 +81,8 @@ _lib = None
 
 SYMBOLS = ["gi_create", "gi_destroy", "gi_last_error", "gi_status_string", "gi_set_mesh", "gi_set_mesh_device",
            "gi_set_transform", "gi_set_transform_matrix", "gi_build", "gi_intersect", "gi_result_free",
-           "gi_intersect_oneshot", "gi_share_mesh", "gi_clear_device", "gi_reconstruct_device",
+           "gi_intersect_oneshot", "gi_set_batch", "gi_intersect_batch", "gi_share_mesh", "gi_clear_device",
+           "gi_reconstruct_device",
            "gi_server_create", "gi_server_add_mesh", "gi_server_submit", "gi_server_wait", "gi_server_destroy",
            "gi_exact_pair_host"]
 
@@ -107,6 +114,8 @@ def lib():
         ip = C.POINTER(C.c_int32)
         L.gi_exact_pair_host.argtypes = [dp, dp, ip, ip, ip, ip, ip, C.POINTER(C.c_uint64), C.c_int32]
         L.gi_share_mesh.argtypes = [vp, C.c_int, vp, C.c_int]
+        L.gi_set_batch.argtypes = [vp, C.POINTER(GiMeshPair), C.c_uint32]
+        L.gi_intersect_batch.argtypes = [vp, C.POINTER(GiOptions), C.POINTER(GiResult), C.POINTER(C.c_uint64)]
         L.gi_server_create.argtypes = [C.POINTER(GiServerConfig), dp, C.c_uint32, up, C.c_uint32, C.POINTER(vp)]
         L.gi_server_add_mesh.argtypes = [vp, dp, C.c_uint32, up, C.c_uint32, ip]
         L.gi_server_submit.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_uint64)]
@@ -213,6 +222,48 @@ class Intersector:
             return result_to_py(res, want_segments, decode)
         finally:
             lib().gi_result_free(C.byref(res))
+
+    def set_batch(self, pairs):
+        """pairs: [((xyzA, triA), (xyzB, triB)), ...] host arrays."""
+        arr = (GiMeshPair * len(pairs))()
+        keep = []
+        for i, ((xa, ta), (xb, tb)) in enumerate(pairs):
+            xa = np.ascontiguousarray(xa, dtype=np.float64).reshape(-1, 3)
+            ta = np.ascontiguousarray(ta, dtype=np.uint32).reshape(-1, 3)
+            xb = np.ascontiguousarray(xb, dtype=np.float64).reshape(-1, 3)
+            tb = np.ascontiguousarray(tb, dtype=np.uint32).reshape(-1, 3)
+            keep += [xa, ta, xb, tb]
+            arr[i] = GiMeshPair(xa.ctypes.data_as(C.POINTER(C.c_double)), ta.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                xa.shape[0], ta.shape[0], xb.ctypes.data_as(C.POINTER(C.c_double)),
+                                tb.ctypes.data_as(C.POINTER(C.c_uint32)), xb.shape[0], tb.shape[0])
+        _check(lib().gi_set_batch(self._ctx, arr, len(pairs)), self._ctx)
+        self._n_batch = len(pairs)
+
+    def intersect_batch_raw(self, output_on_device=False, want_segments=True, rebuild=False):
+        """(GiResult, pair_offsets); the caller must gi_result_free the struct."""
+        res = GiResult()
+        offs = np.zeros(self._n_batch + 1, dtype=np.uint64)
+        opt = GiOptions(int(output_on_device), int(want_segments), int(rebuild), 0)
+        _check(lib().gi_intersect_batch(self._ctx, C.byref(opt), C.byref(res),
+                                        offs.ctypes.data_as(C.POINTER(C.c_uint64))), self._ctx)
+        return res, offs
+
+    def intersect_batch(self, want_segments=True, rebuild=False, decode=True):
+        """Per mesh pair results: a list of dicts like intersect()'s, plus the
+        whole-batch stats."""
+        res, offs = self.intersect_batch_raw(False, want_segments, rebuild)
+        try:
+            whole = result_to_py(res, want_segments, decode)
+        finally:
+            lib().gi_result_free(C.byref(res))
+        out = []
+        for i in range(self._n_batch):
+            lo, hi = int(offs[i]), int(offs[i + 1])
+            r = {"pairs": whole["pairs"][lo:hi]}
+            if want_segments and decode:
+                r["segments"] = whole["segments"][lo:hi]
+            out.append(r)
+        return out, whole["stats"]
 
 
 def result_to_py(res: GiResult, want_segments=True, decode=True):

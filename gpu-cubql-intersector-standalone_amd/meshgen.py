@@ -124,8 +124,8 @@ def uv_sphere(nu: int, nv: int, seed: int, amp: float = 0.05):
     st, ct = np.sin(th), np.cos(th)
     ring = np.stack([np.outer(st, np.cos(ph)), np.outer(st, np.sin(ph)), np.outer(ct, np.ones(nu))], axis=-1)
     pts = np.concatenate([[[0.0, 0.0, 1.0]], ring.reshape(-1, 3), [[0.0, 0.0, -1.0]]], axis=0)
-    r = 1.0 + amp * value_noise3(pts, seed)
-    pts = pts * r[:, None]
+    if amp != 0.0:
+        pts = pts * (1.0 + amp * value_noise3(pts, seed))[:, None]
     top, bot = 0, pts.shape[0] - 1
 
     def vid(band, k):  # vertex on latitude line `band` (0 = north pole, nv = south pole)
@@ -284,3 +284,99 @@ def serving_workload(n_requests: int = 400, qps: float = 200.0, seed: int = 7, s
         else:
             reqs.append((float(arrivals[i]), 0, affine_about(R, c, t[i] * GRID_SCALE)))
     return dict(static=(snap(sp), st), meshes=[(snap(tp), tt), (snap(0.1 * pp), pt)], requests=reqs)
+
+
+# ---------------------------------------------------------------- batches
+
+
+def geodesic_icosphere(freq: int):
+    """Icosahedron with every face cut into freq^2 triangles (20 * freq^2),
+    points projected to the unit sphere. A point (u A + v B + w C) / freq,
+    u + v + w = freq, is evaluated left to right; on a shared edge the zero
+    weight's term is exactly 0, so both faces produce the same bits and the
+    vertices are merged exactly."""
+    t = (1.0 + np.sqrt(5.0)) / 2.0
+    V = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+                  (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], dtype=np.float64)
+    F = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+                  (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+                  (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)])
+    f = int(freq)
+    ii, jj = np.meshgrid(np.arange(f + 1), np.arange(f + 1), indexing="ij")
+    keep = ii + jj <= f
+    v_w = jj[keep].astype(np.float64)             # weight of C
+    u_w = (f - ii[keep] - jj[keep]).astype(np.float64)  # weight of A
+    b_w = ii[keep].astype(np.float64)             # weight of B
+    A, B, Cc = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    P = ((u_w[None, :, None] * A[:, None, :] + b_w[None, :, None] * B[:, None, :]) + v_w[None, :, None] * Cc[:, None, :])
+    P = P / float(f)
+    P = P / np.sqrt((P * P).sum(axis=2, keepdims=True))
+    # local index of (i, j) within a face's point list
+    lid = -np.ones((f + 1, f + 1), dtype=np.int64)
+    lid[keep] = np.arange(keep.sum())
+    tris = []
+    for i in range(f):
+        for j in range(f - i):
+            tris.append((lid[i, j], lid[i + 1, j], lid[i, j + 1]))
+            if i + j + 1 < f:
+                tris.append((lid[i + 1, j], lid[i + 1, j + 1], lid[i, j + 1]))
+    tl = np.array(tris, dtype=np.int64)
+    npf = int(keep.sum())
+    allp = P.reshape(-1, 3)
+    allt = (tl[None, :, :] + (np.arange(20) * npf)[:, None, None]).reshape(-1, 3)
+    # merge bit-identical points: unique over a 64-bit mix of the coordinate
+    # bits, then check that every merged point really is identical
+    bits = np.ascontiguousarray(allp).view(np.uint64).reshape(-1, 3)
+    h = bits[:, 0] * np.uint64(0x9E3779B97F4A7C15)
+    h = (h ^ (h >> np.uint64(29))) + bits[:, 1] * np.uint64(0xBF58476D1CE4E5B9)
+    h = (h ^ (h >> np.uint64(31))) + bits[:, 2] * np.uint64(0x94D049BB133111EB)
+    _, first, inv = np.unique(h, return_index=True, return_inverse=True)
+    uniq = allp[first]
+    assert np.array_equal(uniq[inv.reshape(-1)], allp), "hash collision between distinct points"
+    return uniq, inv.reshape(-1)[allt].astype(np.uint32)
+
+
+def _batch_mesh(kind: int, n: int, seed: int):
+    """kind 0 icosphere (20 f^2), 1 displaced UV sphere (2 nu nv), 2 torus
+    (2 nu nv, R = 0.8, r = 0.3); size nearest to n triangles."""
+    if kind == 0:
+        return geodesic_icosphere(max(1, int(round(np.sqrt(n / 20.0)))))
+    if kind == 1:
+        nv = max(2, int(round(np.sqrt(n / 4.0))))
+        nu = max(3, int(round(n / (2.0 * nv))))
+        p, t = uv_sphere(nu, nv, seed, amp=0.0)
+        ph = seed * 0.618
+        r = 1.0 + 0.05 * np.sin(4.0 * p[:, 0] + ph) * np.sin(4.0 * p[:, 1] + 2 * ph) * np.sin(4.0 * p[:, 2] + 3 * ph)
+        return p * r[:, None], t
+    nv = max(3, int(round(np.sqrt(n / (2.0 * 0.8 / 0.3)))))
+    nu = max(3, int(round(n / (2.0 * nv))))
+    return torus(nu, nv)
+
+
+def batch_workload(n_pairs: int = 4096, seed: int = 6, nmin: int = 500, nmax: int = 200000):
+    """Independent mesh pairs: each side an icosphere, displaced sphere or
+    torus (uniform), N log-uniform in [nmin, nmax], rotated uniformly on SO(3)
+    (quaternion of 4 normals); B's centre at distance U[0.3, 1.5] (radius 1)
+    from A's in a uniform direction; both snapped to the 2^22-per-unit grid.
+    Returns a list of ((xyzA, triA), (xyzB, triB))."""
+    kinds = (uniform01(seed, 2 * n_pairs, 0) * 3).astype(np.int64).reshape(n_pairs, 2)
+    sizes = np.exp(np.log(nmin) + uniform01(seed, 2 * n_pairs, 1) * np.log(nmax / nmin)).reshape(n_pairs, 2)
+    q = np.stack([gaussian(seed, 2 * n_pairs, 2 + k) for k in range(4)], axis=1).reshape(n_pairs, 2, 4)
+    d = np.stack([gaussian(seed, n_pairs, 6 + k) for k in range(3)], axis=1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    dist = 0.3 + 1.2 * uniform01(seed, n_pairs, 30)
+    cache = {}
+    out = []
+    for i in range(n_pairs):
+        sides = []
+        for s in range(2):
+            key = (int(kinds[i, s]), int(round(sizes[i, s])))
+            if key not in cache:
+                cache[key] = _batch_mesh(key[0], key[1], 17 + key[1] % 97)
+            p, t = cache[key]
+            p = p @ quat_matrix(q[i, s]).T
+            if s == 1:
+                p = p + d[i] * dist[i]
+            sides.append((snap(p), t))
+        out.append((sides[0], sides[1]))
+    return out

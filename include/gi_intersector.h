@@ -145,6 +145,33 @@ int gi_intersect_oneshot(const gi_config* cfg, const double* xyzA, uint32_t nvA,
                          const double* xyzB, uint32_t nvB, const uint32_t* triB, uint32_t ntB,
                          const gi_options* opt, gi_result* out);
 
+/* ------------------------------------------------------------- batches
+ * Many independent mesh pairs in one pass: every stage (vertex preparation,
+ * boxes, Morton sort, BVH build, traversal, filter, exact stage, result sort)
+ * runs once over the whole batch instead of once per pair. Replaces the
+ * reference's per-pair loop of its batch driver
+ * (apps/archive/main_massive.cpp:237-304, one runGridForestIntersectionPipeline
+ * per pair) and the batchMultiplier chunk loop
+ * (src/single_tree/SingleTreeBatchIntersector.cu:66-73, 170-455).
+ * Meshes are taken as placed (no transform); host arrays, copied. */
+typedef struct {
+    const double* xyzA;     /* 3 * nvA */
+    const uint32_t* triA;   /* 3 * ntA, indices into this mesh's vertices */
+    uint32_t nvA, ntA;
+    const double* xyzB;
+    const uint32_t* triB;
+    uint32_t nvB, ntB;
+} gi_mesh_pair;
+
+/* Replaces both mesh slots with the batch (at most 2^30 - 1 triangles per
+ * side in total). gi_intersect then refuses the context; use: */
+int gi_set_batch(gi_context* ctx, const gi_mesh_pair* pairs, uint32_t n_pairs);
+/* One result for the whole batch: the pairs of mesh pair i are
+ * [pair_offsets[i], pair_offsets[i+1]) (n_pairs + 1 entries, caller-owned),
+ * with triangle indices local to that pair's meshes, sorted, and their
+ * segments as in gi_intersect. */
+int gi_intersect_batch(gi_context* ctx, const gi_options* opt, gi_result* out, uint64_t* pair_offsets);
+
 /* Makes dst's slot a read-only view of src's (built) slot: no copy; src must
  * outlive dst's use of it. Used to serve many queries against one resident
  * BVH (no reference counterpart: the reference keeps one controller per pair). */
