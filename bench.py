@@ -261,6 +261,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement")
     ap.add_argument("--batch-pairs", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched measurement (for kernel profiles of it)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-qps", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -278,6 +280,9 @@ def main():
 
     gi = ge.package_module("gi_ctypes")
     mg = ge.package_module("meshgen")
+    if args.batch_only:
+        print(json.dumps({"batched": batch_leg(gi, mg, args.batch_pairs, args.batch_steps)}), flush=True)
+        return
     (xa, ta), (xb, tb) = mg.sphere_torus(args.res, args.res // 2)
     dev = torch.device("cuda:0")
     dxa = torch.from_numpy(np.ascontiguousarray(xa)).to(dev)
