@@ -611,6 +611,7 @@ const char* gi_status_string(int status) {
         case GI_ERR_NOT_READY: return "mesh not set";
         case GI_ERR_INTERNAL: return "internal consistency check failed";
         case GI_ERR_TOO_LARGE: return "mesh too large";
+        case GI_ERR_IO: return "file cannot be read, parsed or written";
         default: return "unknown status";
     }
 }

@@ -60,6 +60,14 @@ class GiMeshPair(C.Structure):
                 ("nvB", C.c_uint32), ("ntB", C.c_uint32)]
 
 
+class GiOffMesh(C.Structure):
+    _fields_ = [("soup_points", C.c_uint64), ("soup_polygons", C.c_uint64), ("extra_fan_triangles", C.c_uint64),
+                ("n_dropped_degenerate", C.c_uint64), ("n_dropped_duplicate", C.c_uint64), ("nv", C.c_uint32),
+                ("nt", C.c_uint32), ("xyz", C.POINTER(C.c_double)), ("tri", C.POINTER(C.c_uint32)),
+                ("face_to_row", C.POINTER(C.c_uint32)), ("dropped_degenerate", C.POINTER(C.c_uint32)),
+                ("dropped_duplicate", C.POINTER(C.c_uint32))]
+
+
 class GiServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("snap_to_grid", C.c_int32),
                 ("want_segments", C.c_int32), ("hbm_reserve_bytes", C.c_uint64)]
@@ -84,7 +92,8 @@ SYMBOLS = ["gi_create", "gi_destroy", "gi_last_error", "gi_status_string", "gi_s
            "gi_intersect_oneshot", "gi_set_batch", "gi_intersect_batch", "gi_share_mesh", "gi_clear_device",
            "gi_reconstruct_device",
            "gi_server_create", "gi_server_add_mesh", "gi_server_submit", "gi_server_wait", "gi_server_destroy",
-           "gi_exact_pair_host"]
+           "gi_exact_pair_host", "gi_off_read", "gi_off_free", "gi_off_write", "gi_pairs_csv_write",
+           "gi_pairs_diff_csv_write", "gi_segments_csv_write", "gi_segments_obj_write", "gi_meshio_last_error"]
 
 
 def lib():
@@ -114,6 +123,17 @@ def lib():
         ip = C.POINTER(C.c_int32)
         L.gi_exact_pair_host.argtypes = [dp, dp, ip, ip, ip, ip, ip, C.POINTER(C.c_uint64), C.c_int32]
         L.gi_share_mesh.argtypes = [vp, C.c_int, vp, C.c_int]
+        cp = C.c_char_p
+        L.gi_off_read.argtypes = [cp, C.POINTER(GiOffMesh), C.POINTER(vp)]
+        L.gi_off_free.argtypes = [vp]
+        L.gi_off_write.argtypes = [cp, dp, C.c_uint32, up, C.c_uint32]
+        L.gi_pairs_csv_write.argtypes = [cp, up, C.c_uint64, dp, C.c_uint32, up, C.c_uint32, dp, C.c_uint32, up,
+                                         C.c_uint32]
+        L.gi_pairs_diff_csv_write.argtypes = [cp, up, C.c_uint64, C.c_uint64, up, C.c_uint64, C.c_uint64, cp, dp,
+                                              C.c_uint32, up, C.c_uint32, dp, C.c_uint32, up, C.c_uint32]
+        L.gi_segments_csv_write.argtypes = [cp, C.POINTER(GiResult)]
+        L.gi_segments_obj_write.argtypes = [cp, C.POINTER(GiResult)]
+        L.gi_meshio_last_error.restype = C.c_char_p
         L.gi_set_batch.argtypes = [vp, C.POINTER(GiMeshPair), C.c_uint32]
         L.gi_intersect_batch.argtypes = [vp, C.POINTER(GiOptions), C.POINTER(GiResult), C.POINTER(C.c_uint64)]
         L.gi_server_create.argtypes = [C.POINTER(GiServerConfig), dp, C.c_uint32, up, C.c_uint32, C.POINTER(vp)]
@@ -122,7 +142,7 @@ def lib():
         L.gi_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(GiResult), C.POINTER(GiRequestInfo)]
         L.gi_server_destroy.argtypes = [vp]
         for name in SYMBOLS:
-            if name not in ("gi_last_error", "gi_status_string"):
+            if name not in ("gi_last_error", "gi_status_string", "gi_meshio_last_error"):
                 getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -284,6 +304,74 @@ def result_to_py(res: GiResult, want_segments=True, decode=True):
     elif want_segments:
         out["segments"] = [[] for _ in range(n)]
     return out
+
+
+def _io_check(st):
+    if st != 0:
+        raise GiError(st, f"{lib().gi_status_string(st).decode()}: {lib().gi_meshio_last_error().decode()}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _up(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _mesh_arrays(xyz, tri):
+    return (np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3),
+            np.ascontiguousarray(tri, dtype=np.uint32).reshape(-1, 3))
+
+
+def read_off(path):
+    """OFF -> dict(xyz, tri, face_to_row, dropped_degenerate, dropped_duplicate,
+    soup_points, soup_polygons, extra_fan_triangles), with the canonical face
+    order of the reference's loader."""
+    m = GiOffMesh()
+    owner = C.c_void_p()
+    _io_check(lib().gi_off_read(os.fsencode(path), C.byref(m), C.byref(owner)))
+    try:
+        def arr(p, n, shape):
+            if n == 0:
+                return np.zeros(shape, dtype=np.float64 if p._type_ is C.c_double else np.uint32)
+            return np.ctypeslib.as_array(p, shape=(n,)).copy().reshape(shape)
+        return dict(xyz=arr(m.xyz, 3 * m.nv, (m.nv, 3)), tri=arr(m.tri, 3 * m.nt, (m.nt, 3)),
+                    face_to_row=arr(m.face_to_row, m.nt, (m.nt,)),
+                    dropped_degenerate=arr(m.dropped_degenerate, m.n_dropped_degenerate, (m.n_dropped_degenerate,)),
+                    dropped_duplicate=arr(m.dropped_duplicate, m.n_dropped_duplicate, (m.n_dropped_duplicate,)),
+                    soup_points=m.soup_points, soup_polygons=m.soup_polygons,
+                    extra_fan_triangles=m.extra_fan_triangles)
+    finally:
+        lib().gi_off_free(owner)
+
+
+def write_off(path, xyz, tri):
+    xyz, tri = _mesh_arrays(xyz, tri)
+    _io_check(lib().gi_off_write(os.fsencode(path), _dp(xyz), xyz.shape[0], _up(tri), tri.shape[0]))
+
+
+def write_pairs_csv(path, pairs, mesh_a, mesh_b):
+    """The reference's run CSV (FaceA, FaceB, vertex indices, coordinates)."""
+    pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+    (xa, ta), (xb, tb) = _mesh_arrays(*mesh_a), _mesh_arrays(*mesh_b)
+    _io_check(lib().gi_pairs_csv_write(os.fsencode(path), _up(pairs), pairs.shape[0], _dp(xa), xa.shape[0], _up(ta),
+                                       ta.shape[0], _dp(xb), xb.shape[0], _up(tb), tb.shape[0]))
+
+
+def write_pairs_diff_csv(path, pairs1, id1, pairs2, id2, filt, mesh_a, mesh_b):
+    p1 = np.ascontiguousarray(pairs1, dtype=np.uint32).reshape(-1, 2)
+    p2 = np.ascontiguousarray(pairs2, dtype=np.uint32).reshape(-1, 2)
+    (xa, ta), (xb, tb) = _mesh_arrays(*mesh_a), _mesh_arrays(*mesh_b)
+    _io_check(lib().gi_pairs_diff_csv_write(os.fsencode(path), _up(p1), p1.shape[0], id1, _up(p2), p2.shape[0], id2,
+                                            filt.encode(), _dp(xa), xa.shape[0], _up(ta), ta.shape[0], _dp(xb),
+                                            xb.shape[0], _up(tb), tb.shape[0]))
+
+
+def write_segments(res: GiResult, path, fmt="csv"):
+    """Exact segments of a host result: 'csv' (exact integers + doubles) or 'obj' (polyline)."""
+    fn = lib().gi_segments_csv_write if fmt == "csv" else lib().gi_segments_obj_write
+    _io_check(fn(os.fsencode(path), C.byref(res)))
 
 
 class Server:

@@ -59,7 +59,8 @@ typedef enum {
     GI_ERR_NO_DEVICE = 4,
     GI_ERR_NOT_READY = 5,   /* a mesh slot has not been set */
     GI_ERR_INTERNAL = 6,    /* an internal consistency check failed */
-    GI_ERR_TOO_LARGE = 7    /* more than 2^30 triangles in one mesh */
+    GI_ERR_TOO_LARGE = 7,   /* more than 2^30 triangles in one mesh */
+    GI_ERR_IO = 8           /* file cannot be read, parsed or written (gi_meshio.h) */
 } gi_status;
 
 typedef struct gi_context gi_context;
