@@ -75,6 +75,7 @@ struct MeshSlot {
     // [h_toff[m], h_toff[m+1]); tiles = (mesh, first element) pairs
     uint32_t forest = 0;
     std::vector<uint32_t> h_voff, h_toff, h_vtiles, h_ttiles, h_vtile0;
+    int morton_axis = 10;      // Morton bits per axis of the kept sorted keys
     DevBuf src, tri, v64, vi, lo, hi, keys, vals, tkeys, tvals, nodes, range, blocks;
     DevBuf voff, toff, vtiles, ttiles, vtile0, fbounds, roots;
     void release() {
@@ -107,6 +108,12 @@ int bits_for(uint32_t n) { return n <= 1 ? 1 : 32 - __builtin_clz(n - 1); }
 struct gi_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // mesh B's build runs on `side`, concurrently with mesh A's on `stream`
+    // (fork/join by events); build_stream[slot] is the stream of that slot
+    hipStream_t side = nullptr;
+    hipStream_t build_stream[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    DevBuf bhist[2], bscan[2], bpart[2];  // per-slot build scratch
     std::string err;
     int host_threads = 0;
     uint32_t cand_reserve = 0;
@@ -121,7 +128,8 @@ struct gi_context {
     void release_device() {
         for (auto& m : mesh) m.release();
         DevBuf* all[] = {&bounds, &counters, &hist, &scan_tmp, &partials, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
-                         &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2, &pair_off};
+                         &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2, &pair_off,
+                         &bhist[0], &bhist[1], &bscan[0], &bscan[1], &bpart[0], &bpart[1]};
         for (DevBuf* b : all) b->release();
         cap_cand = cap_exact = cap_rec = cap_seg = 0;
     }
@@ -141,17 +149,35 @@ int fail(gi_context* ctx, const GiError& e) {
     return e.status;
 }
 
+// side stream starts after everything queued so far on the main stream
+void fork_side(gi_context* ctx) {
+    HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+    HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+}
+
+// main stream continues after everything queued so far on the side stream
+void join_side(gi_context* ctx) {
+    HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
+    HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+}
+
+void sync_all(gi_context* ctx) {
+    HIP_TRY(hipStreamSynchronize(ctx->side));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+}
+
 void upload_mesh(gi_context* ctx, int slot) {
     MeshSlot& m = ctx->mesh[slot];
     double* src = m.src.ensure<double>(3 * (size_t)m.nv);
     uint32_t* tri = m.tri.ensure<uint32_t>(3 * (size_t)m.nt);
-    if (m.nv) HIP_TRY(hipMemcpyAsync(src, m.h_xyz.data(), sizeof(double) * 3 * m.nv, hipMemcpyHostToDevice, ctx->stream));
-    if (m.nt) HIP_TRY(hipMemcpyAsync(tri, m.h_tri.data(), sizeof(uint32_t) * 3 * m.nt, hipMemcpyHostToDevice, ctx->stream));
+    hipStream_t st = ctx->build_stream[slot];
+    if (m.nv) HIP_TRY(hipMemcpyAsync(src, m.h_xyz.data(), sizeof(double) * 3 * m.nv, hipMemcpyHostToDevice, st));
+    if (m.nt) HIP_TRY(hipMemcpyAsync(tri, m.h_tri.data(), sizeof(uint32_t) * 3 * m.nt, hipMemcpyHostToDevice, st));
     if (m.forest) {
         auto put = [&](DevBuf& d, const std::vector<uint32_t>& h) {
             uint32_t* p = d.ensure<uint32_t>(h.size());
             if (!h.empty())
-                HIP_TRY(hipMemcpyAsync(p, h.data(), 4 * h.size(), hipMemcpyHostToDevice, ctx->stream));
+                HIP_TRY(hipMemcpyAsync(p, h.data(), 4 * h.size(), hipMemcpyHostToDevice, st));
         };
         put(m.voff, m.h_voff);
         put(m.toff, m.h_toff);
@@ -159,7 +185,7 @@ void upload_mesh(gi_context* ctx, int slot) {
         put(m.ttiles, m.h_ttiles);
         put(m.vtile0, m.h_vtile0);
         // the host tables must outlive the asynchronous copies
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipStreamSynchronize(st));
     }
     m.on_device = true;
     m.built = false;
@@ -167,11 +193,11 @@ void upload_mesh(gi_context* ctx, int slot) {
 }
 
 // A batch: the same stages over all its meshes at once, one launch each.
-// Keys mesh << 30 | Morton: one sort groups by mesh, one Karras build makes
+// Keys mesh << 3 axis bits | Morton: one sort groups by mesh, one Karras build makes
 // every mesh's BVH a subtree, whose roots are then looked up.
 void prepare_forest(gi_context* ctx, int slot) {
     MeshSlot& m = ctx->mesh[slot];
-    hipStream_t s = ctx->stream;
+    hipStream_t s = ctx->build_stream[slot];
     uint32_t* grid = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2) + CNT_COUNT;
     size_t nv = m.nv, nt = m.nt;
     const uint32_t M = m.forest;
@@ -189,14 +215,17 @@ void prepare_forest(gi_context* ctx, int slot) {
     uint64_t* tk = m.tkeys.ensure<uint64_t>(nt);
     uint32_t* tv = m.tvals.ensure<uint32_t>(nt);
     uint32_t* bounds = m.fbounds.ensure<uint32_t>(6 * (size_t)M);
-    float* partials = ctx->partials.ensure<float>(6 * (size_t)std::max(n_vtiles, kPrepareMaxBlocks));
+    float* partials = ctx->bpart[slot].ensure<float>(6 * (size_t)std::max(n_vtiles, kPrepareMaxBlocks));
     launch_prepare_forest(m.src.get<double>(), m.vtiles.get<uint2>(), n_vtiles, m.voff.get<uint32_t>(),
                           m.vtile0.get<uint32_t>(), M, v64, vi, grid + slot, partials, bounds, s);
+    uint32_t largest = 0;
+    for (uint32_t i = 0; i < M; ++i) largest = std::max(largest, m.h_toff[i + 1] - m.h_toff[i]);
+    m.morton_axis = morton_axis_bits(largest);
     launch_tri_boxes_forest(m.tri.get<uint32_t>(), m.ttiles.get<uint2>(), n_ttiles, m.toff.get<uint32_t>(), v64, vi,
-                            grid + slot, bounds, lo, hi, keys, vals, s);
-    uint32_t* hist = ctx->hist.ensure<uint32_t>(sort_hist_elems(nt));
-    uint32_t* stmp = ctx->scan_tmp.ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
-    m.sorted_in_tmp = radix_sort_u64(keys, vals, tk, tv, m.nt, 30 + bits_for(M), hist, stmp, s);
+                            grid + slot, bounds, m.morton_axis, lo, hi, keys, vals, s);
+    uint32_t* hist = ctx->bhist[slot].ensure<uint32_t>(sort_hist_elems(nt));
+    uint32_t* stmp = ctx->bscan[slot].ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
+    m.sorted_in_tmp = radix_sort_u64(keys, vals, tk, tv, m.nt, 3 * m.morton_axis + bits_for(M), hist, stmp, s);
     launch_build_bvh(m.sorted_in_tmp ? tk : keys, m.sorted_in_tmp ? tv : vals, m.nt, lo, hi, nodes, range, blocks, s,
                      ctx->ev_chunk[2 * slot], ctx->ev_chunk[2 * slot + 1]);
     launch_forest_roots(range, nodes, m.nt, m.toff.get<uint32_t>(), M, m.roots.ensure<uint2>(M), s);
@@ -216,7 +245,7 @@ void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
         prepare_forest(ctx, slot);
         return;
     }
-    hipStream_t s = ctx->stream;
+    hipStream_t s = ctx->build_stream[slot];
     uint32_t* grid = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2) + CNT_COUNT;
     uint32_t* bounds = ctx->bounds.ensure<uint32_t>(12) + 6 * slot;
     size_t nv = m.nv, nt = m.nt;
@@ -234,14 +263,15 @@ void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
     uint32_t* tv = m.tvals.ensure<uint32_t>(nt);
     const bool topology = !m.built || rebuild;
     launch_init_mesh_scalars(grid + slot, bounds, s);
-    float* partials = ctx->partials.ensure<float>(6 * (size_t)kPrepareMaxBlocks);
+    float* partials = ctx->bpart[slot].ensure<float>(6 * (size_t)kPrepareMaxBlocks);
     launch_prepare_vertices(m.src.get<double>(), m.nv, m.xf, v64, vi, grid + slot, partials, bounds, s);
-    launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, vi, grid + slot, bounds, lo, hi, topology ? keys : nullptr,
-                     topology ? vals : nullptr, s);
+    if (topology) m.morton_axis = morton_axis_bits(m.nt);
+    launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, vi, grid + slot, bounds, m.morton_axis, lo, hi,
+                     topology ? keys : nullptr, topology ? vals : nullptr, s);
     if (topology) {
-        uint32_t* hist = ctx->hist.ensure<uint32_t>(sort_hist_elems(nt));
-        uint32_t* stmp = ctx->scan_tmp.ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
-        m.sorted_in_tmp = radix_sort_u32(keys, vals, tk, tv, m.nt, 30, hist, stmp, s);
+        uint32_t* hist = ctx->bhist[slot].ensure<uint32_t>(sort_hist_elems(nt));
+        uint32_t* stmp = ctx->bscan[slot].ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
+        m.sorted_in_tmp = radix_sort_u32(keys, vals, tk, tv, m.nt, 3 * m.morton_axis, hist, stmp, s);
         m.built = true;
     }
     // build, or refit on the kept sorted keys (same topology, new boxes)
@@ -319,8 +349,12 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     gi_stats& st = out->stats;
     HIP_TRY(hipEventRecord(ctx->ev[0], s));
     MA.chunk_timed = MB.chunk_timed = false;
+    ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);  // shared by both builds: allocate before the fork
+    ctx->bounds.ensure<uint32_t>(12);
+    fork_side(ctx);
     prepare_mesh(ctx, 0, o.rebuild != 0);
     prepare_mesh(ctx, 1, o.rebuild != 0);
+    join_side(ctx);
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
 
     if (ctx->cap_cand == 0) {
@@ -343,8 +377,8 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         launch_fill_u32(counters, 0u, CNT_COUNT, s);
         HIP_TRY(hipEventRecord(ctx->ev[6], s));
         const uint64_t* keysB = M ? (MB.sorted_in_tmp ? MB.tkeys.get<uint64_t>() : MB.keys.get<uint64_t>()) : nullptr;
-        launch_traverse(A, B, keysB, M ? MA.roots.get<uint2>() : nullptr, ctx->cand.get<uint2>(), ctx->cap_cand,
-                        counters, s);
+        launch_traverse(A, B, keysB, 3 * MB.morton_axis, M ? MA.roots.get<uint2>() : nullptr, ctx->cand.get<uint2>(),
+                        ctx->cap_cand, counters, s);
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
         launch_filter(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, grid, ctx->exact.get<uint2>(), ctx->cap_exact,
                       counters, s);
@@ -635,6 +669,11 @@ int gi_create(const gi_config* cfg, gi_context** out) {
         ctx->cand_reserve = cfg ? cfg->candidate_reserve : 0;
         HIP_TRY(hipSetDevice(ctx->device));
         HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+        ctx->build_stream[0] = ctx->stream;
+        ctx->build_stream[1] = ctx->side;
+        HIP_TRY(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
         for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
         for (auto& e : ctx->ev_chunk) HIP_TRY(hipEventCreate(&e));
         for (auto& m : ctx->mesh) m.xf = identity_xform();
@@ -650,12 +689,16 @@ int gi_create(const gi_config* cfg, gi_context** out) {
 int gi_destroy(gi_context* ctx) {
     if (!ctx) return GI_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->release_device();
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->ev_chunk)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return GI_OK;
@@ -773,9 +816,13 @@ int gi_build(gi_context* ctx) {
     if (!ctx) return GI_ERR_INVALID_ARGUMENT;
     try {
         HIP_TRY(hipSetDevice(ctx->device));
+        ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);
+        ctx->bounds.ensure<uint32_t>(12);
+        fork_side(ctx);
         for (int sl = 0; sl < 2; ++sl)
             if (ctx->mesh[sl].set) prepare_mesh(ctx, sl, false);
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        join_side(ctx);
+        sync_all(ctx);
         return GI_OK;
     } catch (const GiError& e) {
         return fail(ctx, e);
@@ -827,8 +874,12 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
     if (src->mesh[src_slot].forest) return GI_ERR_INVALID_ARGUMENT;
     try {
         HIP_TRY(hipSetDevice(src->device));
+        src->counters.ensure<uint32_t>(CNT_COUNT + 2);
+        src->bounds.ensure<uint32_t>(12);
+        fork_side(src);
         prepare_mesh(src, src_slot, false);
-        HIP_TRY(hipStreamSynchronize(src->stream));
+        join_side(src);
+        sync_all(src);
         MeshSlot& d = dst->mesh[dst_slot];
         const MeshSlot& s = src->mesh[src_slot];
         d.release();
@@ -842,6 +893,7 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         d.dirty = false;
         d.xf = s.xf;
         d.sorted_in_tmp = s.sorted_in_tmp;
+        d.morton_axis = s.morton_axis;
         d.borrowed = true;
         DevBuf* dd[] = {&d.src, &d.tri, &d.v64, &d.vi, &d.lo, &d.hi, &d.keys, &d.vals, &d.tkeys, &d.tvals, &d.nodes,
                         &d.range, &d.blocks};
@@ -940,13 +992,13 @@ int gi_set_batch(gi_context* ctx, const gi_mesh_pair* pairs, uint32_t n_pairs) {
         if (nt[sd] >= (1ull << 30) || nv[sd] >= (1ull << 32)) return GI_ERR_TOO_LARGE;
     try {
         HIP_TRY(hipSetDevice(ctx->device));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        sync_all(ctx);
         for (int sd = 0; sd < 2; ++sd) {
             if (ctx->mesh[sd].borrowed) ctx->mesh[sd].release();
             fill_forest(ctx->mesh[sd], pairs, n_pairs, sd == 1);
             upload_mesh(ctx, sd);
         }
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        sync_all(ctx);
         return GI_OK;
     } catch (const GiError& e) {
         for (auto& m : ctx->mesh) m.set = false;
@@ -974,6 +1026,7 @@ int gi_intersect_batch(gi_context* ctx, const gi_options* opt, gi_result* out, u
 int gi_clear_device(gi_context* ctx) {
     if (!ctx) return GI_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->side);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->release_device();
     return GI_OK;
@@ -988,11 +1041,15 @@ int gi_reconstruct_device(gi_context* ctx) {
         }
     try {
         HIP_TRY(hipSetDevice(ctx->device));
+        ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);
+        ctx->bounds.ensure<uint32_t>(12);
+        fork_side(ctx);
         for (int sl = 0; sl < 2; ++sl) {
             if (!ctx->mesh[sl].on_device) upload_mesh(ctx, sl);
             prepare_mesh(ctx, sl, true);
         }
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        join_side(ctx);
+        sync_all(ctx);
         return GI_OK;
     } catch (const GiError& e) {
         return fail(ctx, e);

@@ -9,6 +9,14 @@ namespace gi {
 
 constexpr uint32_t kSentinel = 0xFFFFFFFFu;
 
+// Morton precision of the BVH keys, bits per axis. Only the tree's quality
+// depends on it (equal keys are ordered by triangle index); 8 bits per axis
+// (24-bit keys, 3 sort passes) serve meshes up to 2^21 triangles as well as
+// 10 bits (30-bit keys, 4 passes) and sort a pass faster.
+inline int morton_axis_bits(uint32_t largest_mesh_triangles) {
+    return largest_mesh_triangles <= (1u << 21) ? 8 : 10;
+}
+
 // Pipeline counters (device, uint32 each), cleared by launch_fill at the
 // start of every intersection.
 enum Counter {
@@ -55,7 +63,7 @@ void launch_prepare_vertices(const double* src, uint32_t nv, Xform xf, double4* 
 // Triangle boxes; with keys/vals non-null also Morton keys (frame bounds6)
 // and the identity permutation.
 void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,
-                      const uint32_t* bounds6, float4* lo, float4* hi, uint32_t* keys, uint32_t* vals,
+                      const uint32_t* bounds6, int axis_bits, float4* lo, float4* hi, uint32_t* keys, uint32_t* vals,
                       hipStream_t s);
 
 // Forest (a batch of meshes stored back to back, indices global): tiles of
@@ -67,10 +75,10 @@ constexpr uint32_t kForestTile = 1024;
 void launch_prepare_forest(const double* src, const uint2* vtiles, uint32_t n_vtiles, const uint32_t* voff,
                            const uint32_t* mesh_vtile0, uint32_t M, double4* v64, int4* vi, uint32_t* grid_flag,
                            float* partials, uint32_t* bounds, hipStream_t s);
-// boxes and keys mesh << 30 | Morton code (sort on 30 + bits(M) bits)
+// boxes and keys mesh << 3 axis_bits | Morton code (sort on 3 axis_bits + bits(M) bits)
 void launch_tri_boxes_forest(const uint32_t* tri, const uint2* ttiles, uint32_t n_ttiles, const uint32_t* toff,
                              const double4* v64, const int4* vi, const uint32_t* grid_flag, const uint32_t* bounds,
-                             float4* lo, float4* hi, uint64_t* keys, uint32_t* vals, hipStream_t s);
+                             int axis_bits, float4* lo, float4* hi, uint64_t* keys, uint32_t* vals, hipStream_t s);
 
 // LSD radix sort of (key, value) pairs, 8 bits per pass. tmp_* are ping-pong
 // buffers of n elements; hist needs sort_hist_elems(n) entries, scan_tmp
@@ -92,7 +100,10 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t*
 // subtree); range[i] = its leaf range. `blocks` needs block_levels_elems(n)
 // float4. Running it again on the same sorted keys with new prim boxes is the
 // refit after a rigid motion (same topology, new boxes).
-constexpr uint32_t kBuildChunk = 512;
+#ifndef GI_BUILD_CHUNK
+#define GI_BUILD_CHUNK 512
+#endif
+constexpr uint32_t kBuildChunk = GI_BUILD_CHUNK;  // sorted leaves per build workgroup (multiple of 256)
 constexpr int kMaxBlockLevels = 6;
 struct BlockLevels {
     const float4* level[kMaxBlockLevels];
@@ -111,9 +122,10 @@ void launch_build_bvh(const uint64_t* sorted_keys, const uint32_t* sorted_prims,
 void launch_forest_roots(const uint2* range, const float4* nodes, uint32_t n, const uint32_t* toff, uint32_t M,
                          uint2* roots, hipStream_t s);
 
-// keysB/rootsA null: whole trees; else forests (sorted B keys, A mesh roots)
-void launch_traverse(MeshDev A, MeshDev B, const uint64_t* keysB, const uint2* rootsA, uint2* cand, uint32_t cap,
-                     uint32_t* counters, hipStream_t s);
+// keysB/rootsA null: whole trees; else forests (sorted B keys whose mesh id
+// starts at bit mesh_shift, A mesh roots)
+void launch_traverse(MeshDev A, MeshDev B, const uint64_t* keysB, int mesh_shift, const uint2* rootsA, uint2* cand,
+                     uint32_t cap, uint32_t* counters, hipStream_t s);
 void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const uint32_t* grid_flags,
                    uint2* exact_list, uint32_t exact_cap, uint32_t* counters, hipStream_t s);
 void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,

@@ -187,14 +187,17 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
     return v;
 }
 
-// 30-bit Morton code of a box centroid in the mesh bounds frame
-__device__ __forceinline__ uint32_t morton30(const float* bl, const float* bh, const float* bmin, const float* bext) {
+// Morton code (3 x axis_bits bits, axis_bits <= 10) of a box centroid in the
+// mesh bounds frame
+__device__ __forceinline__ uint32_t morton_code(const float* bl, const float* bh, const float* bmin, const float* bext,
+                                                int axis_bits) {
+    const float cells = (float)(1u << axis_bits);
     uint32_t q[3];
     for (int k = 0; k < 3; ++k) {
         float c = 0.5f * bl[k] + 0.5f * bh[k];
         float u = (bext[k] > 0.f && isfinite(bext[k])) ? (c - bmin[k]) / bext[k] : 0.f;
         u = isfinite(u) ? u : 0.f;
-        q[k] = (uint32_t)fminf(fmaxf(u * 1024.f, 0.f), 1023.f);
+        q[k] = (uint32_t)fminf(fmaxf(u * cells, 0.f), cells - 1.f);
     }
     return (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
 }
@@ -237,9 +240,9 @@ __device__ __forceinline__ void morton_frame(const uint32_t* __restrict__ bounds
 __global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ tri, uint32_t nt,
                                                    const double4* __restrict__ v, const int4* __restrict__ vi,
                                                    const uint32_t* __restrict__ grid_flag,
-                                                   const uint32_t* __restrict__ bounds6, float4* __restrict__ lo,
-                                                   float4* __restrict__ hi, uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals) {
+                                                   const uint32_t* __restrict__ bounds6, int axis_bits,
+                                                   float4* __restrict__ lo, float4* __restrict__ hi,
+                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     float bl[3], bh[3];
@@ -249,16 +252,17 @@ __global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ 
     if (keys) {
         float bmin[3], bext[3];
         morton_frame(bounds6, bmin, bext);
-        keys[i] = morton30(bl, bh, bmin, bext);
+        keys[i] = morton_code(bl, bh, bmin, bext, axis_bits);
         vals[i] = i;
     }
 }
 
 void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,
-                      const uint32_t* bounds6, float4* lo, float4* hi, uint32_t* keys, uint32_t* vals,
+                      const uint32_t* bounds6, int axis_bits, float4* lo, float4* hi, uint32_t* keys, uint32_t* vals,
                       hipStream_t s) {
     if (nt == 0) return;
-    k_tri_boxes<<<blocks_for(nt, 256), 256, 0, s>>>(tri, nt, v64, vi, grid_flag, bounds6, lo, hi, keys, vals);
+    k_tri_boxes<<<blocks_for(nt, 256), 256, 0, s>>>(tri, nt, v64, vi, grid_flag, bounds6, axis_bits, lo, hi, keys,
+                                                    vals);
     GI_LAUNCH_CHECK();
 }
 
@@ -318,7 +322,7 @@ __global__ __launch_bounds__(256) void k_forest_bounds(const float* __restrict__
         for (int k = 0; k < 6; ++k) bounds[6 * (size_t)m + k] = f2key(b[k]);
 }
 
-// Triangle boxes and 64-bit keys (mesh id << 30 | Morton code in the mesh's
+// Triangle boxes and 64-bit keys (mesh id << 3 axis_bits | Morton code in the mesh's
 // own frame): one radix sort then groups the batch by mesh and orders each
 // mesh along its Morton curve.
 __global__ __launch_bounds__(256) void k_tri_boxes_forest(const uint32_t* __restrict__ tri,
@@ -326,7 +330,7 @@ __global__ __launch_bounds__(256) void k_tri_boxes_forest(const uint32_t* __rest
                                                           const uint32_t* __restrict__ toff,
                                                           const double4* __restrict__ v, const int4* __restrict__ vi,
                                                           const uint32_t* __restrict__ grid_flag,
-                                                          const uint32_t* __restrict__ bounds,
+                                                          const uint32_t* __restrict__ bounds, int axis_bits,
                                                           float4* __restrict__ lo, float4* __restrict__ hi,
                                                           uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint2 t = tiles[blockIdx.x];
@@ -339,7 +343,7 @@ __global__ __launch_bounds__(256) void k_tri_boxes_forest(const uint32_t* __rest
         tri_box(tri, i, grid, v, vi, bl, bh);
         lo[i] = make_float4(bl[0], bl[1], bl[2], 0.f);
         hi[i] = make_float4(bh[0], bh[1], bh[2], 0.f);
-        keys[i] = ((uint64_t)t.x << 30) | morton30(bl, bh, bmin, bext);
+        keys[i] = ((uint64_t)t.x << (3 * axis_bits)) | morton_code(bl, bh, bmin, bext, axis_bits);
         vals[i] = i;
     }
 }
@@ -356,9 +360,10 @@ void launch_prepare_forest(const double* src, const uint2* vtiles, uint32_t n_vt
 
 void launch_tri_boxes_forest(const uint32_t* tri, const uint2* ttiles, uint32_t n_ttiles, const uint32_t* toff,
                              const double4* v64, const int4* vi, const uint32_t* grid_flag, const uint32_t* bounds,
-                             float4* lo, float4* hi, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+                             int axis_bits, float4* lo, float4* hi, uint64_t* keys, uint32_t* vals, hipStream_t s) {
     if (n_ttiles == 0) return;
-    k_tri_boxes_forest<<<n_ttiles, 256, 0, s>>>(tri, ttiles, toff, v64, vi, grid_flag, bounds, lo, hi, keys, vals);
+    k_tri_boxes_forest<<<n_ttiles, 256, 0, s>>>(tri, ttiles, toff, v64, vi, grid_flag, bounds, axis_bits, lo, hi, keys,
+                                                vals);
     GI_LAUNCH_CHECK();
 }
 
@@ -518,24 +523,17 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_sort_digit_scan(uint32_t* __re
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
+// Stable rank + scatter of one tile: keys are ranked in (wave, item, lane)
+// order, their original order, and key i of digit d goes to gbase[d] + the
+// count of d in earlier waves of the tile + its rank in its wave. wc must be
+// zero on entry; gbase must be complete (a barrier follows the first use).
 template <class K>
-__global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                                 K* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                                 uint32_t n, int shift,
-                                                                 const uint32_t* __restrict__ hist, uint32_t ntiles,
-                                                                 const uint32_t* __restrict__ totals) {
-    __shared__ uint32_t wc[SORT_THREADS / 64][RADIX];
-    __shared__ uint32_t gbase[RADIX];
+__device__ __forceinline__ void tile_rank_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                  K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
+                                                  int shift, uint32_t tile, uint32_t (*wc)[RADIX],
+                                                  const uint32_t* gbase) {
     uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    for (int d = threadIdx.x; d < (SORT_THREADS / 64) * RADIX; d += SORT_THREADS) (&wc[0][0])[d] = 0;
-    {
-        // digit base = exclusive prefix of the digit totals (RADIX == SORT_THREADS)
-        uint32_t t = totals[threadIdx.x];
-        uint32_t ex = block_exclusive_sum(t, nullptr);
-        gbase[threadIdx.x] = ex + hist[(size_t)threadIdx.x * ntiles + blockIdx.x];
-    }
-    __syncthreads();
-    uint32_t base = blockIdx.x * SORT_TILE + wave * (64 * SORT_ITEMS);
+    uint32_t base = tile * SORT_TILE + wave * (64 * SORT_ITEMS);
     uint64_t lt = lanemask_lt();
     K k[SORT_ITEMS];
     uint32_t v[SORT_ITEMS];
@@ -583,32 +581,93 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(const K* __rest
     }
 }
 
-// Whole LSD radix sort in one workgroup (one launch) for small inputs, where
-// the multi-block passes are bound by launch gaps. Keys are ranked in chunks
-// of 1024 in (chunk, wave, lane) order, which keeps every pass stable.
-constexpr uint32_t kSmallSortMax = 4096;
+template <class K>
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                                 K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                                 uint32_t n, int shift,
+                                                                 const uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                                 const uint32_t* __restrict__ totals) {
+    __shared__ uint32_t wc[SORT_THREADS / 64][RADIX];
+    __shared__ uint32_t gbase[RADIX];
+    for (int d = threadIdx.x; d < (SORT_THREADS / 64) * RADIX; d += SORT_THREADS) (&wc[0][0])[d] = 0;
+    {
+        // digit base = exclusive prefix of the digit totals (RADIX == SORT_THREADS)
+        uint32_t t = totals[threadIdx.x];
+        uint32_t ex = block_exclusive_sum(t, nullptr);
+        gbase[threadIdx.x] = ex + hist[(size_t)threadIdx.x * ntiles + blockIdx.x];
+    }
+    __syncthreads();
+    tile_rank_scatter<K>(kin, vin, kout, vout, n, shift, blockIdx.x, wc, gbase);
+}
+
+// Whole LSD radix sort in one workgroup (one launch) for inputs up to
+// small_sort_max keys, where multi-block passes are bound by launch gaps. Every
+// thread keeps SMALL_ITEMS keys in registers; wave w owns the contiguous
+// segment [w * 64 * SMALL_ITEMS, (w + 1) * 64 * SMALL_ITEMS) and ranks it item
+// by item (ballot match per 64 keys, running per-digit counts in LDS), so the
+// order (wave, item, lane) is the input order and every pass is stable. A pass
+// scatters through global memory (ping-pong buffers) and reloads after a
+// workgroup barrier. Values wait in LDS, keys and ranks in registers.
 constexpr int SMALL_THREADS = 1024;
+// Up to 4 keys per thread: beyond 4,096 keys one workgroup (one CU) takes
+// longer per pass than the three launches of a multi-tile pass.
+template <class K>
+constexpr int small_items() { return 4; }
+template <class K>
+constexpr uint32_t small_sort_max() { return SMALL_THREADS * small_items<K>(); }
 
 template <class K>
 __global__ __launch_bounds__(SMALL_THREADS) void k_sort_small(K* k0, uint32_t* v0, K* k1, uint32_t* v1, uint32_t n,
                                                               int passes) {
+    constexpr int SMALL_ITEMS = small_items<K>();
+    constexpr uint32_t kSmallSortMax = small_sort_max<K>();
     __shared__ uint32_t base[RADIX];
-    __shared__ uint32_t chunk_tot[RADIX];
     __shared__ uint32_t wc[SMALL_THREADS / 64][RADIX];
+    __shared__ uint32_t sval[kSmallSortMax];
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
+    const uint32_t seg = wave * 64 * SMALL_ITEMS + lane;
+    K key[SMALL_ITEMS];
+    uint32_t rank2[SMALL_ITEMS / 2];  // ranks < 2^16, two per register
+    for (int it = 0; it < SMALL_ITEMS; ++it) {
+        uint32_t i = seg + 64 * it;
+        key[it] = i < n ? k0[i] : (K)0;
+        if (i < n) sval[i] = v0[i];
+    }
     for (int p = 0; p < passes; ++p) {
-        const K* ks = (p & 1) ? k1 : k0;
-        const uint32_t* vs = (p & 1) ? v1 : v0;
         K* kd = (p & 1) ? k0 : k1;
         uint32_t* vd = (p & 1) ? v0 : v1;
         const int shift = 8 * p;
-        if (threadIdx.x < RADIX) base[threadIdx.x] = 0;
+        for (int d = threadIdx.x; d < (SMALL_THREADS / 64) * RADIX; d += SMALL_THREADS) (&wc[0][0])[d] = 0;
+        for (int q = 0; q < SMALL_ITEMS / 2; ++q) rank2[q] = 0;
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += SMALL_THREADS)
-            atomicAdd(&base[(uint32_t)(ks[i] >> shift) & (RADIX - 1)], 1u);
+        for (int it = 0; it < SMALL_ITEMS; ++it) {
+            bool valid = seg + 64 * it < n;
+            uint32_t d = (uint32_t)(key[it] >> shift) & (RADIX - 1);
+            uint64_t match = __ballot(valid);
+            for (int b = 0; b < 8; ++b) {
+                uint64_t bb = __ballot((d >> b) & 1u);
+                match &= ((d >> b) & 1u) ? bb : ~bb;
+            }
+            uint32_t peers = (uint32_t)__popcll(match & lt);
+            uint32_t cur = wc[wave][d];
+            rank2[it >> 1] |= (cur + peers) << (16 * (it & 1));
+            wave_sync_lds();
+            if (valid && peers == 0) wc[wave][d] = cur + (uint32_t)__popcll(match);
+            wave_sync_lds();
+        }
         __syncthreads();
-        if (threadIdx.x < 64) {  // exclusive scan of the 256 digit counts by one wave
+        if (threadIdx.x < RADIX) {  // per digit: exclusive prefix over the waves, total to base
+            uint32_t run = 0;
+            for (int w = 0; w < SMALL_THREADS / 64; ++w) {
+                uint32_t t = wc[w][threadIdx.x];
+                wc[w][threadIdx.x] = run;
+                run += t;
+            }
+            base[threadIdx.x] = run;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {  // exclusive scan of the 256 digit totals by one wave
             uint32_t c[4], s = 0;
             for (int q = 0; q < 4; ++q) { c[q] = base[4 * threadIdx.x + q]; s += c[q]; }
             uint32_t x = s;
@@ -620,41 +679,20 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_sort_small(K* k0, uint32_t* v
             for (int q = 0; q < 4; ++q) { base[4 * threadIdx.x + q] = off; off += c[q]; }
         }
         __syncthreads();
-        for (uint32_t c0 = 0; c0 < n; c0 += SMALL_THREADS) {
-            for (int d = threadIdx.x; d < (SMALL_THREADS / 64) * RADIX; d += SMALL_THREADS) (&wc[0][0])[d] = 0;
-            __syncthreads();
-            uint32_t i = c0 + threadIdx.x;
-            bool valid = i < n;
-            K key = valid ? ks[i] : (K)0;
-            uint32_t val = valid ? vs[i] : 0u;
-            uint32_t d = (uint32_t)(key >> shift) & (RADIX - 1);
-            uint64_t match = __ballot(valid);
-            for (int b = 0; b < 8; ++b) {
-                uint64_t bb = __ballot((d >> b) & 1u);
-                match &= ((d >> b) & 1u) ? bb : ~bb;
-            }
-            uint32_t rank = (uint32_t)__popcll(match & lt);
-            if (valid && rank == 0) wc[wave][d] = (uint32_t)__popcll(match);
-            __syncthreads();
-            if (threadIdx.x < RADIX) {
-                uint32_t run = 0;
-                for (int w = 0; w < SMALL_THREADS / 64; ++w) {
-                    uint32_t t = wc[w][threadIdx.x];
-                    wc[w][threadIdx.x] = run;
-                    run += t;
-                }
-                chunk_tot[threadIdx.x] = run;
-            }
-            __syncthreads();
-            if (valid) {
-                uint32_t pos = base[d] + wc[wave][d] + rank;
-                kd[pos] = key;
-                vd[pos] = val;
-            }
-            __syncthreads();
-            if (threadIdx.x < RADIX) base[threadIdx.x] += chunk_tot[threadIdx.x];
-            __syncthreads();
+        for (int it = 0; it < SMALL_ITEMS; ++it) {
+            if (seg + 64 * it >= n) continue;
+            uint32_t d = (uint32_t)(key[it] >> shift) & (RADIX - 1);
+            uint32_t pos = base[d] + wc[wave][d] + ((rank2[it >> 1] >> (16 * (it & 1))) & 0xFFFFu);
+            kd[pos] = key[it];
+            vd[pos] = sval[seg + 64 * it];
         }
+        __syncthreads();
+        if (p + 1 < passes)
+            for (int it = 0; it < SMALL_ITEMS; ++it) {
+                uint32_t i = seg + 64 * it;
+                key[it] = i < n ? kd[i] : (K)0;
+                if (i < n) sval[i] = vd[i];
+            }
     }
 }
 
@@ -664,7 +702,7 @@ template <class K>
 static int radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32_t n, int bits, uint32_t* hist,
                            uint32_t* scan_tmp, hipStream_t s) {
     if (n <= 1) return 0;
-    if (n <= kSmallSortMax) {
+    if (n <= small_sort_max<K>()) {
         int passes = (bits + 7) / 8;
         k_sort_small<K><<<1, SMALL_THREADS, 0, s>>>(keys, vals, tk, tv, n, passes);
         GI_LAUNCH_CHECK();
@@ -712,7 +750,7 @@ int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t*
 // A refit after a rigid motion reruns the same kernels on the same sorted
 // keys: the topology comes out identical, only the boxes change.
 
-// Sorted keys (30-bit Morton codes, or mesh id << 30 | Morton code for a
+// Sorted keys (Morton codes, or mesh id << 3 axis_bits | Morton code for a
 // forest), the chunk's slice read from LDS.
 template <class K>
 struct KeySrc {
@@ -1004,11 +1042,11 @@ __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t fill, uint2* 
 // cluster), stackless traversal of A along the ropes. Overlapping leaf pairs
 // are compacted per wave (ballot + popcount) into LDS and flushed to the
 // global candidate queue with one atomic per TRAV_STAGE pairs.
-// Forest (batch): leaf k of B belongs to mesh keysB[k] >> 30 and walks only
+// Forest (batch): leaf k of B belongs to mesh keysB[k] >> mesh_shift and walks only
 // the subtree of the same mesh of A, from its root to the rope leaving it.
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restrict__ nodesA, uint32_t nA,
                                                            const float4* __restrict__ nodesB, uint32_t nB,
-                                                           const uint64_t* __restrict__ keysB,
+                                                           const uint64_t* __restrict__ keysB, int mesh_shift,
                                                            const uint2* __restrict__ rootsA,
                                                            uint2* __restrict__ cand, uint32_t cap,
                                                            uint32_t* __restrict__ counters) {
@@ -1026,7 +1064,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
         b = __float_as_uint(L.w);
         node = 0;
         if (rootsA) {
-            uint2 r = rootsA[keysB[k] >> 30];
+            uint2 r = rootsA[keysB[k] >> mesh_shift];
             node = r.x;
             end = r.y;
         }
@@ -1060,11 +1098,11 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
     if (fill) flush_stage(stage, fill, cand, cap, &counters[CNT_TRAVERSAL_HITS]);
 }
 
-void launch_traverse(MeshDev A, MeshDev B, const uint64_t* keysB, const uint2* rootsA, uint2* cand, uint32_t cap,
-                     uint32_t* counters, hipStream_t s) {
+void launch_traverse(MeshDev A, MeshDev B, const uint64_t* keysB, int mesh_shift, const uint2* rootsA, uint2* cand,
+                     uint32_t cap, uint32_t* counters, hipStream_t s) {
     if (A.nt == 0 || B.nt == 0) return;
-    k_traverse<<<blocks_for(B.nt, TRAV_THREADS), TRAV_THREADS, 0, s>>>(A.nodes, A.nt, B.nodes, B.nt, keysB, rootsA,
-                                                                       cand, cap, counters);
+    k_traverse<<<blocks_for(B.nt, TRAV_THREADS), TRAV_THREADS, 0, s>>>(A.nodes, A.nt, B.nodes, B.nt, keysB, mesh_shift,
+                                                                       rootsA, cand, cap, counters);
     GI_LAUNCH_CHECK();
 }
 

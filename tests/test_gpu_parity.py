@@ -31,9 +31,18 @@ def test_sphere_torus_small_grid():
     assert len(res["pairs"]) > 100
 
 
+def test_result_sort_on_a_few_tiles():
+    # 409,600 triangles per mesh give between 4,096 and 12,288 intersecting
+    # pairs: just past the one-workgroup sort, the result sort runs the
+    # multi-tile passes on a handful of 1,024-key tiles (partial last tile)
+    (xa, ta), (xb, tb) = mg.sphere_torus(640, 320)
+    res, ref = _check(xa, ta, xb, tb, True)
+    assert 4096 < len(res["pairs"]) <= 12288
+
+
 def test_sphere_torus_multi_block_sort_and_chunks():
     # 131,072 triangles per mesh: the Morton sort takes the multi-block radix
-    # path (more than 32,768 keys) and the BVH spans 256 build chunks
+    # path (more than 16,384 keys) and the BVH spans 256 build chunks
     (xa, ta), (xb, tb) = mg.sphere_torus(256, 256)
     res, ref = _check(xa, ta, xb, tb, True)
     assert len(res["pairs"]) > 1000
