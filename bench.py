@@ -39,17 +39,17 @@ def algorithmic_bytes(nvA, ntA, nvB, ntB, st):
     return per_mesh + 112 * st["candidates"] + 8 * st["pairs"] + 128 * st["segments"]
 
 
-def kernel_bytes(name, ntA, ntB, st):
-    """Compulsory bytes of one launch of a single kernel."""
+def kernel_bytes(name, leaves, st):
+    """Compulsory bytes of one launch of a single kernel over `leaves` leaves."""
     if name == "k_build_chunks":
         # per leaf: sorted key 4 B + prim id 4 B + its FP32 box 32 B read;
         # leaf and internal node 32 B each + leaf range 8 B written
-        n = (ntA + ntB) / 2.0
+        n = float(leaves)
         return 4 * n + 4 * n + 32 * n + 32 * (2 * n - 1) + 8 * (n - 1)
     if name == "k_traverse":
         # every B leaf once (32 B), 8 B per emitted pair (A nodes are read as
         # far as the queries need them, which the model does not count)
-        return 32 * ntB + 8 * st["traversal_hits"]
+        return 32 * leaves + 8 * st["traversal_hits"]
     raise KeyError(name)
 
 
@@ -318,12 +318,14 @@ def main():
            ("ms_total", "ms_build", "ms_traverse", "ms_filter", "ms_exact", "ms_output", "ms_bvh_chunks")}
     value = st["candidates"] / (ms / 1000.0)
 
-    # dominant kernel: the LBVH build+refit kernel (largest share of the step),
-    # timed by HIP events the library records around its launches
+    # dominant kernel: the LBVH build+refit kernel (largest share of the step;
+    # one launch builds both trees), timed by HIP events the library records
+    # around it on its stream
     dom = "k_build_chunks"
     launches = sum(s["bvh_chunk_launches"] for s in stats)
     kernel_ms = sum(s["ms_bvh_chunks"] for s in stats) / max(launches, 1)
-    kb = kernel_bytes(dom, ta.shape[0], tb.shape[0], st)
+    leaves_per_launch = sum(s["bvh_chunk_leaves"] for s in stats) / max(launches, 1)
+    kb = kernel_bytes(dom, leaves_per_launch, st)
     achieved = kb / (kernel_ms / 1000.0) / 1e9
     traffic = measured_traffic(dom)
     b_alg = algorithmic_bytes(xa.shape[0], ta.shape[0], xb.shape[0], tb.shape[0], st)
@@ -347,7 +349,8 @@ def main():
                    "vertices_a": int(xa.shape[0]), "vertices_b": int(xb.shape[0])},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_ms": kernel_ms, "launches_timed": launches, "algorithmic_bytes": kb},
+                     "kernel_ms": kernel_ms, "launches_timed": launches, "leaves_per_launch": leaves_per_launch,
+                     "algorithmic_bytes": kb},
         "pipeline": {"candidates": st["candidates"], "traversal_hits": st["traversal_hits"], "pairs": st["pairs"],
                      "segments": st["segments"], "filter_no": st["filter_no"], "filter_green": st["filter_green"],
                      "filter_yellow": st["filter_yellow"], "exact_yellow_yes": st["exact_yellow_yes"],

@@ -68,7 +68,6 @@ struct MeshSlot {
     bool dirty = true;       // vertices/boxes/BVH boxes need an update
     Xform xf;
     int sorted_in_tmp = 0;   // the sorted Morton keys live in (tkeys, tvals)
-    bool chunk_timed = false;  // ev_chunk of this slot bracket its last k_build_chunks
     bool borrowed = false;     // device buffers belong to another context (gi_share_mesh)
     // batch ("forest"): `forest` meshes stored back to back, indices global;
     // mesh m owns vertices [h_voff[m], h_voff[m+1]) and triangles
@@ -124,7 +123,9 @@ struct gi_context {
     DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2, pair_off;
     std::vector<uint32_t> h_pair_off;  // per mesh pair result offsets of the last batch query
     hipEvent_t ev[8] = {};
-    hipEvent_t ev_chunk[4] = {};  // k_build_chunks begin/end for mesh A and B
+    hipEvent_t ev_chunk[4] = {};  // [0..1] bracket the last k_build_chunks launch
+    bool chunk_timed = false;     // that launch ran in the current call
+    uint32_t chunk_leaves = 0;    // leaves it covered (both trees when both were built)
     void release_device() {
         for (auto& m : mesh) m.release();
         DevBuf* all[] = {&bounds, &counters, &hist, &scan_tmp, &partials, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
@@ -193,12 +194,12 @@ void upload_mesh(gi_context* ctx, int slot) {
 }
 
 // A batch: the same stages over all its meshes at once, one launch each.
-// Keys mesh << 3 axis bits | Morton: one sort groups by mesh, one Karras build makes
-// every mesh's BVH a subtree, whose roots are then looked up.
-void prepare_forest(gi_context* ctx, int slot) {
+// Keys mesh << 3 axis bits | Morton: one sort groups by mesh, one Karras build
+// makes every mesh's BVH a subtree, whose roots are then looked up.
+void stage_forest(gi_context* ctx, int slot) {
     MeshSlot& m = ctx->mesh[slot];
     hipStream_t s = ctx->build_stream[slot];
-    uint32_t* grid = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2) + CNT_COUNT;
+    uint32_t* grid = ctx->counters.get<uint32_t>() + CNT_COUNT;
     size_t nv = m.nv, nt = m.nt;
     const uint32_t M = m.forest;
     const uint32_t n_vtiles = (uint32_t)(m.h_vtiles.size() / 2), n_ttiles = (uint32_t)(m.h_ttiles.size() / 2);
@@ -206,10 +207,10 @@ void prepare_forest(gi_context* ctx, int slot) {
     int4* vi = m.vi.ensure<int4>(nv);
     float4* lo = m.lo.ensure<float4>(nt);
     float4* hi = m.hi.ensure<float4>(nt);
-    size_t nnodes = nt ? 2 * nt - 1 : 1;
-    float4* nodes = m.nodes.ensure<float4>(2 * nnodes);
-    uint2* range = m.range.ensure<uint2>(nt);
-    float4* blocks = m.blocks.ensure<float4>(block_levels_elems((uint32_t)nt));
+    m.nodes.ensure<float4>(2 * (nt ? 2 * nt - 1 : 1));
+    m.range.ensure<uint2>(nt);
+    m.blocks.ensure<float4>(block_levels_elems((uint32_t)nt));
+    m.roots.ensure<uint2>(M);
     uint64_t* keys = m.keys.ensure<uint64_t>(nt);
     uint32_t* vals = m.vals.ensure<uint32_t>(nt);
     uint64_t* tk = m.tkeys.ensure<uint64_t>(nt);
@@ -226,37 +227,31 @@ void prepare_forest(gi_context* ctx, int slot) {
     uint32_t* hist = ctx->bhist[slot].ensure<uint32_t>(sort_hist_elems(nt));
     uint32_t* stmp = ctx->bscan[slot].ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
     m.sorted_in_tmp = radix_sort_u64(keys, vals, tk, tv, m.nt, 3 * m.morton_axis + bits_for(M), hist, stmp, s);
-    launch_build_bvh(m.sorted_in_tmp ? tk : keys, m.sorted_in_tmp ? tv : vals, m.nt, lo, hi, nodes, range, blocks, s,
-                     ctx->ev_chunk[2 * slot], ctx->ev_chunk[2 * slot + 1]);
-    launch_forest_roots(range, nodes, m.nt, m.toff.get<uint32_t>(), M, m.roots.ensure<uint2>(M), s);
-    m.chunk_timed = m.nt > 0;
-    HIP_TRY(hipGetLastError());
-    m.built = true;
-    m.dirty = false;
 }
 
-// vertices -> boxes -> (Morton sort + Karras build | leaf refresh) -> refit
-void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
+// The part of a mesh's build that runs on the slot's own stream: vertices
+// (transform) -> boxes -> Morton sort (new topology only). Returns true when
+// the slot's tree must then be built or refit (build_meshes).
+bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
     MeshSlot& m = ctx->mesh[slot];
-    if (m.borrowed) return;  // built and kept current by its owner
+    if (m.borrowed) return false;  // built and kept current by its owner
     if (!m.on_device) upload_mesh(ctx, slot);
-    if (!m.dirty && m.built && !rebuild) return;
+    if (!m.dirty && m.built && !rebuild) return false;
     if (m.forest) {
-        prepare_forest(ctx, slot);
-        return;
+        stage_forest(ctx, slot);
+        return true;
     }
     hipStream_t s = ctx->build_stream[slot];
-    uint32_t* grid = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2) + CNT_COUNT;
-    uint32_t* bounds = ctx->bounds.ensure<uint32_t>(12) + 6 * slot;
+    uint32_t* grid = ctx->counters.get<uint32_t>() + CNT_COUNT;
+    uint32_t* bounds = ctx->bounds.get<uint32_t>() + 6 * slot;
     size_t nv = m.nv, nt = m.nt;
     double4* v64 = m.v64.ensure<double4>(nv);
     int4* vi = m.vi.ensure<int4>(nv);
     float4* lo = m.lo.ensure<float4>(nt);
     float4* hi = m.hi.ensure<float4>(nt);
-    size_t nnodes = nt ? 2 * nt - 1 : 1;
-    float4* nodes = m.nodes.ensure<float4>(2 * nnodes);
-    uint2* range = m.range.ensure<uint2>(nt);
-    float4* blocks = m.blocks.ensure<float4>(block_levels_elems((uint32_t)nt));
+    m.nodes.ensure<float4>(2 * (nt ? 2 * nt - 1 : 1));
+    m.range.ensure<uint2>(nt);
+    m.blocks.ensure<float4>(block_levels_elems((uint32_t)nt));
     uint32_t* keys = m.keys.ensure<uint32_t>(nt);
     uint32_t* vals = m.vals.ensure<uint32_t>(nt);
     uint32_t* tk = m.tkeys.ensure<uint32_t>(nt);
@@ -272,14 +267,72 @@ void prepare_mesh(gi_context* ctx, int slot, bool rebuild) {
         uint32_t* hist = ctx->bhist[slot].ensure<uint32_t>(sort_hist_elems(nt));
         uint32_t* stmp = ctx->bscan[slot].ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
         m.sorted_in_tmp = radix_sort_u32(keys, vals, tk, tv, m.nt, 3 * m.morton_axis, hist, stmp, s);
-        m.built = true;
     }
-    // build, or refit on the kept sorted keys (same topology, new boxes)
-    launch_build_bvh(m.sorted_in_tmp ? tk : keys, m.sorted_in_tmp ? tv : vals, m.nt, lo, hi, nodes, range, blocks, s,
-                     ctx->ev_chunk[2 * slot], ctx->ev_chunk[2 * slot + 1]);
-    m.chunk_timed = m.nt > 0;
+    return true;
+}
+
+template <class K>
+BuildJob<K> slot_job(MeshSlot& m) {
+    BuildJob<K> j;
+    j.keys = m.sorted_in_tmp ? m.tkeys.get<K>() : m.keys.get<K>();
+    j.prims = m.sorted_in_tmp ? m.tvals.get<uint32_t>() : m.vals.get<uint32_t>();
+    j.n = m.nt;
+    j.lo = m.lo.get<float4>();
+    j.hi = m.hi.get<float4>();
+    j.nodes = m.nodes.get<float4>();
+    j.range = m.range.get<uint2>();
+    j.blocks = m.blocks.get<float4>();
+    return j;
+}
+
+// Brings both trees up to date. The per-slot stages of the two meshes run
+// concurrently (main and side stream); after the join, one launch per build
+// kernel serves every tree that needs a build or refit (a refit reruns the
+// build on the kept sorted keys: same topology, new boxes). The
+// k_build_chunks launch is bracketed by ev_chunk[0..1].
+void build_meshes(gi_context* ctx, bool rebuild) {
+    ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);  // shared by both stages: allocate before the fork
+    ctx->bounds.ensure<uint32_t>(12);
+    ctx->chunk_timed = false;
+    ctx->chunk_leaves = 0;
+    fork_side(ctx);
+    bool need[2] = {false, false};
+    for (int sl = 0; sl < 2; ++sl)
+        if (ctx->mesh[sl].set) need[sl] = stage_mesh(ctx, sl, rebuild);
+    join_side(ctx);
+    BuildJob<uint32_t> j32[2];
+    BuildJob<uint64_t> j64[2];
+    int n32 = 0, n64 = 0;
+    for (int sl = 0; sl < 2; ++sl) {
+        if (!need[sl]) continue;
+        MeshSlot& m = ctx->mesh[sl];
+        if (m.forest) j64[n64++] = slot_job<uint64_t>(m);
+        else j32[n32++] = slot_job<uint32_t>(m);
+    }
+    hipStream_t s = ctx->stream;
+    auto run = [&](auto* jobs, int nj) {
+        if (nj == 0) return;
+        uint32_t leaves = 0;
+        for (int j = 0; j < nj; ++j) leaves += jobs[j].n;
+        bool time_it = !ctx->chunk_timed && leaves > 0;
+        launch_build_bvh(jobs, nj, s, time_it ? ctx->ev_chunk[0] : nullptr, time_it ? ctx->ev_chunk[1] : nullptr);
+        if (time_it) {
+            ctx->chunk_timed = true;
+            ctx->chunk_leaves = leaves;
+        }
+    };
+    run(j32, n32);
+    run(j64, n64);
+    for (int sl = 0; sl < 2; ++sl) {
+        if (!need[sl]) continue;
+        MeshSlot& m = ctx->mesh[sl];
+        if (m.forest)
+            launch_forest_roots(m.range.get<uint2>(), m.nodes.get<float4>(), m.nt, m.toff.get<uint32_t>(), m.forest,
+                                m.roots.get<uint2>(), s);
+        m.built = true;
+        m.dirty = false;
+    }
     HIP_TRY(hipGetLastError());
-    m.dirty = false;
 }
 
 MeshDev mesh_dev(const MeshSlot& m) {
@@ -348,13 +401,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     hipStream_t s = ctx->stream;
     gi_stats& st = out->stats;
     HIP_TRY(hipEventRecord(ctx->ev[0], s));
-    MA.chunk_timed = MB.chunk_timed = false;
-    ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);  // shared by both builds: allocate before the fork
-    ctx->bounds.ensure<uint32_t>(12);
-    fork_side(ctx);
-    prepare_mesh(ctx, 0, o.rebuild != 0);
-    prepare_mesh(ctx, 1, o.rebuild != 0);
-    join_side(ctx);
+    build_meshes(ctx, o.rebuild != 0);
     HIP_TRY(hipEventRecord(ctx->ev[1], s));
 
     if (ctx->cap_cand == 0) {
@@ -408,11 +455,11 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     st.filter_yellow = hc[CNT_FILTER_YELLOW];
     st.grid = hgrid[0] && hgrid[1];
     st.ms_build = elapsed(ctx->ev[0], ctx->ev[1]);
-    for (int sl = 0; sl < 2; ++sl)
-        if (ctx->mesh[sl].chunk_timed) {
-            st.ms_bvh_chunks += elapsed(ctx->ev_chunk[2 * sl], ctx->ev_chunk[2 * sl + 1]);
-            st.bvh_chunk_launches++;
-        }
+    if (ctx->chunk_timed) {
+        st.ms_bvh_chunks = elapsed(ctx->ev_chunk[0], ctx->ev_chunk[1]);
+        st.bvh_chunk_launches = 1;
+        st.bvh_chunk_leaves = ctx->chunk_leaves;
+    }
     st.ms_traverse = elapsed(ctx->ev[6], ctx->ev[2]);
     st.ms_filter = elapsed(ctx->ev[2], ctx->ev[3]);
 
@@ -816,12 +863,7 @@ int gi_build(gi_context* ctx) {
     if (!ctx) return GI_ERR_INVALID_ARGUMENT;
     try {
         HIP_TRY(hipSetDevice(ctx->device));
-        ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);
-        ctx->bounds.ensure<uint32_t>(12);
-        fork_side(ctx);
-        for (int sl = 0; sl < 2; ++sl)
-            if (ctx->mesh[sl].set) prepare_mesh(ctx, sl, false);
-        join_side(ctx);
+        build_meshes(ctx, false);
         sync_all(ctx);
         return GI_OK;
     } catch (const GiError& e) {
@@ -874,11 +916,7 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
     if (src->mesh[src_slot].forest) return GI_ERR_INVALID_ARGUMENT;
     try {
         HIP_TRY(hipSetDevice(src->device));
-        src->counters.ensure<uint32_t>(CNT_COUNT + 2);
-        src->bounds.ensure<uint32_t>(12);
-        fork_side(src);
-        prepare_mesh(src, src_slot, false);
-        join_side(src);
+        build_meshes(src, false);
         sync_all(src);
         MeshSlot& d = dst->mesh[dst_slot];
         const MeshSlot& s = src->mesh[src_slot];
@@ -1041,14 +1079,7 @@ int gi_reconstruct_device(gi_context* ctx) {
         }
     try {
         HIP_TRY(hipSetDevice(ctx->device));
-        ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);
-        ctx->bounds.ensure<uint32_t>(12);
-        fork_side(ctx);
-        for (int sl = 0; sl < 2; ++sl) {
-            if (!ctx->mesh[sl].on_device) upload_mesh(ctx, sl);
-            prepare_mesh(ctx, sl, true);
-        }
-        join_side(ctx);
+        build_meshes(ctx, true);
         sync_all(ctx);
         return GI_OK;
     } catch (const GiError& e) {

@@ -109,15 +109,26 @@ struct BlockLevels {
     const float4* level[kMaxBlockLevels];
 };
 size_t block_levels_elems(uint32_t n);
-// chunk_begin/chunk_end (may be null) bracket the k_build_chunks launch.
-void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
-                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s,
-                      hipEvent_t chunk_begin, hipEvent_t chunk_end);
-// the same over a forest's 64-bit keys: one tree whose mesh subtrees are the
-// per-mesh BVHs
-void launch_build_bvh(const uint64_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
-                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s,
-                      hipEvent_t chunk_begin, hipEvent_t chunk_end);
+// One tree to build (keys: 32-bit Morton codes, or 64-bit forest keys whose
+// mesh ids lead, so that each mesh's BVH is a subtree of the forest).
+template <class K>
+struct BuildJob {
+    const K* keys;
+    const uint32_t* prims;
+    uint32_t n;
+    const float4* lo;
+    const float4* hi;
+    float4* nodes;
+    uint2* range;
+    float4* blocks;
+};
+// Builds one or two trees with one launch per kernel (the blocks of a launch
+// are split between the trees); chunk_begin/chunk_end (may be null) bracket
+// the k_build_chunks launch.
+void launch_build_bvh(const BuildJob<uint32_t>* jobs, int njobs, hipStream_t s, hipEvent_t chunk_begin,
+                      hipEvent_t chunk_end);
+void launch_build_bvh(const BuildJob<uint64_t>* jobs, int njobs, hipStream_t s, hipEvent_t chunk_begin,
+                      hipEvent_t chunk_end);
 // roots[m] = (subtree root of mesh m, rope leaving it)
 void launch_forest_roots(const uint2* range, const float4* nodes, uint32_t n, const uint32_t* toff, uint32_t M,
                          uint2* roots, hipStream_t s);

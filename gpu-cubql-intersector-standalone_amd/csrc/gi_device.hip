@@ -784,11 +784,32 @@ __device__ __forceinline__ uint32_t chain_top(const KeySrc<K>& k, int m) {
     return (uint32_t)(k.n - 1 + m);
 }
 
+// Up to two trees per launch: blocks [0, split) build tree 0, the rest tree 1.
+// Fields are picked with selects on a block-uniform flag, so the kernel
+// arguments are never indexed dynamically.
 template <class K>
-__global__ __launch_bounds__(256) void k_build_chunks(const K* __restrict__ keys, const uint32_t* __restrict__ prims,
-                                                      int n, const float4* __restrict__ lo,
-                                                      const float4* __restrict__ hi, float4* __restrict__ nodes,
-                                                      uint2* __restrict__ range) {
+struct ChunkJobs {
+    const K* keys0; const K* keys1;
+    const uint32_t* prims0; const uint32_t* prims1;
+    const float4* lo0; const float4* lo1;
+    const float4* hi0; const float4* hi1;
+    float4* nodes0; float4* nodes1;
+    uint2* range0; uint2* range1;
+    int n0, n1;
+    uint32_t split;
+};
+
+template <class K>
+__global__ __launch_bounds__(256) void k_build_chunks(ChunkJobs<K> J) {
+    const bool second = blockIdx.x >= J.split;
+    const K* __restrict__ keys = second ? J.keys1 : J.keys0;
+    const uint32_t* __restrict__ prims = second ? J.prims1 : J.prims0;
+    const float4* __restrict__ lo = second ? J.lo1 : J.lo0;
+    const float4* __restrict__ hi = second ? J.hi1 : J.hi0;
+    float4* __restrict__ nodes = second ? J.nodes1 : J.nodes0;
+    uint2* __restrict__ range = second ? J.range1 : J.range0;
+    const int n = second ? J.n1 : J.n0;
+    const uint32_t chunk = blockIdx.x - (second ? J.split : 0u);
     __shared__ K skey[kBuildChunk];
     __shared__ uint32_t par_leaf[kBuildChunk];
     __shared__ uint32_t par_int[kBuildChunk];
@@ -796,7 +817,7 @@ __global__ __launch_bounds__(256) void k_build_chunks(const K* __restrict__ keys
     __shared__ uint32_t rope_int[kBuildChunk];
     __shared__ uint32_t flag[kBuildChunk];
     __shared__ uint32_t ubox[kBuildChunk][6];  // merged boxes as order-preserving uint keys
-    const int cs = blockIdx.x * (int)kBuildChunk;
+    const int cs = (int)chunk * (int)kBuildChunk;
     const int ce = min(n, cs + (int)kBuildChunk);
     for (int k = threadIdx.x; k < (int)kBuildChunk; k += blockDim.x) {
         if (cs + k < ce) skey[k] = keys[cs + k];
@@ -879,10 +900,21 @@ __global__ __launch_bounds__(256) void k_build_chunks(const K* __restrict__ keys
     }
 }
 
-// one wave per output block of 64 input elements
-__global__ __launch_bounds__(256) void k_block_boxes(const float4* __restrict__ in, uint32_t n_in,
-                                                     float4* __restrict__ out, uint32_t n_out) {
-    uint32_t blk = blockIdx.x * 4 + (threadIdx.x >> 6);
+// one wave per output block of 64 input elements; blocks [0, split) serve
+// tree 0, the rest tree 1
+struct BlockBoxJobs {
+    const float4* in0; const float4* in1;
+    float4* out0; float4* out1;
+    uint32_t n_in0, n_in1, n_out0, n_out1;
+    uint32_t split;
+};
+
+__global__ __launch_bounds__(256) void k_block_boxes(BlockBoxJobs J) {
+    const bool second = blockIdx.x >= J.split;
+    const float4* __restrict__ in = second ? J.in1 : J.in0;
+    float4* __restrict__ out = second ? J.out1 : J.out0;
+    const uint32_t n_in = second ? J.n_in1 : J.n_in0, n_out = second ? J.n_out1 : J.n_out0;
+    uint32_t blk = (blockIdx.x - (second ? J.split : 0u)) * 4 + (threadIdx.x >> 6);
     if (blk >= n_out) return;
     float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
     uint32_t idx = blk * 64 + lane_id();
@@ -924,10 +956,23 @@ __device__ __forceinline__ void union_range_box(uint32_t n, const uint2* __restr
 }
 
 // one wave per 64 consecutive internal nodes; the top ones among them are
-// done one after the other by the whole wave
-__global__ __launch_bounds__(256) void k_top_boxes(uint32_t n, const uint2* __restrict__ range, float4* nodes,
-                                                   BlockLevels lv) {
-    uint32_t base = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+// done one after the other by the whole wave. Blocks [0, split) serve tree 0.
+struct TopBoxJobs {
+    BlockLevels lv0, lv1;
+    const uint2* range0; const uint2* range1;
+    float4* nodes0; float4* nodes1;
+    uint32_t n0, n1;
+    uint32_t split;
+};
+
+__global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J) {
+    const bool second = blockIdx.x >= J.split;
+    const uint32_t n = second ? J.n1 : J.n0;
+    const uint2* __restrict__ range = second ? J.range1 : J.range0;
+    float4* nodes = second ? J.nodes1 : J.nodes0;
+    BlockLevels lv;
+    for (int k = 0; k < kMaxBlockLevels; ++k) lv.level[k] = second ? J.lv1.level[k] : J.lv0.level[k];
+    uint32_t base = ((blockIdx.x - (second ? J.split : 0u)) * 4 + (threadIdx.x >> 6)) * 64;
     uint32_t i = base + lane_id();
     bool top = false;
     if (i + 1 < n) {
@@ -955,39 +1000,67 @@ size_t block_levels_elems(uint32_t n) {
 }
 
 template <class K>
-static void build_bvh_impl(const K* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
-                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s,
-                      hipEvent_t chunk_begin, hipEvent_t chunk_end) {
-    if (n == 0) return;
+static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, hipStream_t s, hipEvent_t chunk_begin,
+                           hipEvent_t chunk_end) {
+    const BuildJob<K>* j0 = &jobs[0];
+    const BuildJob<K>* j1 = njobs > 1 ? &jobs[1] : nullptr;
+    const uint32_t n0 = j0->n, n1 = j1 ? j1->n : 0;
+    if (n0 + n1 == 0) return;
+    ChunkJobs<K> cj;
+    cj.keys0 = j0->keys; cj.prims0 = j0->prims; cj.lo0 = j0->lo; cj.hi0 = j0->hi;
+    cj.nodes0 = j0->nodes; cj.range0 = j0->range; cj.n0 = (int)n0;
+    const BuildJob<K>* o = j1 ? j1 : j0;
+    cj.keys1 = o->keys; cj.prims1 = o->prims; cj.lo1 = o->lo; cj.hi1 = o->hi;
+    cj.nodes1 = o->nodes; cj.range1 = o->range; cj.n1 = (int)n1;
+    cj.split = blocks_for(n0, kBuildChunk);
+    const uint32_t chunk_blocks = cj.split + blocks_for(n1, kBuildChunk);
     if (chunk_begin) (void)hipEventRecord(chunk_begin, s);
-    k_build_chunks<K><<<blocks_for(n, kBuildChunk), 256, 0, s>>>(sorted_keys, sorted_prims, (int)n, lo, hi, nodes, range);
+    k_build_chunks<K><<<chunk_blocks, 256, 0, s>>>(cj);
     if (chunk_end) (void)hipEventRecord(chunk_end, s);
-    if (n < 2) return;
-    BlockLevels lv;
-    lv.level[0] = nodes + 2 * (size_t)(n - 1);
-    size_t off = 0;
-    for (int k = 1; k < kMaxBlockLevels; ++k) {
-        lv.level[k] = blocks + off;
-        off += 2 * (size_t)block_level_count(n, k);
+    // 64-ary levels of leaf boxes for the nodes that cross chunks
+    BlockLevels lv[2];
+    const BuildJob<K>* jj[2] = {j0, o};
+    uint32_t nn[2] = {n0, n1};
+    for (int t = 0; t < 2; ++t) {
+        lv[t].level[0] = jj[t]->nodes + 2 * (size_t)(nn[t] ? nn[t] - 1 : 0);
+        size_t off = 0;
+        for (int k = 1; k < kMaxBlockLevels; ++k) {
+            lv[t].level[k] = jj[t]->blocks + off;
+            off += 2 * (size_t)block_level_count(nn[t], k);
+        }
     }
     for (int k = 1; k < kMaxBlockLevels; ++k) {
-        uint32_t n_in = block_level_count(n, k - 1), n_out = block_level_count(n, k);
-        if (n_in <= 1) break;
-        k_block_boxes<<<blocks_for(n_out, 4), 256, 0, s>>>(lv.level[k - 1], n_in, (float4*)lv.level[k], n_out);
+        BlockBoxJobs bj;
+        uint32_t nin[2], nout[2];
+        for (int t = 0; t < 2; ++t) {
+            nin[t] = block_level_count(nn[t], k - 1);
+            nout[t] = nin[t] > 1 ? block_level_count(nn[t], k) : 0;  // levels stop at one box
+        }
+        if (nout[0] == 0 && nout[1] == 0) break;
+        bj.in0 = lv[0].level[k - 1]; bj.out0 = (float4*)lv[0].level[k]; bj.n_in0 = nin[0]; bj.n_out0 = nout[0];
+        bj.in1 = lv[1].level[k - 1]; bj.out1 = (float4*)lv[1].level[k]; bj.n_in1 = nin[1]; bj.n_out1 = nout[1];
+        bj.split = blocks_for(nout[0], 4);
+        k_block_boxes<<<bj.split + blocks_for(nout[1], 4), 256, 0, s>>>(bj);
     }
-    k_top_boxes<<<blocks_for(blocks_for(n - 1, 64), 4), 256, 0, s>>>(n, range, nodes, lv);
+    TopBoxJobs tj;
+    tj.lv0 = lv[0]; tj.lv1 = lv[1];
+    tj.range0 = j0->range; tj.range1 = o->range;
+    tj.nodes0 = j0->nodes; tj.nodes1 = o->nodes;
+    tj.n0 = n0; tj.n1 = n1;
+    // one wave per 64 internal nodes (n - 1 of them); a one-leaf tree has none
+    tj.split = n0 > 1 ? blocks_for(blocks_for(n0 - 1, 64), 4) : 0;
+    uint32_t top_blocks = tj.split + (n1 > 1 ? blocks_for(blocks_for(n1 - 1, 64), 4) : 0);
+    if (top_blocks) k_top_boxes<<<top_blocks, 256, 0, s>>>(tj);
     GI_LAUNCH_CHECK();
 }
 
-void launch_build_bvh(const uint32_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
-                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s,
-                      hipEvent_t chunk_begin, hipEvent_t chunk_end) {
-    build_bvh_impl<uint32_t>(sorted_keys, sorted_prims, n, lo, hi, nodes, range, blocks, s, chunk_begin, chunk_end);
+void launch_build_bvh(const BuildJob<uint32_t>* jobs, int njobs, hipStream_t s, hipEvent_t chunk_begin,
+                      hipEvent_t chunk_end) {
+    build_bvh_impl<uint32_t>(jobs, njobs, s, chunk_begin, chunk_end);
 }
-void launch_build_bvh(const uint64_t* sorted_keys, const uint32_t* sorted_prims, uint32_t n, const float4* lo,
-                      const float4* hi, float4* nodes, uint2* range, float4* blocks, hipStream_t s,
-                      hipEvent_t chunk_begin, hipEvent_t chunk_end) {
-    build_bvh_impl<uint64_t>(sorted_keys, sorted_prims, n, lo, hi, nodes, range, blocks, s, chunk_begin, chunk_end);
+void launch_build_bvh(const BuildJob<uint64_t>* jobs, int njobs, hipStream_t s, hipEvent_t chunk_begin,
+                      hipEvent_t chunk_end) {
+    build_bvh_impl<uint64_t>(jobs, njobs, s, chunk_begin, chunk_end);
 }
 
 // Subtree of each forest mesh: the node whose leaf range is exactly the

@@ -97,9 +97,9 @@ typedef struct {
     int32_t retries;            /* capacity regrowths during this call */
     /* milliseconds (GPU events; host stage by wall clock) */
     float ms_total, ms_build, ms_traverse, ms_filter, ms_exact, ms_output, ms_host;
-    float ms_bvh_chunks;        /* sum of the k_build_chunks launches of this call */
-    int32_t bvh_chunk_launches; /* how many (0, 1 or 2: one per mesh built or refit) */
-    int32_t pad;
+    float ms_bvh_chunks;        /* the k_build_chunks launch of this call (both trees in one launch) */
+    int32_t bvh_chunk_launches; /* 0 or 1 */
+    uint32_t bvh_chunk_leaves;  /* leaves that launch built or refit */
 } gi_stats;
 
 typedef struct {
