@@ -75,9 +75,11 @@ struct MeshSlot {
     uint32_t forest = 0;
     std::vector<uint32_t> h_voff, h_toff, h_vtiles, h_ttiles, h_vtile0;
     int morton_axis = 10;      // Morton bits per axis of the kept sorted keys
+    uint64_t generation = 0;   // bumped whenever the slot's mesh or buffers are replaced
     DevBuf src, tri, v64, vi, lo, hi, keys, vals, tkeys, tvals, nodes, range, blocks;
     DevBuf voff, toff, vtiles, ttiles, vtile0, fbounds, roots;
     void release() {
+        ++generation;
         DevBuf* forest_bufs[] = {&voff, &toff, &vtiles, &ttiles, &vtile0, &fbounds, &roots};
         for (DevBuf* b : forest_bufs) b->release();
         DevBuf* all[] = {&src, &tri, &v64, &vi, &lo, &hi, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks};
@@ -123,10 +125,31 @@ struct gi_context {
     DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2, pair_off;
     std::vector<uint32_t> h_pair_off;  // per mesh pair result offsets of the last batch query
     hipEvent_t ev[8] = {};
+    // pinned host staging: per-slot transforms (copied to xf_dev before the
+    // vertex kernels) and the counter read-back; graph replays read/write them
+    Xform* h_xf = nullptr;
+    uint32_t* h_counters = nullptr;
+    DevBuf xf_dev;
+    // captured launch sequence of a query up to the counter read-back, valid
+    // while graph_sig (sizes, flags, device pointers) is unchanged
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_sig, graph_sig_seen;
+    bool graph_off = false;  // capture failed once: queries stay eager
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_sig.clear();
+    }
     hipEvent_t ev_chunk[4] = {};  // [0..1] bracket the last k_build_chunks launch
     bool chunk_timed = false;     // that launch ran in the current call
     uint32_t chunk_leaves = 0;    // leaves it covered (both trees when both were built)
     void release_device() {
+        drop_graph();
+        graph_sig_seen.clear();
+        xf_dev.release();
         for (auto& m : mesh) m.release();
         DevBuf* all[] = {&bounds, &counters, &hist, &scan_tmp, &partials, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
                          &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2, &pair_off,
@@ -259,7 +282,10 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
     const bool topology = !m.built || rebuild;
     launch_init_mesh_scalars(grid + slot, bounds, s);
     float* partials = ctx->bpart[slot].ensure<float>(6 * (size_t)kPrepareMaxBlocks);
-    launch_prepare_vertices(m.src.get<double>(), m.nv, m.xf, v64, vi, grid + slot, partials, bounds, s);
+    Xform* d_xf = ctx->xf_dev.get<Xform>() + slot;
+    ctx->h_xf[slot] = m.xf;
+    HIP_TRY(hipMemcpyAsync(d_xf, ctx->h_xf + slot, sizeof(Xform), hipMemcpyHostToDevice, s));
+    launch_prepare_vertices(m.src.get<double>(), m.nv, d_xf, v64, vi, grid + slot, partials, bounds, s);
     if (topology) m.morton_axis = morton_axis_bits(m.nt);
     launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, vi, grid + slot, bounds, m.morton_axis, lo, hi,
                      topology ? keys : nullptr, topology ? vals : nullptr, s);
@@ -290,9 +316,10 @@ BuildJob<K> slot_job(MeshSlot& m) {
 // kernel serves every tree that needs a build or refit (a refit reruns the
 // build on the kept sorted keys: same topology, new boxes). The
 // k_build_chunks launch is bracketed by ev_chunk[0..1].
-void build_meshes(gi_context* ctx, bool rebuild) {
+void build_meshes(gi_context* ctx, bool rebuild, bool timed = true) {
     ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);  // shared by both stages: allocate before the fork
     ctx->bounds.ensure<uint32_t>(12);
+    ctx->xf_dev.ensure<Xform>(2);
     ctx->chunk_timed = false;
     ctx->chunk_leaves = 0;
     fork_side(ctx);
@@ -314,7 +341,7 @@ void build_meshes(gi_context* ctx, bool rebuild) {
         if (nj == 0) return;
         uint32_t leaves = 0;
         for (int j = 0; j < nj; ++j) leaves += jobs[j].n;
-        bool time_it = !ctx->chunk_timed && leaves > 0;
+        bool time_it = timed && !ctx->chunk_timed && leaves > 0;
         launch_build_bvh(jobs, nj, s, time_it ? ctx->ev_chunk[0] : nullptr, time_it ? ctx->ev_chunk[1] : nullptr);
         if (time_it) {
             ctx->chunk_timed = true;
@@ -384,6 +411,152 @@ void empty_result(gi_result* out, int on_device) {
     (void)on_device;
 }
 
+// Everything of a query up to the counter read-back: the trees (when `build`),
+// the counter reset, traversal, filter, exact stage and the copy of the
+// counters to pinned host memory. `timed` records the stage events (not
+// inside a graph capture).
+void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool timed) {
+    hipStream_t s = ctx->stream;
+    MeshSlot& MA = ctx->mesh[0];
+    MeshSlot& MB = ctx->mesh[1];
+    const uint32_t M = MA.forest;
+    if (build) {
+        if (timed) HIP_TRY(hipEventRecord(ctx->ev[0], s));
+        build_meshes(ctx, o.rebuild != 0, timed);
+        if (timed) HIP_TRY(hipEventRecord(ctx->ev[1], s));
+    }
+    uint32_t* counters = ctx->counters.get<uint32_t>();
+    uint32_t* grid = counters + CNT_COUNT;
+    MeshDev A = mesh_dev(MA), B = mesh_dev(MB);
+    int bitsB = bits_for(MB.nt);
+    launch_fill_u32(counters, 0u, CNT_COUNT, s);
+    if (timed) HIP_TRY(hipEventRecord(ctx->ev[6], s));
+    const uint64_t* keysB = M ? (MB.sorted_in_tmp ? MB.tkeys.get<uint64_t>() : MB.keys.get<uint64_t>()) : nullptr;
+    launch_traverse(A, B, keysB, 3 * MB.morton_axis, M ? MA.roots.get<uint2>() : nullptr, ctx->cand.get<uint2>(),
+                    ctx->cap_cand, counters, s);
+    if (timed) HIP_TRY(hipEventRecord(ctx->ev[2], s));
+    launch_filter(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, grid, ctx->exact.get<uint2>(), ctx->cap_exact, counters,
+                  s);
+    if (timed) HIP_TRY(hipEventRecord(ctx->ev[3], s));
+    launch_exact(A, B, ctx->exact.get<uint2>(), ctx->cap_exact, grid, bitsB, ctx->rec_key.get<uint64_t>(),
+                 ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<__int128>(), ctx->cap_seg, counters, s);
+    if (timed) HIP_TRY(hipEventRecord(ctx->ev[4], s));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(ctx->h_counters, counters, sizeof(uint32_t) * (CNT_COUNT + 2), hipMemcpyDeviceToHost, s));
+}
+
+bool slot_needs_stage(const MeshSlot& m, bool rebuild) {
+    return m.set && !m.borrowed && (m.dirty || !m.built || rebuild);
+}
+
+// Everything enqueue_presync bakes into launches: sizes, flags, device
+// pointers (a reallocation changes them), capacities.
+std::vector<uint64_t> presync_signature(gi_context* ctx, const gi_options& o) {
+    std::vector<uint64_t> v;
+    v.push_back((uint64_t)o.rebuild);
+    auto buf = [&](const DevBuf& b) {
+        v.push_back((uint64_t)(uintptr_t)b.p);
+        v.push_back((uint64_t)b.bytes);
+    };
+    for (const MeshSlot& m : ctx->mesh) {
+        v.push_back(m.nv);
+        v.push_back(m.nt);
+        v.push_back(m.forest);
+        v.push_back(((uint64_t)m.set << 0) | ((uint64_t)m.borrowed << 1) | ((uint64_t)m.on_device << 2) |
+                    ((uint64_t)m.built << 3) | ((uint64_t)m.dirty << 4) | ((uint64_t)m.sorted_in_tmp << 5));
+        v.push_back((uint64_t)m.morton_axis);
+        v.push_back(m.generation);
+        v.push_back(m.h_vtiles.size());
+        v.push_back(m.h_ttiles.size());
+        for (const DevBuf* b : {&m.src, &m.tri, &m.v64, &m.vi, &m.lo, &m.hi, &m.keys, &m.vals, &m.tkeys, &m.tvals,
+                                &m.nodes, &m.range, &m.blocks, &m.voff, &m.toff, &m.vtiles, &m.ttiles, &m.vtile0,
+                                &m.fbounds, &m.roots})
+            buf(*b);
+    }
+    for (const DevBuf* b : {&ctx->counters, &ctx->bounds, &ctx->xf_dev, &ctx->cand, &ctx->exact, &ctx->rec_key,
+                            &ctx->rec_seg, &ctx->seg_pool, &ctx->bhist[0], &ctx->bhist[1], &ctx->bscan[0],
+                            &ctx->bscan[1], &ctx->bpart[0], &ctx->bpart[1]})
+        buf(*b);
+    v.push_back(ctx->cap_cand);
+    v.push_back(ctx->cap_exact);
+    v.push_back(ctx->cap_rec);
+    v.push_back(ctx->cap_seg);
+    return v;
+}
+
+// With use_graph: the first query of a given signature runs eagerly, the
+// second is captured into a graph (launched at once), later ones replay it.
+// A replay first stages the current transforms in pinned memory (the graph
+// copies them to the device) and applies the host-side effects the capture
+// had (trees marked current). Returns true when the graph was launched.
+bool launch_presync_graph(gi_context* ctx, const gi_options& o) {
+    if (!o.use_graph || ctx->graph_off) return false;
+    for (const MeshSlot& m : ctx->mesh)
+        if (m.set && !m.on_device && !m.borrowed) return false;  // uploads stay outside graphs
+    hipStream_t s = ctx->stream;
+    std::vector<uint64_t> sig = presync_signature(ctx, o);
+    bool need[2];
+    for (int sl = 0; sl < 2; ++sl) need[sl] = slot_needs_stage(ctx->mesh[sl], o.rebuild != 0);
+    if (ctx->graph_exec && sig == ctx->graph_sig) {
+        for (int sl = 0; sl < 2; ++sl)
+            if (need[sl]) ctx->h_xf[sl] = ctx->mesh[sl].xf;
+        HIP_TRY(hipEventRecord(ctx->ev[0], s));
+        HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
+        HIP_TRY(hipEventRecord(ctx->ev[4], s));
+        for (int sl = 0; sl < 2; ++sl)
+            if (need[sl]) {
+                ctx->mesh[sl].built = true;
+                ctx->mesh[sl].dirty = false;
+            }
+        ctx->chunk_timed = false;
+        return true;
+    }
+    if (sig != ctx->graph_sig_seen) {
+        ctx->graph_sig_seen = sig;
+        return false;
+    }
+    ctx->drop_graph();
+    bool was_built[2], was_dirty[2];
+    for (int sl = 0; sl < 2; ++sl) {
+        was_built[sl] = ctx->mesh[sl].built;
+        was_dirty[sl] = ctx->mesh[sl].dirty;
+    }
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->graph_off = true;
+        return false;
+    }
+    bool ok = true;
+    try {
+        enqueue_presync(ctx, o, true, false);
+    } catch (const GiError&) {
+        ok = false;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (!ok || e != hipSuccess || !g ||
+        hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        ctx->graph_exec = nullptr;
+        (void)hipGetLastError();
+        ctx->graph_off = true;
+        // the capture marked the trees current without running anything
+        for (int sl = 0; sl < 2; ++sl) {
+            ctx->mesh[sl].built = was_built[sl];
+            ctx->mesh[sl].dirty = was_dirty[sl];
+        }
+        if (!ok) throw GiError{GI_ERR_HIP, "graph capture of the query failed"};
+        return false;
+    }
+    ctx->graph = g;
+    ctx->graph_sig = sig;
+    ctx->chunk_timed = false;
+    HIP_TRY(hipEventRecord(ctx->ev[0], s));
+    HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
+    HIP_TRY(hipEventRecord(ctx->ev[4], s));
+    return true;
+}
+
 int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     memset(out, 0, sizeof(*out));
     gi_options o = {0, 1, 0, 0};
@@ -400,10 +573,6 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     }
     hipStream_t s = ctx->stream;
     gi_stats& st = out->stats;
-    HIP_TRY(hipEventRecord(ctx->ev[0], s));
-    build_meshes(ctx, o.rebuild != 0);
-    HIP_TRY(hipEventRecord(ctx->ev[1], s));
-
     if (ctx->cap_cand == 0) {
         // first guess 8 candidates per triangle, at most 2^26 (the queues
         // grow on overflow; a 2^26 guess is ~3 GB of arenas with segments)
@@ -414,27 +583,16 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         ensure_arenas(ctx, (uint32_t)c, (uint32_t)std::max<uint64_t>(c / 4, 1024), (uint32_t)std::max<uint64_t>(c / 4, 1024),
                       (uint32_t)std::max<uint64_t>(c / 4, 1024));
     }
-    uint32_t* counters = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);
-    uint32_t* grid = counters + CNT_COUNT;
+    ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);
+    ctx->bounds.ensure<uint32_t>(12);
+    ctx->xf_dev.ensure<Xform>(2);
     int bitsA = bits_for(MA.nt), bitsB = bits_for(MB.nt);
-    uint32_t hc[CNT_COUNT + 2];
-    uint32_t* hgrid = hc + CNT_COUNT;
-    MeshDev A = mesh_dev(MA), B = mesh_dev(MB);
+    const uint32_t* hc = ctx->h_counters;
+    const uint32_t* hgrid = hc + CNT_COUNT;
+    const bool graphed = launch_presync_graph(ctx, o);
+    if (!graphed) enqueue_presync(ctx, o, true, true);
     for (int attempt = 0;; ++attempt) {
-        launch_fill_u32(counters, 0u, CNT_COUNT, s);
-        HIP_TRY(hipEventRecord(ctx->ev[6], s));
-        const uint64_t* keysB = M ? (MB.sorted_in_tmp ? MB.tkeys.get<uint64_t>() : MB.keys.get<uint64_t>()) : nullptr;
-        launch_traverse(A, B, keysB, 3 * MB.morton_axis, M ? MA.roots.get<uint2>() : nullptr, ctx->cand.get<uint2>(),
-                        ctx->cap_cand, counters, s);
-        HIP_TRY(hipEventRecord(ctx->ev[2], s));
-        launch_filter(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, grid, ctx->exact.get<uint2>(), ctx->cap_exact,
-                      counters, s);
-        HIP_TRY(hipEventRecord(ctx->ev[3], s));
-        launch_exact(A, B, ctx->exact.get<uint2>(), ctx->cap_exact, grid, bitsB, ctx->rec_key.get<uint64_t>(),
-                     ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<__int128>(), ctx->cap_seg, counters, s);
-        HIP_TRY(hipEventRecord(ctx->ev[4], s));
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+        if (attempt) enqueue_presync(ctx, o, false, true);  // trees are current: queries only
         HIP_TRY(hipStreamSynchronize(s));
         bool over = hc[CNT_TRAVERSAL_HITS] > ctx->cap_cand || hc[CNT_EXACT_LIST] > ctx->cap_exact ||
                     hc[CNT_RESULTS] > ctx->cap_rec || hc[CNT_SEGMENTS] > ctx->cap_seg;
@@ -447,6 +605,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         ensure_arenas(ctx, nc, ne, nr, ns);
         st.retries++;
     }
+    MeshDev A = mesh_dev(MA), B = mesh_dev(MB);
     if (hc[CNT_UNSOUND]) throw GiError{GI_ERR_INTERNAL, "FP64 filter certified a pair the exact stage rejects"};
     st.traversal_hits = hc[CNT_TRAVERSAL_HITS];
     st.candidates = hc[CNT_CANDIDATES];
@@ -454,14 +613,16 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     st.filter_green = hc[CNT_FILTER_GREEN];
     st.filter_yellow = hc[CNT_FILTER_YELLOW];
     st.grid = hgrid[0] && hgrid[1];
-    st.ms_build = elapsed(ctx->ev[0], ctx->ev[1]);
-    if (ctx->chunk_timed) {
-        st.ms_bvh_chunks = elapsed(ctx->ev_chunk[0], ctx->ev_chunk[1]);
-        st.bvh_chunk_launches = 1;
-        st.bvh_chunk_leaves = ctx->chunk_leaves;
+    if (!graphed) {  // a replayed graph records no stage events: only the totals are timed
+        st.ms_build = elapsed(ctx->ev[0], ctx->ev[1]);
+        if (ctx->chunk_timed) {
+            st.ms_bvh_chunks = elapsed(ctx->ev_chunk[0], ctx->ev_chunk[1]);
+            st.bvh_chunk_launches = 1;
+            st.bvh_chunk_leaves = ctx->chunk_leaves;
+        }
+        st.ms_traverse = elapsed(ctx->ev[6], ctx->ev[2]);
+        st.ms_filter = elapsed(ctx->ev[2], ctx->ev[3]);
     }
-    st.ms_traverse = elapsed(ctx->ev[6], ctx->ev[2]);
-    st.ms_filter = elapsed(ctx->ev[2], ctx->ev[3]);
 
     if (st.grid) {
         st.exact_pairs = hc[CNT_EXACT_LIST];
@@ -469,7 +630,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         st.exact_yellow_no = hc[CNT_EXACT_YELLOW_NO];
         st.coplanar = hc[CNT_COPLANAR];
         st.degenerate = hc[CNT_DEGENERATE];
-        st.ms_exact = elapsed(ctx->ev[3], ctx->ev[4]);
+        if (!graphed) st.ms_exact = elapsed(ctx->ev[3], ctx->ev[4]);
         uint32_t R = hc[CNT_RESULTS], S = o.want_segments ? hc[CNT_SEGMENTS] : 0;
         uint64_t* keys = ctx->rec_key.get<uint64_t>();
         uint32_t* order = ctx->order.ensure<uint32_t>(R);
@@ -720,6 +881,9 @@ int gi_create(const gi_config* cfg, gi_context** out) {
         ctx->build_stream[0] = ctx->stream;
         ctx->build_stream[1] = ctx->side;
         HIP_TRY(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+        HIP_TRY(hipHostMalloc((void**)&ctx->h_xf, 2 * sizeof(Xform), hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void**)&ctx->h_counters, sizeof(uint32_t) * (CNT_COUNT + 2), hipHostMallocDefault));
+        memset(ctx->h_counters, 0, sizeof(uint32_t) * (CNT_COUNT + 2));
         HIP_TRY(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
         for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
         for (auto& e : ctx->ev_chunk) HIP_TRY(hipEventCreate(&e));
@@ -744,6 +908,8 @@ int gi_destroy(gi_context* ctx) {
     for (auto& e : ctx->ev_chunk)
         if (e) (void)hipEventDestroy(e);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->h_xf) (void)hipHostFree(ctx->h_xf);
+    if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -773,6 +939,7 @@ int gi_set_mesh(gi_context* ctx, int slot, const double* xyz, uint32_t nv, const
         HIP_TRY(hipSetDevice(ctx->device));
         MeshSlot& m = ctx->mesh[slot];
         if (m.borrowed) m.release();
+        ++m.generation;
         m.forest = 0;
         m.h_xyz.assign(xyz, xyz + 3 * (size_t)nv);
         m.h_tri.assign(tri, tri + 3 * (size_t)nt);
@@ -796,6 +963,7 @@ int gi_set_mesh_device(gi_context* ctx, int slot, const double* d_xyz, uint32_t 
         HIP_TRY(hipSetDevice(ctx->device));
         MeshSlot& m = ctx->mesh[slot];
         if (m.borrowed) m.release();
+        ++m.generation;
         m.forest = 0;
         m.nv = nv;
         m.nt = nt;
@@ -920,7 +1088,7 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         sync_all(src);
         MeshSlot& d = dst->mesh[dst_slot];
         const MeshSlot& s = src->mesh[src_slot];
-        d.release();
+        d.release();  // bumps the generation
         d.h_xyz = s.h_xyz;
         d.h_tri = s.h_tri;
         d.nv = s.nv;

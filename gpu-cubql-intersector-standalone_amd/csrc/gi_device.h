@@ -58,8 +58,10 @@ void launch_init_mesh_scalars(uint32_t* grid_flag, uint32_t* bounds6, hipStream_
 // bounds6 receives the mesh bounds (3 min, 3 max order-preserving float
 // keys); partials needs 6 * kPrepareMaxBlocks floats.
 constexpr uint32_t kPrepareMaxBlocks = 4096;
-void launch_prepare_vertices(const double* src, uint32_t nv, Xform xf, double4* v64, int4* vi, uint32_t* grid_flag,
-                             float* partials, uint32_t* bounds6, hipStream_t s);
+// xf: the transform in device memory (so that a captured launch sequence
+// picks up a new transform from the copy that precedes it)
+void launch_prepare_vertices(const double* src, uint32_t nv, const Xform* xf, double4* v64, int4* vi,
+                             uint32_t* grid_flag, float* partials, uint32_t* bounds6, hipStream_t s);
 // Triangle boxes; with keys/vals non-null also Morton keys (frame bounds6)
 // and the identity permutation.
 void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,

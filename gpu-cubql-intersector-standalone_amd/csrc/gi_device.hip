@@ -156,9 +156,11 @@ __device__ __forceinline__ bool prepare_vertex(const double* __restrict__ src, u
 
 // Moves the vertices (when a transform is set), classifies the mesh as grid
 // or not, writes the integer copy and per-block partial bounds (Morton frame).
-__global__ __launch_bounds__(256) void k_prepare_vertices(const double* __restrict__ src, uint32_t nv, Xform xf,
-                                                          double4* __restrict__ v64, int4* __restrict__ vi,
-                                                          uint32_t* grid_flag, float* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_prepare_vertices(const double* __restrict__ src, uint32_t nv,
+                                                          const Xform* __restrict__ xfp, double4* __restrict__ v64,
+                                                          int4* __restrict__ vi, uint32_t* grid_flag,
+                                                          float* __restrict__ part) {
+    const Xform xf = *xfp;
     bool grid = true;
     float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x)
@@ -168,8 +170,8 @@ __global__ __launch_bounds__(256) void k_prepare_vertices(const double* __restri
     block_box_partial(l, h, part + 6 * (size_t)blockIdx.x);
 }
 
-void launch_prepare_vertices(const double* src, uint32_t nv, Xform xf, double4* v64, int4* vi, uint32_t* grid_flag,
-                             float* partials, uint32_t* bounds6, hipStream_t s) {
+void launch_prepare_vertices(const double* src, uint32_t nv, const Xform* xf, double4* v64, int4* vi,
+                             uint32_t* grid_flag, float* partials, uint32_t* bounds6, hipStream_t s) {
     if (nv == 0) return;
     uint32_t blocks = std::min<uint32_t>(blocks_for(nv, 256), kPrepareMaxBlocks);
     k_prepare_vertices<<<blocks, 256, 0, s>>>(src, nv, xf, v64, vi, grid_flag, partials);
@@ -970,8 +972,7 @@ __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J) {
     const uint32_t n = second ? J.n1 : J.n0;
     const uint2* __restrict__ range = second ? J.range1 : J.range0;
     float4* nodes = second ? J.nodes1 : J.nodes0;
-    BlockLevels lv;
-    for (int k = 0; k < kMaxBlockLevels; ++k) lv.level[k] = second ? J.lv1.level[k] : J.lv0.level[k];
+    const BlockLevels& lv = second ? J.lv1 : J.lv0;  // read in place from the kernel arguments
     uint32_t base = ((blockIdx.x - (second ? J.split : 0u)) * 4 + (threadIdx.x >> 6)) * 64;
     uint32_t i = base + lane_id();
     bool top = false;

@@ -24,7 +24,7 @@ class GiConfig(C.Structure):
 
 class GiOptions(C.Structure):
     _fields_ = [("output_on_device", C.c_int32), ("want_segments", C.c_int32), ("rebuild", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("use_graph", C.c_int32)]
 
 
 _STAT_U64 = ["traversal_hits", "candidates", "filter_no", "filter_green", "filter_yellow", "exact_pairs",
@@ -230,15 +230,15 @@ class Intersector:
     def reconstruct_device(self):
         _check(lib().gi_reconstruct_device(self._ctx), self._ctx)
 
-    def intersect_raw(self, output_on_device=False, want_segments=True, rebuild=False) -> GiResult:
+    def intersect_raw(self, output_on_device=False, want_segments=True, rebuild=False, use_graph=False) -> GiResult:
         """Runs the query; the caller must gi_result_free the returned struct."""
         res = GiResult()
-        opt = GiOptions(int(output_on_device), int(want_segments), int(rebuild), 0)
+        opt = GiOptions(int(output_on_device), int(want_segments), int(rebuild), int(use_graph))
         _check(lib().gi_intersect(self._ctx, C.byref(opt), C.byref(res)), self._ctx)
         return res
 
-    def intersect(self, want_segments=True, rebuild=False, decode=True):
-        res = self.intersect_raw(False, want_segments, rebuild)
+    def intersect(self, want_segments=True, rebuild=False, decode=True, use_graph=False):
+        res = self.intersect_raw(False, want_segments, rebuild, use_graph)
         try:
             return result_to_py(res, want_segments, decode)
         finally:

@@ -80,7 +80,9 @@ typedef struct {
                                    the context until its next gi_intersect */
     int32_t want_segments;      /* 0: pairs only */
     int32_t rebuild;            /* 1: rebuild both BVHs from scratch before the query */
-    int32_t reserved;
+    int32_t use_graph;          /* 1: replay the query's launches up to the counter read-back as a
+                                   HIP graph while sizes and buffers stay the same (captured on
+                                   the second such query); stage timings are then not reported */
 } gi_options;
 
 typedef struct {
