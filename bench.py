@@ -172,7 +172,7 @@ def serving_leg(gi, mg, n_requests, qps, slots):
     }
 
 
-def batch_leg(gi, mg, n_pairs, steps):
+def batch_leg(gi, mg, n_pairs, steps, autotune=True):
     """Batched workload: n_pairs independent mesh pairs (seed 6), each side an
     icosphere, displaced sphere or torus with N log-uniform in [500, 200000],
     B placed at U[0.3, 1.5] radii from A, 2^24 grid. Batched = one pass of
@@ -191,6 +191,7 @@ def batch_leg(gi, mg, n_pairs, steps):
     t0 = time.perf_counter()
     eng.set_batch(pairs)
     set_s = time.perf_counter() - t0
+    tuning = eng.autotune(reps=1, rebuild=True) if autotune else eng.get_tuning()
     for _ in range(2):  # warm-up: arenas grow to their final sizes here
         res, offs = eng.intersect_batch_raw(output_on_device=True, want_segments=True, rebuild=True)
         free(byref(res))
@@ -244,6 +245,7 @@ def batch_leg(gi, mg, n_pairs, steps):
         "speedup_e2e": plain_s / e2e_batch_s,
         "candidates": st["candidates"], "intersecting_pairs": st["pairs"], "segments": st["segments"],
         "parity_with_plain": parity,
+        "tuning": tuning,
         "generate_s": gen_s, "set_batch_s": set_s,
     }
 
@@ -258,6 +260,7 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement")
     ap.add_argument("--batch-pairs", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=5)
@@ -281,7 +284,8 @@ def main():
     gi = ge.package_module("gi_ctypes")
     mg = ge.package_module("meshgen")
     if args.batch_only:
-        print(json.dumps({"batched": batch_leg(gi, mg, args.batch_pairs, args.batch_steps)}), flush=True)
+        print(json.dumps({"batched": batch_leg(gi, mg, args.batch_pairs, args.batch_steps, not args.no_autotune)}),
+              flush=True)
         return
     (xa, ta), (xb, tb) = mg.sphere_torus(args.res, args.res // 2)
     dev = torch.device("cuda:0")
@@ -294,6 +298,8 @@ def main():
     eng = gi.Intersector(device=0)
     eng.set_mesh_device(0, dxa.data_ptr(), xa.shape[0], dta.data_ptr(), ta.shape[0])
     eng.set_mesh_device(1, dxb.data_ptr(), xb.shape[0], dtb.data_ptr(), tb.shape[0])
+
+    tuning = eng.get_tuning() if args.no_autotune else eng.autotune(reps=3, rebuild=True)
 
     def step():
         res = eng.intersect_raw(output_on_device=True, want_segments=True, rebuild=True)
@@ -347,6 +353,7 @@ def main():
                                "both BVH builds + query + exact segments per step",
                    "triangles_a": int(ta.shape[0]), "triangles_b": int(tb.shape[0]),
                    "vertices_a": int(xa.shape[0]), "vertices_b": int(xb.shape[0])},
+        "tuning": tuning,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_ms": kernel_ms, "launches_timed": launches, "leaves_per_launch": leaves_per_launch,
@@ -376,7 +383,7 @@ def main():
         print(f"copy calibration: {args.copy_calib} MiB read + {args.copy_calib} MiB written per dispatch",
               file=sys.stderr)
     if not args.no_batch:
-        line["batched"] = batch_leg(gi, mg, args.batch_pairs, args.batch_steps)
+        line["batched"] = batch_leg(gi, mg, args.batch_pairs, args.batch_steps, not args.no_autotune)
     if not args.no_serve:
         line["serving"] = serving_leg(gi, mg, args.serve_requests, args.serve_qps, args.serve_slots)
     if not args.no_cpu_baseline:

@@ -124,6 +124,7 @@ struct gi_context {
     uint32_t cap_cand = 0, cap_exact = 0, cap_rec = 0, cap_seg = 0;
     DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2, pair_off;
     std::vector<uint32_t> h_pair_off;  // per mesh pair result offsets of the last batch query
+    gi_tuning tune = {2, 2048, 1024, (int32_t)kBuildChunk};  // launch shapes (gi_autotune)
     hipEvent_t ev[8] = {};
     // pinned host staging: per-slot transforms (copied to xf_dev before the
     // vertex kernels) and the counter read-back; graph replays read/write them
@@ -342,7 +343,8 @@ void build_meshes(gi_context* ctx, bool rebuild, bool timed = true) {
         uint32_t leaves = 0;
         for (int j = 0; j < nj; ++j) leaves += jobs[j].n;
         bool time_it = timed && !ctx->chunk_timed && leaves > 0;
-        launch_build_bvh(jobs, nj, s, time_it ? ctx->ev_chunk[0] : nullptr, time_it ? ctx->ev_chunk[1] : nullptr);
+        launch_build_bvh(jobs, nj, (uint32_t)ctx->tune.build_chunk, s, time_it ? ctx->ev_chunk[0] : nullptr,
+                         time_it ? ctx->ev_chunk[1] : nullptr);
         if (time_it) {
             ctx->chunk_timed = true;
             ctx->chunk_leaves = leaves;
@@ -433,13 +435,14 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
     if (timed) HIP_TRY(hipEventRecord(ctx->ev[6], s));
     const uint64_t* keysB = M ? (MB.sorted_in_tmp ? MB.tkeys.get<uint64_t>() : MB.keys.get<uint64_t>()) : nullptr;
     launch_traverse(A, B, keysB, 3 * MB.morton_axis, M ? MA.roots.get<uint2>() : nullptr, ctx->cand.get<uint2>(),
-                    ctx->cap_cand, counters, s);
+                    ctx->cap_cand, counters, ctx->tune.traverse_variant, s);
     if (timed) HIP_TRY(hipEventRecord(ctx->ev[2], s));
     launch_filter(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, grid, ctx->exact.get<uint2>(), ctx->cap_exact, counters,
-                  s);
+                  ctx->tune.filter_blocks, s);
     if (timed) HIP_TRY(hipEventRecord(ctx->ev[3], s));
     launch_exact(A, B, ctx->exact.get<uint2>(), ctx->cap_exact, grid, bitsB, ctx->rec_key.get<uint64_t>(),
-                 ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<__int128>(), ctx->cap_seg, counters, s);
+                 ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<__int128>(), ctx->cap_seg, counters,
+                 ctx->tune.exact_blocks, s);
     if (timed) HIP_TRY(hipEventRecord(ctx->ev[4], s));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(ctx->h_counters, counters, sizeof(uint32_t) * (CNT_COUNT + 2), hipMemcpyDeviceToHost, s));
@@ -477,6 +480,10 @@ std::vector<uint64_t> presync_signature(gi_context* ctx, const gi_options& o) {
                             &ctx->rec_seg, &ctx->seg_pool, &ctx->bhist[0], &ctx->bhist[1], &ctx->bscan[0],
                             &ctx->bscan[1], &ctx->bpart[0], &ctx->bpart[1]})
         buf(*b);
+    v.push_back((uint64_t)ctx->tune.traverse_variant);
+    v.push_back((uint64_t)ctx->tune.filter_blocks);
+    v.push_back((uint64_t)ctx->tune.exact_blocks);
+    v.push_back((uint64_t)ctx->tune.build_chunk);
     v.push_back(ctx->cap_cand);
     v.push_back(ctx->cap_exact);
     v.push_back(ctx->cap_rec);
@@ -1224,6 +1231,82 @@ int gi_intersect_batch(gi_context* ctx, const gi_options* opt, gi_result* out, u
         if (pair_offsets)
             for (size_t i = 0; i < ctx->h_pair_off.size(); ++i) pair_offsets[i] = ctx->h_pair_off[i];
         return st;
+    } catch (const GiError& e) {
+        return fail(ctx, e);
+    }
+}
+
+int gi_get_tuning(const gi_context* ctx, gi_tuning* out) {
+    if (!ctx || !out) return GI_ERR_INVALID_ARGUMENT;
+    *out = ctx->tune;
+    return GI_OK;
+}
+
+int gi_set_tuning(gi_context* ctx, const gi_tuning* t) {
+    if (!ctx || !t) return GI_ERR_INVALID_ARGUMENT;
+    if (t->traverse_variant < 0 || t->traverse_variant > 3 || t->filter_blocks < 1 || t->filter_blocks > 65535 ||
+        t->exact_blocks < 1 || t->exact_blocks > 65535 || (t->build_chunk != 256 && t->build_chunk != 512)) {
+        ctx->err = "tuning out of range";
+        return GI_ERR_INVALID_ARGUMENT;
+    }
+    ctx->tune = *t;
+    return GI_OK;
+}
+
+// Coordinate search over the launch shapes on the context's current meshes:
+// for each knob in turn (build chunk, traversal shape, filter width, exact
+// width), every candidate value is timed on `reps` queries by the event
+// timing of the stage it affects (median), and the fastest is kept. Results
+// do not depend on these settings, only the time does.
+int gi_autotune(gi_context* ctx, const gi_options* opt, int32_t reps, gi_tuning* out) {
+    if (!ctx) return GI_ERR_INVALID_ARGUMENT;
+    if (!ctx->mesh[0].set || !ctx->mesh[1].set) return GI_ERR_NOT_READY;
+    if (reps <= 0) reps = 3;
+    gi_options o = {1, 1, 0, 0};
+    if (opt) o = *opt;
+    o.output_on_device = 1;
+    o.use_graph = 0;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        auto query = [&](bool rebuild) {
+            gi_result res;
+            gi_options q = o;
+            q.rebuild = rebuild ? 1 : o.rebuild;
+            run_intersect(ctx, &q, &res);
+            gi_stats st = res.stats;
+            gi_result_free(&res);
+            return st;
+        };
+        auto median_ms = [&](int stage) {
+            std::vector<float> v;
+            for (int r = 0; r < reps; ++r) {
+                gi_stats st = query(stage == 0);
+                v.push_back(stage == 0 ? st.ms_build : stage == 1 ? st.ms_traverse : stage == 2 ? st.ms_filter
+                                                                                                : st.ms_exact);
+            }
+            std::sort(v.begin(), v.end());
+            return v[v.size() / 2];
+        };
+        query(true);  // arenas reach their sizes
+        auto pick = [&](int32_t* knob, const std::vector<int32_t>& values, int stage) {
+            int32_t best = *knob;
+            float best_ms = 1e30f;
+            for (int32_t val : values) {
+                *knob = val;
+                float ms = median_ms(stage);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = val;
+                }
+            }
+            *knob = best;
+        };
+        pick(&ctx->tune.build_chunk, {256, 512}, 0);
+        pick(&ctx->tune.traverse_variant, {0, 1, 2, 3}, 1);
+        pick(&ctx->tune.filter_blocks, {512, 1024, 2048, 4096, 8192}, 2);
+        pick(&ctx->tune.exact_blocks, {256, 512, 1024, 2048, 4096}, 3);
+        if (out) *out = ctx->tune;
+        return GI_OK;
     } catch (const GiError& e) {
         return fail(ctx, e);
     }

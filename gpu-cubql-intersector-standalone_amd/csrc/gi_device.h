@@ -102,10 +102,7 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t*
 // subtree); range[i] = its leaf range. `blocks` needs block_levels_elems(n)
 // float4. Running it again on the same sorted keys with new prim boxes is the
 // refit after a rigid motion (same topology, new boxes).
-#ifndef GI_BUILD_CHUNK
-#define GI_BUILD_CHUNK 512
-#endif
-constexpr uint32_t kBuildChunk = GI_BUILD_CHUNK;  // sorted leaves per build workgroup (multiple of 256)
+constexpr uint32_t kBuildChunk = 512;  // default sorted leaves per build workgroup (256 or 512)
 constexpr int kMaxBlockLevels = 6;
 struct BlockLevels {
     const float4* level[kMaxBlockLevels];
@@ -127,24 +124,27 @@ struct BuildJob {
 // Builds one or two trees with one launch per kernel (the blocks of a launch
 // are split between the trees); chunk_begin/chunk_end (may be null) bracket
 // the k_build_chunks launch.
-void launch_build_bvh(const BuildJob<uint32_t>* jobs, int njobs, hipStream_t s, hipEvent_t chunk_begin,
-                      hipEvent_t chunk_end);
-void launch_build_bvh(const BuildJob<uint64_t>* jobs, int njobs, hipStream_t s, hipEvent_t chunk_begin,
-                      hipEvent_t chunk_end);
+void launch_build_bvh(const BuildJob<uint32_t>* jobs, int njobs, uint32_t chunk, hipStream_t s,
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end);
+void launch_build_bvh(const BuildJob<uint64_t>* jobs, int njobs, uint32_t chunk, hipStream_t s,
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end);
 // roots[m] = (subtree root of mesh m, rope leaving it)
 void launch_forest_roots(const uint2* range, const float4* nodes, uint32_t n, const uint32_t* toff, uint32_t M,
                          uint2* roots, hipStream_t s);
 
 // keysB/rootsA null: whole trees; else forests (sorted B keys whose mesh id
 // starts at bit mesh_shift, A mesh roots)
+// variant: launch shape of the traversal, 0..3 = (workgroup size, candidate
+// pairs staged per wave) (64, 256), (128, 256), (256, 256), (256, 128)
 void launch_traverse(MeshDev A, MeshDev B, const uint64_t* keysB, int mesh_shift, const uint2* rootsA, uint2* cand,
-                     uint32_t cap, uint32_t* counters, hipStream_t s);
+                     uint32_t cap, uint32_t* counters, int variant, hipStream_t s);
+// blocks: workgroups of the grid-stride launch
 void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const uint32_t* grid_flags,
-                   uint2* exact_list, uint32_t exact_cap, uint32_t* counters, hipStream_t s);
+                   uint2* exact_list, uint32_t exact_cap, uint32_t* counters, int blocks, hipStream_t s);
 void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
                   int bitsB,
                   uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, __int128* seg_pool, uint32_t seg_cap,
-                  uint32_t* counters, hipStream_t s);
+                  uint32_t* counters, int blocks, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
 void launch_gather_pairs(const uint64_t* sorted_key, const uint32_t* order, const uint2* rec_seg, uint32_t n,
                          int bitsB, uint2* pairs, uint32_t* nseg, hipStream_t s);

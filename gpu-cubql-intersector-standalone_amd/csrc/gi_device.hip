@@ -741,7 +741,7 @@ int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t*
 
 // --------------------------------------------------------- BVH (Karras)
 //
-// Build and refit are one kernel per chunk of kBuildChunk sorted leaves:
+// Build and refit are one kernel per chunk of CHUNK (256 or 512) sorted leaves:
 // the chunk's Morton keys sit in LDS for Karras' prefix searches, every
 // internal node whose leaf range lies inside the chunk is refit bottom-up
 // through LDS only (parents, merge flags and merged boxes never leave LDS,
@@ -801,7 +801,7 @@ struct ChunkJobs {
     uint32_t split;
 };
 
-template <class K>
+template <class K, int CHUNK>
 __global__ __launch_bounds__(256) void k_build_chunks(ChunkJobs<K> J) {
     const bool second = blockIdx.x >= J.split;
     const K* __restrict__ keys = second ? J.keys1 : J.keys0;
@@ -812,16 +812,16 @@ __global__ __launch_bounds__(256) void k_build_chunks(ChunkJobs<K> J) {
     uint2* __restrict__ range = second ? J.range1 : J.range0;
     const int n = second ? J.n1 : J.n0;
     const uint32_t chunk = blockIdx.x - (second ? J.split : 0u);
-    __shared__ K skey[kBuildChunk];
-    __shared__ uint32_t par_leaf[kBuildChunk];
-    __shared__ uint32_t par_int[kBuildChunk];
-    __shared__ uint32_t left_int[kBuildChunk];
-    __shared__ uint32_t rope_int[kBuildChunk];
-    __shared__ uint32_t flag[kBuildChunk];
-    __shared__ uint32_t ubox[kBuildChunk][6];  // merged boxes as order-preserving uint keys
-    const int cs = (int)chunk * (int)kBuildChunk;
-    const int ce = min(n, cs + (int)kBuildChunk);
-    for (int k = threadIdx.x; k < (int)kBuildChunk; k += blockDim.x) {
+    __shared__ K skey[CHUNK];
+    __shared__ uint32_t par_leaf[CHUNK];
+    __shared__ uint32_t par_int[CHUNK];
+    __shared__ uint32_t left_int[CHUNK];
+    __shared__ uint32_t rope_int[CHUNK];
+    __shared__ uint32_t flag[CHUNK];
+    __shared__ uint32_t ubox[CHUNK][6];  // merged boxes as order-preserving uint keys
+    const int cs = (int)chunk * (int)CHUNK;
+    const int ce = min(n, cs + (int)CHUNK);
+    for (int k = threadIdx.x; k < (int)CHUNK; k += blockDim.x) {
         if (cs + k < ce) skey[k] = keys[cs + k];
         par_leaf[k] = kSentinel;
         par_int[k] = kSentinel;
@@ -868,8 +868,8 @@ __global__ __launch_bounds__(256) void k_build_chunks(ChunkJobs<K> J) {
             nodes[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rope));
         }
     }
-    float lb[kBuildChunk / 256][6];
-    for (int u = 0; u < (int)(kBuildChunk / 256); ++u) {
+    float lb[CHUNK / 256][6];
+    for (int u = 0; u < (int)(CHUNK / 256); ++u) {
         int jl = cs + threadIdx.x + 256 * u;
         if (jl >= ce) continue;
         uint32_t prim = prims[jl];
@@ -881,7 +881,7 @@ __global__ __launch_bounds__(256) void k_build_chunks(ChunkJobs<K> J) {
         lb[u][3] = h.x; lb[u][4] = h.y; lb[u][5] = h.z;
     }
     __syncthreads();
-    for (int u = 0; u < (int)(kBuildChunk / 256); ++u) {
+    for (int u = 0; u < (int)(CHUNK / 256); ++u) {
         int jl = cs + threadIdx.x + 256 * u;
         if (jl >= ce) continue;
         float b[6];
@@ -967,7 +967,7 @@ struct TopBoxJobs {
     uint32_t split;
 };
 
-__global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J) {
+__global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J, uint32_t chunk) {
     const bool second = blockIdx.x >= J.split;
     const uint32_t n = second ? J.n1 : J.n0;
     const uint2* __restrict__ range = second ? J.range1 : J.range0;
@@ -978,7 +978,7 @@ __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J) {
     bool top = false;
     if (i + 1 < n) {
         uint2 r = range[i];
-        top = r.x / kBuildChunk != r.y / kBuildChunk;
+        top = r.x / chunk != r.y / chunk;
     }
     uint64_t m = __ballot(top);
     while (m) {
@@ -1001,7 +1001,7 @@ size_t block_levels_elems(uint32_t n) {
 }
 
 template <class K>
-static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, hipStream_t s, hipEvent_t chunk_begin,
+static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, hipStream_t s, hipEvent_t chunk_begin,
                            hipEvent_t chunk_end) {
     const BuildJob<K>* j0 = &jobs[0];
     const BuildJob<K>* j1 = njobs > 1 ? &jobs[1] : nullptr;
@@ -1013,10 +1013,11 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, hipStream_t s, hi
     const BuildJob<K>* o = j1 ? j1 : j0;
     cj.keys1 = o->keys; cj.prims1 = o->prims; cj.lo1 = o->lo; cj.hi1 = o->hi;
     cj.nodes1 = o->nodes; cj.range1 = o->range; cj.n1 = (int)n1;
-    cj.split = blocks_for(n0, kBuildChunk);
-    const uint32_t chunk_blocks = cj.split + blocks_for(n1, kBuildChunk);
+    cj.split = blocks_for(n0, chunk);
+    const uint32_t chunk_blocks = cj.split + blocks_for(n1, chunk);
     if (chunk_begin) (void)hipEventRecord(chunk_begin, s);
-    k_build_chunks<K><<<chunk_blocks, 256, 0, s>>>(cj);
+    if (chunk == 256) k_build_chunks<K, 256><<<chunk_blocks, 256, 0, s>>>(cj);
+    else k_build_chunks<K, 512><<<chunk_blocks, 256, 0, s>>>(cj);
     if (chunk_end) (void)hipEventRecord(chunk_end, s);
     // 64-ary levels of leaf boxes for the nodes that cross chunks
     BlockLevels lv[2];
@@ -1051,17 +1052,17 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, hipStream_t s, hi
     // one wave per 64 internal nodes (n - 1 of them); a one-leaf tree has none
     tj.split = n0 > 1 ? blocks_for(blocks_for(n0 - 1, 64), 4) : 0;
     uint32_t top_blocks = tj.split + (n1 > 1 ? blocks_for(blocks_for(n1 - 1, 64), 4) : 0);
-    if (top_blocks) k_top_boxes<<<top_blocks, 256, 0, s>>>(tj);
+    if (top_blocks) k_top_boxes<<<top_blocks, 256, 0, s>>>(tj, chunk);
     GI_LAUNCH_CHECK();
 }
 
-void launch_build_bvh(const BuildJob<uint32_t>* jobs, int njobs, hipStream_t s, hipEvent_t chunk_begin,
-                      hipEvent_t chunk_end) {
-    build_bvh_impl<uint32_t>(jobs, njobs, s, chunk_begin, chunk_end);
+void launch_build_bvh(const BuildJob<uint32_t>* jobs, int njobs, uint32_t chunk, hipStream_t s,
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end) {
+    build_bvh_impl<uint32_t>(jobs, njobs, chunk, s, chunk_begin, chunk_end);
 }
-void launch_build_bvh(const BuildJob<uint64_t>* jobs, int njobs, hipStream_t s, hipEvent_t chunk_begin,
-                      hipEvent_t chunk_end) {
-    build_bvh_impl<uint64_t>(jobs, njobs, s, chunk_begin, chunk_end);
+void launch_build_bvh(const BuildJob<uint64_t>* jobs, int njobs, uint32_t chunk, hipStream_t s,
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end) {
+    build_bvh_impl<uint64_t>(jobs, njobs, chunk, s, chunk_begin, chunk_end);
 }
 
 // Subtree of each forest mesh: the node whose leaf range is exactly the
@@ -1096,8 +1097,6 @@ void launch_forest_roots(const uint2* range, const float4* nodes, uint32_t n, co
 
 // --------------------------------------------------------------- traversal
 
-constexpr int TRAV_THREADS = 256;
-constexpr uint32_t TRAV_STAGE = 256;  // candidate pairs staged per wave in LDS
 
 __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t fill, uint2* cand, uint32_t cap,
                                             uint32_t* counter) {
@@ -1115,9 +1114,10 @@ __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t fill, uint2* 
 // One thread per leaf of B (in B's Morton order, so a wave queries a compact
 // cluster), stackless traversal of A along the ropes. Overlapping leaf pairs
 // are compacted per wave (ballot + popcount) into LDS and flushed to the
-// global candidate queue with one atomic per TRAV_STAGE pairs.
+// global candidate queue with one atomic per STAGE pairs (the queue chunk).
 // Forest (batch): leaf k of B belongs to mesh keysB[k] >> mesh_shift and walks only
 // the subtree of the same mesh of A, from its root to the rope leaving it.
+template <int TRAV_THREADS, uint32_t TRAV_STAGE>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restrict__ nodesA, uint32_t nA,
                                                            const float4* __restrict__ nodesB, uint32_t nB,
                                                            const uint64_t* __restrict__ keysB, int mesh_shift,
@@ -1172,11 +1172,22 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
     if (fill) flush_stage(stage, fill, cand, cap, &counters[CNT_TRAVERSAL_HITS]);
 }
 
+template <int T, uint32_t S>
+static void traverse_as(MeshDev A, MeshDev B, const uint64_t* keysB, int mesh_shift, const uint2* rootsA,
+                        uint2* cand, uint32_t cap, uint32_t* counters, hipStream_t s) {
+    k_traverse<T, S><<<blocks_for(B.nt, T), T, 0, s>>>(A.nodes, A.nt, B.nodes, B.nt, keysB, mesh_shift, rootsA, cand,
+                                                       cap, counters);
+}
+
 void launch_traverse(MeshDev A, MeshDev B, const uint64_t* keysB, int mesh_shift, const uint2* rootsA, uint2* cand,
-                     uint32_t cap, uint32_t* counters, hipStream_t s) {
+                     uint32_t cap, uint32_t* counters, int variant, hipStream_t s) {
     if (A.nt == 0 || B.nt == 0) return;
-    k_traverse<<<blocks_for(B.nt, TRAV_THREADS), TRAV_THREADS, 0, s>>>(A.nodes, A.nt, B.nodes, B.nt, keysB, mesh_shift,
-                                                                       rootsA, cand, cap, counters);
+    switch (variant) {  // (workgroup size, candidate pairs staged per wave)
+        case 0: traverse_as<64, 256>(A, B, keysB, mesh_shift, rootsA, cand, cap, counters, s); break;
+        case 1: traverse_as<128, 256>(A, B, keysB, mesh_shift, rootsA, cand, cap, counters, s); break;
+        case 3: traverse_as<256, 128>(A, B, keysB, mesh_shift, rootsA, cand, cap, counters, s); break;
+        default: traverse_as<256, 256>(A, B, keysB, mesh_shift, rootsA, cand, cap, counters, s); break;
+    }
     GI_LAUNCH_CHECK();
 }
 
@@ -1266,9 +1277,9 @@ __global__ __launch_bounds__(256) void k_filter(MeshDev A, MeshDev B, const uint
 }
 
 void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const uint32_t* grid_flags,
-                   uint2* exact_list, uint32_t exact_cap, uint32_t* counters, hipStream_t s) {
+                   uint2* exact_list, uint32_t exact_cap, uint32_t* counters, int blocks, hipStream_t s) {
     if (A.nt == 0 || B.nt == 0) return;
-    k_filter<<<2048, 256, 0, s>>>(A, B, cand, cap, grid_flags, exact_list, exact_cap, counters);
+    k_filter<<<blocks, 256, 0, s>>>(A, B, cand, cap, grid_flags, exact_list, exact_cap, counters);
     GI_LAUNCH_CHECK();
 }
 
@@ -1354,9 +1365,9 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
 
 void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
                   int bitsB, uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, __int128* seg_pool,
-                  uint32_t seg_cap, uint32_t* counters, hipStream_t s) {
+                  uint32_t seg_cap, uint32_t* counters, int blocks, hipStream_t s) {
     if (A.nt == 0 || B.nt == 0) return;
-    k_exact<<<1024, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap, seg_pool,
+    k_exact<<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap, seg_pool,
                                  seg_cap, counters);
     GI_LAUNCH_CHECK();
 }

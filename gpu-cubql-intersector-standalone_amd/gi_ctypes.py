@@ -55,6 +55,14 @@ class GiResult(C.Structure):
                 ("seg_f64", C.POINTER(C.c_double)), ("stats", GiStats), ("internal", C.c_void_p)]
 
 
+class GiTuning(C.Structure):
+    _fields_ = [("traverse_variant", C.c_int32), ("filter_blocks", C.c_int32), ("exact_blocks", C.c_int32),
+                ("build_chunk", C.c_int32)]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
 class GiMeshPair(C.Structure):
     _fields_ = [("xyzA", C.POINTER(C.c_double)), ("triA", C.POINTER(C.c_uint32)), ("nvA", C.c_uint32),
                 ("ntA", C.c_uint32), ("xyzB", C.POINTER(C.c_double)), ("triB", C.POINTER(C.c_uint32)),
@@ -90,7 +98,8 @@ _lib = None
 
 SYMBOLS = ["gi_create", "gi_destroy", "gi_last_error", "gi_status_string", "gi_set_mesh", "gi_set_mesh_device",
            "gi_set_transform", "gi_set_transform_matrix", "gi_build", "gi_intersect", "gi_result_free",
-           "gi_intersect_oneshot", "gi_set_batch", "gi_intersect_batch", "gi_share_mesh", "gi_clear_device",
+           "gi_intersect_oneshot", "gi_set_batch", "gi_intersect_batch", "gi_share_mesh", "gi_get_tuning",
+           "gi_set_tuning", "gi_autotune", "gi_clear_device",
            "gi_reconstruct_device",
            "gi_server_create", "gi_server_add_mesh", "gi_server_submit", "gi_server_wait", "gi_server_destroy",
            "gi_exact_pair_host", "gi_off_read", "gi_off_free", "gi_off_write", "gi_pairs_csv_write",
@@ -124,6 +133,9 @@ def lib():
         ip = C.POINTER(C.c_int32)
         L.gi_exact_pair_host.argtypes = [dp, dp, ip, ip, ip, ip, ip, C.POINTER(C.c_uint64), C.c_int32]
         L.gi_share_mesh.argtypes = [vp, C.c_int, vp, C.c_int]
+        L.gi_get_tuning.argtypes = [vp, C.POINTER(GiTuning)]
+        L.gi_set_tuning.argtypes = [vp, C.POINTER(GiTuning)]
+        L.gi_autotune.argtypes = [vp, C.POINTER(GiOptions), C.c_int32, C.POINTER(GiTuning)]
         cp = C.c_char_p
         L.gi_off_read.argtypes = [cp, C.POINTER(GiOffMesh), C.POINTER(vp)]
         L.gi_off_free.argtypes = [vp]
@@ -243,6 +255,24 @@ class Intersector:
             return result_to_py(res, want_segments, decode)
         finally:
             lib().gi_result_free(C.byref(res))
+
+    def get_tuning(self) -> dict:
+        t = GiTuning()
+        _check(lib().gi_get_tuning(self._ctx, C.byref(t)), self._ctx)
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = GiTuning()
+        _check(lib().gi_get_tuning(self._ctx, C.byref(t)), self._ctx)
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        _check(lib().gi_set_tuning(self._ctx, C.byref(t)), self._ctx)
+
+    def autotune(self, reps=3, want_segments=True, rebuild=False) -> dict:
+        t = GiTuning()
+        opt = GiOptions(1, int(want_segments), int(rebuild), 0)
+        _check(lib().gi_autotune(self._ctx, C.byref(opt), reps, C.byref(t)), self._ctx)
+        return t.as_dict()
 
     def set_batch(self, pairs):
         """pairs: [((xyzA, triA), (xyzB, triB)), ...] host arrays."""

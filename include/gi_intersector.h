@@ -180,6 +180,27 @@ int gi_intersect_batch(gi_context* ctx, const gi_options* opt, gi_result* out, u
  * BVH (no reference counterpart: the reference keeps one controller per pair). */
 int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot);
 
+/* ------------------------------------------------------------- tuning
+ * Launch shapes of the context's queries. No reference counterpart: the
+ * reference's hand-set tunables (level-cut levels, leafThreshold,
+ * batchMultiplier, dual-tree steps; apps/mainApplication.cpp:827-839) shape an
+ * algorithm this library does not use. One triangle per leaf and stackless
+ * rope traversal leave no leaf size or stack depth to tune. */
+typedef struct {
+    int32_t traverse_variant; /* 0..3: (workgroup size, candidate pairs staged per wave before one
+                                 queue append) = (64,256), (128,256), (256,256), (256,128) */
+    int32_t filter_blocks;    /* workgroups of the grid-stride FP64 filter launch */
+    int32_t exact_blocks;     /* workgroups of the grid-stride exact-stage launch */
+    int32_t build_chunk;      /* sorted leaves per BVH build workgroup: 256 or 512 */
+} gi_tuning;
+
+int gi_get_tuning(const gi_context* ctx, gi_tuning* out);
+int gi_set_tuning(gi_context* ctx, const gi_tuning* tuning);
+/* Times every candidate value of each knob in turn on the context's current
+ * meshes (reps queries each, median of the affected stage's time) and keeps
+ * the fastest; results never depend on the tuning. */
+int gi_autotune(gi_context* ctx, const gi_options* opt, int32_t reps, gi_tuning* out);
+
 /* Frees all device memory of the context but keeps the host copies; the next
  * call rebuilds (reference: clearGPU / reconstructGPU). */
 int gi_clear_device(gi_context* ctx);
