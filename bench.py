@@ -367,6 +367,18 @@ def main():
                      "stage_ms": avg, "end_to_end_algorithmic_GBs": b_alg / (ms / 1000.0) / 1e9,
                      "end_to_end_hbm_frac": b_alg / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS},
     }
+    # end to end from host arrays (both meshes uploaded and both BVHs built per
+    # run) to sorted pairs and exact segments in host memory
+    e2e = []
+    for _ in range(6):
+        t0 = time.perf_counter()
+        eng.set_mesh(0, xa, ta)
+        eng.set_mesh(1, xb, tb)
+        res = eng.intersect_raw(output_on_device=False, want_segments=True)
+        e2e.append(time.perf_counter() - t0)
+        assert res.n_pairs == st["pairs"]
+        gi.lib().gi_result_free(gi.C.byref(res))
+    line["pipeline"]["e2e_host_ms"] = float(np.median(e2e[1:])) * 1e3
     if args.check:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py
