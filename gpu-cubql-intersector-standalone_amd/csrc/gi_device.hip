@@ -786,6 +786,36 @@ __device__ __forceinline__ uint32_t chain_top(const KeySrc<K>& k, int m) {
     return (uint32_t)(k.n - 1 + m);
 }
 
+// Karras: range, split and children of internal node i, given its direction
+// d and dmin = delta(i, i - d). Probes outside [lo_ok, hi_ok] read as -1: the
+// caller guarantees the range ends inside, and past its end the common
+// prefix with key i can only shrink (sorted keys), so the comparisons with
+// dmin come out the same.
+template <class K>
+__device__ __forceinline__ void karras_node(const KeySrc<K>& KS, int i, int d, int dmin, int lo_ok, int hi_ok,
+                                            int& lo_i, int& hi_i, uint32_t& left, uint32_t& right, uint32_t& rope) {
+    const int n = KS.n;
+    auto dl = [&](int j) { return (j < lo_ok || j > hi_ok) ? -1 : delta(KS, i, j); };
+    int lmax = 2;
+    while (dl(i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (dl(i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(KS, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (dl(i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + (d < 0 ? -1 : 0);
+    lo_i = i < j ? i : j;
+    hi_i = i < j ? j : i;
+    left = lo_i == gamma ? (uint32_t)(n - 1 + gamma) : (uint32_t)gamma;
+    right = hi_i == gamma + 1 ? (uint32_t)(n - 1 + gamma + 1) : (uint32_t)(gamma + 1);
+    rope = hi_i == n - 1 ? kSentinel : chain_top(KS, hi_i + 1);
+}
+
 // Up to two trees per launch: blocks [0, split) build tree 0, the rest tree 1.
 // Fields are picked with selects on a block-uniform flag, so the kernel
 // arguments are never indexed dynamically.
@@ -797,6 +827,7 @@ struct ChunkJobs {
     const float4* hi0; const float4* hi1;
     float4* nodes0; float4* nodes1;
     uint2* range0; uint2* range1;
+    float4* level1_0; float4* level1_1;  // boxes of the 64-leaf blocks (first level of the block hierarchy)
     int n0, n1;
     uint32_t split;
 };
@@ -810,6 +841,7 @@ __global__ __launch_bounds__(256) void k_build_chunks(ChunkJobs<K> J) {
     const float4* __restrict__ hi = second ? J.hi1 : J.hi0;
     float4* __restrict__ nodes = second ? J.nodes1 : J.nodes0;
     uint2* __restrict__ range = second ? J.range1 : J.range0;
+    float4* __restrict__ level1 = second ? J.level1_1 : J.level1_0;
     const int n = second ? J.n1 : J.n0;
     const uint32_t chunk = blockIdx.x - (second ? J.split : 0u);
     __shared__ K skey[CHUNK];
@@ -835,27 +867,18 @@ __global__ __launch_bounds__(256) void k_build_chunks(ChunkJobs<K> J) {
     for (int i = cs + threadIdx.x; i < iend; i += blockDim.x) {
         int d = delta(KS, i, i + 1) > delta(KS, i, i - 1) ? 1 : -1;
         int dmin = delta(KS, i, i - d);
-        int lmax = 2;
-        while (delta(KS, i, i + lmax * d) > dmin) lmax <<= 1;
-        int l = 0;
-        for (int t = lmax >> 1; t >= 1; t >>= 1)
-            if (delta(KS, i, i + (l + t) * d) > dmin) l += t;
-        int j = i + l * d;
-        int dnode = delta(KS, i, j);
-        int s = 0, t = l;
-        do {
-            t = (t + 1) >> 1;
-            if (delta(KS, i, i + (s + t) * d) > dnode) s += t;
-        } while (t > 1);
-        int gamma = i + s * d + (d < 0 ? -1 : 0);
-        int lo_i = i < j ? i : j, hi_i = i < j ? j : i;
-        uint32_t left = lo_i == gamma ? (uint32_t)(n - 1 + gamma) : (uint32_t)gamma;
-        uint32_t right = hi_i == gamma + 1 ? (uint32_t)(n - 1 + gamma + 1) : (uint32_t)(gamma + 1);
-        uint32_t rope = hi_i == n - 1 ? kSentinel : chain_top(KS, hi_i + 1);
+        // one probe past the chunk edge tells whether the range leaves the
+        // chunk (sorted keys: the common prefix only shrinks with distance);
+        // an in-chunk node then searches the LDS keys only
+        const int edge = d > 0 ? ce : cs - 1;
+        const bool top = edge >= 0 && edge < n && delta(KS, i, edge) > dmin;
+        int lo_i, hi_i;
+        uint32_t left, right, rope;
+        karras_node(KS, i, d, dmin, top ? 0 : cs, top ? n - 1 : ce - 1, lo_i, hi_i, left, right, rope);
         range[i] = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
-        left_int[i - cs] = left;
-        rope_int[i - cs] = rope;
-        if (lo_i >= cs && hi_i < ce) {  // in-chunk node: both children are in the chunk too
+        if (!top) {  // both children are in the chunk too
+            left_int[i - cs] = left;
+            rope_int[i - cs] = rope;
             uint32_t c[2] = {left, right};
             for (int e = 0; e < 2; ++e) {
                 if (c[e] >= (uint32_t)(n - 1)) par_leaf[c[e] - (uint32_t)(n - 1) - (uint32_t)cs] = (uint32_t)i;
@@ -879,6 +902,20 @@ __global__ __launch_bounds__(256) void k_build_chunks(ChunkJobs<K> J) {
         nodes[2 * (size_t)(n - 1 + jl) + 1] = make_float4(h.x, h.y, h.z, __uint_as_float(rope));
         lb[u][0] = l.x; lb[u][1] = l.y; lb[u][2] = l.z;
         lb[u][3] = h.x; lb[u][4] = h.y; lb[u][5] = h.z;
+    }
+    // a wave holds 64 consecutive leaves per u (chunks are 64-aligned): their
+    // union is one box of the first block level
+    for (int u = 0; u < (int)(CHUNK / 256); ++u) {
+        const int wfirst = cs + 256 * u + 64 * (int)(threadIdx.x >> 6);
+        if (wfirst >= ce) continue;
+        float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        if (cs + (int)threadIdx.x + 256 * u < ce)
+            for (int k = 0; k < 6; ++k) b[k] = lb[u][k];
+        wave_box_reduce(b);
+        if (lane_id() == 0) {
+            level1[2 * (size_t)(wfirst >> 6)] = make_float4(b[0], b[1], b[2], 0.f);
+            level1[2 * (size_t)(wfirst >> 6) + 1] = make_float4(b[3], b[4], b[5], 0.f);
+        }
     }
     __syncthreads();
     for (int u = 0; u < (int)(CHUNK / 256); ++u) {
@@ -929,36 +966,51 @@ __global__ __launch_bounds__(256) void k_block_boxes(BlockBoxJobs J) {
 }
 
 // wave-cooperative union of the leaf boxes of node's range (at most two
-// partial 64-element reads per level of the block hierarchy)
-__device__ __forceinline__ void union_range_box(uint32_t n, const uint2* __restrict__ range, float4* nodes,
-                                                const BlockLevels& lv, uint32_t node) {
-    uint2 r = range[node];
-    uint32_t L = r.x, R = r.y + 1;
-    uint32_t lane = lane_id();
-    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (int lvl = 0;; ++lvl) {
-        const float4* E = lv.level[lvl];
+// partial 64-element windows per level of the block hierarchy). The windows
+// follow from the range alone, so they are listed first and then all read at
+// once: the loads of the levels are independent and in flight together.
+__device__ __forceinline__ void union_range_box(uint32_t lo, uint32_t hi, uint32_t left, uint32_t rope,
+                                                float4* nodes, const BlockLevels& lv, uint32_t node) {
+    uint32_t L = lo, R = hi + 1;
+    const uint32_t lane = lane_id();
+    const float4* wE[2 * kMaxBlockLevels];
+    uint32_t wS[2 * kMaxBlockLevels], wC[2 * kMaxBlockLevels];
+    bool done = false;
+#pragma unroll
+    for (int lvl = 0; lvl < kMaxBlockLevels; ++lvl) {
+        wE[2 * lvl] = wE[2 * lvl + 1] = lv.level[lvl];
+        wS[2 * lvl] = wS[2 * lvl + 1] = 0;
+        wC[2 * lvl] = wC[2 * lvl + 1] = 0;
+        if (done) continue;
         if (R - L <= 64) {
-            if (lane < R - L) box_absorb(E, L + lane, b);
-            break;
+            wS[2 * lvl] = L;
+            wC[2 * lvl] = R - L;
+            done = true;
+            continue;
         }
         uint32_t Lh = (L + 63u) & ~63u, Rt = R & ~63u;
-        if (lane < Lh - L) box_absorb(E, L + lane, b);
-        if (lane < R - Rt) box_absorb(E, Rt + lane, b);
+        wS[2 * lvl] = L;
+        wC[2 * lvl] = Lh - L;
+        wS[2 * lvl + 1] = Rt;
+        wC[2 * lvl + 1] = R - Rt;
         L = Lh >> 6;
         R = Rt >> 6;
-        if (L >= R) break;
+        if (L >= R) done = true;
     }
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+    for (int w = 0; w < 2 * kMaxBlockLevels; ++w)
+        if (lane < wC[w]) box_absorb(wE[w], wS[w] + lane, b);
     wave_box_reduce(b);
     if (lane == 0) {
-        float4 pl = nodes[2 * (size_t)node], ph = nodes[2 * (size_t)node + 1];
-        nodes[2 * (size_t)node] = make_float4(b[0], b[1], b[2], pl.w);
-        nodes[2 * (size_t)node + 1] = make_float4(b[3], b[4], b[5], ph.w);
+        nodes[2 * (size_t)node] = make_float4(b[0], b[1], b[2], __uint_as_float(left));
+        nodes[2 * (size_t)node + 1] = make_float4(b[3], b[4], b[5], __uint_as_float(rope));
     }
 }
 
-// one wave per 64 consecutive internal nodes; the top ones among them are
-// done one after the other by the whole wave. Blocks [0, split) serve tree 0.
+// One wave per 64 consecutive internal nodes; the top ones among them (range
+// across a chunk boundary) get their boxes one after the other from the whole
+// wave. Blocks [0, split) serve tree 0.
 struct TopBoxJobs {
     BlockLevels lv0, lv1;
     const uint2* range0; const uint2* range1;
@@ -975,16 +1027,20 @@ __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J, uint32_t chunk)
     const BlockLevels& lv = second ? J.lv1 : J.lv0;  // read in place from the kernel arguments
     uint32_t base = ((blockIdx.x - (second ? J.split : 0u)) * 4 + (threadIdx.x >> 6)) * 64;
     uint32_t i = base + lane_id();
+    uint2 r = make_uint2(0, 0);
     bool top = false;
     if (i + 1 < n) {
-        uint2 r = range[i];
+        r = range[i];
         top = r.x / chunk != r.y / chunk;
     }
     uint64_t m = __ballot(top);
     while (m) {
         uint32_t b = (uint32_t)__ffsll((long long)m) - 1;
         m &= m - 1;
-        union_range_box(n, range, nodes, lv, base + b);
+        uint32_t node = base + b;
+        uint32_t L = (uint32_t)__shfl((int)r.x, (int)b), R = (uint32_t)__shfl((int)r.y, (int)b);
+        uint32_t lf = __float_as_uint(nodes[2 * (size_t)node].w), rp = __float_as_uint(nodes[2 * (size_t)node + 1].w);
+        union_range_box(L, R, lf, rp, nodes, lv, node);
     }
 }
 
@@ -1009,10 +1065,10 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     if (n0 + n1 == 0) return;
     ChunkJobs<K> cj;
     cj.keys0 = j0->keys; cj.prims0 = j0->prims; cj.lo0 = j0->lo; cj.hi0 = j0->hi;
-    cj.nodes0 = j0->nodes; cj.range0 = j0->range; cj.n0 = (int)n0;
+    cj.nodes0 = j0->nodes; cj.range0 = j0->range; cj.level1_0 = j0->blocks; cj.n0 = (int)n0;
     const BuildJob<K>* o = j1 ? j1 : j0;
     cj.keys1 = o->keys; cj.prims1 = o->prims; cj.lo1 = o->lo; cj.hi1 = o->hi;
-    cj.nodes1 = o->nodes; cj.range1 = o->range; cj.n1 = (int)n1;
+    cj.nodes1 = o->nodes; cj.range1 = o->range; cj.level1_1 = o->blocks; cj.n1 = (int)n1;
     cj.split = blocks_for(n0, chunk);
     const uint32_t chunk_blocks = cj.split + blocks_for(n1, chunk);
     if (chunk_begin) (void)hipEventRecord(chunk_begin, s);
@@ -1031,7 +1087,8 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
             off += 2 * (size_t)block_level_count(nn[t], k);
         }
     }
-    for (int k = 1; k < kMaxBlockLevels; ++k) {
+    // level 1 (64-leaf blocks) was written by k_build_chunks
+    for (int k = 2; k < kMaxBlockLevels; ++k) {
         BlockBoxJobs bj;
         uint32_t nin[2], nout[2];
         for (int t = 0; t < 2; ++t) {
