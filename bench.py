@@ -71,6 +71,24 @@ def measured_traffic(kernel):
         return None
 
 
+def device_copy(nbytes, reps):
+    """Plain device-to-device copies of nbytes (the reference dispatch for
+    scaling the rocprofv3 FETCH_SIZE / WRITE_SIZE counters), through the HIP
+    runtime the library itself uses."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    try:
+        for _ in range(reps):
+            assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+        assert hip.hipDeviceSynchronize() == 0
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+
+
 def cpu_baseline(xa, ta, xb, tb, stride, reps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
@@ -261,6 +279,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each step's launches up to the counter read-back as a HIP graph")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement")
     ap.add_argument("--batch-pairs", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=5)
@@ -279,8 +299,6 @@ def main():
               "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
     gi = ge.package_module("gi_ctypes")
     mg = ge.package_module("meshgen")
     if args.batch_only:
@@ -288,33 +306,28 @@ def main():
               flush=True)
         return
     (xa, ta), (xb, tb) = mg.sphere_torus(args.res, args.res // 2)
-    dev = torch.device("cuda:0")
-    dxa = torch.from_numpy(np.ascontiguousarray(xa)).to(dev)
-    dta = torch.from_numpy(np.ascontiguousarray(ta).view(np.int32)).to(dev)
-    dxb = torch.from_numpy(np.ascontiguousarray(xb)).to(dev)
-    dtb = torch.from_numpy(np.ascontiguousarray(tb).view(np.int32)).to(dev)
-    torch.cuda.synchronize()
-
+    # both meshes are uploaded once: every step reads them from HBM
     eng = gi.Intersector(device=0)
-    eng.set_mesh_device(0, dxa.data_ptr(), xa.shape[0], dta.data_ptr(), ta.shape[0])
-    eng.set_mesh_device(1, dxb.data_ptr(), xb.shape[0], dtb.data_ptr(), tb.shape[0])
+    eng.set_mesh(0, xa, ta)
+    eng.set_mesh(1, xb, tb)
+    eng.build()
 
     tuning = eng.get_tuning() if args.no_autotune else eng.autotune(reps=3, rebuild=True)
 
     def step():
-        res = eng.intersect_raw(output_on_device=True, want_segments=True, rebuild=True)
+        res = eng.intersect_raw(output_on_device=True, want_segments=True, rebuild=True, use_graph=args.graph)
         st = res.stats.as_dict()
         gi.lib().gi_result_free(gi.C.byref(res))
         return st
 
+    # a step returns once its results are complete in device memory (the
+    # library synchronises its streams), so the clock brackets finished work
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         stats.append(step())
-    torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1000.0 / args.steps
     st = stats[-1]
@@ -354,6 +367,7 @@ def main():
                    "triangles_a": int(ta.shape[0]), "triangles_b": int(tb.shape[0]),
                    "vertices_a": int(xa.shape[0]), "vertices_b": int(xb.shape[0])},
         "tuning": tuning,
+        "graph_steps": sum(s["graph"] for s in stats),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_ms": kernel_ms, "launches_timed": launches, "leaves_per_launch": leaves_per_launch,
@@ -387,11 +401,7 @@ def main():
         line["check"] = bool(np.array_equal(r["pairs"], ref["pairs"]))
     eng.close()
     if args.copy_calib:
-        src = torch.empty(args.copy_calib * (1 << 20) // 4, dtype=torch.float32, device=dev).fill_(1.0)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+        device_copy(args.copy_calib << 20, 3)
         print(f"copy calibration: {args.copy_calib} MiB read + {args.copy_calib} MiB written per dispatch",
               file=sys.stderr)
     if not args.no_batch:

@@ -64,11 +64,17 @@ struct ExactTT {
     // Piece of an edge clipped by a triangle: points e0 + t (e1 - e0), t in [t0, t1]
     struct Piece { int edge; Rat t0, t1; };
 
-    struct Result {
+    // Result of a pair without its coplanar pieces: everything a
+    // non-coplanar or degenerate pair needs. With `deferred` set (coplanar
+    // pair met by intersect_fast) nothing else is valid.
+    struct Fast {
         int hit, coplanar, degenerate, nseg;
-        HP p, q;                // the single segment of non-coplanar/degenerate pairs
-        Piece pc[GI_SEG_MAX];   // coplanar boundary edges (edge 0..2: A, 3..5: B)
         int point_from_piece;   // coplanar single-point overlap: index into pc
+        int deferred;
+        HP p, q;                // the single segment of non-coplanar/degenerate pairs
+    };
+    struct Result : Fast {
+        Piece pc[GI_SEG_MAX];   // coplanar boundary edges (edge 0..2: A, 3..5: B)
     };
 
     // ------------------------------------------------------------ vectors
@@ -183,23 +189,30 @@ struct ExactTT {
     }
 
     // ------------------------------------------------ degenerate shapes
+    // T[i] for a run-time i by selects (constant indices keep T in registers)
+    static GI_HD V3 pick(const V3* T, int i) { return i == 0 ? T[0] : (i == 1 ? T[1] : T[2]); }
+
     // 2 for a proper triangle; 1 for a segment T[e0]T[e1]; 0 for a point.
     static GI_HD int simplexDim(const V3* T, const V3& n, int& e0, int& e1) {
         e0 = 0; e1 = 0;
         if (!isZero(n)) return 2;
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
             int lo = 0, hi = 0;
+            C vlo = T[0].v[k], vhi = T[0].v[k];
+#pragma unroll
             for (int i = 1; i < 3; ++i) {
-                if (P::sgnC(T[i].v[k] - T[lo].v[k]) < 0) lo = i;
-                if (P::sgnC(T[i].v[k] - T[hi].v[k]) > 0) hi = i;
+                if (P::sgnC(T[i].v[k] - vlo) < 0) { lo = i; vlo = T[i].v[k]; }
+                if (P::sgnC(T[i].v[k] - vhi) > 0) { hi = i; vhi = T[i].v[k]; }
             }
-            if (P::sgnC(T[hi].v[k] - T[lo].v[k]) != 0) { e0 = lo; e1 = hi; return 1; }
+            if (P::sgnC(vhi - vlo) != 0) { e0 = lo; e1 = hi; return 1; }
         }
         return 0;
     }
 
     // segment [p, q] (possibly p == q) against a proper triangle T with normal nT
-    static GI_HD int segTri(const V3& p, const V3& q, const V3* T, const V3& nT, Result& r) {
+    template <class R>
+    static GI_HD int segTri(const V3& p, const V3& q, const V3* T, const V3& nT, R& r) {
         B op = dotB(nT, sub(p, T[0]));
         B oq = dotB(nT, sub(q, T[0]));
         int sp = P::sgnB(op), sq = P::sgnB(oq);
@@ -227,7 +240,8 @@ struct ExactTT {
     }
 
     // closed segments [p, q] and [s, t] (each possibly a point)
-    static GI_HD int segSeg(const V3& p, const V3& q, const V3& s, const V3& t, Result& r) {
+    template <class R>
+    static GI_HD int segSeg(const V3& p, const V3& q, const V3& s, const V3& t, R& r) {
         bool pt1 = eq(p, q), pt2 = eq(s, t);
         if (pt1 && pt2) {
             if (!eq(p, s)) return 0;
@@ -271,9 +285,44 @@ struct ExactTT {
         return 1;
     }
 
+    // The pieces of the edges of T met by the (non-coplanar) triangle O, added
+    // to the span along axis k; o/s: orientations of T's vertices against O's
+    // plane. Unrolled, every index is a compile-time constant.
+    static GI_HD void edgesAgainst(const V3* T, const V3* O, const V3& nO, const B* o, const int* s, int k,
+                                   Span& span) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int j = i == 2 ? 0 : i + 1;
+            if (s[i] * s[j] > 0) continue;
+            if (s[i] == 0 && s[j] == 0) {
+                int kk = dropAxis(nO);
+                int ia = kk == 0 ? 1 : 0, ib = kk == 2 ? 1 : 2;
+                Rat t0, t1;
+                int kind = clip2d(T[i], T[j], O, ia, ib, t0, t1);
+                if (kind == 0) continue;
+                spanAdd(span, hpOnEdge(T[i], T[j], t0), k);
+                if (kind == 2) spanAdd(span, hpOnEdge(T[i], T[j], t1), k);
+                continue;
+            }
+            int e0 = P::sgnB(orient3d(T[i], T[j], O[0], O[1]));
+            int e1 = P::sgnB(orient3d(T[i], T[j], O[1], O[2]));
+            int e2 = P::sgnB(orient3d(T[i], T[j], O[2], O[0]));
+            if (!((e0 >= 0 && e1 >= 0 && e2 >= 0) || (e0 <= 0 && e1 <= 0 && e2 <= 0))) continue;
+            HP X = s[i] == 0 ? hpInt(T[i]) : (s[j] == 0 ? hpInt(T[j]) : hpCrossing(T[i], T[j], o[i], o[j]));
+            spanAdd(span, X, k);
+        }
+    }
+
     // ------------------------------------------------------------- main
-    static GI_HD void intersect(const V3* A, const V3* Bt, Result& r) {
-        r.hit = 0; r.coplanar = 0; r.degenerate = 0; r.nseg = 0; r.point_from_piece = -1;
+    // Full decision and construction.
+    static GI_HD void intersect(const V3* A, const V3* Bt, Result& r) { intersect_t<Result, true>(A, Bt, r); }
+    // The same without the coplanar case, which is only flagged (deferred):
+    // no polygon pieces, so nothing indexed at run time in per-lane storage.
+    static GI_HD void intersect_fast(const V3* A, const V3* Bt, Fast& r) { intersect_t<Fast, false>(A, Bt, r); }
+
+    template <class R, bool kCoplanar>
+    static GI_HD void intersect_t(const V3* A, const V3* Bt, R& r) {
+        r.hit = 0; r.coplanar = 0; r.degenerate = 0; r.nseg = 0; r.point_from_piece = -1; r.deferred = 0;
         V3 nA = cross(sub(A[1], A[0]), sub(A[2], A[0]));
         V3 nB = cross(sub(Bt[1], Bt[0]), sub(Bt[2], Bt[0]));
         int a0, a1, b0, b1;
@@ -281,15 +330,20 @@ struct ExactTT {
         int dB = simplexDim(Bt, nB, b0, b1);
         if (dA < 2 || dB < 2) {
             r.degenerate = 1;
-            if (dA == 2) r.hit = segTri(Bt[b0], Bt[b1], A, nA, r);
-            else if (dB == 2) r.hit = segTri(A[a0], A[a1], Bt, nB, r);
-            else r.hit = segSeg(A[a0], A[a1], Bt[b0], Bt[b1], r);
+            const V3 pa0 = pick(A, a0), pa1 = pick(A, a1), pb0 = pick(Bt, b0), pb1 = pick(Bt, b1);
+            if (dA == 2) r.hit = segTri(pb0, pb1, A, nA, r);
+            else if (dB == 2) r.hit = segTri(pa0, pa1, Bt, nB, r);
+            else r.hit = segSeg(pa0, pa1, pb0, pb1, r);
             return;
         }
         B oB[3], oA[3];
         int sB[3], sA[3];
         for (int i = 0; i < 3; ++i) { oB[i] = dotB(nA, sub(Bt[i], A[0])); sB[i] = P::sgnB(oB[i]); }
-        if (sB[0] == 0 && sB[1] == 0 && sB[2] == 0) { coplanar(A, Bt, nA, r); return; }
+        if (sB[0] == 0 && sB[1] == 0 && sB[2] == 0) {
+            if constexpr (kCoplanar) coplanar(A, Bt, nA, r);
+            else r.deferred = 1;
+            return;
+        }
         if (sB[0] * sB[1] > 0 && sB[1] * sB[2] > 0) return;
         for (int i = 0; i < 3; ++i) { oA[i] = dotB(nB, sub(A[i], Bt[0])); sA[i] = P::sgnB(oA[i]); }
         if (sA[0] * sA[1] > 0 && sA[1] * sA[2] > 0) return;
@@ -304,33 +358,8 @@ struct ExactTT {
         if (P::cmpAbsB(dir[2], dir[k]) > 0) k = 2;
         Span span;
         span.n = 0;
-        for (int side = 0; side < 2; ++side) {
-            const V3* T = side == 0 ? A : Bt;       // edges of T ...
-            const V3* O = side == 0 ? Bt : A;       // ... against triangle O
-            const V3& nO = side == 0 ? nB : nA;
-            const B* o = side == 0 ? oA : oB;
-            const int* s = side == 0 ? sA : sB;
-            for (int i = 0; i < 3; ++i) {
-                int j = i == 2 ? 0 : i + 1;
-                if (s[i] * s[j] > 0) continue;
-                if (s[i] == 0 && s[j] == 0) {
-                    int kk = dropAxis(nO);
-                    int ia = kk == 0 ? 1 : 0, ib = kk == 2 ? 1 : 2;
-                    Rat t0, t1;
-                    int kind = clip2d(T[i], T[j], O, ia, ib, t0, t1);
-                    if (kind == 0) continue;
-                    spanAdd(span, hpOnEdge(T[i], T[j], t0), k);
-                    if (kind == 2) spanAdd(span, hpOnEdge(T[i], T[j], t1), k);
-                    continue;
-                }
-                int e0 = P::sgnB(orient3d(T[i], T[j], O[0], O[1]));
-                int e1 = P::sgnB(orient3d(T[i], T[j], O[1], O[2]));
-                int e2 = P::sgnB(orient3d(T[i], T[j], O[2], O[0]));
-                if (!((e0 >= 0 && e1 >= 0 && e2 >= 0) || (e0 <= 0 && e1 <= 0 && e2 <= 0))) continue;
-                HP X = s[i] == 0 ? hpInt(T[i]) : (s[j] == 0 ? hpInt(T[j]) : hpCrossing(T[i], T[j], o[i], o[j]));
-                spanAdd(span, X, k);
-            }
-        }
+        edgesAgainst(A, Bt, nB, oA, sA, k, span);  // edges of A against triangle B
+        edgesAgainst(Bt, A, nA, oB, sB, k, span);  // edges of B against triangle A
         if (span.n == 0) return;
         r.hit = 1;
         r.nseg = 1;

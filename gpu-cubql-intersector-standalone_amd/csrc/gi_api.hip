@@ -137,6 +137,9 @@ struct gi_context {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<uint64_t> graph_sig, graph_sig_seen;
     bool graph_off = false;  // capture failed once: queries stay eager
+    bool capturing = false;  // a capture is open: timing events become graph nodes
+    bool graph_chunk_timed = false;  // what the capture's build left in chunk_timed/leaves
+    uint32_t graph_chunk_leaves = 0;
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -184,6 +187,12 @@ void fork_side(gi_context* ctx) {
 void join_side(gi_context* ctx) {
     HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
     HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+}
+
+// timing event on stream s; inside a capture it becomes an event node of the
+// graph (external record), so that every replay sets its timestamp
+void record(gi_context* ctx, hipEvent_t e, hipStream_t s) {
+    HIP_TRY(hipEventRecordWithFlags(e, s, ctx->capturing ? hipEventRecordExternal : hipEventRecordDefault));
 }
 
 void sync_all(gi_context* ctx) {
@@ -344,7 +353,8 @@ void build_meshes(gi_context* ctx, bool rebuild, bool timed = true) {
         for (int j = 0; j < nj; ++j) leaves += jobs[j].n;
         bool time_it = timed && !ctx->chunk_timed && leaves > 0;
         launch_build_bvh(jobs, nj, (uint32_t)ctx->tune.build_chunk, s, time_it ? ctx->ev_chunk[0] : nullptr,
-                         time_it ? ctx->ev_chunk[1] : nullptr);
+                         time_it ? ctx->ev_chunk[1] : nullptr,
+                         ctx->capturing ? hipEventRecordExternal : hipEventRecordDefault);
         if (time_it) {
             ctx->chunk_timed = true;
             ctx->chunk_leaves = leaves;
@@ -423,27 +433,27 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
     MeshSlot& MB = ctx->mesh[1];
     const uint32_t M = MA.forest;
     if (build) {
-        if (timed) HIP_TRY(hipEventRecord(ctx->ev[0], s));
+        if (timed) record(ctx, ctx->ev[0], s);
         build_meshes(ctx, o.rebuild != 0, timed);
-        if (timed) HIP_TRY(hipEventRecord(ctx->ev[1], s));
+        if (timed) record(ctx, ctx->ev[1], s);
     }
     uint32_t* counters = ctx->counters.get<uint32_t>();
     uint32_t* grid = counters + CNT_COUNT;
     MeshDev A = mesh_dev(MA), B = mesh_dev(MB);
     int bitsB = bits_for(MB.nt);
     launch_fill_u32(counters, 0u, CNT_COUNT, s);
-    if (timed) HIP_TRY(hipEventRecord(ctx->ev[6], s));
+    if (timed) record(ctx, ctx->ev[6], s);
     const uint64_t* keysB = M ? (MB.sorted_in_tmp ? MB.tkeys.get<uint64_t>() : MB.keys.get<uint64_t>()) : nullptr;
     launch_traverse(A, B, keysB, 3 * MB.morton_axis, M ? MA.roots.get<uint2>() : nullptr, ctx->cand.get<uint2>(),
                     ctx->cap_cand, counters, ctx->tune.traverse_variant, s);
-    if (timed) HIP_TRY(hipEventRecord(ctx->ev[2], s));
+    if (timed) record(ctx, ctx->ev[2], s);
     launch_filter(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, grid, ctx->exact.get<uint2>(), ctx->cap_exact, counters,
                   ctx->tune.filter_blocks, s);
-    if (timed) HIP_TRY(hipEventRecord(ctx->ev[3], s));
+    if (timed) record(ctx, ctx->ev[3], s);
     launch_exact(A, B, ctx->exact.get<uint2>(), ctx->cap_exact, grid, bitsB, ctx->rec_key.get<uint64_t>(),
-                 ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<__int128>(), ctx->cap_seg, counters,
-                 ctx->tune.exact_blocks, s);
-    if (timed) HIP_TRY(hipEventRecord(ctx->ev[4], s));
+                 ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<__int128>(), ctx->cap_seg,
+                 ctx->cand.get<uint2>(), ctx->cap_cand, counters, ctx->tune.exact_blocks, s);
+    if (timed) record(ctx, ctx->ev[4], s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(ctx->h_counters, counters, sizeof(uint32_t) * (CNT_COUNT + 2), hipMemcpyDeviceToHost, s));
 }
@@ -507,15 +517,14 @@ bool launch_presync_graph(gi_context* ctx, const gi_options& o) {
     if (ctx->graph_exec && sig == ctx->graph_sig) {
         for (int sl = 0; sl < 2; ++sl)
             if (need[sl]) ctx->h_xf[sl] = ctx->mesh[sl].xf;
-        HIP_TRY(hipEventRecord(ctx->ev[0], s));
-        HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
-        HIP_TRY(hipEventRecord(ctx->ev[4], s));
+        HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));  // records the stage events itself
         for (int sl = 0; sl < 2; ++sl)
             if (need[sl]) {
                 ctx->mesh[sl].built = true;
                 ctx->mesh[sl].dirty = false;
             }
-        ctx->chunk_timed = false;
+        ctx->chunk_timed = ctx->graph_chunk_timed;
+        ctx->chunk_leaves = ctx->graph_chunk_leaves;
         return true;
     }
     if (sig != ctx->graph_sig_seen) {
@@ -534,11 +543,15 @@ bool launch_presync_graph(gi_context* ctx, const gi_options& o) {
         return false;
     }
     bool ok = true;
+    std::string why;
+    ctx->capturing = true;
     try {
-        enqueue_presync(ctx, o, true, false);
-    } catch (const GiError&) {
+        enqueue_presync(ctx, o, true, true);
+    } catch (const GiError& ex) {
         ok = false;
+        why = ex.msg;
     }
+    ctx->capturing = false;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
     if (!ok || e != hipSuccess || !g ||
@@ -552,15 +565,14 @@ bool launch_presync_graph(gi_context* ctx, const gi_options& o) {
             ctx->mesh[sl].built = was_built[sl];
             ctx->mesh[sl].dirty = was_dirty[sl];
         }
-        if (!ok) throw GiError{GI_ERR_HIP, "graph capture of the query failed"};
+        if (!ok) throw GiError{GI_ERR_HIP, "graph capture of the query failed: " + why};
         return false;
     }
     ctx->graph = g;
     ctx->graph_sig = sig;
-    ctx->chunk_timed = false;
-    HIP_TRY(hipEventRecord(ctx->ev[0], s));
+    ctx->graph_chunk_timed = ctx->chunk_timed;
+    ctx->graph_chunk_leaves = ctx->chunk_leaves;
     HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
-    HIP_TRY(hipEventRecord(ctx->ev[4], s));
     return true;
 }
 
@@ -620,16 +632,15 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     st.filter_green = hc[CNT_FILTER_GREEN];
     st.filter_yellow = hc[CNT_FILTER_YELLOW];
     st.grid = hgrid[0] && hgrid[1];
-    if (!graphed) {  // a replayed graph records no stage events: only the totals are timed
-        st.ms_build = elapsed(ctx->ev[0], ctx->ev[1]);
-        if (ctx->chunk_timed) {
-            st.ms_bvh_chunks = elapsed(ctx->ev_chunk[0], ctx->ev_chunk[1]);
-            st.bvh_chunk_launches = 1;
-            st.bvh_chunk_leaves = ctx->chunk_leaves;
-        }
-        st.ms_traverse = elapsed(ctx->ev[6], ctx->ev[2]);
-        st.ms_filter = elapsed(ctx->ev[2], ctx->ev[3]);
+    st.ms_build = elapsed(ctx->ev[0], ctx->ev[1]);
+    if (ctx->chunk_timed) {
+        st.ms_bvh_chunks = elapsed(ctx->ev_chunk[0], ctx->ev_chunk[1]);
+        st.bvh_chunk_launches = 1;
+        st.bvh_chunk_leaves = ctx->chunk_leaves;
     }
+    st.ms_traverse = elapsed(ctx->ev[6], ctx->ev[2]);
+    st.ms_filter = elapsed(ctx->ev[2], ctx->ev[3]);
+    st.graph = graphed ? 1 : 0;
 
     if (st.grid) {
         st.exact_pairs = hc[CNT_EXACT_LIST];
@@ -637,7 +648,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         st.exact_yellow_no = hc[CNT_EXACT_YELLOW_NO];
         st.coplanar = hc[CNT_COPLANAR];
         st.degenerate = hc[CNT_DEGENERATE];
-        if (!graphed) st.ms_exact = elapsed(ctx->ev[3], ctx->ev[4]);
+        st.ms_exact = elapsed(ctx->ev[3], ctx->ev[4]);
         uint32_t R = hc[CNT_RESULTS], S = o.want_segments ? hc[CNT_SEGMENTS] : 0;
         uint64_t* keys = ctx->rec_key.get<uint64_t>();
         uint32_t* order = ctx->order.ensure<uint32_t>(R);
@@ -666,7 +677,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
             HIP_TRY(hipMemcpyAsync(ctx->h_pair_off.data(), poff, 4 * ((size_t)M + 1), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));  // pageable destination
         }
-        HIP_TRY(hipEventRecord(ctx->ev[5], s));
+        record(ctx, ctx->ev[5], s);
         HIP_TRY(hipGetLastError());
         st.pairs = R;
         st.segments = S;

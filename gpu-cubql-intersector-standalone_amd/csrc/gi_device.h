@@ -33,6 +33,7 @@ enum Counter {
     CNT_COPLANAR,
     CNT_DEGENERATE,
     CNT_UNSOUND,             // filter said GREEN, exact said no (must stay 0)
+    CNT_DEFERRED,            // coplanar pairs moved from the fast exact pass to the full one
     CNT_COUNT
 };
 
@@ -123,11 +124,12 @@ struct BuildJob {
 };
 // Builds one or two trees with one launch per kernel (the blocks of a launch
 // are split between the trees); chunk_begin/chunk_end (may be null) bracket
-// the k_build_chunks launch.
+// the k_build_chunks launch, recorded with event_flags (hipEventRecordExternal
+// inside a graph capture, so that a replay times the launch).
 void launch_build_bvh(const BuildJob<uint32_t>* jobs, int njobs, uint32_t chunk, hipStream_t s,
-                      hipEvent_t chunk_begin, hipEvent_t chunk_end);
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end, unsigned event_flags);
 void launch_build_bvh(const BuildJob<uint64_t>* jobs, int njobs, uint32_t chunk, hipStream_t s,
-                      hipEvent_t chunk_begin, hipEvent_t chunk_end);
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end, unsigned event_flags);
 // roots[m] = (subtree root of mesh m, rope leaving it)
 void launch_forest_roots(const uint2* range, const float4* nodes, uint32_t n, const uint32_t* toff, uint32_t M,
                          uint2* roots, hipStream_t s);
@@ -141,10 +143,13 @@ void launch_traverse(MeshDev A, MeshDev B, const uint64_t* keysB, int mesh_shift
 // blocks: workgroups of the grid-stride launch
 void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const uint32_t* grid_flags,
                    uint2* exact_list, uint32_t exact_cap, uint32_t* counters, int blocks, hipStream_t s);
+// defer: room for the coplanar pairs the fast pass hands to the full pass
+// (at least as many entries as the exact list; the candidate queue, free by
+// then, serves)
 void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
                   int bitsB,
                   uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, __int128* seg_pool, uint32_t seg_cap,
-                  uint32_t* counters, int blocks, hipStream_t s);
+                  uint2* defer, uint32_t defer_cap, uint32_t* counters, int blocks, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
 void launch_gather_pairs(const uint64_t* sorted_key, const uint32_t* order, const uint2* rec_seg, uint32_t n,
                          int bitsB, uint2* pairs, uint32_t* nseg, hipStream_t s);

@@ -19,6 +19,8 @@
 //                     src/single_tree/SingleTreeBatchIntersector.cu:276-384
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "exact_tritri.h"
 #include "filter.h"
 #include "gi_device.h"
@@ -1058,7 +1060,7 @@ size_t block_levels_elems(uint32_t n) {
 
 template <class K>
 static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, hipStream_t s, hipEvent_t chunk_begin,
-                           hipEvent_t chunk_end) {
+                           hipEvent_t chunk_end, unsigned event_flags) {
     const BuildJob<K>* j0 = &jobs[0];
     const BuildJob<K>* j1 = njobs > 1 ? &jobs[1] : nullptr;
     const uint32_t n0 = j0->n, n1 = j1 ? j1->n : 0;
@@ -1071,10 +1073,10 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     cj.nodes1 = o->nodes; cj.range1 = o->range; cj.level1_1 = o->blocks; cj.n1 = (int)n1;
     cj.split = blocks_for(n0, chunk);
     const uint32_t chunk_blocks = cj.split + blocks_for(n1, chunk);
-    if (chunk_begin) (void)hipEventRecord(chunk_begin, s);
+    if (chunk_begin) (void)hipEventRecordWithFlags(chunk_begin, s, event_flags);
     if (chunk == 256) k_build_chunks<K, 256><<<chunk_blocks, 256, 0, s>>>(cj);
     else k_build_chunks<K, 512><<<chunk_blocks, 256, 0, s>>>(cj);
-    if (chunk_end) (void)hipEventRecord(chunk_end, s);
+    if (chunk_end) (void)hipEventRecordWithFlags(chunk_end, s, event_flags);
     // 64-ary levels of leaf boxes for the nodes that cross chunks
     BlockLevels lv[2];
     const BuildJob<K>* jj[2] = {j0, o};
@@ -1114,12 +1116,12 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
 }
 
 void launch_build_bvh(const BuildJob<uint32_t>* jobs, int njobs, uint32_t chunk, hipStream_t s,
-                      hipEvent_t chunk_begin, hipEvent_t chunk_end) {
-    build_bvh_impl<uint32_t>(jobs, njobs, chunk, s, chunk_begin, chunk_end);
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end, unsigned event_flags) {
+    build_bvh_impl<uint32_t>(jobs, njobs, chunk, s, chunk_begin, chunk_end, event_flags);
 }
 void launch_build_bvh(const BuildJob<uint64_t>* jobs, int njobs, uint32_t chunk, hipStream_t s,
-                      hipEvent_t chunk_begin, hipEvent_t chunk_end) {
-    build_bvh_impl<uint64_t>(jobs, njobs, chunk, s, chunk_begin, chunk_end);
+                      hipEvent_t chunk_begin, hipEvent_t chunk_end, unsigned event_flags) {
+    build_bvh_impl<uint64_t>(jobs, njobs, chunk, s, chunk_begin, chunk_end, event_flags);
 }
 
 // Subtree of each forest mesh: the node whose leaf range is exactly the
@@ -1354,24 +1356,34 @@ __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& tot
     return x - v;
 }
 
+// Exact decision and segments of the listed pairs. The fast instance keeps
+// its per-lane state in registers: a coplanar pair (polygon pieces) is only
+// moved to the deferred list, which the full instance then processes with
+// the whole result (pieces in per-lane memory). Records and segments of both
+// are appended with one atomic per wave.
+template <bool kFast>
 __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2* __restrict__ list, uint32_t cap,
                                                const uint32_t* __restrict__ grid_flags, int bitsB,
                                                uint64_t* __restrict__ rec_key, uint2* __restrict__ rec_seg,
                                                uint32_t rec_cap, __int128* __restrict__ seg_pool, uint32_t seg_cap,
+                                               uint2* __restrict__ defer, uint32_t defer_cap,
                                                uint32_t* __restrict__ counters) {
+    typedef typename std::conditional<kFast, GridExact::Fast, GridExact::Result>::type R;
     if (!(grid_flags[0] && grid_flags[1])) return;  // general doubles: host residual path
-    uint32_t n = counters[CNT_EXACT_LIST];
+    uint32_t n = counters[kFast ? CNT_EXACT_LIST : CNT_DEFERRED];
     if (n > cap) n = cap;
     uint32_t c_yes = 0, c_no = 0, c_cop = 0, c_deg = 0, c_bad = 0;
     for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
         uint32_t i = base + threadIdx.x;
-        GridExact::Result r;
+        R r;
         r.hit = 0;
         r.nseg = 0;
+        r.deferred = 0;
         GridExact::V3 PA[3], PB[3];
         uint32_t a = 0, b = 0;
+        uint2 e = make_uint2(0, 0);
         if (i < n) {
-            uint2 e = list[i];
+            e = list[i];
             a = e.x;
             b = e.y & kIndexMask;
             uint32_t status = e.y >> kStatusShift;
@@ -1382,15 +1394,24 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
                 PA[v].v[0] = p.x; PA[v].v[1] = p.y; PA[v].v[2] = p.z;
                 PB[v].v[0] = q.x; PB[v].v[1] = q.y; PB[v].v[2] = q.z;
             }
-            GridExact::intersect(PA, PB, r);
-            if (status == PAIR_YELLOW) { c_yes += r.hit; c_no += !r.hit; }
-            if (status == PAIR_GREEN && !r.hit) c_bad++;
-            c_cop += r.hit && r.coplanar;
-            c_deg += r.hit && r.degenerate;
+            if constexpr (kFast) GridExact::intersect_fast(PA, PB, r);
+            else GridExact::intersect(PA, PB, r);
+            if (!r.deferred) {
+                if (status == PAIR_YELLOW) { c_yes += r.hit; c_no += !r.hit; }
+                if (status == PAIR_GREEN && !r.hit) c_bad++;
+                c_cop += r.hit && r.coplanar;
+                c_deg += r.hit && r.degenerate;
+            }
         }
+        if constexpr (kFast) {
+            uint32_t dslot;
+            wave_append(r.deferred != 0, &counters[CNT_DEFERRED], dslot);
+            if (r.deferred && dslot < defer_cap) defer[dslot] = e;
+        }
+        const bool hit = r.hit && !r.deferred;
         uint32_t slot;
-        wave_append(r.hit != 0, &counters[CNT_RESULTS], slot);
-        uint32_t nseg = r.hit ? (uint32_t)r.nseg : 0u;
+        wave_append(hit, &counters[CNT_RESULTS], slot);
+        uint32_t nseg = hit ? (uint32_t)r.nseg : 0u;
         uint32_t wtot;
         uint32_t soff = wave_exclusive_sum(nseg, wtot);
         uint32_t sbase = 0;
@@ -1398,14 +1419,19 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
             if (lane_id() == 0) sbase = atomicAdd(&counters[CNT_SEGMENTS], wtot);
             sbase = __shfl(sbase, 0);
         }
-        if (r.hit && slot < rec_cap) {
+        if (hit && slot < rec_cap) {
             rec_key[slot] = ((uint64_t)a << bitsB) | (uint64_t)b;
             rec_seg[slot] = make_uint2(sbase + soff, nseg);
             for (uint32_t sI = 0; sI < nseg; ++sI) {
                 uint32_t dst = sbase + soff + sI;
                 if (dst >= seg_cap) break;
                 GridExact::HP p, q;
-                GridExact::segment(PA, PB, r, (int)sI, p, q);
+                if constexpr (kFast) {
+                    p = r.p;
+                    q = r.q;
+                } else {
+                    GridExact::segment(PA, PB, r, (int)sI, p, q);
+                }
                 __int128* o = seg_pool + 8 * (size_t)dst;
                 o[0] = p.h[0]; o[1] = p.h[1]; o[2] = p.h[2]; o[3] = p.w;
                 o[4] = q.h[0]; o[5] = q.h[1]; o[6] = q.h[2]; o[7] = q.w;
@@ -1422,10 +1448,13 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
 
 void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
                   int bitsB, uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, __int128* seg_pool,
-                  uint32_t seg_cap, uint32_t* counters, int blocks, hipStream_t s) {
+                  uint32_t seg_cap, uint2* defer, uint32_t defer_cap, uint32_t* counters, int blocks, hipStream_t s) {
     if (A.nt == 0 || B.nt == 0) return;
-    k_exact<<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap, seg_pool,
-                                 seg_cap, counters);
+    k_exact<true><<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
+                                         seg_pool, seg_cap, defer, defer_cap, counters);
+    // the deferred (coplanar) pairs: their count is read on the device
+    k_exact<false><<<blocks, 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
+                                          seg_pool, seg_cap, nullptr, 0, counters);
     GI_LAUNCH_CHECK();
 }
 

@@ -36,7 +36,8 @@ _STAT_F32 = ["ms_total", "ms_build", "ms_traverse", "ms_filter", "ms_exact", "ms
 
 class GiStats(C.Structure):
     _fields_ = ([(n, C.c_uint64) for n in _STAT_U64] + [("grid", C.c_int32), ("retries", C.c_int32)]
-                + [(n, C.c_float) for n in _STAT_F32] + [("bvh_chunk_launches", C.c_int32), ("bvh_chunk_leaves", C.c_uint32)])
+                + [(n, C.c_float) for n in _STAT_F32] + [("bvh_chunk_launches", C.c_int32), ("bvh_chunk_leaves", C.c_uint32),
+                                                              ("graph", C.c_int32), ("pad", C.c_int32)])
 
     def as_dict(self):
         d = {n: int(getattr(self, n)) for n in _STAT_U64}
@@ -45,6 +46,7 @@ class GiStats(C.Structure):
         d["retries"] = int(self.retries)
         d["bvh_chunk_launches"] = int(self.bvh_chunk_launches)
         d["bvh_chunk_leaves"] = int(self.bvh_chunk_leaves)
+        d["graph"] = int(self.graph)
         return d
 
 

@@ -82,7 +82,7 @@ typedef struct {
     int32_t rebuild;            /* 1: rebuild both BVHs from scratch before the query */
     int32_t use_graph;          /* 1: replay the query's launches up to the counter read-back as a
                                    HIP graph while sizes and buffers stay the same (captured on
-                                   the second such query); stage timings are then not reported */
+                                   the second such query; its timing events are graph nodes) */
 } gi_options;
 
 typedef struct {
@@ -102,6 +102,8 @@ typedef struct {
     float ms_bvh_chunks;        /* the k_build_chunks launch of this call (both trees in one launch) */
     int32_t bvh_chunk_launches; /* 0 or 1 */
     uint32_t bvh_chunk_leaves;  /* leaves that launch built or refit */
+    int32_t graph;              /* 1: the launches up to the counter read-back replayed a HIP graph */
+    int32_t pad;
 } gi_stats;
 
 typedef struct {
