@@ -76,13 +76,13 @@ struct MeshSlot {
     std::vector<uint32_t> h_voff, h_toff, h_vtiles, h_ttiles, h_vtile0;
     int morton_axis = 10;      // Morton bits per axis of the kept sorted keys
     uint64_t generation = 0;   // bumped whenever the slot's mesh or buffers are replaced
-    DevBuf src, tri, v64, vi, lo, hi, keys, vals, tkeys, tvals, nodes, range, blocks;
+    DevBuf src, tri, v64, vi, tbox, keys, vals, tkeys, tvals, nodes, range, blocks;
     DevBuf voff, toff, vtiles, ttiles, vtile0, fbounds, roots;
     void release() {
         ++generation;
         DevBuf* forest_bufs[] = {&voff, &toff, &vtiles, &ttiles, &vtile0, &fbounds, &roots};
         for (DevBuf* b : forest_bufs) b->release();
-        DevBuf* all[] = {&src, &tri, &v64, &vi, &lo, &hi, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks};
+        DevBuf* all[] = {&src, &tri, &v64, &vi, &tbox, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks};
         for (DevBuf* b : all) {
             if (borrowed) { b->p = nullptr; b->bytes = 0; }
             else b->release();
@@ -238,8 +238,7 @@ void stage_forest(gi_context* ctx, int slot) {
     const uint32_t n_vtiles = (uint32_t)(m.h_vtiles.size() / 2), n_ttiles = (uint32_t)(m.h_ttiles.size() / 2);
     double4* v64 = m.v64.ensure<double4>(nv);
     int4* vi = m.vi.ensure<int4>(nv);
-    float4* lo = m.lo.ensure<float4>(nt);
-    float4* hi = m.hi.ensure<float4>(nt);
+    float4* tbox = m.tbox.ensure<float4>(2 * nt);
     m.nodes.ensure<float4>(2 * (nt ? 2 * nt - 1 : 1));
     m.range.ensure<uint2>(nt);
     m.blocks.ensure<float4>(block_levels_elems((uint32_t)nt));
@@ -256,7 +255,7 @@ void stage_forest(gi_context* ctx, int slot) {
     for (uint32_t i = 0; i < M; ++i) largest = std::max(largest, m.h_toff[i + 1] - m.h_toff[i]);
     m.morton_axis = morton_axis_bits(largest);
     launch_tri_boxes_forest(m.tri.get<uint32_t>(), m.ttiles.get<uint2>(), n_ttiles, m.toff.get<uint32_t>(), v64, vi,
-                            grid + slot, bounds, m.morton_axis, lo, hi, keys, vals, s);
+                            grid + slot, bounds, m.morton_axis, tbox, keys, vals, s);
     uint32_t* hist = ctx->bhist[slot].ensure<uint32_t>(sort_hist_elems(nt));
     uint32_t* stmp = ctx->bscan[slot].ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
     m.sorted_in_tmp = radix_sort_u64(keys, vals, tk, tv, m.nt, 3 * m.morton_axis + bits_for(M), hist, stmp, s);
@@ -280,8 +279,7 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
     size_t nv = m.nv, nt = m.nt;
     double4* v64 = m.v64.ensure<double4>(nv);
     int4* vi = m.vi.ensure<int4>(nv);
-    float4* lo = m.lo.ensure<float4>(nt);
-    float4* hi = m.hi.ensure<float4>(nt);
+    float4* tbox = m.tbox.ensure<float4>(2 * nt);
     m.nodes.ensure<float4>(2 * (nt ? 2 * nt - 1 : 1));
     m.range.ensure<uint2>(nt);
     m.blocks.ensure<float4>(block_levels_elems((uint32_t)nt));
@@ -297,7 +295,7 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
     HIP_TRY(hipMemcpyAsync(d_xf, ctx->h_xf + slot, sizeof(Xform), hipMemcpyHostToDevice, s));
     launch_prepare_vertices(m.src.get<double>(), m.nv, d_xf, v64, vi, grid + slot, partials, bounds, s);
     if (topology) m.morton_axis = morton_axis_bits(m.nt);
-    launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, vi, grid + slot, bounds, m.morton_axis, lo, hi,
+    launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, vi, grid + slot, bounds, m.morton_axis, tbox,
                      topology ? keys : nullptr, topology ? vals : nullptr, s);
     if (topology) {
         uint32_t* hist = ctx->bhist[slot].ensure<uint32_t>(sort_hist_elems(nt));
@@ -313,8 +311,7 @@ BuildJob<K> slot_job(MeshSlot& m) {
     j.keys = m.sorted_in_tmp ? m.tkeys.get<K>() : m.keys.get<K>();
     j.prims = m.sorted_in_tmp ? m.tvals.get<uint32_t>() : m.vals.get<uint32_t>();
     j.n = m.nt;
-    j.lo = m.lo.get<float4>();
-    j.hi = m.hi.get<float4>();
+    j.boxes = m.tbox.get<float4>();
     j.nodes = m.nodes.get<float4>();
     j.range = m.range.get<uint2>();
     j.blocks = m.blocks.get<float4>();
@@ -481,7 +478,7 @@ std::vector<uint64_t> presync_signature(gi_context* ctx, const gi_options& o) {
         v.push_back(m.generation);
         v.push_back(m.h_vtiles.size());
         v.push_back(m.h_ttiles.size());
-        for (const DevBuf* b : {&m.src, &m.tri, &m.v64, &m.vi, &m.lo, &m.hi, &m.keys, &m.vals, &m.tkeys, &m.tvals,
+        for (const DevBuf* b : {&m.src, &m.tri, &m.v64, &m.vi, &m.tbox, &m.keys, &m.vals, &m.tkeys, &m.tvals,
                                 &m.nodes, &m.range, &m.blocks, &m.voff, &m.toff, &m.vtiles, &m.ttiles, &m.vtile0,
                                 &m.fbounds, &m.roots})
             buf(*b);
@@ -1119,11 +1116,12 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         d.sorted_in_tmp = s.sorted_in_tmp;
         d.morton_axis = s.morton_axis;
         d.borrowed = true;
-        DevBuf* dd[] = {&d.src, &d.tri, &d.v64, &d.vi, &d.lo, &d.hi, &d.keys, &d.vals, &d.tkeys, &d.tvals, &d.nodes,
+        DevBuf* dd[] = {&d.src, &d.tri, &d.v64, &d.vi, &d.tbox, &d.keys, &d.vals, &d.tkeys, &d.tvals, &d.nodes,
                         &d.range, &d.blocks};
-        const DevBuf* ss[] = {&s.src, &s.tri, &s.v64, &s.vi, &s.lo, &s.hi, &s.keys, &s.vals, &s.tkeys, &s.tvals,
+        const DevBuf* ss[] = {&s.src, &s.tri, &s.v64, &s.vi, &s.tbox, &s.keys, &s.vals, &s.tkeys, &s.tvals,
                               &s.nodes, &s.range, &s.blocks};
-        for (int i = 0; i < 13; ++i) { dd[i]->p = ss[i]->p; dd[i]->bytes = ss[i]->bytes; }
+        static_assert(sizeof(dd) / sizeof(dd[0]) == sizeof(ss) / sizeof(ss[0]), "buffer lists differ");
+        for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); ++i) { dd[i]->p = ss[i]->p; dd[i]->bytes = ss[i]->bytes; }
         // the owner's grid classification of this mesh
         uint32_t* dgrid = dst->counters.ensure<uint32_t>(CNT_COUNT + 2) + CNT_COUNT;
         const uint32_t* sgrid = src->counters.get<uint32_t>() + CNT_COUNT;

@@ -66,7 +66,7 @@ void launch_prepare_vertices(const double* src, uint32_t nv, const Xform* xf, do
 // Triangle boxes; with keys/vals non-null also Morton keys (frame bounds6)
 // and the identity permutation.
 void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,
-                      const uint32_t* bounds6, int axis_bits, float4* lo, float4* hi, uint32_t* keys, uint32_t* vals,
+                      const uint32_t* bounds6, int axis_bits, float4* box, uint32_t* keys, uint32_t* vals,
                       hipStream_t s);
 
 // Forest (a batch of meshes stored back to back, indices global): tiles of
@@ -81,7 +81,7 @@ void launch_prepare_forest(const double* src, const uint2* vtiles, uint32_t n_vt
 // boxes and keys mesh << 3 axis_bits | Morton code (sort on 3 axis_bits + bits(M) bits)
 void launch_tri_boxes_forest(const uint32_t* tri, const uint2* ttiles, uint32_t n_ttiles, const uint32_t* toff,
                              const double4* v64, const int4* vi, const uint32_t* grid_flag, const uint32_t* bounds,
-                             int axis_bits, float4* lo, float4* hi, uint64_t* keys, uint32_t* vals, hipStream_t s);
+                             int axis_bits, float4* box, uint64_t* keys, uint32_t* vals, hipStream_t s);
 
 // LSD radix sort of (key, value) pairs, 8 bits per pass. tmp_* are ping-pong
 // buffers of n elements; hist needs sort_hist_elems(n) entries, scan_tmp
@@ -116,8 +116,7 @@ struct BuildJob {
     const K* keys;
     const uint32_t* prims;
     uint32_t n;
-    const float4* lo;
-    const float4* hi;
+    const float4* boxes;  // triangle boxes: lo, hi of triangle t at 2t, 2t + 1
     float4* nodes;
     uint2* range;
     float4* blocks;

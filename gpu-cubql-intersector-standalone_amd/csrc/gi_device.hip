@@ -245,14 +245,14 @@ __global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ 
                                                    const double4* __restrict__ v, const int4* __restrict__ vi,
                                                    const uint32_t* __restrict__ grid_flag,
                                                    const uint32_t* __restrict__ bounds6, int axis_bits,
-                                                   float4* __restrict__ lo, float4* __restrict__ hi,
-                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                                   float4* __restrict__ box, uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     float bl[3], bh[3];
     tri_box(tri, i, *grid_flag != 0, v, vi, bl, bh);
-    lo[i] = make_float4(bl[0], bl[1], bl[2], 0.f);
-    hi[i] = make_float4(bh[0], bh[1], bh[2], 0.f);
+    box[2 * (size_t)i] = make_float4(bl[0], bl[1], bl[2], 0.f);
+    box[2 * (size_t)i + 1] = make_float4(bh[0], bh[1], bh[2], 0.f);
     if (keys) {
         float bmin[3], bext[3];
         morton_frame(bounds6, bmin, bext);
@@ -262,10 +262,10 @@ __global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ 
 }
 
 void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,
-                      const uint32_t* bounds6, int axis_bits, float4* lo, float4* hi, uint32_t* keys, uint32_t* vals,
+                      const uint32_t* bounds6, int axis_bits, float4* box, uint32_t* keys, uint32_t* vals,
                       hipStream_t s) {
     if (nt == 0) return;
-    k_tri_boxes<<<blocks_for(nt, 256), 256, 0, s>>>(tri, nt, v64, vi, grid_flag, bounds6, axis_bits, lo, hi, keys,
+    k_tri_boxes<<<blocks_for(nt, 256), 256, 0, s>>>(tri, nt, v64, vi, grid_flag, bounds6, axis_bits, box, keys,
                                                     vals);
     GI_LAUNCH_CHECK();
 }
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void k_tri_boxes_forest(const uint32_t* __rest
                                                           const double4* __restrict__ v, const int4* __restrict__ vi,
                                                           const uint32_t* __restrict__ grid_flag,
                                                           const uint32_t* __restrict__ bounds, int axis_bits,
-                                                          float4* __restrict__ lo, float4* __restrict__ hi,
+                                                          float4* __restrict__ box,
                                                           uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint2 t = tiles[blockIdx.x];
     const uint32_t end = min(t.y + kForestTile, toff[t.x + 1]);
@@ -345,8 +345,8 @@ __global__ __launch_bounds__(256) void k_tri_boxes_forest(const uint32_t* __rest
     for (uint32_t i = t.y + threadIdx.x; i < end; i += blockDim.x) {
         float bl[3], bh[3];
         tri_box(tri, i, grid, v, vi, bl, bh);
-        lo[i] = make_float4(bl[0], bl[1], bl[2], 0.f);
-        hi[i] = make_float4(bh[0], bh[1], bh[2], 0.f);
+        box[2 * (size_t)i] = make_float4(bl[0], bl[1], bl[2], 0.f);
+        box[2 * (size_t)i + 1] = make_float4(bh[0], bh[1], bh[2], 0.f);
         keys[i] = ((uint64_t)t.x << (3 * axis_bits)) | morton_code(bl, bh, bmin, bext, axis_bits);
         vals[i] = i;
     }
@@ -364,9 +364,9 @@ void launch_prepare_forest(const double* src, const uint2* vtiles, uint32_t n_vt
 
 void launch_tri_boxes_forest(const uint32_t* tri, const uint2* ttiles, uint32_t n_ttiles, const uint32_t* toff,
                              const double4* v64, const int4* vi, const uint32_t* grid_flag, const uint32_t* bounds,
-                             int axis_bits, float4* lo, float4* hi, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+                             int axis_bits, float4* box, uint64_t* keys, uint32_t* vals, hipStream_t s) {
     if (n_ttiles == 0) return;
-    k_tri_boxes_forest<<<n_ttiles, 256, 0, s>>>(tri, ttiles, toff, v64, vi, grid_flag, bounds, axis_bits, lo, hi, keys,
+    k_tri_boxes_forest<<<n_ttiles, 256, 0, s>>>(tri, ttiles, toff, v64, vi, grid_flag, bounds, axis_bits, box, keys,
                                                 vals);
     GI_LAUNCH_CHECK();
 }
@@ -743,14 +743,17 @@ int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t*
 
 // --------------------------------------------------------- BVH (Karras)
 //
-// Build and refit are one kernel per chunk of CHUNK (256 or 512) sorted leaves:
-// the chunk's Morton keys sit in LDS for Karras' prefix searches, every
-// internal node whose leaf range lies inside the chunk is refit bottom-up
-// through LDS only (parents, merge flags and merged boxes never leave LDS,
-// so no device-scope fence is needed; on gfx950 those write back and
-// invalidate the per-XCD L2, which made a global-atomic refit cost 6 ms for
-// 2^20 triangles). The few "top" nodes whose range crosses a chunk boundary
-// take the union of their leaf range from a 64-ary hierarchy of block boxes.
+// Build and refit are one kernel per chunk of CHUNK (256 or 512) sorted
+// leaves, one thread per leaf and per internal node: the chunk's Morton keys
+// sit in LDS for Karras' prefix searches, and every internal node whose leaf
+// range lies inside the chunk takes the union of its leaves from a table of
+// aligned block boxes (1, 2, 4, ... 64 leaves) built in LDS by wave
+// butterflies. No bottom-up merge: no parents, flags or atomics, one barrier
+// (a global-atomic bottom-up refit, with the device-scope fences it needs on
+// gfx950's per-XCD L2s, cost 6 ms for 2^20 triangles; an LDS-atomic one
+// still a quarter of this kernel). The few "top" nodes whose range crosses a
+// chunk boundary take the union of their leaf range from a 64-ary hierarchy
+// of block boxes.
 // A refit after a rigid motion reruns the same kernels on the same sorted
 // keys: the topology comes out identical, only the boxes change.
 
@@ -825,8 +828,7 @@ template <class K>
 struct ChunkJobs {
     const K* keys0; const K* keys1;
     const uint32_t* prims0; const uint32_t* prims1;
-    const float4* lo0; const float4* lo1;
-    const float4* hi0; const float4* hi1;
+    const float4* box0; const float4* box1;  // triangle boxes, (lo, hi) per triangle
     float4* nodes0; float4* nodes1;
     uint2* range0; uint2* range1;
     float4* level1_0; float4* level1_1;  // boxes of the 64-leaf blocks (first level of the block hierarchy)
@@ -834,111 +836,101 @@ struct ChunkJobs {
     uint32_t split;
 };
 
+// First box of level k in a chunk's aligned-block table (level k holds the
+// CHUNK >> k boxes of leaves [m 2^k, (m + 1) 2^k), k = 0..6).
+__host__ __device__ constexpr int block_table_offset(int chunk, int k) { return 2 * (chunk - (chunk >> k)); }
+
 template <class K, int CHUNK>
-__global__ __launch_bounds__(256) void k_build_chunks(ChunkJobs<K> J) {
+__global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
     const bool second = blockIdx.x >= J.split;
     const K* __restrict__ keys = second ? J.keys1 : J.keys0;
     const uint32_t* __restrict__ prims = second ? J.prims1 : J.prims0;
-    const float4* __restrict__ lo = second ? J.lo1 : J.lo0;
-    const float4* __restrict__ hi = second ? J.hi1 : J.hi0;
+    const float4* __restrict__ tbox = second ? J.box1 : J.box0;
     float4* __restrict__ nodes = second ? J.nodes1 : J.nodes0;
     uint2* __restrict__ range = second ? J.range1 : J.range0;
     float4* __restrict__ level1 = second ? J.level1_1 : J.level1_0;
     const int n = second ? J.n1 : J.n0;
     const uint32_t chunk = blockIdx.x - (second ? J.split : 0u);
     __shared__ K skey[CHUNK];
-    __shared__ uint32_t par_leaf[CHUNK];
-    __shared__ uint32_t par_int[CHUNK];
-    __shared__ uint32_t left_int[CHUNK];
-    __shared__ uint32_t rope_int[CHUNK];
-    __shared__ uint32_t flag[CHUNK];
-    __shared__ uint32_t ubox[CHUNK][6];  // merged boxes as order-preserving uint keys
-    const int cs = (int)chunk * (int)CHUNK;
-    const int ce = min(n, cs + (int)CHUNK);
-    for (int k = threadIdx.x; k < (int)CHUNK; k += blockDim.x) {
-        if (cs + k < ce) skey[k] = keys[cs + k];
-        par_leaf[k] = kSentinel;
-        par_int[k] = kSentinel;
-        flag[k] = 0;
-        ubox[k][0] = ubox[k][1] = ubox[k][2] = 0xFFFFFFFFu;
-        ubox[k][3] = ubox[k][4] = ubox[k][5] = 0u;
+    __shared__ float4 sbox[2 * block_table_offset(CHUNK, 7)];  // (lo, hi) pairs of levels 0..6
+    const int t = (int)threadIdx.x;
+    const int cs = (int)chunk * CHUNK;
+    const int ce = min(n, cs + CHUNK);
+    const int jl = cs + t;  // this thread's leaf, and its internal node
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    uint32_t prim = 0;
+    if (jl < ce) {
+        skey[t] = keys[jl];
+        prim = prims[jl];
+        float4 l = tbox[2 * (size_t)prim], h = tbox[2 * (size_t)prim + 1];
+        b[0] = l.x; b[1] = l.y; b[2] = l.z;
+        b[3] = h.x; b[4] = h.y; b[5] = h.z;
+    }
+    const float leaf[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
+    sbox[2 * t] = make_float4(b[0], b[1], b[2], 0.f);
+    sbox[2 * t + 1] = make_float4(b[3], b[4], b[5], 0.f);
+    // aligned 2^k-leaf blocks by butterfly reduction within the wave: after
+    // the exchange at distance 2^(k-1) a lane holds the union of its block
+#pragma unroll
+    for (int k = 1; k <= 6; ++k) {
+        const int o = 1 << (k - 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) b[c] = fminf(b[c], __shfl_xor(b[c], o));
+#pragma unroll
+        for (int c = 3; c < 6; ++c) b[c] = fmaxf(b[c], __shfl_xor(b[c], o));
+        if ((t & ((1 << k) - 1)) == 0) {
+            const int e = 2 * (block_table_offset(CHUNK, k) + (t >> k));
+            sbox[e] = make_float4(b[0], b[1], b[2], 0.f);
+            sbox[e + 1] = make_float4(b[3], b[4], b[5], 0.f);
+        }
+    }
+    // the 64-leaf block is the first level of the block hierarchy
+    if ((t & 63) == 0 && jl < ce) {
+        level1[2 * (size_t)(jl >> 6)] = make_float4(b[0], b[1], b[2], 0.f);
+        level1[2 * (size_t)(jl >> 6) + 1] = make_float4(b[3], b[4], b[5], 0.f);
     }
     __syncthreads();
     const KeySrc<K> KS{keys, skey, cs, ce, n};
-    const int iend = min(ce, n - 1);
-    for (int i = cs + threadIdx.x; i < iend; i += blockDim.x) {
-        int d = delta(KS, i, i + 1) > delta(KS, i, i - 1) ? 1 : -1;
-        int dmin = delta(KS, i, i - d);
-        // one probe past the chunk edge tells whether the range leaves the
-        // chunk (sorted keys: the common prefix only shrinks with distance);
-        // an in-chunk node then searches the LDS keys only
-        const int edge = d > 0 ? ce : cs - 1;
-        const bool top = edge >= 0 && edge < n && delta(KS, i, edge) > dmin;
-        int lo_i, hi_i;
-        uint32_t left, right, rope;
-        karras_node(KS, i, d, dmin, top ? 0 : cs, top ? n - 1 : ce - 1, lo_i, hi_i, left, right, rope);
-        range[i] = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
-        if (!top) {  // both children are in the chunk too
-            left_int[i - cs] = left;
-            rope_int[i - cs] = rope;
-            uint32_t c[2] = {left, right};
-            for (int e = 0; e < 2; ++e) {
-                if (c[e] >= (uint32_t)(n - 1)) par_leaf[c[e] - (uint32_t)(n - 1) - (uint32_t)cs] = (uint32_t)i;
-                else par_int[c[e] - (uint32_t)cs] = (uint32_t)i;
-            }
-        } else {
-            // top node: written whole here, its box comes from k_top_boxes;
-            // in-chunk nodes are written once, with their box, below
-            nodes[2 * (size_t)i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(left));
-            nodes[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rope));
+    if (jl < ce) {
+        const uint32_t rope = jl == n - 1 ? kSentinel : chain_top(KS, jl + 1);
+        nodes[2 * (size_t)(n - 1 + jl)] = make_float4(leaf[0], leaf[1], leaf[2], __uint_as_float(prim));
+        nodes[2 * (size_t)(n - 1 + jl) + 1] = make_float4(leaf[3], leaf[4], leaf[5], __uint_as_float(rope));
+    }
+    if (jl >= min(ce, n - 1)) return;
+    const int d = delta(KS, jl, jl + 1) > delta(KS, jl, jl - 1) ? 1 : -1;
+    const int dmin = delta(KS, jl, jl - d);
+    // one probe past the chunk edge tells whether the range leaves the chunk
+    // (sorted keys: the common prefix only shrinks with distance); an
+    // in-chunk node then searches the LDS keys only
+    const int edge = d > 0 ? ce : cs - 1;
+    const bool top = edge >= 0 && edge < n && delta(KS, jl, edge) > dmin;
+    int lo_i, hi_i;
+    uint32_t left, right, rope;
+    karras_node(KS, jl, d, dmin, top ? 0 : cs, top ? n - 1 : ce - 1, lo_i, hi_i, left, right, rope);
+    range[jl] = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
+    float q[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (!top) {
+        // the union of leaves [lo_i, hi_i] as disjoint aligned blocks, at
+        // most two per level below 64 and whole 64-leaf blocks above
+        q[0] = q[1] = q[2] = INFINITY;
+        q[3] = q[4] = q[5] = -INFINITY;
+        int L = lo_i - cs, R = hi_i - cs + 1;
+        auto absorb = [&](int k, int m) {
+            const int e = 2 * (block_table_offset(CHUNK, k) + m);
+            const float4 l = sbox[e], h = sbox[e + 1];
+            q[0] = fminf(q[0], l.x); q[1] = fminf(q[1], l.y); q[2] = fminf(q[2], l.z);
+            q[3] = fmaxf(q[3], h.x); q[4] = fmaxf(q[4], h.y); q[5] = fmaxf(q[5], h.z);
+        };
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if (L < R && ((L >> k) & 1)) { absorb(k, L >> k); L += 1 << k; }
+            if (L < R && ((R >> k) & 1)) { absorb(k, (R >> k) - 1); R -= 1 << k; }
         }
+        for (int m = L >> 6; m < (R >> 6); ++m) absorb(6, m);
     }
-    float lb[CHUNK / 256][6];
-    for (int u = 0; u < (int)(CHUNK / 256); ++u) {
-        int jl = cs + threadIdx.x + 256 * u;
-        if (jl >= ce) continue;
-        uint32_t prim = prims[jl];
-        float4 l = lo[prim], h = hi[prim];
-        uint32_t rope = jl == n - 1 ? kSentinel : chain_top(KS, jl + 1);
-        nodes[2 * (size_t)(n - 1 + jl)] = make_float4(l.x, l.y, l.z, __uint_as_float(prim));
-        nodes[2 * (size_t)(n - 1 + jl) + 1] = make_float4(h.x, h.y, h.z, __uint_as_float(rope));
-        lb[u][0] = l.x; lb[u][1] = l.y; lb[u][2] = l.z;
-        lb[u][3] = h.x; lb[u][4] = h.y; lb[u][5] = h.z;
-    }
-    // a wave holds 64 consecutive leaves per u (chunks are 64-aligned): their
-    // union is one box of the first block level
-    for (int u = 0; u < (int)(CHUNK / 256); ++u) {
-        const int wfirst = cs + 256 * u + 64 * (int)(threadIdx.x >> 6);
-        if (wfirst >= ce) continue;
-        float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-        if (cs + (int)threadIdx.x + 256 * u < ce)
-            for (int k = 0; k < 6; ++k) b[k] = lb[u][k];
-        wave_box_reduce(b);
-        if (lane_id() == 0) {
-            level1[2 * (size_t)(wfirst >> 6)] = make_float4(b[0], b[1], b[2], 0.f);
-            level1[2 * (size_t)(wfirst >> 6) + 1] = make_float4(b[3], b[4], b[5], 0.f);
-        }
-    }
-    __syncthreads();
-    for (int u = 0; u < (int)(CHUNK / 256); ++u) {
-        int jl = cs + threadIdx.x + 256 * u;
-        if (jl >= ce) continue;
-        float b[6];
-        for (int k = 0; k < 6; ++k) b[k] = lb[u][k];
-        uint32_t p = par_leaf[jl - cs];
-        while (p != kSentinel) {
-            uint32_t slot = p - (uint32_t)cs;
-            for (int k = 0; k < 3; ++k) atomicMin(&ubox[slot][k], f2key(b[k]));
-            for (int k = 3; k < 6; ++k) atomicMax(&ubox[slot][k], f2key(b[k]));
-            __threadfence_block();
-            if (atomicAdd(&flag[slot], 1u) == 0) break;  // the sibling continues upward
-            __threadfence_block();
-            for (int k = 0; k < 6; ++k) b[k] = key2f(atomicAdd(&ubox[slot][k], 0u));
-            nodes[2 * (size_t)p] = make_float4(b[0], b[1], b[2], __uint_as_float(left_int[slot]));
-            nodes[2 * (size_t)p + 1] = make_float4(b[3], b[4], b[5], __uint_as_float(rope_int[slot]));
-            p = par_int[slot];
-        }
-    }
+    // a top node gets its box from k_top_boxes
+    nodes[2 * (size_t)jl] = make_float4(q[0], q[1], q[2], __uint_as_float(left));
+    nodes[2 * (size_t)jl + 1] = make_float4(q[3], q[4], q[5], __uint_as_float(rope));
 }
 
 // one wave per output block of 64 input elements; blocks [0, split) serve
@@ -1066,16 +1058,16 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     const uint32_t n0 = j0->n, n1 = j1 ? j1->n : 0;
     if (n0 + n1 == 0) return;
     ChunkJobs<K> cj;
-    cj.keys0 = j0->keys; cj.prims0 = j0->prims; cj.lo0 = j0->lo; cj.hi0 = j0->hi;
+    cj.keys0 = j0->keys; cj.prims0 = j0->prims; cj.box0 = j0->boxes;
     cj.nodes0 = j0->nodes; cj.range0 = j0->range; cj.level1_0 = j0->blocks; cj.n0 = (int)n0;
     const BuildJob<K>* o = j1 ? j1 : j0;
-    cj.keys1 = o->keys; cj.prims1 = o->prims; cj.lo1 = o->lo; cj.hi1 = o->hi;
+    cj.keys1 = o->keys; cj.prims1 = o->prims; cj.box1 = o->boxes;
     cj.nodes1 = o->nodes; cj.range1 = o->range; cj.level1_1 = o->blocks; cj.n1 = (int)n1;
     cj.split = blocks_for(n0, chunk);
     const uint32_t chunk_blocks = cj.split + blocks_for(n1, chunk);
     if (chunk_begin) (void)hipEventRecordWithFlags(chunk_begin, s, event_flags);
     if (chunk == 256) k_build_chunks<K, 256><<<chunk_blocks, 256, 0, s>>>(cj);
-    else k_build_chunks<K, 512><<<chunk_blocks, 256, 0, s>>>(cj);
+    else k_build_chunks<K, 512><<<chunk_blocks, 512, 0, s>>>(cj);
     if (chunk_end) (void)hipEventRecordWithFlags(chunk_end, s, event_flags);
     // 64-ary levels of leaf boxes for the nodes that cross chunks
     BlockLevels lv[2];
