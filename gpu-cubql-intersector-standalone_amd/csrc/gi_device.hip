@@ -32,6 +32,17 @@ namespace gi {
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8), each XCD with its own L2. Renumbered so that XCD x takes one
+// contiguous run of the G logical blocks, neighbouring work (adjacent chunks,
+// neighbouring leaves along the Morton curve) shares one L2. A bijection on
+// [0, G) for any G.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
+    constexpr uint32_t X = 8;
+    const uint32_t q = G / X, r = G % X, x = b % X;
+    return x * q + min(x, r) + b / X;
+}
+
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -842,7 +853,8 @@ __host__ __device__ constexpr int block_table_offset(int chunk, int k) { return 
 
 template <class K, int CHUNK>
 __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
-    const bool second = blockIdx.x >= J.split;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const bool second = blk >= J.split;
     const K* __restrict__ keys = second ? J.keys1 : J.keys0;
     const uint32_t* __restrict__ prims = second ? J.prims1 : J.prims0;
     const float4* __restrict__ tbox = second ? J.box1 : J.box0;
@@ -850,7 +862,7 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
     uint2* __restrict__ range = second ? J.range1 : J.range0;
     float4* __restrict__ level1 = second ? J.level1_1 : J.level1_0;
     const int n = second ? J.n1 : J.n0;
-    const uint32_t chunk = blockIdx.x - (second ? J.split : 0u);
+    const uint32_t chunk = blk - (second ? J.split : 0u);
     __shared__ K skey[CHUNK];
     __shared__ float4 sbox[2 * block_table_offset(CHUNK, 7)];  // (lo, hi) pairs of levels 0..6
     const int t = (int)threadIdx.x;
@@ -1014,12 +1026,13 @@ struct TopBoxJobs {
 };
 
 __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J, uint32_t chunk) {
-    const bool second = blockIdx.x >= J.split;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const bool second = blk >= J.split;
     const uint32_t n = second ? J.n1 : J.n0;
     const uint2* __restrict__ range = second ? J.range1 : J.range0;
     float4* nodes = second ? J.nodes1 : J.nodes0;
     const BlockLevels& lv = second ? J.lv1 : J.lv0;  // read in place from the kernel arguments
-    uint32_t base = ((blockIdx.x - (second ? J.split : 0u)) * 4 + (threadIdx.x >> 6)) * 64;
+    uint32_t base = ((blk - (second ? J.split : 0u)) * 4 + (threadIdx.x >> 6)) * 64;
     uint32_t i = base + lane_id();
     uint2 r = make_uint2(0, 0);
     bool top = false;
@@ -1177,7 +1190,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
                                                            uint32_t* __restrict__ counters) {
     __shared__ uint2 stage_all[TRAV_THREADS / 64][TRAV_STAGE];
     uint2* stage = stage_all[threadIdx.x >> 6];
-    uint32_t k = blockIdx.x * TRAV_THREADS + threadIdx.x;
+    uint32_t k = xcd_block(blockIdx.x, gridDim.x) * TRAV_THREADS + threadIdx.x;
     float qlx = 0.f, qly = 0.f, qlz = 0.f, qhx = 0.f, qhy = 0.f, qhz = 0.f;
     uint32_t b = 0;
     uint32_t node = kSentinel, end = kSentinel;
