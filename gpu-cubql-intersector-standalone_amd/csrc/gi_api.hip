@@ -424,6 +424,17 @@ void empty_result(gi_result* out, int on_device) {
 // the counter reset, traversal, filter, exact stage and the copy of the
 // counters to pinned host memory. `timed` records the stage events (not
 // inside a graph capture).
+// Segment slots of the output buffers that the pre-read-back output kernels
+// write: room for the largest result set they take (at most
+// output_small_max() records, each with at most kMaxSegmentsPerPair segments,
+// within the arenas).
+constexpr uint32_t kMaxSegmentsPerPair = 6;
+
+size_t output_seg_slots(const gi_context* ctx) {
+    const uint32_t cap = std::min(output_small_max(), ctx->cap_rec);
+    return std::max<size_t>(std::min<size_t>((size_t)kMaxSegmentsPerPair * cap, ctx->cap_seg), 1);
+}
+
 void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool timed) {
     hipStream_t s = ctx->stream;
     MeshSlot& MA = ctx->mesh[0];
@@ -451,8 +462,26 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
                  ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<__int128>(), ctx->cap_seg,
                  ctx->cand.get<uint2>(), ctx->cap_cand, counters, ctx->tune.exact_blocks, s);
     if (timed) record(ctx, ctx->ev[4], s);
+    // sorted output of a small result set, before the read-back (buffers
+    // from ensure_output_bufs)
+    launch_output_small(ctx->rec_key.get<uint64_t>(), ctx->rec_seg.get<uint2>(), counters, ctx->cap_rec, ctx->cap_seg,
+                        bits_for(MA.nt) + bitsB, bitsB, o.want_segments != 0, ctx->pairs.get<uint2>(),
+                        ctx->order.get<uint32_t>(), ctx->offs.get<uint32_t>(), ctx->segs.get<__int128>(),
+                        ctx->f64.get<double>(), (uint32_t)output_seg_slots(ctx), ctx->seg_pool.get<__int128>(), s);
+    if (timed) record(ctx, ctx->ev[5], s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(ctx->h_counters, counters, sizeof(uint32_t) * (CNT_COUNT + 2), hipMemcpyDeviceToHost, s));
+}
+
+void ensure_output_bufs(gi_context* ctx, const gi_options& o) {
+    const uint32_t cap = std::min(output_small_max(), ctx->cap_rec);
+    ctx->pairs.ensure<uint2>(std::max(cap, 1u));
+    ctx->order.ensure<uint32_t>(std::max(cap, 1u));
+    ctx->offs.ensure<uint32_t>((size_t)cap + 1);
+    if (o.want_segments) {
+        ctx->segs.ensure<__int128>(8 * output_seg_slots(ctx));
+        ctx->f64.ensure<double>(6 * output_seg_slots(ctx));
+    }
 }
 
 bool slot_needs_stage(const MeshSlot& m, bool rebuild) {
@@ -464,6 +493,7 @@ bool slot_needs_stage(const MeshSlot& m, bool rebuild) {
 std::vector<uint64_t> presync_signature(gi_context* ctx, const gi_options& o) {
     std::vector<uint64_t> v;
     v.push_back((uint64_t)o.rebuild);
+    v.push_back((uint64_t)o.want_segments);
     auto buf = [&](const DevBuf& b) {
         v.push_back((uint64_t)(uintptr_t)b.p);
         v.push_back((uint64_t)b.bytes);
@@ -485,7 +515,8 @@ std::vector<uint64_t> presync_signature(gi_context* ctx, const gi_options& o) {
     }
     for (const DevBuf* b : {&ctx->counters, &ctx->bounds, &ctx->xf_dev, &ctx->cand, &ctx->exact, &ctx->rec_key,
                             &ctx->rec_seg, &ctx->seg_pool, &ctx->bhist[0], &ctx->bhist[1], &ctx->bscan[0],
-                            &ctx->bscan[1], &ctx->bpart[0], &ctx->bpart[1]})
+                            &ctx->bscan[1], &ctx->bpart[0], &ctx->bpart[1], &ctx->pairs, &ctx->order, &ctx->offs,
+                            &ctx->segs, &ctx->f64})
         buf(*b);
     v.push_back((uint64_t)ctx->tune.traverse_variant);
     v.push_back((uint64_t)ctx->tune.filter_blocks);
@@ -605,10 +636,14 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     int bitsA = bits_for(MA.nt), bitsB = bits_for(MB.nt);
     const uint32_t* hc = ctx->h_counters;
     const uint32_t* hgrid = hc + CNT_COUNT;
+    ensure_output_bufs(ctx, o);
     const bool graphed = launch_presync_graph(ctx, o);
     if (!graphed) enqueue_presync(ctx, o, true, true);
     for (int attempt = 0;; ++attempt) {
-        if (attempt) enqueue_presync(ctx, o, false, true);  // trees are current: queries only
+        if (attempt) {  // trees are current: queries only
+            ensure_output_bufs(ctx, o);
+            enqueue_presync(ctx, o, false, true);
+        }
         HIP_TRY(hipStreamSynchronize(s));
         bool over = hc[CNT_TRAVERSAL_HITS] > ctx->cap_cand || hc[CNT_EXACT_LIST] > ctx->cap_exact ||
                     hc[CNT_RESULTS] > ctx->cap_rec || hc[CNT_SEGMENTS] > ctx->cap_seg;
@@ -647,34 +682,50 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         st.degenerate = hc[CNT_DEGENERATE];
         st.ms_exact = elapsed(ctx->ev[3], ctx->ev[4]);
         uint32_t R = hc[CNT_RESULTS], S = o.want_segments ? hc[CNT_SEGMENTS] : 0;
-        uint64_t* keys = ctx->rec_key.get<uint64_t>();
-        uint32_t* order = ctx->order.ensure<uint32_t>(R);
-        uint64_t* tk = ctx->tkeys.ensure<uint64_t>(R);
-        uint32_t* tv = ctx->tvals.ensure<uint32_t>(R);
-        uint32_t* hist = ctx->hist.ensure<uint32_t>(sort_hist_elems(R));
-        uint32_t* stmp = ctx->scan_tmp.ensure<uint32_t>(scan_tmp_elems(std::max(sort_hist_elems(R), (size_t)R + 1)));
-        uint2* pairs = ctx->pairs.ensure<uint2>(R);
-        uint32_t* nseg = ctx->nseg.ensure<uint32_t>(R);
-        uint32_t* offs = ctx->offs.ensure<uint32_t>((size_t)R + 1);
-        __int128* segs = ctx->segs.ensure<__int128>(8 * (size_t)S);
-        double* f64 = ctx->f64.ensure<double>(6 * (size_t)S);
-        launch_iota(order, R, s);
-        if (radix_sort_u64(keys, order, tk, tv, R, bitsA + bitsB, hist, stmp, s)) {
-            keys = tk;
-            order = tv;
+        // small result sets were sorted and gathered before the read-back
+        const bool done = output_small_fits(R, bitsA + bitsB);
+        if (!done) {
+            uint64_t* keys = ctx->rec_key.get<uint64_t>();
+            uint32_t* order = ctx->order.ensure<uint32_t>(R);
+            uint64_t* tk = ctx->tkeys.ensure<uint64_t>(R);
+            uint32_t* tv = ctx->tvals.ensure<uint32_t>(R);
+            uint32_t* hist = ctx->hist.ensure<uint32_t>(sort_hist_elems(R));
+            uint32_t* stmp = ctx->scan_tmp.ensure<uint32_t>(scan_tmp_elems(std::max(sort_hist_elems(R), (size_t)R + 1)));
+            uint2* pairs = ctx->pairs.ensure<uint2>(R);
+            uint32_t* nseg = ctx->nseg.ensure<uint32_t>(R);
+            uint32_t* offs = ctx->offs.ensure<uint32_t>((size_t)R + 1);
+            __int128* segs = ctx->segs.ensure<__int128>(8 * (size_t)S);
+            double* f64 = ctx->f64.ensure<double>(6 * (size_t)S);
+            launch_iota(order, R, s);
+            if (radix_sort_u64(keys, order, tk, tv, R, bitsA + bitsB, hist, stmp, s)) {
+                keys = tk;
+                order = tv;
+            }
+            launch_gather_pairs(keys, order, ctx->rec_seg.get<uint2>(), R, bitsB, pairs, nseg, s);
+            if (!o.want_segments) launch_fill_u32(nseg, 0u, R, s);
+            exclusive_scan_u32(nseg, offs, R, stmp, offs + R, s);
+            if (o.want_segments)
+                launch_gather_segments(order, ctx->rec_seg.get<uint2>(), offs, R, ctx->seg_pool.get<__int128>(), segs,
+                                       f64, s);
         }
-        launch_gather_pairs(keys, order, ctx->rec_seg.get<uint2>(), R, bitsB, pairs, nseg, s);
-        if (!o.want_segments) launch_fill_u32(nseg, 0u, R, s);
-        exclusive_scan_u32(nseg, offs, R, stmp, offs + R, s);
-        if (o.want_segments)
-            launch_gather_segments(order, ctx->rec_seg.get<uint2>(), offs, R, ctx->seg_pool.get<__int128>(), segs, f64, s);
+        if (done && S > output_seg_slots(ctx)) {  // more segments than the slots the device gather had
+            __int128* segs = ctx->segs.ensure<__int128>(8 * (size_t)S);
+            double* f64 = ctx->f64.ensure<double>(6 * (size_t)S);
+            launch_gather_segments(ctx->order.get<uint32_t>(), ctx->rec_seg.get<uint2>(), ctx->offs.get<uint32_t>(), R,
+                                   ctx->seg_pool.get<__int128>(), segs, f64, s);
+        }
+        const bool host_work = !done || S > output_seg_slots(ctx) || M;
+        uint2* pairs = ctx->pairs.get<uint2>();
+        uint32_t* offs = ctx->offs.get<uint32_t>();
+        __int128* segs = ctx->segs.get<__int128>();
+        double* f64 = ctx->f64.get<double>();
         if (M) {
             uint32_t* poff = ctx->pair_off.ensure<uint32_t>((size_t)M + 1);
             launch_forest_localize(pairs, R, MA.toff.get<uint32_t>(), MB.toff.get<uint32_t>(), M, poff, s);
             HIP_TRY(hipMemcpyAsync(ctx->h_pair_off.data(), poff, 4 * ((size_t)M + 1), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));  // pageable destination
         }
-        record(ctx, ctx->ev[5], s);
+        if (host_work) record(ctx, ctx->ev[5], s);
         HIP_TRY(hipGetLastError());
         st.pairs = R;
         st.segments = S;

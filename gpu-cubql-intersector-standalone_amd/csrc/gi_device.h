@@ -150,6 +150,16 @@ void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_
                   uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, __int128* seg_pool, uint32_t seg_cap,
                   uint2* defer, uint32_t defer_cap, uint32_t* counters, int blocks, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
+// Sorted output of up to output_small_max() result records in one workgroup,
+// then their segments; the record count is read on the device, so this is
+// queued before the counters reach the host. Does nothing unless
+// output_small_fits(R, key_bits) and nothing overflowed.
+uint32_t output_small_max();
+bool output_small_fits(uint32_t R, int key_bits);
+void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const uint32_t* counters, uint32_t cap_rec,
+                         uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint2* pairs, uint32_t* order,
+                         uint32_t* offs, __int128* seg_out, double* seg_f64, uint32_t out_slots,
+                         const __int128* seg_pool, hipStream_t s);
 void launch_gather_pairs(const uint64_t* sorted_key, const uint32_t* order, const uint2* rec_seg, uint32_t n,
                          int bitsB, uint2* pairs, uint32_t* nseg, hipStream_t s);
 void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const uint32_t* seg_offsets, uint32_t n,

@@ -1500,14 +1500,10 @@ __device__ __forceinline__ double i128_to_double(__int128 x) {
     return neg ? -d : d;
 }
 
-__global__ void k_gather_segments(const uint32_t* __restrict__ order, const uint2* __restrict__ rec_seg,
-                                  const uint32_t* __restrict__ seg_offsets, uint32_t n,
-                                  const __int128* __restrict__ pool, __int128* __restrict__ out,
-                                  double* __restrict__ f64) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint2 rs = rec_seg[order[i]];
-    uint32_t dst = seg_offsets[i];
+// the rs.y segments of one record (at rs.x in the pool) to dst onward:
+// homogeneous int128 endpoints, and the doubles they divide to
+__device__ __forceinline__ void copy_record_segments(uint2 rs, uint32_t dst, const __int128* __restrict__ pool,
+                                                     __int128* __restrict__ out, double* __restrict__ f64) {
     for (uint32_t s = 0; s < rs.y; ++s) {
         const __int128* src = pool + 8 * (size_t)(rs.x + s);
         __int128* o = out + 8 * (size_t)(dst + s);
@@ -1524,10 +1520,194 @@ __global__ void k_gather_segments(const uint32_t* __restrict__ order, const uint
     }
 }
 
+__global__ void k_gather_segments(const uint32_t* __restrict__ order, const uint2* __restrict__ rec_seg,
+                                  const uint32_t* __restrict__ seg_offsets, uint32_t n,
+                                  const __int128* __restrict__ pool, __int128* __restrict__ out,
+                                  double* __restrict__ f64) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    copy_record_segments(rec_seg[order[i]], seg_offsets[i], pool, out, f64);
+}
+
 void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const uint32_t* seg_offsets, uint32_t n,
                             const __int128* seg_pool, __int128* seg_out, double* seg_f64, hipStream_t s) {
     if (n == 0) return;
     k_gather_segments<<<blocks_for(n, 256), 256, 0, s>>>(order, rec_seg, seg_offsets, n, seg_pool, seg_out, seg_f64);
+    GI_LAUNCH_CHECK();
+}
+
+// One workgroup turns up to 16,384 result records into the sorted output in
+// one launch (the multi-tile path spends 20-odd launches on a few thousand
+// records). It runs before the host has read the counters: the record count
+// R comes from device memory, and the kernel does nothing unless R fits (the
+// host then takes the multi-tile path after the read-back). Each key
+// (faceA << bitsB | faceB) carries its record index in the low idx_bits, so
+// the LSD passes over the key bits move no separate values; the keys live in
+// LDS between passes (128 KB) and the ranking is the one of k_sort_small
+// (stable, wave ballot match). Then, per sorted position: the pair, the
+// record index (order) and the exclusive scan of the segment counts
+// (offsets, with the total at offs[R]).
+constexpr int OUT_THREADS = 1024, OUT_ITEMS = 16;
+constexpr uint32_t kOutSmallMax = OUT_THREADS * OUT_ITEMS;
+
+__host__ __device__ __forceinline__ int bits_of_count(uint32_t n) { return n <= 1 ? 1 : 32 - __builtin_clz(n - 1); }
+
+// the records and segments were all stored (no overflow) and fit one workgroup
+__device__ __forceinline__ bool output_small_ok(const uint32_t* counters, uint32_t cap_rec, uint32_t cap_seg,
+                                                int key_bits, uint32_t& R) {
+    R = counters[CNT_RESULTS];
+    return R <= cap_rec && counters[CNT_SEGMENTS] <= cap_seg && R <= kOutSmallMax &&
+           key_bits + bits_of_count(R) <= 64;
+}
+
+__global__ __launch_bounds__(OUT_THREADS) void k_output_small(const uint64_t* __restrict__ rec_key,
+                                                              const uint2* __restrict__ rec_seg,
+                                                              const uint32_t* __restrict__ counters, uint32_t cap_rec,
+                                                              uint32_t cap_seg, int key_bits, int bitsB,
+                                                              int want_segments, uint2* __restrict__ pairs,
+                                                              uint32_t* __restrict__ order,
+                                                              uint32_t* __restrict__ offs) {
+    uint32_t R;
+    if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R)) return;
+    const int idx_bits = bits_of_count(R);
+    __shared__ uint64_t skey[kOutSmallMax];
+    __shared__ uint32_t wc[OUT_THREADS / 64][RADIX];
+    __shared__ uint32_t base[RADIX];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt();
+    const uint32_t seg = wave * 64 * OUT_ITEMS + lane;
+    uint64_t key[OUT_ITEMS];
+    uint32_t rank2[OUT_ITEMS / 2];  // ranks < 2^16, two per register
+#pragma unroll
+    for (int it = 0; it < OUT_ITEMS; ++it) {
+        const uint32_t i = seg + 64 * it;
+        key[it] = i < R ? (rec_key[i] << idx_bits) | i : 0ull;
+    }
+    const int passes = (key_bits + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = idx_bits + 8 * p;
+        for (int d = threadIdx.x; d < (OUT_THREADS / 64) * RADIX; d += OUT_THREADS) (&wc[0][0])[d] = 0;
+#pragma unroll
+        for (int q = 0; q < OUT_ITEMS / 2; ++q) rank2[q] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < OUT_ITEMS; ++it) {
+            const bool valid = seg + 64 * it < R;
+            const uint32_t d = (uint32_t)(key[it] >> shift) & (RADIX - 1);
+            uint64_t match = __ballot(valid);
+            for (int b = 0; b < 8; ++b) {
+                uint64_t bb = __ballot((d >> b) & 1u);
+                match &= ((d >> b) & 1u) ? bb : ~bb;
+            }
+            const uint32_t peers = (uint32_t)__popcll(match & lt);
+            const uint32_t cur = wc[wave][d];
+            rank2[it >> 1] |= (cur + peers) << (16 * (it & 1));
+            wave_sync_lds();
+            if (valid && peers == 0) wc[wave][d] = cur + (uint32_t)__popcll(match);
+            wave_sync_lds();
+        }
+        __syncthreads();
+        if (threadIdx.x < RADIX) {  // per digit: exclusive prefix over the waves, total to base
+            uint32_t run = 0;
+            for (int w = 0; w < OUT_THREADS / 64; ++w) {
+                const uint32_t t = wc[w][threadIdx.x];
+                wc[w][threadIdx.x] = run;
+                run += t;
+            }
+            base[threadIdx.x] = run;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {  // exclusive scan of the 256 digit totals by one wave
+            uint32_t c[4], sum = 0;
+            for (int q = 0; q < 4; ++q) { c[q] = base[4 * threadIdx.x + q]; sum += c[q]; }
+            uint32_t x = sum;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o);
+                if (lane >= (uint32_t)o) x += y;
+            }
+            uint32_t off = x - sum;
+            for (int q = 0; q < 4; ++q) { base[4 * threadIdx.x + q] = off; off += c[q]; }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < OUT_ITEMS; ++it) {
+            if (seg + 64 * it >= R) continue;
+            const uint32_t d = (uint32_t)(key[it] >> shift) & (RADIX - 1);
+            skey[base[d] + wc[wave][d] + ((rank2[it >> 1] >> (16 * (it & 1))) & 0xFFFFu)] = key[it];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < OUT_ITEMS; ++it)
+            if (seg + 64 * it < R) key[it] = skey[seg + 64 * it];
+    }
+    // sorted key of position seg + 64 it in key[it]. Segment counts scanned in
+    // position order (item-major, lane-minor within a wave, then by wave):
+    // wave totals first, then each wave scans again from its offset.
+    const uint64_t idx_mask = (1ull << idx_bits) - 1ull;
+    auto count = [&](int it) -> uint32_t {
+        return (seg + 64 * it < R && want_segments) ? rec_seg[(uint32_t)(key[it] & idx_mask)].y : 0u;
+    };
+    uint32_t wsum = 0;
+#pragma unroll
+    for (int it = 0; it < OUT_ITEMS; ++it) wsum += count(it);
+    for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o);
+    __syncthreads();  // wc is free again: wave totals go to wc[0]
+    if (lane == 0) wc[0][wave] = wsum;
+    __syncthreads();
+    uint32_t running = 0, all = 0;
+    for (int w = 0; w < OUT_THREADS / 64; ++w) {
+        const uint32_t t = wc[0][w];
+        running += (uint32_t)w < wave ? t : 0u;
+        all += t;
+    }
+    const uint64_t maskB = (1ull << bitsB) - 1ull;
+#pragma unroll
+    for (int it = 0; it < OUT_ITEMS; ++it) {
+        const uint32_t c = count(it);
+        uint32_t x = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= (uint32_t)o) x += y;
+        }
+        const uint32_t i = seg + 64 * it;
+        if (i < R) {
+            const uint64_t pk = key[it] >> idx_bits;
+            pairs[i] = make_uint2((uint32_t)(pk >> bitsB), (uint32_t)(pk & maskB));
+            order[i] = (uint32_t)(key[it] & idx_mask);
+            offs[i] = running + x - c;
+        }
+        running += __shfl(x, 63);
+    }
+    if (threadIdx.x == 0) offs[R] = all;
+}
+
+// segments of the sorted records, for the records k_output_small ordered
+__global__ __launch_bounds__(256) void k_gather_segments_small(const uint32_t* __restrict__ order,
+                                                               const uint2* __restrict__ rec_seg,
+                                                               const uint32_t* __restrict__ seg_offsets,
+                                                               const uint32_t* __restrict__ counters, uint32_t cap_rec,
+                                                               uint32_t cap_seg, int key_bits, uint32_t out_slots,
+                                                               const __int128* __restrict__ pool,
+                                                               __int128* __restrict__ out, double* __restrict__ f64) {
+    uint32_t R;
+    if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R) || seg_offsets[R] > out_slots) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x)
+        copy_record_segments(rec_seg[order[i]], seg_offsets[i], pool, out, f64);
+}
+
+uint32_t output_small_max() { return kOutSmallMax; }
+
+bool output_small_fits(uint32_t R, int key_bits) { return R <= kOutSmallMax && key_bits + bits_of_count(R) <= 64; }
+
+void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const uint32_t* counters, uint32_t cap_rec,
+                         uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint2* pairs, uint32_t* order,
+                         uint32_t* offs, __int128* seg_out, double* seg_f64, uint32_t out_slots,
+                         const __int128* seg_pool, hipStream_t s) {
+    k_output_small<<<1, OUT_THREADS, 0, s>>>(rec_key, rec_seg, counters, cap_rec, cap_seg, key_bits, bitsB,
+                                             want_segments ? 1 : 0, pairs, order, offs);
+    if (want_segments)
+        k_gather_segments_small<<<kOutSmallMax / 256, 256, 0, s>>>(order, rec_seg, offs, counters, cap_rec, cap_seg,
+                                                                   key_bits, out_slots, seg_pool, seg_out, seg_f64);
     GI_LAUNCH_CHECK();
 }
 

@@ -31,13 +31,22 @@ def test_sphere_torus_small_grid():
     assert len(res["pairs"]) > 100
 
 
-def test_result_sort_on_a_few_tiles():
-    # 409,600 triangles per mesh give between 4,096 and 12,288 intersecting
-    # pairs: just past the one-workgroup sort, the result sort runs the
-    # multi-tile passes on a handful of 1,024-key tiles (partial last tile)
+def test_result_output_one_workgroup():
+    # 409,600 triangles per mesh give between 4,096 and 16,384 intersecting
+    # pairs: the one-workgroup output kernel (16 keys per thread, several
+    # items per lane, LDS passes) sorts them and scans their segment counts
     (xa, ta), (xb, tb) = mg.sphere_torus(640, 320)
     res, ref = _check(xa, ta, xb, tb, True)
-    assert 4096 < len(res["pairs"]) <= 12288
+    assert 4096 < len(res["pairs"]) <= 16384
+
+
+def test_result_sort_multi_tile():
+    # a 96 x 96 height field against its 90-degree turn crosses along many
+    # curves: more than 16,384 pairs, so the result sort runs the multi-tile
+    # radix passes, the gather and the multi-block scan
+    (p, t), _, (r, _) = mg.height_field_pairs(cells=96)
+    res, ref = _check(p, t, r, t, True)
+    assert len(res["pairs"]) > 16384
 
 
 def test_sphere_torus_multi_block_sort_and_chunks():
