@@ -465,7 +465,8 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
     // sorted output of a small result set, before the read-back (buffers
     // from ensure_output_bufs)
     launch_output_small(ctx->rec_key.get<uint64_t>(), ctx->rec_seg.get<uint2>(), counters, ctx->cap_rec, ctx->cap_seg,
-                        bits_for(MA.nt) + bitsB, bitsB, o.want_segments != 0, ctx->pairs.get<uint2>(),
+                        bits_for(MA.nt) + bitsB, bitsB, o.want_segments != 0, ctx->tkeys.get<uint64_t>(),
+                        ctx->pairs.get<uint2>(),
                         ctx->order.get<uint32_t>(), ctx->offs.get<uint32_t>(), ctx->segs.get<__int128>(),
                         ctx->f64.get<double>(), (uint32_t)output_seg_slots(ctx), ctx->seg_pool.get<__int128>(), s);
     if (timed) record(ctx, ctx->ev[5], s);
@@ -475,6 +476,7 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
 
 void ensure_output_bufs(gi_context* ctx, const gi_options& o) {
     const uint32_t cap = std::min(output_small_max(), ctx->cap_rec);
+    ctx->tkeys.ensure<uint64_t>(std::max(cap, 1u));  // sorted runs before the merge
     ctx->pairs.ensure<uint2>(std::max(cap, 1u));
     ctx->order.ensure<uint32_t>(std::max(cap, 1u));
     ctx->offs.ensure<uint32_t>((size_t)cap + 1);
@@ -516,7 +518,7 @@ std::vector<uint64_t> presync_signature(gi_context* ctx, const gi_options& o) {
     for (const DevBuf* b : {&ctx->counters, &ctx->bounds, &ctx->xf_dev, &ctx->cand, &ctx->exact, &ctx->rec_key,
                             &ctx->rec_seg, &ctx->seg_pool, &ctx->bhist[0], &ctx->bhist[1], &ctx->bscan[0],
                             &ctx->bscan[1], &ctx->bpart[0], &ctx->bpart[1], &ctx->pairs, &ctx->order, &ctx->offs,
-                            &ctx->segs, &ctx->f64})
+                            &ctx->segs, &ctx->f64, &ctx->tkeys})
         buf(*b);
     v.push_back((uint64_t)ctx->tune.traverse_variant);
     v.push_back((uint64_t)ctx->tune.filter_blocks);

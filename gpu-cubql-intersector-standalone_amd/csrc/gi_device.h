@@ -157,8 +157,8 @@ void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
 uint32_t output_small_max();
 bool output_small_fits(uint32_t R, int key_bits);
 void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const uint32_t* counters, uint32_t cap_rec,
-                         uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint2* pairs, uint32_t* order,
-                         uint32_t* offs, __int128* seg_out, double* seg_f64, uint32_t out_slots,
+                         uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint64_t* runs, uint2* pairs,
+                         uint32_t* order, uint32_t* offs, __int128* seg_out, double* seg_f64, uint32_t out_slots,
                          const __int128* seg_pool, hipStream_t s);
 void launch_gather_pairs(const uint64_t* sorted_key, const uint32_t* order, const uint2* rec_seg, uint32_t n,
                          int bitsB, uint2* pairs, uint32_t* nseg, hipStream_t s);

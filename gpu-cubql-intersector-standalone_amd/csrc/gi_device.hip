@@ -1536,23 +1536,30 @@ void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const u
     GI_LAUNCH_CHECK();
 }
 
-// One workgroup turns up to 16,384 result records into the sorted output in
-// one launch (the multi-tile path spends 20-odd launches on a few thousand
-// records). It runs before the host has read the counters: the record count
-// R comes from device memory, and the kernel does nothing unless R fits (the
-// host then takes the multi-tile path after the read-back). Each key
-// (faceA << bitsB | faceB) carries its record index in the low idx_bits, so
-// the LSD passes over the key bits move no separate values; the keys live in
-// LDS between passes (128 KB) and the ranking is the one of k_sort_small
-// (stable, wave ballot match). Then, per sorted position: the pair, the
-// record index (order) and the exclusive scan of the segment counts
-// (offsets, with the total at offs[R]).
-constexpr int OUT_THREADS = 1024, OUT_ITEMS = 16;
-constexpr uint32_t kOutSmallMax = OUT_THREADS * OUT_ITEMS;
+// Sorted output of up to 16,384 result records in two launches, queued
+// before the host reads the counters: the record count R comes from device
+// memory, and nothing runs unless R fits (the host then takes the multi-tile
+// path after the read-back). The multi-tile path spends 20-odd launches on a
+// few thousand records; a single workgroup is bound by the VALU of one CU
+// (the ballot-match ranking costs ~8 wave64 instructions per key bit).
+//   k_output_runs   kRuns workgroups, each an LSD radix sort of kRunLen
+//                   records in LDS (ranking as in k_sort_small: stable,
+//                   wave ballot match, per-wave digit counts);
+//   k_output_merge  one workgroup merges the sorted runs pairwise (merge
+//                   path, in LDS), then writes per sorted position the pair,
+//                   the record index (order) and the exclusive scan of the
+//                   segment counts (offsets, total at offs[R]).
+// Each key (faceA << bitsB | faceB) carries its record index in the low
+// idx_bits, so the passes move no separate values (keys are distinct).
+constexpr int RUN_THREADS = 1024, RUN_ITEMS = 2;
+constexpr uint32_t kRunLen = RUN_THREADS * RUN_ITEMS;
+constexpr uint32_t kRuns = 8;
+constexpr uint32_t kOutSmallMax = kRunLen * kRuns;
+constexpr int MERGE_THREADS = 1024, MERGE_ITEMS = (int)(kOutSmallMax / MERGE_THREADS);
 
 __host__ __device__ __forceinline__ int bits_of_count(uint32_t n) { return n <= 1 ? 1 : 32 - __builtin_clz(n - 1); }
 
-// the records and segments were all stored (no overflow) and fit one workgroup
+// the records and segments were all stored (no overflow) and fit
 __device__ __forceinline__ bool output_small_ok(const uint32_t* counters, uint32_t cap_rec, uint32_t cap_seg,
                                                 int key_bits, uint32_t& R) {
     R = counters[CNT_RESULTS];
@@ -1560,39 +1567,37 @@ __device__ __forceinline__ bool output_small_ok(const uint32_t* counters, uint32
            key_bits + bits_of_count(R) <= 64;
 }
 
-__global__ __launch_bounds__(OUT_THREADS) void k_output_small(const uint64_t* __restrict__ rec_key,
-                                                              const uint2* __restrict__ rec_seg,
-                                                              const uint32_t* __restrict__ counters, uint32_t cap_rec,
-                                                              uint32_t cap_seg, int key_bits, int bitsB,
-                                                              int want_segments, uint2* __restrict__ pairs,
-                                                              uint32_t* __restrict__ order,
-                                                              uint32_t* __restrict__ offs) {
+__global__ __launch_bounds__(RUN_THREADS) void k_output_runs(const uint64_t* __restrict__ rec_key,
+                                                             const uint32_t* __restrict__ counters, uint32_t cap_rec,
+                                                             uint32_t cap_seg, int key_bits,
+                                                             uint64_t* __restrict__ runs) {
     uint32_t R;
     if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R)) return;
+    const uint32_t r0 = blockIdx.x * kRunLen;
+    if (r0 >= R) return;
+    const uint32_t n = min(kRunLen, R - r0);
     const int idx_bits = bits_of_count(R);
-    __shared__ uint64_t skey[kOutSmallMax];
-    __shared__ uint32_t wc[OUT_THREADS / 64][RADIX];
+    __shared__ uint64_t skey[kRunLen];
+    __shared__ uint32_t wc[RUN_THREADS / 64][RADIX];
     __shared__ uint32_t base[RADIX];
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
-    const uint32_t seg = wave * 64 * OUT_ITEMS + lane;
-    uint64_t key[OUT_ITEMS];
-    uint32_t rank2[OUT_ITEMS / 2];  // ranks < 2^16, two per register
+    const uint32_t seg = wave * 64 * RUN_ITEMS + lane;
+    uint64_t key[RUN_ITEMS];
+    uint32_t rank[RUN_ITEMS];
 #pragma unroll
-    for (int it = 0; it < OUT_ITEMS; ++it) {
+    for (int it = 0; it < RUN_ITEMS; ++it) {
         const uint32_t i = seg + 64 * it;
-        key[it] = i < R ? (rec_key[i] << idx_bits) | i : 0ull;
+        key[it] = i < n ? (rec_key[r0 + i] << idx_bits) | (r0 + i) : 0ull;
     }
     const int passes = (key_bits + 7) / 8;
     for (int p = 0; p < passes; ++p) {
         const int shift = idx_bits + 8 * p;
-        for (int d = threadIdx.x; d < (OUT_THREADS / 64) * RADIX; d += OUT_THREADS) (&wc[0][0])[d] = 0;
-#pragma unroll
-        for (int q = 0; q < OUT_ITEMS / 2; ++q) rank2[q] = 0;
+        for (int d = threadIdx.x; d < (RUN_THREADS / 64) * RADIX; d += RUN_THREADS) (&wc[0][0])[d] = 0;
         __syncthreads();
 #pragma unroll
-        for (int it = 0; it < OUT_ITEMS; ++it) {
-            const bool valid = seg + 64 * it < R;
+        for (int it = 0; it < RUN_ITEMS; ++it) {
+            const bool valid = seg + 64 * it < n;
             const uint32_t d = (uint32_t)(key[it] >> shift) & (RADIX - 1);
             uint64_t match = __ballot(valid);
             for (int b = 0; b < 8; ++b) {
@@ -1601,7 +1606,7 @@ __global__ __launch_bounds__(OUT_THREADS) void k_output_small(const uint64_t* __
             }
             const uint32_t peers = (uint32_t)__popcll(match & lt);
             const uint32_t cur = wc[wave][d];
-            rank2[it >> 1] |= (cur + peers) << (16 * (it & 1));
+            rank[it] = cur + peers;
             wave_sync_lds();
             if (valid && peers == 0) wc[wave][d] = cur + (uint32_t)__popcll(match);
             wave_sync_lds();
@@ -1609,7 +1614,7 @@ __global__ __launch_bounds__(OUT_THREADS) void k_output_small(const uint64_t* __
         __syncthreads();
         if (threadIdx.x < RADIX) {  // per digit: exclusive prefix over the waves, total to base
             uint32_t run = 0;
-            for (int w = 0; w < OUT_THREADS / 64; ++w) {
+            for (int w = 0; w < RUN_THREADS / 64; ++w) {
                 const uint32_t t = wc[w][threadIdx.x];
                 wc[w][threadIdx.x] = run;
                 run += t;
@@ -1630,58 +1635,115 @@ __global__ __launch_bounds__(OUT_THREADS) void k_output_small(const uint64_t* __
         }
         __syncthreads();
 #pragma unroll
-        for (int it = 0; it < OUT_ITEMS; ++it) {
-            if (seg + 64 * it >= R) continue;
+        for (int it = 0; it < RUN_ITEMS; ++it) {
+            if (seg + 64 * it >= n) continue;
             const uint32_t d = (uint32_t)(key[it] >> shift) & (RADIX - 1);
-            skey[base[d] + wc[wave][d] + ((rank2[it >> 1] >> (16 * (it & 1))) & 0xFFFFu)] = key[it];
+            skey[base[d] + wc[wave][d] + rank[it]] = key[it];
         }
         __syncthreads();
 #pragma unroll
-        for (int it = 0; it < OUT_ITEMS; ++it)
-            if (seg + 64 * it < R) key[it] = skey[seg + 64 * it];
+        for (int it = 0; it < RUN_ITEMS; ++it)
+            if (seg + 64 * it < n) key[it] = skey[seg + 64 * it];
     }
-    // sorted key of position seg + 64 it in key[it]. Segment counts scanned in
-    // position order (item-major, lane-minor within a wave, then by wave):
-    // wave totals first, then each wave scans again from its offset.
-    const uint64_t idx_mask = (1ull << idx_bits) - 1ull;
-    auto count = [&](int it) -> uint32_t {
-        return (seg + 64 * it < R && want_segments) ? rec_seg[(uint32_t)(key[it] & idx_mask)].y : 0u;
-    };
-    uint32_t wsum = 0;
 #pragma unroll
-    for (int it = 0; it < OUT_ITEMS; ++it) wsum += count(it);
-    for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o);
-    __syncthreads();  // wc is free again: wave totals go to wc[0]
-    if (lane == 0) wc[0][wave] = wsum;
-    __syncthreads();
-    uint32_t running = 0, all = 0;
-    for (int w = 0; w < OUT_THREADS / 64; ++w) {
-        const uint32_t t = wc[0][w];
-        running += (uint32_t)w < wave ? t : 0u;
-        all += t;
-    }
-    const uint64_t maskB = (1ull << bitsB) - 1ull;
-#pragma unroll
-    for (int it = 0; it < OUT_ITEMS; ++it) {
-        const uint32_t c = count(it);
-        uint32_t x = c;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o);
-            if (lane >= (uint32_t)o) x += y;
-        }
-        const uint32_t i = seg + 64 * it;
-        if (i < R) {
-            const uint64_t pk = key[it] >> idx_bits;
-            pairs[i] = make_uint2((uint32_t)(pk >> bitsB), (uint32_t)(pk & maskB));
-            order[i] = (uint32_t)(key[it] & idx_mask);
-            offs[i] = running + x - c;
-        }
-        running += __shfl(x, 63);
-    }
-    if (threadIdx.x == 0) offs[R] = all;
+    for (int it = 0; it < RUN_ITEMS; ++it)
+        if (seg + 64 * it < n) runs[r0 + seg + 64 * it] = key[it];
 }
 
-// segments of the sorted records, for the records k_output_small ordered
+// LDS index with one pad slot per 32 keys: a thread's MERGE_ITEMS contiguous
+// keys then start in different banks across the wave
+__device__ __forceinline__ uint32_t skewed(uint32_t i) { return i + (i >> 5); }
+
+__global__ __launch_bounds__(MERGE_THREADS) void k_output_merge(const uint64_t* __restrict__ runs,
+                                                                const uint2* __restrict__ rec_seg,
+                                                                const uint32_t* __restrict__ counters, uint32_t cap_rec,
+                                                                uint32_t cap_seg, int key_bits, int bitsB,
+                                                                int want_segments, uint2* __restrict__ pairs,
+                                                                uint32_t* __restrict__ order,
+                                                                uint32_t* __restrict__ offs) {
+    uint32_t R;
+    if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R)) return;
+    const int idx_bits = bits_of_count(R);
+    __shared__ uint64_t sk[kOutSmallMax + kOutSmallMax / 32];
+    __shared__ uint8_t local_off[kOutSmallMax];
+    __shared__ uint32_t thread_base[MERGE_THREADS];
+    __shared__ uint32_t wsum[MERGE_THREADS / 64];
+    const uint32_t t = threadIdx.x, lane = lane_id(), wave = threadIdx.x >> 6;
+    for (uint32_t i = t; i < R; i += MERGE_THREADS) sk[skewed(i)] = runs[i];
+    __syncthreads();
+    const uint32_t o0 = t * MERGE_ITEMS;  // this thread's output positions o0 .. o0 + MERGE_ITEMS - 1
+    uint64_t v[MERGE_ITEMS];
+    for (uint32_t L = kRunLen; L < R; L <<= 1) {  // 2L is a multiple of MERGE_ITEMS: one pair per thread
+        if (o0 < R) {
+            const uint32_t p0 = o0 / (2 * L) * (2 * L);
+            const uint32_t a0 = p0, na = min(p0 + L, R) - p0;
+            const uint32_t b0 = a0 + na, nb = min(p0 + 2 * L, R) - b0;
+            const uint32_t k = o0 - p0;
+            // merge path: the first k merged keys take i from run A, k - i from run B
+            uint32_t lo = k > nb ? k - nb : 0u, hi = min(k, na);
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sk[skewed(a0 + mid)] <= sk[skewed(b0 + k - 1 - mid)]) lo = mid + 1;
+                else hi = mid;
+            }
+            uint32_t i = lo, j = k - lo;
+#pragma unroll
+            for (int e = 0; e < MERGE_ITEMS; ++e) {
+                const uint64_t ka = i < na ? sk[skewed(a0 + i)] : ~0ull;
+                const uint64_t kb = j < nb ? sk[skewed(b0 + j)] : ~0ull;
+                const bool take_a = i < na && (j >= nb || ka <= kb);
+                v[e] = take_a ? ka : kb;
+                i += take_a ? 1u : 0u;
+                j += take_a ? 0u : 1u;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < MERGE_ITEMS; ++e)
+            if (o0 + e < R) sk[skewed(o0 + e)] = v[e];
+        __syncthreads();
+    }
+    // Segment offsets: the exclusive scan of the counts in sorted order. Each
+    // thread scans its MERGE_ITEMS contiguous positions (local offsets, at
+    // most MERGE_ITEMS * 6 < 256, kept as bytes), the workgroup scans the
+    // thread sums; the outputs are then written with consecutive lanes on
+    // consecutive positions (a thread's contiguous positions would make every
+    // store instruction touch 64 cache lines).
+    const uint64_t idx_mask = (1ull << idx_bits) - 1ull;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int e = 0; e < MERGE_ITEMS; ++e) {
+        const uint32_t i = o0 + e;
+        const uint32_t c = (i < R && want_segments) ? rec_seg[(uint32_t)(sk[skewed(i)] & idx_mask)].y : 0u;
+        local_off[i] = (uint8_t)sum;
+        sum += c;
+    }
+    uint32_t x = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint32_t before = x - sum, all = 0;
+    for (uint32_t w = 0; w < MERGE_THREADS / 64; ++w) {
+        before += w < wave ? wsum[w] : 0u;
+        all += wsum[w];
+    }
+    thread_base[t] = before;
+    __syncthreads();
+    const uint64_t maskB = (1ull << bitsB) - 1ull;
+    for (uint32_t i = t; i < R; i += MERGE_THREADS) {
+        const uint64_t key = sk[skewed(i)];
+        const uint64_t pk = key >> idx_bits;
+        pairs[i] = make_uint2((uint32_t)(pk >> bitsB), (uint32_t)(pk & maskB));
+        order[i] = (uint32_t)(key & idx_mask);
+        offs[i] = thread_base[i / MERGE_ITEMS] + local_off[i];
+    }
+    if (t == 0) offs[R] = all;
+}
+
+// segments of the sorted records, for the records k_output_merge ordered
 __global__ __launch_bounds__(256) void k_gather_segments_small(const uint32_t* __restrict__ order,
                                                                const uint2* __restrict__ rec_seg,
                                                                const uint32_t* __restrict__ seg_offsets,
@@ -1700,11 +1762,12 @@ uint32_t output_small_max() { return kOutSmallMax; }
 bool output_small_fits(uint32_t R, int key_bits) { return R <= kOutSmallMax && key_bits + bits_of_count(R) <= 64; }
 
 void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const uint32_t* counters, uint32_t cap_rec,
-                         uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint2* pairs, uint32_t* order,
-                         uint32_t* offs, __int128* seg_out, double* seg_f64, uint32_t out_slots,
+                         uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint64_t* runs, uint2* pairs,
+                         uint32_t* order, uint32_t* offs, __int128* seg_out, double* seg_f64, uint32_t out_slots,
                          const __int128* seg_pool, hipStream_t s) {
-    k_output_small<<<1, OUT_THREADS, 0, s>>>(rec_key, rec_seg, counters, cap_rec, cap_seg, key_bits, bitsB,
-                                             want_segments ? 1 : 0, pairs, order, offs);
+    k_output_runs<<<kRuns, RUN_THREADS, 0, s>>>(rec_key, counters, cap_rec, cap_seg, key_bits, runs);
+    k_output_merge<<<1, MERGE_THREADS, 0, s>>>(runs, rec_seg, counters, cap_rec, cap_seg, key_bits, bitsB,
+                                               want_segments ? 1 : 0, pairs, order, offs);
     if (want_segments)
         k_gather_segments_small<<<kOutSmallMax / 256, 256, 0, s>>>(order, rec_seg, offs, counters, cap_rec, cap_seg,
                                                                    key_bits, out_slots, seg_pool, seg_out, seg_f64);
