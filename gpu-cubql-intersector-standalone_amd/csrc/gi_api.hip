@@ -75,6 +75,7 @@ struct MeshSlot {
     uint32_t forest = 0;
     std::vector<uint32_t> h_voff, h_toff, h_vtiles, h_ttiles, h_vtile0;
     int morton_axis = 10;      // Morton bits per axis of the kept sorted keys
+    double src_lo[3] = {0.0, 0.0, 0.0}, src_hi[3] = {0.0, 0.0, 0.0};  // bounds of the source vertices
     uint64_t generation = 0;   // bumped whenever the slot's mesh or buffers are replaced
     DevBuf src, tri, v64, vi, tbox, keys, vals, tkeys, tvals, nodes, range, blocks;
     DevBuf voff, toff, vtiles, ttiles, vtile0, fbounds, roots;
@@ -102,6 +103,18 @@ Xform identity_xform() {
     return x;
 }
 
+// bounds of n vertices (NaN coordinates skipped; an empty set gives zeros)
+void vertex_bounds(const double* xyz, size_t n, double* lo, double* hi) {
+    for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin(lo[k], xyz[3 * i + k]);
+            hi[k] = fmax(hi[k], xyz[3 * i + k]);
+        }
+    for (int k = 0; k < 3; ++k)
+        if (!(lo[k] <= hi[k])) lo[k] = hi[k] = 0.0;
+}
+
 int bits_for(uint32_t n) { return n <= 1 ? 1 : 32 - __builtin_clz(n - 1); }
 
 }  // namespace
@@ -115,11 +128,12 @@ struct gi_context {
     hipStream_t build_stream[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DevBuf bhist[2], bscan[2], bpart[2];  // per-slot build scratch
+    SortJobU32 pending_sort[2] = {};       // Morton sorts queued by the slot stages (n = 0: none)
     std::string err;
     int host_threads = 0;
     uint32_t cand_reserve = 0;
     MeshSlot mesh[2];
-    DevBuf bounds, counters, hist, scan_tmp, partials;
+    DevBuf counters, hist, scan_tmp, partials;
     DevBuf cand, exact, rec_key, rec_seg, seg_pool;
     uint32_t cap_cand = 0, cap_exact = 0, cap_rec = 0, cap_seg = 0;
     DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2, pair_off;
@@ -155,7 +169,7 @@ struct gi_context {
         graph_sig_seen.clear();
         xf_dev.release();
         for (auto& m : mesh) m.release();
-        DevBuf* all[] = {&bounds, &counters, &hist, &scan_tmp, &partials, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
+        DevBuf* all[] = {&counters, &hist, &scan_tmp, &partials, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
                          &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2, &pair_off,
                          &bhist[0], &bhist[1], &bscan[0], &bscan[1], &bpart[0], &bpart[1]};
         for (DevBuf* b : all) b->release();
@@ -193,6 +207,21 @@ void join_side(gi_context* ctx) {
 // graph (external record), so that every replay sets its timestamp
 void record(gi_context* ctx, hipEvent_t e, hipStream_t s) {
     HIP_TRY(hipEventRecordWithFlags(e, s, ctx->capturing ? hipEventRecordExternal : hipEventRecordDefault));
+}
+
+// the slot's vertices, boxes (and for a new topology keys) must be staged
+bool slot_needs_stage(const MeshSlot& m, bool rebuild) {
+    return m.set && !m.borrowed && (m.dirty || !m.built || rebuild);
+}
+
+// the transform record a slot's stage launch reads from device memory
+Xform staged_xform(const MeshSlot& m) {
+    Xform x = m.xf;
+    for (int k = 0; k < 3; ++k) {
+        x.lo[k] = m.src_lo[k];
+        x.hi[k] = m.src_hi[k];
+    }
+    return x;
 }
 
 void sync_all(gi_context* ctx) {
@@ -274,8 +303,7 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
         return true;
     }
     hipStream_t s = ctx->build_stream[slot];
-    uint32_t* grid = ctx->counters.get<uint32_t>() + CNT_COUNT;
-    uint32_t* bounds = ctx->bounds.get<uint32_t>() + 6 * slot;
+    uint32_t* grid = ctx->counters.get<uint32_t>() + CNT_COUNT;  // set to 1 by the query's init
     size_t nv = m.nv, nt = m.nt;
     double4* v64 = m.v64.ensure<double4>(nv);
     int4* vi = m.vi.ensure<int4>(nv);
@@ -288,20 +316,22 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
     uint32_t* tk = m.tkeys.ensure<uint32_t>(nt);
     uint32_t* tv = m.tvals.ensure<uint32_t>(nt);
     const bool topology = !m.built || rebuild;
-    launch_init_mesh_scalars(grid + slot, bounds, s);
-    float* partials = ctx->bpart[slot].ensure<float>(6 * (size_t)kPrepareMaxBlocks);
     Xform* d_xf = ctx->xf_dev.get<Xform>() + slot;
-    ctx->h_xf[slot] = m.xf;
+    ctx->h_xf[slot] = staged_xform(m);
     HIP_TRY(hipMemcpyAsync(d_xf, ctx->h_xf + slot, sizeof(Xform), hipMemcpyHostToDevice, s));
-    launch_prepare_vertices(m.src.get<double>(), m.nv, d_xf, v64, vi, grid + slot, partials, bounds, s);
     if (topology) m.morton_axis = morton_axis_bits(m.nt);
-    launch_tri_boxes(m.tri.get<uint32_t>(), m.nt, v64, vi, grid + slot, bounds, m.morton_axis, tbox,
-                     topology ? keys : nullptr, topology ? vals : nullptr, s);
+    uint32_t* hist = nullptr;
+    uint32_t* stmp = nullptr;
     if (topology) {
-        uint32_t* hist = ctx->bhist[slot].ensure<uint32_t>(sort_hist_elems(nt));
-        uint32_t* stmp = ctx->bscan[slot].ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
-        m.sorted_in_tmp = radix_sort_u32(keys, vals, tk, tv, m.nt, 3 * m.morton_axis, hist, stmp, s);
+        hist = ctx->bhist[slot].ensure<uint32_t>(sort_hist_elems(nt));
+        stmp = ctx->bscan[slot].ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
     }
+    const bool hist0 = topology && radix_sort_takes_hist0(m.nt);
+    launch_stage_mesh(m.src.get<double>(), m.nv, m.tri.get<uint32_t>(), m.nt, d_xf, v64, vi, grid + slot,
+                      m.morton_axis, tbox, topology ? keys : nullptr, topology ? vals : nullptr,
+                      hist0 ? hist : nullptr, s);
+    // the Morton sort runs after the join, together with the other slot's
+    if (topology) ctx->pending_sort[slot] = SortJobU32{keys, vals, tk, tv, m.nt, 3 * m.morton_axis, hist0, hist, stmp};
     return true;
 }
 
@@ -323,17 +353,38 @@ BuildJob<K> slot_job(MeshSlot& m) {
 // kernel serves every tree that needs a build or refit (a refit reruns the
 // build on the kept sorted keys: same topology, new boxes). The
 // k_build_chunks launch is bracketed by ev_chunk[0..1].
-void build_meshes(gi_context* ctx, bool rebuild, bool timed = true) {
-    ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);  // shared by both stages: allocate before the fork
-    ctx->bounds.ensure<uint32_t>(12);
+void build_meshes(gi_context* ctx, bool rebuild, bool timed = true, bool zero_counters = false) {
+    uint32_t* counters = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);  // shared by both stages: allocate before the fork
     ctx->xf_dev.ensure<Xform>(2);
     ctx->chunk_timed = false;
     ctx->chunk_leaves = 0;
+    // grid flags of the slots staged below to 1 (their vertices clear them),
+    // before the per-slot stages fork
+    uint32_t stage_mask = 0;
+    for (int sl = 0; sl < 2; ++sl) {
+        const MeshSlot& m = ctx->mesh[sl];
+        if (!m.forest && slot_needs_stage(m, rebuild)) stage_mask |= 1u << sl;
+    }
+    if (stage_mask || zero_counters)
+        launch_query_init(counters, zero_counters ? CNT_COUNT : 0u, counters + CNT_COUNT, stage_mask, ctx->stream);
     fork_side(ctx);
     bool need[2] = {false, false};
-    for (int sl = 0; sl < 2; ++sl)
+    for (int sl = 0; sl < 2; ++sl) {
+        ctx->pending_sort[sl] = SortJobU32{};
         if (ctx->mesh[sl].set) need[sl] = stage_mesh(ctx, sl, rebuild);
+    }
     join_side(ctx);
+    {  // both meshes' Morton sorts, pass by pass in shared launches
+        SortJobU32 jobs[2];
+        int slot_of[2], nj = 0, in_tmp[2] = {0, 0};
+        for (int sl = 0; sl < 2; ++sl)
+            if (ctx->pending_sort[sl].keys) {
+                jobs[nj] = ctx->pending_sort[sl];
+                slot_of[nj++] = sl;
+            }
+        radix_sort_u32_pair(jobs, nj, ctx->stream, in_tmp);
+        for (int j = 0; j < nj; ++j) ctx->mesh[slot_of[j]].sorted_in_tmp = in_tmp[j];
+    }
     BuildJob<uint32_t> j32[2];
     BuildJob<uint64_t> j64[2];
     int n32 = 0, n64 = 0;
@@ -440,16 +491,17 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
     MeshSlot& MA = ctx->mesh[0];
     MeshSlot& MB = ctx->mesh[1];
     const uint32_t M = MA.forest;
-    if (build) {
+    uint32_t* counters = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);
+    if (build) {  // the build's first launch also zeroes the counters
         if (timed) record(ctx, ctx->ev[0], s);
-        build_meshes(ctx, o.rebuild != 0, timed);
+        build_meshes(ctx, o.rebuild != 0, timed, true);
         if (timed) record(ctx, ctx->ev[1], s);
+    } else {
+        launch_query_init(counters, CNT_COUNT, counters + CNT_COUNT, 0u, s);
     }
-    uint32_t* counters = ctx->counters.get<uint32_t>();
     uint32_t* grid = counters + CNT_COUNT;
     MeshDev A = mesh_dev(MA), B = mesh_dev(MB);
     int bitsB = bits_for(MB.nt);
-    launch_fill_u32(counters, 0u, CNT_COUNT, s);
     if (timed) record(ctx, ctx->ev[6], s);
     const uint64_t* keysB = M ? (MB.sorted_in_tmp ? MB.tkeys.get<uint64_t>() : MB.keys.get<uint64_t>()) : nullptr;
     launch_traverse(A, B, keysB, 3 * MB.morton_axis, M ? MA.roots.get<uint2>() : nullptr, ctx->cand.get<uint2>(),
@@ -486,9 +538,6 @@ void ensure_output_bufs(gi_context* ctx, const gi_options& o) {
     }
 }
 
-bool slot_needs_stage(const MeshSlot& m, bool rebuild) {
-    return m.set && !m.borrowed && (m.dirty || !m.built || rebuild);
-}
 
 // Everything enqueue_presync bakes into launches: sizes, flags, device
 // pointers (a reallocation changes them), capacities.
@@ -515,7 +564,7 @@ std::vector<uint64_t> presync_signature(gi_context* ctx, const gi_options& o) {
                                 &m.fbounds, &m.roots})
             buf(*b);
     }
-    for (const DevBuf* b : {&ctx->counters, &ctx->bounds, &ctx->xf_dev, &ctx->cand, &ctx->exact, &ctx->rec_key,
+    for (const DevBuf* b : {&ctx->counters, &ctx->xf_dev, &ctx->cand, &ctx->exact, &ctx->rec_key,
                             &ctx->rec_seg, &ctx->seg_pool, &ctx->bhist[0], &ctx->bhist[1], &ctx->bscan[0],
                             &ctx->bscan[1], &ctx->bpart[0], &ctx->bpart[1], &ctx->pairs, &ctx->order, &ctx->offs,
                             &ctx->segs, &ctx->f64, &ctx->tkeys})
@@ -546,7 +595,7 @@ bool launch_presync_graph(gi_context* ctx, const gi_options& o) {
     for (int sl = 0; sl < 2; ++sl) need[sl] = slot_needs_stage(ctx->mesh[sl], o.rebuild != 0);
     if (ctx->graph_exec && sig == ctx->graph_sig) {
         for (int sl = 0; sl < 2; ++sl)
-            if (need[sl]) ctx->h_xf[sl] = ctx->mesh[sl].xf;
+            if (need[sl]) ctx->h_xf[sl] = staged_xform(ctx->mesh[sl]);
         HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));  // records the stage events itself
         for (int sl = 0; sl < 2; ++sl)
             if (need[sl]) {
@@ -630,7 +679,6 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
                       (uint32_t)std::max<uint64_t>(c / 4, 1024));
     }
     ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);
-    ctx->bounds.ensure<uint32_t>(12);
     ctx->xf_dev.ensure<Xform>(2);
     int bitsA = bits_for(MA.nt), bitsB = bits_for(MB.nt);
     const uint32_t* hc = ctx->h_counters;
@@ -1008,6 +1056,7 @@ int gi_set_mesh(gi_context* ctx, int slot, const double* xyz, uint32_t nv, const
         m.forest = 0;
         m.h_xyz.assign(xyz, xyz + 3 * (size_t)nv);
         m.h_tri.assign(tri, tri + 3 * (size_t)nt);
+        vertex_bounds(xyz, nv, m.src_lo, m.src_hi);
         m.nv = nv;
         m.nt = nt;
         m.set = true;
@@ -1044,6 +1093,7 @@ int gi_set_mesh_device(gi_context* ctx, int slot, const double* d_xyz, uint32_t 
         if (nv) HIP_TRY(hipMemcpyAsync(m.h_xyz.data(), d_xyz, 24 * (size_t)nv, hipMemcpyDeviceToHost, ctx->stream));
         if (nt) HIP_TRY(hipMemcpyAsync(m.h_tri.data(), d_tri, 12 * (size_t)nt, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        vertex_bounds(m.h_xyz.data(), nv, m.src_lo, m.src_hi);
         for (size_t i = 0; i < 3 * (size_t)nt; ++i)
             if (m.h_tri[i] >= nv) {
                 m.set = false;
@@ -1163,6 +1213,7 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         d.built = true;
         d.dirty = false;
         d.xf = s.xf;
+        for (int k = 0; k < 3; ++k) { d.src_lo[k] = s.src_lo[k]; d.src_hi[k] = s.src_hi[k]; }
         d.sorted_in_tmp = s.sorted_in_tmp;
         d.morton_axis = s.morton_axis;
         d.borrowed = true;

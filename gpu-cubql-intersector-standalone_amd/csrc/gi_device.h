@@ -43,6 +43,7 @@ struct Xform {
     double m[3][4];
     int identity;
     int snap;  // round the result to the nearest integer (grid meshes)
+    double lo[3], hi[3];  // bounds of the source vertices: moved, their corners frame the Morton codes
 };
 
 struct MeshDev {
@@ -55,19 +56,20 @@ struct MeshDev {
 
 void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
 // grid flag = 1, bounds6 = empty box (order-preserving keys)
-void launch_init_mesh_scalars(uint32_t* grid_flag, uint32_t* bounds6, hipStream_t s);
-// bounds6 receives the mesh bounds (3 min, 3 max order-preserving float
-// keys); partials needs 6 * kPrepareMaxBlocks floats.
+// zero counters[0, nzero) and set grid_flags[slot] = 1 for the slots in stage_mask
+void launch_query_init(uint32_t* counters, uint32_t nzero, uint32_t* grid_flags, uint32_t stage_mask, hipStream_t s);
+// Grid-stride width of the vertex part of a mesh's stage launch.
 constexpr uint32_t kPrepareMaxBlocks = 4096;
-// xf: the transform in device memory (so that a captured launch sequence
-// picks up a new transform from the copy that precedes it)
-void launch_prepare_vertices(const double* src, uint32_t nv, const Xform* xf, double4* v64, int4* vi,
-                             uint32_t* grid_flag, float* partials, uint32_t* bounds6, hipStream_t s);
-// Triangle boxes; with keys/vals non-null also Morton keys (frame bounds6)
-// and the identity permutation.
-void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,
-                      const uint32_t* bounds6, int axis_bits, float4* box, uint32_t* keys, uint32_t* vals,
-                      hipStream_t s);
+// One launch for a mesh's vertices (moved by *xf, in device memory so that a
+// captured launch sequence picks up a new transform from the copy before it;
+// grid flag cleared when a vertex is off the grid) and its triangles (boxes,
+// and with keys non-null the Morton keys, identity values and, with hist0
+// non-null, the first radix pass's per-tile digit counts, as radix_sort_u32
+// takes them with hist0_ready). The Morton frame is the box of xf's moved
+// source-bound corners (xf->lo, xf->hi).
+void launch_stage_mesh(const double* src, uint32_t nv, const uint32_t* tri, uint32_t nt, const Xform* xf,
+                       double4* v64, int4* vi, uint32_t* grid_flag, int axis_bits, float4* box, uint32_t* keys,
+                       uint32_t* vals, uint32_t* hist0, hipStream_t s);
 
 // Forest (a batch of meshes stored back to back, indices global): tiles of
 // kForestTile vertices/triangles, each inside one mesh, tile = (mesh, first).
@@ -90,7 +92,21 @@ void launch_tri_boxes_forest(const uint32_t* tri, const uint2* ttiles, uint32_t 
 size_t sort_hist_elems(size_t n);
 size_t scan_tmp_elems(size_t n);
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
-                   uint32_t* hist, uint32_t* scan_tmp, hipStream_t s);
+                   uint32_t* hist, uint32_t* scan_tmp, hipStream_t s, bool hist0_ready = false);
+// whether a sort of n uint32 keys runs the multi-tile passes (and so can take
+// the first pass's counts from launch_stage_mesh)
+bool radix_sort_takes_hist0(uint32_t n);
+// Two meshes' Morton sorts at once: each multi-tile pass of both runs in the
+// same launches. in_tmp[j] = 1 when sort j's result ends in (tkeys, tvals).
+struct SortJobU32 {
+    uint32_t* keys; uint32_t* vals; uint32_t* tkeys; uint32_t* tvals;
+    uint32_t n;
+    int bits;
+    bool hist0_ready;
+    uint32_t* hist;      // sort_hist_elems(n)
+    uint32_t* scan_tmp;  // scan_tmp_elems(sort_hist_elems(n))
+};
+void radix_sort_u32_pair(const SortJobU32* jobs, int njobs, hipStream_t s, int* in_tmp);
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
                    uint32_t* hist, uint32_t* scan_tmp, hipStream_t s);
 // exclusive scan of n uint32 (in place allowed); writes the total to *total

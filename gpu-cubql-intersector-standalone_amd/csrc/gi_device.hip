@@ -65,13 +65,17 @@ void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s) {
     GI_LAUNCH_CHECK();
 }
 
-__global__ void k_init_mesh_scalars(uint32_t* grid_flag, uint32_t* bounds6) {
-    if (threadIdx.x == 0) *grid_flag = 1u;
-    if (threadIdx.x < 6) bounds6[threadIdx.x] = threadIdx.x < 3 ? 0xFFFFFFFFu : 0u;
+// Zero the first nzero stage counters, and mark the grid flag of every slot
+// in `stage_mask` (a slot about to be prepared) as "all on the grid" until
+// one of its vertices proves otherwise.
+__global__ void k_query_init(uint32_t* counters, uint32_t nzero, uint32_t* grid_flags, uint32_t stage_mask) {
+    const uint32_t t = threadIdx.x;
+    if (t < nzero) counters[t] = 0u;
+    if (t < 2 && ((stage_mask >> t) & 1u)) grid_flags[t] = 1u;
 }
 
-void launch_init_mesh_scalars(uint32_t* grid_flag, uint32_t* bounds6, hipStream_t s) {
-    k_init_mesh_scalars<<<1, 64, 0, s>>>(grid_flag, bounds6);
+void launch_query_init(uint32_t* counters, uint32_t nzero, uint32_t* grid_flags, uint32_t stage_mask, hipStream_t s) {
+    k_query_init<<<1, 64, 0, s>>>(counters, nzero, grid_flags, stage_mask);
     GI_LAUNCH_CHECK();
 }
 
@@ -129,27 +133,12 @@ __device__ __forceinline__ void block_box_partial(const float* l, const float* h
     }
 }
 
-// one block: the partial boxes -> bounds6 (order-preserving keys)
-__global__ __launch_bounds__(256) void k_reduce_bounds(const float* __restrict__ part, uint32_t nparts,
-                                                       uint32_t* bounds6) {
-    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x)
-        for (int k = 0; k < 3; ++k) {
-            l[k] = fminf(l[k], part[6 * i + k]);
-            h[k] = fmaxf(h[k], part[6 * i + 3 + k]);
-        }
-    __shared__ float fin[6];
-    block_box_partial(l, h, fin);
-    __syncthreads();
-    if (threadIdx.x < 6) bounds6[threadIdx.x] = f2key(fin[threadIdx.x]);
-}
-
-// One vertex: optional transform and rounding, the double and integer
-// copies, the grid test and the running FP32 bounds (Morton frame only, so
-// they need not be rounded outward).
-__device__ __forceinline__ bool prepare_vertex(const double* __restrict__ src, uint32_t i, const Xform& xf,
-                                               double4* __restrict__ v64, int4* __restrict__ vi, float* l, float* h) {
-    double x = src[3 * (size_t)i], y = src[3 * (size_t)i + 1], z = src[3 * (size_t)i + 2];
+// vertex i as the query sees it: moved, and snapped for grid meshes
+__device__ __forceinline__ void moved_vertex(const double* __restrict__ src, uint32_t i, const Xform& xf, double& x,
+                                             double& y, double& z) {
+    x = src[3 * (size_t)i];
+    y = src[3 * (size_t)i + 1];
+    z = src[3 * (size_t)i + 2];
     if (!xf.identity) {
         double X = xform_coord(xf, 0, x, y, z);
         double Y = xform_coord(xf, 1, x, y, z);
@@ -157,6 +146,15 @@ __device__ __forceinline__ bool prepare_vertex(const double* __restrict__ src, u
         x = X; y = Y; z = Z;
     }
     if (xf.snap) { x = rint(x); y = rint(y); z = rint(z); }
+}
+
+// One vertex: moved, the double and integer copies, the grid test and the
+// running FP32 bounds (Morton frame only, so they need not be rounded
+// outward).
+__device__ __forceinline__ bool prepare_vertex(const double* __restrict__ src, uint32_t i, const Xform& xf,
+                                               double4* __restrict__ v64, int4* __restrict__ vi, float* l, float* h) {
+    double x, y, z;
+    moved_vertex(src, i, xf, x, y, z);
     v64[i] = make_double4(x, y, z, 0.0);
     const double L = 67108864.0;  // 2^26
     bool g = x == rint(x) && y == rint(y) && z == rint(z) && fabs(x) <= L && fabs(y) <= L && fabs(z) <= L;
@@ -169,28 +167,6 @@ __device__ __forceinline__ bool prepare_vertex(const double* __restrict__ src, u
 
 // Moves the vertices (when a transform is set), classifies the mesh as grid
 // or not, writes the integer copy and per-block partial bounds (Morton frame).
-__global__ __launch_bounds__(256) void k_prepare_vertices(const double* __restrict__ src, uint32_t nv,
-                                                          const Xform* __restrict__ xfp, double4* __restrict__ v64,
-                                                          int4* __restrict__ vi, uint32_t* grid_flag,
-                                                          float* __restrict__ part) {
-    const Xform xf = *xfp;
-    bool grid = true;
-    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x)
-        grid = prepare_vertex(src, i, xf, v64, vi, l, h) && grid;
-    uint64_t bad = __ballot(!grid);
-    if (bad && lane_id() == 0) atomicAnd(grid_flag, 0u);
-    block_box_partial(l, h, part + 6 * (size_t)blockIdx.x);
-}
-
-void launch_prepare_vertices(const double* src, uint32_t nv, const Xform* xf, double4* v64, int4* vi,
-                             uint32_t* grid_flag, float* partials, uint32_t* bounds6, hipStream_t s) {
-    if (nv == 0) return;
-    uint32_t blocks = std::min<uint32_t>(blocks_for(nv, 256), kPrepareMaxBlocks);
-    k_prepare_vertices<<<blocks, 256, 0, s>>>(src, nv, xf, v64, vi, grid_flag, partials);
-    k_reduce_bounds<<<1, 256, 0, s>>>(partials, blocks, bounds6);
-    GI_LAUNCH_CHECK();
-}
 
 // ---------------------------------------------------------- triangle boxes
 
@@ -250,34 +226,99 @@ __device__ __forceinline__ void morton_frame(const uint32_t* __restrict__ bounds
     }
 }
 
-// Per triangle: its box, and, when keys != nullptr, the Morton key of its
-// centroid with the triangle index as value.
-__global__ __launch_bounds__(256) void k_tri_boxes(const uint32_t* __restrict__ tri, uint32_t nt,
-                                                   const double4* __restrict__ v, const int4* __restrict__ vi,
-                                                   const uint32_t* __restrict__ grid_flag,
-                                                   const uint32_t* __restrict__ bounds6, int axis_bits,
-                                                   float4* __restrict__ box, uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nt) return;
-    float bl[3], bh[3];
-    tri_box(tri, i, *grid_flag != 0, v, vi, bl, bh);
-    box[2 * (size_t)i] = make_float4(bl[0], bl[1], bl[2], 0.f);
-    box[2 * (size_t)i + 1] = make_float4(bh[0], bh[1], bh[2], 0.f);
-    if (keys) {
-        float bmin[3], bext[3];
-        morton_frame(bounds6, bmin, bext);
-        keys[i] = morton_code(bl, bh, bmin, bext, axis_bits);
-        vals[i] = i;
+// radix sort tiles: SORT_ITEMS keys per thread, 8-bit digits
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS = 4;
+constexpr uint32_t SORT_TILE = SORT_THREADS * SORT_ITEMS;
+constexpr int RADIX = 256;
+
+// Everything of one mesh before the Morton sort, in one launch. Blocks
+// [0, vblocks) prepare the vertices (move, snap, grid check, double and int
+// copies); every later block takes one sort tile of 1024 triangles: box,
+// Morton key, identity value and the tile's histogram of the first sort
+// digit (the sort's first count pass). A triangle moves its three vertices
+// itself, with the same arithmetic as the vertex blocks, so the two halves
+// do not wait for each other; the Morton frame comes from the host (the
+// box of the moved corners of the source vertices' bounds), not from a
+// reduction in this launch; centroids outside it would be clamped to the
+// border cells. The boxes use the doubles: for a grid mesh the integer
+// vertices are the same values.
+__global__ __launch_bounds__(256) void k_stage_mesh(const double* __restrict__ src, uint32_t nv,
+                                                    const uint32_t* __restrict__ tri, uint32_t nt,
+                                                    const Xform* __restrict__ xfp, double4* __restrict__ v64,
+                                                    int4* __restrict__ vi, uint32_t* grid_flag, uint32_t vblocks,
+                                                    int axis_bits, float4* __restrict__ box,
+                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                    uint32_t* __restrict__ hist0) {
+    const Xform xf = *xfp;
+    if (blockIdx.x < vblocks) {
+        bool grid = true;
+        float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += vblocks * blockDim.x)
+            grid = prepare_vertex(src, i, xf, v64, vi, l, h) && grid;
+        if (__ballot(!grid) && lane_id() == 0) atomicAnd(grid_flag, 0u);
+        return;
+    }
+    const uint32_t tile = blockIdx.x - vblocks;
+    __shared__ uint32_t hs[4][RADIX];
+    __shared__ float frame[6];  // Morton frame: lo[3], ext[3]
+    if (hist0)
+        for (int d = threadIdx.x; d < 4 * RADIX; d += blockDim.x) (&hs[0][0])[d] = 0u;
+    if (threadIdx.x == 0) {
+        double lo[3] = {xf.lo[0], xf.lo[1], xf.lo[2]}, hi[3] = {xf.hi[0], xf.hi[1], xf.hi[2]};
+        if (!xf.identity) {
+            for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+            for (int c = 0; c < 8; ++c) {
+                const double x = (c & 1) ? xf.hi[0] : xf.lo[0], y = (c & 2) ? xf.hi[1] : xf.lo[1],
+                             z = (c & 4) ? xf.hi[2] : xf.lo[2];
+                for (int k = 0; k < 3; ++k) {
+                    const double v = xform_coord(xf, k, x, y, z);
+                    lo[k] = fmin(lo[k], v);
+                    hi[k] = fmax(hi[k], v);
+                }
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            frame[k] = (float)lo[k];
+            frame[3 + k] = (float)hi[k] - (float)lo[k];
+        }
+    }
+    __syncthreads();
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        const uint32_t i = tile * SORT_TILE + it * 256 + threadIdx.x;
+        if (i >= nt) continue;
+        double p[3][3];
+        for (int c = 0; c < 3; ++c) moved_vertex(src, tri[3 * (size_t)i + c], xf, p[c][0], p[c][1], p[c][2]);
+        float bl[3], bh[3];
+        for (int k = 0; k < 3; ++k) {
+            bl[k] = f32_down(fmin(fmin(p[0][k], p[1][k]), p[2][k]));
+            bh[k] = f32_up(fmax(fmax(p[0][k], p[1][k]), p[2][k]));
+        }
+        box[2 * (size_t)i] = make_float4(bl[0], bl[1], bl[2], 0.f);
+        box[2 * (size_t)i + 1] = make_float4(bh[0], bh[1], bh[2], 0.f);
+        if (keys) {
+            const uint32_t key = morton_code(bl, bh, frame, frame + 3, axis_bits);
+            keys[i] = key;
+            vals[i] = i;
+            if (hist0) atomicAdd(&hs[threadIdx.x >> 6][key & (RADIX - 1)], 1u);
+        }
+    }
+    if (hist0) {
+        __syncthreads();
+        const uint32_t ntiles = (nt + SORT_TILE - 1) / SORT_TILE;
+        for (int d = threadIdx.x; d < RADIX; d += blockDim.x)
+            hist0[(size_t)d * ntiles + tile] = hs[0][d] + hs[1][d] + hs[2][d] + hs[3][d];
     }
 }
 
-void launch_tri_boxes(const uint32_t* tri, uint32_t nt, const double4* v64, const int4* vi, const uint32_t* grid_flag,
-                      const uint32_t* bounds6, int axis_bits, float4* box, uint32_t* keys, uint32_t* vals,
-                      hipStream_t s) {
-    if (nt == 0) return;
-    k_tri_boxes<<<blocks_for(nt, 256), 256, 0, s>>>(tri, nt, v64, vi, grid_flag, bounds6, axis_bits, box, keys,
-                                                    vals);
+void launch_stage_mesh(const double* src, uint32_t nv, const uint32_t* tri, uint32_t nt, const Xform* xf,
+                       double4* v64, int4* vi, uint32_t* grid_flag, int axis_bits, float4* box, uint32_t* keys,
+                       uint32_t* vals, uint32_t* hist0, hipStream_t s) {
+    const uint32_t vblocks = nv ? std::min<uint32_t>(blocks_for(nv, 256), kPrepareMaxBlocks) : 0u;
+    const uint32_t tblocks = blocks_for(nt, SORT_TILE);
+    if (vblocks + tblocks == 0) return;
+    k_stage_mesh<<<vblocks + tblocks, 256, 0, s>>>(src, nv, tri, nt, xf, v64, vi, grid_flag, vblocks, axis_bits, box,
+                                                   keys, vals, hist0);
     GI_LAUNCH_CHECK();
 }
 
@@ -482,21 +523,38 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t*
 
 // -------------------------------------------------------------- radix sort
 
-constexpr int SORT_THREADS = 256;
-constexpr int SORT_ITEMS = 4;
-constexpr uint32_t SORT_TILE = SORT_THREADS * SORT_ITEMS;
-constexpr int RADIX = 256;
-
 size_t sort_hist_elems(size_t n) { return RADIX * ((n + SORT_TILE - 1) / SORT_TILE) + 1; }
 
+// One pass of up to two independent sorts (two meshes' Morton keys) in each
+// launch: upsweep and downsweep blocks [0, split) serve sort a, the rest sort
+// b; digit-scan blocks [0, scan_split) sort a. Fields are picked with selects
+// on a block-uniform flag (no dynamic indexing of the kernel arguments).
 template <class K>
-__global__ __launch_bounds__(SORT_THREADS) void k_sort_upsweep(const K* __restrict__ keys, uint32_t n, int shift,
-                                                               uint32_t* __restrict__ hist, uint32_t ntiles) {
+struct SortPass {
+    const K* kin_a; const K* kin_b;
+    const uint32_t* vin_a; const uint32_t* vin_b;
+    K* kout_a; K* kout_b;
+    uint32_t* vout_a; uint32_t* vout_b;
+    uint32_t* hist_a; uint32_t* hist_b;      // per-tile digit counts, digit-major (d * ntiles + tile)
+    uint32_t* totals_a; uint32_t* totals_b;  // digit totals
+    uint32_t n_a, n_b, ntiles_a, ntiles_b;
+    uint32_t split, scan_split;
+    int shift;
+};
+
+template <class K>
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_upsweep(SortPass<K> P) {
+    const bool b = blockIdx.x >= P.split;
+    const K* __restrict__ keys = b ? P.kin_b : P.kin_a;
+    uint32_t* __restrict__ hist = b ? P.hist_b : P.hist_a;
+    const uint32_t n = b ? P.n_b : P.n_a, ntiles = b ? P.ntiles_b : P.ntiles_a;
+    const uint32_t tile = blockIdx.x - (b ? P.split : 0u);
+    const int shift = P.shift;
     __shared__ uint32_t h[SORT_THREADS / 64][RADIX];
     uint32_t wave = threadIdx.x >> 6;
     for (int d = threadIdx.x; d < (SORT_THREADS / 64) * RADIX; d += SORT_THREADS) (&h[0][0])[d] = 0;
     __syncthreads();
-    uint32_t base = blockIdx.x * SORT_TILE;
+    uint32_t base = tile * SORT_TILE;
     for (int it = 0; it < SORT_ITEMS; ++it) {
         uint32_t i = base + it * SORT_THREADS + threadIdx.x;
         if (i < n) atomicAdd(&h[wave][(uint32_t)(keys[i] >> shift) & (RADIX - 1)], 1u);
@@ -505,7 +563,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_upsweep(const K* __restri
     for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) {
         uint32_t c = 0;
         for (int w = 0; w < SORT_THREADS / 64; ++w) c += h[w][d];
-        hist[(size_t)d * ntiles + blockIdx.x] = c;
+        hist[(size_t)d * ntiles + tile] = c;
     }
 }
 
@@ -514,9 +572,13 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_upsweep(const K* __restri
 // One block per digit: exclusive scan of that digit's per-tile counts (in
 // place) and the digit total. The digit totals are scanned by each
 // downsweep block, so a pass is three launches.
-__global__ __launch_bounds__(SCAN_THREADS) void k_sort_digit_scan(uint32_t* __restrict__ hist, uint32_t ntiles,
-                                                                  uint32_t* __restrict__ totals) {
-    uint32_t* h = hist + (size_t)blockIdx.x * ntiles;
+template <class K>
+__global__ __launch_bounds__(SCAN_THREADS) void k_sort_digit_scan(SortPass<K> P) {
+    const bool b = blockIdx.x >= P.scan_split;
+    const uint32_t ntiles = b ? P.ntiles_b : P.ntiles_a;
+    uint32_t* totals = b ? P.totals_b : P.totals_a;
+    const uint32_t digit = blockIdx.x - (b ? P.scan_split : 0u);
+    uint32_t* h = (b ? P.hist_b : P.hist_a) + (size_t)digit * ntiles;
     uint32_t carry = 0;
     for (uint32_t base = 0; base < ntiles; base += SCAN_TILE) {
         uint32_t v[SCAN_ITEMS];
@@ -535,7 +597,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_sort_digit_scan(uint32_t* __re
         }
         carry += tot;
     }
-    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+    if (threadIdx.x == 0) totals[digit] = carry;
 }
 
 // Stable rank + scatter of one tile: keys are ranked in (wave, item, lane)
@@ -597,11 +659,16 @@ __device__ __forceinline__ void tile_rank_scatter(const K* __restrict__ kin, con
 }
 
 template <class K>
-__global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                                 K* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                                 uint32_t n, int shift,
-                                                                 const uint32_t* __restrict__ hist, uint32_t ntiles,
-                                                                 const uint32_t* __restrict__ totals) {
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(SortPass<K> P) {
+    const bool b = blockIdx.x >= P.split;
+    const K* __restrict__ kin = b ? P.kin_b : P.kin_a;
+    const uint32_t* __restrict__ vin = b ? P.vin_b : P.vin_a;
+    K* __restrict__ kout = b ? P.kout_b : P.kout_a;
+    uint32_t* __restrict__ vout = b ? P.vout_b : P.vout_a;
+    const uint32_t* __restrict__ hist = b ? P.hist_b : P.hist_a;
+    const uint32_t* __restrict__ totals = b ? P.totals_b : P.totals_a;
+    const uint32_t n = b ? P.n_b : P.n_a, ntiles = b ? P.ntiles_b : P.ntiles_a;
+    const uint32_t tile = blockIdx.x - (b ? P.split : 0u);
     __shared__ uint32_t wc[SORT_THREADS / 64][RADIX];
     __shared__ uint32_t gbase[RADIX];
     for (int d = threadIdx.x; d < (SORT_THREADS / 64) * RADIX; d += SORT_THREADS) (&wc[0][0])[d] = 0;
@@ -609,10 +676,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(const K* __rest
         // digit base = exclusive prefix of the digit totals (RADIX == SORT_THREADS)
         uint32_t t = totals[threadIdx.x];
         uint32_t ex = block_exclusive_sum(t, nullptr);
-        gbase[threadIdx.x] = ex + hist[(size_t)threadIdx.x * ntiles + blockIdx.x];
+        gbase[threadIdx.x] = ex + hist[(size_t)threadIdx.x * ntiles + tile];
     }
     __syncthreads();
-    tile_rank_scatter<K>(kin, vin, kout, vout, n, shift, blockIdx.x, wc, gbase);
+    tile_rank_scatter<K>(kin, vin, kout, vout, n, P.shift, tile, wc, gbase);
 }
 
 // Whole LSD radix sort in one workgroup (one launch) for inputs up to
@@ -711,11 +778,81 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_sort_small(K* k0, uint32_t* v
     }
 }
 
+// One sort's state across passes: ping-pong between (keys, vals) and (tk, tv).
+template <class K>
+struct SortRun {
+    K* keys; uint32_t* vals; K* tk; uint32_t* tv;
+    uint32_t n, ntiles;
+    int passes;
+    bool hist0_ready;  // the first pass's per-tile counts are already in hist
+    uint32_t* hist; uint32_t* totals;
+};
+
+// The multi-tile passes of one or two sorts; pass p of both runs in the same
+// three launches (a sort with fewer passes drops out). Returns, per sort, 1
+// when the sorted data ends in (tk, tv).
+template <class K>
+static void multi_tile_passes(SortRun<K>* r, int nr, hipStream_t s, int* parity) {
+    int maxp = 0;
+    for (int j = 0; j < nr; ++j) maxp = std::max(maxp, r[j].passes);
+    for (int p = 0; p < maxp; ++p) {
+        SortPass<K> P{};
+        P.shift = 8 * p;
+        bool act[2] = {false, false};
+        for (int j = 0; j < nr; ++j) act[j] = p < r[j].passes;
+        auto fill = [&](int j, bool second) {
+            const SortRun<K>& q = r[j];
+            const bool odd = p & 1;
+            const K* ki = odd ? q.tk : q.keys;
+            const uint32_t* vi = odd ? q.tv : q.vals;
+            K* ko = odd ? q.keys : q.tk;
+            uint32_t* vo = odd ? q.vals : q.tv;
+            if (!second) {
+                P.kin_a = ki; P.vin_a = vi; P.kout_a = ko; P.vout_a = vo;
+                P.hist_a = q.hist; P.totals_a = q.totals; P.n_a = q.n; P.ntiles_a = q.ntiles;
+            } else {
+                P.kin_b = ki; P.vin_b = vi; P.kout_b = ko; P.vout_b = vo;
+                P.hist_b = q.hist; P.totals_b = q.totals; P.n_b = q.n; P.ntiles_b = q.ntiles;
+            }
+        };
+        // slot a: the first active sort, slot b: the second (if any)
+        int ja = -1, jb = -1;
+        for (int j = 0; j < nr; ++j)
+            if (act[j]) { if (ja < 0) ja = j; else jb = j; }
+        if (ja < 0) break;
+        fill(ja, false);
+        if (jb >= 0) fill(jb, true);
+        // upsweep: only the sorts whose counts for this pass are not there yet
+        {
+            SortPass<K> U{};
+            U.shift = P.shift;
+            uint32_t nblk[2] = {0u, 0u};
+            int k = 0;
+            for (int j : {ja, jb}) {
+                if (j < 0 || (p == 0 && r[j].hist0_ready)) continue;
+                const K* ki = (p & 1) ? r[j].tk : r[j].keys;
+                if (k == 0) { U.kin_a = ki; U.hist_a = r[j].hist; U.n_a = r[j].n; U.ntiles_a = r[j].ntiles; }
+                else { U.kin_b = ki; U.hist_b = r[j].hist; U.n_b = r[j].n; U.ntiles_b = r[j].ntiles; }
+                nblk[k++] = r[j].ntiles;
+            }
+            U.split = nblk[0];
+            if (nblk[0] + nblk[1]) k_sort_upsweep<K><<<nblk[0] + nblk[1], SORT_THREADS, 0, s>>>(U);
+        }
+        P.split = P.ntiles_a;
+        P.scan_split = RADIX;
+        const uint32_t scans = jb >= 0 ? 2 * RADIX : RADIX;
+        k_sort_digit_scan<K><<<scans, SCAN_THREADS, 0, s>>>(P);
+        k_sort_downsweep<K><<<P.ntiles_a + (jb >= 0 ? P.ntiles_b : 0u), SORT_THREADS, 0, s>>>(P);
+    }
+    for (int j = 0; j < nr; ++j) parity[j] = r[j].passes & 1;
+    GI_LAUNCH_CHECK();
+}
+
 // Ping-pongs between (keys, vals) and (tk, tv); returns 1 when the sorted
 // data ends in (tk, tv), 0 when it ends in (keys, vals).
 template <class K>
 static int radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32_t n, int bits, uint32_t* hist,
-                           uint32_t* scan_tmp, hipStream_t s) {
+                           uint32_t* scan_tmp, hipStream_t s, bool hist0_ready = false) {
     if (n <= 1) return 0;
     if (n <= small_sort_max<K>()) {
         int passes = (bits + 7) / 8;
@@ -723,29 +860,38 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32_
         GI_LAUNCH_CHECK();
         return passes & 1;
     }
-    uint32_t ntiles = (n + SORT_TILE - 1) / SORT_TILE;
-    int passes = (bits + 7) / 8;
-    K* src_k = keys;
-    uint32_t* src_v = vals;
-    K* dst_k = tk;
-    uint32_t* dst_v = tv;
-    uint32_t* totals = scan_tmp;  // RADIX entries
-    for (int p = 0; p < passes; ++p) {
-        int shift = 8 * p;
-        k_sort_upsweep<K><<<ntiles, SORT_THREADS, 0, s>>>(src_k, n, shift, hist, ntiles);
-        k_sort_digit_scan<<<RADIX, SCAN_THREADS, 0, s>>>(hist, ntiles, totals);
-        k_sort_downsweep<K><<<ntiles, SORT_THREADS, 0, s>>>(src_k, src_v, dst_k, dst_v, n, shift, hist, ntiles,
-                                                            totals);
-        K* t = src_k; src_k = dst_k; dst_k = t;
-        uint32_t* tvv = src_v; src_v = dst_v; dst_v = tvv;
-    }
-    GI_LAUNCH_CHECK();
-    return src_k != keys ? 1 : 0;
+    SortRun<K> r{keys, vals, tk, tv, n, (n + SORT_TILE - 1) / SORT_TILE, (bits + 7) / 8, hist0_ready, hist, scan_tmp};
+    int parity = 0;
+    multi_tile_passes<K>(&r, 1, s, &parity);
+    return parity;
 }
 
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
-                   uint32_t* hist, uint32_t* scan_tmp, hipStream_t s) {
-    return radix_sort_impl<uint32_t>(keys, vals, tmp_keys, tmp_vals, n, bits, hist, scan_tmp, s);
+                   uint32_t* hist, uint32_t* scan_tmp, hipStream_t s, bool hist0_ready) {
+    return radix_sort_impl<uint32_t>(keys, vals, tmp_keys, tmp_vals, n, bits, hist, scan_tmp, s, hist0_ready);
+}
+
+bool radix_sort_takes_hist0(uint32_t n) { return n > small_sort_max<uint32_t>(); }
+
+void radix_sort_u32_pair(const SortJobU32* jobs, int njobs, hipStream_t s, int* in_tmp) {
+    SortRun<uint32_t> r[2];
+    int map[2], nr = 0;
+    for (int j = 0; j < njobs; ++j) {
+        const SortJobU32& q = jobs[j];
+        in_tmp[j] = 0;
+        if (q.n <= 1) continue;
+        if (!radix_sort_takes_hist0(q.n)) {  // one workgroup, its own launch
+            in_tmp[j] = radix_sort_impl<uint32_t>(q.keys, q.vals, q.tkeys, q.tvals, q.n, q.bits, q.hist, q.scan_tmp, s);
+            continue;
+        }
+        r[nr] = SortRun<uint32_t>{q.keys, q.vals, q.tkeys, q.tvals, q.n, (q.n + SORT_TILE - 1) / SORT_TILE,
+                                  (q.bits + 7) / 8, q.hist0_ready, q.hist, q.scan_tmp};
+        map[nr++] = j;
+    }
+    if (nr == 0) return;
+    int parity[2] = {0, 0};
+    multi_tile_passes<uint32_t>(r, nr, s, parity);
+    for (int k = 0; k < nr; ++k) in_tmp[map[k]] = parity[k];
 }
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
                    uint32_t* hist, uint32_t* scan_tmp, hipStream_t s) {
