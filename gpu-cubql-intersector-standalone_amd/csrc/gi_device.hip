@@ -1403,16 +1403,36 @@ void launch_traverse(MeshDev A, MeshDev B, const uint64_t* keysB, int mesh_shift
 
 // ------------------------------------------------------------------ filter
 
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter, uint32_t& slot) {
-    uint64_t m = __ballot(pred);
-    uint32_t base = 0;
-    if (m) {
-        uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
-        if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-        base = __shfl(base, (int)leader);
+
+__device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& total) {
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane_id() >= (uint32_t)o) x += y;
     }
-    slot = base + (uint32_t)__popcll(m & lanemask_lt());
-    return (uint32_t)__popcll(m);
+    total = __shfl(x, 63);
+    return x - v;
+}
+
+// Reserves v slots for every thread of a 256-thread workgroup with one atomic
+// on `counter` and returns the thread's first slot (exclusive prefix in
+// thread order). Same-address atomics from every wave serialise (~10 ns
+// each on this GPU), so a stage that appends from all its waves at once
+// reserves per workgroup. Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_reserve(uint32_t v, uint32_t* counter) {
+    __shared__ uint32_t wtot[4], wbase[4];
+    uint32_t total;
+    const uint32_t ex = wave_exclusive_sum(v, total);
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 0) wtot[wave] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+        uint32_t b = t ? atomicAdd(counter, t) : 0u;
+        for (int w = 0; w < 4; ++w) { wbase[w] = b; b += wtot[w]; }
+    }
+    __syncthreads();
+    return wbase[wave] + ex;
 }
 
 __device__ __forceinline__ uint32_t block_sum(uint32_t v) {
@@ -1474,9 +1494,8 @@ __global__ __launch_bounds__(256) void k_filter(MeshDev A, MeshDev B, const uint
                 c_yellow += status == PAIR_YELLOW;
             }
         }
-        uint32_t slot;
-        bool emit = status != PAIR_NO;
-        wave_append(emit, &counters[CNT_EXACT_LIST], slot);
+        const bool emit = status != PAIR_NO;
+        const uint32_t slot = block_reserve(emit ? 1u : 0u, &counters[CNT_EXACT_LIST]);
         if (emit && slot < exact_cap) exact_list[slot] = make_uint2(a, b | ((uint32_t)status << kStatusShift));
     }
     uint32_t t;
@@ -1497,15 +1516,6 @@ void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const 
 
 typedef ExactTT<GridPolicy> GridExact;
 
-__device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& total) {
-    uint32_t x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o);
-        if (lane_id() >= (uint32_t)o) x += y;
-    }
-    total = __shfl(x, 63);
-    return x - v;
-}
 
 // Exact decision and segments of the listed pairs. The fast instance keeps
 // its per-lane state in registers: a coplanar pair (polygon pieces) is only
@@ -1555,26 +1565,18 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
             }
         }
         if constexpr (kFast) {
-            uint32_t dslot;
-            wave_append(r.deferred != 0, &counters[CNT_DEFERRED], dslot);
+            const uint32_t dslot = block_reserve(r.deferred ? 1u : 0u, &counters[CNT_DEFERRED]);
             if (r.deferred && dslot < defer_cap) defer[dslot] = e;
         }
         const bool hit = r.hit && !r.deferred;
-        uint32_t slot;
-        wave_append(hit, &counters[CNT_RESULTS], slot);
-        uint32_t nseg = hit ? (uint32_t)r.nseg : 0u;
-        uint32_t wtot;
-        uint32_t soff = wave_exclusive_sum(nseg, wtot);
-        uint32_t sbase = 0;
-        if (wtot) {
-            if (lane_id() == 0) sbase = atomicAdd(&counters[CNT_SEGMENTS], wtot);
-            sbase = __shfl(sbase, 0);
-        }
+        const uint32_t slot = block_reserve(hit ? 1u : 0u, &counters[CNT_RESULTS]);
+        const uint32_t nseg = hit ? (uint32_t)r.nseg : 0u;
+        const uint32_t sfirst = block_reserve(nseg, &counters[CNT_SEGMENTS]);
         if (hit && slot < rec_cap) {
             rec_key[slot] = ((uint64_t)a << bitsB) | (uint64_t)b;
-            rec_seg[slot] = make_uint2(sbase + soff, nseg);
+            rec_seg[slot] = make_uint2(sfirst, nseg);
             for (uint32_t sI = 0; sI < nseg; ++sI) {
-                uint32_t dst = sbase + soff + sI;
+                uint32_t dst = sfirst + sI;
                 if (dst >= seg_cap) break;
                 GridExact::HP p, q;
                 if constexpr (kFast) {
