@@ -103,16 +103,21 @@ Xform identity_xform() {
     return x;
 }
 
-// bounds of n vertices (NaN coordinates skipped; an empty set gives zeros)
-void vertex_bounds(const double* xyz, size_t n, double* lo, double* hi) {
-    for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+// bounds of n vertices (NaN coordinates skipped; an empty set gives zeros),
+// copying them to `copy` when it is non-null (one pass over the input)
+void vertex_bounds(const double* xyz, size_t n, double* lo, double* hi, double* copy = nullptr) {
+    double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (size_t i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) {
-            lo[k] = fmin(lo[k], xyz[3 * i + k]);
-            hi[k] = fmax(hi[k], xyz[3 * i + k]);
+            const double x = xyz[3 * i + k];
+            if (copy) copy[3 * i + k] = x;
+            l[k] = x < l[k] ? x : l[k];
+            h[k] = x > h[k] ? x : h[k];
         }
-    for (int k = 0; k < 3; ++k)
-        if (!(lo[k] <= hi[k])) lo[k] = hi[k] = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = l[k] <= h[k] ? l[k] : 0.0;
+        hi[k] = l[k] <= h[k] ? h[k] : 0.0;
+    }
 }
 
 int bits_for(uint32_t n) { return n <= 1 ? 1 : 32 - __builtin_clz(n - 1); }
@@ -1054,9 +1059,9 @@ int gi_set_mesh(gi_context* ctx, int slot, const double* xyz, uint32_t nv, const
         if (m.borrowed) m.release();
         ++m.generation;
         m.forest = 0;
-        m.h_xyz.assign(xyz, xyz + 3 * (size_t)nv);
+        m.h_xyz.resize(3 * (size_t)nv);
+        vertex_bounds(xyz, nv, m.src_lo, m.src_hi, m.h_xyz.data());
         m.h_tri.assign(tri, tri + 3 * (size_t)nt);
-        vertex_bounds(xyz, nv, m.src_lo, m.src_hi);
         m.nv = nv;
         m.nt = nt;
         m.set = true;
