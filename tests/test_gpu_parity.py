@@ -57,6 +57,18 @@ def test_sphere_torus_multi_block_sort_and_chunks():
     assert len(res["pairs"]) > 1000
 
 
+def test_sphere_above_2_21_triangles_with_small_torus():
+    # a 2,359,296-triangle sphere: past 2^21 triangles its Morton keys take 10
+    # bits per axis (30-bit keys, 4 radix passes) while the 65,536-triangle
+    # torus keeps 8 (3 passes), so the two sorts sharing each pass's launches
+    # differ in tiles and in passes (the torus drops out of the last one)
+    (xa, ta), _ = mg.sphere_torus(1536, 768)
+    _, (xb, tb) = mg.sphere_torus(256, 128)
+    assert len(ta) > (1 << 21) and len(tb) < (1 << 21)
+    res, ref = _check(xa, ta, xb, tb, True)
+    assert len(res["pairs"]) > 1000
+
+
 def test_icosphere_pair_doubles_host_residual():
     (xa, ta), (xb, tb) = mg.icosphere_pair(level=4)
     res, ref = _check(xa, ta, xb, tb, False)
