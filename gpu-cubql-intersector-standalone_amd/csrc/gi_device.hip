@@ -1240,13 +1240,15 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
             off += 2 * (size_t)block_level_count(nn[t], k);
         }
     }
-    // level 1 (64-leaf blocks) was written by k_build_chunks
+    // level 1 (64-leaf blocks) was written by k_build_chunks. A union reads
+    // level k only where a range covers more than 64 boxes of level k - 1, so
+    // level k is built only when that level has more than 64 boxes.
     for (int k = 2; k < kMaxBlockLevels; ++k) {
         BlockBoxJobs bj;
         uint32_t nin[2], nout[2];
         for (int t = 0; t < 2; ++t) {
             nin[t] = block_level_count(nn[t], k - 1);
-            nout[t] = nin[t] > 1 ? block_level_count(nn[t], k) : 0;  // levels stop at one box
+            nout[t] = nin[t] > 64 ? block_level_count(nn[t], k) : 0;
         }
         if (nout[0] == 0 && nout[1] == 0) break;
         bj.in0 = lv[0].level[k - 1]; bj.out0 = (float4*)lv[0].level[k]; bj.n_in0 = nin[0]; bj.n_out0 = nout[0];
