@@ -78,12 +78,14 @@ struct MeshSlot {
     double src_lo[3] = {0.0, 0.0, 0.0}, src_hi[3] = {0.0, 0.0, 0.0};  // bounds of the source vertices
     uint64_t generation = 0;   // bumped whenever the slot's mesh or buffers are replaced
     DevBuf src, tri, v64, vi, tbox, keys, vals, tkeys, tvals, nodes, range, blocks;
+    DevBuf samples;  // every 2^key_sample_shift(nt)-th sorted Morton key (multi-tile sorts only)
     DevBuf voff, toff, vtiles, ttiles, vtile0, fbounds, roots;
     void release() {
         ++generation;
         DevBuf* forest_bufs[] = {&voff, &toff, &vtiles, &ttiles, &vtile0, &fbounds, &roots};
         for (DevBuf* b : forest_bufs) b->release();
-        DevBuf* all[] = {&src, &tri, &v64, &vi, &tbox, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks};
+        DevBuf* all[] = {&src, &tri, &v64, &vi, &tbox, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks,
+                         &samples};
         for (DevBuf* b : all) {
             if (borrowed) { b->p = nullptr; b->bytes = 0; }
             else b->release();
@@ -332,11 +334,13 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
         stmp = ctx->bscan[slot].ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
     }
     const bool hist0 = topology && radix_sort_takes_hist0(m.nt);
+    uint32_t* samp = hist0 ? m.samples.ensure<uint32_t>(key_sample_count(m.nt)) : nullptr;
     launch_stage_mesh(m.src.get<double>(), m.nv, m.tri.get<uint32_t>(), m.nt, d_xf, v64, vi, grid + slot,
                       m.morton_axis, tbox, topology ? keys : nullptr, topology ? vals : nullptr,
                       hist0 ? hist : nullptr, s);
     // the Morton sort runs after the join, together with the other slot's
-    if (topology) ctx->pending_sort[slot] = SortJobU32{keys, vals, tk, tv, m.nt, 3 * m.morton_axis, hist0, hist, stmp};
+    if (topology)
+        ctx->pending_sort[slot] = SortJobU32{keys, vals, tk, tv, m.nt, 3 * m.morton_axis, hist0, hist, stmp, samp};
     return true;
 }
 
@@ -350,6 +354,10 @@ BuildJob<K> slot_job(MeshSlot& m) {
     j.nodes = m.nodes.get<float4>();
     j.range = m.range.get<uint2>();
     j.blocks = m.blocks.get<float4>();
+    // the samples the last topology sort wrote stay valid for a refit (same keys)
+    j.samples = nullptr;
+    if constexpr (sizeof(K) == 4)
+        if (!m.forest && radix_sort_takes_hist0(m.nt)) j.samples = m.samples.get<K>();
     return j;
 }
 
@@ -1223,9 +1231,9 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         d.morton_axis = s.morton_axis;
         d.borrowed = true;
         DevBuf* dd[] = {&d.src, &d.tri, &d.v64, &d.vi, &d.tbox, &d.keys, &d.vals, &d.tkeys, &d.tvals, &d.nodes,
-                        &d.range, &d.blocks};
+                        &d.range, &d.blocks, &d.samples};
         const DevBuf* ss[] = {&s.src, &s.tri, &s.v64, &s.vi, &s.tbox, &s.keys, &s.vals, &s.tkeys, &s.tvals,
-                              &s.nodes, &s.range, &s.blocks};
+                              &s.nodes, &s.range, &s.blocks, &s.samples};
         static_assert(sizeof(dd) / sizeof(dd[0]) == sizeof(ss) / sizeof(ss[0]), "buffer lists differ");
         for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); ++i) { dd[i]->p = ss[i]->p; dd[i]->bytes = ss[i]->bytes; }
         // the owner's grid classification of this mesh

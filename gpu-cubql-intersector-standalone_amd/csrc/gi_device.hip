@@ -24,6 +24,7 @@
 #include "exact_tritri.h"
 #include "filter.h"
 #include "gi_device.h"
+#include "gi_karras.h"
 
 namespace gi {
 
@@ -540,6 +541,8 @@ struct SortPass {
     uint32_t n_a, n_b, ntiles_a, ntiles_b;
     uint32_t split, scan_split;
     int shift;
+    K* samp_a; K* samp_b;  // last pass: every 2^samp_shift-th output key goes here (null: none)
+    int samp_shift_a, samp_shift_b;
 };
 
 template <class K>
@@ -608,7 +611,8 @@ template <class K>
 __device__ __forceinline__ void tile_rank_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                   K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
                                                   int shift, uint32_t tile, uint32_t (*wc)[RADIX],
-                                                  const uint32_t* gbase) {
+                                                  const uint32_t* gbase, K* __restrict__ samp = nullptr,
+                                                  int samp_shift = 0) {
     uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     uint32_t base = tile * SORT_TILE + wave * (64 * SORT_ITEMS);
     uint64_t lt = lanemask_lt();
@@ -655,6 +659,7 @@ __device__ __forceinline__ void tile_rank_scatter(const K* __restrict__ kin, con
         uint32_t pos = gbase[d] + wc[wave][d] + rank[it];
         kout[pos] = k[it];
         vout[pos] = v[it];
+        if (samp != nullptr && (pos & ((1u << samp_shift) - 1u)) == 0u) samp[pos >> samp_shift] = k[it];
     }
 }
 
@@ -679,7 +684,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(SortPass<K> P) 
         gbase[threadIdx.x] = ex + hist[(size_t)threadIdx.x * ntiles + tile];
     }
     __syncthreads();
-    tile_rank_scatter<K>(kin, vin, kout, vout, n, P.shift, tile, wc, gbase);
+    tile_rank_scatter<K>(kin, vin, kout, vout, n, P.shift, tile, wc, gbase, b ? P.samp_b : P.samp_a,
+                         b ? P.samp_shift_b : P.samp_shift_a);
 }
 
 // Whole LSD radix sort in one workgroup (one launch) for inputs up to
@@ -786,6 +792,7 @@ struct SortRun {
     int passes;
     bool hist0_ready;  // the first pass's per-tile counts are already in hist
     uint32_t* hist; uint32_t* totals;
+    K* samples;        // key samples written by the last pass (null: none)
 };
 
 // The multi-tile passes of one or two sorts; pass p of both runs in the same
@@ -807,12 +814,16 @@ static void multi_tile_passes(SortRun<K>* r, int nr, hipStream_t s, int* parity)
             const uint32_t* vi = odd ? q.tv : q.vals;
             K* ko = odd ? q.keys : q.tk;
             uint32_t* vo = odd ? q.vals : q.tv;
+            K* samp = p == q.passes - 1 ? q.samples : nullptr;
+            const int ssh = key_sample_shift(q.n);
             if (!second) {
                 P.kin_a = ki; P.vin_a = vi; P.kout_a = ko; P.vout_a = vo;
                 P.hist_a = q.hist; P.totals_a = q.totals; P.n_a = q.n; P.ntiles_a = q.ntiles;
+                P.samp_a = samp; P.samp_shift_a = ssh;
             } else {
                 P.kin_b = ki; P.vin_b = vi; P.kout_b = ko; P.vout_b = vo;
                 P.hist_b = q.hist; P.totals_b = q.totals; P.n_b = q.n; P.ntiles_b = q.ntiles;
+                P.samp_b = samp; P.samp_shift_b = ssh;
             }
         };
         // slot a: the first active sort, slot b: the second (if any)
@@ -860,7 +871,8 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32_
         GI_LAUNCH_CHECK();
         return passes & 1;
     }
-    SortRun<K> r{keys, vals, tk, tv, n, (n + SORT_TILE - 1) / SORT_TILE, (bits + 7) / 8, hist0_ready, hist, scan_tmp};
+    SortRun<K> r{keys, vals, tk, tv, n, (n + SORT_TILE - 1) / SORT_TILE, (bits + 7) / 8, hist0_ready, hist, scan_tmp,
+                 nullptr};
     int parity = 0;
     multi_tile_passes<K>(&r, 1, s, &parity);
     return parity;
@@ -872,6 +884,16 @@ int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t*
 }
 
 bool radix_sort_takes_hist0(uint32_t n) { return n > small_sort_max<uint32_t>(); }
+
+int key_sample_shift(uint32_t n) {
+    int sh = 8;
+    while (((uint64_t)n + (1ull << sh) - 1) >> sh > kMaxKeySamples) ++sh;
+    return sh;
+}
+uint32_t key_sample_count(uint32_t n) {
+    const int sh = key_sample_shift(n);
+    return (uint32_t)(((uint64_t)n + (1ull << sh) - 1) >> sh);
+}
 
 void radix_sort_u32_pair(const SortJobU32* jobs, int njobs, hipStream_t s, int* in_tmp) {
     SortRun<uint32_t> r[2];
@@ -885,7 +907,7 @@ void radix_sort_u32_pair(const SortJobU32* jobs, int njobs, hipStream_t s, int* 
             continue;
         }
         r[nr] = SortRun<uint32_t>{q.keys, q.vals, q.tkeys, q.tvals, q.n, (q.n + SORT_TILE - 1) / SORT_TILE,
-                                  (q.bits + 7) / 8, q.hist0_ready, q.hist, q.scan_tmp};
+                                  (q.bits + 7) / 8, q.hist0_ready, q.hist, q.scan_tmp, q.samples};
         map[nr++] = j;
     }
     if (nr == 0) return;
@@ -914,70 +936,6 @@ int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t*
 // A refit after a rigid motion reruns the same kernels on the same sorted
 // keys: the topology comes out identical, only the boxes change.
 
-// Sorted keys (Morton codes, or mesh id << 3 axis_bits | Morton code for a
-// forest), the chunk's slice read from LDS.
-template <class K>
-struct KeySrc {
-    const K* __restrict__ g;
-    const K* s;
-    int cs, ce, n;
-    __device__ __forceinline__ K at(int j) const { return (j >= cs && j < ce) ? s[j - cs] : g[j]; }
-};
-
-__device__ __forceinline__ int clz_key(uint32_t x) { return __clz(x); }
-__device__ __forceinline__ int clz_key(uint64_t x) { return __clzll((long long)x); }
-
-// common-prefix length of the composite keys (key, index). In a forest the
-// mesh id bits lead, so the leaves of one mesh share a longer prefix with
-// each other than with any other leaf and form one subtree.
-template <class K>
-__device__ __forceinline__ int delta(const KeySrc<K>& k, int i, int j) {
-    if (j < 0 || j >= k.n) return -1;
-    K a = k.at(i), b = k.at(j);
-    if (a != b) return clz_key(a ^ b);
-    return 8 * (int)sizeof(K) + __clz((uint32_t)i ^ (uint32_t)j);
-}
-
-// The node visited right after every subtree whose last leaf is m - 1: the
-// right child that starts at leaf m. In Karras' numbering a right child
-// covering [m, x] is internal node m, which is the case iff internal node m
-// extends to the right; otherwise it is leaf m.
-template <class K>
-__device__ __forceinline__ uint32_t chain_top(const KeySrc<K>& k, int m) {
-    if (m < k.n - 1 && delta(k, m, m + 1) > delta(k, m, m - 1)) return (uint32_t)m;
-    return (uint32_t)(k.n - 1 + m);
-}
-
-// Karras: range, split and children of internal node i, given its direction
-// d and dmin = delta(i, i - d). Probes outside [lo_ok, hi_ok] read as -1: the
-// caller guarantees the range ends inside, and past its end the common
-// prefix with key i can only shrink (sorted keys), so the comparisons with
-// dmin come out the same.
-template <class K>
-__device__ __forceinline__ void karras_node(const KeySrc<K>& KS, int i, int d, int dmin, int lo_ok, int hi_ok,
-                                            int& lo_i, int& hi_i, uint32_t& left, uint32_t& right, uint32_t& rope) {
-    const int n = KS.n;
-    auto dl = [&](int j) { return (j < lo_ok || j > hi_ok) ? -1 : delta(KS, i, j); };
-    int lmax = 2;
-    while (dl(i + lmax * d) > dmin) lmax <<= 1;
-    int l = 0;
-    for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (dl(i + (l + t) * d) > dmin) l += t;
-    int j = i + l * d;
-    int dnode = delta(KS, i, j);
-    int s = 0, t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (dl(i + (s + t) * d) > dnode) s += t;
-    } while (t > 1);
-    int gamma = i + s * d + (d < 0 ? -1 : 0);
-    lo_i = i < j ? i : j;
-    hi_i = i < j ? j : i;
-    left = lo_i == gamma ? (uint32_t)(n - 1 + gamma) : (uint32_t)gamma;
-    right = hi_i == gamma + 1 ? (uint32_t)(n - 1 + gamma + 1) : (uint32_t)(gamma + 1);
-    rope = hi_i == n - 1 ? kSentinel : chain_top(KS, hi_i + 1);
-}
-
 // Up to two trees per launch: blocks [0, split) build tree 0, the rest tree 1.
 // Fields are picked with selects on a block-uniform flag, so the kernel
 // arguments are never indexed dynamically.
@@ -989,7 +947,9 @@ struct ChunkJobs {
     float4* nodes0; float4* nodes1;
     uint2* range0; uint2* range1;
     float4* level1_0; float4* level1_1;  // boxes of the 64-leaf blocks (first level of the block hierarchy)
+    const K* samp0; const K* samp1;      // key samples (null: none)
     int n0, n1;
+    int samp_shift0, samp_shift1;
     uint32_t split;
 };
 
@@ -1008,6 +968,7 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
     uint2* __restrict__ range = second ? J.range1 : J.range0;
     float4* __restrict__ level1 = second ? J.level1_1 : J.level1_0;
     const int n = second ? J.n1 : J.n0;
+    const K* __restrict__ samp = second ? J.samp1 : J.samp0;
     const uint32_t chunk = blk - (second ? J.split : 0u);
     __shared__ K skey[CHUNK];
     __shared__ float4 sbox[2 * block_table_offset(CHUNK, 7)];  // (lo, hi) pairs of levels 0..6
@@ -1055,16 +1016,10 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
         nodes[2 * (size_t)(n - 1 + jl) + 1] = make_float4(leaf[3], leaf[4], leaf[5], __uint_as_float(rope));
     }
     if (jl >= min(ce, n - 1)) return;
-    const int d = delta(KS, jl, jl + 1) > delta(KS, jl, jl - 1) ? 1 : -1;
-    const int dmin = delta(KS, jl, jl - d);
-    // one probe past the chunk edge tells whether the range leaves the chunk
-    // (sorted keys: the common prefix only shrinks with distance); an
-    // in-chunk node then searches the LDS keys only
-    const int edge = d > 0 ? ce : cs - 1;
-    const bool top = edge >= 0 && edge < n && delta(KS, jl, edge) > dmin;
     int lo_i, hi_i;
     uint32_t left, right, rope;
-    karras_node(KS, jl, d, dmin, top ? 0 : cs, top ? n - 1 : ce - 1, lo_i, hi_i, left, right, rope);
+    const bool top = karras_chunk_node(KS, samp, second ? J.samp_shift1 : J.samp_shift0, jl, skey[t], lo_i, hi_i,
+                                       left, right, rope);
     range[jl] = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
     float q[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (!top) {
@@ -1222,6 +1177,11 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     const BuildJob<K>* o = j1 ? j1 : j0;
     cj.keys1 = o->keys; cj.prims1 = o->prims; cj.box1 = o->boxes;
     cj.nodes1 = o->nodes; cj.range1 = o->range; cj.level1_1 = o->blocks; cj.n1 = (int)n1;
+    // key samples: every 2^key_sample_shift(n)-th sorted key
+    cj.samp0 = j0->samples;
+    cj.samp_shift0 = key_sample_shift(n0);
+    cj.samp1 = j1 ? j1->samples : nullptr;
+    cj.samp_shift1 = key_sample_shift(n1);
     cj.split = blocks_for(n0, chunk);
     const uint32_t chunk_blocks = cj.split + blocks_for(n1, chunk);
     if (chunk_begin) (void)hipEventRecordWithFlags(chunk_begin, s, event_flags);
