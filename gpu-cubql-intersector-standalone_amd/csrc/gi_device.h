@@ -126,7 +126,8 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t*
 // n-1+k) holds prim sorted_prims[k] and its box; internal node i holds its
 // box, its left child and its rope (next node in pre-order after its
 // subtree); range[i] = its leaf range. `blocks` needs block_levels_elems(n)
-// float4. Running it again on the same sorted keys with new prim boxes is the
+// float4 (64-ary levels of leaf boxes, then a mask of the internal nodes
+// whose range crosses a build chunk). Running it again on the same sorted keys with new prim boxes is the
 // refit after a rigid motion (same topology, new boxes).
 constexpr uint32_t kBuildChunk = 512;  // default sorted leaves per build workgroup (256 or 512)
 constexpr int kMaxBlockLevels = 6;

@@ -957,6 +957,7 @@ struct ChunkJobs {
     float4* nodes0; float4* nodes1;
     uint2* range0; uint2* range1;
     float4* level1_0; float4* level1_1;  // boxes of the 64-leaf blocks (first level of the block hierarchy)
+    uint64_t* topmask0; uint64_t* topmask1;  // per 64 internal nodes: which are top nodes
     const K* samp0; const K* samp1;      // key samples (null: none)
     int n0, n1;
     int samp_shift0, samp_shift1;
@@ -977,6 +978,7 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
     float4* __restrict__ nodes = second ? J.nodes1 : J.nodes0;
     uint2* __restrict__ range = second ? J.range1 : J.range0;
     float4* __restrict__ level1 = second ? J.level1_1 : J.level1_0;
+    uint64_t* __restrict__ topmask = second ? J.topmask1 : J.topmask0;
     const int n = second ? J.n1 : J.n0;
     const K* __restrict__ samp = second ? J.samp1 : J.samp0;
     const uint32_t chunk = blk - (second ? J.split : 0u);
@@ -1031,27 +1033,28 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
     const bool top = karras_chunk_node(KS, samp, second ? J.samp_shift1 : J.samp_shift0, jl, skey[t], lo_i, hi_i,
                                        left, right, rope);
     range[jl] = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
-    float q[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (!top) {
-        // the union of leaves [lo_i, hi_i] as disjoint aligned blocks, at
-        // most two per level below 64 and whole 64-leaf blocks above
-        q[0] = q[1] = q[2] = INFINITY;
-        q[3] = q[4] = q[5] = -INFINITY;
-        int L = lo_i - cs, R = hi_i - cs + 1;
-        auto absorb = [&](int k, int m) {
-            const int e = 2 * (block_table_offset(CHUNK, k) + m);
-            const float4 l = sbox[e], h = sbox[e + 1];
-            q[0] = fminf(q[0], l.x); q[1] = fminf(q[1], l.y); q[2] = fminf(q[2], l.z);
-            q[3] = fmaxf(q[3], h.x); q[4] = fmaxf(q[4], h.y); q[5] = fmaxf(q[5], h.z);
-        };
+    // which of the wave's 64 internal nodes are top nodes (lane 0 holds the
+    // first; a wave with none left has no internal node)
+    const uint64_t tops = __ballot(top);
+    if ((t & 63) == 0) topmask[jl >> 6] = tops;
+    // the union of the range's leaves inside the chunk (all of them for an
+    // in-chunk node; k_top_boxes adds a top node's leaves beyond the chunk
+    // edge) as disjoint aligned blocks, at most two per level below 64 and
+    // whole 64-leaf blocks above
+    float q[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    int L = max(lo_i, cs) - cs, R = min(hi_i, ce - 1) - cs + 1;
+    auto absorb = [&](int k, int m) {
+        const int e = 2 * (block_table_offset(CHUNK, k) + m);
+        const float4 l = sbox[e], h = sbox[e + 1];
+        q[0] = fminf(q[0], l.x); q[1] = fminf(q[1], l.y); q[2] = fminf(q[2], l.z);
+        q[3] = fmaxf(q[3], h.x); q[4] = fmaxf(q[4], h.y); q[5] = fmaxf(q[5], h.z);
+    };
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            if (L < R && ((L >> k) & 1)) { absorb(k, L >> k); L += 1 << k; }
-            if (L < R && ((R >> k) & 1)) { absorb(k, (R >> k) - 1); R -= 1 << k; }
-        }
-        for (int m = L >> 6; m < (R >> 6); ++m) absorb(6, m);
+    for (int k = 0; k < 6; ++k) {
+        if (L < R && ((L >> k) & 1)) { absorb(k, L >> k); L += 1 << k; }
+        if (L < R && ((R >> k) & 1)) { absorb(k, (R >> k) - 1); R -= 1 << k; }
     }
-    // a top node gets its box from k_top_boxes
+    for (int m = L >> 6; m < (R >> 6); ++m) absorb(6, m);
     nodes[2 * (size_t)jl] = make_float4(q[0], q[1], q[2], __uint_as_float(left));
     nodes[2 * (size_t)jl + 1] = make_float4(q[3], q[4], q[5], __uint_as_float(rope));
 }
@@ -1082,55 +1085,20 @@ __global__ __launch_bounds__(256) void k_block_boxes(BlockBoxJobs J) {
     }
 }
 
-// wave-cooperative union of the leaf boxes of node's range (at most two
-// partial 64-element windows per level of the block hierarchy). The windows
-// follow from the range alone, so they are listed first and then all read at
-// once: the loads of the levels are independent and in flight together.
-__device__ __forceinline__ void union_range_box(uint32_t lo, uint32_t hi, uint32_t left, uint32_t rope,
-                                                float4* nodes, const BlockLevels& lv, uint32_t node) {
-    uint32_t L = lo, R = hi + 1;
-    const uint32_t lane = lane_id();
-    const float4* wE[2 * kMaxBlockLevels];
-    uint32_t wS[2 * kMaxBlockLevels], wC[2 * kMaxBlockLevels];
-    bool done = false;
-#pragma unroll
-    for (int lvl = 0; lvl < kMaxBlockLevels; ++lvl) {
-        wE[2 * lvl] = wE[2 * lvl + 1] = lv.level[lvl];
-        wS[2 * lvl] = wS[2 * lvl + 1] = 0;
-        wC[2 * lvl] = wC[2 * lvl + 1] = 0;
-        if (done) continue;
-        if (R - L <= 64) {
-            wS[2 * lvl] = L;
-            wC[2 * lvl] = R - L;
-            done = true;
-            continue;
-        }
-        uint32_t Lh = (L + 63u) & ~63u, Rt = R & ~63u;
-        wS[2 * lvl] = L;
-        wC[2 * lvl] = Lh - L;
-        wS[2 * lvl + 1] = Rt;
-        wC[2 * lvl + 1] = R - Rt;
-        L = Lh >> 6;
-        R = Rt >> 6;
-        if (L >= R) done = true;
-    }
-    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-#pragma unroll
-    for (int w = 0; w < 2 * kMaxBlockLevels; ++w)
-        if (lane < wC[w]) box_absorb(wE[w], wS[w] + lane, b);
-    wave_box_reduce(b);
-    if (lane == 0) {
-        nodes[2 * (size_t)node] = make_float4(b[0], b[1], b[2], __uint_as_float(left));
-        nodes[2 * (size_t)node + 1] = make_float4(b[3], b[4], b[5], __uint_as_float(rope));
-    }
-}
+// Boxes of the top nodes. The chunk kernel left in each the box of its
+// range's leaves inside its chunk; the leaves beyond the chunk edge (one side
+// only: the range holds the node's own leaf) come from the block hierarchy as
+// disjoint aligned windows, at most two per level. A wave takes TOP_GROUPS
+// groups of 64 internal nodes (the chunk kernel's masks) and gives each of
+// their top nodes a quarter of the wave, four nodes at a time, so that the
+// nodes' load chains overlap; a quarter issues all its window loads before
+// it reduces. Blocks [0, split) serve tree 0.
+constexpr uint32_t TOP_GROUPS = 4, TOP_LANES = 16;
 
-// One wave per 64 consecutive internal nodes; the top ones among them (range
-// across a chunk boundary) get their boxes one after the other from the whole
-// wave. Blocks [0, split) serve tree 0.
 struct TopBoxJobs {
     BlockLevels lv0, lv1;
     const uint2* range0; const uint2* range1;
+    const uint64_t* topmask0; const uint64_t* topmask1;
     float4* nodes0; float4* nodes1;
     uint32_t n0, n1;
     uint32_t split;
@@ -1143,22 +1111,77 @@ __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J, uint32_t chunk)
     const uint2* __restrict__ range = second ? J.range1 : J.range0;
     float4* nodes = second ? J.nodes1 : J.nodes0;
     const BlockLevels& lv = second ? J.lv1 : J.lv0;  // read in place from the kernel arguments
-    uint32_t base = ((blk - (second ? J.split : 0u)) * 4 + (threadIdx.x >> 6)) * 64;
-    uint32_t i = base + lane_id();
-    uint2 r = make_uint2(0, 0);
-    bool top = false;
-    if (i + 1 < n) {
-        r = range[i];
-        top = r.x / chunk != r.y / chunk;
+    const uint32_t groups = n > 1 ? (n - 1 + 63) / 64 : 0u;  // groups of the n - 1 internal nodes
+    const uint32_t g0 = ((blk - (second ? J.split : 0u)) * 4 + (threadIdx.x >> 6)) * TOP_GROUPS;
+    if (g0 >= groups) return;
+    const uint32_t lane = lane_id(), quarter = lane / TOP_LANES, ql = lane % TOP_LANES;
+    const uint64_t* __restrict__ topmask = second ? J.topmask1 : J.topmask0;
+    const uint64_t own = lane < TOP_GROUPS && g0 + lane < groups ? topmask[g0 + lane] : 0ull;
+    uint64_t m[TOP_GROUPS];
+    uint32_t c[TOP_GROUPS], total = 0;
+#pragma unroll
+    for (uint32_t g = 0; g < TOP_GROUPS; ++g) {
+        m[g] = __shfl(own, (int)g);
+        c[g] = (uint32_t)__popcll(m[g]);
+        total += c[g];
     }
-    uint64_t m = __ballot(top);
-    while (m) {
-        uint32_t b = (uint32_t)__ffsll((long long)m) - 1;
-        m &= m - 1;
-        uint32_t node = base + b;
-        uint32_t L = (uint32_t)__shfl((int)r.x, (int)b), R = (uint32_t)__shfl((int)r.y, (int)b);
-        uint32_t lf = __float_as_uint(nodes[2 * (size_t)node].w), rp = __float_as_uint(nodes[2 * (size_t)node + 1].w);
-        union_range_box(L, R, lf, rp, nodes, lv, node);
+    for (uint32_t k = quarter; k < total; k += 64 / TOP_LANES) {
+        // the k-th top node of the wave's groups
+        uint32_t g = 0, kk = k;
+        uint64_t mm = m[0];
+#pragma unroll
+        for (uint32_t q = 0; q + 1 < TOP_GROUPS; ++q)
+            if (g == q && kk >= c[q]) {
+                kk -= c[q];
+                g = q + 1;
+                mm = m[q + 1];
+            }
+        for (; kk > 0; --kk) mm &= mm - 1;
+        const uint32_t node = (g0 + g) * 64 + (uint32_t)__ffsll((long long)mm) - 1;
+        const uint2 r = range[node];
+        const float4 part_lo = nodes[2 * (size_t)node], part_hi = nodes[2 * (size_t)node + 1];
+        const uint32_t cs = node / chunk * chunk, ce = min(cs + chunk, n);
+        uint32_t L = r.x < cs ? r.x : ce, R = r.x < cs ? cs : r.y + 1;  // the leaves [L, R) beyond the chunk
+        // windows (first element, count) per level and side
+        uint32_t wS[2 * kMaxBlockLevels], wC[2 * kMaxBlockLevels];
+        bool done = false;
+#pragma unroll
+        for (int lvl = 0; lvl < kMaxBlockLevels; ++lvl) {
+            wS[2 * lvl] = wS[2 * lvl + 1] = 0;
+            wC[2 * lvl] = wC[2 * lvl + 1] = 0;
+            if (done) continue;
+            if (R - L <= 64) {
+                wS[2 * lvl] = L;
+                wC[2 * lvl] = R - L;
+                done = true;
+                continue;
+            }
+            const uint32_t Lh = (L + 63u) & ~63u, Rt = R & ~63u;
+            wS[2 * lvl] = L;
+            wC[2 * lvl] = Lh - L;
+            wS[2 * lvl + 1] = Rt;
+            wC[2 * lvl + 1] = R - Rt;
+            L = Lh >> 6;
+            R = Rt >> 6;
+            if (L >= R) done = true;
+        }
+        float b[6] = {part_lo.x, part_lo.y, part_lo.z, part_hi.x, part_hi.y, part_hi.z};
+#pragma unroll
+        for (int w = 0; w < 2 * kMaxBlockLevels; ++w)
+#pragma unroll
+            for (uint32_t e = 0; e < 64; e += TOP_LANES)
+                if (ql + e < wC[w]) box_absorb(lv.level[w / 2], wS[w] + ql + e, b);
+#pragma unroll
+        for (int o = TOP_LANES / 2; o > 0; o >>= 1) {  // within the quarter
+#pragma unroll
+            for (int q = 0; q < 3; ++q) b[q] = fminf(b[q], __shfl_xor(b[q], o));
+#pragma unroll
+            for (int q = 3; q < 6; ++q) b[q] = fmaxf(b[q], __shfl_xor(b[q], o));
+        }
+        if (ql == 0) {  // .w: the left child and the rope, as the chunk kernel wrote them
+            nodes[2 * (size_t)node] = make_float4(b[0], b[1], b[2], part_lo.w);
+            nodes[2 * (size_t)node + 1] = make_float4(b[3], b[4], b[5], part_hi.w);
+        }
     }
 }
 
@@ -1168,11 +1191,15 @@ uint32_t block_level_count(uint32_t n, int level) {
     return (uint32_t)c;
 }
 
-size_t block_levels_elems(uint32_t n) {
+// float4 offset of the top-node masks (one uint64 per 64 internal nodes)
+// after the block levels
+static size_t top_mask_offset(uint32_t n) {
     size_t total = 0;
     for (int k = 1; k < kMaxBlockLevels; ++k) total += 2 * (size_t)block_level_count(n, k);
     return total;
 }
+
+size_t block_levels_elems(uint32_t n) { return top_mask_offset(n) + (block_level_count(n, 1) + 1) / 2; }
 
 template <class K>
 static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, hipStream_t s, hipEvent_t chunk_begin,
@@ -1192,6 +1219,10 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     cj.samp_shift0 = key_sample_shift(n0);
     cj.samp1 = j1 ? j1->samples : nullptr;
     cj.samp_shift1 = key_sample_shift(n1);
+    uint64_t* topmask[2] = {(uint64_t*)(j0->blocks + top_mask_offset(n0)),
+                            (uint64_t*)(o->blocks + top_mask_offset(n1))};
+    cj.topmask0 = topmask[0];
+    cj.topmask1 = topmask[1];
     cj.split = blocks_for(n0, chunk);
     const uint32_t chunk_blocks = cj.split + blocks_for(n1, chunk);
     if (chunk_begin) (void)hipEventRecordWithFlags(chunk_begin, s, event_flags);
@@ -1229,11 +1260,14 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     TopBoxJobs tj;
     tj.lv0 = lv[0]; tj.lv1 = lv[1];
     tj.range0 = j0->range; tj.range1 = o->range;
+    tj.topmask0 = topmask[0]; tj.topmask1 = topmask[1];
     tj.nodes0 = j0->nodes; tj.nodes1 = o->nodes;
     tj.n0 = n0; tj.n1 = n1;
-    // one wave per 64 internal nodes (n - 1 of them); a one-leaf tree has none
-    tj.split = n0 > 1 ? blocks_for(blocks_for(n0 - 1, 64), 4) : 0;
-    uint32_t top_blocks = tj.split + (n1 > 1 ? blocks_for(blocks_for(n1 - 1, 64), 4) : 0);
+    // one wave per TOP_GROUPS groups of 64 internal nodes (n - 1 of them); a
+    // one-leaf tree has none
+    auto top_waves = [](uint32_t nn) { return nn > 1 ? blocks_for(blocks_for(nn - 1, 64), TOP_GROUPS) : 0u; };
+    tj.split = blocks_for(top_waves(n0), 4);
+    uint32_t top_blocks = tj.split + blocks_for(top_waves(n1), 4);
     if (top_blocks) k_top_boxes<<<top_blocks, 256, 0, s>>>(tj, chunk);
     GI_LAUNCH_CHECK();
 }
