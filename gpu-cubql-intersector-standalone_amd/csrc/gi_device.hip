@@ -612,12 +612,28 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_sort_digit_scan(SortPass<K> P)
     if (threadIdx.x == 0) totals[digit] = carry;
 }
 
-// Stable rank + scatter of one tile: keys are ranked in (wave, item, lane)
-// order, their original order, and key i of digit d goes to gbase[d] + the
-// count of d in earlier waves of the tile + its rank in its wave. wc must be
-// zero on entry; gbase must be complete (a barrier follows the first use).
+// A tile's keys and values into registers: wave w holds the contiguous
+// segment [w * 64 * SORT_ITEMS, (w + 1) * 64 * SORT_ITEMS) of the tile, item
+// by item.
 template <class K>
-__device__ __forceinline__ void tile_rank_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+__device__ __forceinline__ void tile_load(const K* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t n,
+                                          uint32_t tile, K (&k)[SORT_ITEMS], uint32_t (&v)[SORT_ITEMS]) {
+    const uint32_t base = tile * SORT_TILE + (threadIdx.x >> 6) * (64 * SORT_ITEMS) + lane_id();
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        const uint32_t i = base + it * 64;
+        k[it] = i < n ? kin[i] : (K)0;
+        v[it] = i < n ? vin[i] : 0u;
+    }
+}
+
+// Stable rank + scatter of one tile (k, v as tile_load reads them): keys are
+// ranked in (wave, item, lane) order, their original order, and key i of
+// digit d goes to gbase[d] + the count of d in earlier waves of the tile +
+// its rank in its wave. wc must be zero on entry; gbase must be complete (a
+// barrier follows the first use).
+template <class K>
+__device__ __forceinline__ void tile_rank_scatter(const K (&k)[SORT_ITEMS], const uint32_t (&v)[SORT_ITEMS],
                                                   K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
                                                   int shift, uint32_t tile, uint32_t (*wc)[RADIX],
                                                   const uint32_t* gbase, K* __restrict__ samp = nullptr,
@@ -625,15 +641,7 @@ __device__ __forceinline__ void tile_rank_scatter(const K* __restrict__ kin, con
     uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     uint32_t base = tile * SORT_TILE + wave * (64 * SORT_ITEMS);
     uint64_t lt = lanemask_lt();
-    K k[SORT_ITEMS];
-    uint32_t v[SORT_ITEMS];
     uint32_t rank[SORT_ITEMS];
-    for (int it = 0; it < SORT_ITEMS; ++it) {
-        uint32_t i = base + it * 64 + lane;
-        bool valid = i < n;
-        k[it] = valid ? kin[i] : (K)0;
-        v[it] = valid ? vin[i] : 0u;
-    }
     for (int it = 0; it < SORT_ITEMS; ++it) {
         uint32_t i = base + it * 64 + lane;
         bool valid = i < n;
@@ -686,6 +694,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(SortPass<K> P) 
     const uint32_t tile = blk - (b ? P.split : 0u);
     __shared__ uint32_t wc[SORT_THREADS / 64][RADIX];
     __shared__ uint32_t gbase[RADIX];
+    // the tile's keys first: their loads overlap those of the digit counts
+    K k[SORT_ITEMS];
+    uint32_t v[SORT_ITEMS];
+    tile_load<K>(kin, vin, n, tile, k, v);
     for (int d = threadIdx.x; d < (SORT_THREADS / 64) * RADIX; d += SORT_THREADS) (&wc[0][0])[d] = 0;
     {
         // digit base = exclusive prefix of the digit totals (RADIX == SORT_THREADS)
@@ -694,7 +706,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(SortPass<K> P) 
         gbase[threadIdx.x] = ex + hist[(size_t)threadIdx.x * ntiles + tile];
     }
     __syncthreads();
-    tile_rank_scatter<K>(kin, vin, kout, vout, n, P.shift, tile, wc, gbase, b ? P.samp_b : P.samp_a,
+    tile_rank_scatter<K>(k, v, kout, vout, n, P.shift, tile, wc, gbase, b ? P.samp_b : P.samp_a,
                          b ? P.samp_shift_b : P.samp_shift_a);
 }
 
