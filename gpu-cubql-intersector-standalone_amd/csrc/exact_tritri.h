@@ -78,6 +78,9 @@ struct ExactTT {
     };
 
     // ------------------------------------------------------------ vectors
+    // Component k of a (run-time k) by selects: an array indexed at run time
+    // would be kept in per-lane scratch memory on the GPU.
+    static GI_HD C comp(const V3& a, int k) { return k == 0 ? a.v[0] : (k == 1 ? a.v[1] : a.v[2]); }
     static GI_HD V3 sub(const V3& a, const V3& b) {
         V3 r;
         for (int k = 0; k < 3; ++k) r.v[k] = a.v[k] - b.v[k];
@@ -105,13 +108,13 @@ struct ExactTT {
     }
     // axis to drop when projecting onto the plane of normal n
     static GI_HD int dropAxis(const V3& n) {
-        int k = 0;
-        if (P::cmpAbsC(n.v[1], n.v[k]) > 0) k = 1;
-        if (P::cmpAbsC(n.v[2], n.v[k]) > 0) k = 2;
+        int k = P::cmpAbsC(n.v[1], n.v[0]) > 0 ? 1 : 0;
+        if (P::cmpAbsC(n.v[2], comp(n, k)) > 0) k = 2;
         return k;
     }
     static GI_HD C orient2d(const V3& p, const V3& q, const V3& r, int ia, int ib) {
-        return (q.v[ia] - p.v[ia]) * (r.v[ib] - p.v[ib]) - (q.v[ib] - p.v[ib]) * (r.v[ia] - p.v[ia]);
+        const C pa = comp(p, ia), pb = comp(p, ib);
+        return (comp(q, ia) - pa) * (comp(r, ib) - pb) - (comp(q, ib) - pb) * (comp(r, ia) - pa);
     }
     static GI_HD int ratCmp(const Rat& a, const Rat& b) {
         B l = P::mulCC(a.n, b.d), r = P::mulCC(b.n, a.d);
@@ -142,8 +145,9 @@ struct ExactTT {
         }
         return r;
     }
+    static GI_HD B hcomp(const HP& a, int k) { return k == 0 ? a.h[0] : (k == 1 ? a.h[1] : a.h[2]); }
     static GI_HD int hpCmpAxis(const HP& a, const HP& b, int k) {
-        return P::cmpH(P::mulBB(a.h[k], b.w), P::mulBB(b.h[k], a.w));
+        return P::cmpH(P::mulBB(hcomp(a, k), b.w), P::mulBB(hcomp(b, k), a.w));
     }
 
     // Running extremes of points along axis k.
@@ -165,6 +169,7 @@ struct ExactTT {
         int sigma = P::sgnC(orient2d(T[0], T[1], T[2], ia, ib));
         t0.n = C(0); t0.d = C(1);
         t1.n = C(1); t1.d = C(1);
+#pragma unroll
         for (int e = 0; e < 3; ++e) {
             const V3& u = T[e];
             const V3& v = T[e == 2 ? 0 : e + 1];
@@ -263,15 +268,14 @@ struct ExactTT {
         V3 c = cross(pq, st);
         if (isZero(c)) {
             if (!isZero(cross(pq, sub(s, p)))) return 0;  // parallel, distinct lines
-            int k = 0;
-            while (P::sgnC(pq.v[k]) == 0) ++k;
-            const V3& lo1 = P::sgnC(p.v[k] - q.v[k]) <= 0 ? p : q;
-            const V3& hi1 = P::sgnC(p.v[k] - q.v[k]) <= 0 ? q : p;
-            const V3& lo2 = P::sgnC(s.v[k] - t.v[k]) <= 0 ? s : t;
-            const V3& hi2 = P::sgnC(s.v[k] - t.v[k]) <= 0 ? t : s;
-            const V3& lo = P::sgnC(lo1.v[k] - lo2.v[k]) >= 0 ? lo1 : lo2;
-            const V3& hi = P::sgnC(hi1.v[k] - hi2.v[k]) <= 0 ? hi1 : hi2;
-            if (P::sgnC(lo.v[k] - hi.v[k]) > 0) return 0;
+            const int k = P::sgnC(pq.v[0]) != 0 ? 0 : (P::sgnC(pq.v[1]) != 0 ? 1 : 2);
+            const V3& lo1 = P::sgnC(comp(p, k) - comp(q, k)) <= 0 ? p : q;
+            const V3& hi1 = P::sgnC(comp(p, k) - comp(q, k)) <= 0 ? q : p;
+            const V3& lo2 = P::sgnC(comp(s, k) - comp(t, k)) <= 0 ? s : t;
+            const V3& hi2 = P::sgnC(comp(s, k) - comp(t, k)) <= 0 ? t : s;
+            const V3& lo = P::sgnC(comp(lo1, k) - comp(lo2, k)) >= 0 ? lo1 : lo2;
+            const V3& hi = P::sgnC(comp(hi1, k) - comp(hi2, k)) <= 0 ? hi1 : hi2;
+            if (P::sgnC(comp(lo, k) - comp(hi, k)) > 0) return 0;
             r.p = hpInt(lo); r.q = hpInt(hi); r.nseg = 1;
             return 1;
         }
@@ -353,9 +357,8 @@ struct ExactTT {
         dir[0] = P::mulCC(nA.v[1], nB.v[2]) - P::mulCC(nA.v[2], nB.v[1]);
         dir[1] = P::mulCC(nA.v[2], nB.v[0]) - P::mulCC(nA.v[0], nB.v[2]);
         dir[2] = P::mulCC(nA.v[0], nB.v[1]) - P::mulCC(nA.v[1], nB.v[0]);
-        int k = 0;
-        if (P::cmpAbsB(dir[1], dir[k]) > 0) k = 1;
-        if (P::cmpAbsB(dir[2], dir[k]) > 0) k = 2;
+        int k = P::cmpAbsB(dir[1], dir[0]) > 0 ? 1 : 0;
+        if (P::cmpAbsB(dir[2], k ? dir[1] : dir[0]) > 0) k = 2;
         Span span;
         span.n = 0;
         edgesAgainst(A, Bt, nB, oA, sA, k, span);  // edges of A against triangle B
