@@ -541,7 +541,7 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
 
 void ensure_output_bufs(gi_context* ctx, const gi_options& o) {
     const uint32_t cap = std::min(output_small_max(), ctx->cap_rec);
-    ctx->tkeys.ensure<uint64_t>(std::max(cap, 1u));  // sorted runs before the merge
+    ctx->tkeys.ensure<uint64_t>(output_small_runs_elems());  // sorted runs of the small output
     ctx->pairs.ensure<uint2>(std::max(cap, 1u));
     ctx->order.ensure<uint32_t>(std::max(cap, 1u));
     ctx->offs.ensure<uint32_t>((size_t)cap + 1);

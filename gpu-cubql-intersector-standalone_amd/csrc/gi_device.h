@@ -182,6 +182,8 @@ void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
 // queued before the counters reach the host. Does nothing unless
 // output_small_fits(R, key_bits) and nothing overflowed.
 uint32_t output_small_max();
+// runs: the sorted runs, their segment-count prefixes and totals (uint64 elements)
+size_t output_small_runs_elems();
 bool output_small_fits(uint32_t R, int key_bits);
 void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const uint32_t* counters, uint32_t cap_rec,
                          uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint64_t* runs, uint2* pairs,

@@ -1711,17 +1711,23 @@ void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const u
 //   k_output_runs   kRuns workgroups, each an LSD radix sort of kRunLen
 //                   records in LDS (ranking as in k_sort_small: stable,
 //                   wave ballot match, per-wave digit counts);
-//   k_output_merge  one workgroup merges the sorted runs pairwise (merge
-//                   path, in LDS), then writes per sorted position the pair,
-//                   the record index (order) and the exclusive scan of the
-//                   segment counts (offsets, total at offs[R]).
+//                   and, per run position, the exclusive prefix of the
+//                   segment counts in run order (cpre, run totals after it);
+//   k_output_place  every record's place is its position in its run plus,
+//                   for each other run, how many of that run's keys are
+//                   smaller (binary searches in LDS, the runs' searches
+//                   interleaved), and its segment offset the sum of the
+//                   runs' count prefixes at those positions: one thread per
+//                   record, 16 workgroups (a single merging workgroup took
+//                   23 us for 10^4 records); writes per place the pair, the
+//                   record index (order) and the offset (total at offs[R]).
 // Each key (faceA << bitsB | faceB) carries its record index in the low
 // idx_bits, so the passes move no separate values (keys are distinct).
 constexpr int RUN_THREADS = 1024, RUN_ITEMS = 2;
 constexpr uint32_t kRunLen = RUN_THREADS * RUN_ITEMS;
 constexpr uint32_t kRuns = 8;
 constexpr uint32_t kOutSmallMax = kRunLen * kRuns;
-constexpr int MERGE_THREADS = 1024, MERGE_ITEMS = (int)(kOutSmallMax / MERGE_THREADS);
+constexpr int PLACE_THREADS = 1024;
 
 __host__ __device__ __forceinline__ int bits_of_count(uint32_t n) { return n <= 1 ? 1 : 32 - __builtin_clz(n - 1); }
 
@@ -1734,9 +1740,11 @@ __device__ __forceinline__ bool output_small_ok(const uint32_t* counters, uint32
 }
 
 __global__ __launch_bounds__(RUN_THREADS) void k_output_runs(const uint64_t* __restrict__ rec_key,
+                                                             const uint2* __restrict__ rec_seg,
                                                              const uint32_t* __restrict__ counters, uint32_t cap_rec,
-                                                             uint32_t cap_seg, int key_bits,
-                                                             uint64_t* __restrict__ runs) {
+                                                             uint32_t cap_seg, int key_bits, int want_segments,
+                                                             uint64_t* __restrict__ runs,
+                                                             uint32_t* __restrict__ cpre) {
     uint32_t R;
     if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R)) return;
     const uint32_t r0 = blockIdx.x * kRunLen;
@@ -1814,75 +1822,21 @@ __global__ __launch_bounds__(RUN_THREADS) void k_output_runs(const uint64_t* __r
 #pragma unroll
     for (int it = 0; it < RUN_ITEMS; ++it)
         if (seg + 64 * it < n) runs[r0 + seg + 64 * it] = key[it];
-}
-
-// LDS index with one pad slot per 32 keys: a thread's MERGE_ITEMS contiguous
-// keys then start in different banks across the wave
-__device__ __forceinline__ uint32_t skewed(uint32_t i) { return i + (i >> 5); }
-
-__global__ __launch_bounds__(MERGE_THREADS) void k_output_merge(const uint64_t* __restrict__ runs,
-                                                                const uint2* __restrict__ rec_seg,
-                                                                const uint32_t* __restrict__ counters, uint32_t cap_rec,
-                                                                uint32_t cap_seg, int key_bits, int bitsB,
-                                                                int want_segments, uint2* __restrict__ pairs,
-                                                                uint32_t* __restrict__ order,
-                                                                uint32_t* __restrict__ offs) {
-    uint32_t R;
-    if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R)) return;
-    const int idx_bits = bits_of_count(R);
-    __shared__ uint64_t sk[kOutSmallMax + kOutSmallMax / 32];
-    __shared__ uint8_t local_off[kOutSmallMax];
-    __shared__ uint32_t thread_base[MERGE_THREADS];
-    __shared__ uint32_t wsum[MERGE_THREADS / 64];
-    const uint32_t t = threadIdx.x, lane = lane_id(), wave = threadIdx.x >> 6;
-    for (uint32_t i = t; i < R; i += MERGE_THREADS) sk[skewed(i)] = runs[i];
-    __syncthreads();
-    const uint32_t o0 = t * MERGE_ITEMS;  // this thread's output positions o0 .. o0 + MERGE_ITEMS - 1
-    uint64_t v[MERGE_ITEMS];
-    for (uint32_t L = kRunLen; L < R; L <<= 1) {  // 2L is a multiple of MERGE_ITEMS: one pair per thread
-        if (o0 < R) {
-            const uint32_t p0 = o0 / (2 * L) * (2 * L);
-            const uint32_t a0 = p0, na = min(p0 + L, R) - p0;
-            const uint32_t b0 = a0 + na, nb = min(p0 + 2 * L, R) - b0;
-            const uint32_t k = o0 - p0;
-            // merge path: the first k merged keys take i from run A, k - i from run B
-            uint32_t lo = k > nb ? k - nb : 0u, hi = min(k, na);
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (sk[skewed(a0 + mid)] <= sk[skewed(b0 + k - 1 - mid)]) lo = mid + 1;
-                else hi = mid;
-            }
-            uint32_t i = lo, j = k - lo;
-#pragma unroll
-            for (int e = 0; e < MERGE_ITEMS; ++e) {
-                const uint64_t ka = i < na ? sk[skewed(a0 + i)] : ~0ull;
-                const uint64_t kb = j < nb ? sk[skewed(b0 + j)] : ~0ull;
-                const bool take_a = i < na && (j >= nb || ka <= kb);
-                v[e] = take_a ? ka : kb;
-                i += take_a ? 1u : 0u;
-                j += take_a ? 0u : 1u;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < MERGE_ITEMS; ++e)
-            if (o0 + e < R) sk[skewed(o0 + e)] = v[e];
-        __syncthreads();
-    }
-    // Segment offsets: the exclusive scan of the counts in sorted order. Each
-    // thread scans its MERGE_ITEMS contiguous positions (local offsets, at
-    // most MERGE_ITEMS * 6 < 256, kept as bytes), the workgroup scans the
-    // thread sums; the outputs are then written with consecutive lanes on
-    // consecutive positions (a thread's contiguous positions would make every
-    // store instruction touch 64 cache lines).
+    // segment counts in run order, then their exclusive prefix: thread t
+    // scans positions RUN_ITEMS t .. RUN_ITEMS t + RUN_ITEMS - 1
+    __shared__ uint32_t cnt[kRunLen];
+    __shared__ uint32_t wsum[RUN_THREADS / 64];
     const uint64_t idx_mask = (1ull << idx_bits) - 1ull;
-    uint32_t sum = 0;
 #pragma unroll
-    for (int e = 0; e < MERGE_ITEMS; ++e) {
-        const uint32_t i = o0 + e;
-        const uint32_t c = (i < R && want_segments) ? rec_seg[(uint32_t)(sk[skewed(i)] & idx_mask)].y : 0u;
-        local_off[i] = (uint8_t)sum;
-        sum += c;
+    for (int it = 0; it < RUN_ITEMS; ++it)
+        if (seg + 64 * it < n) cnt[seg + 64 * it] = want_segments ? rec_seg[(uint32_t)(key[it] & idx_mask)].y : 0u;
+    __syncthreads();
+    const uint32_t q0 = threadIdx.x * RUN_ITEMS;
+    uint32_t c[RUN_ITEMS], sum = 0;
+#pragma unroll
+    for (int e = 0; e < RUN_ITEMS; ++e) {
+        c[e] = q0 + e < n ? cnt[q0 + e] : 0u;
+        sum += c[e];
     }
     uint32_t x = sum;
     for (int o = 1; o < 64; o <<= 1) {
@@ -1892,24 +1846,62 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_output_merge(const uint64_t* 
     if (lane == 63) wsum[wave] = x;
     __syncthreads();
     uint32_t before = x - sum, all = 0;
-    for (uint32_t w = 0; w < MERGE_THREADS / 64; ++w) {
+    for (uint32_t w = 0; w < RUN_THREADS / 64; ++w) {
         before += w < wave ? wsum[w] : 0u;
         all += wsum[w];
     }
-    thread_base[t] = before;
-    __syncthreads();
-    const uint64_t maskB = (1ull << bitsB) - 1ull;
-    for (uint32_t i = t; i < R; i += MERGE_THREADS) {
-        const uint64_t key = sk[skewed(i)];
-        const uint64_t pk = key >> idx_bits;
-        pairs[i] = make_uint2((uint32_t)(pk >> bitsB), (uint32_t)(pk & maskB));
-        order[i] = (uint32_t)(key & idx_mask);
-        offs[i] = thread_base[i / MERGE_ITEMS] + local_off[i];
+#pragma unroll
+    for (int e = 0; e < RUN_ITEMS; ++e) {
+        if (q0 + e < n) cpre[r0 + q0 + e] = before;
+        before += c[e];
     }
-    if (t == 0) offs[R] = all;
+    if (threadIdx.x == 0) cpre[kOutSmallMax + blockIdx.x] = all;
 }
 
-// segments of the sorted records, for the records k_output_merge ordered
+__global__ __launch_bounds__(PLACE_THREADS) void k_output_place(const uint64_t* __restrict__ runs,
+                                                                const uint32_t* __restrict__ cpre,
+                                                                const uint32_t* __restrict__ counters, uint32_t cap_rec,
+                                                                uint32_t cap_seg, int key_bits, int bitsB,
+                                                                uint2* __restrict__ pairs,
+                                                                uint32_t* __restrict__ order,
+                                                                uint32_t* __restrict__ offs) {
+    uint32_t R;
+    if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R)) return;
+    const uint32_t g = blockIdx.x * PLACE_THREADS + threadIdx.x;  // this thread's record, in run order
+    if (blockIdx.x * PLACE_THREADS >= R) return;
+    const int idx_bits = bits_of_count(R);
+    __shared__ uint64_t sk[kOutSmallMax];
+    for (uint32_t i = threadIdx.x; i < R; i += PLACE_THREADS) sk[i] = runs[i];
+    __syncthreads();
+    if (g >= R) return;
+    const uint64_t x = sk[g];
+    const uint32_t own = g / kRunLen;
+    uint32_t pos = 0, off = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kRuns; ++r) {
+        const uint32_t r0 = r * kRunLen, len = r0 < R ? min(kRunLen, R - r0) : 0u;
+        // keys of run r below x (keys are distinct: each carries its record index)
+        uint32_t p = 0;
+#pragma unroll
+        for (uint32_t step = kRunLen; step > 0; step >>= 1)
+            if (p + step <= len && sk[r0 + p + step - 1] < x) p += step;
+        p = r == own ? g - r0 : p;
+        pos += p;
+        off += p < len ? cpre[r0 + p] : (len ? cpre[kOutSmallMax + r] : 0u);
+    }
+    const uint64_t pk = x >> idx_bits;
+    const uint64_t maskB = (1ull << bitsB) - 1ull;
+    pairs[pos] = make_uint2((uint32_t)(pk >> bitsB), (uint32_t)(pk & maskB));
+    order[pos] = (uint32_t)(x & ((1ull << idx_bits) - 1ull));
+    offs[pos] = off;
+    if (g == 0) {
+        uint32_t all = 0;
+        for (uint32_t r = 0; r * kRunLen < R; ++r) all += cpre[kOutSmallMax + r];
+        offs[R] = all;
+    }
+}
+
+// segments of the sorted records, for the records k_output_place ordered
 __global__ __launch_bounds__(256) void k_gather_segments_small(const uint32_t* __restrict__ order,
                                                                const uint2* __restrict__ rec_seg,
                                                                const uint32_t* __restrict__ seg_offsets,
@@ -1925,15 +1917,19 @@ __global__ __launch_bounds__(256) void k_gather_segments_small(const uint32_t* _
 
 uint32_t output_small_max() { return kOutSmallMax; }
 
+size_t output_small_runs_elems() { return kOutSmallMax + (kOutSmallMax + kRuns + 1) / 2; }
+
 bool output_small_fits(uint32_t R, int key_bits) { return R <= kOutSmallMax && key_bits + bits_of_count(R) <= 64; }
 
 void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const uint32_t* counters, uint32_t cap_rec,
                          uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint64_t* runs, uint2* pairs,
                          uint32_t* order, uint32_t* offs, __int128* seg_out, double* seg_f64, uint32_t out_slots,
                          const __int128* seg_pool, hipStream_t s) {
-    k_output_runs<<<kRuns, RUN_THREADS, 0, s>>>(rec_key, counters, cap_rec, cap_seg, key_bits, runs);
-    k_output_merge<<<1, MERGE_THREADS, 0, s>>>(runs, rec_seg, counters, cap_rec, cap_seg, key_bits, bitsB,
-                                               want_segments ? 1 : 0, pairs, order, offs);
+    uint32_t* cpre = (uint32_t*)(runs + kOutSmallMax);
+    k_output_runs<<<kRuns, RUN_THREADS, 0, s>>>(rec_key, rec_seg, counters, cap_rec, cap_seg, key_bits,
+                                                want_segments ? 1 : 0, runs, cpre);
+    k_output_place<<<kOutSmallMax / PLACE_THREADS, PLACE_THREADS, 0, s>>>(runs, cpre, counters, cap_rec, cap_seg,
+                                                                        key_bits, bitsB, pairs, order, offs);
     if (want_segments)
         k_gather_segments_small<<<kOutSmallMax / 256, 256, 0, s>>>(order, rec_seg, offs, counters, cap_rec, cap_seg,
                                                                    key_bits, out_slots, seg_pool, seg_out, seg_f64);
