@@ -295,8 +295,9 @@ def main():
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the library runs a query on "
+              "one device (several GPUs would each take a slice of B's Morton-ordered leaves against a "
+              "local copy of A's tree)", file=sys.stderr)
         sys.exit(2)
 
     gi = ge.package_module("gi_ctypes")
