@@ -279,9 +279,9 @@ void stage_forest(gi_context* ctx, int slot) {
     m.range.ensure<uint2>(nt);
     m.blocks.ensure<float4>(block_levels_elems((uint32_t)nt));
     m.roots.ensure<uint2>(M);
-    uint64_t* keys = m.keys.ensure<uint64_t>(nt);
+    uint32_t* keys = m.keys.ensure<uint32_t>(nt);
     uint32_t* vals = m.vals.ensure<uint32_t>(nt);
-    uint64_t* tk = m.tkeys.ensure<uint64_t>(nt);
+    uint32_t* tk = m.tkeys.ensure<uint32_t>(nt);
     uint32_t* tv = m.tvals.ensure<uint32_t>(nt);
     uint32_t* bounds = m.fbounds.ensure<uint32_t>(6 * (size_t)M);
     float* partials = ctx->bpart[slot].ensure<float>(6 * (size_t)std::max(n_vtiles, kPrepareMaxBlocks));
@@ -289,12 +289,18 @@ void stage_forest(gi_context* ctx, int slot) {
                           m.vtile0.get<uint32_t>(), M, v64, vi, grid + slot, partials, bounds, s);
     uint32_t largest = 0;
     for (uint32_t i = 0; i < M; ++i) largest = std::max(largest, m.h_toff[i + 1] - m.h_toff[i]);
-    m.morton_axis = morton_axis_bits(largest);
+    // mesh id and Morton code within 32 bits (32-bit keys; four 8-bit sort
+    // passes for 4,096 meshes, at 6 bits per axis: the batch's meshes are
+    // small, and the candidate set does not depend on the tree)
+    m.morton_axis = std::min(morton_axis_bits(largest), (32 - bits_for(M)) / 3);  // >= 1: gi_set_batch
     launch_tri_boxes_forest(m.tri.get<uint32_t>(), m.ttiles.get<uint2>(), n_ttiles, m.toff.get<uint32_t>(), v64, vi,
                             grid + slot, bounds, m.morton_axis, tbox, keys, vals, s);
     uint32_t* hist = ctx->bhist[slot].ensure<uint32_t>(sort_hist_elems(nt));
     uint32_t* stmp = ctx->bscan[slot].ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
-    m.sorted_in_tmp = radix_sort_u64(keys, vals, tk, tv, m.nt, 3 * m.morton_axis + bits_for(M), hist, stmp, s);
+    uint32_t* samp = radix_sort_takes_hist0(m.nt) ? m.samples.ensure<uint32_t>(key_sample_count(m.nt)) : nullptr;
+    // the sort runs after the join, together with the other slot's
+    ctx->pending_sort[slot] =
+        SortJobU32{keys, vals, tk, tv, m.nt, 3 * m.morton_axis + bits_for(M), false, hist, stmp, samp};
 }
 
 // The part of a mesh's build that runs on the slot's own stream: vertices
@@ -355,9 +361,7 @@ BuildJob<K> slot_job(MeshSlot& m) {
     j.range = m.range.get<uint2>();
     j.blocks = m.blocks.get<float4>();
     // the samples the last topology sort wrote stay valid for a refit (same keys)
-    j.samples = nullptr;
-    if constexpr (sizeof(K) == 4)
-        if (!m.forest && radix_sort_takes_hist0(m.nt)) j.samples = m.samples.get<K>();
+    j.samples = radix_sort_takes_hist0(m.nt) ? m.samples.get<K>() : nullptr;
     return j;
 }
 
@@ -398,21 +402,15 @@ void build_meshes(gi_context* ctx, bool rebuild, bool timed = true, bool zero_co
         radix_sort_u32_pair(jobs, nj, ctx->stream, in_tmp);
         for (int j = 0; j < nj; ++j) ctx->mesh[slot_of[j]].sorted_in_tmp = in_tmp[j];
     }
-    BuildJob<uint32_t> j32[2];
-    BuildJob<uint64_t> j64[2];
-    int n32 = 0, n64 = 0;
-    for (int sl = 0; sl < 2; ++sl) {
-        if (!need[sl]) continue;
-        MeshSlot& m = ctx->mesh[sl];
-        if (m.forest) j64[n64++] = slot_job<uint64_t>(m);
-        else j32[n32++] = slot_job<uint32_t>(m);
-    }
+    BuildJob<uint32_t> jobs[2];
+    int nj = 0;
+    for (int sl = 0; sl < 2; ++sl)
+        if (need[sl]) jobs[nj++] = slot_job<uint32_t>(ctx->mesh[sl]);
     hipStream_t s = ctx->stream;
-    auto run = [&](auto* jobs, int nj) {
-        if (nj == 0) return;
-        uint32_t leaves = 0;
-        for (int j = 0; j < nj; ++j) leaves += jobs[j].n;
-        bool time_it = timed && !ctx->chunk_timed && leaves > 0;
+    uint32_t leaves = 0;
+    for (int j = 0; j < nj; ++j) leaves += jobs[j].n;
+    if (nj) {
+        const bool time_it = timed && leaves > 0;
         launch_build_bvh(jobs, nj, (uint32_t)ctx->tune.build_chunk, s, time_it ? ctx->ev_chunk[0] : nullptr,
                          time_it ? ctx->ev_chunk[1] : nullptr,
                          ctx->capturing ? hipEventRecordExternal : hipEventRecordDefault);
@@ -420,9 +418,7 @@ void build_meshes(gi_context* ctx, bool rebuild, bool timed = true, bool zero_co
             ctx->chunk_timed = true;
             ctx->chunk_leaves = leaves;
         }
-    };
-    run(j32, n32);
-    run(j64, n64);
+    }
     for (int sl = 0; sl < 2; ++sl) {
         if (!need[sl]) continue;
         MeshSlot& m = ctx->mesh[sl];
@@ -516,7 +512,7 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
     MeshDev A = mesh_dev(MA), B = mesh_dev(MB);
     int bitsB = bits_for(MB.nt);
     if (timed) record(ctx, ctx->ev[6], s);
-    const uint64_t* keysB = M ? (MB.sorted_in_tmp ? MB.tkeys.get<uint64_t>() : MB.keys.get<uint64_t>()) : nullptr;
+    const uint32_t* keysB = M ? (MB.sorted_in_tmp ? MB.tkeys.get<uint32_t>() : MB.keys.get<uint32_t>()) : nullptr;
     launch_traverse(A, B, keysB, 3 * MB.morton_axis, M ? MA.roots.get<uint2>() : nullptr, ctx->cand.get<uint2>(),
                     ctx->cap_cand, counters, ctx->tune.traverse_variant, s);
     if (timed) record(ctx, ctx->ev[2], s);
@@ -1323,9 +1319,11 @@ int gi_set_batch(gi_context* ctx, const gi_mesh_pair* pairs, uint32_t n_pairs) {
         }
     }
     // 2^30 triangles per side: node indices and the status bits of the
-    // candidate lists; 2^32 vertices: global indices
+    // candidate lists; 2^32 vertices: global indices; 2^29 pairs: the mesh id
+    // leaves at least one Morton bit per axis in the 32-bit BVH keys
     for (int sd = 0; sd < 2; ++sd)
         if (nt[sd] >= (1ull << 30) || nv[sd] >= (1ull << 32)) return GI_ERR_TOO_LARGE;
+    if (n_pairs > (1u << 29)) return GI_ERR_TOO_LARGE;
     try {
         HIP_TRY(hipSetDevice(ctx->device));
         sync_all(ctx);

@@ -80,10 +80,11 @@ constexpr uint32_t kForestTile = 1024;
 void launch_prepare_forest(const double* src, const uint2* vtiles, uint32_t n_vtiles, const uint32_t* voff,
                            const uint32_t* mesh_vtile0, uint32_t M, double4* v64, int4* vi, uint32_t* grid_flag,
                            float* partials, uint32_t* bounds, hipStream_t s);
-// boxes and keys mesh << 3 axis_bits | Morton code (sort on 3 axis_bits + bits(M) bits)
+// boxes and keys mesh << 3 axis_bits | Morton code (3 axis_bits + bits(M) <= 32;
+// sort on that many bits)
 void launch_tri_boxes_forest(const uint32_t* tri, const uint2* ttiles, uint32_t n_ttiles, const uint32_t* toff,
                              const double4* v64, const int4* vi, const uint32_t* grid_flag, const uint32_t* bounds,
-                             int axis_bits, float4* box, uint64_t* keys, uint32_t* vals, hipStream_t s);
+                             int axis_bits, float4* box, uint32_t* keys, uint32_t* vals, hipStream_t s);
 
 // LSD radix sort of (key, value) pairs, 8 bits per pass. tmp_* are ping-pong
 // buffers of n elements; hist needs sort_hist_elems(n) entries, scan_tmp
@@ -135,8 +136,8 @@ struct BlockLevels {
     const float4* level[kMaxBlockLevels];
 };
 size_t block_levels_elems(uint32_t n);
-// One tree to build (keys: 32-bit Morton codes, or 64-bit forest keys whose
-// mesh ids lead, so that each mesh's BVH is a subtree of the forest).
+// One tree to build (keys: Morton codes, or forest keys whose mesh ids lead,
+// so that each mesh's BVH is a subtree of the forest).
 template <class K>
 struct BuildJob {
     const K* keys;
@@ -154,8 +155,6 @@ struct BuildJob {
 // inside a graph capture, so that a replay times the launch).
 void launch_build_bvh(const BuildJob<uint32_t>* jobs, int njobs, uint32_t chunk, hipStream_t s,
                       hipEvent_t chunk_begin, hipEvent_t chunk_end, unsigned event_flags);
-void launch_build_bvh(const BuildJob<uint64_t>* jobs, int njobs, uint32_t chunk, hipStream_t s,
-                      hipEvent_t chunk_begin, hipEvent_t chunk_end, unsigned event_flags);
 // roots[m] = (subtree root of mesh m, rope leaving it)
 void launch_forest_roots(const uint2* range, const float4* nodes, uint32_t n, const uint32_t* toff, uint32_t M,
                          uint2* roots, hipStream_t s);
@@ -164,7 +163,7 @@ void launch_forest_roots(const uint2* range, const float4* nodes, uint32_t n, co
 // starts at bit mesh_shift, A mesh roots)
 // variant: launch shape of the traversal, 0..3 = (workgroup size, candidate
 // pairs staged per wave) (64, 256), (128, 256), (256, 256), (256, 128)
-void launch_traverse(MeshDev A, MeshDev B, const uint64_t* keysB, int mesh_shift, const uint2* rootsA, uint2* cand,
+void launch_traverse(MeshDev A, MeshDev B, const uint32_t* keysB, int mesh_shift, const uint2* rootsA, uint2* cand,
                      uint32_t cap, uint32_t* counters, int variant, hipStream_t s);
 // blocks: workgroups of the grid-stride launch
 void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const uint32_t* grid_flags,

@@ -382,9 +382,9 @@ __global__ __launch_bounds__(256) void k_forest_bounds(const float* __restrict__
         for (int k = 0; k < 6; ++k) bounds[6 * (size_t)m + k] = f2key(b[k]);
 }
 
-// Triangle boxes and 64-bit keys (mesh id << 3 axis_bits | Morton code in the mesh's
-// own frame): one radix sort then groups the batch by mesh and orders each
-// mesh along its Morton curve.
+// Triangle boxes and keys (mesh id << 3 axis_bits | Morton code in the mesh's
+// own frame, within 32 bits): one radix sort then groups the batch by mesh and
+// orders each mesh along its Morton curve.
 __global__ __launch_bounds__(256) void k_tri_boxes_forest(const uint32_t* __restrict__ tri,
                                                           const uint2* __restrict__ tiles,
                                                           const uint32_t* __restrict__ toff,
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(256) void k_tri_boxes_forest(const uint32_t* __rest
                                                           const uint32_t* __restrict__ grid_flag,
                                                           const uint32_t* __restrict__ bounds, int axis_bits,
                                                           float4* __restrict__ box,
-                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint2 t = tiles[blockIdx.x];
     const uint32_t end = min(t.y + kForestTile, toff[t.x + 1]);
     const bool grid = *grid_flag != 0;
@@ -403,7 +403,7 @@ __global__ __launch_bounds__(256) void k_tri_boxes_forest(const uint32_t* __rest
         tri_box(tri, i, grid, v, vi, bl, bh);
         box[2 * (size_t)i] = make_float4(bl[0], bl[1], bl[2], 0.f);
         box[2 * (size_t)i + 1] = make_float4(bh[0], bh[1], bh[2], 0.f);
-        keys[i] = ((uint64_t)t.x << (3 * axis_bits)) | morton_code(bl, bh, bmin, bext, axis_bits);
+        keys[i] = (t.x << (3 * axis_bits)) | morton_code(bl, bh, bmin, bext, axis_bits);
         vals[i] = i;
     }
 }
@@ -420,7 +420,7 @@ void launch_prepare_forest(const double* src, const uint2* vtiles, uint32_t n_vt
 
 void launch_tri_boxes_forest(const uint32_t* tri, const uint2* ttiles, uint32_t n_ttiles, const uint32_t* toff,
                              const double4* v64, const int4* vi, const uint32_t* grid_flag, const uint32_t* bounds,
-                             int axis_bits, float4* box, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+                             int axis_bits, float4* box, uint32_t* keys, uint32_t* vals, hipStream_t s) {
     if (n_ttiles == 0) return;
     k_tri_boxes_forest<<<n_ttiles, 256, 0, s>>>(tri, ttiles, toff, v64, vi, grid_flag, bounds, axis_bits, box, keys,
                                                 vals);
@@ -1288,10 +1288,6 @@ void launch_build_bvh(const BuildJob<uint32_t>* jobs, int njobs, uint32_t chunk,
                       hipEvent_t chunk_begin, hipEvent_t chunk_end, unsigned event_flags) {
     build_bvh_impl<uint32_t>(jobs, njobs, chunk, s, chunk_begin, chunk_end, event_flags);
 }
-void launch_build_bvh(const BuildJob<uint64_t>* jobs, int njobs, uint32_t chunk, hipStream_t s,
-                      hipEvent_t chunk_begin, hipEvent_t chunk_end, unsigned event_flags) {
-    build_bvh_impl<uint64_t>(jobs, njobs, chunk, s, chunk_begin, chunk_end, event_flags);
-}
 
 // Subtree of each forest mesh: the node whose leaf range is exactly the
 // mesh's [toff[m], toff[m+1]) (in Karras' numbering internal node first or
@@ -1348,7 +1344,7 @@ __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t fill, uint2* 
 template <int TRAV_THREADS, uint32_t TRAV_STAGE>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restrict__ nodesA, uint32_t nA,
                                                            const float4* __restrict__ nodesB, uint32_t nB,
-                                                           const uint64_t* __restrict__ keysB, int mesh_shift,
+                                                           const uint32_t* __restrict__ keysB, int mesh_shift,
                                                            const uint2* __restrict__ rootsA,
                                                            uint2* __restrict__ cand, uint32_t cap,
                                                            uint32_t* __restrict__ counters) {
@@ -1401,13 +1397,13 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
 }
 
 template <int T, uint32_t S>
-static void traverse_as(MeshDev A, MeshDev B, const uint64_t* keysB, int mesh_shift, const uint2* rootsA,
+static void traverse_as(MeshDev A, MeshDev B, const uint32_t* keysB, int mesh_shift, const uint2* rootsA,
                         uint2* cand, uint32_t cap, uint32_t* counters, hipStream_t s) {
     k_traverse<T, S><<<blocks_for(B.nt, T), T, 0, s>>>(A.nodes, A.nt, B.nodes, B.nt, keysB, mesh_shift, rootsA, cand,
                                                        cap, counters);
 }
 
-void launch_traverse(MeshDev A, MeshDev B, const uint64_t* keysB, int mesh_shift, const uint2* rootsA, uint2* cand,
+void launch_traverse(MeshDev A, MeshDev B, const uint32_t* keysB, int mesh_shift, const uint2* rootsA, uint2* cand,
                      uint32_t cap, uint32_t* counters, int variant, hipStream_t s) {
     if (A.nt == 0 || B.nt == 0) return;
     switch (variant) {  // (workgroup size, candidate pairs staged per wave)

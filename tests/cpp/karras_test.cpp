@@ -179,7 +179,15 @@ int main() {
         for (auto& v : k) v = (uint32_t)(rng() & 0xffu);
         bad |= run_set("small", k, {0, 1, 2});
     }
-    {  // forest keys: mesh id above 30 bits of Morton code, searched without samples
+    {  // batch keys as the library builds them: mesh id above 18 bits of Morton code, within 32 bits
+        std::vector<uint32_t> k;
+        for (uint32_t m = 0; m < 300; ++m) {
+            const int nm = 1 + (int)(rng() % 900);
+            for (int i = 0; i < nm; ++i) k.push_back((m << 18) | (uint32_t)(rng() & 0x3ffffu));
+        }
+        bad |= run_set("forest32", k, {0, 8});
+    }
+    {  // 64-bit forest keys: mesh id above 30 bits of Morton code, searched without samples
         std::vector<uint64_t> k;
         for (uint64_t m = 0; m < 40; ++m) {
             const int nm = 1 + (int)(rng() % 3000);

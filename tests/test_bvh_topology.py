@@ -1,7 +1,8 @@
 """BVH topology on the CPU: the per-node routine of k_build_chunks
 (csrc/gi_karras.h, host and device code) compiled for the host and run over
-every node of Morton, uniform, duplicate-heavy, clustered, tiny and 64-bit
-forest key sets, at chunk sizes 64/256/512 and several key-sample strides.
+every node of Morton, uniform, duplicate-heavy, clustered, tiny and batch
+(mesh id above the Morton code, 32 and 64 bits) key sets, at chunk sizes
+64/256/512 and several key-sample strides.
 Each node must equal a plain Karras build (Karras 2012), and the tree must be
 well formed (every node reached once, children splitting their parent's
 range, ropes = pre-order successors). The reference's BVH (cuBQL's builder
@@ -28,6 +29,6 @@ def test_chunk_node_routine_equals_plain_karras_and_tree_is_well_formed(tmp_path
     run = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert run.returncode == 0 and run.stdout.rstrip().endswith("ALL OK"), run.stdout[-3000:] + run.stderr
     # every key set ran at every chunk size
-    for name in ("sphere_morton", "uniform", "duplicates", "clustered", "small", "forest64"):
+    for name in ("sphere_morton", "uniform", "duplicates", "clustered", "small", "forest32", "forest64"):
         for chunk in (64, 256, 512):
             assert f"ok {name} " in run.stdout and f"chunk={chunk} " in run.stdout
