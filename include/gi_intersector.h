@@ -169,7 +169,8 @@ typedef struct {
 } gi_mesh_pair;
 
 /* Replaces both mesh slots with the batch (at most 2^30 - 1 triangles per
- * side in total). gi_intersect then refuses the context; use: */
+ * side in total and 2^29 pairs, else GI_ERR_TOO_LARGE). gi_intersect then
+ * refuses the context; use: */
 int gi_set_batch(gi_context* ctx, const gi_mesh_pair* pairs, uint32_t n_pairs);
 /* One result for the whole batch: the pairs of mesh pair i are
  * [pair_offsets[i], pair_offsets[i+1]) (n_pairs + 1 entries, caller-owned),
