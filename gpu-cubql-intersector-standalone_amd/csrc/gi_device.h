@@ -64,7 +64,7 @@ constexpr uint32_t kPrepareMaxBlocks = 4096;
 // captured launch sequence picks up a new transform from the copy before it;
 // grid flag cleared when a vertex is off the grid) and its triangles (boxes,
 // and with keys non-null the Morton keys, identity values and, with hist0
-// non-null, the first radix pass's per-tile digit counts, as radix_sort_u32
+// non-null, the first radix pass's per-tile digit counts, as a SortJobU32
 // takes them with hist0_ready). The Morton frame is the box of xf's moved
 // source-bound corners (xf->lo, xf->hi).
 void launch_stage_mesh(const double* src, uint32_t nv, const uint32_t* tri, uint32_t nt, const Xform* xf,
@@ -92,10 +92,8 @@ void launch_tri_boxes_forest(const uint32_t* tri, const uint2* ttiles, uint32_t 
 // the tmp buffers, 0 when it ends in (keys, vals).
 size_t sort_hist_elems(size_t n);
 size_t scan_tmp_elems(size_t n);
-int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
-                   uint32_t* hist, uint32_t* scan_tmp, hipStream_t s, bool hist0_ready = false);
 // whether a sort of n uint32 keys runs the multi-tile passes (and so can take
-// the first pass's counts from launch_stage_mesh)
+// the first pass's counts from launch_stage_mesh, and writes key samples)
 bool radix_sort_takes_hist0(uint32_t n);
 // Key samples: the multi-tile sort's last pass also writes every
 // 2^key_sample_shift(n)-th sorted key, key_sample_count(n) <= kMaxKeySamples

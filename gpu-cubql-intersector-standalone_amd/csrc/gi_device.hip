@@ -900,10 +900,6 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32_
     return parity;
 }
 
-int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
-                   uint32_t* hist, uint32_t* scan_tmp, hipStream_t s, bool hist0_ready) {
-    return radix_sort_impl<uint32_t>(keys, vals, tmp_keys, tmp_vals, n, bits, hist, scan_tmp, s, hist0_ready);
-}
 
 bool radix_sort_takes_hist0(uint32_t n) { return n > small_sort_max<uint32_t>(); }
 
