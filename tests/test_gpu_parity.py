@@ -33,8 +33,9 @@ def test_sphere_torus_small_grid():
 
 def test_result_output_one_workgroup():
     # 409,600 triangles per mesh give between 4,096 and 16,384 intersecting
-    # pairs: the one-workgroup output kernel (16 keys per thread, several
-    # items per lane, LDS passes) sorts them and scans their segment counts
+    # pairs: the small-result output (several 2,048-record runs sorted in LDS,
+    # records placed by their ranks in the other runs) sorts them and derives
+    # their segment offsets from the runs' count prefixes
     (xa, ta), (xb, tb) = mg.sphere_torus(640, 320)
     res, ref = _check(xa, ta, xb, tb, True)
     assert 4096 < len(res["pairs"]) <= 16384
@@ -83,6 +84,40 @@ def test_height_field_shift_and_rotation():
     # contacts dominate and must be exercised
     assert res2["stats"]["coplanar"] > 0
     assert res2["stats"]["exact_yellow_yes"] > 0
+
+
+@pytest.mark.parametrize("copy", ["shift", "rot90"])
+def test_full_size_height_field_on_sampled_triangles(copy):
+    """The degeneracy-heavy pair at its full size (512 x 512 cells, 524,288
+    triangles a side; the oracle takes over a minute on the whole pair): the
+    GPU result of the whole pair, restricted to every 17th triangle of A,
+    equals the oracle's result for those triangles, as a mesh of their own,
+    against all of B. A pair's verdict and segment depend on its two
+    triangles only, so the restriction is exact."""
+    (p, t), (s, _), (r, _) = mg.height_field_pairs(cells=512)
+    q = s if copy == "shift" else r
+    eng = gi.Intersector(device=0)
+    try:
+        eng.set_mesh(0, p, t)
+        eng.set_mesh(1, q, t)
+        res = eng.intersect(decode=False)
+    finally:
+        eng.close()
+    pairs, offs = res["pairs"], res["seg_offsets"].astype(np.int64)
+    assert res["stats"]["grid"] == 1
+    assert np.all(np.diff(pairs[:, 0].astype(np.int64) * (1 << 32) + pairs[:, 1]) > 0)  # sorted, no repeats
+    stride = 17
+    sel = np.nonzero(pairs[:, 0] % stride == 0)[0]
+    ref = oracle_py.intersect(p, np.ascontiguousarray(t[::stride]), q, t, nthreads=8)
+    got = pairs[sel].copy()
+    got[:, 0] //= stride
+    np.testing.assert_array_equal(got, ref["pairs"])
+    assert ref["stats"]["n_coplanar"] > 0  # the sample reaches the coplanar (deferred) construction
+    counts = offs[sel + 1] - offs[sel]
+    sub_offs = np.concatenate([[0], np.cumsum(counts)])
+    idx = np.repeat(offs[sel] - sub_offs[:-1], counts) + np.arange(int(sub_offs[-1]))
+    exp2 = res["seg_exp2"][idx] if res["seg_exp2"] is not None else None
+    assert gi.decode_segments(res["seg_exact"][idx], exp2, sub_offs) == ref["segments"]
 
 
 def test_transform_refit_matches_oracle_on_moved_vertices():
