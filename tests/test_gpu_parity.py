@@ -70,6 +70,14 @@ def test_sphere_above_2_21_triangles_with_small_torus():
     assert len(res["pairs"]) > 1000
 
 
+def test_bench_workload_full_size():
+    # the measured workload itself: displaced sphere x rotated torus, 1,048,576
+    # triangles each, every pair, candidate count and segment against the oracle
+    (xa, ta), (xb, tb) = mg.sphere_torus(1024, 512)
+    res, ref = _check(xa, ta, xb, tb, True)
+    assert len(res["pairs"]) > 10000
+
+
 def test_icosphere_pair_doubles_host_residual():
     (xa, ta), (xb, tb) = mg.icosphere_pair(level=4)
     res, ref = _check(xa, ta, xb, tb, False)
