@@ -78,8 +78,11 @@ def test_bench_workload_full_size():
     assert len(res["pairs"]) > 10000
 
 
-def test_icosphere_pair_doubles_host_residual():
-    (xa, ta), (xb, tb) = mg.icosphere_pair(level=4)
+@pytest.mark.parametrize("level", [4, 5])
+def test_icosphere_pair_doubles_host_residual(level):
+    # level 5 is the reference's CPU-runnable case: 20,480 triangles each, raw
+    # doubles (off the integer grid), so the exact stage runs on the host
+    (xa, ta), (xb, tb) = mg.icosphere_pair(level=level)
     res, ref = _check(xa, ta, xb, tb, False)
     assert len(res["pairs"]) > 50
 
