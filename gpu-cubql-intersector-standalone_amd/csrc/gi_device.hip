@@ -949,8 +949,10 @@ int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t*
 // (a global-atomic bottom-up refit, with the device-scope fences it needs on
 // gfx950's per-XCD L2s, cost 6 ms for 2^20 triangles; an LDS-atomic one
 // still a quarter of this kernel). The few "top" nodes whose range crosses a
-// chunk boundary take the union of their leaf range from a 64-ary hierarchy
-// of block boxes.
+// chunk boundary search the sort's key samples and then one stride of keys
+// (gi_karras.h), keep the box of their leaves inside the chunk, and get the
+// rest of their leaf range from a 64-ary hierarchy of block boxes
+// (k_top_boxes).
 // A refit after a rigid motion reruns the same kernels on the same sorted
 // keys: the topology comes out identical, only the boxes change.
 
