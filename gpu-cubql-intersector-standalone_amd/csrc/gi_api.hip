@@ -66,6 +66,10 @@ struct MeshSlot {
     bool on_device = false;  // device buffers hold the mesh
     bool built = false;      // BVH topology is valid
     bool dirty = true;       // vertices/boxes/BVH boxes need an update
+    // Mesh B's BVH is walked by nothing (its leaves are the queries against
+    // A's tree): B builds its leaf nodes only, unless its tree is shared into
+    // an A slot (gi_share_mesh), which needs the whole tree.
+    bool full_tree = false;
     Xform xf;
     int sorted_in_tmp = 0;   // the sorted Morton keys live in (tkeys, tvals)
     bool borrowed = false;     // device buffers belong to another context (gi_share_mesh)
@@ -92,6 +96,7 @@ struct MeshSlot {
         }
         if (borrowed) { borrowed = false; set = false; }
         on_device = false;
+        full_tree = false;
         built = false;
         dirty = true;
     }
@@ -351,8 +356,9 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
 }
 
 template <class K>
-BuildJob<K> slot_job(MeshSlot& m) {
+BuildJob<K> slot_job(MeshSlot& m, int slot) {
     BuildJob<K> j;
+    j.leaves_only = slot == GI_MESH_B && !m.full_tree;
     j.keys = m.sorted_in_tmp ? m.tkeys.get<K>() : m.keys.get<K>();
     j.prims = m.sorted_in_tmp ? m.tvals.get<uint32_t>() : m.vals.get<uint32_t>();
     j.n = m.nt;
@@ -405,7 +411,7 @@ void build_meshes(gi_context* ctx, bool rebuild, bool timed = true, bool zero_co
     BuildJob<uint32_t> jobs[2];
     int nj = 0;
     for (int sl = 0; sl < 2; ++sl)
-        if (need[sl]) jobs[nj++] = slot_job<uint32_t>(ctx->mesh[sl]);
+        if (need[sl]) jobs[nj++] = slot_job<uint32_t>(ctx->mesh[sl], sl);
     hipStream_t s = ctx->stream;
     uint32_t leaves = 0;
     for (int j = 0; j < nj; ++j) leaves += jobs[j].n;
@@ -422,7 +428,7 @@ void build_meshes(gi_context* ctx, bool rebuild, bool timed = true, bool zero_co
     for (int sl = 0; sl < 2; ++sl) {
         if (!need[sl]) continue;
         MeshSlot& m = ctx->mesh[sl];
-        if (m.forest)
+        if (m.forest && (sl == GI_MESH_A || m.full_tree))  // B's mesh roots are walked by nothing
             launch_forest_roots(m.range.get<uint2>(), m.nodes.get<float4>(), m.nt, m.toff.get<uint32_t>(), m.forest,
                                 m.roots.get<uint2>(), s);
         m.built = true;
@@ -1208,6 +1214,13 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
     if (src->mesh[src_slot].forest) return GI_ERR_INVALID_ARGUMENT;
     try {
         HIP_TRY(hipSetDevice(src->device));
+        MeshSlot& sm = src->mesh[src_slot];
+        if (dst_slot == GI_MESH_A && !sm.full_tree && src_slot == GI_MESH_B) {
+            if (sm.borrowed) return GI_ERR_INVALID_ARGUMENT;  // its owner built the leaf nodes only
+            // an A slot walks the tree: build B's internal nodes from now on
+            sm.full_tree = true;
+            sm.dirty = true;
+        }
         build_meshes(src, false);
         sync_all(src);
         MeshSlot& d = dst->mesh[dst_slot];
@@ -1225,6 +1238,7 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         for (int k = 0; k < 3; ++k) { d.src_lo[k] = s.src_lo[k]; d.src_hi[k] = s.src_hi[k]; }
         d.sorted_in_tmp = s.sorted_in_tmp;
         d.morton_axis = s.morton_axis;
+        d.full_tree = s.full_tree;
         d.borrowed = true;
         DevBuf* dd[] = {&d.src, &d.tri, &d.v64, &d.vi, &d.tbox, &d.keys, &d.vals, &d.tkeys, &d.tvals, &d.nodes,
                         &d.range, &d.blocks, &d.samples};

@@ -146,6 +146,9 @@ struct BuildJob {
     uint2* range;
     float4* blocks;
     const K* samples;  // the sort's key samples (key_sample_count(n)), or null
+    // only the leaf nodes (box, prim; rope kSentinel), for a tree whose leaves
+    // are queries and whose internal nodes nothing reads
+    bool leaves_only;
 };
 // Builds one or two trees with one launch per kernel (the blocks of a launch
 // are split between the trees); chunk_begin/chunk_end (may be null) bracket

@@ -44,6 +44,22 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
     return x * q + min(x, r) + b / X;
 }
 
+// The same for a launch whose blocks [0, nA) serve one tree and the rest
+// another: every XCD takes a contiguous run of each tree's blocks, its share
+// of the first tree first (block b runs on XCD b % 8; of the blocks on XCD x,
+// as many as [0, nA) holds for x go to the first tree). The two trees can
+// differ in work per block and still load every XCD alike.
+__device__ __forceinline__ uint32_t xcd_block_split(uint32_t b, uint32_t G, uint32_t nA, bool& second) {
+    constexpr uint32_t X = 8;
+    const uint32_t x = b % X, r = b / X;
+    const uint32_t qA = nA > x ? (nA - 1 - x) / X + 1 : 0u;           // first-tree blocks on XCD x
+    const uint32_t startA = x * (nA / X) + min(x, nA % X);            // their first block number
+    second = r >= qA;
+    if (!second) return startA + r;
+    const uint32_t startG = x * (G / X) + min(x, G % X);              // XCD x's first block overall
+    return startG - startA + (r - qA);                                // second tree's block number
+}
+
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -969,6 +985,7 @@ struct ChunkJobs {
     float4* level1_0; float4* level1_1;  // boxes of the 64-leaf blocks (first level of the block hierarchy)
     uint64_t* topmask0; uint64_t* topmask1;  // per 64 internal nodes: which are top nodes
     const K* samp0; const K* samp1;      // key samples (null: none)
+    int leaves0, leaves1;                // leaf nodes only (see BuildJob::leaves_only)
     int n0, n1;
     int samp_shift0, samp_shift1;
     uint32_t split;
@@ -980,8 +997,8 @@ __host__ __device__ constexpr int block_table_offset(int chunk, int k) { return 
 
 template <class K, int CHUNK>
 __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const bool second = blk >= J.split;
+    bool second;
+    const uint32_t chunk = xcd_block_split(blockIdx.x, gridDim.x, J.split, second);
     const K* __restrict__ keys = second ? J.keys1 : J.keys0;
     const uint32_t* __restrict__ prims = second ? J.prims1 : J.prims0;
     const float4* __restrict__ tbox = second ? J.box1 : J.box0;
@@ -991,7 +1008,16 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
     uint64_t* __restrict__ topmask = second ? J.topmask1 : J.topmask0;
     const int n = second ? J.n1 : J.n0;
     const K* __restrict__ samp = second ? J.samp1 : J.samp0;
-    const uint32_t chunk = blk - (second ? J.split : 0u);
+    if (second ? J.leaves1 : J.leaves0) {  // block-uniform: the leaf nodes alone
+        const int jl = (int)chunk * CHUNK + (int)threadIdx.x;
+        if (jl < n) {
+            const uint32_t prim = prims[jl];
+            const float4 l = tbox[2 * (size_t)prim], h = tbox[2 * (size_t)prim + 1];
+            nodes[2 * (size_t)(n - 1 + jl)] = make_float4(l.x, l.y, l.z, __uint_as_float(prim));
+            nodes[2 * (size_t)(n - 1 + jl) + 1] = make_float4(h.x, h.y, h.z, __uint_as_float(kSentinel));
+        }
+        return;
+    }
     __shared__ K skey[CHUNK];
     __shared__ float4 sbox[2 * block_table_offset(CHUNK, 7)];  // (lo, hi) pairs of levels 0..6
     const int t = (int)threadIdx.x;
@@ -1229,6 +1255,8 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     cj.samp_shift0 = key_sample_shift(n0);
     cj.samp1 = j1 ? j1->samples : nullptr;
     cj.samp_shift1 = key_sample_shift(n1);
+    cj.leaves0 = j0->leaves_only ? 1 : 0;
+    cj.leaves1 = j1 && j1->leaves_only ? 1 : 0;
     uint64_t* topmask[2] = {(uint64_t*)(j0->blocks + top_mask_offset(n0)),
                             (uint64_t*)(o->blocks + top_mask_offset(n1))};
     cj.topmask0 = topmask[0];
@@ -1239,10 +1267,11 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     if (chunk == 256) k_build_chunks<K, 256><<<chunk_blocks, 256, 0, s>>>(cj);
     else k_build_chunks<K, 512><<<chunk_blocks, 512, 0, s>>>(cj);
     if (chunk_end) (void)hipEventRecordWithFlags(chunk_end, s, event_flags);
-    // 64-ary levels of leaf boxes for the nodes that cross chunks
+    // 64-ary levels of leaf boxes for the nodes that cross chunks (none for a
+    // tree of leaf nodes only)
     BlockLevels lv[2];
     const BuildJob<K>* jj[2] = {j0, o};
-    uint32_t nn[2] = {n0, n1};
+    uint32_t nn[2] = {j0->leaves_only ? 0u : n0, j1 && j1->leaves_only ? 0u : n1};
     for (int t = 0; t < 2; ++t) {
         lv[t].level[0] = jj[t]->nodes + 2 * (size_t)(nn[t] ? nn[t] - 1 : 0);
         size_t off = 0;
@@ -1272,12 +1301,12 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     tj.range0 = j0->range; tj.range1 = o->range;
     tj.topmask0 = topmask[0]; tj.topmask1 = topmask[1];
     tj.nodes0 = j0->nodes; tj.nodes1 = o->nodes;
-    tj.n0 = n0; tj.n1 = n1;
+    tj.n0 = nn[0]; tj.n1 = nn[1];
     // one wave per TOP_GROUPS groups of 64 internal nodes (n - 1 of them); a
-    // one-leaf tree has none
-    auto top_waves = [](uint32_t nn) { return nn > 1 ? blocks_for(blocks_for(nn - 1, 64), TOP_GROUPS) : 0u; };
-    tj.split = blocks_for(top_waves(n0), 4);
-    uint32_t top_blocks = tj.split + blocks_for(top_waves(n1), 4);
+    // one-leaf tree or one of leaf nodes only has none
+    auto top_waves = [](uint32_t m) { return m > 1 ? blocks_for(blocks_for(m - 1, 64), TOP_GROUPS) : 0u; };
+    tj.split = blocks_for(top_waves(nn[0]), 4);
+    uint32_t top_blocks = tj.split + blocks_for(top_waves(nn[1]), 4);
     if (top_blocks) k_top_boxes<<<top_blocks, 256, 0, s>>>(tj, chunk);
     GI_LAUNCH_CHECK();
 }
