@@ -157,6 +157,10 @@ struct gi_context {
     Xform* h_xf = nullptr;
     uint32_t* h_counters = nullptr;
     DevBuf xf_dev;
+    // what an eager stage last copied into xf_dev[slot]: an unchanged
+    // transform is not sent again (captures and replays clear the flag)
+    Xform xf_last[2] = {};
+    bool xf_current[2] = {false, false};
     // captured launch sequence of a query up to the counter read-back, valid
     // while graph_sig (sizes, flags, device pointers) is unchanged
     hipGraph_t graph = nullptr;
@@ -180,6 +184,7 @@ struct gi_context {
         drop_graph();
         graph_sig_seen.clear();
         xf_dev.release();
+        xf_current[0] = xf_current[1] = false;
         for (auto& m : mesh) m.release();
         DevBuf* all[] = {&counters, &hist, &scan_tmp, &partials, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
                          &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2, &pair_off,
@@ -335,8 +340,13 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
     uint32_t* tv = m.tvals.ensure<uint32_t>(nt);
     const bool topology = !m.built || rebuild;
     Xform* d_xf = ctx->xf_dev.get<Xform>() + slot;
-    ctx->h_xf[slot] = staged_xform(m);
-    HIP_TRY(hipMemcpyAsync(d_xf, ctx->h_xf + slot, sizeof(Xform), hipMemcpyHostToDevice, s));
+    const Xform x = staged_xform(m);
+    if (ctx->capturing || !ctx->xf_current[slot] || memcmp(&x, &ctx->xf_last[slot], sizeof(Xform)) != 0) {
+        ctx->h_xf[slot] = x;
+        HIP_TRY(hipMemcpyAsync(d_xf, ctx->h_xf + slot, sizeof(Xform), hipMemcpyHostToDevice, s));
+        ctx->xf_last[slot] = x;
+        ctx->xf_current[slot] = !ctx->capturing;
+    }
     if (topology) m.morton_axis = morton_axis_bits(m.nt);
     uint32_t* hist = nullptr;
     uint32_t* stmp = nullptr;
@@ -610,7 +620,10 @@ bool launch_presync_graph(gi_context* ctx, const gi_options& o) {
     for (int sl = 0; sl < 2; ++sl) need[sl] = slot_needs_stage(ctx->mesh[sl], o.rebuild != 0);
     if (ctx->graph_exec && sig == ctx->graph_sig) {
         for (int sl = 0; sl < 2; ++sl)
-            if (need[sl]) ctx->h_xf[sl] = staged_xform(ctx->mesh[sl]);
+            if (need[sl]) {
+                ctx->h_xf[sl] = staged_xform(ctx->mesh[sl]);
+                ctx->xf_current[sl] = false;  // the replay copies h_xf itself
+            }
         HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));  // records the stage events itself
         for (int sl = 0; sl < 2; ++sl)
             if (need[sl]) {
