@@ -59,3 +59,38 @@ def test_graph_replay_with_rebuild_every_step():
             assert r["segments"] == ref["segments"]
     finally:
         eng.close()
+
+
+def test_transform_switched_back_between_eager_queries_and_replays():
+    """The device copy of a slot's transform is only refreshed when the staged
+    transform changes. Switching between two transforms, eager and replayed
+    queries interleaved (a replay writes the transform itself), every query
+    must equal a fresh context's result for its transform; step 9 is an eager
+    query with the transform of step 7 right after a replay with the other."""
+    (xa, ta), (xb, tb) = mg.sphere_torus(96, 48)
+    ms = _transforms(2, seed=5)
+    refs = []
+    for m in ms:
+        e = gi.Intersector()
+        try:
+            e.set_mesh(0, xa, ta)
+            e.set_mesh(1, xb, tb)
+            e.set_transform_matrix(1, m, snap=True)
+            refs.append(e.intersect())
+        finally:
+            e.close()
+    assert not np.array_equal(refs[0]["pairs"], refs[1]["pairs"])
+    seq = [(0, 0), (1, 0), (0, 0), (0, 1), (1, 1), (0, 1), (1, 1), (0, 0), (1, 1), (0, 0), (0, 0), (1, 0)]
+    eng = gi.Intersector()
+    try:
+        eng.set_mesh(0, xa, ta)
+        eng.set_mesh(1, xb, tb)
+        for i, (k, g) in enumerate(seq):
+            eng.set_transform_matrix(1, ms[k], snap=True)
+            r = eng.intersect(use_graph=bool(g))
+            np.testing.assert_array_equal(r["pairs"], refs[k]["pairs"], err_msg=f"step {i}")
+            assert r["segments"] == refs[k]["segments"], f"step {i}"
+            if i == 8:
+                assert r["stats"]["graph"] == 1  # the step before the check of step 9 was a replay
+    finally:
+        eng.close()
