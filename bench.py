@@ -4,8 +4,9 @@ Workload (one step): a displaced UV sphere (1024x512, 2^20 triangles) and a
 rotated torus (1024x512, 2^20 triangles), both snapped to the 2^24 integer
 grid, resident in HBM before timing. Each step runs the whole device path from
 those arrays: vertex preparation, FP32 boxes and Morton sort for both meshes,
-the LBVH of mesh A and the sorted leaf nodes of mesh B (B's leaves are the
-traversal's queries; nothing walks a tree of B), BVH traversal, FP64 filter,
+the LBVH of mesh A (B's triangles in Morton order are the traversal's
+queries, read with their boxes; nothing walks a tree of B), BVH traversal,
+FP64 filter,
 exact int128 stage with
 exact segments, sort of the intersecting pairs and gather of their segments
 (output left in device memory).
@@ -41,20 +42,20 @@ def algorithmic_bytes(nvA, ntA, nvB, ntB, st):
     return per_mesh + 112 * st["candidates"] + 8 * st["pairs"] + 128 * st["segments"]
 
 
-def kernel_bytes(name, leaves, st, leaves_only=0):
-    """Compulsory bytes of one launch of a single kernel over `leaves` leaves
-    (k_build_chunks: of which `leaves_only` belong to mesh B's tree, which
-    keeps its leaf nodes only)."""
+def kernel_bytes(name, st, ntA, ntB):
+    """Compulsory bytes of one launch of a single kernel of the step (one
+    launch of each per step; A's tree is the only tree built: B's triangles in
+    Morton order are the traversal's queries)."""
     if name == "k_build_chunks":
-        # A's tree, per leaf: sorted key 4 B + prim id 4 B + its FP32 box 32 B
-        # read; leaf and internal node 32 B each + leaf range 8 B written.
-        # B's: prim id 4 B + box 32 B read, leaf node 32 B written.
-        n = float(leaves - leaves_only)
-        return 4 * n + 4 * n + 32 * n + 32 * (2 * n - 1) + 8 * (n - 1) + 68.0 * leaves_only
+        # per leaf of A: sorted key 4 B + triangle id 4 B + its FP32 box 32 B
+        # read; leaf and internal node 32 B each + leaf range 8 B written
+        n = float(ntA)
+        return 4 * n + 4 * n + 32 * n + 32 * (2 * n - 1) + 8 * (n - 1)
     if name == "k_traverse":
-        # every B leaf once (32 B), 8 B per emitted pair (A nodes are read as
-        # far as the queries need them, which the model does not count)
-        return 32 * leaves + 8 * st["traversal_hits"]
+        # the cost model's traversal read, A's nodes once (2 x 32 B per
+        # triangle of A); B's triangles once (id 4 B + box 32 B); 8 B per
+        # candidate pair written
+        return 64.0 * ntA + 36.0 * ntB + 8 * st["traversal_hits"]
     raise KeyError(name)
 
 
@@ -343,17 +344,24 @@ def main():
            ("ms_total", "ms_build", "ms_traverse", "ms_filter", "ms_exact", "ms_output", "ms_bvh_chunks")}
     value = st["candidates"] / (ms / 1000.0)
 
-    # dominant kernel: the LBVH build+refit kernel (largest share of the step;
-    # one launch builds A's tree and B's leaf nodes), timed by HIP events the
-    # library records around it on its stream
-    dom = "k_build_chunks"
+    # the two largest single kernels of a step, each timed by HIP events the
+    # library records around its one launch per step on its stream: the BVH
+    # build (k_build_chunks, A's tree) and the traversal (k_traverse); the
+    # dominant kernel is the longer of the two (the two k_stage_mesh launches
+    # run side by side on two streams, each shorter than either)
     launches = sum(s["bvh_chunk_launches"] for s in stats)
-    kernel_ms = sum(s["ms_bvh_chunks"] for s in stats) / max(launches, 1)
-    leaves_per_launch = sum(s["bvh_chunk_leaves"] for s in stats) / max(launches, 1)
-    # mesh B (the query leaves) keeps only its leaf nodes
-    kb = kernel_bytes(dom, leaves_per_launch, st, leaves_only=tb.shape[0])
-    achieved = kb / (kernel_ms / 1000.0) / 1e9
-    traffic = measured_traffic(dom)
+    assert launches == len(stats), "one build launch per step"
+    assert all(s["bvh_chunk_leaves"] == ta.shape[0] for s in stats), "the build covers A's tree only"
+    kernel_ms = {"k_build_chunks": sum(s["ms_bvh_chunks"] for s in stats) / launches,
+                 "k_traverse": avg["ms_traverse"]}
+    roofs = {}
+    for k, kms in kernel_ms.items():
+        kb = kernel_bytes(k, st, ta.shape[0], tb.shape[0])
+        achieved = kb / (kms / 1000.0) / 1e9
+        roofs[k] = {"bound": "hbm", "kernel": k, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(k), "kernel_ms": kms,
+                    "launches_timed": launches, "algorithmic_bytes": kb}
+    dom = max(kernel_ms, key=kernel_ms.get)
     b_alg = algorithmic_bytes(xa.shape[0], ta.shape[0], xb.shape[0], tb.shape[0], st)
 
     line = {
@@ -375,10 +383,8 @@ def main():
                    "vertices_a": int(xa.shape[0]), "vertices_b": int(xb.shape[0])},
         "tuning": tuning,
         "graph_steps": sum(s["graph"] for s in stats),
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_ms": kernel_ms, "launches_timed": launches, "leaves_per_launch": leaves_per_launch,
-                     "algorithmic_bytes": kb},
+        "roofline": roofs[dom],
+        "roofline_other": [r for k, r in roofs.items() if k != dom],
         "pipeline": {"candidates": st["candidates"], "traversal_hits": st["traversal_hits"], "pairs": st["pairs"],
                      "segments": st["segments"], "filter_no": st["filter_no"], "filter_green": st["filter_green"],
                      "filter_yellow": st["filter_yellow"], "exact_yellow_yes": st["exact_yellow_yes"],

@@ -66,9 +66,9 @@ struct MeshSlot {
     bool on_device = false;  // device buffers hold the mesh
     bool built = false;      // BVH topology is valid
     bool dirty = true;       // vertices/boxes/BVH boxes need an update
-    // Mesh B's BVH is walked by nothing (its leaves are the queries against
-    // A's tree): B builds its leaf nodes only, unless its tree is shared into
-    // an A slot (gi_share_mesh), which needs the whole tree.
+    // Mesh B's BVH is walked by nothing (its triangles, in Morton order with
+    // their boxes, are the queries against A's tree): B builds no tree unless
+    // it is shared into an A slot (gi_share_mesh), which needs the whole tree.
     bool full_tree = false;
     Xform xf;
     int sorted_in_tmp = 0;   // the sorted Morton keys live in (tkeys, tvals)
@@ -366,9 +366,8 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
 }
 
 template <class K>
-BuildJob<K> slot_job(MeshSlot& m, int slot) {
+BuildJob<K> slot_job(MeshSlot& m) {
     BuildJob<K> j;
-    j.leaves_only = slot == GI_MESH_B && !m.full_tree;
     j.keys = m.sorted_in_tmp ? m.tkeys.get<K>() : m.keys.get<K>();
     j.prims = m.sorted_in_tmp ? m.tvals.get<uint32_t>() : m.vals.get<uint32_t>();
     j.n = m.nt;
@@ -421,7 +420,8 @@ void build_meshes(gi_context* ctx, bool rebuild, bool timed = true, bool zero_co
     BuildJob<uint32_t> jobs[2];
     int nj = 0;
     for (int sl = 0; sl < 2; ++sl)
-        if (need[sl]) jobs[nj++] = slot_job<uint32_t>(ctx->mesh[sl], sl);
+        if (need[sl] && (sl == GI_MESH_A || ctx->mesh[sl].full_tree))  // B's queries need its order and boxes only
+            jobs[nj++] = slot_job<uint32_t>(ctx->mesh[sl]);
     hipStream_t s = ctx->stream;
     uint32_t leaves = 0;
     for (int j = 0; j < nj; ++j) leaves += jobs[j].n;
@@ -455,6 +455,8 @@ MeshDev mesh_dev(const MeshSlot& m) {
     d.vi = m.vi.get<int4>();
     d.tri = m.tri.get<uint32_t>();
     d.nodes = m.nodes.get<float4>();
+    d.order = m.sorted_in_tmp ? m.tvals.get<uint32_t>() : m.vals.get<uint32_t>();
+    d.tbox = m.tbox.get<float4>();
     return d;
 }
 
@@ -1229,8 +1231,8 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         HIP_TRY(hipSetDevice(src->device));
         MeshSlot& sm = src->mesh[src_slot];
         if (dst_slot == GI_MESH_A && !sm.full_tree && src_slot == GI_MESH_B) {
-            if (sm.borrowed) return GI_ERR_INVALID_ARGUMENT;  // its owner built the leaf nodes only
-            // an A slot walks the tree: build B's internal nodes from now on
+            if (sm.borrowed) return GI_ERR_INVALID_ARGUMENT;  // its owner built no tree for it
+            // an A slot walks the tree: build B's tree from now on (on the kept sorted keys)
             sm.full_tree = true;
             sm.dirty = true;
         }

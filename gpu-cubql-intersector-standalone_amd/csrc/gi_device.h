@@ -51,7 +51,9 @@ struct MeshDev {
     const double4* v64;  // vertices used by the predicates
     const int4* vi;      // integer copy (valid when the mesh is on the grid)
     const uint32_t* tri; // 3 * nt
-    const float4* nodes; // BVH, 2 float4 per node, 2*nt - 1 nodes
+    const float4* nodes; // BVH, 2 float4 per node, 2*nt - 1 nodes (a walked mesh only)
+    const uint32_t* order;  // triangle ids in Morton order (the sorted values)
+    const float4* tbox;     // triangle boxes: lo, hi of triangle t at 2t, 2t + 1
 };
 
 void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
@@ -146,9 +148,6 @@ struct BuildJob {
     uint2* range;
     float4* blocks;
     const K* samples;  // the sort's key samples (key_sample_count(n)), or null
-    // only the leaf nodes (box, prim; rope kSentinel), for a tree whose leaves
-    // are queries and whose internal nodes nothing reads
-    bool leaves_only;
 };
 // Builds one or two trees with one launch per kernel (the blocks of a launch
 // are split between the trees); chunk_begin/chunk_end (may be null) bracket

@@ -985,7 +985,6 @@ struct ChunkJobs {
     float4* level1_0; float4* level1_1;  // boxes of the 64-leaf blocks (first level of the block hierarchy)
     uint64_t* topmask0; uint64_t* topmask1;  // per 64 internal nodes: which are top nodes
     const K* samp0; const K* samp1;      // key samples (null: none)
-    int leaves0, leaves1;                // leaf nodes only (see BuildJob::leaves_only)
     int n0, n1;
     int samp_shift0, samp_shift1;
     uint32_t split;
@@ -1008,16 +1007,6 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
     uint64_t* __restrict__ topmask = second ? J.topmask1 : J.topmask0;
     const int n = second ? J.n1 : J.n0;
     const K* __restrict__ samp = second ? J.samp1 : J.samp0;
-    if (second ? J.leaves1 : J.leaves0) {  // block-uniform: the leaf nodes alone
-        const int jl = (int)chunk * CHUNK + (int)threadIdx.x;
-        if (jl < n) {
-            const uint32_t prim = prims[jl];
-            const float4 l = tbox[2 * (size_t)prim], h = tbox[2 * (size_t)prim + 1];
-            nodes[2 * (size_t)(n - 1 + jl)] = make_float4(l.x, l.y, l.z, __uint_as_float(prim));
-            nodes[2 * (size_t)(n - 1 + jl) + 1] = make_float4(h.x, h.y, h.z, __uint_as_float(kSentinel));
-        }
-        return;
-    }
     __shared__ K skey[CHUNK];
     __shared__ float4 sbox[2 * block_table_offset(CHUNK, 7)];  // (lo, hi) pairs of levels 0..6
     const int t = (int)threadIdx.x;
@@ -1255,8 +1244,6 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     cj.samp_shift0 = key_sample_shift(n0);
     cj.samp1 = j1 ? j1->samples : nullptr;
     cj.samp_shift1 = key_sample_shift(n1);
-    cj.leaves0 = j0->leaves_only ? 1 : 0;
-    cj.leaves1 = j1 && j1->leaves_only ? 1 : 0;
     uint64_t* topmask[2] = {(uint64_t*)(j0->blocks + top_mask_offset(n0)),
                             (uint64_t*)(o->blocks + top_mask_offset(n1))};
     cj.topmask0 = topmask[0];
@@ -1267,11 +1254,10 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     if (chunk == 256) k_build_chunks<K, 256><<<chunk_blocks, 256, 0, s>>>(cj);
     else k_build_chunks<K, 512><<<chunk_blocks, 512, 0, s>>>(cj);
     if (chunk_end) (void)hipEventRecordWithFlags(chunk_end, s, event_flags);
-    // 64-ary levels of leaf boxes for the nodes that cross chunks (none for a
-    // tree of leaf nodes only)
+    // 64-ary levels of leaf boxes for the nodes that cross chunks
     BlockLevels lv[2];
     const BuildJob<K>* jj[2] = {j0, o};
-    uint32_t nn[2] = {j0->leaves_only ? 0u : n0, j1 && j1->leaves_only ? 0u : n1};
+    uint32_t nn[2] = {n0, n1};
     for (int t = 0; t < 2; ++t) {
         lv[t].level[0] = jj[t]->nodes + 2 * (size_t)(nn[t] ? nn[t] - 1 : 0);
         size_t off = 0;
@@ -1303,7 +1289,7 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     tj.nodes0 = j0->nodes; tj.nodes1 = o->nodes;
     tj.n0 = nn[0]; tj.n1 = nn[1];
     // one wave per TOP_GROUPS groups of 64 internal nodes (n - 1 of them); a
-    // one-leaf tree or one of leaf nodes only has none
+    // one-leaf tree has none
     auto top_waves = [](uint32_t m) { return m > 1 ? blocks_for(blocks_for(m - 1, 64), TOP_GROUPS) : 0u; };
     tj.split = blocks_for(top_waves(nn[0]), 4);
     uint32_t top_blocks = tj.split + blocks_for(top_waves(nn[1]), 4);
@@ -1362,15 +1348,17 @@ __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t fill, uint2* 
     wave_sync_lds();
 }
 
-// One thread per leaf of B (in B's Morton order, so a wave queries a compact
-// cluster), stackless traversal of A along the ropes. Overlapping leaf pairs
+// One thread per triangle of B, in B's Morton order (orderB, so a wave
+// queries a compact cluster; B needs no tree of its own), its box read from
+// tboxB; stackless traversal of A along the ropes. Overlapping leaf pairs
 // are compacted per wave (ballot + popcount) into LDS and flushed to the
 // global candidate queue with one atomic per STAGE pairs (the queue chunk).
 // Forest (batch): leaf k of B belongs to mesh keysB[k] >> mesh_shift and walks only
 // the subtree of the same mesh of A, from its root to the rope leaving it.
 template <int TRAV_THREADS, uint32_t TRAV_STAGE>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restrict__ nodesA, uint32_t nA,
-                                                           const float4* __restrict__ nodesB, uint32_t nB,
+                                                           const uint32_t* __restrict__ orderB,
+                                                           const float4* __restrict__ tboxB, uint32_t nB,
                                                            const uint32_t* __restrict__ keysB, int mesh_shift,
                                                            const uint2* __restrict__ rootsA,
                                                            uint2* __restrict__ cand, uint32_t cap,
@@ -1382,11 +1370,10 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
     uint32_t b = 0;
     uint32_t node = kSentinel, end = kSentinel;
     if (k < nB) {
-        size_t leaf = (size_t)(nB - 1 + k);
-        float4 L = nodesB[2 * leaf], H = nodesB[2 * leaf + 1];
+        b = orderB[k];
+        float4 L = tboxB[2 * (size_t)b], H = tboxB[2 * (size_t)b + 1];
         qlx = L.x; qly = L.y; qlz = L.z;
         qhx = H.x; qhy = H.y; qhz = H.z;
-        b = __float_as_uint(L.w);
         node = 0;
         if (rootsA) {
             uint2 r = rootsA[keysB[k] >> mesh_shift];
@@ -1426,8 +1413,8 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
 template <int T, uint32_t S>
 static void traverse_as(MeshDev A, MeshDev B, const uint32_t* keysB, int mesh_shift, const uint2* rootsA,
                         uint2* cand, uint32_t cap, uint32_t* counters, hipStream_t s) {
-    k_traverse<T, S><<<blocks_for(B.nt, T), T, 0, s>>>(A.nodes, A.nt, B.nodes, B.nt, keysB, mesh_shift, rootsA, cand,
-                                                       cap, counters);
+    k_traverse<T, S><<<blocks_for(B.nt, T), T, 0, s>>>(A.nodes, A.nt, B.order, B.tbox, B.nt, keysB, mesh_shift, rootsA,
+                                                       cand, cap, counters);
 }
 
 void launch_traverse(MeshDev A, MeshDev B, const uint32_t* keysB, int mesh_shift, const uint2* rootsA, uint2* cand,

@@ -181,8 +181,8 @@ int gi_intersect_batch(gi_context* ctx, const gi_options* opt, gi_result* out, u
 /* Makes dst's slot a read-only view of src's (built) slot: no copy; src must
  * outlive dst's use of it. Used to serve many queries against one resident
  * BVH (no reference counterpart: the reference keeps one controller per pair).
- * A B slot keeps only its BVH's leaf nodes (B's leaves are the queries); shared
- * into an A slot, src's B builds its whole tree first (GI_ERR_INVALID_ARGUMENT
+ * A B slot builds no BVH (its triangles, in Morton order, are the queries);
+ * shared into an A slot, src's B builds its tree first (GI_ERR_INVALID_ARGUMENT
  * when src's B slot is itself a view). */
 int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot);
 

@@ -91,15 +91,15 @@ def test_unknown_mesh_and_ticket_are_rejected():
 
 
 def test_mesh_b_shared_into_an_a_slot_gets_its_whole_tree():
-    # a B slot keeps only its BVH's leaf nodes; shared into another context's
-    # A slot, whose tree the traversal walks, it must build the rest first
+    # a B slot builds no BVH; shared into another context's A slot, whose tree
+    # the traversal walks, it must build its tree first
     (xa, ta), (xb, tb) = mg.sphere_torus(128, 64)
     owner = gi.Intersector(device=0)
     user = gi.Intersector(device=0)
     try:
         owner.set_mesh(0, xb, tb)
         owner.set_mesh(1, xa, ta)
-        first = owner.intersect()  # B = the sphere, leaf nodes only
+        first = owner.intersect()  # B = the sphere, no tree
         assert gi.lib().gi_share_mesh(user._ctx, 0, owner._ctx, 1) == 0
         user.set_mesh(1, xb, tb)
         got = user.intersect()
