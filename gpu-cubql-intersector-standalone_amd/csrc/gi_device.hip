@@ -1348,6 +1348,69 @@ __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t fill, uint2* 
     wave_sync_lds();
 }
 
+__device__ __forceinline__ bool box_overlap(const float* c, float4 L, float4 H) {
+    return (L.x <= c[3]) & (c[0] <= H.x) & (L.y <= c[4]) & (c[1] <= H.y) & (L.z <= c[5]) & (c[2] <= H.z);
+}
+
+// Wave-shared descent of k_traverse. When every lane with a walk to do starts
+// at the same node, the union box of their queries walks down from it while
+// exactly one child of the current node overlaps the union: a child the union
+// misses holds no leaf any lane's query overlaps, so every lane's candidates
+// lie below the other. The lanes then start their own walks at the node where
+// both children overlap the union (a leaf stops the descent too), each ending
+// at that node's rope, or end at once where the union misses the node or both
+// its children. The node index is wave-uniform, so the descent's node loads
+// are scalar and leave the vector memory path (whose per-lane return is what
+// a lane walk pays per node) to the lanes' own walks. Forest walks take part
+// when a wave's queries belong to one mesh (the same start and end rope).
+__device__ __forceinline__ void shared_descent(const float4* __restrict__ nodes, uint32_t leafBase, float qlx,
+                                               float qly, float qlz, float qhx, float qhy, float qhz,
+                                               uint32_t& node, uint32_t& end) {
+    const bool walks = node != kSentinel;  // node == end was folded into kSentinel
+    if (!__any(walks)) return;
+    const uint32_t lead = (uint32_t)__builtin_ctzll(__ballot(walks));
+    const uint32_t s0 = __shfl(node, (int)lead), e0 = __shfl(end, (int)lead);
+    if (!__all(!walks || (node == s0 && end == e0))) return;
+    float c[6] = {walks ? qlx : INFINITY, walks ? qly : INFINITY, walks ? qlz : INFINITY,
+                  walks ? qhx : -INFINITY, walks ? qhy : -INFINITY, walks ? qhz : -INFINITY};
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) c[k] = fminf(c[k], __shfl_xor(c[k], o));
+#pragma unroll
+        for (int k = 3; k < 6; ++k) c[k] = fmaxf(c[k], __shfl_xor(c[k], o));
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) c[k] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__float_as_uint(c[k])));
+    uint32_t v = __builtin_amdgcn_readfirstlane(s0);
+    const uint32_t vend = __builtin_amdgcn_readfirstlane(e0);
+    float4 L = nodes[2 * (size_t)v], H = nodes[2 * (size_t)v + 1];
+    bool none = !box_overlap(c, L, H);
+    uint32_t vrope = vend;
+    while (!none && v < leafBase) {
+        const uint32_t left = __builtin_amdgcn_readfirstlane(__float_as_uint(L.w));
+        const float4 lL = nodes[2 * (size_t)left], lH = nodes[2 * (size_t)left + 1];
+        const uint32_t right = __builtin_amdgcn_readfirstlane(__float_as_uint(lH.w));  // left's rope
+        const float4 rL = nodes[2 * (size_t)right], rH = nodes[2 * (size_t)right + 1];
+        const bool ol = box_overlap(c, lL, lH), orr = box_overlap(c, rL, rH);
+        if (ol && orr) break;
+        if (!ol && !orr) {
+            none = true;
+            break;
+        }
+        v = ol ? left : right;
+        L = ol ? lL : rL;
+        H = ol ? lH : rH;
+        vrope = __builtin_amdgcn_readfirstlane(__float_as_uint(H.w));
+    }
+    if (!walks) return;
+    if (none) {
+        node = end = kSentinel;
+    } else {
+        node = v;
+        end = v == s0 ? vend : vrope;
+    }
+}
+
 // One thread per triangle of B, in B's Morton order (orderB, so a wave
 // queries a compact cluster; B needs no tree of its own), its box read from
 // tboxB; stackless traversal of A along the ropes. Overlapping leaf pairs
@@ -1385,13 +1448,16 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
     uint64_t lt = lanemask_lt();
     uint32_t fill = 0;
     if (node == end) node = end = kSentinel;
+    shared_descent(nodesA, leafBase, qlx, qly, qlz, qhx, qhy, qhz, node, end);
     // a walk ends at its end rope; the rope chain itself ends in kSentinel
     while (__any(node != end && node != kSentinel)) {
         bool hit = false;
         uint32_t a = 0;
         if (node != end && node != kSentinel) {
             float4 L = nodesA[2 * (size_t)node], H = nodesA[2 * (size_t)node + 1];
-            bool ov = L.x <= qhx && qlx <= H.x && L.y <= qhy && qly <= H.y && L.z <= qhz && qlz <= H.z;
+            // all six compares unconditionally (&, not &&): the node's two
+            // float4 loads then stay two 16-byte loads
+            const bool ov = (L.x <= qhx) & (qlx <= H.x) & (L.y <= qhy) & (qly <= H.y) & (L.z <= qhz) & (qlz <= H.z);
             uint32_t w0 = __float_as_uint(L.w), w1 = __float_as_uint(H.w);
             if (ov && node >= leafBase) { hit = true; a = w0; }
             node = (ov && node < leafBase) ? w0 : w1;
