@@ -99,7 +99,7 @@ typedef struct {
     int32_t retries;            /* capacity regrowths during this call */
     /* milliseconds (GPU events; host stage by wall clock) */
     float ms_total, ms_build, ms_traverse, ms_filter, ms_exact, ms_output, ms_host;
-    float ms_bvh_chunks;        /* the k_build_chunks launch of this call (both trees in one launch) */
+    float ms_bvh_chunks;        /* the k_build_chunks launch of this call (A's tree; a B shared into an A slot builds in the same launch) */
     int32_t bvh_chunk_launches; /* 0 or 1 */
     uint32_t bvh_chunk_leaves;  /* leaves that launch built or refit */
     int32_t graph;              /* 1: the launches up to the counter read-back replayed a HIP graph */
