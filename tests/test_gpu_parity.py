@@ -176,3 +176,33 @@ def test_empty_and_single_triangle_meshes():
     assert len(res["pairs"]) == 1
     empty = run_gpu(xa, np.zeros((0, 3), dtype=np.uint32), xb, ta)
     assert len(empty["pairs"]) == 0
+
+
+@pytest.mark.parametrize("na", [2, 3, 5])
+def test_tiny_tree_against_many_queries(na):
+    # a tree of 2-5 triangles (its root's children are leaves or one level
+    # above them) walked by full waves of queries: the wave-shared descent
+    # stops at once or after one level, and the lanes finish the walk
+    (xa, ta), (xb, tb) = mg.sphere_torus(96, 48)
+    hits = oracle_py.intersect(xa, ta, xb, tb, nthreads=8, want_segments=False)["pairs"]
+    picked = np.unique(hits[:, 0])[:: max(1, len(np.unique(hits[:, 0])) // na)][:na]  # A triangles that cross B
+    sel = np.ascontiguousarray(ta[picked])
+    used, inv = np.unique(sel, return_inverse=True)
+    xa2, ta2 = np.ascontiguousarray(xa[used]), inv.reshape(-1, 3).astype(np.uint32)
+    res, _ = _check(xa2, ta2, xb, tb, True)
+    assert len(res["pairs"]) >= na
+
+
+def test_queries_that_miss_the_tree():
+    # B far beside A: every wave's union box misses A's root (the descent ends
+    # the whole wave); then B inside A's box but off its surface: unions that
+    # miss both children of a node end there too
+    (xa, ta), (xb, tb) = mg.sphere_torus(96, 48)
+    far = np.rint(0.3 * (xb - xb.mean(axis=0)) + 0.4 * mg.GRID_SCALE)  # a small torus in the grid's corner
+    assert far.min() >= 0 and far.max() < xa.min()
+    res, ref = _check(xa, ta, far, tb, True)
+    assert len(res["pairs"]) == 0 and res["stats"]["candidates"] == 0
+    c = xa.mean(axis=0)
+    small = np.rint(c + 0.2 * (xb - xb.mean(axis=0)))  # a torus a fifth the size, at the sphere's centre
+    res, ref = _check(xa, ta, small, tb, True)
+    assert len(res["pairs"]) == 0
