@@ -12,6 +12,12 @@ exact segments, sort of the intersecting pairs and gather of their segments
 (output left in device memory).
 
 value = exact candidate pairs decided per second (candidates / step time).
+roofline = HBM roofline of the dominant kernel, the longer of the step's two
+largest single launches (the traversal and A's BVH build), each timed by the
+library's HIP events around its one launch per step; the other one is
+reported under roofline_other. Achieved bandwidth = the kernel's algorithmic
+bytes per launch / its average duration; traffic = HBM bytes per launch from
+the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload.
 The CPU baseline is the oracle (C restatement of the reference CPU path:
 box sweep + filtered exact predicates + exact segments) on the same meshes,
 timed on this host's cores (median of 5 runs after a warm-up).
