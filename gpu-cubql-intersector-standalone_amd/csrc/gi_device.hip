@@ -1357,9 +1357,12 @@ __device__ __forceinline__ bool box_overlap(const float* c, float4 L, float4 H) 
 // exactly one child of the current node overlaps the union: a child the union
 // misses holds no leaf any lane's query overlaps, so every lane's candidates
 // lie below the other. The lanes then start their own walks at the node where
-// both children overlap the union (a leaf stops the descent too), each ending
-// at that node's rope, or end at once where the union misses the node or both
-// its children. The node index is wave-uniform, so the descent's node loads
+// both children overlap the union (a leaf stops the descent too): a lane
+// whose own box overlaps both children starts at the left one and ends at the
+// node's rope, one that overlaps one child walks that child's subtree only,
+// and one that overlaps neither ends there (the two child boxes are already
+// loaded). All lanes end at once where the union misses the node or both its
+// children. The node index is wave-uniform, so the descent's node loads
 // are scalar and leave the vector memory path (whose per-lane return is what
 // a lane walk pays per node) to the lanes' own walks. Forest walks take part
 // when a wave's queries belong to one mesh (the same start and end rope).
@@ -1386,13 +1389,21 @@ __device__ __forceinline__ void shared_descent(const float4* __restrict__ nodes,
     float4 L = nodes[2 * (size_t)v], H = nodes[2 * (size_t)v + 1];
     bool none = !box_overlap(c, L, H);
     uint32_t vrope = vend;
+    bool forked = false;
+    uint32_t fl = 0, fr = 0;
+    float4 flL, flH, frL, frH;
     while (!none && v < leafBase) {
         const uint32_t left = __builtin_amdgcn_readfirstlane(__float_as_uint(L.w));
         const float4 lL = nodes[2 * (size_t)left], lH = nodes[2 * (size_t)left + 1];
         const uint32_t right = __builtin_amdgcn_readfirstlane(__float_as_uint(lH.w));  // left's rope
         const float4 rL = nodes[2 * (size_t)right], rH = nodes[2 * (size_t)right + 1];
         const bool ol = box_overlap(c, lL, lH), orr = box_overlap(c, rL, rH);
-        if (ol && orr) break;
+        if (ol && orr) {
+            forked = true;
+            fl = left; fr = right;
+            flL = lL; flH = lH; frL = rL; frH = rH;
+            break;
+        }
         if (!ol && !orr) {
             none = true;
             break;
@@ -1403,11 +1414,17 @@ __device__ __forceinline__ void shared_descent(const float4* __restrict__ nodes,
         vrope = __builtin_amdgcn_readfirstlane(__float_as_uint(H.w));
     }
     if (!walks) return;
+    const uint32_t vout = v == s0 ? vend : vrope;  // the rope leaving v
     if (none) {
         node = end = kSentinel;
+    } else if (forked) {
+        const float q[6] = {qlx, qly, qlz, qhx, qhy, qhz};
+        const bool a = box_overlap(q, flL, flH), b = box_overlap(q, frL, frH);
+        node = a ? fl : (b ? fr : kSentinel);
+        end = a ? (b ? vout : fr) : (b ? vout : kSentinel);  // fr is the left child's rope
     } else {
         node = v;
-        end = v == s0 ? vend : vrope;
+        end = vout;
     }
 }
 
