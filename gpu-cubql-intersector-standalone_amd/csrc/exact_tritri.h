@@ -20,8 +20,13 @@
 // * Zero-area triangles are the segment or point they span.
 //
 // The algorithm is written once over a number policy P: the GPU runs it on
-// int64/int128/I256 for grid coordinates (|x| <= 2^26), the host residual
-// path on arbitrary-precision integers for general doubles.
+// int64/int128/I256 for grid coordinates (|x| <= 2^26) and on
+// int128/W256/M512 for general doubles scaled to integers below 2^61; the
+// host residual path runs it on arbitrary-precision integers for the pairs
+// beyond that.
+//
+// Every result also counts the orient3d and orient2d signs it evaluated
+// (npred): the cascade's predicates per stage.
 #pragma once
 
 #include "gi_int.h"
@@ -50,6 +55,26 @@ struct GridPolicy {
     }
 };
 
+// General doubles on the GPU: a pair's coordinates as integers with
+// |x| < 2^61 (see gi_int.h for the widths this bound gives).
+struct WidePolicy {
+    typedef i128 C;
+    typedef W256 B;
+    typedef M512 H;
+    static GI_HD B toB(C x) { return w256_from(x); }
+    static GI_HD B mulCC(C a, C b) { return w256_mul(a, b); }
+    static GI_HD B mulBC(const B& a, C b) { return w256_mul(a, b); }
+    static GI_HD H mulBB(const B& a, const B& b) { return m512_mul(a, b); }
+    static GI_HD int cmpH(const H& a, const H& b) { return m512_cmp(a, b); }
+    static GI_HD int sgnC(C x) { return sgn128(x); }
+    static GI_HD int sgnB(const B& x) { return w256_sgn(x); }
+    static GI_HD int cmpAbsC(C a, C b) {
+        u128 ua = uabs128(a), ub = uabs128(b);
+        return (ua > ub) - (ua < ub);
+    }
+    static GI_HD int cmpAbsB(const B& a, const B& b) { return w256_cmp_abs(a, b); }
+};
+
 enum { GI_SEG_MAX = 6 };
 
 template <class P>
@@ -69,6 +94,7 @@ struct ExactTT {
     // pair met by intersect_fast) nothing else is valid.
     struct Fast {
         int hit, coplanar, degenerate, nseg;
+        int npred;              // orient3d and orient2d signs evaluated
         int point_from_piece;   // coplanar single-point overlap: index into pc
         int deferred;
         HP p, q;                // the single segment of non-coplanar/degenerate pairs
@@ -153,6 +179,7 @@ struct ExactTT {
     // Running extremes of points along axis k.
     struct Span {
         int n;
+        int npred;  // signs evaluated while collecting the pieces
         HP lo, hi;
     };
     static GI_HD void spanAdd(Span& s, const HP& x, int k) {
@@ -165,8 +192,9 @@ struct ExactTT {
     // ----------------------------------------------------- 2D clipping
     // Clip the segment p + t (q - p), t in [0, 1], by the closed triangle T in
     // the (ia, ib) projection. Returns 0 (empty), 1 (one point t0), 2 (t0 < t1).
-    static GI_HD int clip2d(const V3& p, const V3& q, const V3* T, int ia, int ib, Rat& t0, Rat& t1) {
+    static GI_HD int clip2d(const V3& p, const V3& q, const V3* T, int ia, int ib, Rat& t0, Rat& t1, int& np) {
         int sigma = P::sgnC(orient2d(T[0], T[1], T[2], ia, ib));
+        np += 1;
         t0.n = C(0); t0.d = C(1);
         t1.n = C(1); t1.d = C(1);
 #pragma unroll
@@ -175,6 +203,7 @@ struct ExactTT {
             const V3& v = T[e == 2 ? 0 : e + 1];
             C g0 = orient2d(u, v, p, ia, ib);
             C g1 = orient2d(u, v, q, ia, ib);
+            np += 2;
             if (sigma < 0) { g0 = -g0; g1 = -g1; }
             int s0 = P::sgnC(g0), s1 = P::sgnC(g1);
             if (s0 < 0 && s1 < 0) return 0;
@@ -221,13 +250,14 @@ struct ExactTT {
         B op = dotB(nT, sub(p, T[0]));
         B oq = dotB(nT, sub(q, T[0]));
         int sp = P::sgnB(op), sq = P::sgnB(oq);
+        r.npred += 2;
         if (sp * sq > 0) return 0;
         if (sp == 0 && sq == 0) {
             int ia, ib, k = dropAxis(nT);
             ia = k == 0 ? 1 : 0;
             ib = k == 2 ? 1 : 2;
             Rat t0, t1;
-            int kind = clip2d(p, q, T, ia, ib, t0, t1);
+            int kind = clip2d(p, q, T, ia, ib, t0, t1, r.npred);
             if (kind == 0) return 0;
             if (eq(p, q)) { r.p = hpInt(p); r.q = r.p; }
             else { r.p = hpOnEdge(p, q, t0); r.q = kind == 1 ? r.p : hpOnEdge(p, q, t1); }
@@ -237,6 +267,7 @@ struct ExactTT {
         int e0 = P::sgnB(orient3d(p, q, T[0], T[1]));
         int e1 = P::sgnB(orient3d(p, q, T[1], T[2]));
         int e2 = P::sgnB(orient3d(p, q, T[2], T[0]));
+        r.npred += 3;
         if (!((e0 >= 0 && e1 >= 0 && e2 >= 0) || (e0 <= 0 && e1 <= 0 && e2 <= 0))) return 0;
         r.p = sp == 0 ? hpInt(p) : (sq == 0 ? hpInt(q) : hpCrossing(p, q, op, oq));
         r.q = r.p;
@@ -258,11 +289,13 @@ struct ExactTT {
             const V3& a = pt1 ? s : p;
             const V3& b = pt1 ? t : q;
             V3 ab = sub(b, a), ax = sub(X, a), bx = sub(X, b);
+            r.npred += 1;
             if (!isZero(cross(ab, ax))) return 0;
             if (P::sgnB(dotB(ax, ab)) < 0 || P::sgnB(dotB(bx, ab)) > 0) return 0;
             r.p = hpInt(X); r.q = r.p; r.nseg = 1;
             return 1;
         }
+        r.npred += 1;
         if (P::sgnB(orient3d(p, q, s, t)) != 0) return 0;
         V3 pq = sub(q, p), st = sub(t, s);
         V3 c = cross(pq, st);
@@ -283,6 +316,7 @@ struct ExactTT {
         int ia = k == 0 ? 1 : 0, ib = k == 2 ? 1 : 2;
         C o1 = orient2d(p, q, s, ia, ib), o2 = orient2d(p, q, t, ia, ib);
         C o3 = orient2d(s, t, p, ia, ib), o4 = orient2d(s, t, q, ia, ib);
+        r.npred += 4;
         if (P::sgnC(o1) * P::sgnC(o2) > 0 || P::sgnC(o3) * P::sgnC(o4) > 0) return 0;
         r.p = hpCrossing(s, t, P::toB(o1), P::toB(o2));
         r.q = r.p; r.nseg = 1;
@@ -302,7 +336,7 @@ struct ExactTT {
                 int kk = dropAxis(nO);
                 int ia = kk == 0 ? 1 : 0, ib = kk == 2 ? 1 : 2;
                 Rat t0, t1;
-                int kind = clip2d(T[i], T[j], O, ia, ib, t0, t1);
+                int kind = clip2d(T[i], T[j], O, ia, ib, t0, t1, span.npred);
                 if (kind == 0) continue;
                 spanAdd(span, hpOnEdge(T[i], T[j], t0), k);
                 if (kind == 2) spanAdd(span, hpOnEdge(T[i], T[j], t1), k);
@@ -311,6 +345,7 @@ struct ExactTT {
             int e0 = P::sgnB(orient3d(T[i], T[j], O[0], O[1]));
             int e1 = P::sgnB(orient3d(T[i], T[j], O[1], O[2]));
             int e2 = P::sgnB(orient3d(T[i], T[j], O[2], O[0]));
+            span.npred += 3;
             if (!((e0 >= 0 && e1 >= 0 && e2 >= 0) || (e0 <= 0 && e1 <= 0 && e2 <= 0))) continue;
             HP X = s[i] == 0 ? hpInt(T[i]) : (s[j] == 0 ? hpInt(T[j]) : hpCrossing(T[i], T[j], o[i], o[j]));
             spanAdd(span, X, k);
@@ -327,6 +362,7 @@ struct ExactTT {
     template <class R, bool kCoplanar>
     static GI_HD void intersect_t(const V3* A, const V3* Bt, R& r) {
         r.hit = 0; r.coplanar = 0; r.degenerate = 0; r.nseg = 0; r.point_from_piece = -1; r.deferred = 0;
+        r.npred = 0;
         V3 nA = cross(sub(A[1], A[0]), sub(A[2], A[0]));
         V3 nB = cross(sub(Bt[1], Bt[0]), sub(Bt[2], Bt[0]));
         int a0, a1, b0, b1;
@@ -343,6 +379,7 @@ struct ExactTT {
         B oB[3], oA[3];
         int sB[3], sA[3];
         for (int i = 0; i < 3; ++i) { oB[i] = dotB(nA, sub(Bt[i], A[0])); sB[i] = P::sgnB(oB[i]); }
+        r.npred += 3;
         if (sB[0] == 0 && sB[1] == 0 && sB[2] == 0) {
             if constexpr (kCoplanar) coplanar(A, Bt, nA, r);
             else r.deferred = 1;
@@ -350,6 +387,7 @@ struct ExactTT {
         }
         if (sB[0] * sB[1] > 0 && sB[1] * sB[2] > 0) return;
         for (int i = 0; i < 3; ++i) { oA[i] = dotB(nB, sub(A[i], Bt[0])); sA[i] = P::sgnB(oA[i]); }
+        r.npred += 3;
         if (sA[0] * sA[1] > 0 && sA[1] * sA[2] > 0) return;
 
         // axis of the largest |component| of L's direction nA x nB
@@ -361,8 +399,10 @@ struct ExactTT {
         if (P::cmpAbsB(dir[2], k ? dir[1] : dir[0]) > 0) k = 2;
         Span span;
         span.n = 0;
+        span.npred = 0;
         edgesAgainst(A, Bt, nB, oA, sA, k, span);  // edges of A against triangle B
         edgesAgainst(Bt, A, nA, oB, sB, k, span);  // edges of B against triangle A
+        r.npred += span.npred;
         if (span.n == 0) return;
         r.hit = 1;
         r.nseg = 1;
@@ -386,11 +426,12 @@ struct ExactTT {
                     int g = f == 2 ? 0 : f + 1;
                     collinear = P::sgnC(orient2d(A[f], A[g], T[i], ia, ib)) == 0 &&
                                 P::sgnC(orient2d(A[f], A[g], T[j], ia, ib)) == 0;
+                    r.npred += 2;
                 }
                 if (collinear) {
                     // its overlap with T_A still decides intersection
                     Rat t0, t1;
-                    if (!any && clip2d(T[i], T[j], O, ia, ib, t0, t1)) {
+                    if (!any && clip2d(T[i], T[j], O, ia, ib, t0, t1, r.npred)) {
                         any = 1;
                         r.pc[GI_SEG_MAX - 1].edge = e; r.pc[GI_SEG_MAX - 1].t0 = t0; r.pc[GI_SEG_MAX - 1].t1 = t0;
                         r.point_from_piece = GI_SEG_MAX - 1;
@@ -399,7 +440,7 @@ struct ExactTT {
                 }
             }
             Rat t0, t1;
-            int kind = clip2d(T[i], T[j], O, ia, ib, t0, t1);
+            int kind = clip2d(T[i], T[j], O, ia, ib, t0, t1, r.npred);
             if (kind == 0) continue;
             if (kind == 1) {
                 if (!any) {

@@ -56,9 +56,10 @@ __device__ __forceinline__ int orient3d_cert(const d3& a, const d3& b, const d3&
 
 // edgeTriWithSigns (OriginShiftedFloatPredicateCore.cuh:70-85)
 __device__ __forceinline__ int edge_tri_cert(const d3& a, const d3& b, int sa, int sb, const d3& p, const d3& q,
-                                             const d3& r) {
+                                             const d3& r, int& np) {
     if (sa == 0 || sb == 0) return PAIR_YELLOW;
     if (sa == sb) return PAIR_NO;
+    np += 3;
     int e0 = orient3d_cert(a, b, p, q);
     int e1 = orient3d_cert(a, b, q, r);
     int e2 = orient3d_cert(a, b, r, p);
@@ -68,23 +69,26 @@ __device__ __forceinline__ int edge_tri_cert(const d3& a, const d3& b, int sa, i
 }
 
 // classifyLocal (OriginShiftedFloatPredicateCore.cuh:91-124), absolute
-// coordinates: the certified orient3d needs no anchor shift.
-__device__ __forceinline__ int classify_pair_fp64(const d3* A, const d3* B) {
+// coordinates: the certified orient3d needs no anchor shift. np counts the
+// orient3d evaluations.
+__device__ __forceinline__ int classify_pair_fp64(const d3* A, const d3* B, int& np) {
     int sB0 = orient3d_cert(A[0], A[1], A[2], B[0]);
     int sB1 = orient3d_cert(A[0], A[1], A[2], B[1]);
     int sB2 = orient3d_cert(A[0], A[1], A[2], B[2]);
+    np += 3;
     if (sB0 != 0 && sB0 == sB1 && sB1 == sB2) return PAIR_NO;
     int sA0 = orient3d_cert(B[0], B[1], B[2], A[0]);
     int sA1 = orient3d_cert(B[0], B[1], B[2], A[1]);
     int sA2 = orient3d_cert(B[0], B[1], B[2], A[2]);
+    np += 3;
     if (sA0 != 0 && sA0 == sA1 && sA1 == sA2) return PAIR_NO;
     int r;
-    r = edge_tri_cert(A[0], A[1], sA0, sA1, B[0], B[1], B[2]); if (r != PAIR_NO) return r;
-    r = edge_tri_cert(A[1], A[2], sA1, sA2, B[0], B[1], B[2]); if (r != PAIR_NO) return r;
-    r = edge_tri_cert(A[2], A[0], sA2, sA0, B[0], B[1], B[2]); if (r != PAIR_NO) return r;
-    r = edge_tri_cert(B[0], B[1], sB0, sB1, A[0], A[1], A[2]); if (r != PAIR_NO) return r;
-    r = edge_tri_cert(B[1], B[2], sB1, sB2, A[0], A[1], A[2]); if (r != PAIR_NO) return r;
-    r = edge_tri_cert(B[2], B[0], sB2, sB0, A[0], A[1], A[2]); if (r != PAIR_NO) return r;
+    r = edge_tri_cert(A[0], A[1], sA0, sA1, B[0], B[1], B[2], np); if (r != PAIR_NO) return r;
+    r = edge_tri_cert(A[1], A[2], sA1, sA2, B[0], B[1], B[2], np); if (r != PAIR_NO) return r;
+    r = edge_tri_cert(A[2], A[0], sA2, sA0, B[0], B[1], B[2], np); if (r != PAIR_NO) return r;
+    r = edge_tri_cert(B[0], B[1], sB0, sB1, A[0], A[1], A[2], np); if (r != PAIR_NO) return r;
+    r = edge_tri_cert(B[1], B[2], sB1, sB2, A[0], A[1], A[2], np); if (r != PAIR_NO) return r;
+    r = edge_tri_cert(B[2], B[0], sB2, sB0, A[0], A[1], A[2], np); if (r != PAIR_NO) return r;
     return PAIR_NO;
 }
 

@@ -146,7 +146,7 @@ struct gi_context {
     uint32_t cand_reserve = 0;
     MeshSlot mesh[2];
     DevBuf counters, hist, scan_tmp, partials;
-    DevBuf cand, exact, rec_key, rec_seg, seg_pool;
+    DevBuf cand, exact, rec_key, rec_seg, seg_pool, seg_exp, resid;
     uint32_t cap_cand = 0, cap_exact = 0, cap_rec = 0, cap_seg = 0;
     DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2, pair_off;
     std::vector<uint32_t> h_pair_off;  // per mesh pair result offsets of the last batch query
@@ -187,7 +187,7 @@ struct gi_context {
         xf_current[0] = xf_current[1] = false;
         for (auto& m : mesh) m.release();
         DevBuf* all[] = {&counters, &hist, &scan_tmp, &partials, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
-                         &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2, &pair_off,
+                         &seg_exp, &resid, &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2, &pair_off,
                          &bhist[0], &bhist[1], &bscan[0], &bscan[1], &bpart[0], &bpart[1]};
         for (DevBuf* b : all) b->release();
         cap_cand = cap_exact = cap_rec = cap_seg = 0;
@@ -462,13 +462,21 @@ MeshDev mesh_dev(const MeshSlot& m) {
 
 void ensure_arenas(gi_context* ctx, uint32_t cand, uint32_t exact, uint32_t rec, uint32_t seg) {
     if (cand > ctx->cap_cand) { ctx->cand.ensure<uint2>(cand); ctx->cap_cand = cand; }
-    if (exact > ctx->cap_exact) { ctx->exact.ensure<uint2>(exact); ctx->cap_exact = exact; }
+    if (exact > ctx->cap_exact) {
+        ctx->exact.ensure<uint2>(exact);
+        ctx->resid.ensure<uint2>(exact);  // every listed pair may be left to the host
+        ctx->cap_exact = exact;
+    }
     if (rec > ctx->cap_rec) {
         ctx->rec_key.ensure<uint64_t>(rec);
         ctx->rec_seg.ensure<uint2>(rec);
         ctx->cap_rec = rec;
     }
-    if (seg > ctx->cap_seg) { ctx->seg_pool.ensure<__int128>(8 * (size_t)seg); ctx->cap_seg = seg; }
+    if (seg > ctx->cap_seg) {
+        ctx->seg_pool.ensure<uint64_t>(kSegPoolWords * (size_t)seg);  // room for the wide form
+        ctx->seg_exp.ensure<int32_t>(seg);
+        ctx->cap_seg = seg;
+    }
 }
 
 uint32_t grow(uint64_t need, uint32_t cap) {
@@ -538,16 +546,18 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
                   ctx->tune.filter_blocks, s);
     if (timed) record(ctx, ctx->ev[3], s);
     launch_exact(A, B, ctx->exact.get<uint2>(), ctx->cap_exact, grid, bitsB, ctx->rec_key.get<uint64_t>(),
-                 ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<__int128>(), ctx->cap_seg,
-                 ctx->cand.get<uint2>(), ctx->cap_cand, counters, ctx->tune.exact_blocks, s);
+                 ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<uint64_t>(), ctx->seg_exp.get<int32_t>(),
+                 ctx->cap_seg, ctx->cand.get<uint2>(), ctx->cap_cand, ctx->resid.get<uint2>(), ctx->cap_exact,
+                 counters, ctx->tune.exact_blocks, s);
     if (timed) record(ctx, ctx->ev[4], s);
     // sorted output of a small result set, before the read-back (buffers
     // from ensure_output_bufs)
     launch_output_small(ctx->rec_key.get<uint64_t>(), ctx->rec_seg.get<uint2>(), counters, ctx->cap_rec, ctx->cap_seg,
                         bits_for(MA.nt) + bitsB, bitsB, o.want_segments != 0, ctx->tkeys.get<uint64_t>(),
                         ctx->pairs.get<uint2>(),
-                        ctx->order.get<uint32_t>(), ctx->offs.get<uint32_t>(), ctx->segs.get<__int128>(),
-                        ctx->f64.get<double>(), (uint32_t)output_seg_slots(ctx), ctx->seg_pool.get<__int128>(), s);
+                        ctx->order.get<uint32_t>(), ctx->offs.get<uint32_t>(), ctx->segs.get<uint64_t>(),
+                        ctx->exp2.get<int32_t>(), ctx->f64.get<double>(), (uint32_t)output_seg_slots(ctx),
+                        ctx->seg_pool.get<uint64_t>(), ctx->seg_exp.get<int32_t>(), grid, s);
     if (timed) record(ctx, ctx->ev[5], s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(ctx->h_counters, counters, sizeof(uint32_t) * (CNT_COUNT + 2), hipMemcpyDeviceToHost, s));
@@ -560,7 +570,8 @@ void ensure_output_bufs(gi_context* ctx, const gi_options& o) {
     ctx->order.ensure<uint32_t>(std::max(cap, 1u));
     ctx->offs.ensure<uint32_t>((size_t)cap + 1);
     if (o.want_segments) {
-        ctx->segs.ensure<__int128>(8 * output_seg_slots(ctx));
+        ctx->segs.ensure<uint64_t>(kSegPoolWords * output_seg_slots(ctx));
+        ctx->exp2.ensure<int32_t>(output_seg_slots(ctx));
         ctx->f64.ensure<double>(6 * output_seg_slots(ctx));
     }
 }
@@ -592,7 +603,7 @@ std::vector<uint64_t> presync_signature(gi_context* ctx, const gi_options& o) {
             buf(*b);
     }
     for (const DevBuf* b : {&ctx->counters, &ctx->xf_dev, &ctx->cand, &ctx->exact, &ctx->rec_key,
-                            &ctx->rec_seg, &ctx->seg_pool, &ctx->bhist[0], &ctx->bhist[1], &ctx->bscan[0],
+                            &ctx->rec_seg, &ctx->seg_pool, &ctx->seg_exp, &ctx->resid, &ctx->exp2, &ctx->bhist[0], &ctx->bhist[1], &ctx->bscan[0],
                             &ctx->bscan[1], &ctx->bpart[0], &ctx->bpart[1], &ctx->pairs, &ctx->order, &ctx->offs,
                             &ctx->segs, &ctx->f64, &ctx->tkeys})
         buf(*b);
@@ -682,6 +693,177 @@ bool launch_presync_graph(gi_context* ctx, const gi_options& o) {
     return true;
 }
 
+// Largest count a 32-bit candidate list can hold (capacities grow up to it).
+constexpr uint32_t kMaxListEntries = 0xFFFFFFF0u;
+
+// Decimal-free rendering of one host-stage number: little-endian two's
+// complement words as a double.
+double words_to_double(const uint64_t* x, int limbs) {
+    const bool neg = (x[limbs - 1] >> 63) != 0;
+    std::vector<uint64_t> m(x, x + limbs);
+    if (neg) {
+        uint64_t c = 1;
+        for (int l = 0; l < limbs; ++l) {
+            m[l] = ~m[l] + c;
+            c = (c && m[l] == 0) ? 1 : 0;
+        }
+    }
+    double v = 0.0;
+    for (int l = limbs - 1; l >= 0; --l) v = v * 18446744073709551616.0 + (double)m[l];
+    return neg ? -v : v;
+}
+
+// The end of a query whose exact stage left Q pairs to the host: the GPU's
+// sorted result (R pairs, S segments of L limbs, global indices) comes to the
+// host, the residual pairs are decided and constructed there
+// (host_exact_batch), and the two sorted lists are merged by (a, b). A batch
+// is split per mesh pair afterwards.
+int finish_with_residual(gi_context* ctx, const gi_options& o, gi_result* out, uint32_t Q, int L, uint32_t R,
+                         uint32_t S, const uint2* pairs, const uint32_t* offs, const uint64_t* segs,
+                         const int32_t* exp2, const double* f64) {
+    hipStream_t s = ctx->stream;
+    MeshSlot& MA = ctx->mesh[0];
+    MeshSlot& MB = ctx->mesh[1];
+    const uint32_t M = MA.forest;
+    gi_stats& st = out->stats;
+    record(ctx, ctx->ev[5], s);
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t words = 8 * (size_t)L;
+    std::vector<uint32_t> gp(2 * (size_t)R), go((size_t)R + 1), list(2 * (size_t)Q);
+    std::vector<uint64_t> gs(words * S);
+    std::vector<int32_t> ge(S, 0);
+    std::vector<double> vA(4 * (size_t)MA.nv), vB(4 * (size_t)MB.nv);
+    if (R) HIP_TRY(hipMemcpyAsync(gp.data(), pairs, 8 * (size_t)R, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(go.data(), offs, 4 * ((size_t)R + 1), hipMemcpyDeviceToHost, s));
+    if (S) {
+        HIP_TRY(hipMemcpyAsync(gs.data(), segs, 8 * words * S, hipMemcpyDeviceToHost, s));
+        if (!st.grid) HIP_TRY(hipMemcpyAsync(ge.data(), exp2, 4 * (size_t)S, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipMemcpyAsync(list.data(), ctx->resid.get<uint2>(), 8 * (size_t)Q, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(vA.data(), MA.v64.get<double4>(), 32 * (size_t)MA.nv, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(vB.data(), MB.v64.get<double4>(), 32 * (size_t)MB.nv, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<HostHit> hits;
+    uint64_t yes = 0, hpred = 0;
+    const int nth = ctx->host_threads > 0 ? ctx->host_threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    const int hst = host_exact_batch(vA.data(), MA.h_tri.data(), vB.data(), MB.h_tri.data(), list.data(), Q, nth, hits,
+                                     &yes, &hpred);
+    if (hst == kHostNonFinite) throw GiError{GI_ERR_INVALID_ARGUMENT, "a moved vertex coordinate is not finite"};
+    if (hst != 0) throw GiError{GI_ERR_INTERNAL, "FP64 filter certified a pair the exact stage rejects"};
+    int LL = L;
+    for (const HostHit& h : hits) LL = std::max(LL, h.limbs);
+    // merge the two (a, b)-sorted lists
+    HostResultStore* hs = new HostResultStore();
+    const size_t n = (size_t)R + hits.size();
+    hs->pairs.resize(2 * n);
+    hs->offs.resize(n + 1);
+    size_t nseg_all = S;
+    if (o.want_segments)
+        for (const HostHit& h : hits) nseg_all += (size_t)h.nseg;
+    else
+        nseg_all = 0;
+    hs->seg.assign(nseg_all * 8 * (size_t)LL, 0);
+    hs->exp2.assign(nseg_all, 0);
+    hs->f64.resize(6 * nseg_all);
+    auto put_number = [&](size_t si, int e, const uint64_t* src, int limbs) {
+        uint64_t* dst = &hs->seg[(si * 8 + e) * (size_t)LL];
+        const uint64_t ext = (src[limbs - 1] >> 63) ? ~0ull : 0ull;
+        for (int l = 0; l < LL; ++l) dst[l] = l < limbs ? src[l] : ext;
+    };
+    auto render = [&](size_t si) {
+        for (int e = 0; e < 2; ++e) {
+            const double w = words_to_double(&hs->seg[(si * 8 + 4 * e + 3) * (size_t)LL], LL);
+            for (int k = 0; k < 3; ++k)
+                hs->f64[6 * si + 3 * e + k] =
+                    ldexp(words_to_double(&hs->seg[(si * 8 + 4 * e + k) * (size_t)LL], LL) / w, hs->exp2[si]);
+        }
+    };
+    size_t gi_ = 0, hi = 0, si = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const bool take_gpu = hi >= hits.size() ||
+                              (gi_ < R && ((uint64_t)gp[2 * gi_] << 32 | gp[2 * gi_ + 1]) <
+                                              ((uint64_t)hits[hi].a << 32 | hits[hi].b));
+        hs->offs[k] = (uint32_t)si;
+        if (take_gpu) {
+            hs->pairs[2 * k] = gp[2 * gi_];
+            hs->pairs[2 * k + 1] = gp[2 * gi_ + 1];
+            if (o.want_segments)
+                for (uint32_t sg = go[gi_]; sg < go[gi_ + 1]; ++sg, ++si) {
+                    for (int e = 0; e < 8; ++e) put_number(si, e, &gs[(sg * 8 + (size_t)e) * L], L);
+                    hs->exp2[si] = ge[sg];
+                    render(si);
+                }
+            ++gi_;
+        } else {
+            const HostHit& h = hits[hi];
+            hs->pairs[2 * k] = h.a;
+            hs->pairs[2 * k + 1] = h.b;
+            st.coplanar += h.coplanar;
+            st.degenerate += h.degenerate;
+            if (o.want_segments)
+                for (int sg = 0; sg < h.nseg; ++sg, ++si) {
+                    for (int e = 0; e < 8; ++e) put_number(si, e, &h.data[((size_t)sg * 8 + e) * h.limbs], h.limbs);
+                    hs->exp2[si] = h.exp2;
+                    render(si);
+                }
+            ++hi;
+        }
+    }
+    hs->offs[n] = (uint32_t)si;
+    if (M) {  // batch: per mesh pair offsets, per-mesh indices
+        size_t m = 0;
+        ctx->h_pair_off[0] = 0;
+        for (size_t i = 0; i < n; ++i) {
+            while (m + 1 < M && MA.h_toff[m + 1] <= hs->pairs[2 * i]) ctx->h_pair_off[++m] = (uint32_t)i;
+            hs->pairs[2 * i] -= MA.h_toff[m];
+            hs->pairs[2 * i + 1] -= MB.h_toff[m];
+        }
+        while (m + 1 <= M) ctx->h_pair_off[++m] = (uint32_t)n;
+    }
+    st.host_pairs = Q;
+    st.host_yes = yes;
+    st.host_predicates = hpred;
+    st.pairs = n;
+    st.segments = nseg_all;
+    st.ms_host = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    st.ms_output = elapsed(ctx->ev[4], ctx->ev[5]);
+    st.ms_total = elapsed(ctx->ev[0], ctx->ev[5]) + st.ms_host;
+    out->n_pairs = n;
+    out->n_segments = nseg_all;
+    out->exact_limbs = LL;
+    out->pairs = hs->pairs.data();
+    out->seg_offsets = hs->offs.data();
+    out->seg_exact = hs->seg.data();
+    out->seg_exp2 = hs->exp2.data();
+    out->seg_f64 = hs->f64.data();
+    out->on_device = 0;
+    out->internal = hs;
+    if (o.output_on_device) {
+        // into the context's device result buffers
+        uint32_t* dp = ctx->pairs.ensure<uint32_t>(hs->pairs.size());
+        uint32_t* doffs = ctx->offs.ensure<uint32_t>(hs->offs.size());
+        uint64_t* dseg = ctx->segs.ensure<uint64_t>(hs->seg.size());
+        int32_t* dexp = ctx->exp2.ensure<int32_t>(hs->exp2.size());
+        double* df = ctx->f64.ensure<double>(hs->f64.size());
+        if (!hs->pairs.empty()) HIP_TRY(hipMemcpy(dp, hs->pairs.data(), 4 * hs->pairs.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(doffs, hs->offs.data(), 4 * hs->offs.size(), hipMemcpyHostToDevice));
+        if (nseg_all) {
+            HIP_TRY(hipMemcpy(dseg, hs->seg.data(), 8 * hs->seg.size(), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(dexp, hs->exp2.data(), 4 * hs->exp2.size(), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(df, hs->f64.data(), 8 * hs->f64.size(), hipMemcpyHostToDevice));
+        }
+        delete hs;
+        out->pairs = dp;
+        out->seg_offsets = doffs;
+        out->seg_exact = dseg;
+        out->seg_exp2 = dexp;
+        out->seg_f64 = df;
+        out->on_device = 1;
+        out->internal = nullptr;
+    }
+    return GI_OK;
+}
+
 int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     memset(out, 0, sizeof(*out));
     gi_options o = {0, 1, 0, 0};
@@ -722,6 +904,8 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
             enqueue_presync(ctx, o, false, true);
         }
         HIP_TRY(hipStreamSynchronize(s));
+        if (hc[CNT_OVERFLOW] || hc[CNT_TRAVERSAL_HITS] > kMaxListEntries)
+            throw GiError{GI_ERR_TOO_LARGE, "more candidate pairs than a 32-bit list can hold"};
         bool over = hc[CNT_TRAVERSAL_HITS] > ctx->cap_cand || hc[CNT_EXACT_LIST] > ctx->cap_exact ||
                     hc[CNT_RESULTS] > ctx->cap_rec || hc[CNT_SEGMENTS] > ctx->cap_seg;
         if (!over) break;
@@ -733,13 +917,14 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         ensure_arenas(ctx, nc, ne, nr, ns);
         st.retries++;
     }
-    MeshDev A = mesh_dev(MA), B = mesh_dev(MB);
     if (hc[CNT_UNSOUND]) throw GiError{GI_ERR_INTERNAL, "FP64 filter certified a pair the exact stage rejects"};
     st.traversal_hits = hc[CNT_TRAVERSAL_HITS];
     st.candidates = hc[CNT_CANDIDATES];
     st.filter_no = hc[CNT_FILTER_NO];
     st.filter_green = hc[CNT_FILTER_GREEN];
     st.filter_yellow = hc[CNT_FILTER_YELLOW];
+    st.filter_predicates = hc[CNT_FILTER_PRED];
+    st.exact_predicates = hc[CNT_EXACT_PRED];
     st.grid = hgrid[0] && hgrid[1];
     st.ms_build = elapsed(ctx->ev[0], ctx->ev[1]);
     if (ctx->chunk_timed) {
@@ -751,205 +936,107 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     st.ms_filter = elapsed(ctx->ev[2], ctx->ev[3]);
     st.graph = graphed ? 1 : 0;
 
-    if (st.grid) {
-        st.exact_pairs = hc[CNT_EXACT_LIST];
-        st.exact_yellow_yes = hc[CNT_EXACT_YELLOW_YES];
-        st.exact_yellow_no = hc[CNT_EXACT_YELLOW_NO];
-        st.coplanar = hc[CNT_COPLANAR];
-        st.degenerate = hc[CNT_DEGENERATE];
-        st.ms_exact = elapsed(ctx->ev[3], ctx->ev[4]);
-        uint32_t R = hc[CNT_RESULTS], S = o.want_segments ? hc[CNT_SEGMENTS] : 0;
-        // small result sets were sorted and gathered before the read-back
-        const bool done = output_small_fits(R, bitsA + bitsB);
-        if (!done) {
-            uint64_t* keys = ctx->rec_key.get<uint64_t>();
-            uint32_t* order = ctx->order.ensure<uint32_t>(R);
-            uint64_t* tk = ctx->tkeys.ensure<uint64_t>(R);
-            uint32_t* tv = ctx->tvals.ensure<uint32_t>(R);
-            uint32_t* hist = ctx->hist.ensure<uint32_t>(sort_hist_elems(R));
-            uint32_t* stmp = ctx->scan_tmp.ensure<uint32_t>(scan_tmp_elems(std::max(sort_hist_elems(R), (size_t)R + 1)));
-            uint2* pairs = ctx->pairs.ensure<uint2>(R);
-            uint32_t* nseg = ctx->nseg.ensure<uint32_t>(R);
-            uint32_t* offs = ctx->offs.ensure<uint32_t>((size_t)R + 1);
-            __int128* segs = ctx->segs.ensure<__int128>(8 * (size_t)S);
-            double* f64 = ctx->f64.ensure<double>(6 * (size_t)S);
-            launch_iota(order, R, s);
-            if (radix_sort_u64(keys, order, tk, tv, R, bitsA + bitsB, hist, stmp, s)) {
-                keys = tk;
-                order = tv;
-            }
-            launch_gather_pairs(keys, order, ctx->rec_seg.get<uint2>(), R, bitsB, pairs, nseg, s);
-            if (!o.want_segments) launch_fill_u32(nseg, 0u, R, s);
-            exclusive_scan_u32(nseg, offs, R, stmp, offs + R, s);
-            if (o.want_segments)
-                launch_gather_segments(order, ctx->rec_seg.get<uint2>(), offs, R, ctx->seg_pool.get<__int128>(), segs,
-                                       f64, s);
+    // the exact stage ran on the GPU: int64/int128 for grid meshes, the wide
+    // integer frames for general doubles; pairs that did not fit a frame are
+    // the residual, decided on the host and merged below
+    const uint32_t Q = hc[CNT_RESIDUAL];
+    const int L = st.grid ? kSegLimbsGrid : kSegLimbsWide;
+    st.exact_pairs = hc[CNT_EXACT_LIST] - Q;
+    st.exact_yellow_yes = hc[CNT_EXACT_YELLOW_YES];
+    st.exact_yellow_no = hc[CNT_EXACT_YELLOW_NO];
+    st.coplanar = hc[CNT_COPLANAR];
+    st.degenerate = hc[CNT_DEGENERATE];
+    st.ms_exact = elapsed(ctx->ev[3], ctx->ev[4]);
+    uint32_t R = hc[CNT_RESULTS], S = o.want_segments ? hc[CNT_SEGMENTS] : 0;
+    const size_t words = 8 * (size_t)L;
+    // small result sets were sorted and gathered before the read-back
+    const bool done = output_small_fits(R, bitsA + bitsB);
+    if (!done) {
+        uint64_t* keys = ctx->rec_key.get<uint64_t>();
+        uint32_t* order = ctx->order.ensure<uint32_t>(R);
+        uint64_t* tk = ctx->tkeys.ensure<uint64_t>(R);
+        uint32_t* tv = ctx->tvals.ensure<uint32_t>(R);
+        uint32_t* hist = ctx->hist.ensure<uint32_t>(sort_hist_elems(R));
+        uint32_t* stmp = ctx->scan_tmp.ensure<uint32_t>(scan_tmp_elems(std::max(sort_hist_elems(R), (size_t)R + 1)));
+        uint2* pairs = ctx->pairs.ensure<uint2>(R);
+        uint32_t* nseg = ctx->nseg.ensure<uint32_t>(R);
+        uint32_t* offs = ctx->offs.ensure<uint32_t>((size_t)R + 1);
+        uint64_t* segs = ctx->segs.ensure<uint64_t>(words * S);
+        int32_t* exp2 = ctx->exp2.ensure<int32_t>(S);
+        double* f64 = ctx->f64.ensure<double>(6 * (size_t)S);
+        launch_iota(order, R, s);
+        if (radix_sort_u64(keys, order, tk, tv, R, bitsA + bitsB, hist, stmp, s)) {
+            keys = tk;
+            order = tv;
         }
-        if (done && S > output_seg_slots(ctx)) {  // more segments than the slots the device gather had
-            __int128* segs = ctx->segs.ensure<__int128>(8 * (size_t)S);
-            double* f64 = ctx->f64.ensure<double>(6 * (size_t)S);
-            launch_gather_segments(ctx->order.get<uint32_t>(), ctx->rec_seg.get<uint2>(), ctx->offs.get<uint32_t>(), R,
-                                   ctx->seg_pool.get<__int128>(), segs, f64, s);
-        }
-        const bool host_work = !done || S > output_seg_slots(ctx) || M;
-        uint2* pairs = ctx->pairs.get<uint2>();
-        uint32_t* offs = ctx->offs.get<uint32_t>();
-        __int128* segs = ctx->segs.get<__int128>();
-        double* f64 = ctx->f64.get<double>();
-        if (M) {
-            uint32_t* poff = ctx->pair_off.ensure<uint32_t>((size_t)M + 1);
-            launch_forest_localize(pairs, R, MA.toff.get<uint32_t>(), MB.toff.get<uint32_t>(), M, poff, s);
-            HIP_TRY(hipMemcpyAsync(ctx->h_pair_off.data(), poff, 4 * ((size_t)M + 1), hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));  // pageable destination
-        }
-        if (host_work) record(ctx, ctx->ev[5], s);
-        HIP_TRY(hipGetLastError());
-        st.pairs = R;
-        st.segments = S;
-        out->n_pairs = R;
-        out->n_segments = S;
-        out->exact_limbs = 2;
-        if (o.output_on_device) {
-            HIP_TRY(hipEventSynchronize(ctx->ev[5]));
-            out->on_device = 1;
-            out->pairs = (uint32_t*)pairs;
-            out->seg_offsets = offs;
-            out->seg_exact = (uint64_t*)segs;
-            out->seg_exp2 = nullptr;
-            out->seg_f64 = f64;
-            out->internal = nullptr;
-        } else {
-            HostResultStore* hs = new HostResultStore();
-            hs->pairs.resize(2 * (size_t)R);
-            hs->offs.resize((size_t)R + 1);
-            hs->seg.resize(16 * (size_t)S);
-            hs->f64.resize(6 * (size_t)S);
-            if (R) HIP_TRY(hipMemcpyAsync(hs->pairs.data(), pairs, 8 * (size_t)R, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(hs->offs.data(), offs, 4 * ((size_t)R + 1), hipMemcpyDeviceToHost, s));
-            if (S) {
-                HIP_TRY(hipMemcpyAsync(hs->seg.data(), segs, 128 * (size_t)S, hipMemcpyDeviceToHost, s));
-                HIP_TRY(hipMemcpyAsync(hs->f64.data(), f64, 48 * (size_t)S, hipMemcpyDeviceToHost, s));
-            }
-            HIP_TRY(hipStreamSynchronize(s));
-            out->on_device = 0;
-            out->pairs = hs->pairs.data();
-            out->seg_offsets = hs->offs.data();
-            out->seg_exact = hs->seg.data();
-            out->seg_exp2 = nullptr;
-            out->seg_f64 = hs->f64.data();
-            out->internal = hs;
-        }
-        st.ms_output = elapsed(ctx->ev[4], ctx->ev[5]);
-        st.ms_total = elapsed(ctx->ev[0], ctx->ev[5]);
-        return GI_OK;
+        launch_gather_pairs(keys, order, ctx->rec_seg.get<uint2>(), R, bitsB, pairs, nseg, s);
+        if (!o.want_segments) launch_fill_u32(nseg, 0u, R, s);
+        exclusive_scan_u32(nseg, offs, R, stmp, offs + R, s);
+        if (o.want_segments)
+            launch_gather_segments(order, ctx->rec_seg.get<uint2>(), offs, R, ctx->seg_pool.get<uint64_t>(),
+                                   ctx->seg_exp.get<int32_t>(), L, segs, exp2, f64, s);
     }
-
-    // General doubles: the GREEN/YELLOW list goes to the host residual stage.
-    auto t0 = std::chrono::steady_clock::now();
-    uint32_t n = hc[CNT_EXACT_LIST];
-    std::vector<uint32_t> list(2 * (size_t)n);
-    std::vector<double> vA(4 * (size_t)MA.nv), vB(4 * (size_t)MB.nv);
-    if (n) HIP_TRY(hipMemcpyAsync(list.data(), ctx->exact.get<uint2>(), 8 * (size_t)n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(vA.data(), MA.v64.get<double4>(), 32 * (size_t)MA.nv, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(vB.data(), MB.v64.get<double4>(), 32 * (size_t)MB.nv, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    std::vector<HostHit> hits;
-    uint64_t yes = 0;
-    int nth = ctx->host_threads > 0 ? ctx->host_threads : (int)std::max(1u, std::thread::hardware_concurrency());
-    if (host_exact_batch(vA.data(), MA.h_tri.data(), vB.data(), MB.h_tri.data(), list.data(), n, nth, hits, &yes) != 0)
-        throw GiError{GI_ERR_INTERNAL, "host residual stage failed (overflow or unsound filter)"};
-    if (M) {  // batch: per mesh pair offsets, per-mesh indices
-        size_t m = 0;
-        for (size_t i = 0; i < hits.size(); ++i) {
-            while (m + 1 < M && MA.h_toff[m + 1] <= hits[i].a) ctx->h_pair_off[++m] = (uint32_t)i;
-            hits[i].a -= MA.h_toff[m];
-            hits[i].b -= MB.h_toff[m];
-        }
-        while (m + 1 <= M) ctx->h_pair_off[++m] = (uint32_t)hits.size();
+    if (done && S > output_seg_slots(ctx)) {  // more segments than the slots the device gather had
+        uint64_t* segs = ctx->segs.ensure<uint64_t>(words * S);
+        int32_t* exp2 = ctx->exp2.ensure<int32_t>(S);
+        double* f64 = ctx->f64.ensure<double>(6 * (size_t)S);
+        launch_gather_segments(ctx->order.get<uint32_t>(), ctx->rec_seg.get<uint2>(), ctx->offs.get<uint32_t>(), R,
+                               ctx->seg_pool.get<uint64_t>(), ctx->seg_exp.get<int32_t>(), L, segs, exp2, f64, s);
     }
-    HostResultStore* hs = new HostResultStore();
-    int limbs = 1;
-    for (auto& h : hits) limbs = std::max(limbs, h.limbs);
-    hs->pairs.resize(2 * hits.size());
-    hs->offs.resize(hits.size() + 1);
-    size_t S = 0;
-    for (size_t i = 0; i < hits.size(); ++i) {
-        hs->pairs[2 * i] = hits[i].a;
-        hs->pairs[2 * i + 1] = hits[i].b;
-        hs->offs[i] = (uint32_t)S;
-        if (o.want_segments) S += (size_t)hits[i].nseg;
-        st.coplanar += hits[i].coplanar;
-        st.degenerate += hits[i].degenerate;
+    uint2* pairs = ctx->pairs.get<uint2>();
+    uint32_t* offs = ctx->offs.get<uint32_t>();
+    uint64_t* segs = ctx->segs.get<uint64_t>();
+    int32_t* exp2 = ctx->exp2.get<int32_t>();
+    double* f64 = ctx->f64.get<double>();
+    if (Q) return finish_with_residual(ctx, o, out, Q, L, R, S, pairs, offs, segs, exp2, f64);
+    const bool host_work = !done || S > output_seg_slots(ctx) || M;
+    if (M) {
+        uint32_t* poff = ctx->pair_off.ensure<uint32_t>((size_t)M + 1);
+        launch_forest_localize(pairs, R, MA.toff.get<uint32_t>(), MB.toff.get<uint32_t>(), M, poff, s);
+        HIP_TRY(hipMemcpyAsync(ctx->h_pair_off.data(), poff, 4 * ((size_t)M + 1), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));  // pageable destination
     }
-    hs->offs[hits.size()] = (uint32_t)S;
-    hs->seg.assign(S * 8 * (size_t)limbs, 0);
-    hs->exp2.resize(S);
-    hs->f64.resize(6 * S);
-    if (o.want_segments) {
-        size_t si = 0;
-        for (auto& h : hits)
-            for (int sg = 0; sg < h.nseg; ++sg, ++si) {
-                hs->exp2[si] = h.exp2;
-                double wv[2];
-                for (int e = 0; e < 8; ++e) {
-                    const uint64_t* src = &h.data[((size_t)sg * 8 + e) * h.limbs];
-                    uint64_t* dst = &hs->seg[(si * 8 + e) * (size_t)limbs];
-                    uint64_t ext = (src[h.limbs - 1] >> 63) ? ~0ull : 0ull;
-                    for (int l = 0; l < limbs; ++l) dst[l] = l < h.limbs ? src[l] : ext;
-                }
-                // double rendering: value(h)/value(w) * 2^exp2
-                for (int e = 0; e < 2; ++e) {
-                    auto val = [&](int idx) {
-                        const uint64_t* x = &hs->seg[(si * 8 + 4 * e + idx) * (size_t)limbs];
-                        bool neg = x[limbs - 1] >> 63;
-                        double v = 0.0;
-                        for (int l = limbs - 1; l >= 0; --l) v = v * 18446744073709551616.0 + (double)(neg ? ~x[l] : x[l]);
-                        return neg ? -(v + 1.0) : v;
-                    };
-                    wv[e] = val(3);
-                    for (int k = 0; k < 3; ++k) hs->f64[6 * si + 3 * e + k] = ldexp(val(k) / wv[e], h.exp2);
-                }
-            }
-    }
-    st.host_pairs = n;
-    st.host_yes = yes;
-    st.pairs = hits.size();
+    if (host_work) record(ctx, ctx->ev[5], s);
+    HIP_TRY(hipGetLastError());
+    st.pairs = R;
     st.segments = S;
-    st.ms_host = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    st.ms_total = elapsed(ctx->ev[0], ctx->ev[4]) + st.ms_host;
-    out->n_pairs = hits.size();
+    out->n_pairs = R;
     out->n_segments = S;
-    out->exact_limbs = limbs;
-    out->pairs = hs->pairs.data();
-    out->seg_offsets = hs->offs.data();
-    out->seg_exact = hs->seg.data();
-    out->seg_exp2 = hs->exp2.data();
-    out->seg_f64 = hs->f64.data();
-    out->on_device = 0;
-    out->internal = hs;
+    out->exact_limbs = L;
     if (o.output_on_device) {
-        // copy into the context's device result buffers
-        uint32_t* dp = ctx->pairs.ensure<uint32_t>(hs->pairs.size());
-        uint32_t* doffs = ctx->offs.ensure<uint32_t>(hs->offs.size());
-        uint64_t* dseg = (uint64_t*)ctx->segs.ensure<uint64_t>(hs->seg.size());
-        int32_t* dexp = ctx->exp2.ensure<int32_t>(hs->exp2.size());
-        double* df = ctx->f64.ensure<double>(hs->f64.size());
-        if (!hs->pairs.empty()) HIP_TRY(hipMemcpy(dp, hs->pairs.data(), 4 * hs->pairs.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(doffs, hs->offs.data(), 4 * hs->offs.size(), hipMemcpyHostToDevice));
-        if (S) {
-            HIP_TRY(hipMemcpy(dseg, hs->seg.data(), 8 * hs->seg.size(), hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(dexp, hs->exp2.data(), 4 * hs->exp2.size(), hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(df, hs->f64.data(), 8 * hs->f64.size(), hipMemcpyHostToDevice));
-        }
-        delete hs;
-        out->pairs = dp;
-        out->seg_offsets = doffs;
-        out->seg_exact = dseg;
-        out->seg_exp2 = dexp;
-        out->seg_f64 = df;
+        HIP_TRY(hipEventSynchronize(ctx->ev[5]));
         out->on_device = 1;
+        out->pairs = (uint32_t*)pairs;
+        out->seg_offsets = offs;
+        out->seg_exact = segs;
+        out->seg_exp2 = st.grid ? nullptr : exp2;
+        out->seg_f64 = f64;
         out->internal = nullptr;
+    } else {
+        HostResultStore* hs = new HostResultStore();
+        hs->pairs.resize(2 * (size_t)R);
+        hs->offs.resize((size_t)R + 1);
+        hs->seg.resize(words * S);
+        hs->f64.resize(6 * (size_t)S);
+        if (!st.grid) hs->exp2.resize(S);
+        if (R) HIP_TRY(hipMemcpyAsync(hs->pairs.data(), pairs, 8 * (size_t)R, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(hs->offs.data(), offs, 4 * ((size_t)R + 1), hipMemcpyDeviceToHost, s));
+        if (S) {
+            HIP_TRY(hipMemcpyAsync(hs->seg.data(), segs, 8 * words * S, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(hs->f64.data(), f64, 48 * (size_t)S, hipMemcpyDeviceToHost, s));
+            if (!st.grid) HIP_TRY(hipMemcpyAsync(hs->exp2.data(), exp2, 4 * (size_t)S, hipMemcpyDeviceToHost, s));
+        }
+        HIP_TRY(hipStreamSynchronize(s));
+        out->on_device = 0;
+        out->pairs = hs->pairs.data();
+        out->seg_offsets = hs->offs.data();
+        out->seg_exact = hs->seg.data();
+        out->seg_exp2 = st.grid ? nullptr : hs->exp2.data();
+        out->seg_f64 = hs->f64.data();
+        out->internal = hs;
     }
+    st.ms_output = elapsed(ctx->ev[4], ctx->ev[5]);
+    st.ms_total = elapsed(ctx->ev[0], ctx->ev[5]);
     return GI_OK;
 }
 
@@ -1178,6 +1265,29 @@ int gi_build(gi_context* ctx) {
         HIP_TRY(hipSetDevice(ctx->device));
         build_meshes(ctx, false);
         sync_all(ctx);
+        return GI_OK;
+    } catch (const GiError& e) {
+        return fail(ctx, e);
+    }
+}
+
+int gi_get_vertices(gi_context* ctx, int slot, double* xyz, uint32_t nv) {
+    if (!ctx || slot < 0 || slot > 1 || (nv && !xyz)) return GI_ERR_INVALID_ARGUMENT;
+    MeshSlot& m = ctx->mesh[slot];
+    if (!m.set) return GI_ERR_NOT_READY;
+    if (nv != m.nv) {
+        ctx->err = "vertex count differs from the slot's";
+        return GI_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        build_meshes(ctx, false);
+        sync_all(ctx);
+        std::vector<double> v4(4 * (size_t)nv);
+        if (nv)
+            HIP_TRY(hipMemcpy(v4.data(), m.v64.get<double4>(), 32 * (size_t)nv, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nv; ++i)
+            for (int k = 0; k < 3; ++k) xyz[3 * i + k] = v4[4 * i + k];
         return GI_OK;
     } catch (const GiError& e) {
         return fail(ctx, e);
@@ -1494,7 +1604,7 @@ int gi_exact_pair_host(const double a[9], const double b[9], int32_t* hit, int32
     HostHit h;
     bool ok;
     bool r = host_exact_pair(a, b, h, &ok);
-    if (!ok) return GI_ERR_TOO_LARGE;
+    if (!ok) return GI_ERR_INVALID_ARGUMENT;  // a non-finite coordinate
     *hit = r ? 1 : 0;
     if (coplanar) *coplanar = r ? h.coplanar : 0;
     *nseg = r ? h.nseg : 0;

@@ -34,8 +34,20 @@ enum Counter {
     CNT_DEGENERATE,
     CNT_UNSOUND,             // filter said GREEN, exact said no (must stay 0)
     CNT_DEFERRED,            // coplanar pairs moved from the fast exact pass to the full one
+    CNT_RESIDUAL,            // pairs left to the host residual stage (off-grid pairs that do not fit)
+    CNT_FILTER_PRED,         // orient3d signs evaluated by the FP64 filter
+    CNT_EXACT_PRED,          // orient3d and orient2d signs evaluated by the exact stage
+    CNT_OVERFLOW,            // a count wrapped past 2^32 (the query is too large)
     CNT_COUNT
 };
+
+// Exact segments on the device: 8 numbers (x0 y0 z0 w0 x1 y1 z1 w1) of
+// `limbs` 64-bit words each, two's complement: 2 limbs (int128) for grid
+// meshes, 4 (256 bits) with a power-of-two scale per segment for general
+// doubles. Pools are sized for the wider form.
+constexpr int kSegLimbsGrid = 2;
+constexpr int kSegLimbsWide = 4;
+constexpr size_t kSegPoolWords = 8 * kSegLimbsWide;  // uint64 words per pool slot
 
 // Rigid transform applied as CGAL's Aff_transformation_3 evaluates it:
 // x' = ((m00*x + m01*y) + m02*z) + t0, every operation rounded separately.
@@ -170,11 +182,14 @@ void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const 
                    uint2* exact_list, uint32_t exact_cap, uint32_t* counters, int blocks, hipStream_t s);
 // defer: room for the coplanar pairs the fast pass hands to the full pass
 // (at least as many entries as the exact list; the candidate queue, free by
-// then, serves)
+// then, serves). Grid meshes run the int64/int128 stage; general doubles the
+// wide stage (int128/256-bit), whose pairs that do not fit go to `resid`
+// (resid_cap entries, the host residual stage). seg_pool holds kSegPoolWords
+// words per slot, seg_exp one scale per slot (wide stage).
 void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
-                  int bitsB,
-                  uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, __int128* seg_pool, uint32_t seg_cap,
-                  uint2* defer, uint32_t defer_cap, uint32_t* counters, int blocks, hipStream_t s);
+                  int bitsB, uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, uint64_t* seg_pool,
+                  int32_t* seg_exp, uint32_t seg_cap, uint2* defer, uint32_t defer_cap, uint2* resid,
+                  uint32_t resid_cap, uint32_t* counters, int blocks, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
 // Sorted output of up to output_small_max() result records in one workgroup,
 // then their segments; the record count is read on the device, so this is
@@ -184,14 +199,19 @@ uint32_t output_small_max();
 // runs: the sorted runs, their segment-count prefixes and totals (uint64 elements)
 size_t output_small_runs_elems();
 bool output_small_fits(uint32_t R, int key_bits);
+// Segments are gathered at the width the exact stage wrote (grid flags on
+// the device: 2 limbs, else 4 with scales copied to seg_exp_out); seg_out
+// holds out_slots segments of kSegPoolWords words.
 void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const uint32_t* counters, uint32_t cap_rec,
                          uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint64_t* runs, uint2* pairs,
-                         uint32_t* order, uint32_t* offs, __int128* seg_out, double* seg_f64, uint32_t out_slots,
-                         const __int128* seg_pool, hipStream_t s);
+                         uint32_t* order, uint32_t* offs, uint64_t* seg_out, int32_t* seg_exp_out, double* seg_f64,
+                         uint32_t out_slots, const uint64_t* seg_pool, const int32_t* seg_exp,
+                         const uint32_t* grid_flags, hipStream_t s);
 void launch_gather_pairs(const uint64_t* sorted_key, const uint32_t* order, const uint2* rec_seg, uint32_t n,
                          int bitsB, uint2* pairs, uint32_t* nseg, hipStream_t s);
 void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const uint32_t* seg_offsets, uint32_t n,
-                            const __int128* seg_pool, __int128* seg_out, double* seg_f64, hipStream_t s);
+                            const uint64_t* seg_pool, const int32_t* seg_exp, int limbs, uint64_t* seg_out,
+                            int32_t* seg_exp_out, double* seg_f64, hipStream_t s);
 
 // forest results: pair_off[m] (M+1 entries) delimits mesh pair m's sorted
 // pairs, whose indices become per-mesh

@@ -22,6 +22,7 @@
 #include <type_traits>
 
 #include "exact_tritri.h"
+#include "exact_wide.h"
 #include "filter.h"
 #include "gi_device.h"
 #include "gi_karras.h"
@@ -1336,10 +1337,13 @@ void launch_forest_roots(const uint2* range, const float4* nodes, uint32_t n, co
 
 
 __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t fill, uint2* cand, uint32_t cap,
-                                            uint32_t* counter) {
+                                            uint32_t* counters) {
     wave_sync_lds();
     uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(counter, fill);
+    if (lane_id() == 0) {
+        base = atomicAdd(&counters[CNT_TRAVERSAL_HITS], fill);
+        if (u32_add_wraps(base, fill)) atomicOr(&counters[CNT_OVERFLOW], 1u);
+    }
     base = __shfl(base, 0);
     for (uint32_t i = lane_id(); i < fill; i += 64) {
         uint32_t pos = base + i;
@@ -1483,14 +1487,14 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
         if (m) {
             uint32_t cnt = (uint32_t)__popcll(m);
             if (fill + cnt > TRAV_STAGE) {
-                flush_stage(stage, fill, cand, cap, &counters[CNT_TRAVERSAL_HITS]);
+                flush_stage(stage, fill, cand, cap, counters);
                 fill = 0;
             }
             if (hit) stage[fill + (uint32_t)__popcll(m & lt)] = make_uint2(a, b);
             fill += cnt;
         }
     }
-    if (fill) flush_stage(stage, fill, cand, cap, &counters[CNT_TRAVERSAL_HITS]);
+    if (fill) flush_stage(stage, fill, cand, cap, counters);
 }
 
 template <int T, uint32_t S>
@@ -1530,7 +1534,8 @@ __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& tot
 // thread order). Same-address atomics from every wave serialise (~10 ns
 // each on this GPU), so a stage that appends from all its waves at once
 // reserves per workgroup. Every thread of the block must call it.
-__device__ __forceinline__ uint32_t block_reserve(uint32_t v, uint32_t* counter) {
+// A count that wraps past 2^32 sets counters[CNT_OVERFLOW].
+__device__ __forceinline__ uint32_t block_reserve(uint32_t v, uint32_t* counters, int which) {
     __shared__ uint32_t wtot[4], wbase[4];
     uint32_t total;
     const uint32_t ex = wave_exclusive_sum(v, total);
@@ -1539,7 +1544,8 @@ __device__ __forceinline__ uint32_t block_reserve(uint32_t v, uint32_t* counter)
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t t = wtot[0] + wtot[1] + wtot[2] + wtot[3];
-        uint32_t b = t ? atomicAdd(counter, t) : 0u;
+        uint32_t b = t ? atomicAdd(&counters[which], t) : 0u;
+        if (u32_add_wraps(b, t)) atomicOr(&counters[CNT_OVERFLOW], 1u);
         for (int w = 0; w < 4; ++w) { wbase[w] = b; b += wtot[w]; }
     }
     __syncthreads();
@@ -1566,6 +1572,7 @@ __global__ __launch_bounds__(256) void k_filter(MeshDev A, MeshDev B, const uint
     if (n > cap) n = cap;
     const bool grid = grid_flags[0] && grid_flags[1];
     uint32_t c_cand = 0, c_no = 0, c_green = 0, c_yellow = 0;
+    int c_pred = 0;
     for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
         uint32_t i = base + threadIdx.x;
         int status = PAIR_NO;
@@ -1599,17 +1606,18 @@ __global__ __launch_bounds__(256) void k_filter(MeshDev A, MeshDev B, const uint
                       fmax(fmax(PB[0].z, PB[1].z), PB[2].z) >= fmin(fmin(PA[0].z, PA[1].z), PA[2].z);
             if (ov) {
                 c_cand++;
-                status = classify_pair_fp64(PA, PB);
+                status = classify_pair_fp64(PA, PB, c_pred);
                 c_no += status == PAIR_NO;
                 c_green += status == PAIR_GREEN;
                 c_yellow += status == PAIR_YELLOW;
             }
         }
         const bool emit = status != PAIR_NO;
-        const uint32_t slot = block_reserve(emit ? 1u : 0u, &counters[CNT_EXACT_LIST]);
+        const uint32_t slot = block_reserve(emit ? 1u : 0u, counters, CNT_EXACT_LIST);
         if (emit && slot < exact_cap) exact_list[slot] = make_uint2(a, b | ((uint32_t)status << kStatusShift));
     }
     uint32_t t;
+    t = block_sum((uint32_t)c_pred); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_FILTER_PRED], t);
     t = block_sum(c_cand); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_CANDIDATES], t);
     t = block_sum(c_no); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_FILTER_NO], t);
     t = block_sum(c_green); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_FILTER_GREEN], t);
@@ -1627,24 +1635,45 @@ void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const 
 
 typedef ExactTT<GridPolicy> GridExact;
 
+// Per-block tallies of an exact pass, added to the counters once per block.
+struct ExactTally {
+    uint32_t yes = 0, no = 0, cop = 0, deg = 0, bad = 0, pred = 0;
+    __device__ void add(const int status, const int hit, const int coplanar, const int degenerate) {
+        if (status == PAIR_YELLOW) { yes += hit; no += !hit; }
+        if (status == PAIR_GREEN && !hit) bad++;
+        cop += hit && coplanar;
+        deg += hit && degenerate;
+    }
+    __device__ void flush(uint32_t* counters) {
+        uint32_t t;
+        t = block_sum(yes); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_EXACT_YELLOW_YES], t);
+        t = block_sum(no); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_EXACT_YELLOW_NO], t);
+        t = block_sum(cop); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_COPLANAR], t);
+        t = block_sum(deg); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_DEGENERATE], t);
+        t = block_sum(bad); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_UNSOUND], t);
+        t = block_sum(pred); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_EXACT_PRED], t);
+    }
+};
 
-// Exact decision and segments of the listed pairs. The fast instance keeps
-// its per-lane state in registers: a coplanar pair (polygon pieces) is only
-// moved to the deferred list, which the full instance then processes with
-// the whole result (pieces in per-lane memory). Records and segments of both
-// are appended with one atomic per wave.
+// Exact decision and segments of the listed pairs of grid meshes. The fast
+// instance keeps its per-lane state in registers: a coplanar pair (polygon
+// pieces) is only moved to the deferred list, which the full instance then
+// processes with the whole result (pieces in per-lane memory). Records and
+// segments of both are appended with one atomic per workgroup; a segment
+// slot holds 8 int128 (x0 y0 z0 w0 x1 y1 z1 w1).
 template <bool kFast>
 __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2* __restrict__ list, uint32_t cap,
                                                const uint32_t* __restrict__ grid_flags, int bitsB,
                                                uint64_t* __restrict__ rec_key, uint2* __restrict__ rec_seg,
-                                               uint32_t rec_cap, __int128* __restrict__ seg_pool, uint32_t seg_cap,
-                                               uint2* __restrict__ defer, uint32_t defer_cap,
+                                               uint32_t rec_cap, uint64_t* __restrict__ seg_pool_words,
+                                               uint32_t seg_cap, uint2* __restrict__ defer, uint32_t defer_cap,
                                                uint32_t* __restrict__ counters) {
     typedef typename std::conditional<kFast, GridExact::Fast, GridExact::Result>::type R;
-    if (!(grid_flags[0] && grid_flags[1])) return;  // general doubles: host residual path
+    if (!(grid_flags[0] && grid_flags[1])) return;  // general doubles: k_exact_wide
+    __int128* seg_pool = (__int128*)seg_pool_words;
     uint32_t n = counters[kFast ? CNT_EXACT_LIST : CNT_DEFERRED];
     if (n > cap) n = cap;
-    uint32_t c_yes = 0, c_no = 0, c_cop = 0, c_deg = 0, c_bad = 0;
+    ExactTally tally;
     for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
         uint32_t i = base + threadIdx.x;
         R r;
@@ -1668,21 +1697,17 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
             }
             if constexpr (kFast) GridExact::intersect_fast(PA, PB, r);
             else GridExact::intersect(PA, PB, r);
-            if (!r.deferred) {
-                if (status == PAIR_YELLOW) { c_yes += r.hit; c_no += !r.hit; }
-                if (status == PAIR_GREEN && !r.hit) c_bad++;
-                c_cop += r.hit && r.coplanar;
-                c_deg += r.hit && r.degenerate;
-            }
+            tally.pred += (uint32_t)r.npred;
+            if (!r.deferred) tally.add((int)status, r.hit, r.coplanar, r.degenerate);
         }
         if constexpr (kFast) {
-            const uint32_t dslot = block_reserve(r.deferred ? 1u : 0u, &counters[CNT_DEFERRED]);
+            const uint32_t dslot = block_reserve(r.deferred ? 1u : 0u, counters, CNT_DEFERRED);
             if (r.deferred && dslot < defer_cap) defer[dslot] = e;
         }
         const bool hit = r.hit && !r.deferred;
-        const uint32_t slot = block_reserve(hit ? 1u : 0u, &counters[CNT_RESULTS]);
+        const uint32_t slot = block_reserve(hit ? 1u : 0u, counters, CNT_RESULTS);
         const uint32_t nseg = hit ? (uint32_t)r.nseg : 0u;
-        const uint32_t sfirst = block_reserve(nseg, &counters[CNT_SEGMENTS]);
+        const uint32_t sfirst = block_reserve(nseg, counters, CNT_SEGMENTS);
         if (hit && slot < rec_cap) {
             rec_key[slot] = ((uint64_t)a << bitsB) | (uint64_t)b;
             rec_seg[slot] = make_uint2(sfirst, nseg);
@@ -1702,23 +1727,135 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
             }
         }
     }
-    uint32_t t;
-    t = block_sum(c_yes); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_EXACT_YELLOW_YES], t);
-    t = block_sum(c_no); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_EXACT_YELLOW_NO], t);
-    t = block_sum(c_cop); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_COPLANAR], t);
-    t = block_sum(c_deg); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_DEGENERATE], t);
-    t = block_sum(c_bad); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_UNSOUND], t);
+    tally.flush(counters);
+}
+
+__device__ __forceinline__ void store_w256(uint64_t* __restrict__ o, const W256& x) {
+    o[0] = x.w[0]; o[1] = x.w[1]; o[2] = x.w[2]; o[3] = x.w[3];
+}
+
+// The same for general doubles: each pair is moved into its own integer
+// frame (exact_wide.h) and decided and constructed there on int128/256-bit
+// numbers; a segment slot holds 8 numbers of 4 limbs and the frame's
+// power-of-two scale (seg_exp). A pair whose frame or endpoints do not fit
+// goes to the residual list for the host stage, and counts nowhere else.
+template <bool kFast>
+__global__ __launch_bounds__(256) void k_exact_wide(MeshDev A, MeshDev B, const uint2* __restrict__ list,
+                                                    uint32_t cap, const uint32_t* __restrict__ grid_flags, int bitsB,
+                                                    uint64_t* __restrict__ rec_key, uint2* __restrict__ rec_seg,
+                                                    uint32_t rec_cap, uint64_t* __restrict__ seg_pool,
+                                                    int32_t* __restrict__ seg_exp, uint32_t seg_cap,
+                                                    uint2* __restrict__ defer, uint32_t defer_cap,
+                                                    uint2* __restrict__ resid, uint32_t resid_cap,
+                                                    uint32_t* __restrict__ counters) {
+    typedef typename std::conditional<kFast, WideExact::Fast, WideExact::Result>::type R;
+    if (grid_flags[0] && grid_flags[1]) return;  // grid meshes: k_exact
+    uint32_t n = counters[kFast ? CNT_EXACT_LIST : CNT_DEFERRED];
+    if (n > cap) n = cap;
+    ExactTally tally;
+    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        uint32_t i = base + threadIdx.x;
+        R r;
+        r.hit = 0;
+        r.nseg = 0;
+        r.deferred = 0;
+        WideExact::V3 PA[3], PB[3];
+        WideFrame fr;
+        fr.exp2 = 0;
+        fr.shifted = 0;
+        bool residual = false;
+        uint32_t a = 0, b = 0;
+        uint2 e = make_uint2(0, 0);
+        if (i < n) {
+            e = list[i];
+            a = e.x;
+            b = e.y & kIndexMask;
+            const uint32_t status = e.y >> kStatusShift;
+            const uint32_t* ta = A.tri + 3 * (size_t)a;
+            const uint32_t* tb = B.tri + 3 * (size_t)b;
+            double pa[9], pb[9];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const double4 p = A.v64[ta[v]], q = B.v64[tb[v]];
+                pa[3 * v] = p.x; pa[3 * v + 1] = p.y; pa[3 * v + 2] = p.z;
+                pb[3 * v] = q.x; pb[3 * v + 1] = q.y; pb[3 * v + 2] = q.z;
+            }
+            residual = !wide_frame(pa, pb, PA, PB, fr);
+            if (!residual) {
+                if constexpr (kFast) {
+                    WideExact::intersect_fast(PA, PB, r);
+                    if (r.hit && !r.deferred) residual = !wide_absolute(fr, r.p) || !wide_absolute(fr, r.q);
+                } else {
+                    WideExact::intersect(PA, PB, r);
+                    for (int sI = 0; sI < r.nseg && !residual; ++sI) {
+                        WideExact::HP p, q;
+                        WideExact::segment(PA, PB, r, sI, p, q);
+                        residual = !wide_absolute(fr, p) || !wide_absolute(fr, q);
+                    }
+                }
+            }
+            if (residual) {
+                r.hit = 0;
+                r.deferred = 0;
+            } else {
+                tally.pred += (uint32_t)r.npred;
+                if (!r.deferred) tally.add((int)status, r.hit, r.coplanar, r.degenerate);
+            }
+        }
+        const uint32_t rslot = block_reserve(residual ? 1u : 0u, counters, CNT_RESIDUAL);
+        if (residual && rslot < resid_cap) resid[rslot] = e;
+        if constexpr (kFast) {
+            const uint32_t dslot = block_reserve(r.deferred ? 1u : 0u, counters, CNT_DEFERRED);
+            if (r.deferred && dslot < defer_cap) defer[dslot] = e;
+        }
+        const bool hit = r.hit && !r.deferred;
+        const uint32_t slot = block_reserve(hit ? 1u : 0u, counters, CNT_RESULTS);
+        const uint32_t nseg = hit ? (uint32_t)r.nseg : 0u;
+        const uint32_t sfirst = block_reserve(nseg, counters, CNT_SEGMENTS);
+        if (hit && slot < rec_cap) {
+            rec_key[slot] = ((uint64_t)a << bitsB) | (uint64_t)b;
+            rec_seg[slot] = make_uint2(sfirst, nseg);
+            for (uint32_t sI = 0; sI < nseg; ++sI) {
+                const uint32_t dst = sfirst + sI;
+                if (dst >= seg_cap) break;
+                WideExact::HP p, q;
+                if constexpr (kFast) {
+                    p = r.p;
+                    q = r.q;
+                } else {
+                    WideExact::segment(PA, PB, r, (int)sI, p, q);
+                    wide_absolute(fr, p);
+                    wide_absolute(fr, q);
+                }
+                uint64_t* o = seg_pool + (size_t)8 * kSegLimbsWide * dst;
+                store_w256(o + 0, p.h[0]); store_w256(o + 4, p.h[1]); store_w256(o + 8, p.h[2]);
+                store_w256(o + 12, p.w);
+                store_w256(o + 16, q.h[0]); store_w256(o + 20, q.h[1]); store_w256(o + 24, q.h[2]);
+                store_w256(o + 28, q.w);
+                seg_exp[dst] = fr.exp2;
+            }
+        }
+    }
+    tally.flush(counters);
 }
 
 void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
-                  int bitsB, uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, __int128* seg_pool,
-                  uint32_t seg_cap, uint2* defer, uint32_t defer_cap, uint32_t* counters, int blocks, hipStream_t s) {
+                  int bitsB, uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, uint64_t* seg_pool,
+                  int32_t* seg_exp, uint32_t seg_cap, uint2* defer, uint32_t defer_cap, uint2* resid,
+                  uint32_t resid_cap, uint32_t* counters, int blocks, hipStream_t s) {
     if (A.nt == 0 || B.nt == 0) return;
+    // one of each pair of launches returns at once (the grid flags are
+    // decided on the device by the vertex stage)
     k_exact<true><<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
                                          seg_pool, seg_cap, defer, defer_cap, counters);
+    k_exact_wide<true><<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
+                                              rec_cap, seg_pool, seg_exp, seg_cap, defer, defer_cap, resid,
+                                              resid_cap, counters);
     // the deferred (coplanar) pairs: their count is read on the device
     k_exact<false><<<blocks, 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
                                           seg_pool, seg_cap, nullptr, 0, counters);
+    k_exact_wide<false><<<blocks, 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
+                                               seg_pool, seg_exp, seg_cap, nullptr, 0, resid, resid_cap, counters);
     GI_LAUNCH_CHECK();
 }
 
@@ -1752,46 +1889,61 @@ void launch_gather_pairs(const uint64_t* sorted_key, const uint32_t* order, cons
     GI_LAUNCH_CHECK();
 }
 
-__device__ __forceinline__ double i128_to_double(__int128 x) {
-    bool neg = x < 0;
-    unsigned __int128 u = neg ? (unsigned __int128)0 - (unsigned __int128)x : (unsigned __int128)x;
-    double d = (double)(uint64_t)(u >> 64) * 18446744073709551616.0 + (double)(uint64_t)u;
+// a `limbs`-word (2 or 4) two's complement integer as a double: its
+// magnitude's words summed from the top, the sign applied last
+__device__ __forceinline__ double limbs_to_double(const uint64_t* __restrict__ x, int limbs) {
+    const bool neg = (x[limbs - 1] >> 63) != 0;
+    uint64_t m[kSegLimbsWide];
+    uint64_t c = 1;
+    for (int l = 0; l < limbs; ++l) {
+        const uint64_t v = neg ? ~x[l] : x[l];
+        m[l] = neg ? v + c : v;
+        c = neg ? (c && m[l] == 0) : 0;
+    }
+    double d = 0.0;
+    for (int l = limbs - 1; l >= 0; --l) d = d * 18446744073709551616.0 + (double)m[l];
     return neg ? -d : d;
 }
 
 // the rs.y segments of one record (at rs.x in the pool) to dst onward:
-// homogeneous int128 endpoints, and the doubles they divide to
-__device__ __forceinline__ void copy_record_segments(uint2 rs, uint32_t dst, const __int128* __restrict__ pool,
-                                                     __int128* __restrict__ out, double* __restrict__ f64) {
+// homogeneous endpoints of `limbs` words, their power-of-two scales (wide
+// form), and the doubles they divide to
+__device__ __forceinline__ void copy_record_segments(uint2 rs, uint32_t dst, const uint64_t* __restrict__ pool,
+                                                     const int32_t* __restrict__ pool_exp, int limbs,
+                                                     uint64_t* __restrict__ out, int32_t* __restrict__ out_exp,
+                                                     double* __restrict__ f64) {
+    const int words = 8 * limbs;
     for (uint32_t s = 0; s < rs.y; ++s) {
-        const __int128* src = pool + 8 * (size_t)(rs.x + s);
-        __int128* o = out + 8 * (size_t)(dst + s);
+        const uint64_t* src = pool + (size_t)words * (rs.x + s);
+        uint64_t* o = out + (size_t)words * (dst + s);
         double* f = f64 + 6 * (size_t)(dst + s);
+        const int e2 = pool_exp ? pool_exp[rs.x + s] : 0;
+        for (int w = 0; w < words; ++w) o[w] = src[w];
+        if (out_exp) out_exp[dst + s] = e2;
         for (int e = 0; e < 2; ++e) {
-            __int128 w = src[4 * e + 3];
-            double wd = i128_to_double(w);
-            for (int k = 0; k < 3; ++k) {
-                o[4 * e + k] = src[4 * e + k];
-                f[3 * e + k] = i128_to_double(src[4 * e + k]) / wd;
-            }
-            o[4 * e + 3] = w;
+            const double wd = limbs_to_double(src + (4 * e + 3) * limbs, limbs);
+            for (int k = 0; k < 3; ++k) f[3 * e + k] = ldexp(limbs_to_double(src + (4 * e + k) * limbs, limbs) / wd, e2);
         }
     }
 }
 
 __global__ void k_gather_segments(const uint32_t* __restrict__ order, const uint2* __restrict__ rec_seg,
                                   const uint32_t* __restrict__ seg_offsets, uint32_t n,
-                                  const __int128* __restrict__ pool, __int128* __restrict__ out,
+                                  const uint64_t* __restrict__ pool, const int32_t* __restrict__ pool_exp, int limbs,
+                                  uint64_t* __restrict__ out, int32_t* __restrict__ out_exp,
                                   double* __restrict__ f64) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    copy_record_segments(rec_seg[order[i]], seg_offsets[i], pool, out, f64);
+    copy_record_segments(rec_seg[order[i]], seg_offsets[i], pool, pool_exp, limbs, out, out_exp, f64);
 }
 
 void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const uint32_t* seg_offsets, uint32_t n,
-                            const __int128* seg_pool, __int128* seg_out, double* seg_f64, hipStream_t s) {
+                            const uint64_t* seg_pool, const int32_t* seg_exp, int limbs, uint64_t* seg_out,
+                            int32_t* seg_exp_out, double* seg_f64, hipStream_t s) {
     if (n == 0) return;
-    k_gather_segments<<<blocks_for(n, 256), 256, 0, s>>>(order, rec_seg, seg_offsets, n, seg_pool, seg_out, seg_f64);
+    k_gather_segments<<<blocks_for(n, 256), 256, 0, s>>>(order, rec_seg, seg_offsets, n, seg_pool,
+                                                         limbs == kSegLimbsWide ? seg_exp : nullptr, limbs, seg_out,
+                                                         limbs == kSegLimbsWide ? seg_exp_out : nullptr, seg_f64);
     GI_LAUNCH_CHECK();
 }
 
@@ -2000,12 +2152,19 @@ __global__ __launch_bounds__(256) void k_gather_segments_small(const uint32_t* _
                                                                const uint32_t* __restrict__ seg_offsets,
                                                                const uint32_t* __restrict__ counters, uint32_t cap_rec,
                                                                uint32_t cap_seg, int key_bits, uint32_t out_slots,
-                                                               const __int128* __restrict__ pool,
-                                                               __int128* __restrict__ out, double* __restrict__ f64) {
+                                                               const uint64_t* __restrict__ pool,
+                                                               const int32_t* __restrict__ pool_exp,
+                                                               const uint32_t* __restrict__ grid_flags,
+                                                               uint64_t* __restrict__ out,
+                                                               int32_t* __restrict__ out_exp,
+                                                               double* __restrict__ f64) {
     uint32_t R;
     if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R) || seg_offsets[R] > out_slots) return;
+    const bool grid = grid_flags[0] && grid_flags[1];
+    const int limbs = grid ? kSegLimbsGrid : kSegLimbsWide;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x)
-        copy_record_segments(rec_seg[order[i]], seg_offsets[i], pool, out, f64);
+        copy_record_segments(rec_seg[order[i]], seg_offsets[i], pool, grid ? nullptr : pool_exp, limbs, out,
+                             grid ? nullptr : out_exp, f64);
 }
 
 uint32_t output_small_max() { return kOutSmallMax; }
@@ -2016,8 +2175,9 @@ bool output_small_fits(uint32_t R, int key_bits) { return R <= kOutSmallMax && k
 
 void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const uint32_t* counters, uint32_t cap_rec,
                          uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint64_t* runs, uint2* pairs,
-                         uint32_t* order, uint32_t* offs, __int128* seg_out, double* seg_f64, uint32_t out_slots,
-                         const __int128* seg_pool, hipStream_t s) {
+                         uint32_t* order, uint32_t* offs, uint64_t* seg_out, int32_t* seg_exp_out, double* seg_f64,
+                         uint32_t out_slots, const uint64_t* seg_pool, const int32_t* seg_exp,
+                         const uint32_t* grid_flags, hipStream_t s) {
     uint32_t* cpre = (uint32_t*)(runs + kOutSmallMax);
     k_output_runs<<<kRuns, RUN_THREADS, 0, s>>>(rec_key, rec_seg, counters, cap_rec, cap_seg, key_bits,
                                                 want_segments ? 1 : 0, runs, cpre);
@@ -2025,7 +2185,8 @@ void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const ui
                                                                         key_bits, bitsB, pairs, order, offs);
     if (want_segments)
         k_gather_segments_small<<<kOutSmallMax / 256, 256, 0, s>>>(order, rec_seg, offs, counters, cap_rec, cap_seg,
-                                                                   key_bits, out_slots, seg_pool, seg_out, seg_f64);
+                                                                   key_bits, out_slots, seg_pool, seg_exp, grid_flags,
+                                                                   seg_out, seg_exp_out, seg_f64);
     GI_LAUNCH_CHECK();
 }
 

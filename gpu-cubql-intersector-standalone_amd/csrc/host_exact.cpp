@@ -23,8 +23,8 @@ typedef unsigned __int128 u128h;
 struct Overflow {};
 
 // Sign-magnitude integer with up to kLimbs 64-bit limbs.
+template <int kLimbs>
 struct BigNum {
-    static const int kLimbs = 64;
     int sign = 0;  // -1, 0, +1
     int n = 0;
     uint64_t d[kLimbs];
@@ -147,22 +147,29 @@ struct BigNum {
     }
 };
 
+template <int K>
 struct BigPolicy {
-    typedef BigNum C;
-    typedef BigNum B;
-    typedef BigNum H;
-    static BigNum toB(const BigNum& x) { return x; }
-    static BigNum mulCC(const BigNum& a, const BigNum& b) { return a * b; }
-    static BigNum mulBC(const BigNum& a, const BigNum& b) { return a * b; }
-    static BigNum mulBB(const BigNum& a, const BigNum& b) { return a * b; }
-    static int cmpH(const BigNum& a, const BigNum& b) { return (a - b).sign; }
-    static int sgnC(const BigNum& x) { return x.sign; }
-    static int sgnB(const BigNum& x) { return x.sign; }
-    static int cmpAbsC(const BigNum& a, const BigNum& b) { return BigNum::cmpMag(a, b); }
-    static int cmpAbsB(const BigNum& a, const BigNum& b) { return BigNum::cmpMag(a, b); }
+    typedef BigNum<K> N;
+    typedef N C;
+    typedef N B;
+    typedef N H;
+    static N toB(const N& x) { return x; }
+    static N mulCC(const N& a, const N& b) { return a * b; }
+    static N mulBC(const N& a, const N& b) { return a * b; }
+    static N mulBB(const N& a, const N& b) { return a * b; }
+    static int cmpH(const N& a, const N& b) { return (a - b).sign; }
+    static int sgnC(const N& x) { return x.sign; }
+    static int sgnB(const N& x) { return x.sign; }
+    static int cmpAbsC(const N& a, const N& b) { return N::cmpMag(a, b); }
+    static int cmpAbsB(const N& a, const N& b) { return N::cmpMag(a, b); }
 };
 
-typedef ExactTT<BigPolicy> BigExact;
+// Capacities: 64 limbs serve every pair whose coordinates span up to ~560
+// bits of exponent; the largest products of any two finite doubles' pair
+// (coordinates of up to 2,098 bits: 2^971 down to 2^-1074) need 14,701 bits,
+// within 256 limbs.
+constexpr int kLimbsFast = 64;
+constexpr int kLimbsAll = 256;
 
 // lowest-set-bit exponent of a non-zero finite double and its odd mantissa
 void dyadic(double x, int64_t& odd, int& elow) {
@@ -175,10 +182,12 @@ void dyadic(double x, int64_t& odd, int& elow) {
     elow = e - 53 + t;
 }
 
-}  // namespace
-
-bool host_exact_pair(const double* a, const double* b, HostHit& out, bool* ok) {
-    *ok = true;
+// One pair on K-limb integers; false with *fits = false when they overflow.
+template <int K>
+bool exact_pair_k(const double* a, const double* b, HostHit& out, bool* fits, int* npred) {
+    typedef BigNum<K> BigN;
+    typedef ExactTT<BigPolicy<K>> BigExact;
+    *fits = true;
     out.nseg = 0;
     out.coplanar = out.degenerate = 0;
     const double* src[2] = {a, b};
@@ -186,7 +195,6 @@ bool host_exact_pair(const double* a, const double* b, HostHit& out, bool* ok) {
     for (int s = 0; s < 2; ++s)
         for (int i = 0; i < 9; ++i) {
             double x = src[s][i];
-            if (!isfinite(x)) { *ok = false; return false; }
             if (x == 0.0) continue;
             int64_t odd;
             int el;
@@ -194,25 +202,26 @@ bool host_exact_pair(const double* a, const double* b, HostHit& out, bool* ok) {
             if (el < emin) emin = el;
         }
     try {
-        BigExact::V3 A[3], B[3];
+        typename BigExact::V3 A[3], B[3];
         for (int s = 0; s < 2; ++s)
             for (int i = 0; i < 9; ++i) {
                 double x = src[s][i];
-                BigNum& dst = (s == 0 ? A : B)[i / 3].v[i % 3];
-                if (x == 0.0) { dst = BigNum(); continue; }
+                BigN& dst = (s == 0 ? A : B)[i / 3].v[i % 3];
+                if (x == 0.0) { dst = BigN(); continue; }
                 int64_t odd;
                 int el;
                 dyadic(x, odd, el);
-                dst = BigNum::shl(BigNum(odd), el - emin);
+                dst = BigN::shl(BigN(odd), el - emin);
             }
-        BigExact::Result r;
+        typename BigExact::Result r;
         BigExact::intersect(A, B, r);
+        if (npred) *npred = r.npred;
         if (!r.hit) return false;
         out.coplanar = r.coplanar;
         out.degenerate = r.degenerate;
         out.nseg = r.nseg;
         out.exp2 = emin;
-        BigExact::HP P[GI_SEG_MAX][2];
+        typename BigExact::HP P[GI_SEG_MAX][2];
         int limbs = 1;
         for (int i = 0; i < r.nseg; ++i) {
             BigExact::segment(A, B, r, i, P[i][0], P[i][1]);
@@ -230,17 +239,35 @@ bool host_exact_pair(const double* a, const double* b, HostHit& out, bool* ok) {
             }
         return true;
     } catch (const Overflow&) {
-        *ok = false;
+        *fits = false;
         return false;
     }
 }
 
+}  // namespace
+
+bool host_exact_pair(const double* a, const double* b, HostHit& out, bool* ok, int* npred) {
+    *ok = true;
+    for (int i = 0; i < 9; ++i)
+        if (!isfinite(a[i]) || !isfinite(b[i])) {
+            *ok = false;
+            return false;
+        }
+    bool fits;
+    bool hit = exact_pair_k<kLimbsFast>(a, b, out, &fits, npred);
+    if (fits) return hit;
+    hit = exact_pair_k<kLimbsAll>(a, b, out, &fits, npred);
+    if (!fits) *ok = false;  // unreachable for finite doubles
+    return hit;
+}
+
 int host_exact_batch(const double* vA, const uint32_t* triA, const double* vB, const uint32_t* triB,
-                     const uint32_t* list, uint32_t n, int nthreads, std::vector<HostHit>& hits, uint64_t* yellow_yes) {
+                     const uint32_t* list, uint32_t n, int nthreads, std::vector<HostHit>& hits, uint64_t* yellow_yes,
+                     uint64_t* predicates) {
     if (nthreads < 1) nthreads = 1;
     std::vector<std::vector<HostHit>> local((size_t)nthreads);
-    std::atomic<uint64_t> yes{0};
-    std::atomic<int> bad{0};
+    std::atomic<uint64_t> yes{0}, npred{0};
+    std::atomic<int> unsound{0}, nonfinite{0};
     std::atomic<uint32_t> next{0};
     auto work = [&](int t) {
         const uint32_t chunk = 64;
@@ -259,9 +286,11 @@ int host_exact_batch(const double* vA, const uint32_t* triA, const double* vB, c
                     }
                 HostHit h;
                 bool ok;
-                bool hit = host_exact_pair(pa, pb, h, &ok);
-                if (!ok) { bad++; continue; }
-                if (status == 1 && !hit) { bad++; continue; }  // certified GREEN must intersect
+                int np = 0;
+                bool hit = host_exact_pair(pa, pb, h, &ok, &np);
+                npred += (uint64_t)np;
+                if (!ok) { nonfinite++; continue; }
+                if (status == 1 && !hit) { unsound++; continue; }  // certified GREEN must intersect
                 if (!hit) continue;
                 if (status == 2) yes++;
                 h.a = a;
@@ -281,7 +310,9 @@ int host_exact_batch(const double* vA, const uint32_t* triA, const double* vB, c
         return x.a != y.a ? x.a < y.a : x.b < y.b;
     });
     if (yellow_yes) *yellow_yes = yes.load();
-    return bad.load() ? -1 : 0;
+    if (predicates) *predicates = npred.load();
+    if (nonfinite.load()) return kHostNonFinite;
+    return unsound.load() ? kHostUnsound : 0;
 }
 
 }  // namespace gi

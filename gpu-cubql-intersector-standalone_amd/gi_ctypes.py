@@ -29,7 +29,7 @@ class GiOptions(C.Structure):
 
 _STAT_U64 = ["traversal_hits", "candidates", "filter_no", "filter_green", "filter_yellow", "exact_pairs",
              "exact_yellow_yes", "exact_yellow_no", "host_pairs", "host_yes", "pairs", "segments", "coplanar",
-             "degenerate"]
+             "degenerate", "filter_predicates", "exact_predicates", "host_predicates"]
 _STAT_F32 = ["ms_total", "ms_build", "ms_traverse", "ms_filter", "ms_exact", "ms_output", "ms_host",
              "ms_bvh_chunks"]
 
@@ -99,7 +99,8 @@ class GiError(RuntimeError):
 _lib = None
 
 SYMBOLS = ["gi_create", "gi_destroy", "gi_last_error", "gi_status_string", "gi_set_mesh", "gi_set_mesh_device",
-           "gi_set_transform", "gi_set_transform_matrix", "gi_build", "gi_intersect", "gi_result_free",
+           "gi_set_transform", "gi_set_transform_matrix", "gi_build", "gi_get_vertices", "gi_intersect",
+           "gi_result_free",
            "gi_intersect_oneshot", "gi_set_batch", "gi_intersect_batch", "gi_share_mesh", "gi_get_tuning",
            "gi_set_tuning", "gi_autotune", "gi_clear_device",
            "gi_reconstruct_device",
@@ -126,6 +127,7 @@ def lib():
         L.gi_set_transform.argtypes = [vp, C.c_int, dp, dp, dp, C.c_int]
         L.gi_set_transform_matrix.argtypes = [vp, C.c_int, dp, C.c_int]
         L.gi_build.argtypes = [vp]
+        L.gi_get_vertices.argtypes = [vp, C.c_int, dp, C.c_uint32]
         L.gi_intersect.argtypes = [vp, C.POINTER(GiOptions), C.POINTER(GiResult)]
         L.gi_result_free.argtypes = [C.POINTER(GiResult)]
         L.gi_intersect_oneshot.argtypes = [C.POINTER(GiConfig), dp, C.c_uint32, up, C.c_uint32, dp, C.c_uint32, up,
@@ -238,6 +240,12 @@ class Intersector:
     def build(self):
         _check(lib().gi_build(self._ctx), self._ctx)
 
+    def get_vertices(self, slot: int, nv: int) -> np.ndarray:
+        """The slot's vertices as the predicates see them (moved, snapped)."""
+        out = np.zeros((nv, 3), dtype=np.float64)
+        _check(lib().gi_get_vertices(self._ctx, slot, out.ctypes.data_as(C.POINTER(C.c_double)), nv), self._ctx)
+        return out
+
     def clear_device(self):
         _check(lib().gi_clear_device(self._ctx), self._ctx)
 
@@ -317,6 +325,26 @@ class Intersector:
                 r["segments"] = whole["segments"][lo:hi]
             out.append(r)
         return out, whole["stats"]
+
+
+def intersect_oneshot(xa, ta, xb, tb, want_segments=True, device=0, decode=True):
+    """gi_intersect_oneshot: both meshes copied, built and intersected in one
+    call (the reference's runGridForestIntersectionPipeline)."""
+    xa = np.ascontiguousarray(xa, dtype=np.float64).reshape(-1, 3)
+    ta = np.ascontiguousarray(ta, dtype=np.uint32).reshape(-1, 3)
+    xb = np.ascontiguousarray(xb, dtype=np.float64).reshape(-1, 3)
+    tb = np.ascontiguousarray(tb, dtype=np.uint32).reshape(-1, 3)
+    cfg = GiConfig(device, 0, 0, 0)
+    opt = GiOptions(0, int(want_segments), 0, 0)
+    res = GiResult()
+    dp, up = C.POINTER(C.c_double), C.POINTER(C.c_uint32)
+    _check(lib().gi_intersect_oneshot(C.byref(cfg), xa.ctypes.data_as(dp), xa.shape[0], ta.ctypes.data_as(up),
+                                      ta.shape[0], xb.ctypes.data_as(dp), xb.shape[0], tb.ctypes.data_as(up),
+                                      tb.shape[0], C.byref(opt), C.byref(res)))
+    try:
+        return result_to_py(res, want_segments, decode)
+    finally:
+        lib().gi_result_free(C.byref(res))
 
 
 def result_to_py(res: GiResult, want_segments=True, decode=True):

@@ -92,10 +92,15 @@ typedef struct {
     uint64_t filter_no, filter_green, filter_yellow; /* FP64 filter outcomes */
     uint64_t exact_pairs;       /* pairs run through the exact integer stage (GPU) */
     uint64_t exact_yellow_yes, exact_yellow_no;      /* YELLOW pairs resolved exactly */
-    uint64_t host_pairs;        /* pairs run through the host residual stage */
+    uint64_t host_pairs;        /* residual pairs run through the host stage (general doubles whose
+                                   integer frame exceeds the GPU stage's width) */
     uint64_t host_yes;
     uint64_t pairs, segments, coplanar, degenerate;
-    int32_t grid;               /* 1: both meshes on the integer grid (GPU exact stage) */
+    /* predicates evaluated per cascade stage (orient3d signs in the FP64
+       filter; orient3d and orient2d signs in the exact stages) */
+    uint64_t filter_predicates, exact_predicates, host_predicates;
+    int32_t grid;               /* 1: both meshes on the integer grid (int64/int128 exact stage);
+                                   0: general doubles (per-pair integer frames, 256-bit) */
     int32_t retries;            /* capacity regrowths during this call */
     /* milliseconds (GPU events; host stage by wall clock) */
     float ms_total, ms_build, ms_traverse, ms_filter, ms_exact, ms_output, ms_host;
@@ -111,7 +116,8 @@ typedef struct {
     uint32_t* pairs;        /* 2 * n_pairs: (a, b), sorted, unique */
     uint32_t* seg_offsets;  /* n_pairs + 1: segments of pair i are [off[i], off[i+1]) */
     uint64_t n_segments;
-    int32_t exact_limbs;    /* 64-bit limbs per exact number (2 on the grid path) */
+    int32_t exact_limbs;    /* 64-bit limbs per exact number (2 for grid meshes, 4 for general
+                               doubles, more when host-stage pairs need them) */
     int32_t on_device;      /* arrays are device pointers */
     uint64_t* seg_exact;    /* n_segments * 8 * exact_limbs: x0 y0 z0 w0 x1 y1 z1 w1,
                                little-endian two's complement; point = (x/w, y/w, z/w)
@@ -142,6 +148,13 @@ int gi_set_transform(gi_context* ctx, int slot, const double rot_deg[3], const d
 int gi_set_transform_matrix(gi_context* ctx, int slot, const double m[12], int snap_to_grid);
 /* Builds (or refits) whatever is stale. gi_intersect calls it as needed. */
 int gi_build(gi_context* ctx);
+/* Copies the slot's vertices as the predicates see them (after the rigid
+ * transform, and the grid snap when set) to xyz (3 * nv doubles; nv must be
+ * the slot's vertex count); builds first when stale. No reference
+ * counterpart: it exposes the doubles the reference's CGAL path builds with
+ * createRigidTransformation (src/CPU/RotationTools.h:76-104) so that callers
+ * can check them. */
+int gi_get_vertices(gi_context* ctx, int slot, double* xyz, uint32_t nv);
 int gi_intersect(gi_context* ctx, const gi_options* opt, gi_result* out);
 int gi_result_free(gi_result* res);
 

@@ -1,9 +1,14 @@
 // Drives the C++ facade (KernelBVHController) the way the reference's
 // application drives its controller (apps/mainApplication.cpp:723-796):
-// construct -> runIntersectionPipeline -> setTransformBoth -> run again.
+// construct -> runIntersectionPipeline -> setTransformBoth -> run again, then
+// the per-call free function runGridForestIntersectionPipeline
+// (src/GPUIntersector/StandaloneBVHPipeline.h:27-42) with the same motion of B.
 // Meshes come from raw files: uint32 nv, uint32 nt, nv*3 doubles, nt*3 uint32.
+// Arguments: meshA meshB out [rx ry rz tx ty tz [snap]] (snap: 1 rounds the
+// moved vertices to integers, 0 keeps the moved doubles; default 1).
 // Output: one block per run: "run <n>", then "pair a b" lines, then
-// "seg a b x0 y0 z0 w0 x1 y1 z1 w1 exp2" lines (exact decimal integers).
+// "seg a b x0 y0 z0 w0 x1 y1 z1 w1 exp2" lines (exact decimal integers); the
+// free function's run has pairs only.
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -49,11 +54,12 @@ int main(int argc, char** argv) {
         rot = double3{atof(argv[4]), atof(argv[5]), atof(argv[6])};
         tr = double3{atof(argv[7]), atof(argv[8]), atof(argv[9])};
     }
+    const bool snap = argc >= 11 ? atoi(argv[10]) != 0 : true;
     FILE* o = fopen(argv[3], "w");
     if (!o) return 2;
     try {
         KernelBVHController k(0);
-        k.setSnapToGrid(true);
+        k.setSnapToGrid(snap);
         ExecutionStats st;
         k.construct(vA.data(), (int)vA.size(), tA.data(), (int)tA.size(), 12, vB.data(), (int)vB.size(), tB.data(),
                     (int)tB.size(), 12, 20, st);
@@ -69,6 +75,17 @@ int main(int argc, char** argv) {
         free(pairs);
         Point3 cb = k.getCenterB();
         fprintf(o, "centerB %.17g %.17g %.17g\n", cb.x, cb.y, cb.z);
+        if (!snap) {  // the free function keeps the moved doubles, as the reference does
+            ExecutionStats fs;
+            int2* fp = nullptr;
+            size_t fn = 0;
+            runGridForestIntersectionPipeline(vA.data(), (int)vA.size(), tA.data(), (int)tA.size(), 12, vB.data(),
+                                              (int)vB.size(), tB.data(), (int)tB.size(), 12, 1, 12, 1, 20, fs, fp,
+                                              fn, k.getCenterA(), cb, double3{0, 0, 0}, double3{0, 0, 0}, rot, tr);
+            fprintf(o, "run %zu candidates %llu\n", fn, (unsigned long long)fs.finalAabbCandidatePairs);
+            for (size_t i = 0; i < fn; ++i) fprintf(o, "pair %d %d\n", fp[i].x, fp[i].y);
+            free(fp);
+        }
     } catch (const Error& e) {
         fprintf(stderr, "facade error %d: %s\n", e.status(), e.what());
         fclose(o);

@@ -26,7 +26,7 @@ def _edge_pairs():
     return [(empty, (xb, tb)), ((xa, ta), empty), (one, (xa, ta)), ((xa, ta), (xa, ta)), ((xa, ta), far)]
 
 
-def _check_batch(pairs, n_oracle=3, rebuild_twice=True):
+def _check_batch(pairs, n_oracle=None, rebuild_twice=True):
     eng = gi.Intersector()
     try:
         eng.set_batch(pairs)
@@ -47,7 +47,7 @@ def _check_batch(pairs, n_oracle=3, rebuild_twice=True):
     assert sum(len(g["pairs"]) for g in got) == stats["pairs"]
     checked = 0
     for i, ((xa, ta), (xb, tb)) in enumerate(pairs):
-        if checked >= n_oracle or len(ta) == 0 or len(tb) == 0:
+        if (n_oracle is not None and checked >= n_oracle) or len(ta) == 0 or len(tb) == 0:
             continue
         ref = oracle_py.intersect(xa, ta, xb, tb, nthreads=8)
         np.testing.assert_array_equal(got[i]["pairs"], ref["pairs"], err_msg=f"pair {i}")
@@ -64,20 +64,34 @@ def test_grid_batch_matches_single_pairs_and_oracle():
     assert stats["coplanar"] > 0  # the self pair
 
 
-def test_batch_of_float_meshes_takes_the_host_residual_stage():
+def test_batch_of_float_meshes_takes_the_wide_exact_stage():
     a = mg.jittered_icosphere(2, seed=1)
     b = mg.jittered_icosphere(2, seed=2, offset=(0.6, 0.1, 0.05))
     c = mg.jittered_icosphere(3, seed=3, offset=(-0.3, 0.5, 0.0))
     pairs = [(a, b), (b, c), (c, a), (a, a)]
-    got, stats = _check_batch(pairs, n_oracle=2, rebuild_twice=False)
-    assert stats["grid"] == 0 and stats["host_pairs"] > 0
+    got, stats = _check_batch(pairs, rebuild_twice=False)
+    assert stats["grid"] == 0 and stats["exact_pairs"] > 0
+
+
+def test_batch_with_residual_pairs_merges_on_the_host():
+    # tiny coordinates near x = 0 in one mesh: some pairs are left to the host
+    # stage, merged with the GPU's and split per mesh pair there
+    a = mg.jittered_icosphere(2, seed=1)
+    b = mg.jittered_icosphere(2, seed=2, offset=(0.6, 0.1, 0.05))
+    c = mg.jittered_icosphere(3, seed=3, offset=(-0.3, 0.5, 0.0))
+    xa = a[0].copy()
+    near = np.nonzero(np.abs(xa[:, 0]) < 0.05)[0]
+    xa[near, 0] = np.ldexp(1.0, -80)
+    pairs = [(b, c), ((xa, a[1]), b), (c, (xa, a[1])), (b, b)]
+    got, stats = _check_batch(pairs, rebuild_twice=True)
+    assert stats["host_pairs"] > 0 and stats["exact_pairs"] > 0
 
 
 def test_batch_spanning_many_sort_tiles_and_chunks():
     # enough triangles for multi-tile sorts, multi-chunk builds and mesh
     # subtrees whose roots are top nodes
     pairs = mg.batch_workload(40, seed=8, nmin=2000, nmax=60000)
-    _check_batch(pairs, n_oracle=1, rebuild_twice=True)
+    _check_batch(pairs, rebuild_twice=True)
 
 
 def test_batch_context_refuses_single_pair_query():

@@ -16,8 +16,8 @@ def _transforms(n, seed=3):
     return [mg.affine_about(mg.quat_matrix(q[i]), c, 0.3 * t[i] * mg.GRID_SCALE) for i in range(n)]
 
 
-@pytest.mark.parametrize("reserve", [0, 1024])
-def test_graph_replay_equals_eager(reserve):
+@pytest.mark.parametrize("reserve,snap", [(0, True), (1024, True), (0, False)])
+def test_graph_replay_equals_eager(reserve, snap):
     (xa, ta), (xb, tb) = mg.sphere_torus(128, 64)
     eager = gi.Intersector(candidate_reserve=reserve)
     graph = gi.Intersector(candidate_reserve=reserve)
@@ -28,7 +28,7 @@ def test_graph_replay_equals_eager(reserve):
         replays = 0
         for i, m in enumerate(_transforms(9)):
             for e in (eager, graph):
-                e.set_transform_matrix(1, m, snap=True)
+                e.set_transform_matrix(1, m, snap=snap)
             r0 = eager.intersect()
             r1 = graph.intersect(use_graph=True)
             np.testing.assert_array_equal(r0["pairs"], r1["pairs"], err_msg=f"step {i}")

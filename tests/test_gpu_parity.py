@@ -79,12 +79,46 @@ def test_bench_workload_full_size():
 
 
 @pytest.mark.parametrize("level", [4, 5])
-def test_icosphere_pair_doubles_host_residual(level):
+def test_icosphere_pair_doubles_gpu_exact(level):
     # level 5 is the reference's CPU-runnable case: 20,480 triangles each, raw
-    # doubles (off the integer grid), so the exact stage runs on the host
+    # doubles (off the integer grid): the exact stage runs on the GPU in each
+    # pair's integer frame; only pairs whose frame exceeds 61 bits (a
+    # coordinate many binades below the pair's others) go to the host
     (xa, ta), (xb, tb) = mg.icosphere_pair(level=level)
     res, ref = _check(xa, ta, xb, tb, False)
+    st = res["stats"]
     assert len(res["pairs"]) > 50
+    assert st["exact_pairs"] > 0 and st["host_pairs"] <= 0.02 * (st["exact_pairs"] + st["host_pairs"])
+    assert st["exact_predicates"] > 0 and st["filter_predicates"] > 0
+
+
+def test_residual_pairs_merge_with_gpu_pairs():
+    # vertices near the plane y = 0, which the intersection curve of the two
+    # spheres crosses, moved onto it up to tiny y coordinates (2^-70 .. 2^-90):
+    # their pairs' frames exceed 61 bits, so those pairs are decided on the
+    # host and merged, in (a, b) order, with the pairs the GPU decided
+    (xa, ta), (xb, tb) = mg.icosphere_pair(level=4)
+    xa = xa.copy()
+    near = np.nonzero(np.abs(xa[:, 1]) < 0.02)[0]
+    assert len(near) > 10
+    xa[near, 1] = np.ldexp(np.sign(xa[near, 1]) + (xa[near, 1] == 0), -70 - (near % 21))
+    res, ref = _check(xa, ta, xb, tb, False)
+    st = res["stats"]
+    assert st["host_pairs"] > 0 and st["exact_pairs"] > st["host_pairs"]
+    assert len(res["pairs"]) > 50
+
+
+def test_oneshot_entry_point():
+    """gi_intersect_oneshot (the reference's runGridForestIntersectionPipeline:
+    copy, build and intersect in one call) on grid meshes and on doubles."""
+    for (xa, ta), (xb, tb), grid in ((*mg.sphere_torus(96, 48), 1), (*mg.icosphere_pair(level=3), 0)):
+        res = gi.intersect_oneshot(xa, ta, xb, tb)
+        ref = oracle_py.intersect(xa, ta, xb, tb, nthreads=8)
+        assert res["stats"]["grid"] == grid
+        np.testing.assert_array_equal(res["pairs"], ref["pairs"])
+        assert res["segments"] == ref["segments"]
+        assert res["stats"]["candidates"] == ref["stats"]["n_candidates"]
+        assert len(res["pairs"]) > 10
 
 
 def test_height_field_shift_and_rotation():
@@ -151,6 +185,43 @@ def test_transform_refit_matches_oracle_on_moved_vertices():
     np.testing.assert_array_equal(moved["pairs"], ref["pairs"])
     assert moved["segments"] == ref["segments"]
     assert len(first["pairs"]) > 0 and R.shape == (3, 3)
+
+
+@pytest.mark.parametrize("case", ["sphere_torus", "icospheres"])
+def test_transform_unrounded_matches_oracle_bit_for_bit(case):
+    """The reference's own transform flow: the moved vertices are not rounded
+    (setTransformBoth never rounds). The doubles the library moves must be
+    bit-identical to the CGAL-order map evaluated on the host (numpy rounds
+    each product and sum separately, in that order), and the pairs and exact
+    segments equal the oracle's on those doubles."""
+    if case == "sphere_torus":
+        (xa, ta), (xb, tb) = mg.sphere_torus(96, 48)
+        rot, tr = [10.0, -20.0, 35.0], [2.0e5, -1.0e5, 3.0e4]
+    else:
+        (xa, ta), (xb, tb) = mg.icosphere_pair(level=4)
+        rot, tr = [17.0, 41.0, -23.0], [-0.05, 0.02, 0.01]
+    c = xb.mean(axis=0)
+    eng = gi.Intersector()
+    try:
+        eng.set_mesh(0, xa, ta)
+        eng.set_mesh(1, xb, tb)
+        eng.set_transform(1, rot, tr, c.tolist(), snap=False)
+        moved_dev = eng.get_vertices(1, xb.shape[0])
+        moved = eng.intersect()
+    finally:
+        eng.close()
+    xb2 = gi_apply_rigid(xb, rot, tr, c)
+    assert np.array_equal(moved_dev.view(np.uint64), xb2.view(np.uint64))  # bit for bit
+    assert not np.array_equal(xb2, np.rint(xb2))
+    ref = oracle_py.intersect(xa, ta, xb2, tb, nthreads=8)
+    st = moved["stats"]
+    assert st["grid"] == 0
+    np.testing.assert_array_equal(moved["pairs"], ref["pairs"])
+    assert moved["segments"] == ref["segments"]
+    assert st["candidates"] == ref["stats"]["n_candidates"]
+    assert st["coplanar"] == ref["stats"]["n_coplanar"] and st["degenerate"] == ref["stats"]["n_degenerate"]
+    assert len(moved["pairs"]) > 10
+    assert st["host_pairs"] <= 0.02 * (st["exact_pairs"] + st["host_pairs"])
 
 
 def gi_apply_rigid(x, rot, trans, c):

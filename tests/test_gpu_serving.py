@@ -55,6 +55,29 @@ def test_results_alone_and_in_the_mix_are_identical_and_match_oracle():
     assert seen == {0, 1}
 
 
+def test_unsnapped_requests_run_the_wide_exact_stage_and_match_oracle():
+    # the moved vertices keep their doubles (the reference's transform): every
+    # request's pairs and segments equal the oracle on the host-moved doubles,
+    # alone and in the mix
+    w = _small_workload(8)
+    srv = _server(w, slots=4, snap=False)
+    try:
+        alone = [srv.wait(srv.submit(mid, m)) for _, mid, m in w["requests"]]
+        tickets = [srv.submit(mid, m) for _, mid, m in w["requests"]]
+        mixed = [srv.wait(t) for t in tickets]
+    finally:
+        srv.close()
+    xs, ts = w["static"]
+    for (_, mid, m), a, b in zip(w["requests"], alone, mixed):
+        _same(a, b)
+        xm, tm = w["meshes"][mid]
+        ref = oracle_py.intersect(xs, ts, mg.apply_affine(m, xm, snap_to_grid=False), tm, nthreads=8)
+        np.testing.assert_array_equal(a["pairs"], ref["pairs"])
+        assert a["segments"] == ref["segments"]
+        assert a["stats"]["grid"] == 0
+    assert sum(len(r["pairs"]) > 0 for r in alone) >= len(alone) // 2
+
+
 def test_admission_below_reserve_serialises_requests():
     w = _small_workload(8)
     srv = _server(w, slots=4)
