@@ -80,6 +80,7 @@ struct MeshSlot {
     std::vector<uint32_t> h_voff, h_toff, h_vtiles, h_ttiles, h_vtile0;
     int morton_axis = 10;      // Morton bits per axis of the kept sorted keys
     double src_lo[3] = {0.0, 0.0, 0.0}, src_hi[3] = {0.0, 0.0, 0.0};  // bounds of the source vertices
+    int src_grid = -1;  // the source vertices are all integers within 2^26 (1), not (0), unknown (-1)
     uint64_t generation = 0;   // bumped whenever the slot's mesh or buffers are replaced
     DevBuf src, tri, v64, vi, tbox, keys, vals, tkeys, tvals, nodes, range, blocks;
     DevBuf samples;  // every 2^key_sample_shift(nt)-th sorted Morton key (multi-tile sorts only)
@@ -110,24 +111,57 @@ Xform identity_xform() {
     return x;
 }
 
+constexpr double kGridBound = 67108864.0;  // 2^26: the grid stage's coordinate bound
+
 // bounds of n vertices (NaN coordinates skipped; an empty set gives zeros),
-// copying them to `copy` when it is non-null (one pass over the input)
-void vertex_bounds(const double* xyz, size_t n, double* lo, double* hi, double* copy = nullptr) {
+// copying them to `copy` when it is non-null (one pass over the input).
+// Returns whether every coordinate is an integer within 2^26 (the vertex
+// stage's grid test on unmoved vertices).
+bool vertex_bounds(const double* xyz, size_t n, double* lo, double* hi, double* copy = nullptr) {
     double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool grid = true;
     for (size_t i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) {
             const double x = xyz[3 * i + k];
             if (copy) copy[3 * i + k] = x;
             l[k] = x < l[k] ? x : l[k];
             h[k] = x > h[k] ? x : h[k];
+            grid = grid && x == rint(x) && fabs(x) <= kGridBound;
         }
     for (int k = 0; k < 3; ++k) {
         lo[k] = l[k] <= h[k] ? l[k] : 0.0;
         hi[k] = l[k] <= h[k] ? h[k] : 0.0;
     }
+    return grid;
 }
 
 int bits_for(uint32_t n) { return n <= 1 ? 1 : 32 - __builtin_clz(n - 1); }
+
+// What the vertex stage's grid test will find for a slot, when the host can
+// tell: 1 on the grid, 0 off it, -1 unknown. Unmoved vertices keep their
+// source classification; snapped ones are integers, within the bound when
+// the moved corners of the source box are (margin for the rounding of the
+// map); other moves are left to the device.
+int predicted_grid(const MeshSlot& m) {
+    if (m.src_grid < 0) return -1;
+    if (m.xf.identity) {
+        if (!m.xf.snap || m.src_grid == 1) return m.src_grid;
+        const double r = std::max(std::max(fabs(m.src_lo[0]), fabs(m.src_hi[0])),
+                                  std::max(std::max(fabs(m.src_lo[1]), fabs(m.src_hi[1])),
+                                           std::max(fabs(m.src_lo[2]), fabs(m.src_hi[2]))));
+        return r <= kGridBound - 2.0 ? 1 : -1;
+    }
+    if (!m.xf.snap) return -1;
+    for (int c = 0; c < 8; ++c) {
+        const double p[3] = {(c & 1) ? m.src_hi[0] : m.src_lo[0], (c & 2) ? m.src_hi[1] : m.src_lo[1],
+                             (c & 4) ? m.src_hi[2] : m.src_lo[2]};
+        for (int r = 0; r < 3; ++r) {
+            const double v = m.xf.m[r][0] * p[0] + m.xf.m[r][1] * p[1] + m.xf.m[r][2] * p[2] + m.xf.m[r][3];
+            if (!(fabs(v) <= kGridBound - 2.0)) return -1;
+        }
+    }
+    return 1;
+}
 
 }  // namespace
 
@@ -167,6 +201,8 @@ struct gi_context {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<uint64_t> graph_sig, graph_sig_seen;
     bool graph_off = false;  // capture failed once: queries stay eager
+    int exact_kinds = kExactGrid | kExactWide;  // exact stages the last presync launched
+    bool exact_both = false;  // this query launches both exact stages (a prediction failed)
     bool capturing = false;  // a capture is open: timing events become graph nodes
     bool graph_chunk_timed = false;  // what the capture's build left in chunk_timed/leaves
     uint32_t graph_chunk_leaves = 0;
@@ -521,6 +557,16 @@ size_t output_seg_slots(const gi_context* ctx) {
     return std::max<size_t>(std::min<size_t>((size_t)kMaxSegmentsPerPair * cap, ctx->cap_seg), 1);
 }
 
+// The exact stages a query must launch: only the one the grid flags will
+// select when both slots' classification is known on the host.
+int exact_kinds(const gi_context* ctx) {
+    if (ctx->exact_both) return kExactGrid | kExactWide;
+    const int a = predicted_grid(ctx->mesh[0]), b = predicted_grid(ctx->mesh[1]);
+    if (a == 1 && b == 1) return kExactGrid;
+    if (a == 0 || b == 0) return kExactWide;
+    return kExactGrid | kExactWide;
+}
+
 void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool timed) {
     hipStream_t s = ctx->stream;
     MeshSlot& MA = ctx->mesh[0];
@@ -545,10 +591,11 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
     launch_filter(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, grid, ctx->exact.get<uint2>(), ctx->cap_exact, counters,
                   ctx->tune.filter_blocks, s);
     if (timed) record(ctx, ctx->ev[3], s);
+    ctx->exact_kinds = exact_kinds(ctx);
     launch_exact(A, B, ctx->exact.get<uint2>(), ctx->cap_exact, grid, bitsB, ctx->rec_key.get<uint64_t>(),
                  ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<uint64_t>(), ctx->seg_exp.get<int32_t>(),
                  ctx->cap_seg, ctx->cand.get<uint2>(), ctx->cap_cand, ctx->resid.get<uint2>(), ctx->cap_exact,
-                 counters, ctx->tune.exact_blocks, s);
+                 counters, ctx->tune.exact_blocks, ctx->exact_kinds, s);
     if (timed) record(ctx, ctx->ev[4], s);
     // sorted output of a small result set, before the read-back (buffers
     // from ensure_output_bufs)
@@ -611,6 +658,7 @@ std::vector<uint64_t> presync_signature(gi_context* ctx, const gi_options& o) {
     v.push_back((uint64_t)ctx->tune.filter_blocks);
     v.push_back((uint64_t)ctx->tune.exact_blocks);
     v.push_back((uint64_t)ctx->tune.build_chunk);
+    v.push_back((uint64_t)exact_kinds(ctx));
     v.push_back(ctx->cap_cand);
     v.push_back(ctx->cap_exact);
     v.push_back(ctx->cap_rec);
@@ -896,6 +944,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     const uint32_t* hc = ctx->h_counters;
     const uint32_t* hgrid = hc + CNT_COUNT;
     ensure_output_bufs(ctx, o);
+    ctx->exact_both = false;
     const bool graphed = launch_presync_graph(ctx, o);
     if (!graphed) enqueue_presync(ctx, o, true, true);
     for (int attempt = 0;; ++attempt) {
@@ -908,7 +957,11 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
             throw GiError{GI_ERR_TOO_LARGE, "more candidate pairs than a 32-bit list can hold"};
         bool over = hc[CNT_TRAVERSAL_HITS] > ctx->cap_cand || hc[CNT_EXACT_LIST] > ctx->cap_exact ||
                     hc[CNT_RESULTS] > ctx->cap_rec || hc[CNT_SEGMENTS] > ctx->cap_seg;
-        if (!over) break;
+        // the exact stage the grid flags selected must have run (a host
+        // prediction that failed reruns the query with both)
+        const bool ran = ctx->exact_kinds & ((hgrid[0] && hgrid[1]) ? kExactGrid : kExactWide);
+        if (!ran) ctx->exact_both = true;
+        if (!over && ran) break;
         if (attempt > 8) throw GiError{GI_ERR_INTERNAL, "candidate capacity did not converge"};
         uint32_t nc = hc[CNT_TRAVERSAL_HITS] > ctx->cap_cand ? grow(hc[CNT_TRAVERSAL_HITS], ctx->cap_cand) : ctx->cap_cand;
         uint32_t ne = hc[CNT_EXACT_LIST] > ctx->cap_exact ? grow(hc[CNT_EXACT_LIST], ctx->cap_exact) : ctx->cap_exact;
@@ -1172,7 +1225,7 @@ int gi_set_mesh(gi_context* ctx, int slot, const double* xyz, uint32_t nv, const
         ++m.generation;
         m.forest = 0;
         m.h_xyz.resize(3 * (size_t)nv);
-        vertex_bounds(xyz, nv, m.src_lo, m.src_hi, m.h_xyz.data());
+        m.src_grid = vertex_bounds(xyz, nv, m.src_lo, m.src_hi, m.h_xyz.data()) ? 1 : 0;
         m.h_tri.assign(tri, tri + 3 * (size_t)nt);
         m.nv = nv;
         m.nt = nt;
@@ -1210,7 +1263,7 @@ int gi_set_mesh_device(gi_context* ctx, int slot, const double* d_xyz, uint32_t 
         if (nv) HIP_TRY(hipMemcpyAsync(m.h_xyz.data(), d_xyz, 24 * (size_t)nv, hipMemcpyDeviceToHost, ctx->stream));
         if (nt) HIP_TRY(hipMemcpyAsync(m.h_tri.data(), d_tri, 12 * (size_t)nt, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        vertex_bounds(m.h_xyz.data(), nv, m.src_lo, m.src_hi);
+        m.src_grid = vertex_bounds(m.h_xyz.data(), nv, m.src_lo, m.src_hi) ? 1 : 0;
         for (size_t i = 0; i < 3 * (size_t)nt; ++i)
             if (m.h_tri[i] >= nv) {
                 m.set = false;
@@ -1361,6 +1414,7 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         d.dirty = false;
         d.xf = s.xf;
         for (int k = 0; k < 3; ++k) { d.src_lo[k] = s.src_lo[k]; d.src_hi[k] = s.src_hi[k]; }
+        d.src_grid = s.src_grid;
         d.sorted_in_tmp = s.sorted_in_tmp;
         d.morton_axis = s.morton_axis;
         d.full_tree = s.full_tree;
@@ -1426,6 +1480,7 @@ void fill_forest(MeshSlot& m, const gi_mesh_pair* pairs, uint32_t n, bool side_b
         m.h_toff[i + 1] = to;
     }
     m.h_vtile0[n] = (uint32_t)(m.h_vtiles.size() / 2);
+    m.src_grid = vertex_bounds(m.h_xyz.data(), nv, m.src_lo, m.src_hi) ? 1 : 0;
     m.set = true;
     m.on_device = false;
     m.built = false;

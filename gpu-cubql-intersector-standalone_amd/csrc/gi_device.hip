@@ -1842,20 +1842,26 @@ __global__ __launch_bounds__(256) void k_exact_wide(MeshDev A, MeshDev B, const 
 void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
                   int bitsB, uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, uint64_t* seg_pool,
                   int32_t* seg_exp, uint32_t seg_cap, uint2* defer, uint32_t defer_cap, uint2* resid,
-                  uint32_t resid_cap, uint32_t* counters, int blocks, hipStream_t s) {
+                  uint32_t resid_cap, uint32_t* counters, int blocks, int kinds, hipStream_t s) {
     if (A.nt == 0 || B.nt == 0) return;
-    // one of each pair of launches returns at once (the grid flags are
-    // decided on the device by the vertex stage)
-    k_exact<true><<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
-                                         seg_pool, seg_cap, defer, defer_cap, counters);
-    k_exact_wide<true><<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
-                                              rec_cap, seg_pool, seg_exp, seg_cap, defer, defer_cap, resid,
-                                              resid_cap, counters);
-    // the deferred (coplanar) pairs: their count is read on the device
-    k_exact<false><<<blocks, 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
-                                          seg_pool, seg_cap, nullptr, 0, counters);
-    k_exact_wide<false><<<blocks, 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
-                                               seg_pool, seg_exp, seg_cap, nullptr, 0, resid, resid_cap, counters);
+    // kinds & 1: the grid stage, & 2: the wide stage; with both, the one the
+    // grid flags (decided on the device by the vertex stage) do not select
+    // returns at once
+    if (kinds & kExactGrid) {
+        k_exact<true><<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
+                                             rec_cap, seg_pool, seg_cap, defer, defer_cap, counters);
+        // the deferred (coplanar) pairs: their count is read on the device
+        k_exact<false><<<blocks, 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
+                                              seg_pool, seg_cap, nullptr, 0, counters);
+    }
+    if (kinds & kExactWide) {
+        k_exact_wide<true><<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
+                                                  rec_cap, seg_pool, seg_exp, seg_cap, defer, defer_cap, resid,
+                                                  resid_cap, counters);
+        k_exact_wide<false><<<blocks, 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key, rec_seg,
+                                                   rec_cap, seg_pool, seg_exp, seg_cap, nullptr, 0, resid, resid_cap,
+                                                   counters);
+    }
     GI_LAUNCH_CHECK();
 }
 
@@ -1889,40 +1895,57 @@ void launch_gather_pairs(const uint64_t* sorted_key, const uint32_t* order, cons
     GI_LAUNCH_CHECK();
 }
 
-// a `limbs`-word (2 or 4) two's complement integer as a double: its
-// magnitude's words summed from the top, the sign applied last
-__device__ __forceinline__ double limbs_to_double(const uint64_t* __restrict__ x, int limbs) {
-    const bool neg = (x[limbs - 1] >> 63) != 0;
-    uint64_t m[kSegLimbsWide];
+// an L-word (2 or 4) two's complement integer as a double: its magnitude's
+// words summed from the top, the sign applied last
+template <int L>
+__device__ __forceinline__ double limbs_to_double(const uint64_t* __restrict__ x) {
+    const bool neg = (x[L - 1] >> 63) != 0;
+    uint64_t m[L];
     uint64_t c = 1;
-    for (int l = 0; l < limbs; ++l) {
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
         const uint64_t v = neg ? ~x[l] : x[l];
         m[l] = neg ? v + c : v;
         c = neg ? (c && m[l] == 0) : 0;
     }
     double d = 0.0;
-    for (int l = limbs - 1; l >= 0; --l) d = d * 18446744073709551616.0 + (double)m[l];
+#pragma unroll
+    for (int l = L - 1; l >= 0; --l) d = d * 18446744073709551616.0 + (double)m[l];
     return neg ? -d : d;
 }
 
 // the rs.y segments of one record (at rs.x in the pool) to dst onward:
-// homogeneous endpoints of `limbs` words, their power-of-two scales (wide
-// form), and the doubles they divide to
+// homogeneous endpoints of L words, their power-of-two scales (wide form,
+// pool_exp non-null), and the doubles they divide to
+template <int L>
 __device__ __forceinline__ void copy_record_segments(uint2 rs, uint32_t dst, const uint64_t* __restrict__ pool,
-                                                     const int32_t* __restrict__ pool_exp, int limbs,
+                                                     const int32_t* __restrict__ pool_exp,
                                                      uint64_t* __restrict__ out, int32_t* __restrict__ out_exp,
                                                      double* __restrict__ f64) {
-    const int words = 8 * limbs;
+    constexpr int kWords = 8 * L;
     for (uint32_t s = 0; s < rs.y; ++s) {
-        const uint64_t* src = pool + (size_t)words * (rs.x + s);
-        uint64_t* o = out + (size_t)words * (dst + s);
+        const uint64_t* src = pool + (size_t)kWords * (rs.x + s);
+        uint64_t* o = out + (size_t)kWords * (dst + s);
         double* f = f64 + 6 * (size_t)(dst + s);
+        uint64_t w[kWords];
+#pragma unroll
+        for (int i = 0; i < kWords; i += 2) {
+            const ulonglong2 v = *(const ulonglong2*)(src + i);
+            w[i] = v.x;
+            w[i + 1] = v.y;
+        }
+#pragma unroll
+        for (int i = 0; i < kWords; i += 2) *(ulonglong2*)(o + i) = make_ulonglong2(w[i], w[i + 1]);
         const int e2 = pool_exp ? pool_exp[rs.x + s] : 0;
-        for (int w = 0; w < words; ++w) o[w] = src[w];
         if (out_exp) out_exp[dst + s] = e2;
+#pragma unroll
         for (int e = 0; e < 2; ++e) {
-            const double wd = limbs_to_double(src + (4 * e + 3) * limbs, limbs);
-            for (int k = 0; k < 3; ++k) f[3 * e + k] = ldexp(limbs_to_double(src + (4 * e + k) * limbs, limbs) / wd, e2);
+            const double wd = limbs_to_double<L>(w + (4 * e + 3) * L);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double v = limbs_to_double<L>(w + (4 * e + k) * L) / wd;
+                f[3 * e + k] = pool_exp ? ldexp(v, e2) : v;
+            }
         }
     }
 }
@@ -1934,7 +1957,9 @@ __global__ void k_gather_segments(const uint32_t* __restrict__ order, const uint
                                   double* __restrict__ f64) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    copy_record_segments(rec_seg[order[i]], seg_offsets[i], pool, pool_exp, limbs, out, out_exp, f64);
+    if (limbs == kSegLimbsGrid) copy_record_segments<kSegLimbsGrid>(rec_seg[order[i]], seg_offsets[i], pool, nullptr,
+                                                                     out, nullptr, f64);
+    else copy_record_segments<kSegLimbsWide>(rec_seg[order[i]], seg_offsets[i], pool, pool_exp, out, out_exp, f64);
 }
 
 void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const uint32_t* seg_offsets, uint32_t n,
@@ -2161,10 +2186,9 @@ __global__ __launch_bounds__(256) void k_gather_segments_small(const uint32_t* _
     uint32_t R;
     if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R) || seg_offsets[R] > out_slots) return;
     const bool grid = grid_flags[0] && grid_flags[1];
-    const int limbs = grid ? kSegLimbsGrid : kSegLimbsWide;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x)
-        copy_record_segments(rec_seg[order[i]], seg_offsets[i], pool, grid ? nullptr : pool_exp, limbs, out,
-                             grid ? nullptr : out_exp, f64);
+        if (grid) copy_record_segments<kSegLimbsGrid>(rec_seg[order[i]], seg_offsets[i], pool, nullptr, out, nullptr, f64);
+        else copy_record_segments<kSegLimbsWide>(rec_seg[order[i]], seg_offsets[i], pool, pool_exp, out, out_exp, f64);
 }
 
 uint32_t output_small_max() { return kOutSmallMax; }
