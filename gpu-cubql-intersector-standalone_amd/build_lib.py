@@ -21,7 +21,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgi.so")
 ROOT = os.path.dirname(HERE)
 
-SOURCES = ["gi_device.hip", "gi_api.hip", "gi_serve.cpp", "gi_meshio.cpp", "host_exact.cpp", "../host/KernelBVHController.cpp"]
+SOURCES = ["gi_device.hip", "gi_api.hip", "gi_noalloc.hip", "gi_serve.cpp", "gi_meshio.cpp", "host_exact.cpp", "../host/KernelBVHController.cpp"]
 FACADE_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "facade_test.cpp")
 FACADE_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "facade_test")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

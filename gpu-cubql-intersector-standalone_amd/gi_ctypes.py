@@ -101,7 +101,7 @@ _lib = None
 SYMBOLS = ["gi_create", "gi_destroy", "gi_last_error", "gi_status_string", "gi_set_mesh", "gi_set_mesh_device",
            "gi_set_transform", "gi_set_transform_matrix", "gi_build", "gi_get_vertices", "gi_intersect",
            "gi_result_free",
-           "gi_intersect_oneshot", "gi_set_batch", "gi_intersect_batch", "gi_share_mesh", "gi_get_tuning",
+           "gi_intersect_oneshot", "gi_intersect_noalloc", "gi_set_batch", "gi_intersect_batch", "gi_share_mesh", "gi_get_tuning",
            "gi_set_tuning", "gi_autotune", "gi_clear_device",
            "gi_reconstruct_device",
            "gi_server_create", "gi_server_add_mesh", "gi_server_submit", "gi_server_wait", "gi_server_destroy",
@@ -132,6 +132,10 @@ def lib():
         L.gi_result_free.argtypes = [C.POINTER(GiResult)]
         L.gi_intersect_oneshot.argtypes = [C.POINTER(GiConfig), dp, C.c_uint32, up, C.c_uint32, dp, C.c_uint32, up,
                                            C.c_uint32, C.POINTER(GiOptions), C.POINTER(GiResult)]
+        u64p = C.POINTER(C.c_uint64)
+        L.gi_intersect_noalloc.argtypes = [C.c_int32, dp, C.c_uint32, up, C.c_uint32, dp, C.c_uint32, up, C.c_uint32,
+                                           C.c_int32, up, C.c_uint64, up, u64p, C.POINTER(C.c_int32), C.c_uint64,
+                                           u64p, u64p, C.POINTER(C.c_int32), C.POINTER(GiStats)]
         L.gi_clear_device.argtypes = [vp]
         L.gi_reconstruct_device.argtypes = [vp]
         ip = C.POINTER(C.c_int32)
@@ -345,6 +349,45 @@ def intersect_oneshot(xa, ta, xb, tb, want_segments=True, device=0, decode=True)
         return result_to_py(res, want_segments, decode)
     finally:
         lib().gi_result_free(C.byref(res))
+
+
+def intersect_noalloc(xa, ta, xb, tb, want_segments=True, device=0, pairs_cap=None, seg_cap=None):
+    """gi_intersect_noalloc into buffers this wrapper allocates (sized for
+    every triangle pair of the smaller mesh times 6 segments unless given).
+    Returns (status, result dict or None, needed (pairs, segments))."""
+    xa = np.ascontiguousarray(xa, dtype=np.float64).reshape(-1, 3)
+    ta = np.ascontiguousarray(ta, dtype=np.uint32).reshape(-1, 3)
+    xb = np.ascontiguousarray(xb, dtype=np.float64).reshape(-1, 3)
+    tb = np.ascontiguousarray(tb, dtype=np.uint32).reshape(-1, 3)
+    if pairs_cap is None:
+        pairs_cap = 8 * (ta.shape[0] + tb.shape[0]) + 1024
+    if seg_cap is None:
+        seg_cap = pairs_cap
+    pairs = np.zeros(2 * max(pairs_cap, 1), dtype=np.uint32)
+    offs = np.zeros(pairs_cap + 1, dtype=np.uint32)
+    exact = np.zeros(max(seg_cap, 1) * 8 * 4, dtype=np.uint64)
+    exp2 = np.zeros(max(seg_cap, 1), dtype=np.int32)
+    n_pairs, n_seg, limbs = C.c_uint64(), C.c_uint64(), C.c_int32()
+    st = GiStats()
+    dp, up = C.POINTER(C.c_double), C.POINTER(C.c_uint32)
+    rc = lib().gi_intersect_noalloc(device, xa.ctypes.data_as(dp), xa.shape[0], ta.ctypes.data_as(up), ta.shape[0],
+                                    xb.ctypes.data_as(dp), xb.shape[0], tb.ctypes.data_as(up), tb.shape[0],
+                                    int(want_segments), pairs.ctypes.data_as(up), pairs_cap, offs.ctypes.data_as(up),
+                                    exact.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                    exp2.ctypes.data_as(C.POINTER(C.c_int32)), seg_cap, C.byref(n_pairs),
+                                    C.byref(n_seg), C.byref(limbs), C.byref(st))
+    need = (int(n_pairs.value), int(n_seg.value))
+    if rc != 0:
+        return rc, None, need
+    n, S, L = need[0], need[1], int(limbs.value)
+    out = dict(pairs=pairs[: 2 * n].reshape(n, 2).copy(), seg_offsets=offs[: n + 1].copy(), stats=st.as_dict(),
+               n_segments=S, exact_limbs=L)
+    if want_segments:
+        ex = exact[: S * 8 * L].reshape(S, 8, L)
+        out["seg_exact"] = ex.copy()
+        out["seg_exp2"] = exp2[:S].copy()
+        out["segments"] = decode_segments(ex, exp2[:S], out["seg_offsets"]) if S else [[] for _ in range(n)]
+    return rc, out, need
 
 
 def result_to_py(res: GiResult, want_segments=True, decode=True):

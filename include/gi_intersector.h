@@ -163,6 +163,39 @@ int gi_intersect_oneshot(const gi_config* cfg, const double* xyzA, uint32_t nvA,
                          const double* xyzB, uint32_t nvB, const uint32_t* triB, uint32_t ntB,
                          const gi_options* opt, gi_result* out);
 
+/* ------------------------------------------------------------- no allocation
+ * gi_intersect_noalloc: no counterpart in the reference. It is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same result as gi_intersect_oneshot
+ * (sorted pairs and exact segments of two meshes given as host arrays),
+ * computed without any allocation by the library. Every intermediate lives in
+ * one static device array of GI_NOALLOC_SCRATCH_MIB MiB, which each GPU the
+ * library runs on holds until the process ends; work runs on the null
+ * stream; results go to the caller's host buffers.
+ * Limits: at most GI_NOALLOC_MAX_TRIANGLES triangles and
+ * GI_NOALLOC_MAX_VERTICES vertices per mesh (else GI_ERR_TOO_LARGE before any
+ * GPU work), and candidate lists and results within what the array holds
+ * after both meshes (GI_ERR_TOO_LARGE). Pairs off the integer grid that would
+ * need the host residual stage are refused (GI_ERR_TOO_LARGE; use
+ * gi_intersect_oneshot). When the caller's buffers are too small the call
+ * returns GI_ERR_TOO_LARGE with *n_pairs / *n_segments set to what is needed.
+ * Calls share each GPU's array: calls that overlap on one GPU race, so
+ * callers serialise them per GPU.
+ *   pairs:       2 * pairs_cap (a, b), sorted, unique
+ *   seg_offsets: pairs_cap + 1 (always written: at least the leading 0)
+ *   seg_exact:   seg_cap * 8 * 4 words; written as in gi_result with
+ *                *exact_limbs (2 or 4) words per number
+ *   seg_exp2:    seg_cap scales (zero for grid meshes)
+ *   stats:       may be NULL; stage counts, no timings */
+#define GI_NOALLOC_SCRATCH_MIB 128
+#define GI_NOALLOC_MAX_TRIANGLES (1u << 17)
+#define GI_NOALLOC_MAX_VERTICES (1u << 18)
+int gi_intersect_noalloc(int32_t device, const double* xyzA, uint32_t nvA, const uint32_t* triA, uint32_t ntA,
+                         const double* xyzB, uint32_t nvB, const uint32_t* triB, uint32_t ntB, int32_t want_segments,
+                         uint32_t* pairs, uint64_t pairs_cap, uint32_t* seg_offsets, uint64_t* seg_exact,
+                         int32_t* seg_exp2, uint64_t seg_cap, uint64_t* n_pairs, uint64_t* n_segments,
+                         int32_t* exact_limbs, gi_stats* stats);
+
 /* ------------------------------------------------------------- batches
  * Many independent mesh pairs in one pass: every stage (vertex preparation,
  * boxes, Morton sort, BVH build, traversal, filter, exact stage, result sort)
