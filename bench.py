@@ -11,20 +11,33 @@ exact int128 stage with
 exact segments, sort of the intersecting pairs and gather of their segments
 (output left in device memory).
 
-value = exact candidate pairs decided per second (candidates / step time).
+value = exact candidate pairs decided per second (candidates / step time),
+inputs resident in HBM. `e2e` times the same work from pinned host arrays
+(both meshes uploaded, A's BVH built, the query, sorted pairs and exact
+segments copied to host memory), labelled separately.
 roofline = HBM roofline of the dominant kernel, the longer of the step's two
 largest single launches (the traversal and A's BVH build), each timed by the
 library's HIP events around its one launch per step; the other one is
-reported under roofline_other. Achieved bandwidth = the kernel's algorithmic
-bytes per launch / its average duration; traffic = HBM bytes per launch from
-the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload.
+reported under roofline_other. Achieved bandwidth = the kernel's bytes per
+launch / its average duration, where the bytes are the cost model's, capped
+by the measured HBM traffic of the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+summary when that is lower (the traversal reads only the part of A's tree
+near B); the uncapped figure is reported as achieved_model.
+pipeline.predicates: orient3d / orient2d signs evaluated per cascade stage
+(FP64 filter, GPU exact stage, host residual stage) and their rates.
+Other legs: batched (4,096 pairs: one pass vs the per-pair loop, both from
+device-resident inputs and both from host arrays), degeneracy (the 512x512
+height field against its half-cell shift and its 90-degree copy), serving
+(unsnapped, the reference's transform semantics, and snapped).
 The CPU baseline is the oracle (C restatement of the reference CPU path:
 box sweep + filtered exact predicates + exact segments) on the same meshes,
-timed on this host's cores (median of 5 runs after a warm-up).
+timed on this host's cores (median of 5 runs after a warm-up), plus one
+thread on a sample.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -83,6 +96,63 @@ def measured_traffic(kernel):
         return None
 
 
+class PinnedArrays:
+    """Host arrays in pinned (page-locked) memory from hipHostMalloc of the
+    HIP runtime the library uses: the source of the end-to-end timing."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.ptrs = []
+
+    def copy_of(self, a):
+        a = np.ascontiguousarray(a)
+        p = ctypes.c_void_p()
+        assert self.hip.hipHostMalloc(ctypes.byref(p), ctypes.c_size_t(max(a.nbytes, 1)), 0) == 0
+        self.ptrs.append(p)
+        out = np.frombuffer((ctypes.c_char * max(a.nbytes, 1)).from_address(p.value), dtype=a.dtype,
+                            count=a.size).reshape(a.shape)
+        out[...] = a
+        return out
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipHostFree(p)
+        self.ptrs = []
+
+
+class DeviceArena:
+    """Device copies of host arrays (hipMalloc of the library's HIP runtime),
+    for queries from device-resident inputs."""
+
+    def __init__(self, nbytes):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.base = ctypes.c_void_p()
+        assert self.hip.hipMalloc(ctypes.byref(self.base), ctypes.c_size_t(max(nbytes, 256))) == 0
+        self.off = 0
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        ptr = self.base.value + self.off
+        if a.nbytes:
+            assert self.hip.hipMemcpy(ctypes.c_void_p(ptr), a.ctypes.data_as(ctypes.c_void_p),
+                                      ctypes.c_size_t(a.nbytes), 1) == 0  # host to device
+        self.off += (a.nbytes + 255) // 256 * 256
+        return ptr
+
+    def free(self):
+        self.hip.hipFree(self.base)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def device_copy(nbytes, reps):
     """Plain device-to-device copies of nbytes (the reference dispatch for
     scaling the rocprofv3 FETCH_SIZE / WRITE_SIZE counters), through the HIP
@@ -117,25 +187,41 @@ def cpu_baseline(xa, ta, xb, tb, stride, reps):
     dt = float(np.median(times))
     c = r["stats"]["n_candidates"]
     frac = "all" if stride == 1 else f"every {stride}th triangle"
-    return {"value": c / dt, "unit": "exact tri-pairs/s", "cores": cores, "kind": "port",
+    # one thread on every 16th triangle of A (about as much work as one run above per core)
+    sub1 = ta[::16 * stride]
+    t1 = []
+    for i in range(3):
+        t0 = time.perf_counter()
+        r1 = oracle_py.intersect(xa, sub1, xb, tb, nthreads=1, use_filter=True, want_segments=True,
+                                 decode_segments=False)
+        if i:
+            t1.append(time.perf_counter() - t0)
+    d1 = float(np.median(t1))
+    c1 = r1["stats"]["n_candidates"]
+    return {"value": c / dt, "unit": "exact tri-pairs/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"{frac} of mesh A ({len(sub)} of {len(ta)}) vs all of mesh B, median of {reps} runs after "
                       f"one warm-up: {c} candidates, {r['stats']['n_pairs']} pairs with exact segments, "
-                      f"{dt:.2f} s per run ({sum(times):.1f} s timed)"}
+                      f"{dt:.2f} s per run ({sum(times):.1f} s timed)",
+            "single_thread": {"value": c1 / d1, "unit": "exact tri-pairs/s", "cores": 1,
+                              "sample": f"every {16 * stride}th triangle of mesh A ({len(sub1)}) vs all of mesh B, "
+                                        f"median of 2 runs after one warm-up: {c1} candidates, {d1:.2f} s per run"}}
 
 
-def serving_leg(gi, mg, n_requests, qps, slots):
+def serving_leg(gi, mg, n_requests, qps, slots, snap):
     """Serving workload: a resident 2,097,152-triangle displaced sphere; Poisson
     requests (seeded) moving a 262,144-triangle torus (80 %) or a 4,096-triangle
     probe (20 %) by a random rigid motion; each request = transform + refit of
     the moving mesh, query against the resident BVH, exact segments, host copy.
-    Open loop at `qps`, then a closed burst of the same requests (all queued at
-    once) for the saturation throughput."""
+    snap=False keeps the moved doubles (the reference's transform; the GPU
+    exact stage works in per-pair integer frames), snap=True rounds them to the
+    grid. Open loop at `qps`, then a closed burst of the same requests (all
+    queued at once) for the saturation throughput."""
     import queue
     import threading
 
     w = mg.serving_workload(n_requests=n_requests, qps=qps)
     t_setup = time.perf_counter()
-    srv = gi.Server(*w["static"], slots=slots, snap=True, want_segments=True)
+    srv = gi.Server(*w["static"], slots=slots, snap=snap, want_segments=True)
     for m in w["meshes"]:
         srv.add_mesh(*m)
     setup_s = time.perf_counter() - t_setup
@@ -154,7 +240,7 @@ def serving_leg(gi, mg, n_requests, qps, slots):
                 if t is None:
                     return
                 r = srv.wait(t, decode=False)
-                out.append((r["info"], len(r["pairs"]), r["n_segments"]))
+                out.append((r["info"], len(r["pairs"]), r["n_segments"], r["stats"]))
 
         th = threading.Thread(target=waiter)
         th.start()
@@ -178,15 +264,19 @@ def serving_leg(gi, mg, n_requests, qps, slots):
     def pct(v, q):
         return float(np.percentile(np.asarray(v), q))
 
-    lat = [i["latency_ms"] for i, _, _ in open_loop]
-    svc = [i["service_ms"] for i, _, _ in open_loop]
+    lat = [i["latency_ms"] for i, _, _, _ in open_loop]
+    svc = [i["service_ms"] for i, _, _, _ in open_loop]
     by_mesh = {}
-    for (_, mid, _), (i, npairs, _) in zip(reqs, open_loop):
+    for (_, mid, _), (i, npairs, _, _) in zip(reqs, open_loop):
         by_mesh.setdefault(mid, []).append(i["service_ms"])
     return {
         "workload": f"static {w['static'][1].shape[0]} triangles resident; {len(reqs)} Poisson requests at "
                     f"{qps:g}/s (seed 7): torus {w['meshes'][0][1].shape[0]} / probe "
-                    f"{w['meshes'][1][1].shape[0]} triangles, random rigid motion, exact segments to host",
+                    f"{w['meshes'][1][1].shape[0]} triangles, random rigid motion "
+                    f"({'snapped to the grid' if snap else 'moved doubles kept'}), exact segments to host",
+        "snap_to_grid": bool(snap),
+        "host_pairs": int(sum(s["host_pairs"] for _, _, _, s in open_loop)),
+        "exact_pairs": int(sum(s["exact_pairs"] for _, _, _, s in open_loop)),
         "slots": slots, "requests": len(reqs), "qps_offered": qps,
         "qps_achieved": len(reqs) / wall_open,
         "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_ms": max(lat),
@@ -194,10 +284,10 @@ def serving_leg(gi, mg, n_requests, qps, slots):
         "service_p50_torus_ms": pct(by_mesh.get(0, [0.0]), 50),
         "service_p50_probe_ms": pct(by_mesh.get(1, [0.0]), 50),
         "saturation_rps": len(reqs) / wall_burst,
-        "burst_p50_ms": pct([i["latency_ms"] for i, _, _ in burst], 50),
-        "burst_p99_ms": pct([i["latency_ms"] for i, _, _ in burst], 99),
+        "burst_p50_ms": pct([i["latency_ms"] for i, _, _, _ in burst], 50),
+        "burst_p99_ms": pct([i["latency_ms"] for i, _, _, _ in burst], 99),
         "submit_lag_p99_ms": pct(lag, 99) * 1000.0,
-        "pairs_mean": float(np.mean([n for _, n, _ in open_loop])),
+        "pairs_mean": float(np.mean([n for _, n, _, _ in open_loop])),
         "setup_s": setup_s,
     }
 
@@ -207,8 +297,11 @@ def batch_leg(gi, mg, n_pairs, steps, autotune=True):
     icosphere, displaced sphere or torus with N log-uniform in [500, 200000],
     B placed at U[0.3, 1.5] radii from A, 2^24 grid. Batched = one pass of
     launches over the whole batch (gi_intersect_batch); plain = the
-    single-pair API once per pair (gi_set_mesh x2 + gi_intersect). Both are
-    also timed end to end from host arrays (uploads included)."""
+    single-pair API once per pair (set both meshes + gi_intersect). Compared
+    two ways, each like for like: both from device-resident source arrays
+    (batched: a query with rebuild after gi_set_batch; plain:
+    gi_set_mesh_device x2 + gi_intersect per pair, from one arena uploaded
+    beforehand), and both end to end from host arrays (uploads included)."""
     from ctypes import byref
 
     t_gen = time.perf_counter()
@@ -241,15 +334,35 @@ def batch_leg(gi, mg, n_pairs, steps, autotune=True):
     sample = {i: r for i, r in enumerate(eng.intersect_batch(want_segments=False)[0]) if i % 97 == 0}
     eng.close()
 
+    # plain, from device-resident source arrays: every pair's arrays uploaded
+    # to one device arena first (not timed)
+    nbytes = sum(sum((a.nbytes + 255) // 256 * 256 for a in m) for pr in pairs for m in pr)
+    arena = DeviceArena(nbytes)
+    dev = [tuple((arena.put(x), x.shape[0], arena.put(t), t.shape[0]) for x, t in pr) for pr in pairs]
     plain = gi.Intersector()
     plain_counts, plain_cands, plain_sample = [], 0, {}
+    for _ in range(2):  # warm-up: arenas grow
+        for side, (px, nvx, pt, ntx) in enumerate(dev[0]):
+            plain.set_mesh_device(side, px, nvx, pt, ntx)
+        free(byref(plain.intersect_raw(output_on_device=True, want_segments=True)))
+    t0 = time.perf_counter()
+    for d in dev:
+        for side, (px, nvx, pt, ntx) in enumerate(d):
+            plain.set_mesh_device(side, px, nvx, pt, ntx)
+        res = plain.intersect_raw(output_on_device=True, want_segments=True)
+        plain_counts.append(int(res.n_pairs))
+        plain_cands += int(res.stats.candidates)
+        free(byref(res))
+    plain_resident_s = time.perf_counter() - t0
+    arena.free()
+    # plain, end to end from host arrays
+    plain_e2e_counts = []
     t0 = time.perf_counter()
     for (xa, ta), (xb, tb) in pairs:
         plain.set_mesh(0, xa, ta)
         plain.set_mesh(1, xb, tb)
         res = plain.intersect_raw(output_on_device=True, want_segments=True)
-        plain_counts.append(int(res.n_pairs))
-        plain_cands += int(res.stats.candidates)
+        plain_e2e_counts.append(int(res.n_pairs))
         free(byref(res))
     plain_s = time.perf_counter() - t0
     for i in sample:
@@ -259,6 +372,7 @@ def batch_leg(gi, mg, n_pairs, steps, autotune=True):
         plain_sample[i] = plain.intersect(want_segments=False)["pairs"]
     plain.close()
     parity = bool(np.array_equal(counts, np.asarray(plain_counts)) and plain_cands == st["candidates"] and
+                  plain_e2e_counts == plain_counts and
                   all(np.array_equal(sample[i]["pairs"], plain_sample[i]) for i in sample))
     return {
         "workload": f"{n_pairs} mesh pairs (seed 6), {ntri} triangles in total, N log-uniform in [500, 200000], "
@@ -268,16 +382,63 @@ def batch_leg(gi, mg, n_pairs, steps, autotune=True):
         "batched_mesh_pairs_per_s": n_pairs / step_s,
         "batched_exact_tri_pairs_per_s": st["candidates"] / step_s,
         "batched_stage_ms": {k: st[k] for k in ("ms_build", "ms_traverse", "ms_filter", "ms_exact", "ms_output")},
-        "plain_ms": plain_s * 1e3,
-        "plain_mesh_pairs_per_s": n_pairs / plain_s,
-        "speedup_vs_plain": plain_s / step_s,
+        "plain_resident_ms": plain_resident_s * 1e3,
+        "plain_resident_mesh_pairs_per_s": n_pairs / plain_resident_s,
+        "speedup_resident": plain_resident_s / step_s,
         "e2e_batched_ms": e2e_batch_s * 1e3,
+        "plain_e2e_ms": plain_s * 1e3,
         "speedup_e2e": plain_s / e2e_batch_s,
         "candidates": st["candidates"], "intersecting_pairs": st["pairs"], "segments": st["segments"],
         "parity_with_plain": parity,
         "tuning": tuning,
         "generate_s": gen_s, "set_batch_s": set_s,
     }
+
+
+def degeneracy_leg(gi, mg, steps):
+    """The 512 x 512 height field (524,288 triangles, integer coordinates,
+    heights in {0..8}) against its (1/2, 1/2, 0)-cell shift and against its
+    90-degree copy about z, where coplanar, shared-edge and zero-orientation
+    contacts dominate. Each step as in the main leg (device-resident meshes,
+    both restaged, A's BVH rebuilt, query with exact segments). Reports the
+    fraction of candidates each cascade stage decided and the exact stage's
+    time."""
+    (p, t), (sft, _), (rot, _) = mg.height_field_pairs(cells=512)
+    out = {}
+    for name, q in (("shift", sft), ("rot90", rot)):
+        eng = gi.Intersector()
+        eng.set_mesh(0, p, t)
+        eng.set_mesh(1, q, t)
+        eng.build()
+
+        def step():
+            res = eng.intersect_raw(output_on_device=True, want_segments=True, rebuild=True)
+            s = res.stats.as_dict()
+            gi.lib().gi_result_free(gi.C.byref(res))
+            return s
+
+        for _ in range(2):
+            step()
+        t0 = time.perf_counter()
+        sts = [step() for _ in range(steps)]
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        eng.close()
+        s = sts[-1]
+        c = max(s["candidates"], 1)
+        out[name] = {
+            "triangles": int(t.shape[0]), "ms_per_query": ms, "exact_tri_pairs_per_s": s["candidates"] / (ms / 1e3),
+            "candidates": s["candidates"], "pairs": s["pairs"], "segments": s["segments"], "coplanar": s["coplanar"],
+            "degenerate": s["degenerate"], "grid": s["grid"],
+            "resolved": {"filter_no": s["filter_no"] / c, "filter_green": s["filter_green"] / c,
+                         "exact_yellow": (s["filter_yellow"] - s["host_pairs"]) / c, "host": s["host_pairs"] / c},
+            "exact_yellow_yes": s["exact_yellow_yes"], "exact_yellow_no": s["exact_yellow_no"],
+            "predicates": {"filter": s["filter_predicates"], "exact": s["exact_predicates"],
+                           "filter_per_s": s["filter_predicates"] / max(s["ms_filter"], 1e-6) * 1e3,
+                           "exact_per_s": s["exact_predicates"] / max(s["ms_exact"], 1e-6) * 1e3},
+            "stage_ms": {k: float(np.mean([x[k] for x in sts])) for k in
+                         ("ms_build", "ms_traverse", "ms_filter", "ms_exact", "ms_output")},
+        }
+    return out
 
 
 def main():
@@ -298,6 +459,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched measurement (for kernel profiles of it)")
+    ap.add_argument("--no-degeneracy", action="store_true", help="skip the height-field degeneracy measurement")
+    ap.add_argument("--degeneracy-steps", type=int, default=10)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-qps", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -362,11 +525,14 @@ def main():
                  "k_traverse": avg["ms_traverse"]}
     roofs = {}
     for k, kms in kernel_ms.items():
-        kb = kernel_bytes(k, st, ta.shape[0], tb.shape[0])
+        model = kernel_bytes(k, st, ta.shape[0], tb.shape[0])
+        traffic = measured_traffic(k)
+        kb = min(model, traffic) if traffic else model
         achieved = kb / (kms / 1000.0) / 1e9
         roofs[k] = {"bound": "hbm", "kernel": k, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(k), "kernel_ms": kms,
-                    "launches_timed": launches, "algorithmic_bytes": kb}
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_ms": kms,
+                    "launches_timed": launches, "algorithmic_bytes": kb, "model_bytes": model,
+                    "achieved_model": model / (kms / 1000.0) / 1e9}
     dom = max(kernel_ms, key=kernel_ms.get)
     b_alg = algorithmic_bytes(xa.shape[0], ta.shape[0], xb.shape[0], tb.shape[0], st)
 
@@ -383,8 +549,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64+int128",
         "data": "synthetic (seeded displaced sphere and rotated torus, snapped to the 2^24 grid)",
-        "config": {"workload": f"sphere x torus, {ta.shape[0]} x {tb.shape[0]} triangles, 2^24 grid, "
-                               "both BVH builds + query + exact segments per step",
+        "config": {"workload": f"sphere x torus, {ta.shape[0]} x {tb.shape[0]} triangles, 2^24 grid; per step "
+                               "both meshes staged and Morton-sorted, A's BVH built (B's triangles in Morton "
+                               "order are the queries), query with exact segments",
                    "triangles_a": int(ta.shape[0]), "triangles_b": int(tb.shape[0]),
                    "vertices_a": int(xa.shape[0]), "vertices_b": int(xb.shape[0])},
         "tuning": tuning,
@@ -398,20 +565,39 @@ def main():
                      "intersecting_pairs_per_s": st["pairs"] / (ms / 1000.0),
                      "pair_space_per_s": ta.shape[0] * tb.shape[0] / (ms / 1000.0),
                      "stage_ms": avg, "end_to_end_algorithmic_GBs": b_alg / (ms / 1000.0) / 1e9,
-                     "end_to_end_hbm_frac": b_alg / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS},
+                     "end_to_end_hbm_frac": b_alg / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,
+                     "predicates": {
+                         "filter": st["filter_predicates"], "exact": st["exact_predicates"],
+                         "host": st["host_predicates"],
+                         "filter_per_s": st["filter_predicates"] / (avg["ms_filter"] / 1000.0),
+                         "exact_per_s": st["exact_predicates"] / max(avg["ms_exact"] / 1000.0, 1e-9),
+                         "per_s": (st["filter_predicates"] + st["exact_predicates"]) / (ms / 1000.0)},
+                     "resolved": {k: v / max(st["candidates"], 1) for k, v in (
+                         ("filter_no", st["filter_no"]), ("filter_green", st["filter_green"]),
+                         ("exact_yellow", st["filter_yellow"] - st["host_pairs"]), ("host", st["host_pairs"]))}},
     }
-    # end to end from host arrays (both meshes uploaded and both BVHs built per
-    # run) to sorted pairs and exact segments in host memory
-    e2e = []
-    for _ in range(6):
-        t0 = time.perf_counter()
-        eng.set_mesh(0, xa, ta)
-        eng.set_mesh(1, xb, tb)
-        res = eng.intersect_raw(output_on_device=False, want_segments=True)
-        e2e.append(time.perf_counter() - t0)
-        assert res.n_pairs == st["pairs"]
-        gi.lib().gi_result_free(gi.C.byref(res))
-    line["pipeline"]["e2e_host_ms"] = float(np.median(e2e[1:])) * 1e3
+    # end to end from host arrays (both meshes uploaded, A's BVH built, the
+    # query) to sorted pairs and exact segments in host memory: from pinned
+    # arrays (the figure of the end-to-end definition) and from pageable ones
+    pin = PinnedArrays()
+    pinned = tuple(pin.copy_of(a) for a in (xa, ta, xb, tb))
+    e2e = {}
+    for label, (a0, a1, b0, b1) in (("pinned", pinned), ("pageable", (xa, ta, xb, tb))):
+        times = []
+        for _ in range(6):
+            t0 = time.perf_counter()
+            eng.set_mesh(0, a0, a1)
+            eng.set_mesh(1, b0, b1)
+            res = eng.intersect_raw(output_on_device=False, want_segments=True)
+            times.append(time.perf_counter() - t0)
+            assert res.n_pairs == st["pairs"]
+            gi.lib().gi_result_free(gi.C.byref(res))
+        e2e[label] = float(np.median(times[1:]))
+    pin.free()
+    line["e2e"] = {"value": st["candidates"] / e2e["pinned"], "unit": "exact tri-pairs/s",
+                   "ms": e2e["pinned"] * 1e3, "pageable_ms": e2e["pageable"] * 1e3,
+                   "what": "from pinned host arrays: both meshes uploaded, staged and sorted, A's BVH built, the "
+                           "query, sorted pairs and exact segments copied to host memory (median of 5 after one)"}
     if args.check:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py
@@ -425,8 +611,12 @@ def main():
               file=sys.stderr)
     if not args.no_batch:
         line["batched"] = batch_leg(gi, mg, args.batch_pairs, args.batch_steps, not args.no_autotune)
+    if not args.no_degeneracy:
+        line["degeneracy"] = degeneracy_leg(gi, mg, args.degeneracy_steps)
     if not args.no_serve:
-        line["serving"] = serving_leg(gi, mg, args.serve_requests, args.serve_qps, args.serve_slots)
+        line["serving"] = serving_leg(gi, mg, args.serve_requests, args.serve_qps, args.serve_slots, snap=False)
+        line["serving_snapped"] = serving_leg(gi, mg, args.serve_requests, args.serve_qps, args.serve_slots,
+                                              snap=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(xa, ta, xb, tb, args.cpu_stride, args.cpu_reps)
     print(json.dumps(line), flush=True)

@@ -59,8 +59,12 @@ struct DevBuf {
 };
 
 struct MeshSlot {
+    // host copies of the source arrays, made only when needed (clear /
+    // reconstruct, the host residual stage): the upload reads the caller's
+    // arrays directly
     std::vector<double> h_xyz;
     std::vector<uint32_t> h_tri;
+    bool host_valid = false;
     uint32_t nv = 0, nt = 0;
     bool set = false;
     bool on_device = false;  // device buffers hold the mesh
@@ -113,28 +117,6 @@ Xform identity_xform() {
 
 constexpr double kGridBound = 67108864.0;  // 2^26: the grid stage's coordinate bound
 
-// bounds of n vertices (NaN coordinates skipped; an empty set gives zeros),
-// copying them to `copy` when it is non-null (one pass over the input).
-// Returns whether every coordinate is an integer within 2^26 (the vertex
-// stage's grid test on unmoved vertices).
-bool vertex_bounds(const double* xyz, size_t n, double* lo, double* hi, double* copy = nullptr) {
-    double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
-    bool grid = true;
-    for (size_t i = 0; i < n; ++i)
-        for (int k = 0; k < 3; ++k) {
-            const double x = xyz[3 * i + k];
-            if (copy) copy[3 * i + k] = x;
-            l[k] = x < l[k] ? x : l[k];
-            h[k] = x > h[k] ? x : h[k];
-            grid = grid && x == rint(x) && fabs(x) <= kGridBound;
-        }
-    for (int k = 0; k < 3; ++k) {
-        lo[k] = l[k] <= h[k] ? l[k] : 0.0;
-        hi[k] = l[k] <= h[k] ? h[k] : 0.0;
-    }
-    return grid;
-}
-
 int bits_for(uint32_t n) { return n <= 1 ? 1 : 32 - __builtin_clz(n - 1); }
 
 // What the vertex stage's grid test will find for a slot, when the host can
@@ -180,6 +162,7 @@ struct gi_context {
     uint32_t cand_reserve = 0;
     MeshSlot mesh[2];
     DevBuf counters, hist, scan_tmp, partials;
+    DevBuf inspect;  // partial bounds and the MeshInspect of the last upload
     DevBuf cand, exact, rec_key, rec_seg, seg_pool, seg_exp, resid;
     uint32_t cap_cand = 0, cap_exact = 0, cap_rec = 0, cap_seg = 0;
     DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2, pair_off;
@@ -222,7 +205,7 @@ struct gi_context {
         xf_dev.release();
         xf_current[0] = xf_current[1] = false;
         for (auto& m : mesh) m.release();
-        DevBuf* all[] = {&counters, &hist, &scan_tmp, &partials, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
+        DevBuf* all[] = {&counters, &hist, &scan_tmp, &partials, &inspect, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
                          &seg_exp, &resid, &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2, &pair_off,
                          &bhist[0], &bhist[1], &bscan[0], &bscan[1], &bpart[0], &bpart[1]};
         for (DevBuf* b : all) b->release();
@@ -282,8 +265,10 @@ void sync_all(gi_context* ctx) {
     HIP_TRY(hipStreamSynchronize(ctx->stream));
 }
 
+// Device copies from the host copies (reconstruct after gi_clear_device).
 void upload_mesh(gi_context* ctx, int slot) {
     MeshSlot& m = ctx->mesh[slot];
+    if (!m.host_valid) throw GiError{GI_ERR_INTERNAL, "mesh has neither device nor host copies"};
     double* src = m.src.ensure<double>(3 * (size_t)m.nv);
     uint32_t* tri = m.tri.ensure<uint32_t>(3 * (size_t)m.nt);
     hipStream_t st = ctx->build_stream[slot];
@@ -303,6 +288,54 @@ void upload_mesh(gi_context* ctx, int slot) {
         // the host tables must outlive the asynchronous copies
         HIP_TRY(hipStreamSynchronize(st));
     }
+    m.on_device = true;
+    m.built = false;
+    m.dirty = true;
+}
+
+// The slot's source arrays (already on the device) back to its host copies.
+void ensure_host_copy(gi_context* ctx, int slot) {
+    MeshSlot& m = ctx->mesh[slot];
+    if (m.host_valid || !m.set) return;
+    sync_all(ctx);
+    m.h_xyz.resize(3 * (size_t)m.nv);
+    m.h_tri.resize(3 * (size_t)m.nt);
+    if (m.nv) HIP_TRY(hipMemcpy(m.h_xyz.data(), m.src.get<double>(), 24 * (size_t)m.nv, hipMemcpyDeviceToHost));
+    if (m.nt) HIP_TRY(hipMemcpy(m.h_tri.data(), m.tri.get<uint32_t>(), 12 * (size_t)m.nt, hipMemcpyDeviceToHost));
+    m.host_valid = true;
+}
+
+void drop_host_copy(MeshSlot& m) {
+    std::vector<double>().swap(m.h_xyz);
+    std::vector<uint32_t>().swap(m.h_tri);
+    m.host_valid = false;
+}
+
+// After the source arrays of a slot reached the device (src, tri): bounds,
+// grid classification and index check in one launch, read back. For a batch
+// (tri_local) the indices are per mesh: they are checked against their mesh
+// and made global instead. Throws on an out-of-range index.
+void inspect_upload(gi_context* ctx, int slot, bool tri_local) {
+    MeshSlot& m = ctx->mesh[slot];
+    hipStream_t s = ctx->stream;
+    double* scratch = ctx->inspect.ensure<double>(inspect_scratch_doubles() + sizeof(MeshInspect) / sizeof(double) + 1);
+    MeshInspect* out = (MeshInspect*)(scratch + inspect_scratch_doubles());
+    launch_inspect_mesh(m.src.get<double>(), m.nv, m.tri.get<uint32_t>(), tri_local ? 0u : m.nt, scratch, out, s);
+    if (tri_local)
+        launch_forest_globalize(m.tri.get<uint32_t>(), m.ttiles.get<uint2>(), (uint32_t)(m.h_ttiles.size() / 2),
+                                m.toff.get<uint32_t>(), m.voff.get<uint32_t>(), out, s);
+    MeshInspect h;
+    HIP_TRY(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (h.bad_index) {
+        m.set = false;
+        throw GiError{GI_ERR_INVALID_ARGUMENT, "triangle index out of range"};
+    }
+    for (int k = 0; k < 3; ++k) {
+        m.src_lo[k] = h.lo[k];
+        m.src_hi[k] = h.hi[k];
+    }
+    m.src_grid = h.off_grid ? 0 : 1;
     m.on_device = true;
     m.built = false;
     m.dirty = true;
@@ -791,6 +824,8 @@ int finish_with_residual(gi_context* ctx, const gi_options& o, gi_result* out, u
     HIP_TRY(hipMemcpyAsync(vA.data(), MA.v64.get<double4>(), 32 * (size_t)MA.nv, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(vB.data(), MB.v64.get<double4>(), 32 * (size_t)MB.nv, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    ensure_host_copy(ctx, 0);
+    ensure_host_copy(ctx, 1);
     std::vector<HostHit> hits;
     uint64_t yes = 0, hpred = 0;
     const int nth = ctx->host_threads > 0 ? ctx->host_threads : (int)std::max(1u, std::thread::hardware_concurrency());
@@ -1213,25 +1248,24 @@ int gi_set_mesh(gi_context* ctx, int slot, const double* xyz, uint32_t nv, const
     int st = set_mesh_common(ctx, slot, nv, nt);
     if (st) return st;
     if ((nv && !xyz) || (nt && !tri)) return GI_ERR_INVALID_ARGUMENT;
-    for (size_t i = 0; i < 3 * (size_t)nt; ++i)
-        if (tri[i] >= nv) {
-            ctx->err = "triangle index out of range";
-            return GI_ERR_INVALID_ARGUMENT;
-        }
     try {
         HIP_TRY(hipSetDevice(ctx->device));
         MeshSlot& m = ctx->mesh[slot];
         if (m.borrowed) m.release();
         ++m.generation;
         m.forest = 0;
-        m.h_xyz.resize(3 * (size_t)nv);
-        m.src_grid = vertex_bounds(xyz, nv, m.src_lo, m.src_hi, m.h_xyz.data()) ? 1 : 0;
-        m.h_tri.assign(tri, tri + 3 * (size_t)nt);
+        drop_host_copy(m);
         m.nv = nv;
         m.nt = nt;
         m.set = true;
         m.xf = identity_xform();
-        upload_mesh(ctx, slot);
+        // straight from the caller's arrays (DMA when they are pinned); the
+        // copies complete before the call returns
+        double* src = m.src.ensure<double>(3 * (size_t)nv);
+        uint32_t* dtri = m.tri.ensure<uint32_t>(3 * (size_t)nt);
+        if (nv) HIP_TRY(hipMemcpyAsync(src, xyz, 24 * (size_t)nv, hipMemcpyHostToDevice, ctx->stream));
+        if (nt) HIP_TRY(hipMemcpyAsync(dtri, tri, 12 * (size_t)nt, hipMemcpyHostToDevice, ctx->stream));
+        inspect_upload(ctx, slot, false);
         return GI_OK;
     } catch (const GiError& e) {
         return fail(ctx, e);
@@ -1253,26 +1287,12 @@ int gi_set_mesh_device(gi_context* ctx, int slot, const double* d_xyz, uint32_t 
         m.nt = nt;
         m.set = true;
         m.xf = identity_xform();
+        drop_host_copy(m);
         double* src = m.src.ensure<double>(3 * (size_t)nv);
         uint32_t* tri = m.tri.ensure<uint32_t>(3 * (size_t)nt);
         if (nv) HIP_TRY(hipMemcpyAsync(src, d_xyz, 24 * (size_t)nv, hipMemcpyDeviceToDevice, ctx->stream));
         if (nt) HIP_TRY(hipMemcpyAsync(tri, d_tri, 12 * (size_t)nt, hipMemcpyDeviceToDevice, ctx->stream));
-        // host copies serve gi_reconstruct_device and the host residual stage
-        m.h_xyz.resize(3 * (size_t)nv);
-        m.h_tri.resize(3 * (size_t)nt);
-        if (nv) HIP_TRY(hipMemcpyAsync(m.h_xyz.data(), d_xyz, 24 * (size_t)nv, hipMemcpyDeviceToHost, ctx->stream));
-        if (nt) HIP_TRY(hipMemcpyAsync(m.h_tri.data(), d_tri, 12 * (size_t)nt, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        m.src_grid = vertex_bounds(m.h_xyz.data(), nv, m.src_lo, m.src_hi) ? 1 : 0;
-        for (size_t i = 0; i < 3 * (size_t)nt; ++i)
-            if (m.h_tri[i] >= nv) {
-                m.set = false;
-                ctx->err = "triangle index out of range";
-                return GI_ERR_INVALID_ARGUMENT;
-            }
-        m.on_device = true;
-        m.built = false;
-        m.dirty = true;
+        inspect_upload(ctx, slot, false);
         return GI_OK;
     } catch (const GiError& e) {
         return fail(ctx, e);
@@ -1404,8 +1424,7 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         MeshSlot& d = dst->mesh[dst_slot];
         const MeshSlot& s = src->mesh[src_slot];
         d.release();  // bumps the generation
-        d.h_xyz = s.h_xyz;
-        d.h_tri = s.h_tri;
+        drop_host_copy(d);  // made from the shared device arrays when needed
         d.nv = s.nv;
         d.nt = s.nt;
         d.set = true;
@@ -1440,19 +1459,23 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
 
 namespace {
 
-// One side of a batch -> the slot's concatenated host arrays and tile tables.
+// One side of a batch -> the slot's sizes and tile tables (the meshes stored
+// back to back: mesh i's vertices from h_voff[i], triangles from h_toff[i]).
 void fill_forest(MeshSlot& m, const gi_mesh_pair* pairs, uint32_t n, bool side_b) {
     uint64_t nv = 0, nt = 0;
     for (uint32_t i = 0; i < n; ++i) {
         nv += side_b ? pairs[i].nvB : pairs[i].nvA;
         nt += side_b ? pairs[i].ntB : pairs[i].ntA;
     }
-    m.release();
+    // keep the device buffers (a later batch of the same size reuses them;
+    // freeing and mapping tens of GB again costs more than the upload)
+    if (m.borrowed) m.release();
+    ++m.generation;
+    m.full_tree = false;
+    drop_host_copy(m);
     m.forest = n;
     m.nv = (uint32_t)nv;
     m.nt = (uint32_t)nt;
-    m.h_xyz.resize(3 * nv);
-    m.h_tri.resize(3 * nt);
     m.h_voff.assign((size_t)n + 1, 0u);
     m.h_toff.assign((size_t)n + 1, 0u);
     m.h_vtiles.clear();
@@ -1460,11 +1483,7 @@ void fill_forest(MeshSlot& m, const gi_mesh_pair* pairs, uint32_t n, bool side_b
     m.h_vtile0.assign((size_t)n + 1, 0u);
     uint32_t vo = 0, to = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        const double* xyz = side_b ? pairs[i].xyzB : pairs[i].xyzA;
-        const uint32_t* tri = side_b ? pairs[i].triB : pairs[i].triA;
         uint32_t mv = side_b ? pairs[i].nvB : pairs[i].nvA, mt = side_b ? pairs[i].ntB : pairs[i].ntA;
-        if (mv) memcpy(&m.h_xyz[3 * (size_t)vo], xyz, 24 * (size_t)mv);
-        for (size_t k = 0; k < 3 * (size_t)mt; ++k) m.h_tri[3 * (size_t)to + k] = tri[k] + vo;
         m.h_vtile0[i] = (uint32_t)(m.h_vtiles.size() / 2);
         for (uint32_t f = 0; f < mv; f += kForestTile) {
             m.h_vtiles.push_back(i);
@@ -1480,12 +1499,40 @@ void fill_forest(MeshSlot& m, const gi_mesh_pair* pairs, uint32_t n, bool side_b
         m.h_toff[i + 1] = to;
     }
     m.h_vtile0[n] = (uint32_t)(m.h_vtiles.size() / 2);
-    m.src_grid = vertex_bounds(m.h_xyz.data(), nv, m.src_lo, m.src_hi) ? 1 : 0;
     m.set = true;
     m.on_device = false;
     m.built = false;
     m.dirty = true;
     m.xf = identity_xform();
+}
+
+// The batch's arrays straight from the caller's, each mesh to its place in
+// the slot's device arrays (indices still per mesh), the tile tables, then
+// the device check that makes the indices global.
+void upload_forest(gi_context* ctx, int slot, const gi_mesh_pair* pairs, uint32_t n, bool side_b) {
+    MeshSlot& m = ctx->mesh[slot];
+    hipStream_t st = ctx->stream;
+    double* src = m.src.ensure<double>(3 * (size_t)m.nv);
+    uint32_t* tri = m.tri.ensure<uint32_t>(3 * (size_t)m.nt);
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* xyz = side_b ? pairs[i].xyzB : pairs[i].xyzA;
+        const uint32_t* t = side_b ? pairs[i].triB : pairs[i].triA;
+        const uint32_t mv = side_b ? pairs[i].nvB : pairs[i].nvA, mt = side_b ? pairs[i].ntB : pairs[i].ntA;
+        if (mv)
+            HIP_TRY(hipMemcpyAsync(src + 3 * (size_t)m.h_voff[i], xyz, 24 * (size_t)mv, hipMemcpyHostToDevice, st));
+        if (mt)
+            HIP_TRY(hipMemcpyAsync(tri + 3 * (size_t)m.h_toff[i], t, 12 * (size_t)mt, hipMemcpyHostToDevice, st));
+    }
+    auto put = [&](DevBuf& d, const std::vector<uint32_t>& h) {
+        uint32_t* p = d.ensure<uint32_t>(h.size());
+        if (!h.empty()) HIP_TRY(hipMemcpyAsync(p, h.data(), 4 * h.size(), hipMemcpyHostToDevice, st));
+    };
+    put(m.voff, m.h_voff);
+    put(m.toff, m.h_toff);
+    put(m.vtiles, m.h_vtiles);
+    put(m.ttiles, m.h_ttiles);
+    put(m.vtile0, m.h_vtile0);
+    inspect_upload(ctx, slot, true);  // synchronises: the caller's arrays and the tables may go
 }
 
 }  // namespace
@@ -1503,11 +1550,6 @@ int gi_set_batch(gi_context* ctx, const gi_mesh_pair* pairs, uint32_t n_pairs) {
                 ctx->err = "mesh pair " + std::to_string(i) + ": null array";
                 return GI_ERR_INVALID_ARGUMENT;
             }
-            for (size_t k = 0; k < 3 * (size_t)mt[sd]; ++k)
-                if (tri[sd][k] >= mv[sd]) {
-                    ctx->err = "mesh pair " + std::to_string(i) + ": triangle index out of range";
-                    return GI_ERR_INVALID_ARGUMENT;
-                }
             nv[sd] += mv[sd];
             nt[sd] += mt[sd];
         }
@@ -1524,7 +1566,7 @@ int gi_set_batch(gi_context* ctx, const gi_mesh_pair* pairs, uint32_t n_pairs) {
         for (int sd = 0; sd < 2; ++sd) {
             if (ctx->mesh[sd].borrowed) ctx->mesh[sd].release();
             fill_forest(ctx->mesh[sd], pairs, n_pairs, sd == 1);
-            upload_mesh(ctx, sd);
+            upload_forest(ctx, sd, pairs, n_pairs, sd == 1);
         }
         sync_all(ctx);
         return GI_OK;
@@ -1629,7 +1671,13 @@ int gi_autotune(gi_context* ctx, const gi_options* opt, int32_t reps, gi_tuning*
 
 int gi_clear_device(gi_context* ctx) {
     if (!ctx) return GI_ERR_INVALID_ARGUMENT;
-    (void)hipSetDevice(ctx->device);
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        for (int sl = 0; sl < 2; ++sl)  // the host copies reconstruct the device state
+            if (!ctx->mesh[sl].borrowed) ensure_host_copy(ctx, sl);
+    } catch (const GiError& e) {
+        return fail(ctx, e);
+    }
     (void)hipStreamSynchronize(ctx->side);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->release_device();

@@ -69,6 +69,25 @@ struct MeshDev {
 };
 
 void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
+
+// What a mesh upload learns about its arrays on the device: the bounds of
+// the vertices (NaN coordinates skipped; zeros for none), whether one of them
+// is off the integer grid (not an integer within 2^26), and whether a
+// triangle index is out of range.
+struct MeshInspect {
+    double lo[3], hi[3];
+    uint32_t off_grid, bad_index;
+};
+// scratch: inspect_scratch_doubles() doubles; out: one MeshInspect
+constexpr uint32_t kInspectBlocks = 512;
+size_t inspect_scratch_doubles();
+void launch_inspect_mesh(const double* xyz, uint32_t nv, const uint32_t* tri, uint32_t nt, double* scratch,
+                         MeshInspect* out, hipStream_t s);
+// A batch uploaded with per-mesh (local) indices: checks each against its
+// mesh's vertex count (out->bad_index) and makes it global (adds the mesh's
+// first vertex). Tiles as for launch_tri_boxes_forest.
+void launch_forest_globalize(uint32_t* tri, const uint2* ttiles, uint32_t n_ttiles, const uint32_t* toff,
+                             const uint32_t* voff, MeshInspect* out, hipStream_t s);
 // grid flag = 1, bounds6 = empty box (order-preserving keys)
 // zero counters[0, nzero) and set grid_flags[slot] = 1 for the slots in stage_mask
 void launch_query_init(uint32_t* counters, uint32_t nzero, uint32_t* grid_flags, uint32_t stage_mask, hipStream_t s);

@@ -97,6 +97,120 @@ void launch_query_init(uint32_t* counters, uint32_t nzero, uint32_t* grid_flags,
     GI_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------------ inspection
+//
+// One launch per uploaded mesh: blocks [0, kInspectBlocks) take the vertices
+// (grid-stride; per-block bounds to `part`, the off-grid flag by one atomic
+// per wave that saw such a vertex), the rest take the triangle indices (an
+// out-of-range index raises bad_index the same way); then one block joins the
+// partial bounds.
+
+size_t inspect_scratch_doubles() { return 6 * (size_t)kInspectBlocks; }
+
+__device__ __forceinline__ double wave_min_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ double wave_max_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_inspect_mesh(const double* __restrict__ xyz, uint32_t nv,
+                                                      const uint32_t* __restrict__ tri, size_t nidx,
+                                                      double* __restrict__ part, MeshInspect* out) {
+    if (blockIdx.x < kInspectBlocks) {
+        double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+        bool off = false;
+        for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < nv; i += kInspectBlocks * 256)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double x = xyz[3 * (size_t)i + k];
+                l[k] = fmin(l[k], x);
+                h[k] = fmax(h[k], x);
+                off = off || !(x == rint(x) && fabs(x) <= 67108864.0);
+            }
+        if (__ballot(off) && lane_id() == 0) atomicOr(&out->off_grid, 1u);
+        __shared__ double red[6][4];
+        const uint32_t wave = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double mn = wave_min_d(l[k]), mx = wave_max_d(h[k]);
+            if (lane_id() == 0) { red[k][wave] = mn; red[3 + k][wave] = mx; }
+        }
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            const int k = threadIdx.x;
+            double r = red[k][0];
+            for (int w = 1; w < 4; ++w) r = k < 3 ? fmin(r, red[k][w]) : fmax(r, red[k][w]);
+            part[6 * (size_t)blockIdx.x + k] = r;
+        }
+        return;
+    }
+    bool bad = false;
+    const size_t nb = gridDim.x - kInspectBlocks;
+    for (size_t i = (blockIdx.x - kInspectBlocks) * 256 + threadIdx.x; i < nidx; i += nb * 256) bad = bad || tri[i] >= nv;
+    if (__ballot(bad) && lane_id() == 0) atomicOr(&out->bad_index, 1u);
+}
+
+__global__ __launch_bounds__(256) void k_inspect_join(const double* __restrict__ part, MeshInspect* out) {
+    __shared__ double red[6][4];
+    const uint32_t wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double mn = INFINITY, mx = -INFINITY;
+        for (uint32_t b = threadIdx.x; b < kInspectBlocks; b += 256) {
+            mn = fmin(mn, part[6 * (size_t)b + k]);
+            mx = fmax(mx, part[6 * (size_t)b + 3 + k]);
+        }
+        mn = wave_min_d(mn);
+        mx = wave_max_d(mx);
+        if (lane_id() == 0) { red[k][wave] = mn; red[3 + k][wave] = mx; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        double lo = fmin(fmin(red[k][0], red[k][1]), fmin(red[k][2], red[k][3]));
+        double hi = fmax(fmax(red[3 + k][0], red[3 + k][1]), fmax(red[3 + k][2], red[3 + k][3]));
+        const bool any = lo <= hi;  // no finite-or-infinite coordinate: zeros
+        out->lo[k] = any ? lo : 0.0;
+        out->hi[k] = any ? hi : 0.0;
+    }
+}
+
+void launch_inspect_mesh(const double* xyz, uint32_t nv, const uint32_t* tri, uint32_t nt, double* scratch,
+                         MeshInspect* out, hipStream_t s) {
+    launch_fill_u32(&out->off_grid, 0u, 2, s);
+    const size_t nidx = 3 * (size_t)nt;
+    const uint32_t iblocks = nidx ? std::min<uint32_t>(blocks_for(nidx, 256), 2048u) : 0u;
+    k_inspect_mesh<<<kInspectBlocks + iblocks, 256, 0, s>>>(xyz, nv, tri, nidx, scratch, out);
+    k_inspect_join<<<1, 256, 0, s>>>(scratch, out);
+    GI_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(256) void k_forest_globalize(uint32_t* __restrict__ tri, const uint2* __restrict__ tiles,
+                                                          const uint32_t* __restrict__ toff,
+                                                          const uint32_t* __restrict__ voff, MeshInspect* out) {
+    const uint2 t = tiles[blockIdx.x];
+    const uint32_t end = min(t.y + kForestTile, toff[t.x + 1]);
+    const uint32_t v0 = voff[t.x], nvm = voff[t.x + 1] - v0;
+    bool bad = false;
+    for (uint32_t i = t.y + threadIdx.x; i < end; i += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t idx = tri[3 * (size_t)i + k];
+            bad = bad || idx >= nvm;
+            tri[3 * (size_t)i + k] = idx + v0;
+        }
+    if (__ballot(bad) && lane_id() == 0) atomicOr(&out->bad_index, 1u);
+}
+
+void launch_forest_globalize(uint32_t* tri, const uint2* ttiles, uint32_t n_ttiles, const uint32_t* toff,
+                             const uint32_t* voff, MeshInspect* out, hipStream_t s) {
+    if (n_ttiles) k_forest_globalize<<<n_ttiles, 256, 0, s>>>(tri, ttiles, toff, voff, out);
+    GI_LAUNCH_CHECK();
+}
+
 // ------------------------------------------------------- vertex preparation
 
 __device__ __forceinline__ double xform_coord(const Xform& xf, int r, double x, double y, double z) {

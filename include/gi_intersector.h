@@ -133,7 +133,10 @@ int gi_destroy(gi_context* ctx);
 const char* gi_last_error(const gi_context* ctx);
 const char* gi_status_string(int status);
 
-/* Host arrays: xyz = 3*nv doubles, tri = 3*nt vertex indices. Copied. */
+/* Host arrays: xyz = 3*nv doubles, tri = 3*nt vertex indices. Copied to the
+ * device before the call returns (by DMA when the arrays are pinned); the
+ * index range is checked on the device, and an index >= nv leaves the slot
+ * unset (GI_ERR_INVALID_ARGUMENT). */
 int gi_set_mesh(gi_context* ctx, int slot, const double* xyz, uint32_t nv, const uint32_t* tri, uint32_t nt);
 /* Same with device arrays (already resident in HBM). Copied device-to-device. */
 int gi_set_mesh_device(gi_context* ctx, int slot, const double* d_xyz, uint32_t nv, const uint32_t* d_tri,
