@@ -61,10 +61,16 @@ def algorithmic_bytes(nvA, ntA, nvB, ntB, st):
     return per_mesh + 112 * st["candidates"] + 8 * st["pairs"] + 128 * st["segments"]
 
 
-def kernel_bytes(name, st, ntA, ntB):
+def kernel_bytes(name, st, ntA, ntB, nvA, nvB):
     """Compulsory bytes of one launch of a single kernel of the step (one
     launch of each per step; A's tree is the only tree built: B's triangles in
     Morton order are the traversal's queries)."""
+    if name == "k_stage_mesh":
+        # both meshes in one launch. per vertex: source xyz 24 B read, double
+        # copy 32 B + integer copy 16 B written; per triangle: indices 12 B
+        # read (its vertices were read once above), box 32 B + Morton key and
+        # id 8 B written, first-pass digit counts 1 KiB per 1024 triangles
+        return 72.0 * (nvA + nvB) + 53.0 * (ntA + ntB)
     if name == "k_build_chunks":
         # per leaf of A: sorted key 4 B + triangle id 4 B + its FP32 box 32 B
         # read; leaf and internal node 32 B each + leaf range 8 B written
@@ -490,50 +496,59 @@ def main():
 
     tuning = eng.get_tuning() if args.no_autotune else eng.autotune(reps=3, rebuild=True)
 
-    def step():
-        res = eng.intersect_raw(output_on_device=True, want_segments=True, rebuild=True, use_graph=args.graph)
+    def step(timing):
+        res = eng.intersect_raw(output_on_device=True, want_segments=True, rebuild=True, use_graph=args.graph,
+                                stage_timing=timing)
         st = res.stats.as_dict()
         gi.lib().gi_result_free(gi.C.byref(res))
         return st
 
     # a step returns once its results are complete in device memory (the
-    # library synchronises its streams), so the clock brackets finished work
-    for _ in range(args.warmup):
-        step()
+    # library synchronises its streams), so the clock brackets finished work.
+    # The warm-up steps time every stage; the largest single launches of a
+    # step are the staging of both meshes (k_stage_mesh), A's BVH build
+    # (k_build_chunks) and the traversal (k_traverse), and the longest of them
+    # in the warm-up is the dominant kernel. The timed steps record HIP events
+    # around that one launch only (each event costs the queue a few
+    # microseconds); a few more steps with every stage timed give the stage
+    # breakdown.
+    flags = {"k_stage_mesh": gi.TIME_STAGING, "k_build_chunks": gi.TIME_BVH_CHUNKS, "k_traverse": gi.TIME_TRAVERSE}
+    stat_of = {"k_stage_mesh": "ms_stage_meshes", "k_build_chunks": "ms_bvh_chunks", "k_traverse": "ms_traverse"}
+    warm = [step(0) for _ in range(max(args.warmup, 1))]
+    warm_ms = {k: float(np.mean([w[stat_of[k]] for w in warm])) for k in flags}
+    dom = max(warm_ms, key=warm_ms.get)
     stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        stats.append(step())
+        stats.append(step(flags[dom]))
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1000.0 / args.steps
     st = stats[-1]
     for k in ("candidates", "pairs", "segments"):
         assert all(s[k] == st[k] for s in stats), f"{k} varies between steps"
-    avg = {k: float(np.mean([s[k] for s in stats])) for k in
+    staged = [step(0) for _ in range(5)]
+    avg = {k: float(np.mean([s[k] for s in staged])) for k in
            ("ms_total", "ms_build", "ms_traverse", "ms_filter", "ms_exact", "ms_output", "ms_bvh_chunks")}
+    dom_ms = float(np.mean([s[stat_of[dom]] for s in stats]))
     value = st["candidates"] / (ms / 1000.0)
 
-    # the two largest single kernels of a step, each timed by HIP events the
-    # library records around its one launch per step on its stream: the BVH
-    # build (k_build_chunks, A's tree) and the traversal (k_traverse); the
-    # dominant kernel is the longer of the two (the two k_stage_mesh launches
-    # run side by side on two streams, each shorter than either)
-    launches = sum(s["bvh_chunk_launches"] for s in stats)
-    assert launches == len(stats), "one build launch per step"
-    assert all(s["bvh_chunk_leaves"] == ta.shape[0] for s in stats), "the build covers A's tree only"
-    kernel_ms = {"k_build_chunks": sum(s["ms_bvh_chunks"] for s in stats) / launches,
-                 "k_traverse": avg["ms_traverse"]}
+    # the dominant kernel, timed by HIP events around its one launch per step
+    # in the timed steps; the other two from the warm-up steps
+    assert all(s["bvh_chunk_leaves"] in (0, ta.shape[0]) for s in stats + warm), "the build covers A's tree only"
+    kernel_ms = dict(warm_ms)
+    kernel_ms[dom] = dom_ms
     roofs = {}
     for k, kms in kernel_ms.items():
-        model = kernel_bytes(k, st, ta.shape[0], tb.shape[0])
+        model = kernel_bytes(k, st, ta.shape[0], tb.shape[0], xa.shape[0], xb.shape[0])
         traffic = measured_traffic(k)
         kb = min(model, traffic) if traffic else model
         achieved = kb / (kms / 1000.0) / 1e9
         roofs[k] = {"bound": "hbm", "kernel": k, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_ms": kms,
-                    "launches_timed": launches, "algorithmic_bytes": kb, "model_bytes": model,
+                    "launches_timed": len(stats) if k == dom else len(warm),
+                    "timed_in": "the timed steps" if k == dom else "the warm-up steps",
+                    "algorithmic_bytes": kb, "model_bytes": model,
                     "achieved_model": model / (kms / 1000.0) / 1e9}
-    dom = max(kernel_ms, key=kernel_ms.get)
     b_alg = algorithmic_bytes(xa.shape[0], ta.shape[0], xb.shape[0], tb.shape[0], st)
 
     line = {
@@ -558,13 +573,15 @@ def main():
         "graph_steps": sum(s["graph"] for s in stats),
         "roofline": roofs[dom],
         "roofline_other": [r for k, r in roofs.items() if k != dom],
+        "timing_events": f"the timed steps record HIP events around {dom} only",
         "pipeline": {"candidates": st["candidates"], "traversal_hits": st["traversal_hits"], "pairs": st["pairs"],
                      "segments": st["segments"], "filter_no": st["filter_no"], "filter_green": st["filter_green"],
                      "filter_yellow": st["filter_yellow"], "exact_yellow_yes": st["exact_yellow_yes"],
                      "coplanar": st["coplanar"], "degenerate": st["degenerate"],
                      "intersecting_pairs_per_s": st["pairs"] / (ms / 1000.0),
                      "pair_space_per_s": ta.shape[0] * tb.shape[0] / (ms / 1000.0),
-                     "stage_ms": avg, "end_to_end_algorithmic_GBs": b_alg / (ms / 1000.0) / 1e9,
+                     "stage_ms": avg, "stage_ms_from": "5 more steps with every stage timed (stage_timing=0)",
+                     "end_to_end_algorithmic_GBs": b_alg / (ms / 1000.0) / 1e9,
                      "end_to_end_hbm_frac": b_alg / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,
                      "predicates": {
                          "filter": st["filter_predicates"], "exact": st["exact_predicates"],

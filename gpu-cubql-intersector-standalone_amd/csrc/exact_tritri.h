@@ -323,6 +323,32 @@ struct ExactTT {
         return 1;
     }
 
+    // The piece of edge (Ti, Tj) of a triangle met by the (non-coplanar)
+    // triangle O with normal nO: 0, 1 or 2 points on the line L, in p0 (and
+    // p1). oi/oj and si/sj: orientations of Ti and Tj against O's plane and
+    // their signs. np counts the signs evaluated.
+    static GI_HD int edgePiece(const V3& Ti, const V3& Tj, const V3* O, const V3& nO, const B& oi, const B& oj, int si,
+                               int sj, HP& p0, HP& p1, int& np) {
+        if (si * sj > 0) return 0;
+        if (si == 0 && sj == 0) {
+            int kk = dropAxis(nO);
+            int ia = kk == 0 ? 1 : 0, ib = kk == 2 ? 1 : 2;
+            Rat t0, t1;
+            int kind = clip2d(Ti, Tj, O, ia, ib, t0, t1, np);
+            if (kind == 0) return 0;
+            p0 = hpOnEdge(Ti, Tj, t0);
+            if (kind == 2) p1 = hpOnEdge(Ti, Tj, t1);
+            return kind;
+        }
+        int e0 = P::sgnB(orient3d(Ti, Tj, O[0], O[1]));
+        int e1 = P::sgnB(orient3d(Ti, Tj, O[1], O[2]));
+        int e2 = P::sgnB(orient3d(Ti, Tj, O[2], O[0]));
+        np += 3;
+        if (!((e0 >= 0 && e1 >= 0 && e2 >= 0) || (e0 <= 0 && e1 <= 0 && e2 <= 0))) return 0;
+        p0 = si == 0 ? hpInt(Ti) : (sj == 0 ? hpInt(Tj) : hpCrossing(Ti, Tj, oi, oj));
+        return 1;
+    }
+
     // The pieces of the edges of T met by the (non-coplanar) triangle O, added
     // to the span along axis k; o/s: orientations of T's vertices against O's
     // plane. Unrolled, every index is a compile-time constant.
@@ -331,24 +357,10 @@ struct ExactTT {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const int j = i == 2 ? 0 : i + 1;
-            if (s[i] * s[j] > 0) continue;
-            if (s[i] == 0 && s[j] == 0) {
-                int kk = dropAxis(nO);
-                int ia = kk == 0 ? 1 : 0, ib = kk == 2 ? 1 : 2;
-                Rat t0, t1;
-                int kind = clip2d(T[i], T[j], O, ia, ib, t0, t1, span.npred);
-                if (kind == 0) continue;
-                spanAdd(span, hpOnEdge(T[i], T[j], t0), k);
-                if (kind == 2) spanAdd(span, hpOnEdge(T[i], T[j], t1), k);
-                continue;
-            }
-            int e0 = P::sgnB(orient3d(T[i], T[j], O[0], O[1]));
-            int e1 = P::sgnB(orient3d(T[i], T[j], O[1], O[2]));
-            int e2 = P::sgnB(orient3d(T[i], T[j], O[2], O[0]));
-            span.npred += 3;
-            if (!((e0 >= 0 && e1 >= 0 && e2 >= 0) || (e0 <= 0 && e1 <= 0 && e2 <= 0))) continue;
-            HP X = s[i] == 0 ? hpInt(T[i]) : (s[j] == 0 ? hpInt(T[j]) : hpCrossing(T[i], T[j], o[i], o[j]));
-            spanAdd(span, X, k);
+            HP p0, p1;
+            const int m = edgePiece(T[i], T[j], O, nO, o[i], o[j], s[i], s[j], p0, p1, span.npred);
+            if (m >= 1) spanAdd(span, p0, k);
+            if (m == 2) spanAdd(span, p1, k);
         }
     }
 

@@ -198,6 +198,9 @@ struct gi_context {
     }
     hipEvent_t ev_chunk[4] = {};  // [0..1] bracket the last k_build_chunks launch
     bool chunk_timed = false;     // that launch ran in the current call
+    hipEvent_t ev_stage[2] = {};  // bracket the last staging launch of single meshes
+    bool stage_timed = false;     // that launch ran, timed, in the current call
+    bool graph_stage_timed = false;
     uint32_t chunk_leaves = 0;    // leaves it covered (both trees when both were built)
     void release_device() {
         drop_graph();
@@ -271,7 +274,7 @@ void upload_mesh(gi_context* ctx, int slot) {
     if (!m.host_valid) throw GiError{GI_ERR_INTERNAL, "mesh has neither device nor host copies"};
     double* src = m.src.ensure<double>(3 * (size_t)m.nv);
     uint32_t* tri = m.tri.ensure<uint32_t>(3 * (size_t)m.nt);
-    hipStream_t st = ctx->build_stream[slot];
+    hipStream_t st = ctx->stream;
     if (m.nv) HIP_TRY(hipMemcpyAsync(src, m.h_xyz.data(), sizeof(double) * 3 * m.nv, hipMemcpyHostToDevice, st));
     if (m.nt) HIP_TRY(hipMemcpyAsync(tri, m.h_tri.data(), sizeof(uint32_t) * 3 * m.nt, hipMemcpyHostToDevice, st));
     if (m.forest) {
@@ -382,10 +385,12 @@ void stage_forest(gi_context* ctx, int slot) {
         SortJobU32{keys, vals, tk, tv, m.nt, 3 * m.morton_axis + bits_for(M), false, hist, stmp, samp};
 }
 
-// The part of a mesh's build that runs on the slot's own stream: vertices
-// (transform) -> boxes -> Morton sort (new topology only). Returns true when
-// the slot's tree must then be built or refit (build_meshes).
-bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
+// The part of a mesh's build before the sorts: vertices (transform) ->
+// boxes -> Morton keys (new topology only). A batch side launches its stages
+// on the slot's own stream; a single mesh fills *job for the one staging
+// launch both slots share. Returns true when the slot's tree must then be
+// built or refit (build_meshes).
+bool stage_mesh(gi_context* ctx, int slot, bool rebuild, StageJob* job) {
     MeshSlot& m = ctx->mesh[slot];
     if (m.borrowed) return false;  // built and kept current by its owner
     if (!m.on_device) upload_mesh(ctx, slot);
@@ -394,7 +399,7 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
         stage_forest(ctx, slot);
         return true;
     }
-    hipStream_t s = ctx->build_stream[slot];
+    hipStream_t s = ctx->stream;
     uint32_t* grid = ctx->counters.get<uint32_t>() + CNT_COUNT;  // set to 1 by the query's init
     size_t nv = m.nv, nt = m.nt;
     double4* v64 = m.v64.ensure<double4>(nv);
@@ -425,10 +430,10 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild) {
     }
     const bool hist0 = topology && radix_sort_takes_hist0(m.nt);
     uint32_t* samp = hist0 ? m.samples.ensure<uint32_t>(key_sample_count(m.nt)) : nullptr;
-    launch_stage_mesh(m.src.get<double>(), m.nv, m.tri.get<uint32_t>(), m.nt, d_xf, v64, vi, grid + slot,
-                      m.morton_axis, tbox, topology ? keys : nullptr, topology ? vals : nullptr,
-                      hist0 ? hist : nullptr, s);
-    // the Morton sort runs after the join, together with the other slot's
+    *job = StageJob{m.src.get<double>(), m.nv, m.tri.get<uint32_t>(), m.nt, d_xf, v64, vi, grid + slot,
+                    m.morton_axis, tbox, topology ? keys : nullptr, topology ? vals : nullptr,
+                    hist0 ? hist : nullptr};
+    // the Morton sort runs after the staging, together with the other slot's
     if (topology)
         ctx->pending_sort[slot] = SortJobU32{keys, vals, tk, tv, m.nt, 3 * m.morton_axis, hist0, hist, stmp, samp};
     return true;
@@ -454,11 +459,12 @@ BuildJob<K> slot_job(MeshSlot& m) {
 // kernel serves every tree that needs a build or refit (a refit reruns the
 // build on the kept sorted keys: same topology, new boxes). The
 // k_build_chunks launch is bracketed by ev_chunk[0..1].
-void build_meshes(gi_context* ctx, bool rebuild, bool timed = true, bool zero_counters = false) {
+void build_meshes(gi_context* ctx, bool rebuild, int time_mask = 0, bool zero_counters = false) {
     uint32_t* counters = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);  // shared by both stages: allocate before the fork
     ctx->xf_dev.ensure<Xform>(2);
     ctx->chunk_timed = false;
     ctx->chunk_leaves = 0;
+    ctx->stage_timed = false;
     // grid flags of the slots staged below to 1 (their vertices clear them),
     // before the per-slot stages fork
     uint32_t stage_mask = 0;
@@ -468,13 +474,27 @@ void build_meshes(gi_context* ctx, bool rebuild, bool timed = true, bool zero_co
     }
     if (stage_mask || zero_counters)
         launch_query_init(counters, zero_counters ? CNT_COUNT : 0u, counters + CNT_COUNT, stage_mask, ctx->stream);
-    fork_side(ctx);
+    // batch sides stage on their own streams (fork/join); single meshes share
+    // one staging launch on the context's stream
+    const bool forest = ctx->mesh[0].forest || ctx->mesh[1].forest;
+    if (forest) fork_side(ctx);
     bool need[2] = {false, false};
+    StageJob sj[2];
+    int nsj = 0;
     for (int sl = 0; sl < 2; ++sl) {
         ctx->pending_sort[sl] = SortJobU32{};
-        if (ctx->mesh[sl].set) need[sl] = stage_mesh(ctx, sl, rebuild);
+        sj[nsj].nv = sj[nsj].nt = 0;
+        if (ctx->mesh[sl].set) need[sl] = stage_mesh(ctx, sl, rebuild, &sj[nsj]);
+        if (need[sl] && !ctx->mesh[sl].forest) ++nsj;
     }
-    join_side(ctx);
+    if (forest) join_side(ctx);
+    if (nsj) {
+        const bool time_it = (time_mask & GI_TIME_STAGING) != 0;
+        if (time_it) record(ctx, ctx->ev_stage[0], ctx->stream);
+        launch_stage_meshes(sj, nsj, ctx->stream);
+        if (time_it) record(ctx, ctx->ev_stage[1], ctx->stream);
+        ctx->stage_timed = time_it;
+    }
     {  // both meshes' Morton sorts, pass by pass in shared launches
         SortJobU32 jobs[2];
         int slot_of[2], nj = 0, in_tmp[2] = {0, 0};
@@ -495,7 +515,7 @@ void build_meshes(gi_context* ctx, bool rebuild, bool timed = true, bool zero_co
     uint32_t leaves = 0;
     for (int j = 0; j < nj; ++j) leaves += jobs[j].n;
     if (nj) {
-        const bool time_it = timed && leaves > 0;
+        const bool time_it = (time_mask & GI_TIME_BVH_CHUNKS) && leaves > 0;
         launch_build_bvh(jobs, nj, (uint32_t)ctx->tune.build_chunk, s, time_it ? ctx->ev_chunk[0] : nullptr,
                          time_it ? ctx->ev_chunk[1] : nullptr,
                          ctx->capturing ? hipEventRecordExternal : hipEventRecordDefault);
@@ -600,15 +620,22 @@ int exact_kinds(const gi_context* ctx) {
     return kExactGrid | kExactWide;
 }
 
-void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool timed) {
+// The events a query records (gi_options.stage_timing): every stage, or
+// only the launches in the mask.
+constexpr int kTimeAll = GI_TIME_STAGING | GI_TIME_BVH_CHUNKS | GI_TIME_TRAVERSE | 16;  // 16: the other stages
+int timing_mask(const gi_options& o) { return o.stage_timing == 0 ? kTimeAll : (o.stage_timing & 7); }
+
+void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool timed_query) {
     hipStream_t s = ctx->stream;
+    const int mask = timed_query ? timing_mask(o) : 0;
+    const bool timed = mask == kTimeAll, traverse_timed = (mask & GI_TIME_TRAVERSE) != 0;
     MeshSlot& MA = ctx->mesh[0];
     MeshSlot& MB = ctx->mesh[1];
     const uint32_t M = MA.forest;
     uint32_t* counters = ctx->counters.ensure<uint32_t>(CNT_COUNT + 2);
     if (build) {  // the build's first launch also zeroes the counters
         if (timed) record(ctx, ctx->ev[0], s);
-        build_meshes(ctx, o.rebuild != 0, timed, true);
+        build_meshes(ctx, o.rebuild != 0, mask, true);
         if (timed) record(ctx, ctx->ev[1], s);
     } else {
         launch_query_init(counters, CNT_COUNT, counters + CNT_COUNT, 0u, s);
@@ -616,11 +643,11 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
     uint32_t* grid = counters + CNT_COUNT;
     MeshDev A = mesh_dev(MA), B = mesh_dev(MB);
     int bitsB = bits_for(MB.nt);
-    if (timed) record(ctx, ctx->ev[6], s);
+    if (traverse_timed) record(ctx, ctx->ev[6], s);
     const uint32_t* keysB = M ? (MB.sorted_in_tmp ? MB.tkeys.get<uint32_t>() : MB.keys.get<uint32_t>()) : nullptr;
     launch_traverse(A, B, keysB, 3 * MB.morton_axis, M ? MA.roots.get<uint2>() : nullptr, ctx->cand.get<uint2>(),
                     ctx->cap_cand, counters, ctx->tune.traverse_variant, s);
-    if (timed) record(ctx, ctx->ev[2], s);
+    if (traverse_timed) record(ctx, ctx->ev[2], s);
     launch_filter(A, B, ctx->cand.get<uint2>(), ctx->cap_cand, grid, ctx->exact.get<uint2>(), ctx->cap_exact, counters,
                   ctx->tune.filter_blocks, s);
     if (timed) record(ctx, ctx->ev[3], s);
@@ -663,6 +690,7 @@ std::vector<uint64_t> presync_signature(gi_context* ctx, const gi_options& o) {
     std::vector<uint64_t> v;
     v.push_back((uint64_t)o.rebuild);
     v.push_back((uint64_t)o.want_segments);
+    v.push_back((uint64_t)timing_mask(o));
     auto buf = [&](const DevBuf& b) {
         v.push_back((uint64_t)(uintptr_t)b.p);
         v.push_back((uint64_t)b.bytes);
@@ -726,6 +754,7 @@ bool launch_presync_graph(gi_context* ctx, const gi_options& o) {
             }
         ctx->chunk_timed = ctx->graph_chunk_timed;
         ctx->chunk_leaves = ctx->graph_chunk_leaves;
+        ctx->stage_timed = ctx->graph_stage_timed;
         return true;
     }
     if (sig != ctx->graph_sig_seen) {
@@ -770,6 +799,7 @@ bool launch_presync_graph(gi_context* ctx, const gi_options& o) {
     ctx->graph_sig = sig;
     ctx->graph_chunk_timed = ctx->chunk_timed;
     ctx->graph_chunk_leaves = ctx->chunk_leaves;
+    ctx->graph_stage_timed = ctx->stage_timed;
     HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
     return true;
 }
@@ -807,7 +837,8 @@ int finish_with_residual(gi_context* ctx, const gi_options& o, gi_result* out, u
     MeshSlot& MB = ctx->mesh[1];
     const uint32_t M = MA.forest;
     gi_stats& st = out->stats;
-    record(ctx, ctx->ev[5], s);
+    const bool all_timed = timing_mask(o) == kTimeAll;
+    if (all_timed) record(ctx, ctx->ev[5], s);
     const auto t0 = std::chrono::steady_clock::now();
     const size_t words = 8 * (size_t)L;
     std::vector<uint32_t> gp(2 * (size_t)R), go((size_t)R + 1), list(2 * (size_t)Q);
@@ -909,8 +940,10 @@ int finish_with_residual(gi_context* ctx, const gi_options& o, gi_result* out, u
     st.pairs = n;
     st.segments = nseg_all;
     st.ms_host = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    st.ms_output = elapsed(ctx->ev[4], ctx->ev[5]);
-    st.ms_total = elapsed(ctx->ev[0], ctx->ev[5]) + st.ms_host;
+    if (all_timed) {
+        st.ms_output = elapsed(ctx->ev[4], ctx->ev[5]);
+        st.ms_total = elapsed(ctx->ev[0], ctx->ev[5]) + st.ms_host;
+    }
     out->n_pairs = n;
     out->n_segments = nseg_all;
     out->exact_limbs = LL;
@@ -949,7 +982,7 @@ int finish_with_residual(gi_context* ctx, const gi_options& o, gi_result* out, u
 
 int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     memset(out, 0, sizeof(*out));
-    gi_options o = {0, 1, 0, 0};
+    gi_options o = {0, 1, 0, 0, 0, 0};
     if (opt) o = *opt;
     MeshSlot& MA = ctx->mesh[0];
     MeshSlot& MB = ctx->mesh[1];
@@ -1014,14 +1047,17 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     st.filter_predicates = hc[CNT_FILTER_PRED];
     st.exact_predicates = hc[CNT_EXACT_PRED];
     st.grid = hgrid[0] && hgrid[1];
-    st.ms_build = elapsed(ctx->ev[0], ctx->ev[1]);
+    const int tmask = timing_mask(o);
+    const bool all_timed = tmask == kTimeAll;
+    if (all_timed) st.ms_build = elapsed(ctx->ev[0], ctx->ev[1]);
     if (ctx->chunk_timed) {
         st.ms_bvh_chunks = elapsed(ctx->ev_chunk[0], ctx->ev_chunk[1]);
         st.bvh_chunk_launches = 1;
         st.bvh_chunk_leaves = ctx->chunk_leaves;
     }
-    st.ms_traverse = elapsed(ctx->ev[6], ctx->ev[2]);
-    st.ms_filter = elapsed(ctx->ev[2], ctx->ev[3]);
+    if (ctx->stage_timed) st.ms_stage_meshes = elapsed(ctx->ev_stage[0], ctx->ev_stage[1]);
+    if (tmask & GI_TIME_TRAVERSE) st.ms_traverse = elapsed(ctx->ev[6], ctx->ev[2]);
+    if (all_timed) st.ms_filter = elapsed(ctx->ev[2], ctx->ev[3]);
     st.graph = graphed ? 1 : 0;
 
     // the exact stage ran on the GPU: int64/int128 for grid meshes, the wide
@@ -1034,7 +1070,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     st.exact_yellow_no = hc[CNT_EXACT_YELLOW_NO];
     st.coplanar = hc[CNT_COPLANAR];
     st.degenerate = hc[CNT_DEGENERATE];
-    st.ms_exact = elapsed(ctx->ev[3], ctx->ev[4]);
+    if (all_timed) st.ms_exact = elapsed(ctx->ev[3], ctx->ev[4]);
     uint32_t R = hc[CNT_RESULTS], S = o.want_segments ? hc[CNT_SEGMENTS] : 0;
     const size_t words = 8 * (size_t)L;
     // small result sets were sorted and gathered before the read-back
@@ -1084,7 +1120,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         HIP_TRY(hipMemcpyAsync(ctx->h_pair_off.data(), poff, 4 * ((size_t)M + 1), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));  // pageable destination
     }
-    if (host_work) record(ctx, ctx->ev[5], s);
+    if (host_work && all_timed) record(ctx, ctx->ev[5], s);
     HIP_TRY(hipGetLastError());
     st.pairs = R;
     st.segments = S;
@@ -1092,7 +1128,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     out->n_segments = S;
     out->exact_limbs = L;
     if (o.output_on_device) {
-        HIP_TRY(hipEventSynchronize(ctx->ev[5]));
+        HIP_TRY(hipStreamSynchronize(s));
         out->on_device = 1;
         out->pairs = (uint32_t*)pairs;
         out->seg_offsets = offs;
@@ -1123,8 +1159,10 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         out->seg_f64 = hs->f64.data();
         out->internal = hs;
     }
-    st.ms_output = elapsed(ctx->ev[4], ctx->ev[5]);
-    st.ms_total = elapsed(ctx->ev[0], ctx->ev[5]);
+    if (all_timed) {
+        st.ms_output = elapsed(ctx->ev[4], ctx->ev[5]);
+        st.ms_total = elapsed(ctx->ev[0], ctx->ev[5]);
+    }
     return GI_OK;
 }
 
@@ -1205,6 +1243,7 @@ int gi_create(const gi_config* cfg, gi_context** out) {
         HIP_TRY(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
         for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
         for (auto& e : ctx->ev_chunk) HIP_TRY(hipEventCreate(&e));
+        for (auto& e : ctx->ev_stage) HIP_TRY(hipEventCreate(&e));
         for (auto& m : ctx->mesh) m.xf = identity_xform();
         *out = ctx;
         return GI_OK;
@@ -1224,6 +1263,8 @@ int gi_destroy(gi_context* ctx) {
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->ev_chunk)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->ev_stage)
         if (e) (void)hipEventDestroy(e);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->h_xf) (void)hipHostFree(ctx->h_xf);
@@ -1394,7 +1435,7 @@ int gi_intersect_oneshot(const gi_config* cfg, const double* xyzA, uint32_t nvA,
     gi_context* ctx = nullptr;
     int st = gi_create(cfg, &ctx);
     if (st) return st;
-    gi_options o = {0, 1, 0, 0};
+    gi_options o = {0, 1, 0, 0, 0, 0};
     if (opt) o = *opt;
     o.output_on_device = 0;  // the context dies with this call
     st = gi_set_mesh(ctx, 0, xyzA, nvA, triA, ntA);
@@ -1619,10 +1660,11 @@ int gi_autotune(gi_context* ctx, const gi_options* opt, int32_t reps, gi_tuning*
     if (!ctx) return GI_ERR_INVALID_ARGUMENT;
     if (!ctx->mesh[0].set || !ctx->mesh[1].set) return GI_ERR_NOT_READY;
     if (reps <= 0) reps = 3;
-    gi_options o = {1, 1, 0, 0};
+    gi_options o = {1, 1, 0, 0, 0, 0};
     if (opt) o = *opt;
     o.output_on_device = 1;
     o.use_graph = 0;
+    o.stage_timing = 0;  // every stage timed: the stage times pick the launch shapes
     try {
         HIP_TRY(hipSetDevice(ctx->device));
         auto query = [&](bool rebuild) {

@@ -93,16 +93,30 @@ void launch_forest_globalize(uint32_t* tri, const uint2* ttiles, uint32_t n_ttil
 void launch_query_init(uint32_t* counters, uint32_t nzero, uint32_t* grid_flags, uint32_t stage_mask, hipStream_t s);
 // Grid-stride width of the vertex part of a mesh's stage launch.
 constexpr uint32_t kPrepareMaxBlocks = 4096;
-// One launch for a mesh's vertices (moved by *xf, in device memory so that a
+// One mesh's staging: its vertices (moved by *xf, in device memory so that a
 // captured launch sequence picks up a new transform from the copy before it;
 // grid flag cleared when a vertex is off the grid) and its triangles (boxes,
 // and with keys non-null the Morton keys, identity values and, with hist0
 // non-null, the first radix pass's per-tile digit counts, as a SortJobU32
 // takes them with hist0_ready). The Morton frame is the box of xf's moved
 // source-bound corners (xf->lo, xf->hi).
-void launch_stage_mesh(const double* src, uint32_t nv, const uint32_t* tri, uint32_t nt, const Xform* xf,
-                       double4* v64, int4* vi, uint32_t* grid_flag, int axis_bits, float4* box, uint32_t* keys,
-                       uint32_t* vals, uint32_t* hist0, hipStream_t s);
+struct StageJob {
+    const double* src;
+    uint32_t nv;
+    const uint32_t* tri;
+    uint32_t nt;
+    const Xform* xf;
+    double4* v64;
+    int4* vi;
+    uint32_t* grid_flag;
+    int axis_bits;
+    float4* box;
+    uint32_t* keys;
+    uint32_t* vals;
+    uint32_t* hist0;
+};
+// Up to two meshes staged in one launch.
+void launch_stage_meshes(const StageJob* jobs, int njobs, hipStream_t s);
 
 // Forest (a batch of meshes stored back to back, indices global): tiles of
 // kForestTile vertices/triangles, each inside one mesh, tile = (mesh, first).

@@ -19,6 +19,8 @@
 //                     src/single_tree/SingleTreeBatchIntersector.cu:276-384
 #include <hip/hip_runtime.h>
 
+#include <string.h>
+
 #include <type_traits>
 
 #include "exact_tritri.h"
@@ -364,39 +366,51 @@ constexpr int SORT_ITEMS = 4;
 constexpr uint32_t SORT_TILE = SORT_THREADS * SORT_ITEMS;
 constexpr int RADIX = 256;
 
-// Everything of one mesh before the Morton sort, in one launch. Blocks
-// [0, vblocks) prepare the vertices (move, snap, grid check, double and int
-// copies); every later block takes one sort tile of 1024 triangles: box,
-// Morton key, identity value and the tile's histogram of the first sort
-// digit (the sort's first count pass). A triangle moves its three vertices
-// itself, with the same arithmetic as the vertex blocks, so the two halves
-// do not wait for each other; the Morton frame comes from the host (the
-// box of the moved corners of the source vertices' bounds), not from a
-// reduction in this launch; centroids outside it would be clamped to the
-// border cells. The boxes use the doubles: for a grid mesh the integer
-// vertices are the same values.
-__global__ __launch_bounds__(256) void k_stage_mesh(const double* __restrict__ src, uint32_t nv,
-                                                    const uint32_t* __restrict__ tri, uint32_t nt,
-                                                    const Xform* __restrict__ xfp, double4* __restrict__ v64,
-                                                    int4* __restrict__ vi, uint32_t* grid_flag, uint32_t vblocks,
-                                                    int axis_bits, float4* __restrict__ box,
-                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                    uint32_t* __restrict__ hist0) {
-    const Xform xf = *xfp;
-    if (blockIdx.x < vblocks) {
+// Everything of up to two meshes before their Morton sorts, in one launch.
+// Per mesh, vertex blocks prepare the vertices (move, snap, grid check,
+// double and int copies; grid-stride) and triangle blocks take one sort tile
+// of 1024 triangles each: box, Morton key, identity value and the tile's
+// histogram of the first sort digit (the sort's first count pass). A
+// triangle moves its three vertices itself, with the same arithmetic as the
+// vertex blocks, so the two halves do not wait for each other; the Morton
+// frame comes from the host (the box of the moved corners of the source
+// vertices' bounds), not from a reduction in this launch; centroids outside
+// it would be clamped to the border cells. The boxes use the doubles: for a
+// grid mesh the integer vertices are the same values.
+// Block ranges: [vertex blocks of mesh 0][vertex blocks of mesh 1][triangle
+// tiles of mesh 0, padded to a multiple of 8][triangle tiles of mesh 1];
+// every range starts at a multiple of 8 (block b runs on XCD b % 8), so the
+// tiles of a mesh take XCD-contiguous tile numbers (xcd_block) and
+// neighbouring tiles' first-pass count words share an L2. Fields are picked
+// with selects on block-uniform flags.
+struct StageJobs {
+    StageJob j[2];
+    uint32_t vb0, vb1, tb0, tb1;  // blocks of each range (tb0 padded)
+};
+
+__global__ __launch_bounds__(256) void k_stage_mesh(StageJobs J) {
+    const uint32_t b = blockIdx.x;
+    const bool vert = b < J.vb0 + J.vb1;
+    const bool second = vert ? b >= J.vb0 : b >= J.vb0 + J.vb1 + J.tb0;
+    const StageJob& q = second ? J.j[1] : J.j[0];
+    const Xform xf = *q.xf;
+    if (vert) {
+        const uint32_t vb = second ? J.vb1 : J.vb0, b0 = second ? b - J.vb0 : b;
         bool grid = true;
         float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += vblocks * blockDim.x)
-            grid = prepare_vertex(src, i, xf, v64, vi, l, h) && grid;
-        if (__ballot(!grid) && lane_id() == 0) atomicAnd(grid_flag, 0u);
+        for (uint32_t i = b0 * blockDim.x + threadIdx.x; i < q.nv; i += vb * blockDim.x)
+            grid = prepare_vertex(q.src, i, xf, q.v64, q.vi, l, h) && grid;
+        if (__ballot(!grid) && lane_id() == 0) atomicAnd(q.grid_flag, 0u);
         return;
     }
-    // XCD-contiguous tiles (vblocks is a multiple of 8, so tile block b runs
-    // on XCD b % 8): neighbouring tiles' first-pass count words share an L2
-    const uint32_t tile = xcd_block(blockIdx.x - vblocks, gridDim.x - vblocks);
+    const uint32_t t0 = J.vb0 + J.vb1 + (second ? J.tb0 : 0u);
+    const uint32_t ntiles = (q.nt + SORT_TILE - 1) / SORT_TILE;
+    const uint32_t span = second ? J.tb1 : J.tb0;  // padded range of this mesh
+    const uint32_t tile = xcd_block(b - t0, span);
+    if (tile >= ntiles) return;  // padding
     __shared__ uint32_t hs[4][RADIX];
     __shared__ float frame[6];  // Morton frame: lo[3], ext[3]
-    if (hist0)
+    if (q.hist0)
         for (int d = threadIdx.x; d < 4 * RADIX; d += blockDim.x) (&hs[0][0])[d] = 0u;
     if (threadIdx.x == 0) {
         double lo[3] = {xf.lo[0], xf.lo[1], xf.lo[2]}, hi[3] = {xf.hi[0], xf.hi[1], xf.hi[2]};
@@ -420,40 +434,48 @@ __global__ __launch_bounds__(256) void k_stage_mesh(const double* __restrict__ s
     __syncthreads();
     for (int it = 0; it < SORT_ITEMS; ++it) {
         const uint32_t i = tile * SORT_TILE + it * 256 + threadIdx.x;
-        if (i >= nt) continue;
+        if (i >= q.nt) continue;
         double p[3][3];
-        for (int c = 0; c < 3; ++c) moved_vertex(src, tri[3 * (size_t)i + c], xf, p[c][0], p[c][1], p[c][2]);
+        for (int c = 0; c < 3; ++c) moved_vertex(q.src, q.tri[3 * (size_t)i + c], xf, p[c][0], p[c][1], p[c][2]);
         float bl[3], bh[3];
         for (int k = 0; k < 3; ++k) {
             bl[k] = f32_down(fmin(fmin(p[0][k], p[1][k]), p[2][k]));
             bh[k] = f32_up(fmax(fmax(p[0][k], p[1][k]), p[2][k]));
         }
-        box[2 * (size_t)i] = make_float4(bl[0], bl[1], bl[2], 0.f);
-        box[2 * (size_t)i + 1] = make_float4(bh[0], bh[1], bh[2], 0.f);
-        if (keys) {
-            const uint32_t key = morton_code(bl, bh, frame, frame + 3, axis_bits);
-            keys[i] = key;
-            vals[i] = i;
-            if (hist0) atomicAdd(&hs[threadIdx.x >> 6][key & (RADIX - 1)], 1u);
+        q.box[2 * (size_t)i] = make_float4(bl[0], bl[1], bl[2], 0.f);
+        q.box[2 * (size_t)i + 1] = make_float4(bh[0], bh[1], bh[2], 0.f);
+        if (q.keys) {
+            const uint32_t key = morton_code(bl, bh, frame, frame + 3, q.axis_bits);
+            q.keys[i] = key;
+            q.vals[i] = i;
+            if (q.hist0) atomicAdd(&hs[threadIdx.x >> 6][key & (RADIX - 1)], 1u);
         }
     }
-    if (hist0) {
+    if (q.hist0) {
         __syncthreads();
-        const uint32_t ntiles = (nt + SORT_TILE - 1) / SORT_TILE;
         for (int d = threadIdx.x; d < RADIX; d += blockDim.x)
-            hist0[(size_t)d * ntiles + tile] = hs[0][d] + hs[1][d] + hs[2][d] + hs[3][d];
+            q.hist0[(size_t)d * ntiles + tile] = hs[0][d] + hs[1][d] + hs[2][d] + hs[3][d];
     }
 }
 
-void launch_stage_mesh(const double* src, uint32_t nv, const uint32_t* tri, uint32_t nt, const Xform* xf,
-                       double4* v64, int4* vi, uint32_t* grid_flag, int axis_bits, float4* box, uint32_t* keys,
-                       uint32_t* vals, uint32_t* hist0, hipStream_t s) {
-    // a multiple of 8 workgroups (grid-stride loop), so that the triangle tiles after them start on XCD 0
-    const uint32_t vblocks = nv ? (std::min<uint32_t>(blocks_for(nv, 256), kPrepareMaxBlocks) + 7u) & ~7u : 0u;
-    const uint32_t tblocks = blocks_for(nt, SORT_TILE);
-    if (vblocks + tblocks == 0) return;
-    k_stage_mesh<<<vblocks + tblocks, 256, 0, s>>>(src, nv, tri, nt, xf, v64, vi, grid_flag, vblocks, axis_bits, box,
-                                                   keys, vals, hist0);
+void launch_stage_meshes(const StageJob* jobs, int njobs, hipStream_t s) {
+    StageJobs J;
+    memset(&J, 0, sizeof(J));
+    uint32_t vb[2] = {0, 0}, tb[2] = {0, 0};
+    for (int k = 0; k < njobs && k < 2; ++k) {
+        J.j[k] = jobs[k];
+        // multiples of 8 workgroups, so that every range starts on XCD 0
+        vb[k] = jobs[k].nv ? (std::min<uint32_t>(blocks_for(jobs[k].nv, 256), kPrepareMaxBlocks) + 7u) & ~7u : 0u;
+        tb[k] = (blocks_for(jobs[k].nt, SORT_TILE) + 7u) & ~7u;
+    }
+    if (njobs < 2) J.j[1] = J.j[0];
+    J.vb0 = vb[0];
+    J.vb1 = vb[1];
+    J.tb0 = tb[0];
+    J.tb1 = tb[1];
+    const uint32_t blocks = vb[0] + vb[1] + tb[0] + tb[1];
+    if (blocks == 0) return;
+    k_stage_mesh<<<blocks, 256, 0, s>>>(J);
     GI_LAUNCH_CHECK();
 }
 
@@ -1666,20 +1688,60 @@ __device__ __forceinline__ uint32_t block_reserve(uint32_t v, uint32_t* counters
     return wbase[wave] + ex;
 }
 
-__device__ __forceinline__ uint32_t block_sum(uint32_t v) {
-    __shared__ uint32_t red[4];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+// The same for three counters at once (one pair of barriers): thread slots
+// s[j] of v[j] entries each on counters[which[j]].
+__device__ __forceinline__ void block_reserve3(const uint32_t (&v)[3], uint32_t* counters, int w0, int w1, int w2,
+                                               uint32_t (&slot)[3]) {
+    __shared__ uint32_t wtot[3][4], wbase[3][4];
+    uint32_t ex[3];
+    const uint32_t wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        uint32_t total;
+        ex[j] = wave_exclusive_sum(v[j], total);
+        if (lane_id() == 0) wtot[j][wave] = total;
+    }
     __syncthreads();
-    if (lane_id() == 0) red[threadIdx.x >> 6] = v;
+    if (threadIdx.x < 3) {
+        const int j = (int)threadIdx.x;
+        const int which = j == 0 ? w0 : (j == 1 ? w1 : w2);
+        const uint32_t t = wtot[j][0] + wtot[j][1] + wtot[j][2] + wtot[j][3];
+        uint32_t b = t ? atomicAdd(&counters[which], t) : 0u;
+        if (u32_add_wraps(b, t)) atomicOr(&counters[CNT_OVERFLOW], 1u);
+        for (int w = 0; w < 4; ++w) { wbase[j][w] = b; b += wtot[j][w]; }
+    }
     __syncthreads();
-    uint32_t t = red[0] + red[1] + red[2] + red[3];
-    return t;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) slot[j] = wbase[j][wave] + ex[j];
+}
+
+// Block totals of N values added to counters[which[j]] (one pair of
+// barriers; zero totals add nothing).
+template <int N>
+__device__ __forceinline__ void block_add(const uint32_t (&v)[N], uint32_t* counters, const int (&which)[N]) {
+    __shared__ uint32_t red[N][4];
+    const uint32_t wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        uint32_t x = v[j];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+        if (lane_id() == 0) red[j][wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const uint32_t t = red[j][0] + red[j][1] + red[j][2] + red[j][3];
+            if (t) atomicAdd(&counters[which[j]], t);
+        }
+    }
+    __syncthreads();
 }
 
 constexpr uint32_t kStatusShift = 30;
 constexpr uint32_t kIndexMask = (1u << kStatusShift) - 1u;
 
-__global__ __launch_bounds__(256) void k_filter(MeshDev A, MeshDev B, const uint2* __restrict__ cand, uint32_t cap,
+__global__ __launch_bounds__(256, 2) void k_filter(MeshDev A, MeshDev B, const uint2* __restrict__ cand, uint32_t cap,
                                                 const uint32_t* __restrict__ grid_flags, uint2* __restrict__ exact_list,
                                                 uint32_t exact_cap, uint32_t* __restrict__ counters) {
     uint32_t n = counters[CNT_TRAVERSAL_HITS];
@@ -1730,12 +1792,9 @@ __global__ __launch_bounds__(256) void k_filter(MeshDev A, MeshDev B, const uint
         const uint32_t slot = block_reserve(emit ? 1u : 0u, counters, CNT_EXACT_LIST);
         if (emit && slot < exact_cap) exact_list[slot] = make_uint2(a, b | ((uint32_t)status << kStatusShift));
     }
-    uint32_t t;
-    t = block_sum((uint32_t)c_pred); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_FILTER_PRED], t);
-    t = block_sum(c_cand); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_CANDIDATES], t);
-    t = block_sum(c_no); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_FILTER_NO], t);
-    t = block_sum(c_green); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_FILTER_GREEN], t);
-    t = block_sum(c_yellow); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_FILTER_YELLOW], t);
+    const uint32_t tot[5] = {(uint32_t)c_pred, c_cand, c_no, c_green, c_yellow};
+    const int which[5] = {CNT_FILTER_PRED, CNT_CANDIDATES, CNT_FILTER_NO, CNT_FILTER_GREEN, CNT_FILTER_YELLOW};
+    block_add<5>(tot, counters, which);
 }
 
 void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const uint32_t* grid_flags,
@@ -1759,13 +1818,10 @@ struct ExactTally {
         deg += hit && degenerate;
     }
     __device__ void flush(uint32_t* counters) {
-        uint32_t t;
-        t = block_sum(yes); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_EXACT_YELLOW_YES], t);
-        t = block_sum(no); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_EXACT_YELLOW_NO], t);
-        t = block_sum(cop); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_COPLANAR], t);
-        t = block_sum(deg); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_DEGENERATE], t);
-        t = block_sum(bad); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_UNSOUND], t);
-        t = block_sum(pred); if (threadIdx.x == 0 && t) atomicAdd(&counters[CNT_EXACT_PRED], t);
+        const uint32_t v[6] = {yes, no, cop, deg, bad, pred};
+        const int which[6] = {CNT_EXACT_YELLOW_YES, CNT_EXACT_YELLOW_NO, CNT_COPLANAR, CNT_DEGENERATE, CNT_UNSOUND,
+                              CNT_EXACT_PRED};
+        block_add<6>(v, counters, which);
     }
 };
 
@@ -1814,14 +1870,15 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
             tally.pred += (uint32_t)r.npred;
             if (!r.deferred) tally.add((int)status, r.hit, r.coplanar, r.degenerate);
         }
-        if constexpr (kFast) {
-            const uint32_t dslot = block_reserve(r.deferred ? 1u : 0u, counters, CNT_DEFERRED);
-            if (r.deferred && dslot < defer_cap) defer[dslot] = e;
-        }
         const bool hit = r.hit && !r.deferred;
-        const uint32_t slot = block_reserve(hit ? 1u : 0u, counters, CNT_RESULTS);
         const uint32_t nseg = hit ? (uint32_t)r.nseg : 0u;
-        const uint32_t sfirst = block_reserve(nseg, counters, CNT_SEGMENTS);
+        const uint32_t want[3] = {r.deferred ? 1u : 0u, hit ? 1u : 0u, nseg};
+        uint32_t got[3];
+        block_reserve3(want, counters, CNT_DEFERRED, CNT_RESULTS, CNT_SEGMENTS, got);
+        const uint32_t slot = got[1], sfirst = got[2];
+        if constexpr (kFast) {
+            if (r.deferred && got[0] < defer_cap) defer[got[0]] = e;
+        }
         if (hit && slot < rec_cap) {
             rec_key[slot] = ((uint64_t)a << bitsB) | (uint64_t)b;
             rec_seg[slot] = make_uint2(sfirst, nseg);
@@ -1838,6 +1895,195 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
                 __int128* o = seg_pool + 8 * (size_t)dst;
                 o[0] = p.h[0]; o[1] = p.h[1]; o[2] = p.h[2]; o[3] = p.w;
                 o[4] = q.h[0]; o[5] = q.h[1]; o[6] = q.h[2]; o[7] = q.w;
+            }
+        }
+    }
+    tally.flush(counters);
+}
+
+// ----------------------------------------------- grid pairs, 8 lanes each
+//
+// The transversal pairs of grid meshes (the common case: neither triangle
+// degenerate, not coplanar) decided and constructed by 8 lanes per pair, 8
+// pairs per wave: every lane of a pair's group computes the two normals, the
+// six plane orientations and the axis of the line L = plane_A ^ plane_B; lane
+// e < 6 then takes edge e (A's edges 0-2 against triangle B, B's against A)
+// with the template's edgePiece, and three butterfly steps over the group
+// keep the extreme points along that axis, which are the segment's ends
+// (ExactTT::intersect_fast does the same edges one after another in one
+// lane). Degenerate and coplanar pairs, and pairs with an edge lying in the
+// other triangle's plane, go to the deferred list for the full pass. The
+// group's first lane writes the record and the segment.
+constexpr uint32_t kPairLanes = 8;
+
+__device__ __forceinline__ i128 shfl_xor_i128(i128 x, int m) {
+    const u128 u = (u128)x;
+    const unsigned long long lo = __shfl_xor((unsigned long long)(uint64_t)u, m);
+    const unsigned long long hi = __shfl_xor((unsigned long long)(uint64_t)(u >> 64), m);
+    return (i128)(((u128)hi << 64) | (u128)lo);
+}
+
+__device__ __forceinline__ GridExact::HP shfl_xor_hp(const GridExact::HP& p, int m) {
+    GridExact::HP r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r.h[k] = shfl_xor_i128(p.h[k], m);
+    r.w = shfl_xor_i128(p.w, m);
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_exact_grid_lanes(MeshDev A, MeshDev B, const uint2* __restrict__ list,
+                                                          uint32_t cap, const uint32_t* __restrict__ grid_flags,
+                                                          int bitsB, uint64_t* __restrict__ rec_key,
+                                                          uint2* __restrict__ rec_seg, uint32_t rec_cap,
+                                                          uint64_t* __restrict__ seg_pool_words, uint32_t seg_cap,
+                                                          uint2* __restrict__ defer, uint32_t defer_cap,
+                                                          uint32_t* __restrict__ counters) {
+    typedef GridExact::V3 V3;
+    typedef GridExact::HP HP;
+    typedef GridExact::B Bn;
+    if (!(grid_flags[0] && grid_flags[1])) return;  // general doubles: k_exact_wide
+    __int128* seg_pool = (__int128*)seg_pool_words;
+    uint32_t n = counters[CNT_EXACT_LIST];
+    if (n > cap) n = cap;
+    constexpr uint32_t kPerBlock = 256 / kPairLanes;
+    const int e = (int)(lane_id() & (kPairLanes - 1));
+    const bool leader = e == 0;
+    ExactTally tally;
+    for (uint32_t base = blockIdx.x * kPerBlock; base < n; base += gridDim.x * kPerBlock) {
+        const uint32_t i = base + threadIdx.x / kPairLanes;
+        const bool valid = i < n;
+        uint2 en = make_uint2(0, 0);
+        uint32_t a = 0, b = 0, status = 0;
+        V3 PA[3], PB[3];
+        if (valid) {
+            en = list[i];
+            a = en.x;
+            b = en.y & kIndexMask;
+            status = en.y >> kStatusShift;
+            const uint32_t* ta = A.tri + 3 * (size_t)a;
+            const uint32_t* tb = B.tri + 3 * (size_t)b;
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const int4 p = A.vi[ta[v]], q = B.vi[tb[v]];
+                PA[v].v[0] = p.x; PA[v].v[1] = p.y; PA[v].v[2] = p.z;
+                PB[v].v[0] = q.x; PB[v].v[1] = q.y; PB[v].v[2] = q.z;
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 3; ++v)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) PA[v].v[k] = PB[v].v[k] = 0;
+        }
+        // shared by the group: normals, shapes, plane orientations, axis
+        const V3 nA = GridExact::cross(GridExact::sub(PA[1], PA[0]), GridExact::sub(PA[2], PA[0]));
+        const V3 nB = GridExact::cross(GridExact::sub(PB[1], PB[0]), GridExact::sub(PB[2], PB[0]));
+        bool live = valid, deferred = false;
+        int shared_pred = 0;
+        if (live && (GridExact::isZero(nA) || GridExact::isZero(nB))) {  // a degenerate triangle
+            deferred = true;
+            live = false;
+        }
+        Bn oA[3], oB[3];
+        int sA[3], sB[3];
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            oB[v] = GridExact::dotB(nA, GridExact::sub(PB[v], PA[0]));
+            sB[v] = sgn128(oB[v]);
+            oA[v] = GridExact::dotB(nB, GridExact::sub(PA[v], PB[0]));
+            sA[v] = sgn128(oA[v]);
+        }
+        if (live) {
+            shared_pred += 3;
+            if (sB[0] == 0 && sB[1] == 0 && sB[2] == 0) {  // coplanar
+                deferred = true;
+                live = false;
+            } else if (sB[0] * sB[1] > 0 && sB[1] * sB[2] > 0) {
+                live = false;
+            } else {
+                shared_pred += 3;
+                if (sA[0] * sA[1] > 0 && sA[1] * sA[2] > 0) live = false;
+            }
+        }
+        const Bn d0 = (Bn)nA.v[1] * nB.v[2] - (Bn)nA.v[2] * nB.v[1];
+        const Bn d1 = (Bn)nA.v[2] * nB.v[0] - (Bn)nA.v[0] * nB.v[2];
+        const Bn d2 = (Bn)nA.v[0] * nB.v[1] - (Bn)nA.v[1] * nB.v[0];
+        int k = GridPolicy::cmpAbsB(d1, d0) > 0 ? 1 : 0;
+        if (GridPolicy::cmpAbsB(d2, k ? d1 : d0) > 0) k = 2;
+        // this lane's edge
+        HP lo, hi;
+        int cnt = 0, np = 0, in_plane = 0;
+        if (live && e < 6) {
+            // the lane's edge (Ti, Tj), the other triangle O, by component
+            // selects (arrays picked at run time would live in scratch memory)
+            const bool eA = e < 3;
+            V3 Ti, Tj, O[3], nO;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int64_t a0 = PA[0].v[c], a1 = PA[1].v[c], a2 = PA[2].v[c];
+                const int64_t b0 = PB[0].v[c], b1 = PB[1].v[c], b2 = PB[2].v[c];
+                Ti.v[c] = e == 0 ? a0 : e == 1 ? a1 : e == 2 ? a2 : e == 3 ? b0 : e == 4 ? b1 : b2;
+                Tj.v[c] = e == 0 ? a1 : e == 1 ? a2 : e == 2 ? a0 : e == 3 ? b1 : e == 4 ? b2 : b0;
+                O[0].v[c] = eA ? b0 : a0;
+                O[1].v[c] = eA ? b1 : a1;
+                O[2].v[c] = eA ? b2 : a2;
+                nO.v[c] = eA ? nB.v[c] : nA.v[c];
+            }
+            const Bn oi = e == 0 ? oA[0] : e == 1 ? oA[1] : e == 2 ? oA[2] : e == 3 ? oB[0] : e == 4 ? oB[1] : oB[2];
+            const Bn oj = e == 0 ? oA[1] : e == 1 ? oA[2] : e == 2 ? oA[0] : e == 3 ? oB[1] : e == 4 ? oB[2] : oB[0];
+            const int si = sgn128(oi), sj = sgn128(oj);
+            if (si == 0 && sj == 0) {
+                in_plane = 1;  // an edge in the other triangle's plane: the full pass clips it
+            } else {
+                HP p0, p1;
+                const int m = GridExact::edgePiece(Ti, Tj, O, nO, oi, oj, si, sj, p0, p1, np);
+                if (m >= 1) {
+                    lo = p0;
+                    hi = p0;
+                    cnt = 1;
+                }
+            }
+        }
+        // a pair with an edge in the other's plane goes to the full pass whole
+#pragma unroll
+        for (int off = 1; off < (int)kPairLanes; off <<= 1) in_plane |= __shfl_xor(in_plane, off);
+        if (in_plane) {
+            deferred = true;
+            live = false;
+            cnt = 0;
+        }
+        // extremes over the group (every lane takes part in the shuffles)
+#pragma unroll
+        for (int off = 1; off < (int)kPairLanes; off <<= 1) {
+            const int ocnt = __shfl_xor(cnt, off);
+            const HP olo = shfl_xor_hp(lo, off), ohi = shfl_xor_hp(hi, off);
+            if (ocnt) {
+                if (!cnt) {
+                    lo = olo;
+                    hi = ohi;
+                } else {
+                    if (GridExact::hpCmpAxis(olo, lo, k) < 0) lo = olo;
+                    if (GridExact::hpCmpAxis(ohi, hi, k) > 0) hi = ohi;
+                }
+                cnt += ocnt;
+            }
+        }
+        const bool hit = live && cnt > 0;
+        tally.pred += (uint32_t)(np + (leader ? shared_pred : 0));
+        if (leader && valid && !deferred) tally.add((int)status, hit, 0, 0);
+        const bool dput = leader && deferred;
+        const bool put = leader && hit;
+        const uint32_t want[3] = {dput ? 1u : 0u, put ? 1u : 0u, put ? 1u : 0u};
+        uint32_t got[3];
+        block_reserve3(want, counters, CNT_DEFERRED, CNT_RESULTS, CNT_SEGMENTS, got);
+        const uint32_t dslot = got[0], slot = got[1], sfirst = got[2];
+        if (dput && dslot < defer_cap) defer[dslot] = en;
+        if (put && slot < rec_cap) {
+            rec_key[slot] = ((uint64_t)a << bitsB) | (uint64_t)b;
+            rec_seg[slot] = make_uint2(sfirst, 1u);
+            if (sfirst < seg_cap) {
+                __int128* o = seg_pool + 8 * (size_t)sfirst;
+                o[0] = lo.h[0]; o[1] = lo.h[1]; o[2] = lo.h[2]; o[3] = lo.w;
+                o[4] = hi.h[0]; o[5] = hi.h[1]; o[6] = hi.h[2]; o[7] = hi.w;
             }
         }
     }
@@ -1962,8 +2208,8 @@ void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_
     // grid flags (decided on the device by the vertex stage) do not select
     // returns at once
     if (kinds & kExactGrid) {
-        k_exact<true><<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
-                                             rec_cap, seg_pool, seg_cap, defer, defer_cap, counters);
+        k_exact_grid_lanes<<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
+                                                  rec_cap, seg_pool, seg_cap, defer, defer_cap, counters);
         // the deferred (coplanar) pairs: their count is read on the device
         k_exact<false><<<blocks, 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
                                               seg_pool, seg_cap, nullptr, 0, counters);

@@ -230,12 +230,14 @@ extern "C" int gi_intersect_noalloc(int32_t device, const double* xyzA, uint32_t
     uint32_t* grid = counters + CNT_COUNT;
     launch_query_init(counters, CNT_COUNT, grid, 3u, s);
     SortJobU32 jobs[2];
+    StageJob sj[2];
     for (int k = 0; k < 2; ++k) {
         MeshBufs& m = M[k];
-        launch_stage_mesh(m.src, nv[k], m.tri, nt[k], d_xf + k, m.v64, m.vi, grid + k, m.axis, m.tbox, m.keys, m.vals,
-                          m.multi ? m.hist : nullptr, s);
+        sj[k] = StageJob{m.src, nv[k], m.tri, nt[k], d_xf + k, m.v64, m.vi, grid + k, m.axis, m.tbox, m.keys, m.vals,
+                         m.multi ? m.hist : nullptr};
         jobs[k] = SortJobU32{m.keys, m.vals, m.tk, m.tv, nt[k], 3 * m.axis, m.multi, m.hist, m.stmp, m.samples};
     }
+    launch_stage_meshes(sj, 2, s);
     int in_tmp[2] = {0, 0};
     radix_sort_u32_pair(jobs, 2, s, in_tmp);
     BuildJob<uint32_t> bj;

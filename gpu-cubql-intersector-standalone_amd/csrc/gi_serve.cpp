@@ -96,7 +96,9 @@ void worker(gi_server* s, int slot) {
         gi_context* ctx = s->slot_ctx[slot][rq.mesh_id];
         int st = gi_set_transform_matrix(ctx, GI_MESH_B, rq.m, s->cfg.snap_to_grid);
         if (st == GI_OK) {
-            gi_options o = {0, s->cfg.want_segments, 0, 1};  // same shapes every request: replay a graph
+            // same shapes every request: replay a graph; service times come
+            // from the host clock, so no stage timing events
+            gi_options o = {0, s->cfg.want_segments, 0, 1, GI_TIME_NONE, 0};
             st = gi_intersect(ctx, &o, &res);
         }
         Clock::time_point t_done = Clock::now();

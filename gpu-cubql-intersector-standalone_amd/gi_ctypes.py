@@ -24,14 +24,16 @@ class GiConfig(C.Structure):
 
 class GiOptions(C.Structure):
     _fields_ = [("output_on_device", C.c_int32), ("want_segments", C.c_int32), ("rebuild", C.c_int32),
-                ("use_graph", C.c_int32)]
+                ("use_graph", C.c_int32), ("stage_timing", C.c_int32), ("reserved", C.c_int32)]
 
 
 _STAT_U64 = ["traversal_hits", "candidates", "filter_no", "filter_green", "filter_yellow", "exact_pairs",
              "exact_yellow_yes", "exact_yellow_no", "host_pairs", "host_yes", "pairs", "segments", "coplanar",
              "degenerate", "filter_predicates", "exact_predicates", "host_predicates"]
 _STAT_F32 = ["ms_total", "ms_build", "ms_traverse", "ms_filter", "ms_exact", "ms_output", "ms_host",
-             "ms_bvh_chunks"]
+             "ms_bvh_chunks", "ms_stage_meshes", "pad_ms"]
+
+TIME_STAGING, TIME_BVH_CHUNKS, TIME_TRAVERSE, TIME_NONE = 1, 2, 4, 8
 
 
 class GiStats(C.Structure):
@@ -256,10 +258,13 @@ class Intersector:
     def reconstruct_device(self):
         _check(lib().gi_reconstruct_device(self._ctx), self._ctx)
 
-    def intersect_raw(self, output_on_device=False, want_segments=True, rebuild=False, use_graph=False) -> GiResult:
-        """Runs the query; the caller must gi_result_free the returned struct."""
+    def intersect_raw(self, output_on_device=False, want_segments=True, rebuild=False, use_graph=False,
+                      stage_timing=0) -> GiResult:
+        """Runs the query; the caller must gi_result_free the returned struct.
+        stage_timing: 0 every stage timed; else a mask of TIME_STAGING,
+        TIME_BVH_CHUNKS, TIME_TRAVERSE (TIME_NONE: none)."""
         res = GiResult()
-        opt = GiOptions(int(output_on_device), int(want_segments), int(rebuild), int(use_graph))
+        opt = GiOptions(int(output_on_device), int(want_segments), int(rebuild), int(use_graph), int(stage_timing), 0)
         _check(lib().gi_intersect(self._ctx, C.byref(opt), C.byref(res)), self._ctx)
         return res
 

@@ -83,7 +83,19 @@ typedef struct {
     int32_t use_graph;          /* 1: replay the query's launches up to the counter read-back as a
                                    HIP graph while sizes and buffers stay the same (captured on
                                    the second such query; its timing events are graph nodes) */
+    int32_t stage_timing;       /* HIP events that time the stages (each costs the queue a few
+                                   microseconds): 0 every stage (all ms_* statistics); otherwise
+                                   only the launches in the mask GI_TIME_STAGING | GI_TIME_BVH_CHUNKS
+                                   | GI_TIME_TRAVERSE (GI_TIME_NONE: no event at all) */
+    int32_t reserved;
 } gi_options;
+
+enum {
+    GI_TIME_STAGING = 1,    /* the staging launch of both meshes (ms_stage_meshes) */
+    GI_TIME_BVH_CHUNKS = 2, /* the k_build_chunks launch (ms_bvh_chunks) */
+    GI_TIME_TRAVERSE = 4,   /* the traversal (ms_traverse) */
+    GI_TIME_NONE = 8
+};
 
 typedef struct {
     /* candidate pipeline: traversal (FP32 boxes) -> exact boxes -> filter -> exact */
@@ -105,6 +117,8 @@ typedef struct {
     /* milliseconds (GPU events; host stage by wall clock) */
     float ms_total, ms_build, ms_traverse, ms_filter, ms_exact, ms_output, ms_host;
     float ms_bvh_chunks;        /* the k_build_chunks launch of this call (A's tree; a B shared into an A slot builds in the same launch) */
+    float ms_stage_meshes;      /* the staging launch of this call (both single meshes: vertices, boxes, Morton keys) */
+    float pad_ms;
     int32_t bvh_chunk_launches; /* 0 or 1 */
     uint32_t bvh_chunk_leaves;  /* leaves that launch built or refit */
     int32_t graph;              /* 1: the launches up to the counter read-back replayed a HIP graph */
