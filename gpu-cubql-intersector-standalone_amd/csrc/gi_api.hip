@@ -390,6 +390,8 @@ void stage_forest(gi_context* ctx, int slot) {
 // on the slot's own stream; a single mesh fills *job for the one staging
 // launch both slots share. Returns true when the slot's tree must then be
 // built or refit (build_meshes).
+constexpr int kQueryMortonBits = 5;
+
 bool stage_mesh(gi_context* ctx, int slot, bool rebuild, StageJob* job) {
     MeshSlot& m = ctx->mesh[slot];
     if (m.borrowed) return false;  // built and kept current by its owner
@@ -421,7 +423,10 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild, StageJob* job) {
         ctx->xf_last[slot] = x;
         ctx->xf_current[slot] = !ctx->capturing;
     }
-    if (topology) m.morton_axis = morton_axis_bits(m.nt);
+    // B's triangles in Morton order are only the traversal's queries (no
+    // tree): a coarser curve (5 bits per axis, 15-bit keys, two sort passes)
+    // keeps a wave's queries together; a tree needs the finer keys
+    if (topology) m.morton_axis = (slot == GI_MESH_B && !m.full_tree) ? kQueryMortonBits : morton_axis_bits(m.nt);
     uint32_t* hist = nullptr;
     uint32_t* stmp = nullptr;
     if (topology) {
@@ -1456,9 +1461,11 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         MeshSlot& sm = src->mesh[src_slot];
         if (dst_slot == GI_MESH_A && !sm.full_tree && src_slot == GI_MESH_B) {
             if (sm.borrowed) return GI_ERR_INVALID_ARGUMENT;  // its owner built no tree for it
-            // an A slot walks the tree: build B's tree from now on (on the kept sorted keys)
+            // an A slot walks the tree: build B's tree from now on, on the
+            // finer Morton keys a tree needs (a new topology)
             sm.full_tree = true;
             sm.dirty = true;
+            sm.built = false;
         }
         build_meshes(src, false);
         sync_all(src);

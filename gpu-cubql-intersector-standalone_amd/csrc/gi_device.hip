@@ -2211,7 +2211,9 @@ void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_
         k_exact_grid_lanes<<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
                                                   rec_cap, seg_pool, seg_cap, defer, defer_cap, counters);
         // the deferred (coplanar) pairs: their count is read on the device
-        k_exact<false><<<blocks, 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
+        // usually empty: a quarter of the blocks (each pair takes one lane here)
+        k_exact<false><<<std::max(blocks / 4, 64), 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key,
+                                                                rec_seg, rec_cap,
                                               seg_pool, seg_cap, nullptr, 0, counters);
     }
     if (kinds & kExactWide) {
