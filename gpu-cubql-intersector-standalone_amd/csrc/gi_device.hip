@@ -267,6 +267,21 @@ __device__ __forceinline__ void block_box_partial(const float* l, const float* h
     }
 }
 
+// source coordinates as the query sees them: moved, and snapped for grid meshes
+__device__ __forceinline__ void moved_coords(double sx, double sy, double sz, const Xform& xf, double& x, double& y,
+                                             double& z) {
+    x = sx;
+    y = sy;
+    z = sz;
+    if (!xf.identity) {
+        double X = xform_coord(xf, 0, x, y, z);
+        double Y = xform_coord(xf, 1, x, y, z);
+        double Z = xform_coord(xf, 2, x, y, z);
+        x = X; y = Y; z = Z;
+    }
+    if (xf.snap) { x = rint(x); y = rint(y); z = rint(z); }
+}
+
 // vertex i as the query sees it: moved, and snapped for grid meshes
 __device__ __forceinline__ void moved_vertex(const double* __restrict__ src, uint32_t i, const Xform& xf, double& x,
                                              double& y, double& z) {
@@ -432,11 +447,31 @@ __global__ __launch_bounds__(256) void k_stage_mesh(StageJobs J) {
         }
     }
     __syncthreads();
+    // all items' indices, then all their vertices, before any store: the
+    // gathers of the tile's four triangles per thread are in flight together
+    const uint32_t* __restrict__ tri = q.tri;
+    const double* __restrict__ src = q.src;
+    uint32_t id[SORT_ITEMS][3];
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        const uint32_t i = tile * SORT_TILE + it * 256 + threadIdx.x;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) id[it][c] = i < q.nt ? tri[3 * (size_t)i + c] : 0u;
+    }
+    double raw[SORT_ITEMS][3][3];
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) raw[it][c][k] = src[3 * (size_t)id[it][c] + k];
+#pragma unroll
     for (int it = 0; it < SORT_ITEMS; ++it) {
         const uint32_t i = tile * SORT_TILE + it * 256 + threadIdx.x;
         if (i >= q.nt) continue;
         double p[3][3];
-        for (int c = 0; c < 3; ++c) moved_vertex(q.src, q.tri[3 * (size_t)i + c], xf, p[c][0], p[c][1], p[c][2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) moved_coords(raw[it][c][0], raw[it][c][1], raw[it][c][2], xf, p[c][0], p[c][1], p[c][2]);
         float bl[3], bh[3];
         for (int k = 0; k < 3; ++k) {
             bl[k] = f32_down(fmin(fmin(p[0][k], p[1][k]), p[2][k]));
