@@ -15,10 +15,12 @@ value = exact candidate pairs decided per second (candidates / step time),
 inputs resident in HBM. `e2e` times the same work from pinned host arrays
 (both meshes uploaded, A's BVH built, the query, sorted pairs and exact
 segments copied to host memory), labelled separately.
-roofline = HBM roofline of the dominant kernel, the longer of the step's two
-largest single launches (the traversal and A's BVH build), each timed by the
-library's HIP events around its one launch per step; the other one is
-reported under roofline_other. Achieved bandwidth = the kernel's bytes per
+roofline = HBM roofline of the dominant kernel, the longest of the step's
+three largest single launches (staging of both meshes, A's BVH build and the
+traversal), chosen in the warm-up steps, which time all three with the
+library's HIP events; the timed steps bracket only that kernel, and the
+others are reported under roofline_other from a few more steps that time all
+three. Achieved bandwidth = the kernel's bytes per
 launch / its average duration, where the bytes are the cost model's, capped
 by the measured HBM traffic of the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
 summary when that is lower (the traversal reads only the part of A's tree
