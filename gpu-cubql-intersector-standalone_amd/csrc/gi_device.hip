@@ -741,6 +741,21 @@ struct SortPass {
 // the sorted output then fill whole lines in one L2 instead of partial lines
 // in eight (16.4 -> 2.0 MB written per upsweep of 2^21 keys, 31 -> 16.5 MB per
 // downsweep).
+// Lanes of a wave holding the same 8-bit digit (among the valid ones).
+__device__ __forceinline__ uint64_t digit_match(uint32_t d, bool valid) {
+    uint64_t match = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const uint64_t bb = __ballot((d >> b) & 1u);
+        match &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    return match;
+}
+
+// Per-tile digit counts. A wave counts its keys item by item: the lowest lane
+// of each group of equal digits adds the group's size to the wave's row, so
+// no two lanes update one LDS word at once (LDS atomics on Morton digits,
+// many lanes to one digit, serialised).
 template <class K>
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_upsweep(SortPass<K> P) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -751,13 +766,23 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_upsweep(SortPass<K> P) {
     const uint32_t tile = blk - (b ? P.split : 0u);
     const int shift = P.shift;
     __shared__ uint32_t h[SORT_THREADS / 64][RADIX];
-    uint32_t wave = threadIdx.x >> 6;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t base = tile * SORT_TILE + wave * (64 * SORT_ITEMS) + lane_id();
+    K k[SORT_ITEMS];
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        const uint32_t i = base + it * 64;
+        k[it] = i < n ? keys[i] : (K)0;
+    }
     for (int d = threadIdx.x; d < (SORT_THREADS / 64) * RADIX; d += SORT_THREADS) (&h[0][0])[d] = 0;
     __syncthreads();
-    uint32_t base = tile * SORT_TILE;
+    const uint64_t lt = lanemask_lt();
     for (int it = 0; it < SORT_ITEMS; ++it) {
-        uint32_t i = base + it * SORT_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&h[wave][(uint32_t)(keys[i] >> shift) & (RADIX - 1)], 1u);
+        const bool valid = base + it * 64 < n;
+        const uint32_t d = (uint32_t)(k[it] >> shift) & (RADIX - 1);
+        const uint64_t m = digit_match(d, valid);
+        if (valid && (m & lt) == 0) h[wave][d] += (uint32_t)__popcll(m);
+        wave_sync_lds();
     }
     __syncthreads();
     for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) {
@@ -767,8 +792,6 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_upsweep(SortPass<K> P) {
     }
 }
 
-// Stable scatter: keys of a tile are ranked in (wave, item, lane) order,
-// which is their original order.
 // One block per digit: exclusive scan of that digit's per-tile counts (in
 // place) and the digit total. The digit totals are scanned by each
 // downsweep block, so a pass is three launches.
@@ -816,55 +839,65 @@ __device__ __forceinline__ void tile_load(const K* __restrict__ kin, const uint3
 }
 
 // Stable rank + scatter of one tile (k, v as tile_load reads them): keys are
-// ranked in (wave, item, lane) order, their original order, and key i of
-// digit d goes to gbase[d] + the count of d in earlier waves of the tile +
-// its rank in its wave. wc must be zero on entry; gbase must be complete (a
-// barrier follows the first use).
+// ranked in (wave, item, lane) order, their original order. The tile is first
+// put in digit order in LDS (key of digit d at the tile's start of d + the
+// count of d in earlier waves + its rank in its wave), then written out in
+// that order, so consecutive lanes write consecutive addresses of one digit's
+// run (scattered from registers, a store of 64 lanes touched up to 64 lines).
+// wc must be zero on entry; gbase must be complete (a barrier follows the
+// first use).
 template <class K>
 __device__ __forceinline__ void tile_rank_scatter(const K (&k)[SORT_ITEMS], const uint32_t (&v)[SORT_ITEMS],
                                                   K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
                                                   int shift, uint32_t tile, uint32_t (*wc)[RADIX],
-                                                  const uint32_t* gbase, K* __restrict__ samp = nullptr,
-                                                  int samp_shift = 0) {
-    uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    uint32_t base = tile * SORT_TILE + wave * (64 * SORT_ITEMS);
-    uint64_t lt = lanemask_lt();
+                                                  const uint32_t* gbase, uint32_t* lstart, K* lk, uint32_t* lv,
+                                                  K* __restrict__ samp = nullptr, int samp_shift = 0) {
+    static_assert(RADIX == SORT_THREADS, "one digit per thread");
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t base = tile * SORT_TILE + wave * (64 * SORT_ITEMS);
+    const uint64_t lt = lanemask_lt();
     uint32_t rank[SORT_ITEMS];
     for (int it = 0; it < SORT_ITEMS; ++it) {
-        uint32_t i = base + it * 64 + lane;
-        bool valid = i < n;
-        uint32_t d = (uint32_t)(k[it] >> shift) & (RADIX - 1);
-        uint64_t match = __ballot(valid);
-        for (int b = 0; b < 8; ++b) {
-            uint64_t bb = __ballot((d >> b) & 1u);
-            match &= ((d >> b) & 1u) ? bb : ~bb;
-        }
-        uint32_t peers = (uint32_t)__popcll(match & lt);
-        uint32_t cnt = (uint32_t)__popcll(match);
-        uint32_t cur = wc[wave][d];
+        const bool valid = base + it * 64 + lane < n;
+        const uint32_t d = (uint32_t)(k[it] >> shift) & (RADIX - 1);
+        const uint64_t match = digit_match(d, valid);
+        const uint32_t peers = (uint32_t)__popcll(match & lt);
+        const uint32_t cur = wc[wave][d];
         rank[it] = cur + peers;
         wave_sync_lds();
-        if (valid && peers == 0) wc[wave][d] = cur + cnt;
+        if (valid && peers == 0) wc[wave][d] = cur + (uint32_t)__popcll(match);
         wave_sync_lds();
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) {
+    {
+        // digit d (this thread): exclusive prefix over the waves, and the
+        // tile's start of d (exclusive prefix of the tile's digit counts)
+        const uint32_t d = threadIdx.x;
         uint32_t run = 0;
         for (int w = 0; w < SORT_THREADS / 64; ++w) {
-            uint32_t t = wc[w][d];
+            const uint32_t t = wc[w][d];
             wc[w][d] = run;
             run += t;
         }
+        lstart[d] = block_exclusive_sum(run, nullptr);
     }
     __syncthreads();
     for (int it = 0; it < SORT_ITEMS; ++it) {
-        uint32_t i = base + it * 64 + lane;
-        if (i >= n) continue;
-        uint32_t d = (uint32_t)(k[it] >> shift) & (RADIX - 1);
-        uint32_t pos = gbase[d] + wc[wave][d] + rank[it];
-        kout[pos] = k[it];
-        vout[pos] = v[it];
-        if (samp != nullptr && (pos & ((1u << samp_shift) - 1u)) == 0u) samp[pos >> samp_shift] = k[it];
+        if (base + it * 64 + lane >= n) continue;
+        const uint32_t d = (uint32_t)(k[it] >> shift) & (RADIX - 1);
+        const uint32_t lp = lstart[d] + wc[wave][d] + rank[it];
+        lk[lp] = k[it];
+        lv[lp] = v[it];
+    }
+    __syncthreads();
+    const uint32_t cnt = min(SORT_TILE, n - tile * SORT_TILE);
+    for (uint32_t j = threadIdx.x; j < cnt; j += SORT_THREADS) {
+        const K key = lk[j];
+        const uint32_t d = (uint32_t)(key >> shift) & (RADIX - 1);
+        const uint32_t pos = gbase[d] + (j - lstart[d]);
+        kout[pos] = key;
+        vout[pos] = lv[j];
+        if (samp != nullptr && (pos & ((1u << samp_shift) - 1u)) == 0u) samp[pos >> samp_shift] = key;
     }
 }
 
@@ -882,6 +915,9 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(SortPass<K> P) 
     const uint32_t tile = blk - (b ? P.split : 0u);
     __shared__ uint32_t wc[SORT_THREADS / 64][RADIX];
     __shared__ uint32_t gbase[RADIX];
+    __shared__ uint32_t lstart[RADIX];
+    __shared__ K lk[SORT_TILE];
+    __shared__ uint32_t lv[SORT_TILE];
     // the tile's keys first: their loads overlap those of the digit counts
     K k[SORT_ITEMS];
     uint32_t v[SORT_ITEMS];
@@ -894,7 +930,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(SortPass<K> P) 
         gbase[threadIdx.x] = ex + hist[(size_t)threadIdx.x * ntiles + tile];
     }
     __syncthreads();
-    tile_rank_scatter<K>(k, v, kout, vout, n, P.shift, tile, wc, gbase, b ? P.samp_b : P.samp_a,
+    tile_rank_scatter<K>(k, v, kout, vout, n, P.shift, tile, wc, gbase, lstart, lk, lv, b ? P.samp_b : P.samp_a,
                          b ? P.samp_shift_b : P.samp_shift_a);
 }
 
