@@ -17,28 +17,33 @@ inline int morton_axis_bits(uint32_t largest_mesh_triangles) {
     return largest_mesh_triangles <= (1u << 21) ? 8 : 10;
 }
 
-// Pipeline counters (device, uint32 each), cleared by launch_fill at the
-// start of every intersection.
-enum Counter {
-    CNT_TRAVERSAL_HITS = 0,  // FP32 box overlaps emitted by the traversal
-    CNT_CANDIDATES,          // exact (double) box overlaps = candidate pairs
-    CNT_FILTER_NO,
-    CNT_FILTER_GREEN,
-    CNT_FILTER_YELLOW,
-    CNT_EXACT_LIST,          // pairs handed to the exact stage
-    CNT_RESULTS,             // intersecting pairs written by the exact stage
-    CNT_SEGMENTS,            // segments written by the exact stage
-    CNT_EXACT_YELLOW_YES,    // YELLOW pairs the exact stage found intersecting
-    CNT_EXACT_YELLOW_NO,
-    CNT_COPLANAR,
-    CNT_DEGENERATE,
-    CNT_UNSOUND,             // filter said GREEN, exact said no (must stay 0)
-    CNT_DEFERRED,            // coplanar pairs moved from the fast exact pass to the full one
-    CNT_RESIDUAL,            // pairs left to the host residual stage (off-grid pairs that do not fit)
-    CNT_FILTER_PRED,         // orient3d signs evaluated by the FP64 filter
-    CNT_EXACT_PRED,          // orient3d and orient2d signs evaluated by the exact stage
-    CNT_OVERFLOW,            // a count wrapped past 2^32 (the query is too large)
-    CNT_COUNT
+// Pipeline counters (device, uint32 each), cleared at the start of every
+// intersection. Each sits alone in a 256-byte span: the appends of every
+// stage are atomics on these words, and atomics on one line serialise, so
+// counters sharing a line slowed each other's reservations (a filter launch
+// spent 12 us on them). The enum values are word offsets; CNT_COUNT is the
+// size of the block, and the two grid flags follow it.
+constexpr uint32_t kCntStride = 64;
+enum Counter : uint32_t {
+    CNT_TRAVERSAL_HITS = 0 * kCntStride,    // FP32 box overlaps emitted by the traversal
+    CNT_CANDIDATES = 1 * kCntStride,        // exact (double) box overlaps = candidate pairs
+    CNT_FILTER_NO = 2 * kCntStride,
+    CNT_FILTER_GREEN = 3 * kCntStride,
+    CNT_FILTER_YELLOW = 4 * kCntStride,
+    CNT_EXACT_LIST = 5 * kCntStride,        // pairs handed to the exact stage
+    CNT_RESULTS = 6 * kCntStride,           // intersecting pairs written by the exact stage
+    CNT_SEGMENTS = 7 * kCntStride,          // segments written by the exact stage
+    CNT_EXACT_YELLOW_YES = 8 * kCntStride,  // YELLOW pairs the exact stage found intersecting
+    CNT_EXACT_YELLOW_NO = 9 * kCntStride,
+    CNT_COPLANAR = 10 * kCntStride,
+    CNT_DEGENERATE = 11 * kCntStride,
+    CNT_UNSOUND = 12 * kCntStride,          // filter said GREEN, exact said no (must stay 0)
+    CNT_DEFERRED = 13 * kCntStride,         // coplanar pairs moved from the fast exact pass to the full one
+    CNT_RESIDUAL = 14 * kCntStride,         // pairs left to the host residual stage (off-grid pairs that do not fit)
+    CNT_FILTER_PRED = 15 * kCntStride,      // orient3d signs evaluated by the FP64 filter
+    CNT_EXACT_PRED = 16 * kCntStride,       // orient3d and orient2d signs evaluated by the exact stage
+    CNT_OVERFLOW = 17 * kCntStride,         // a count wrapped past 2^32 (the query is too large)
+    CNT_COUNT = 18 * kCntStride
 };
 
 // Exact segments on the device: 8 numbers (x0 y0 z0 w0 x1 y1 z1 w1) of

@@ -85,12 +85,12 @@ void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s) {
     GI_LAUNCH_CHECK();
 }
 
-// Zero the first nzero stage counters, and mark the grid flag of every slot
-// in `stage_mask` (a slot about to be prepared) as "all on the grid" until
-// one of its vertices proves otherwise.
+// Zero the counters in the first nzero words (one per kCntStride), and mark
+// the grid flag of every slot in `stage_mask` (a slot about to be prepared)
+// as "all on the grid" until one of its vertices proves otherwise.
 __global__ void k_query_init(uint32_t* counters, uint32_t nzero, uint32_t* grid_flags, uint32_t stage_mask) {
     const uint32_t t = threadIdx.x;
-    if (t < nzero) counters[t] = 0u;
+    if (t * kCntStride < nzero) counters[t * kCntStride] = 0u;
     if (t < 2 && ((stage_mask >> t) & 1u)) grid_flags[t] = 1u;
 }
 
@@ -1639,6 +1639,14 @@ __device__ __forceinline__ void shared_descent(const float4* __restrict__ nodes,
     }
 }
 
+// A walk handed from a busy lane to an idle one: the rest of a query's walk
+// after the subtree its lane is entering, with the query.
+struct WalkPiece {
+    uint32_t node, end, b, pad;
+    float q[6];
+};
+constexpr uint32_t kPiecesPerWave = 32;  // donors + idle lanes <= 64: at most 32 hand-overs per step
+
 // One thread per triangle of B, in B's Morton order (orderB, so a wave
 // queries a compact cluster; B needs no tree of its own), its box read from
 // tboxB; stackless traversal of A along the ropes. Overlapping leaf pairs
@@ -1646,6 +1654,16 @@ __device__ __forceinline__ void shared_descent(const float4* __restrict__ nodes,
 // global candidate queue with one atomic per STAGE pairs (the queue chunk).
 // Forest (batch): leaf k of B belongs to mesh keysB[k] >> mesh_shift and walks only
 // the subtree of the same mesh of A, from its root to the rope leaving it.
+//
+// Work sharing: a walk is a run of nodes in pre-order, [node, end), and it
+// splits at any node v it enters: [v, rope(v)) is v's subtree, [rope(v),
+// end) the rest. Most lanes of a wave end after the shared descent or a few
+// nodes while a few walk long, so at every step that finds idle lanes, each
+// lane entering the children of an overlapping internal node v (up to the
+// number of idle lanes) hands [rope(v), end) with its query to an idle lane
+// through LDS and keeps v's subtree. The pieces are the entries a stack-based
+// traversal would push, spread over the wave; every node of the walk is still
+// visited by exactly one lane, so the hits are the same.
 template <int TRAV_THREADS, uint32_t TRAV_STAGE>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restrict__ nodesA, uint32_t nA,
                                                            const uint32_t* __restrict__ orderB,
@@ -1655,7 +1673,9 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
                                                            uint2* __restrict__ cand, uint32_t cap,
                                                            uint32_t* __restrict__ counters) {
     __shared__ uint2 stage_all[TRAV_THREADS / 64][TRAV_STAGE];
+    __shared__ WalkPiece pieces_all[TRAV_THREADS / 64][kPiecesPerWave];
     uint2* stage = stage_all[threadIdx.x >> 6];
+    WalkPiece* pieces = pieces_all[threadIdx.x >> 6];
     uint32_t k = xcd_block(blockIdx.x, gridDim.x) * TRAV_THREADS + threadIdx.x;
     float qlx = 0.f, qly = 0.f, qlz = 0.f, qhx = 0.f, qhy = 0.f, qhz = 0.f;
     uint32_t b = 0;
@@ -1673,31 +1693,66 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
         }
     }
     const uint32_t leafBase = nA - 1;
-    uint64_t lt = lanemask_lt();
+    const uint64_t lt = lanemask_lt();
     uint32_t fill = 0;
     if (node == end) node = end = kSentinel;
     shared_descent(nodesA, leafBase, qlx, qly, qlz, qhx, qhy, qhz, node, end);
     // a walk ends at its end rope; the rope chain itself ends in kSentinel
-    while (__any(node != end && node != kSentinel)) {
-        bool hit = false;
-        uint32_t a = 0;
-        if (node != end && node != kSentinel) {
+    for (;;) {
+        const bool active = node != end && node != kSentinel;
+        const uint64_t act = __ballot(active);
+        if (act == 0) break;
+        bool hit = false, give = false;
+        uint32_t a = 0, rest = 0;
+        if (active) {
             float4 L = nodesA[2 * (size_t)node], H = nodesA[2 * (size_t)node + 1];
             // all six compares unconditionally (&, not &&): the node's two
             // float4 loads then stay two 16-byte loads
             const bool ov = (L.x <= qhx) & (qlx <= H.x) & (L.y <= qhy) & (qly <= H.y) & (L.z <= qhz) & (qlz <= H.z);
-            uint32_t w0 = __float_as_uint(L.w), w1 = __float_as_uint(H.w);
+            const uint32_t w0 = __float_as_uint(L.w), w1 = __float_as_uint(H.w);
+            const bool down = ov && node < leafBase;
             if (ov && node >= leafBase) { hit = true; a = w0; }
-            node = (ov && node < leafBase) ? w0 : w1;
+            node = down ? w0 : w1;
+            give = down && w1 != end && w1 != kSentinel;  // a rest to hand over: [rope, end)
+            rest = w1;
         }
-        uint64_t m = __ballot(hit);
-        if (m) {
-            uint32_t cnt = (uint32_t)__popcll(m);
+        const uint64_t donors = __ballot(give), idle = ~act;
+        if (donors != 0 && idle != 0) {
+            const uint32_t m = min((uint32_t)__popcll(donors), (uint32_t)__popcll(idle));
+            if (give) {
+                const uint32_t slot = (uint32_t)__popcll(donors & lt);
+                if (slot < m) {
+                    WalkPiece& p = pieces[slot];
+                    p.node = rest;
+                    p.end = end;
+                    p.b = b;
+                    p.q[0] = qlx; p.q[1] = qly; p.q[2] = qlz;
+                    p.q[3] = qhx; p.q[4] = qhy; p.q[5] = qhz;
+                    end = rest;  // this lane keeps the subtree it is entering
+                }
+            }
+            wave_sync_lds();
+            if (!active) {
+                const uint32_t slot = (uint32_t)__popcll(idle & lt);
+                if (slot < m) {
+                    const WalkPiece& p = pieces[slot];
+                    node = p.node;
+                    end = p.end;
+                    b = p.b;
+                    qlx = p.q[0]; qly = p.q[1]; qlz = p.q[2];
+                    qhx = p.q[3]; qhy = p.q[4]; qhz = p.q[5];
+                }
+            }
+            wave_sync_lds();
+        }
+        uint64_t mh = __ballot(hit);
+        if (mh) {
+            uint32_t cnt = (uint32_t)__popcll(mh);
             if (fill + cnt > TRAV_STAGE) {
                 flush_stage(stage, fill, cand, cap, counters);
                 fill = 0;
             }
-            if (hit) stage[fill + (uint32_t)__popcll(m & lt)] = make_uint2(a, b);
+            if (hit) stage[fill + (uint32_t)__popcll(mh & lt)] = make_uint2(a, b);  // a hit lane was active: its query is unchanged
             fill += cnt;
         }
     }
