@@ -8,6 +8,12 @@
 namespace gi {
 
 constexpr uint32_t kSentinel = 0xFFFFFFFFu;
+// Bit 31 of an internal node's left-child word: its right child is a leaf.
+// With it the right child's index follows from the left's (Karras: children
+// gamma and gamma + 1), so both can be loaded at once. Node indices stay
+// below 2^31 (meshes of fewer than 2^30 triangles).
+constexpr uint32_t kRightLeaf = 0x80000000u;
+constexpr uint32_t kNodeIndexMask = 0x7FFFFFFFu;
 
 // Morton precision of the BVH keys, bits per axis. Only the tree's quality
 // depends on it (equal keys are ordered by triangle index); 8 bits per axis

@@ -1288,7 +1288,8 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
         if (L < R && ((R >> k) & 1)) { absorb(k, (R >> k) - 1); R -= 1 << k; }
     }
     for (int m = L >> 6; m < (R >> 6); ++m) absorb(6, m);
-    nodes[2 * (size_t)jl] = make_float4(q[0], q[1], q[2], __uint_as_float(left));
+    const uint32_t lw = left | (right >= (uint32_t)(n - 1) ? kRightLeaf : 0u);
+    nodes[2 * (size_t)jl] = make_float4(q[0], q[1], q[2], __uint_as_float(lw));
     nodes[2 * (size_t)jl + 1] = make_float4(q[3], q[4], q[5], __uint_as_float(rope));
 }
 
@@ -1604,9 +1605,12 @@ __device__ __forceinline__ void shared_descent(const float4* __restrict__ nodes,
     uint32_t fl = 0, fr = 0;
     float4 flL, flH, frL, frH;
     while (!none && v < leafBase) {
-        const uint32_t left = __builtin_amdgcn_readfirstlane(__float_as_uint(L.w));
+        // children gamma and gamma + 1 (internal or leaf), both loaded at once
+        const uint32_t lw = __builtin_amdgcn_readfirstlane(__float_as_uint(L.w));
+        const uint32_t left = lw & kNodeIndexMask;
+        const uint32_t gamma = left < leafBase ? left : left - leafBase;
+        const uint32_t right = (lw & kRightLeaf) ? leafBase + gamma + 1u : gamma + 1u;
         const float4 lL = nodes[2 * (size_t)left], lH = nodes[2 * (size_t)left + 1];
-        const uint32_t right = __builtin_amdgcn_readfirstlane(__float_as_uint(lH.w));  // left's rope
         const float4 rL = nodes[2 * (size_t)right], rH = nodes[2 * (size_t)right + 1];
         const bool ol = box_overlap(c, lL, lH), orr = box_overlap(c, rL, rH);
         if (ol && orr) {
@@ -1709,7 +1713,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(const float4* __restr
             // all six compares unconditionally (&, not &&): the node's two
             // float4 loads then stay two 16-byte loads
             const bool ov = (L.x <= qhx) & (qlx <= H.x) & (L.y <= qhy) & (qly <= H.y) & (L.z <= qhz) & (qlz <= H.z);
-            const uint32_t w0 = __float_as_uint(L.w), w1 = __float_as_uint(H.w);
+            const uint32_t w0 = __float_as_uint(L.w) & kNodeIndexMask, w1 = __float_as_uint(H.w);
             const bool down = ov && node < leafBase;
             if (ov && node >= leafBase) { hit = true; a = w0; }
             node = down ? w0 : w1;
