@@ -196,6 +196,14 @@ struct gi_context {
         graph = nullptr;
         graph_sig.clear();
     }
+    // pinned staging of large batch uploads (allocated on the first one,
+    // kept until destroy): per worker two buffers, a stream and two events
+    struct Stager {
+        std::vector<void*> buf;
+        std::vector<hipStream_t> st;
+        std::vector<hipEvent_t> ev;
+        int workers = 0;
+    } stager;
     hipEvent_t ev_chunk[4] = {};  // [0..1] bracket the last k_build_chunks launch
     bool chunk_timed = false;     // that launch ran in the current call
     hipEvent_t ev_stage[2] = {};  // bracket the last staging launch of single meshes
@@ -1272,6 +1280,10 @@ int gi_destroy(gi_context* ctx) {
     for (auto& e : ctx->ev_stage)
         if (e) (void)hipEventDestroy(e);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    for (hipStream_t st : ctx->stager.st) (void)hipStreamSynchronize(st);
+    for (void* p : ctx->stager.buf) (void)hipHostFree(p);
+    for (hipEvent_t e : ctx->stager.ev) (void)hipEventDestroy(e);
+    for (hipStream_t st : ctx->stager.st) (void)hipStreamDestroy(st);
     if (ctx->h_xf) (void)hipHostFree(ctx->h_xf);
     if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
@@ -1554,6 +1566,88 @@ void fill_forest(MeshSlot& m, const gi_mesh_pair* pairs, uint32_t n, bool side_b
     m.xf = identity_xform();
 }
 
+// Host arrays copied back to back into one device range through pinned
+// memory. Each of up to kStageWorkers host threads takes a contiguous share
+// of the range and moves it in kStageChunk pieces: its own two pinned
+// buffers take turns, so that its memcpy into one overlaps the DMA out of
+// the other (on the worker's own stream), and the workers' DMAs overlap each
+// other. Copies from pageable memory (hipMemcpyAsync on the caller's array)
+// take the runtime's own staging path one copy at a time, which left a
+// 6.6 GB batch upload at 20 GB/s. Returns once everything is on the device.
+struct HostPiece {
+    const void* p;
+    size_t bytes;
+};
+constexpr int kStageWorkers = 8;
+constexpr size_t kStageChunk = 16u << 20;
+
+void ensure_stager(gi_context* ctx) {
+    auto& S = ctx->stager;
+    if (S.workers) return;
+    const int hw = ctx->host_threads > 0 ? ctx->host_threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    const int w = std::max(1, std::min(kStageWorkers, hw));
+    for (int i = 0; i < w; ++i) {
+        hipStream_t st;
+        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        S.st.push_back(st);
+        for (int k = 0; k < 2; ++k) {
+            void* p = nullptr;
+            HIP_TRY(hipHostMalloc(&p, kStageChunk, hipHostMallocDefault));
+            S.buf.push_back(p);
+            hipEvent_t e;
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            S.ev.push_back(e);
+        }
+    }
+    S.workers = w;
+}
+
+void upload_gather(gi_context* ctx, char* dst, const std::vector<HostPiece>& pieces) {
+    std::vector<size_t> off(pieces.size() + 1, 0);
+    for (size_t i = 0; i < pieces.size(); ++i) off[i + 1] = off[i] + pieces[i].bytes;
+    const size_t total = off.back();
+    if (total == 0) return;
+    ensure_stager(ctx);
+    auto& S = ctx->stager;
+    const int W = (int)std::min<size_t>((size_t)S.workers, (total + kStageChunk - 1) / kStageChunk);
+    std::vector<int> status(W, GI_OK);
+    auto work = [&](int w) {
+        try {
+            HIP_TRY(hipSetDevice(ctx->device));
+            const size_t lo = total * (size_t)w / W, hi = total * (size_t)(w + 1) / W;
+            size_t pi = (size_t)(std::upper_bound(off.begin(), off.end(), lo) - off.begin()) - 1;
+            int turn = 0;
+            for (size_t a = lo; a < hi; a += kStageChunk, turn ^= 1) {
+                const size_t len = std::min(kStageChunk, hi - a);
+                char* buf = (char*)S.buf[2 * w + turn];
+                hipEvent_t ev = S.ev[2 * w + turn];
+                HIP_TRY(hipEventSynchronize(ev));  // this buffer's previous DMA is done
+                for (size_t got = 0; got < len;) {
+                    while (off[pi + 1] <= a + got) ++pi;
+                    const size_t in = a + got - off[pi];
+                    const size_t take = std::min(len - got, pieces[pi].bytes - in);
+                    memcpy(buf + got, (const char*)pieces[pi].p + in, take);
+                    got += take;
+                }
+                HIP_TRY(hipMemcpyAsync(dst + a, buf, len, hipMemcpyHostToDevice, S.st[w]));
+                HIP_TRY(hipEventRecord(ev, S.st[w]));
+            }
+            HIP_TRY(hipStreamSynchronize(S.st[w]));
+        } catch (const GiError& e) {
+            status[w] = e.status;
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < W; ++w) th.emplace_back(work, w);
+    work(0);
+    for (auto& t : th) t.join();
+    for (int w = 0; w < W; ++w)
+        if (status[w] != GI_OK) throw GiError{status[w], "staged upload failed"};
+}
+
+// Batches above this many bytes per array go through upload_gather.
+constexpr size_t kStagedUploadMin = 16u << 20;
+
 // The batch's arrays straight from the caller's, each mesh to its place in
 // the slot's device arrays (indices still per mesh), the tile tables, then
 // the device check that makes the indices global.
@@ -1562,14 +1656,26 @@ void upload_forest(gi_context* ctx, int slot, const gi_mesh_pair* pairs, uint32_
     hipStream_t st = ctx->stream;
     double* src = m.src.ensure<double>(3 * (size_t)m.nv);
     uint32_t* tri = m.tri.ensure<uint32_t>(3 * (size_t)m.nt);
-    for (uint32_t i = 0; i < n; ++i) {
-        const double* xyz = side_b ? pairs[i].xyzB : pairs[i].xyzA;
-        const uint32_t* t = side_b ? pairs[i].triB : pairs[i].triA;
-        const uint32_t mv = side_b ? pairs[i].nvB : pairs[i].nvA, mt = side_b ? pairs[i].ntB : pairs[i].ntA;
-        if (mv)
-            HIP_TRY(hipMemcpyAsync(src + 3 * (size_t)m.h_voff[i], xyz, 24 * (size_t)mv, hipMemcpyHostToDevice, st));
-        if (mt)
-            HIP_TRY(hipMemcpyAsync(tri + 3 * (size_t)m.h_toff[i], t, 12 * (size_t)mt, hipMemcpyHostToDevice, st));
+    if (24 * (size_t)m.nv + 12 * (size_t)m.nt >= kStagedUploadMin) {
+        // the device arrays hold the meshes back to back in pair order
+        std::vector<HostPiece> xyz, idx;
+        for (uint32_t i = 0; i < n; ++i) {
+            xyz.push_back({side_b ? pairs[i].xyzB : pairs[i].xyzA, 24 * (size_t)(side_b ? pairs[i].nvB : pairs[i].nvA)});
+            idx.push_back({side_b ? pairs[i].triB : pairs[i].triA, 12 * (size_t)(side_b ? pairs[i].ntB : pairs[i].ntA)});
+        }
+        HIP_TRY(hipStreamSynchronize(st));  // nothing queued may still read the arrays being replaced
+        upload_gather(ctx, (char*)src, xyz);
+        upload_gather(ctx, (char*)tri, idx);
+    } else {
+        for (uint32_t i = 0; i < n; ++i) {
+            const double* xyz = side_b ? pairs[i].xyzB : pairs[i].xyzA;
+            const uint32_t* t = side_b ? pairs[i].triB : pairs[i].triA;
+            const uint32_t mv = side_b ? pairs[i].nvB : pairs[i].nvA, mt = side_b ? pairs[i].ntB : pairs[i].ntA;
+            if (mv)
+                HIP_TRY(hipMemcpyAsync(src + 3 * (size_t)m.h_voff[i], xyz, 24 * (size_t)mv, hipMemcpyHostToDevice, st));
+            if (mt)
+                HIP_TRY(hipMemcpyAsync(tri + 3 * (size_t)m.h_toff[i], t, 12 * (size_t)mt, hipMemcpyHostToDevice, st));
+        }
     }
     auto put = [&](DevBuf& d, const std::vector<uint32_t>& h) {
         uint32_t* p = d.ensure<uint32_t>(h.size());
