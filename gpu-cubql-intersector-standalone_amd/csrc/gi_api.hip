@@ -173,6 +173,7 @@ struct gi_context {
     // vertex kernels) and the counter read-back; graph replays read/write them
     Xform* h_xf = nullptr;
     uint32_t* h_counters = nullptr;
+    uint32_t* d_counters_host = nullptr;  // h_counters as kernels address it
     DevBuf xf_dev;
     // what an eager stage last copied into xf_dev[slot]: an unchanged
     // transform is not sent again (captures and replays clear the flag)
@@ -677,10 +678,10 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
                         ctx->pairs.get<uint2>(),
                         ctx->order.get<uint32_t>(), ctx->offs.get<uint32_t>(), ctx->segs.get<uint64_t>(),
                         ctx->exp2.get<int32_t>(), ctx->f64.get<double>(), (uint32_t)output_seg_slots(ctx),
-                        ctx->seg_pool.get<uint64_t>(), ctx->seg_exp.get<int32_t>(), grid, s);
+                        ctx->seg_pool.get<uint64_t>(), ctx->seg_exp.get<int32_t>(), grid, ctx->d_counters_host,
+                        s);  // the counters reach ctx->h_counters from the output launches
     if (timed) record(ctx, ctx->ev[5], s);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(ctx->h_counters, counters, sizeof(uint32_t) * (CNT_COUNT + 2), hipMemcpyDeviceToHost, s));
 }
 
 void ensure_output_bufs(gi_context* ctx, const gi_options& o) {
@@ -1253,6 +1254,7 @@ int gi_create(const gi_config* cfg, gi_context** out) {
         HIP_TRY(hipHostMalloc((void**)&ctx->h_xf, 2 * sizeof(Xform), hipHostMallocDefault));
         HIP_TRY(hipHostMalloc((void**)&ctx->h_counters, sizeof(uint32_t) * (CNT_COUNT + 2), hipHostMallocDefault));
         memset(ctx->h_counters, 0, sizeof(uint32_t) * (CNT_COUNT + 2));
+        HIP_TRY(hipHostGetDevicePointer((void**)&ctx->d_counters_host, ctx->h_counters, 0));
         HIP_TRY(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
         for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
         for (auto& e : ctx->ev_chunk) HIP_TRY(hipEventCreate(&e));

@@ -248,12 +248,15 @@ size_t output_small_runs_elems();
 bool output_small_fits(uint32_t R, int key_bits);
 // Segments are gathered at the width the exact stage wrote (grid flags on
 // the device: 2 limbs, else 4 with scales copied to seg_exp_out); seg_out
-// holds out_slots segments of kSegPoolWords words.
+// holds out_slots segments of kSegPoolWords words. counters_out (host pinned
+// memory as the device addresses it, or null) receives the counter words
+// (counters[k * kCntStride]) and the two grid flags (counters[CNT_COUNT],
+// [CNT_COUNT + 1]) at the same offsets, in place of a copy after the launches.
 void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const uint32_t* counters, uint32_t cap_rec,
                          uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint64_t* runs, uint2* pairs,
                          uint32_t* order, uint32_t* offs, uint64_t* seg_out, int32_t* seg_exp_out, double* seg_f64,
                          uint32_t out_slots, const uint64_t* seg_pool, const int32_t* seg_exp,
-                         const uint32_t* grid_flags, hipStream_t s);
+                         const uint32_t* grid_flags, uint32_t* counters_out, hipStream_t s);
 void launch_gather_pairs(const uint64_t* sorted_key, const uint32_t* order, const uint2* rec_seg, uint32_t n,
                          int bitsB, uint2* pairs, uint32_t* nseg, hipStream_t s);
 void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const uint32_t* seg_offsets, uint32_t n,

@@ -2626,7 +2626,16 @@ __global__ __launch_bounds__(PLACE_THREADS) void k_output_place(const uint64_t* 
                                                                 uint32_t cap_seg, int key_bits, int bitsB,
                                                                 uint2* __restrict__ pairs,
                                                                 uint32_t* __restrict__ order,
-                                                                uint32_t* __restrict__ offs) {
+                                                                uint32_t* __restrict__ offs,
+                                                                uint32_t* __restrict__ counters_out) {
+    // the counters are final here (every stage that counts has run): the
+    // first workgroup copies the counter words (one per kCntStride) and the
+    // two grid flags to the host's pinned read-back buffer
+    if (blockIdx.x == 0 && counters_out && threadIdx.x < CNT_COUNT / kCntStride + 2) {
+        const uint32_t i = threadIdx.x < CNT_COUNT / kCntStride ? threadIdx.x * kCntStride
+                                                                 : CNT_COUNT + threadIdx.x - CNT_COUNT / kCntStride;
+        counters_out[i] = counters[i];
+    }
     uint32_t R;
     if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R)) return;
     const uint32_t g = blockIdx.x * PLACE_THREADS + threadIdx.x;  // this thread's record, in run order
@@ -2693,12 +2702,13 @@ void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const ui
                          uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint64_t* runs, uint2* pairs,
                          uint32_t* order, uint32_t* offs, uint64_t* seg_out, int32_t* seg_exp_out, double* seg_f64,
                          uint32_t out_slots, const uint64_t* seg_pool, const int32_t* seg_exp,
-                         const uint32_t* grid_flags, hipStream_t s) {
+                         const uint32_t* grid_flags, uint32_t* counters_out, hipStream_t s) {
     uint32_t* cpre = (uint32_t*)(runs + kOutSmallMax);
     k_output_runs<<<kRuns, RUN_THREADS, 0, s>>>(rec_key, rec_seg, counters, cap_rec, cap_seg, key_bits,
                                                 want_segments ? 1 : 0, runs, cpre);
     k_output_place<<<kOutSmallMax / PLACE_THREADS, PLACE_THREADS, 0, s>>>(runs, cpre, counters, cap_rec, cap_seg,
-                                                                        key_bits, bitsB, pairs, order, offs);
+                                                                        key_bits, bitsB, pairs, order, offs,
+                                                                        counters_out);
     if (want_segments)
         k_gather_segments_small<<<kOutSmallMax / 256, 256, 0, s>>>(order, rec_seg, offs, counters, cap_rec, cap_seg,
                                                                    key_bits, out_slots, seg_pool, seg_exp, grid_flags,

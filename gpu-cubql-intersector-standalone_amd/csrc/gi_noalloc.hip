@@ -273,7 +273,7 @@ extern "C" int gi_intersect_noalloc(int32_t device, const double* xyzA, uint32_t
         launch_exact(A, B, exact, cap_exact, grid, bitsB, rec_key, rec_seg, cap_rec, seg_pool, seg_exp, cap_seg, cand,
                      cap_cand, resid, cap_exact, counters, 1024, kinds, s);
         launch_output_small(rec_key, rec_seg, counters, cap_rec, cap_seg, bitsA + bitsB, bitsB, want_segments != 0,
-                            runs, out_pairs, order, offs, segs, sexp, f64, out_slots, seg_pool, seg_exp, grid, s);
+                            runs, out_pairs, order, offs, segs, sexp, f64, out_slots, seg_pool, seg_exp, grid, nullptr, s);
         NA_TRY(hipGetLastError());
         NA_TRY(hipMemcpy(hc, counters, sizeof(hc), hipMemcpyDeviceToHost));
         const bool is_grid = hc[CNT_COUNT] && hc[CNT_COUNT + 1];
