@@ -49,7 +49,8 @@ enum Counter : uint32_t {
     CNT_FILTER_PRED = 15 * kCntStride,      // orient3d signs evaluated by the FP64 filter
     CNT_EXACT_PRED = 16 * kCntStride,       // orient3d and orient2d signs evaluated by the exact stage
     CNT_OVERFLOW = 17 * kCntStride,         // a count wrapped past 2^32 (the query is too large)
-    CNT_COUNT = 18 * kCntStride
+    CNT_SLOW = 18 * kCntStride,             // grid pairs the one-lane exact pass leaves to the 8-lane one
+    CNT_COUNT = 19 * kCntStride
 };
 
 // Exact segments on the device: 8 numbers (x0 y0 z0 w0 x1 y1 z1 w1) of

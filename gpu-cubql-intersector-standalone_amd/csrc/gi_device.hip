@@ -2031,6 +2031,164 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
     tally.flush(counters);
 }
 
+// ----------------------------------------------- grid pairs, one lane each
+//
+// The common case on grid meshes: each triangle strictly crosses the other's
+// plane (no vertex on the other plane, so neither triangle is degenerate and
+// they are not coplanar). Then T_A ^ T_B is the overlap, on the line L =
+// plane_A ^ plane_B, of T_A ^ L and T_B ^ L, each the segment between the
+// two points where a triangle's edges cross the other plane (the two edges
+// at the vertex whose side differs from the other two's). The four crossing
+// points are exact homogeneous int128 points. On an axis k where L's
+// direction nA x nB is non-zero, points of L are ordered by their k
+// coordinate alone: the overlap is [max of the two lower ends, min of the two
+// upper ends], empty when they cross. Each comparison of h_k / w values
+// first runs in FP64 with a certified bound and falls back to the exact
+// cross-multiplied one (256-bit products) when the bound does not decide.
+// One lane per pair: the 8-lane kernel below computes the shared part of a
+// pair in all of its lanes. Pairs with a zero orientation sign go to the
+// slow list, which that kernel then takes.
+
+// int128 to double, |error| <= 4 eps |x| (eps = 2^-53): the magnitude's two
+// 64-bit halves each rounded once, then their sum
+__device__ __forceinline__ double i128_to_f64(i128 x) {
+    const u128 u = uabs128(x);
+    const double d = (double)(uint64_t)(u >> 64) * 18446744073709551616.0 + (double)(uint64_t)u;
+    return x < 0 ? -d : d;
+}
+
+// sign of xa - xb for xa, xb within 9 eps (relative) of the exact quotients
+// they stand for; 0 when that error could flip it (the bound is 32 eps of
+// the magnitudes)
+__device__ __forceinline__ int cmp_f64_certified(double xa, double xb) {
+    const double d = xa - xb;
+    const double bound = 3.552713678800501e-15 * (fabs(xa) + fabs(xb));  // 2^-48
+    return d > bound ? 1 : (d < -bound ? -1 : 0);
+}
+
+// The two points where triangle (T0, T1, T2) crosses the other plane
+// (orientations o0..o2 against it, signs s0..s2; exactly one vertex, the
+// apex, on its own side). Vertices and values come as separate arguments and
+// are picked by selects: a select between elements of one array would be
+// folded into an indexed access of a per-lane array in scratch memory.
+__device__ __forceinline__ void tri_crossings(GridExact::V3 T0, GridExact::V3 T1, GridExact::V3 T2, i128 o0, i128 o1,
+                                              i128 o2, int s0, int s1, int s2, GridExact::HP& p,
+                                              GridExact::HP& q) {
+    const int ap = s0 == s1 ? 2 : (s0 == s2 ? 1 : 0);
+    GridExact::V3 U, V, W;  // the apex, then the other two in cyclic order
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int64_t t0 = T0.v[c], t1 = T1.v[c], t2 = T2.v[c];
+        U.v[c] = ap == 0 ? t0 : (ap == 1 ? t1 : t2);
+        V.v[c] = ap == 0 ? t1 : (ap == 1 ? t2 : t0);
+        W.v[c] = ap == 0 ? t2 : (ap == 1 ? t0 : t1);
+    }
+    const i128 ou = ap == 0 ? o0 : (ap == 1 ? o1 : o2);
+    const i128 ov = ap == 0 ? o1 : (ap == 1 ? o2 : o0);
+    const i128 ow = ap == 0 ? o2 : (ap == 1 ? o0 : o1);
+    p = GridExact::hpCrossing(U, V, ou, ov);
+    q = GridExact::hpCrossing(U, W, ou, ow);
+}
+
+__global__ __launch_bounds__(256) void k_exact_grid_fast(MeshDev A, MeshDev B, const uint2* __restrict__ list,
+                                                         uint32_t cap, const uint32_t* __restrict__ grid_flags,
+                                                         int bitsB, uint64_t* __restrict__ rec_key,
+                                                         uint2* __restrict__ rec_seg, uint32_t rec_cap,
+                                                         uint64_t* __restrict__ seg_pool_words, uint32_t seg_cap,
+                                                         uint2* __restrict__ slow, uint32_t slow_cap,
+                                                         uint32_t* __restrict__ counters) {
+    typedef GridExact::V3 V3;
+    typedef GridExact::HP HP;
+    if (!(grid_flags[0] && grid_flags[1])) return;  // general doubles: k_exact_wide
+    __int128* seg_pool = (__int128*)seg_pool_words;
+    uint32_t n = counters[CNT_EXACT_LIST];
+    if (n > cap) n = cap;
+    ExactTally tally;
+    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const uint32_t i = base + threadIdx.x;
+        bool to_slow = false, hit = false;
+        uint2 en = make_uint2(0, 0);
+        uint32_t a = 0, b = 0;
+        HP lo, hi;
+        if (i < n) {
+            en = list[i];
+            a = en.x;
+            b = en.y & kIndexMask;
+            const uint32_t status = en.y >> kStatusShift;
+            const uint32_t* ta = A.tri + 3 * (size_t)a;
+            const uint32_t* tb = B.tri + 3 * (size_t)b;
+            V3 PA[3], PB[3];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const int4 p = A.vi[ta[v]], q = B.vi[tb[v]];
+                PA[v].v[0] = p.x; PA[v].v[1] = p.y; PA[v].v[2] = p.z;
+                PB[v].v[0] = q.x; PB[v].v[1] = q.y; PB[v].v[2] = q.z;
+            }
+            const V3 nA = GridExact::cross(GridExact::sub(PA[1], PA[0]), GridExact::sub(PA[2], PA[0]));
+            const V3 nB = GridExact::cross(GridExact::sub(PB[1], PB[0]), GridExact::sub(PB[2], PB[0]));
+            i128 oA[3], oB[3];
+            int sA[3], sB[3];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                oB[v] = GridExact::dotB(nA, GridExact::sub(PB[v], PA[0]));  // B's vertices against plane A
+                sB[v] = sgn128(oB[v]);
+                oA[v] = GridExact::dotB(nB, GridExact::sub(PA[v], PB[0]));  // A's against plane B
+                sA[v] = sgn128(oA[v]);
+            }
+            tally.pred += 6;
+            // a zero sign covers degenerate triangles (zero normal) and coplanar pairs too
+            to_slow = sA[0] == 0 || sA[1] == 0 || sA[2] == 0 || sB[0] == 0 || sB[1] == 0 || sB[2] == 0;
+            const bool apart = (sA[0] == sA[1] && sA[1] == sA[2]) || (sB[0] == sB[1] && sB[1] == sB[2]);
+            if (!to_slow && !apart) {
+                HP pa0, pa1, pb0, pb1;
+                tri_crossings(PA[0], PA[1], PA[2], oA[0], oA[1], oA[2], sA[0], sA[1], sA[2], pa0, pa1);
+                tri_crossings(PB[0], PB[1], PB[2], oB[0], oB[1], oB[2], sB[0], sB[1], sB[2], pb0, pb1);
+                // axis of the largest component of L's direction
+                const i128 d0 = (i128)nA.v[1] * nB.v[2] - (i128)nA.v[2] * nB.v[1];
+                const i128 d1 = (i128)nA.v[2] * nB.v[0] - (i128)nA.v[0] * nB.v[2];
+                const i128 d2 = (i128)nA.v[0] * nB.v[1] - (i128)nA.v[1] * nB.v[0];
+                int k = GridPolicy::cmpAbsB(d1, d0) > 0 ? 1 : 0;
+                if (GridPolicy::cmpAbsB(d2, k ? d1 : d0) > 0) k = 2;
+                auto xk = [&](const HP& P) { return i128_to_f64(GridExact::hcomp(P, k)) / i128_to_f64(P.w); };
+                double xa0 = xk(pa0), xa1 = xk(pa1), xb0 = xk(pb0), xb1 = xk(pb1);
+                auto cmp = [&](const HP& P, double xp, const HP& Q, double xq) {
+                    int c = cmp_f64_certified(xp, xq);
+                    if (c == 0) c = GridExact::hpCmpAxis(P, Q, k);
+                    return c;
+                };
+                tally.pred += 5;
+                if (cmp(pa0, xa0, pa1, xa1) > 0) {  // each triangle's pair in order along k
+                    const HP t = pa0; pa0 = pa1; pa1 = t;
+                    const double x = xa0; xa0 = xa1; xa1 = x;
+                }
+                if (cmp(pb0, xb0, pb1, xb1) > 0) {
+                    const HP t = pb0; pb0 = pb1; pb1 = t;
+                    const double x = xb0; xb0 = xb1; xb1 = x;
+                }
+                const bool la = cmp(pa0, xa0, pb0, xb0) >= 0, ua = cmp(pa1, xa1, pb1, xb1) <= 0;
+                lo = la ? pa0 : pb0;
+                hi = ua ? pa1 : pb1;
+                hit = cmp(lo, la ? xa0 : xb0, hi, ua ? xa1 : xb1) <= 0;
+            }
+            if (!to_slow) tally.add((int)status, hit, 0, 0);
+        }
+        const uint32_t want[3] = {to_slow ? 1u : 0u, hit ? 1u : 0u, hit ? 1u : 0u};
+        uint32_t got[3];
+        block_reserve3(want, counters, CNT_SLOW, CNT_RESULTS, CNT_SEGMENTS, got);
+        if (to_slow && got[0] < slow_cap) slow[got[0]] = en;
+        if (hit && got[1] < rec_cap) {
+            rec_key[got[1]] = ((uint64_t)a << bitsB) | (uint64_t)b;
+            rec_seg[got[1]] = make_uint2(got[2], 1u);
+            if (got[2] < seg_cap) {
+                __int128* o = seg_pool + 8 * (size_t)got[2];
+                o[0] = lo.h[0]; o[1] = lo.h[1]; o[2] = lo.h[2]; o[3] = lo.w;
+                o[4] = hi.h[0]; o[5] = hi.h[1]; o[6] = hi.h[2]; o[7] = hi.w;
+            }
+        }
+    }
+    tally.flush(counters);
+}
+
 // ----------------------------------------------- grid pairs, 8 lanes each
 //
 // The transversal pairs of grid meshes (the common case: neither triangle
@@ -2067,13 +2225,13 @@ __global__ __launch_bounds__(256) void k_exact_grid_lanes(MeshDev A, MeshDev B, 
                                                           uint2* __restrict__ rec_seg, uint32_t rec_cap,
                                                           uint64_t* __restrict__ seg_pool_words, uint32_t seg_cap,
                                                           uint2* __restrict__ defer, uint32_t defer_cap,
-                                                          uint32_t* __restrict__ counters) {
+                                                          uint32_t* __restrict__ counters, int list_counter) {
     typedef GridExact::V3 V3;
     typedef GridExact::HP HP;
     typedef GridExact::B Bn;
     if (!(grid_flags[0] && grid_flags[1])) return;  // general doubles: k_exact_wide
     __int128* seg_pool = (__int128*)seg_pool_words;
-    uint32_t n = counters[CNT_EXACT_LIST];
+    uint32_t n = counters[list_counter];
     if (n > cap) n = cap;
     constexpr uint32_t kPerBlock = 256 / kPairLanes;
     const int e = (int)(lane_id() & (kPairLanes - 1));
@@ -2338,8 +2496,13 @@ void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_
     // grid flags (decided on the device by the vertex stage) do not select
     // returns at once
     if (kinds & kExactGrid) {
-        k_exact_grid_lanes<<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
-                                                  rec_cap, seg_pool, seg_cap, defer, defer_cap, counters);
+        // strictly transversal pairs one lane each; the rest (a zero
+        // orientation sign) through the 8-lane kernel, on the residual list's
+        // room (grid meshes leave nothing to the host)
+        k_exact_grid_fast<<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
+                                                 rec_cap, seg_pool, seg_cap, resid, resid_cap, counters);
+        k_exact_grid_lanes<<<blocks, 256, 0, s>>>(A, B, resid, resid_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
+                                                  seg_pool, seg_cap, defer, defer_cap, counters, (int)CNT_SLOW);
         // the deferred (coplanar) pairs: their count is read on the device
         // usually empty: a quarter of the blocks (each pair takes one lane here)
         k_exact<false><<<std::max(blocks / 4, 64), 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key,
