@@ -168,7 +168,7 @@ struct gi_context {
     DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2, pair_off;
     std::vector<uint32_t> h_pair_off;  // per mesh pair result offsets of the last batch query
     gi_tuning tune = {2, 2048, 1024, (int32_t)kBuildChunk};  // launch shapes (gi_autotune)
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};  // [8..9]: the grid exact stage's passes after the read-back
     // pinned host staging: per-slot transforms (copied to xf_dev before the
     // vertex kernels) and the counter read-back; graph replays read/write them
     Xform* h_xf = nullptr;
@@ -639,6 +639,20 @@ int exact_kinds(const gi_context* ctx) {
 constexpr int kTimeAll = GI_TIME_STAGING | GI_TIME_BVH_CHUNKS | GI_TIME_TRAVERSE | 16;  // 16: the other stages
 int timing_mask(const gi_options& o) { return o.stage_timing == 0 ? kTimeAll : (o.stage_timing & 7); }
 
+// sorted output of a small result set, before the read-back (buffers from
+// ensure_output_bufs); its last launch also copies the counters to
+// ctx->h_counters
+void enqueue_output_small(gi_context* ctx, const gi_options& o) {
+    uint32_t* counters = ctx->counters.get<uint32_t>();
+    const int bitsB = bits_for(ctx->mesh[1].nt);
+    launch_output_small(ctx->rec_key.get<uint64_t>(), ctx->rec_seg.get<uint2>(), counters, ctx->cap_rec, ctx->cap_seg,
+                        bits_for(ctx->mesh[0].nt) + bitsB, bitsB, o.want_segments != 0, ctx->tkeys.get<uint64_t>(),
+                        ctx->pairs.get<uint2>(), ctx->order.get<uint32_t>(), ctx->offs.get<uint32_t>(),
+                        ctx->segs.get<uint64_t>(), ctx->exp2.get<int32_t>(), ctx->f64.get<double>(),
+                        (uint32_t)output_seg_slots(ctx), ctx->seg_pool.get<uint64_t>(), ctx->seg_exp.get<int32_t>(),
+                        counters + CNT_COUNT, ctx->d_counters_host, ctx->stream);
+}
+
 void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool timed_query) {
     hipStream_t s = ctx->stream;
     const int mask = timed_query ? timing_mask(o) : 0;
@@ -666,22 +680,35 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
                   ctx->tune.filter_blocks, s);
     if (timed) record(ctx, ctx->ev[3], s);
     ctx->exact_kinds = exact_kinds(ctx);
+    // the grid pairs the one-lane pass leaves (a zero orientation sign) run
+    // after the read-back, only when there are any (run_intersect)
     launch_exact(A, B, ctx->exact.get<uint2>(), ctx->cap_exact, grid, bitsB, ctx->rec_key.get<uint64_t>(),
                  ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<uint64_t>(), ctx->seg_exp.get<int32_t>(),
                  ctx->cap_seg, ctx->cand.get<uint2>(), ctx->cap_cand, ctx->resid.get<uint2>(), ctx->cap_exact,
-                 counters, ctx->tune.exact_blocks, ctx->exact_kinds, s);
+                 counters, ctx->tune.exact_blocks, ctx->exact_kinds | kExactGridSlowLater, s);
     if (timed) record(ctx, ctx->ev[4], s);
-    // sorted output of a small result set, before the read-back (buffers
-    // from ensure_output_bufs)
-    launch_output_small(ctx->rec_key.get<uint64_t>(), ctx->rec_seg.get<uint2>(), counters, ctx->cap_rec, ctx->cap_seg,
-                        bits_for(MA.nt) + bitsB, bitsB, o.want_segments != 0, ctx->tkeys.get<uint64_t>(),
-                        ctx->pairs.get<uint2>(),
-                        ctx->order.get<uint32_t>(), ctx->offs.get<uint32_t>(), ctx->segs.get<uint64_t>(),
-                        ctx->exp2.get<int32_t>(), ctx->f64.get<double>(), (uint32_t)output_seg_slots(ctx),
-                        ctx->seg_pool.get<uint64_t>(), ctx->seg_exp.get<int32_t>(), grid, ctx->d_counters_host,
-                        s);  // the counters reach ctx->h_counters from the output launches
+    enqueue_output_small(ctx, o);
     if (timed) record(ctx, ctx->ev[5], s);
     HIP_TRY(hipGetLastError());
+}
+
+// After a read-back that found grid pairs left by the one-lane exact pass:
+// their passes, then the small output again over all records (it rewrites
+// the sorted output and the read-back counters). Returns once they are done.
+void run_exact_grid_slow(gi_context* ctx, const gi_options& o, bool timed) {
+    hipStream_t s = ctx->stream;
+    uint32_t* counters = ctx->counters.get<uint32_t>();
+    const MeshDev A = mesh_dev(ctx->mesh[0]), B = mesh_dev(ctx->mesh[1]);
+    if (timed) record(ctx, ctx->ev[8], s);
+    launch_exact_grid_slow(A, B, counters + CNT_COUNT, bits_for(ctx->mesh[1].nt), ctx->rec_key.get<uint64_t>(),
+                           ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<uint64_t>(), ctx->cap_seg,
+                           ctx->cand.get<uint2>(), ctx->cap_cand, ctx->resid.get<uint2>(), ctx->cap_exact, counters,
+                           ctx->tune.exact_blocks, s);
+    if (timed) record(ctx, ctx->ev[9], s);
+    enqueue_output_small(ctx, o);
+    if (timed) record(ctx, ctx->ev[5], s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
 }
 
 void ensure_output_bufs(gi_context* ctx, const gi_options& o) {
@@ -955,7 +982,7 @@ int finish_with_residual(gi_context* ctx, const gi_options& o, gi_result* out, u
     st.segments = nseg_all;
     st.ms_host = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (all_timed) {
-        st.ms_output = elapsed(ctx->ev[4], ctx->ev[5]);
+        st.ms_output = elapsed(ctx->ev[4], ctx->ev[5]);  // general doubles: no grid passes after the read-back
         st.ms_total = elapsed(ctx->ev[0], ctx->ev[5]) + st.ms_host;
     }
     out->n_pairs = n;
@@ -1029,6 +1056,8 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     ctx->exact_both = false;
     const bool graphed = launch_presync_graph(ctx, o);
     if (!graphed) enqueue_presync(ctx, o, true, true);
+    float slow_ms = 0.f;  // the grid exact stage's passes after the read-back (ms_exact)
+    bool slow_ran = false;  // they ran: the output timed from their end (ev[9])
     for (int attempt = 0;; ++attempt) {
         if (attempt) {  // trees are current: queries only
             ensure_output_bufs(ctx, o);
@@ -1043,6 +1072,14 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         // prediction that failed reruns the query with both)
         const bool ran = ctx->exact_kinds & ((hgrid[0] && hgrid[1]) ? kExactGrid : kExactWide);
         if (!ran) ctx->exact_both = true;
+        if (!over && ran && hgrid[0] && hgrid[1] && hc[CNT_SLOW]) {
+            // grid pairs with a zero orientation sign: their passes and the
+            // output again, then the capacities over all records
+            run_exact_grid_slow(ctx, o, timing_mask(o) == kTimeAll);
+            slow_ms = timing_mask(o) == kTimeAll ? elapsed(ctx->ev[8], ctx->ev[9]) : 0.f;
+            slow_ran = true;
+            over = hc[CNT_RESULTS] > ctx->cap_rec || hc[CNT_SEGMENTS] > ctx->cap_seg;
+        }
         if (!over && ran) break;
         if (attempt > 8) throw GiError{GI_ERR_INTERNAL, "candidate capacity did not converge"};
         uint32_t nc = hc[CNT_TRAVERSAL_HITS] > ctx->cap_cand ? grow(hc[CNT_TRAVERSAL_HITS], ctx->cap_cand) : ctx->cap_cand;
@@ -1084,7 +1121,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     st.exact_yellow_no = hc[CNT_EXACT_YELLOW_NO];
     st.coplanar = hc[CNT_COPLANAR];
     st.degenerate = hc[CNT_DEGENERATE];
-    if (all_timed) st.ms_exact = elapsed(ctx->ev[3], ctx->ev[4]);
+    if (all_timed) st.ms_exact = elapsed(ctx->ev[3], ctx->ev[4]) + slow_ms;
     uint32_t R = hc[CNT_RESULTS], S = o.want_segments ? hc[CNT_SEGMENTS] : 0;
     const size_t words = 8 * (size_t)L;
     // small result sets were sorted and gathered before the read-back
@@ -1174,7 +1211,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         out->internal = hs;
     }
     if (all_timed) {
-        st.ms_output = elapsed(ctx->ev[4], ctx->ev[5]);
+        st.ms_output = elapsed(slow_ran ? ctx->ev[9] : ctx->ev[4], ctx->ev[5]);
         st.ms_total = elapsed(ctx->ev[0], ctx->ev[5]);
     }
     return GI_OK;

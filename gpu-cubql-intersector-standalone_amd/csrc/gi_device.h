@@ -227,7 +227,11 @@ void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const 
                    uint2* exact_list, uint32_t exact_cap, uint32_t* counters, int blocks, hipStream_t s);
 // Exact stages a query launches: the grid flags are only known on the device,
 // so the host launches both unless it knows which one they will select.
-enum { kExactGrid = 1, kExactWide = 2 };
+// kExactGridSlowLater: the grid stage's passes for the pairs its one-lane
+// pass leaves (8-lane pass, then the deferred coplanar/degenerate pass) are
+// not queued here; the caller runs launch_exact_grid_slow when the counter
+// read-back shows such pairs.
+enum { kExactGrid = 1, kExactWide = 2, kExactGridSlowLater = 4 };
 // defer: room for the coplanar pairs the fast pass hands to the full pass
 // (at least as many entries as the exact list; the candidate queue, free by
 // then, serves). Grid meshes run the int64/int128 stage; general doubles the
@@ -238,6 +242,10 @@ void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_
                   int bitsB, uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, uint64_t* seg_pool,
                   int32_t* seg_exp, uint32_t seg_cap, uint2* defer, uint32_t defer_cap, uint2* resid,
                   uint32_t resid_cap, uint32_t* counters, int blocks, int kinds, hipStream_t s);
+void launch_exact_grid_slow(MeshDev A, MeshDev B, const uint32_t* grid_flags, int bitsB, uint64_t* rec_key,
+                            uint2* rec_seg, uint32_t rec_cap, uint64_t* seg_pool, uint32_t seg_cap, uint2* defer,
+                            uint32_t defer_cap, uint2* slow, uint32_t slow_cap, uint32_t* counters, int blocks,
+                            hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
 // Sorted output of up to output_small_max() result records in one workgroup,
 // then their segments; the record count is read on the device, so this is

@@ -2487,6 +2487,22 @@ __global__ __launch_bounds__(256) void k_exact_wide(MeshDev A, MeshDev B, const 
     tally.flush(counters);
 }
 
+// The grid pairs the one-lane pass left in `slow` (counted in CNT_SLOW): the
+// 8-lane pass, then the deferred (coplanar, degenerate) pairs it hands on,
+// one lane each with the full construction (their count is read on the
+// device; usually few: a quarter of the blocks).
+void launch_exact_grid_slow(MeshDev A, MeshDev B, const uint32_t* grid_flags, int bitsB, uint64_t* rec_key,
+                            uint2* rec_seg, uint32_t rec_cap, uint64_t* seg_pool, uint32_t seg_cap, uint2* defer,
+                            uint32_t defer_cap, uint2* slow, uint32_t slow_cap, uint32_t* counters, int blocks,
+                            hipStream_t s) {
+    if (A.nt == 0 || B.nt == 0) return;
+    k_exact_grid_lanes<<<blocks, 256, 0, s>>>(A, B, slow, slow_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
+                                              seg_pool, seg_cap, defer, defer_cap, counters, (int)CNT_SLOW);
+    k_exact<false><<<std::max(blocks / 4, 64), 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key, rec_seg,
+                                                            rec_cap, seg_pool, seg_cap, nullptr, 0, counters);
+    GI_LAUNCH_CHECK();
+}
+
 void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
                   int bitsB, uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, uint64_t* seg_pool,
                   int32_t* seg_exp, uint32_t seg_cap, uint2* defer, uint32_t defer_cap, uint2* resid,
@@ -2501,13 +2517,9 @@ void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_
         // room (grid meshes leave nothing to the host)
         k_exact_grid_fast<<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
                                                  rec_cap, seg_pool, seg_cap, resid, resid_cap, counters);
-        k_exact_grid_lanes<<<blocks, 256, 0, s>>>(A, B, resid, resid_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
-                                                  seg_pool, seg_cap, defer, defer_cap, counters, (int)CNT_SLOW);
-        // the deferred (coplanar) pairs: their count is read on the device
-        // usually empty: a quarter of the blocks (each pair takes one lane here)
-        k_exact<false><<<std::max(blocks / 4, 64), 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key,
-                                                                rec_seg, rec_cap,
-                                              seg_pool, seg_cap, nullptr, 0, counters);
+        if (!(kinds & kExactGridSlowLater))
+            launch_exact_grid_slow(A, B, grid_flags, bitsB, rec_key, rec_seg, rec_cap, seg_pool, seg_cap, defer,
+                                   defer_cap, resid, resid_cap, counters, blocks, s);
     }
     if (kinds & kExactWide) {
         k_exact_wide<true><<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
