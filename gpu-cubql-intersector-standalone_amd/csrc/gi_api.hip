@@ -164,6 +164,7 @@ struct gi_context {
     DevBuf counters, hist, scan_tmp, partials;
     DevBuf inspect;  // partial bounds and the MeshInspect of the last upload
     DevBuf cand, exact, rec_key, rec_seg, seg_pool, seg_exp, resid;
+    DevBuf slow;  // pairs the one-lane exact passes leave to the general ones (as many as `exact`)
     uint32_t cap_cand = 0, cap_exact = 0, cap_rec = 0, cap_seg = 0;
     DevBuf order, tkeys, tvals, pairs, nseg, offs, segs, f64, exp2, pair_off;
     std::vector<uint32_t> h_pair_off;  // per mesh pair result offsets of the last batch query
@@ -568,6 +569,7 @@ void ensure_arenas(gi_context* ctx, uint32_t cand, uint32_t exact, uint32_t rec,
     if (exact > ctx->cap_exact) {
         ctx->exact.ensure<uint2>(exact);
         ctx->resid.ensure<uint2>(exact);  // every listed pair may be left to the host
+        ctx->slow.ensure<uint2>(exact);
         ctx->cap_exact = exact;
     }
     if (rec > ctx->cap_rec) {
@@ -684,8 +686,9 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
     // after the read-back, only when there are any (run_intersect)
     launch_exact(A, B, ctx->exact.get<uint2>(), ctx->cap_exact, grid, bitsB, ctx->rec_key.get<uint64_t>(),
                  ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<uint64_t>(), ctx->seg_exp.get<int32_t>(),
-                 ctx->cap_seg, ctx->cand.get<uint2>(), ctx->cap_cand, ctx->resid.get<uint2>(), ctx->cap_exact,
-                 counters, ctx->tune.exact_blocks, ctx->exact_kinds | kExactGridSlowLater, s);
+                 ctx->cap_seg, ctx->cand.get<uint2>(), ctx->cap_cand, ctx->slow.get<uint2>(), ctx->cap_exact,
+                 ctx->resid.get<uint2>(), ctx->cap_exact, counters, ctx->tune.exact_blocks,
+                 ctx->exact_kinds | kExactGridSlowLater, s);
     if (timed) record(ctx, ctx->ev[4], s);
     enqueue_output_small(ctx, o);
     if (timed) record(ctx, ctx->ev[5], s);
@@ -700,10 +703,18 @@ void run_exact_grid_slow(gi_context* ctx, const gi_options& o, bool timed) {
     uint32_t* counters = ctx->counters.get<uint32_t>();
     const MeshDev A = mesh_dev(ctx->mesh[0]), B = mesh_dev(ctx->mesh[1]);
     if (timed) record(ctx, ctx->ev[8], s);
-    launch_exact_grid_slow(A, B, counters + CNT_COUNT, bits_for(ctx->mesh[1].nt), ctx->rec_key.get<uint64_t>(),
-                           ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<uint64_t>(), ctx->cap_seg,
-                           ctx->cand.get<uint2>(), ctx->cap_cand, ctx->resid.get<uint2>(), ctx->cap_exact, counters,
-                           ctx->tune.exact_blocks, s);
+    const uint32_t* hgrid = ctx->h_counters + CNT_COUNT;
+    if (hgrid[0] && hgrid[1])
+        launch_exact_grid_slow(A, B, counters + CNT_COUNT, bits_for(ctx->mesh[1].nt), ctx->rec_key.get<uint64_t>(),
+                               ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<uint64_t>(), ctx->cap_seg,
+                               ctx->cand.get<uint2>(), ctx->cap_cand, ctx->slow.get<uint2>(), ctx->cap_exact, counters,
+                               ctx->tune.exact_blocks, s);
+    else
+        launch_exact_wide_slow(A, B, counters + CNT_COUNT, bits_for(ctx->mesh[1].nt), ctx->rec_key.get<uint64_t>(),
+                               ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<uint64_t>(),
+                               ctx->seg_exp.get<int32_t>(), ctx->cap_seg, ctx->cand.get<uint2>(), ctx->cap_cand,
+                               ctx->slow.get<uint2>(), ctx->cap_exact, ctx->resid.get<uint2>(), ctx->cap_exact,
+                               counters, ctx->tune.exact_blocks, s);
     if (timed) record(ctx, ctx->ev[9], s);
     enqueue_output_small(ctx, o);
     if (timed) record(ctx, ctx->ev[5], s);
@@ -751,7 +762,7 @@ std::vector<uint64_t> presync_signature(gi_context* ctx, const gi_options& o) {
                                 &m.fbounds, &m.roots})
             buf(*b);
     }
-    for (const DevBuf* b : {&ctx->counters, &ctx->xf_dev, &ctx->cand, &ctx->exact, &ctx->rec_key,
+    for (const DevBuf* b : {&ctx->counters, &ctx->xf_dev, &ctx->cand, &ctx->exact, &ctx->slow, &ctx->rec_key,
                             &ctx->rec_seg, &ctx->seg_pool, &ctx->seg_exp, &ctx->resid, &ctx->exp2, &ctx->bhist[0], &ctx->bhist[1], &ctx->bscan[0],
                             &ctx->bscan[1], &ctx->bpart[0], &ctx->bpart[1], &ctx->pairs, &ctx->order, &ctx->offs,
                             &ctx->segs, &ctx->f64, &ctx->tkeys})
@@ -1072,9 +1083,9 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         // prediction that failed reruns the query with both)
         const bool ran = ctx->exact_kinds & ((hgrid[0] && hgrid[1]) ? kExactGrid : kExactWide);
         if (!ran) ctx->exact_both = true;
-        if (!over && ran && hgrid[0] && hgrid[1] && hc[CNT_SLOW]) {
-            // grid pairs with a zero orientation sign: their passes and the
-            // output again, then the capacities over all records
+        if (!over && ran && hc[CNT_SLOW]) {
+            // pairs with a zero orientation sign: their passes and the output
+            // again, then the capacities over all records
             run_exact_grid_slow(ctx, o, timing_mask(o) == kTimeAll);
             slow_ms = timing_mask(o) == kTimeAll ? elapsed(ctx->ev[8], ctx->ev[9]) : 0.f;
             slow_ran = true;

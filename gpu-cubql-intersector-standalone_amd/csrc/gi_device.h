@@ -49,7 +49,7 @@ enum Counter : uint32_t {
     CNT_FILTER_PRED = 15 * kCntStride,      // orient3d signs evaluated by the FP64 filter
     CNT_EXACT_PRED = 16 * kCntStride,       // orient3d and orient2d signs evaluated by the exact stage
     CNT_OVERFLOW = 17 * kCntStride,         // a count wrapped past 2^32 (the query is too large)
-    CNT_SLOW = 18 * kCntStride,             // grid pairs the one-lane exact pass leaves to the 8-lane one
+    CNT_SLOW = 18 * kCntStride,             // pairs the one-lane exact passes leave to the general ones
     CNT_COUNT = 19 * kCntStride
 };
 
@@ -227,10 +227,11 @@ void launch_filter(MeshDev A, MeshDev B, const uint2* cand, uint32_t cap, const 
                    uint2* exact_list, uint32_t exact_cap, uint32_t* counters, int blocks, hipStream_t s);
 // Exact stages a query launches: the grid flags are only known on the device,
 // so the host launches both unless it knows which one they will select.
-// kExactGridSlowLater: the grid stage's passes for the pairs its one-lane
-// pass leaves (8-lane pass, then the deferred coplanar/degenerate pass) are
-// not queued here; the caller runs launch_exact_grid_slow when the counter
-// read-back shows such pairs.
+// kExactGridSlowLater: the passes for the pairs the one-lane passes leave
+// (grid: 8-lane pass, then the deferred coplanar/degenerate pass; general
+// doubles: the general one-lane pass and its deferred pass) are not queued
+// here; the caller runs launch_exact_grid_slow / launch_exact_wide_slow when
+// the counter read-back shows such pairs.
 enum { kExactGrid = 1, kExactWide = 2, kExactGridSlowLater = 4 };
 // defer: room for the coplanar pairs the fast pass hands to the full pass
 // (at least as many entries as the exact list; the candidate queue, free by
@@ -238,10 +239,16 @@ enum { kExactGrid = 1, kExactWide = 2, kExactGridSlowLater = 4 };
 // wide stage (int128/256-bit), whose pairs that do not fit go to `resid`
 // (resid_cap entries, the host residual stage). seg_pool holds kSegPoolWords
 // words per slot, seg_exp one scale per slot (wide stage).
+// slow: room for the pairs the one-lane passes leave (as many as exact_list)
 void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
                   int bitsB, uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, uint64_t* seg_pool,
-                  int32_t* seg_exp, uint32_t seg_cap, uint2* defer, uint32_t defer_cap, uint2* resid,
-                  uint32_t resid_cap, uint32_t* counters, int blocks, int kinds, hipStream_t s);
+                  int32_t* seg_exp, uint32_t seg_cap, uint2* defer, uint32_t defer_cap, uint2* slow,
+                  uint32_t slow_cap, uint2* resid, uint32_t resid_cap, uint32_t* counters, int blocks, int kinds,
+                  hipStream_t s);
+void launch_exact_wide_slow(MeshDev A, MeshDev B, const uint32_t* grid_flags, int bitsB, uint64_t* rec_key,
+                            uint2* rec_seg, uint32_t rec_cap, uint64_t* seg_pool, int32_t* seg_exp, uint32_t seg_cap,
+                            uint2* defer, uint32_t defer_cap, uint2* slow, uint32_t slow_cap, uint2* resid,
+                            uint32_t resid_cap, uint32_t* counters, int blocks, hipStream_t s);
 void launch_exact_grid_slow(MeshDev A, MeshDev B, const uint32_t* grid_flags, int bitsB, uint64_t* rec_key,
                             uint2* rec_seg, uint32_t rec_cap, uint64_t* seg_pool, uint32_t seg_cap, uint2* defer,
                             uint32_t defer_cap, uint2* slow, uint32_t slow_cap, uint32_t* counters, int blocks,

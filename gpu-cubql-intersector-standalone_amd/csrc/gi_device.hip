@@ -2049,20 +2049,30 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
 // pair in all of its lanes. Pairs with a zero orientation sign go to the
 // slow list, which that kernel then takes.
 
-// int128 to double, |error| <= 4 eps |x| (eps = 2^-53): the magnitude's two
-// 64-bit halves each rounded once, then their sum
-__device__ __forceinline__ double i128_to_f64(i128 x) {
+// Exact numbers to double: |error| <= 4 eps |x| for int128 (the magnitude's
+// two 64-bit halves each rounded once, then their sum) and <= 8 eps |x| for
+// the 256-bit numbers (four limbs, three sums of non-negative terms), eps =
+// 2^-53.
+__device__ __forceinline__ double exact_to_f64(i128 x) {
     const u128 u = uabs128(x);
     const double d = (double)(uint64_t)(u >> 64) * 18446744073709551616.0 + (double)(uint64_t)u;
     return x < 0 ? -d : d;
 }
+__device__ __forceinline__ double exact_to_f64(const W256& x) {
+    const W256 u = w256_abs(x);
+    const double hi = (double)u.w[3] * 6.277101735386681e57 + (double)u.w[2] * 3.402823669209385e38;  // 2^192, 2^128
+    const double lo = (double)u.w[1] * 18446744073709551616.0 + (double)u.w[0];
+    const double d = hi + lo;
+    return w256_neg(x) ? -d : d;
+}
 
-// sign of xa - xb for xa, xb within 9 eps (relative) of the exact quotients
-// they stand for; 0 when that error could flip it (the bound is 32 eps of
-// the magnitudes)
+// Sign of xa - xb where each x is the FP64 quotient h / w of exact values
+// converted as above (so within 2 * 8 + 1 = 17 eps of h / w, relative), or 0
+// when that error could flip it: the bound is 2^-46 = 128 eps of the
+// magnitudes, well above the 34 eps the two quotients may be off together.
 __device__ __forceinline__ int cmp_f64_certified(double xa, double xb) {
     const double d = xa - xb;
-    const double bound = 3.552713678800501e-15 * (fabs(xa) + fabs(xb));  // 2^-48
+    const double bound = 1.4210854715202004e-14 * (fabs(xa) + fabs(xb));  // 2^-46
     return d > bound ? 1 : (d < -bound ? -1 : 0);
 }
 
@@ -2071,23 +2081,83 @@ __device__ __forceinline__ int cmp_f64_certified(double xa, double xb) {
 // apex, on its own side). Vertices and values come as separate arguments and
 // are picked by selects: a select between elements of one array would be
 // folded into an indexed access of a per-lane array in scratch memory.
-__device__ __forceinline__ void tri_crossings(GridExact::V3 T0, GridExact::V3 T1, GridExact::V3 T2, i128 o0, i128 o1,
-                                              i128 o2, int s0, int s1, int s2, GridExact::HP& p,
-                                              GridExact::HP& q) {
+template <class E>
+__device__ __forceinline__ void tri_crossings(typename E::V3 T0, typename E::V3 T1, typename E::V3 T2,
+                                              typename E::B o0, typename E::B o1, typename E::B o2, int s0, int s1,
+                                              int s2, typename E::HP& p, typename E::HP& q) {
+    typedef typename E::C C;
+    typedef typename E::B B;
     const int ap = s0 == s1 ? 2 : (s0 == s2 ? 1 : 0);
-    GridExact::V3 U, V, W;  // the apex, then the other two in cyclic order
+    typename E::V3 U, V, W;  // the apex, then the other two in cyclic order
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const int64_t t0 = T0.v[c], t1 = T1.v[c], t2 = T2.v[c];
+        const C t0 = T0.v[c], t1 = T1.v[c], t2 = T2.v[c];
         U.v[c] = ap == 0 ? t0 : (ap == 1 ? t1 : t2);
         V.v[c] = ap == 0 ? t1 : (ap == 1 ? t2 : t0);
         W.v[c] = ap == 0 ? t2 : (ap == 1 ? t0 : t1);
     }
-    const i128 ou = ap == 0 ? o0 : (ap == 1 ? o1 : o2);
-    const i128 ov = ap == 0 ? o1 : (ap == 1 ? o2 : o0);
-    const i128 ow = ap == 0 ? o2 : (ap == 1 ? o0 : o1);
-    p = GridExact::hpCrossing(U, V, ou, ov);
-    q = GridExact::hpCrossing(U, W, ou, ow);
+    const B ou = ap == 0 ? o0 : (ap == 1 ? o1 : o2);
+    const B ov = ap == 0 ? o1 : (ap == 1 ? o2 : o0);
+    const B ow = ap == 0 ? o2 : (ap == 1 ? o0 : o1);
+    p = E::hpCrossing(U, V, ou, ov);
+    q = E::hpCrossing(U, W, ou, ow);
+}
+
+// The strictly transversal case of a pair (each triangle strictly crosses
+// the other's plane): the overlap on L of the two triangles' crossing
+// segments. Returns -1 when a vertex lies on the other plane (the pair is
+// left to the general passes: this also covers degenerate triangles and
+// coplanar pairs), 0 when the triangles do not meet, 1 with the segment
+// [lo, hi] (ordered along an axis of L). npred counts the signs evaluated.
+template <class E, class P>
+__device__ __forceinline__ int transversal_segment(const typename E::V3 (&PA)[3], const typename E::V3 (&PB)[3],
+                                                   typename E::HP& lo, typename E::HP& hi, uint32_t& npred) {
+    typedef typename E::V3 V3;
+    typedef typename E::HP HP;
+    typedef typename E::B B;
+    const V3 nA = E::cross(E::sub(PA[1], PA[0]), E::sub(PA[2], PA[0]));
+    const V3 nB = E::cross(E::sub(PB[1], PB[0]), E::sub(PB[2], PB[0]));
+    B oA[3], oB[3];
+    int sA[3], sB[3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        oB[v] = E::dotB(nA, E::sub(PB[v], PA[0]));  // B's vertices against plane A
+        sB[v] = P::sgnB(oB[v]);
+        oA[v] = E::dotB(nB, E::sub(PA[v], PB[0]));  // A's against plane B
+        sA[v] = P::sgnB(oA[v]);
+    }
+    npred += 6;
+    if (sA[0] == 0 || sA[1] == 0 || sA[2] == 0 || sB[0] == 0 || sB[1] == 0 || sB[2] == 0) return -1;
+    if ((sA[0] == sA[1] && sA[1] == sA[2]) || (sB[0] == sB[1] && sB[1] == sB[2])) return 0;
+    HP pa0, pa1, pb0, pb1;
+    tri_crossings<E>(PA[0], PA[1], PA[2], oA[0], oA[1], oA[2], sA[0], sA[1], sA[2], pa0, pa1);
+    tri_crossings<E>(PB[0], PB[1], PB[2], oB[0], oB[1], oB[2], sB[0], sB[1], sB[2], pb0, pb1);
+    // axis of the largest component of L's direction nA x nB
+    const B d0 = P::mulCC(nA.v[1], nB.v[2]) - P::mulCC(nA.v[2], nB.v[1]);
+    const B d1 = P::mulCC(nA.v[2], nB.v[0]) - P::mulCC(nA.v[0], nB.v[2]);
+    const B d2 = P::mulCC(nA.v[0], nB.v[1]) - P::mulCC(nA.v[1], nB.v[0]);
+    int k = P::cmpAbsB(d1, d0) > 0 ? 1 : 0;
+    if (P::cmpAbsB(d2, k ? d1 : d0) > 0) k = 2;
+    auto xk = [&](const HP& Q) { return exact_to_f64(E::hcomp(Q, k)) / exact_to_f64(Q.w); };
+    double xa0 = xk(pa0), xa1 = xk(pa1), xb0 = xk(pb0), xb1 = xk(pb1);
+    auto cmp = [&](const HP& Q, double xq, const HP& R, double xr) {
+        int c = cmp_f64_certified(xq, xr);
+        if (c == 0) c = E::hpCmpAxis(Q, R, k);
+        return c;
+    };
+    npred += 5;
+    if (cmp(pa0, xa0, pa1, xa1) > 0) {  // each triangle's pair in order along k
+        const HP t = pa0; pa0 = pa1; pa1 = t;
+        const double x = xa0; xa0 = xa1; xa1 = x;
+    }
+    if (cmp(pb0, xb0, pb1, xb1) > 0) {
+        const HP t = pb0; pb0 = pb1; pb1 = t;
+        const double x = xb0; xb0 = xb1; xb1 = x;
+    }
+    const bool la = cmp(pa0, xa0, pb0, xb0) >= 0, ua = cmp(pa1, xa1, pb1, xb1) <= 0;
+    lo = la ? pa0 : pb0;
+    hi = ua ? pa1 : pb1;
+    return cmp(lo, la ? xa0 : xb0, hi, ua ? xa1 : xb1) <= 0 ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_exact_grid_fast(MeshDev A, MeshDev B, const uint2* __restrict__ list,
@@ -2099,7 +2169,7 @@ __global__ __launch_bounds__(256) void k_exact_grid_fast(MeshDev A, MeshDev B, c
                                                          uint32_t* __restrict__ counters) {
     typedef GridExact::V3 V3;
     typedef GridExact::HP HP;
-    if (!(grid_flags[0] && grid_flags[1])) return;  // general doubles: k_exact_wide
+    if (!(grid_flags[0] && grid_flags[1])) return;  // general doubles: k_exact_wide_fast
     __int128* seg_pool = (__int128*)seg_pool_words;
     uint32_t n = counters[CNT_EXACT_LIST];
     if (n > cap) n = cap;
@@ -2114,7 +2184,6 @@ __global__ __launch_bounds__(256) void k_exact_grid_fast(MeshDev A, MeshDev B, c
             en = list[i];
             a = en.x;
             b = en.y & kIndexMask;
-            const uint32_t status = en.y >> kStatusShift;
             const uint32_t* ta = A.tri + 3 * (size_t)a;
             const uint32_t* tb = B.tri + 3 * (size_t)b;
             V3 PA[3], PB[3];
@@ -2124,53 +2193,10 @@ __global__ __launch_bounds__(256) void k_exact_grid_fast(MeshDev A, MeshDev B, c
                 PA[v].v[0] = p.x; PA[v].v[1] = p.y; PA[v].v[2] = p.z;
                 PB[v].v[0] = q.x; PB[v].v[1] = q.y; PB[v].v[2] = q.z;
             }
-            const V3 nA = GridExact::cross(GridExact::sub(PA[1], PA[0]), GridExact::sub(PA[2], PA[0]));
-            const V3 nB = GridExact::cross(GridExact::sub(PB[1], PB[0]), GridExact::sub(PB[2], PB[0]));
-            i128 oA[3], oB[3];
-            int sA[3], sB[3];
-#pragma unroll
-            for (int v = 0; v < 3; ++v) {
-                oB[v] = GridExact::dotB(nA, GridExact::sub(PB[v], PA[0]));  // B's vertices against plane A
-                sB[v] = sgn128(oB[v]);
-                oA[v] = GridExact::dotB(nB, GridExact::sub(PA[v], PB[0]));  // A's against plane B
-                sA[v] = sgn128(oA[v]);
-            }
-            tally.pred += 6;
-            // a zero sign covers degenerate triangles (zero normal) and coplanar pairs too
-            to_slow = sA[0] == 0 || sA[1] == 0 || sA[2] == 0 || sB[0] == 0 || sB[1] == 0 || sB[2] == 0;
-            const bool apart = (sA[0] == sA[1] && sA[1] == sA[2]) || (sB[0] == sB[1] && sB[1] == sB[2]);
-            if (!to_slow && !apart) {
-                HP pa0, pa1, pb0, pb1;
-                tri_crossings(PA[0], PA[1], PA[2], oA[0], oA[1], oA[2], sA[0], sA[1], sA[2], pa0, pa1);
-                tri_crossings(PB[0], PB[1], PB[2], oB[0], oB[1], oB[2], sB[0], sB[1], sB[2], pb0, pb1);
-                // axis of the largest component of L's direction
-                const i128 d0 = (i128)nA.v[1] * nB.v[2] - (i128)nA.v[2] * nB.v[1];
-                const i128 d1 = (i128)nA.v[2] * nB.v[0] - (i128)nA.v[0] * nB.v[2];
-                const i128 d2 = (i128)nA.v[0] * nB.v[1] - (i128)nA.v[1] * nB.v[0];
-                int k = GridPolicy::cmpAbsB(d1, d0) > 0 ? 1 : 0;
-                if (GridPolicy::cmpAbsB(d2, k ? d1 : d0) > 0) k = 2;
-                auto xk = [&](const HP& P) { return i128_to_f64(GridExact::hcomp(P, k)) / i128_to_f64(P.w); };
-                double xa0 = xk(pa0), xa1 = xk(pa1), xb0 = xk(pb0), xb1 = xk(pb1);
-                auto cmp = [&](const HP& P, double xp, const HP& Q, double xq) {
-                    int c = cmp_f64_certified(xp, xq);
-                    if (c == 0) c = GridExact::hpCmpAxis(P, Q, k);
-                    return c;
-                };
-                tally.pred += 5;
-                if (cmp(pa0, xa0, pa1, xa1) > 0) {  // each triangle's pair in order along k
-                    const HP t = pa0; pa0 = pa1; pa1 = t;
-                    const double x = xa0; xa0 = xa1; xa1 = x;
-                }
-                if (cmp(pb0, xb0, pb1, xb1) > 0) {
-                    const HP t = pb0; pb0 = pb1; pb1 = t;
-                    const double x = xb0; xb0 = xb1; xb1 = x;
-                }
-                const bool la = cmp(pa0, xa0, pb0, xb0) >= 0, ua = cmp(pa1, xa1, pb1, xb1) <= 0;
-                lo = la ? pa0 : pb0;
-                hi = ua ? pa1 : pb1;
-                hit = cmp(lo, la ? xa0 : xb0, hi, ua ? xa1 : xb1) <= 0;
-            }
-            if (!to_slow) tally.add((int)status, hit, 0, 0);
+            const int r = transversal_segment<GridExact, GridPolicy>(PA, PB, lo, hi, tally.pred);
+            to_slow = r < 0;
+            hit = r > 0;
+            if (!to_slow) tally.add((int)(en.y >> kStatusShift), hit, 0, 0);
         }
         const uint32_t want[3] = {to_slow ? 1u : 0u, hit ? 1u : 0u, hit ? 1u : 0u};
         uint32_t got[3];
@@ -2395,10 +2421,10 @@ __global__ __launch_bounds__(256) void k_exact_wide(MeshDev A, MeshDev B, const 
                                                     int32_t* __restrict__ seg_exp, uint32_t seg_cap,
                                                     uint2* __restrict__ defer, uint32_t defer_cap,
                                                     uint2* __restrict__ resid, uint32_t resid_cap,
-                                                    uint32_t* __restrict__ counters) {
+                                                    uint32_t* __restrict__ counters, int list_counter) {
     typedef typename std::conditional<kFast, WideExact::Fast, WideExact::Result>::type R;
-    if (grid_flags[0] && grid_flags[1]) return;  // grid meshes: k_exact
-    uint32_t n = counters[kFast ? CNT_EXACT_LIST : CNT_DEFERRED];
+    if (grid_flags[0] && grid_flags[1]) return;  // grid meshes: k_exact_grid_fast
+    uint32_t n = counters[list_counter];
     if (n > cap) n = cap;
     ExactTally tally;
     for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
@@ -2487,6 +2513,99 @@ __global__ __launch_bounds__(256) void k_exact_wide(MeshDev A, MeshDev B, const 
     tally.flush(counters);
 }
 
+// The strictly transversal pairs of general doubles, one lane each: the
+// pair in its integer frame (exact_wide.h), transversal_segment on 256-bit
+// numbers, the endpoints moved back out of the frame. Pairs whose frame or
+// endpoints do not fit go to the host residual list; pairs with a vertex on
+// the other plane to the slow list (k_exact_wide<true> on it).
+__global__ __launch_bounds__(256) void k_exact_wide_fast(MeshDev A, MeshDev B, const uint2* __restrict__ list,
+                                                         uint32_t cap, const uint32_t* __restrict__ grid_flags,
+                                                         int bitsB, uint64_t* __restrict__ rec_key,
+                                                         uint2* __restrict__ rec_seg, uint32_t rec_cap,
+                                                         uint64_t* __restrict__ seg_pool, int32_t* __restrict__ seg_exp,
+                                                         uint32_t seg_cap, uint2* __restrict__ slow, uint32_t slow_cap,
+                                                         uint2* __restrict__ resid, uint32_t resid_cap,
+                                                         uint32_t* __restrict__ counters) {
+    if (grid_flags[0] && grid_flags[1]) return;  // grid meshes: k_exact_grid_fast
+    uint32_t n = counters[CNT_EXACT_LIST];
+    if (n > cap) n = cap;
+    ExactTally tally;
+    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const uint32_t i = base + threadIdx.x;
+        bool residual = false, to_slow = false, hit = false;
+        uint2 en = make_uint2(0, 0);
+        uint32_t a = 0, b = 0;
+        WideExact::HP lo, hi;
+        WideFrame fr;
+        fr.exp2 = 0;
+        fr.shifted = 0;
+        if (i < n) {
+            en = list[i];
+            a = en.x;
+            b = en.y & kIndexMask;
+            const uint32_t* ta = A.tri + 3 * (size_t)a;
+            const uint32_t* tb = B.tri + 3 * (size_t)b;
+            double pa[9], pb[9];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const double4 p = A.v64[ta[v]], q = B.v64[tb[v]];
+                pa[3 * v] = p.x; pa[3 * v + 1] = p.y; pa[3 * v + 2] = p.z;
+                pb[3 * v] = q.x; pb[3 * v + 1] = q.y; pb[3 * v + 2] = q.z;
+            }
+            WideExact::V3 PA[3], PB[3];
+            residual = !wide_frame(pa, pb, PA, PB, fr);
+            if (!residual) {
+                uint32_t np = 0;
+                const int r = transversal_segment<WideExact, WidePolicy>(PA, PB, lo, hi, np);
+                to_slow = r < 0;
+                hit = r > 0;
+                if (hit) residual = !wide_absolute(fr, lo) || !wide_absolute(fr, hi);
+                if (residual) {
+                    hit = false;
+                } else {
+                    tally.pred += np;
+                    if (!to_slow) tally.add((int)(en.y >> kStatusShift), hit, 0, 0);
+                }
+            }
+        }
+        const uint32_t rslot = block_reserve(residual ? 1u : 0u, counters, CNT_RESIDUAL);
+        if (residual && rslot < resid_cap) resid[rslot] = en;
+        const uint32_t want[3] = {to_slow ? 1u : 0u, hit ? 1u : 0u, hit ? 1u : 0u};
+        uint32_t got[3];
+        block_reserve3(want, counters, CNT_SLOW, CNT_RESULTS, CNT_SEGMENTS, got);
+        if (to_slow && got[0] < slow_cap) slow[got[0]] = en;
+        if (hit && got[1] < rec_cap) {
+            rec_key[got[1]] = ((uint64_t)a << bitsB) | (uint64_t)b;
+            rec_seg[got[1]] = make_uint2(got[2], 1u);
+            if (got[2] < seg_cap) {
+                uint64_t* o = seg_pool + (size_t)8 * kSegLimbsWide * got[2];
+                store_w256(o + 0, lo.h[0]); store_w256(o + 4, lo.h[1]); store_w256(o + 8, lo.h[2]);
+                store_w256(o + 12, lo.w);
+                store_w256(o + 16, hi.h[0]); store_w256(o + 20, hi.h[1]); store_w256(o + 24, hi.h[2]);
+                store_w256(o + 28, hi.w);
+                seg_exp[got[2]] = fr.exp2;
+            }
+        }
+    }
+    tally.flush(counters);
+}
+
+// The general doubles' pairs k_exact_wide_fast left in `slow` (CNT_SLOW):
+// the general one-lane pass, then its deferred coplanar pairs.
+void launch_exact_wide_slow(MeshDev A, MeshDev B, const uint32_t* grid_flags, int bitsB, uint64_t* rec_key,
+                            uint2* rec_seg, uint32_t rec_cap, uint64_t* seg_pool, int32_t* seg_exp, uint32_t seg_cap,
+                            uint2* defer, uint32_t defer_cap, uint2* slow, uint32_t slow_cap, uint2* resid,
+                            uint32_t resid_cap, uint32_t* counters, int blocks, hipStream_t s) {
+    if (A.nt == 0 || B.nt == 0) return;
+    k_exact_wide<true><<<blocks, 256, 0, s>>>(A, B, slow, slow_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
+                                              seg_pool, seg_exp, seg_cap, defer, defer_cap, resid, resid_cap, counters,
+                                              (int)CNT_SLOW);
+    k_exact_wide<false><<<blocks, 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key, rec_seg, rec_cap,
+                                               seg_pool, seg_exp, seg_cap, nullptr, 0, resid, resid_cap, counters,
+                                               (int)CNT_DEFERRED);
+    GI_LAUNCH_CHECK();
+}
+
 // The grid pairs the one-lane pass left in `slow` (counted in CNT_SLOW): the
 // 8-lane pass, then the deferred (coplanar, degenerate) pairs it hands on,
 // one lane each with the full construction (their count is read on the
@@ -2505,29 +2624,31 @@ void launch_exact_grid_slow(MeshDev A, MeshDev B, const uint32_t* grid_flags, in
 
 void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_cap, const uint32_t* grid_flags,
                   int bitsB, uint64_t* rec_key, uint2* rec_seg, uint32_t rec_cap, uint64_t* seg_pool,
-                  int32_t* seg_exp, uint32_t seg_cap, uint2* defer, uint32_t defer_cap, uint2* resid,
-                  uint32_t resid_cap, uint32_t* counters, int blocks, int kinds, hipStream_t s) {
+                  int32_t* seg_exp, uint32_t seg_cap, uint2* defer, uint32_t defer_cap, uint2* slow,
+                  uint32_t slow_cap, uint2* resid, uint32_t resid_cap, uint32_t* counters, int blocks, int kinds,
+                  hipStream_t s) {
     if (A.nt == 0 || B.nt == 0) return;
     // kinds & 1: the grid stage, & 2: the wide stage; with both, the one the
     // grid flags (decided on the device by the vertex stage) do not select
     // returns at once
+    // strictly transversal pairs one lane each; the rest (a zero orientation
+    // sign) to the slow list, whose passes run here or, with
+    // kExactGridSlowLater, when the caller has read CNT_SLOW
+    const bool later = (kinds & kExactGridSlowLater) != 0;
     if (kinds & kExactGrid) {
-        // strictly transversal pairs one lane each; the rest (a zero
-        // orientation sign) through the 8-lane kernel, on the residual list's
-        // room (grid meshes leave nothing to the host)
         k_exact_grid_fast<<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
-                                                 rec_cap, seg_pool, seg_cap, resid, resid_cap, counters);
-        if (!(kinds & kExactGridSlowLater))
+                                                 rec_cap, seg_pool, seg_cap, slow, slow_cap, counters);
+        if (!later)
             launch_exact_grid_slow(A, B, grid_flags, bitsB, rec_key, rec_seg, rec_cap, seg_pool, seg_cap, defer,
-                                   defer_cap, resid, resid_cap, counters, blocks, s);
+                                   defer_cap, slow, slow_cap, counters, blocks, s);
     }
     if (kinds & kExactWide) {
-        k_exact_wide<true><<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
-                                                  rec_cap, seg_pool, seg_exp, seg_cap, defer, defer_cap, resid,
-                                                  resid_cap, counters);
-        k_exact_wide<false><<<blocks, 256, 0, s>>>(A, B, defer, defer_cap, grid_flags, bitsB, rec_key, rec_seg,
-                                                   rec_cap, seg_pool, seg_exp, seg_cap, nullptr, 0, resid, resid_cap,
-                                                   counters);
+        k_exact_wide_fast<<<blocks, 256, 0, s>>>(A, B, exact_list, exact_cap, grid_flags, bitsB, rec_key, rec_seg,
+                                                 rec_cap, seg_pool, seg_exp, seg_cap, slow, slow_cap, resid, resid_cap,
+                                                 counters);
+        if (!later)
+            launch_exact_wide_slow(A, B, grid_flags, bitsB, rec_key, rec_seg, rec_cap, seg_pool, seg_exp, seg_cap,
+                                   defer, defer_cap, slow, slow_cap, resid, resid_cap, counters, blocks, s);
     }
     GI_LAUNCH_CHECK();
 }

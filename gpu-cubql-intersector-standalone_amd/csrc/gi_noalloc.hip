@@ -125,10 +125,10 @@ void carve_mesh(Carve& c, uint32_t nv, uint32_t nt, bool tree, MeshBufs& m) {
     m.axis = morton_axis_bits(nt);
 }
 
-// scratch bytes per unit of the arena scale X (X candidates, X/2 exact-list
-// and residual entries, X/4 result records and segments, and the sorted
-// output of X/4 records), a bound used to size X from the space left
-constexpr size_t kArenaBytesPerUnit = 8 + 8 + (8 + 8 + 8 * kSegPoolWords + 4 + 8 + 4 + 4 + 8 + 4 + 4 + 4 + 8 * kSegPoolWords +
+// scratch bytes per unit of the arena scale X (X candidates, X/2 exact-list,
+// residual and slow-list entries, X/4 result records and segments, and the
+// sorted output of X/4 records), a bound used to size X from the space left
+constexpr size_t kArenaBytesPerUnit = 8 + 8 + 4 + (8 + 8 + 8 * kSegPoolWords + 4 + 8 + 4 + 4 + 8 + 4 + 4 + 4 + 8 * kSegPoolWords +
                                                4 + 48 + 8) / 4 + 16;
 
 #define NA_TRY(expr)                                              \
@@ -203,6 +203,7 @@ extern "C" int gi_intersect_noalloc(int32_t device, const double* xyzA, uint32_t
     uint2* cand = c.take<uint2>(cap_cand);
     uint2* exact = c.take<uint2>(cap_exact);
     uint2* resid = c.take<uint2>(cap_exact);
+    uint2* slow = c.take<uint2>(cap_exact);
     uint64_t* rec_key = c.take<uint64_t>(cap_rec);
     uint2* rec_seg = c.take<uint2>(cap_rec);
     uint64_t* seg_pool = c.take<uint64_t>(kSegPoolWords * (size_t)cap_seg);
@@ -271,7 +272,7 @@ extern "C" int gi_intersect_noalloc(int32_t device, const double* xyzA, uint32_t
         launch_traverse(A, B, nullptr, 0, nullptr, cand, cap_cand, counters, 2, s);
         launch_filter(A, B, cand, cap_cand, grid, exact, cap_exact, counters, 2048, s);
         launch_exact(A, B, exact, cap_exact, grid, bitsB, rec_key, rec_seg, cap_rec, seg_pool, seg_exp, cap_seg, cand,
-                     cap_cand, resid, cap_exact, counters, 1024, kinds, s);
+                     cap_cand, slow, cap_exact, resid, cap_exact, counters, 1024, kinds, s);
         launch_output_small(rec_key, rec_seg, counters, cap_rec, cap_seg, bitsA + bitsB, bitsB, want_segments != 0,
                             runs, out_pairs, order, offs, segs, sexp, f64, out_slots, seg_pool, seg_exp, grid, nullptr, s);
         NA_TRY(hipGetLastError());
