@@ -2076,19 +2076,28 @@ __device__ __forceinline__ int cmp_f64_certified(double xa, double xb) {
     return d > bound ? 1 : (d < -bound ? -1 : 0);
 }
 
-// The two points where triangle (T0, T1, T2) crosses the other plane
-// (orientations o0..o2 against it, signs s0..s2; exactly one vertex, the
-// apex, on its own side). Vertices and values come as separate arguments and
+// The segment where triangle (T0, T1, T2) meets the other triangle's plane,
+// from its vertices' orientations o0..o2 against that plane (signs s0..s2,
+// neither all zero nor all of one sign): with no zero sign, the points where
+// the two edges at the vertex alone on its side cross the plane; with one
+// zero, that vertex and, when the other two lie on opposite sides, the point
+// where their edge crosses (else the vertex alone, p = q); with two zeros,
+// the edge between them. Vertices and values come as separate arguments and
 // are picked by selects: a select between elements of one array would be
 // folded into an indexed access of a per-lane array in scratch memory.
 template <class E>
-__device__ __forceinline__ void tri_crossings(typename E::V3 T0, typename E::V3 T1, typename E::V3 T2,
-                                              typename E::B o0, typename E::B o1, typename E::B o2, int s0, int s1,
-                                              int s2, typename E::HP& p, typename E::HP& q) {
+__device__ __forceinline__ void tri_plane_segment(typename E::V3 T0, typename E::V3 T1, typename E::V3 T2,
+                                                  typename E::B o0, typename E::B o1, typename E::B o2, int s0, int s1,
+                                                  int s2, typename E::HP& p, typename E::HP& q) {
     typedef typename E::C C;
     typedef typename E::B B;
-    const int ap = s0 == s1 ? 2 : (s0 == s2 ? 1 : 0);
-    typename E::V3 U, V, W;  // the apex, then the other two in cyclic order
+    const int zeros = (s0 == 0) + (s1 == 0) + (s2 == 0);
+    // the vertex U the segment hangs on: the apex (no zero), the zero vertex
+    // (one zero) or the first zero vertex (two zeros); V, W the others in
+    // cyclic order
+    const int ap = zeros == 0 ? (s0 == s1 ? 2 : (s0 == s2 ? 1 : 0))
+                              : (s0 == 0 ? (zeros == 2 && s1 != 0 ? 2 : 0) : (s1 == 0 ? 1 : 2));
+    typename E::V3 U, V, W;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const C t0 = T0.v[c], t1 = T1.v[c], t2 = T2.v[c];
@@ -2099,16 +2108,30 @@ __device__ __forceinline__ void tri_crossings(typename E::V3 T0, typename E::V3 
     const B ou = ap == 0 ? o0 : (ap == 1 ? o1 : o2);
     const B ov = ap == 0 ? o1 : (ap == 1 ? o2 : o0);
     const B ow = ap == 0 ? o2 : (ap == 1 ? o0 : o1);
-    p = E::hpCrossing(U, V, ou, ov);
-    q = E::hpCrossing(U, W, ou, ow);
+    const int sv = ap == 0 ? s1 : (ap == 1 ? s2 : s0), sw = ap == 0 ? s2 : (ap == 1 ? s0 : s1);
+    if (zeros == 0) {
+        p = E::hpCrossing(U, V, ou, ov);
+        q = E::hpCrossing(U, W, ou, ow);
+    } else if (zeros == 1) {
+        p = E::hpInt(U);
+        q = sv == sw ? p : E::hpCrossing(V, W, ov, ow);
+    } else {  // U and the other zero vertex (V or W) span the segment
+        p = E::hpInt(U);
+        q = E::hpInt(sv == 0 ? V : W);
+    }
 }
 
-// The strictly transversal case of a pair (each triangle strictly crosses
-// the other's plane): the overlap on L of the two triangles' crossing
-// segments. Returns -1 when a vertex lies on the other plane (the pair is
-// left to the general passes: this also covers degenerate triangles and
-// coplanar pairs), 0 when the triangles do not meet, 1 with the segment
-// [lo, hi] (ordered along an axis of L). npred counts the signs evaluated.
+// A pair of non-degenerate, non-coplanar triangles: T_A ^ T_B is the
+// overlap, on the line L = plane_A ^ plane_B, of the segments where each
+// triangle meets the other's plane. Returns -1 when a triangle's vertices
+// all lie on the other's plane (coplanar pair, or a degenerate triangle: a
+// zero normal puts the other triangle's vertices all on its "plane"), left
+// to the general passes; 0 when the triangles do not meet; 1 with the
+// segment [lo, hi] (ordered along an axis of L; lo = hi for a point). When
+// neither triangle has all its vertices on the other's plane, the planes
+// are not parallel (a parallel plane through a vertex of the other triangle
+// would be that triangle's plane) or the triangles lie apart, so L exists.
+// npred counts the signs evaluated.
 template <class E, class P>
 __device__ __forceinline__ int transversal_segment(const typename E::V3 (&PA)[3], const typename E::V3 (&PB)[3],
                                                    typename E::HP& lo, typename E::HP& hi, uint32_t& npred) {
@@ -2127,11 +2150,11 @@ __device__ __forceinline__ int transversal_segment(const typename E::V3 (&PA)[3]
         sA[v] = P::sgnB(oA[v]);
     }
     npred += 6;
-    if (sA[0] == 0 || sA[1] == 0 || sA[2] == 0 || sB[0] == 0 || sB[1] == 0 || sB[2] == 0) return -1;
-    if ((sA[0] == sA[1] && sA[1] == sA[2]) || (sB[0] == sB[1] && sB[1] == sB[2])) return 0;
+    if ((sA[0] == 0 && sA[1] == 0 && sA[2] == 0) || (sB[0] == 0 && sB[1] == 0 && sB[2] == 0)) return -1;
+    if ((sA[0] == sA[1] && sA[1] == sA[2]) || (sB[0] == sB[1] && sB[1] == sB[2])) return 0;  // all on one side
     HP pa0, pa1, pb0, pb1;
-    tri_crossings<E>(PA[0], PA[1], PA[2], oA[0], oA[1], oA[2], sA[0], sA[1], sA[2], pa0, pa1);
-    tri_crossings<E>(PB[0], PB[1], PB[2], oB[0], oB[1], oB[2], sB[0], sB[1], sB[2], pb0, pb1);
+    tri_plane_segment<E>(PA[0], PA[1], PA[2], oA[0], oA[1], oA[2], sA[0], sA[1], sA[2], pa0, pa1);
+    tri_plane_segment<E>(PB[0], PB[1], PB[2], oB[0], oB[1], oB[2], sB[0], sB[1], sB[2], pb0, pb1);
     // axis of the largest component of L's direction nA x nB
     const B d0 = P::mulCC(nA.v[1], nB.v[2]) - P::mulCC(nA.v[2], nB.v[1]);
     const B d1 = P::mulCC(nA.v[2], nB.v[0]) - P::mulCC(nA.v[0], nB.v[2]);
