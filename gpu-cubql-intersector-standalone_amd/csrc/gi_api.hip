@@ -682,8 +682,8 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
                   ctx->tune.filter_blocks, s);
     if (timed) record(ctx, ctx->ev[3], s);
     ctx->exact_kinds = exact_kinds(ctx);
-    // the grid pairs the one-lane pass leaves (a zero orientation sign) run
-    // after the read-back, only when there are any (run_intersect)
+    // the pairs the one-lane pass leaves (coplanar, degenerate) run after
+    // the read-back, only when there are any (run_intersect)
     launch_exact(A, B, ctx->exact.get<uint2>(), ctx->cap_exact, grid, bitsB, ctx->rec_key.get<uint64_t>(),
                  ctx->rec_seg.get<uint2>(), ctx->cap_rec, ctx->seg_pool.get<uint64_t>(), ctx->seg_exp.get<int32_t>(),
                  ctx->cap_seg, ctx->cand.get<uint2>(), ctx->cap_cand, ctx->slow.get<uint2>(), ctx->cap_exact,
@@ -1084,7 +1084,7 @@ int run_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
         const bool ran = ctx->exact_kinds & ((hgrid[0] && hgrid[1]) ? kExactGrid : kExactWide);
         if (!ran) ctx->exact_both = true;
         if (!over && ran && hc[CNT_SLOW]) {
-            // pairs with a zero orientation sign: their passes and the output
+            // coplanar and degenerate pairs: their passes and the output
             // again, then the capacities over all records
             run_exact_grid_slow(ctx, o, timing_mask(o) == kTimeAll);
             slow_ms = timing_mask(o) == kTimeAll ? elapsed(ctx->ev[8], ctx->ev[9]) : 0.f;

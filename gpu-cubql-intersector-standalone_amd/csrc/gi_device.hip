@@ -2033,21 +2033,18 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
 
 // ----------------------------------------------- grid pairs, one lane each
 //
-// The common case on grid meshes: each triangle strictly crosses the other's
-// plane (no vertex on the other plane, so neither triangle is degenerate and
-// they are not coplanar). Then T_A ^ T_B is the overlap, on the line L =
-// plane_A ^ plane_B, of T_A ^ L and T_B ^ L, each the segment between the
-// two points where a triangle's edges cross the other plane (the two edges
-// at the vertex whose side differs from the other two's). The four crossing
-// points are exact homogeneous int128 points. On an axis k where L's
-// direction nA x nB is non-zero, points of L are ordered by their k
-// coordinate alone: the overlap is [max of the two lower ends, min of the two
-// upper ends], empty when they cross. Each comparison of h_k / w values
-// first runs in FP64 with a certified bound and falls back to the exact
-// cross-multiplied one (256-bit products) when the bound does not decide.
-// One lane per pair: the 8-lane kernel below computes the shared part of a
-// pair in all of its lanes. Pairs with a zero orientation sign go to the
-// slow list, which that kernel then takes.
+// Every pair of grid meshes but the coplanar ones and those with a
+// degenerate triangle. T_A ^ T_B is then the overlap, on the line L =
+// plane_A ^ plane_B, of the segments where each triangle meets the other's
+// plane (tri_plane_segment below), whose end points are exact homogeneous
+// int128 points. On an axis k where L's direction nA x nB is non-zero,
+// points of L are ordered by their k coordinate alone: the overlap is [max of
+// the two lower ends, min of the two upper ends], empty when they cross.
+// Each comparison of h_k / w values first runs in FP64 with a certified bound
+// and falls back to the exact cross-multiplied one (256-bit products) when
+// the bound does not decide. One lane per pair: the 8-lane kernel below
+// computes the shared part of a pair in all of its lanes. The coplanar and
+// degenerate pairs go to the slow list, which that kernel then takes.
 
 // Exact numbers to double: |error| <= 4 eps |x| for int128 (the magnitude's
 // two 64-bit halves each rounded once, then their sum) and <= 8 eps |x| for
@@ -2536,11 +2533,11 @@ __global__ __launch_bounds__(256) void k_exact_wide(MeshDev A, MeshDev B, const 
     tally.flush(counters);
 }
 
-// The strictly transversal pairs of general doubles, one lane each: the
-// pair in its integer frame (exact_wide.h), transversal_segment on 256-bit
-// numbers, the endpoints moved back out of the frame. Pairs whose frame or
-// endpoints do not fit go to the host residual list; pairs with a vertex on
-// the other plane to the slow list (k_exact_wide<true> on it).
+// The pairs of general doubles, one lane each: the pair in its integer
+// frame (exact_wide.h), transversal_segment on 256-bit numbers, the end
+// points moved back out of the frame. Pairs whose frame or end points do not
+// fit go to the host residual list; coplanar and degenerate pairs to the slow
+// list (k_exact_wide<true> on it).
 __global__ __launch_bounds__(256) void k_exact_wide_fast(MeshDev A, MeshDev B, const uint2* __restrict__ list,
                                                          uint32_t cap, const uint32_t* __restrict__ grid_flags,
                                                          int bitsB, uint64_t* __restrict__ rec_key,
@@ -2654,8 +2651,8 @@ void launch_exact(MeshDev A, MeshDev B, const uint2* exact_list, uint32_t exact_
     // kinds & 1: the grid stage, & 2: the wide stage; with both, the one the
     // grid flags (decided on the device by the vertex stage) do not select
     // returns at once
-    // strictly transversal pairs one lane each; the rest (a zero orientation
-    // sign) to the slow list, whose passes run here or, with
+    // pairs one lane each; the coplanar and degenerate ones to the slow
+    // list, whose passes run here or, with
     // kExactGridSlowLater, when the caller has read CNT_SLOW
     const bool later = (kinds & kExactGridSlowLater) != 0;
     if (kinds & kExactGrid) {
