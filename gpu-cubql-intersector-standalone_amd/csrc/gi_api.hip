@@ -123,9 +123,22 @@ int bits_for(uint32_t n) { return n <= 1 ? 1 : 32 - __builtin_clz(n - 1); }
 // tell: 1 on the grid, 0 off it, -1 unknown. Unmoved vertices keep their
 // source classification; snapped ones are integers, within the bound when
 // the moved corners of the source box are (margin for the rounding of the
-// map); other moves are left to the device.
+// map). An unsnapped move whose matrix is not a signed permutation is taken
+// to leave the grid (a rotation by any other angle maps integers to
+// non-integers but in rare rational cases; a wrong guess only costs the
+// query a second run with both exact stages); other moves are left to the
+// device.
 int predicted_grid(const MeshSlot& m) {
     if (m.src_grid < 0) return -1;
+    if (!m.xf.identity && !m.xf.snap) {
+        bool perm = true;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                const double v = m.xf.m[r][c];
+                perm = perm && (v == 0.0 || v == 1.0 || v == -1.0);
+            }
+        return perm ? -1 : 0;
+    }
     if (m.xf.identity) {
         if (!m.xf.snap || m.src_grid == 1) return m.src_grid;
         const double r = std::max(std::max(fabs(m.src_lo[0]), fabs(m.src_hi[0])),
