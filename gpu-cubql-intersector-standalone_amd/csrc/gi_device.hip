@@ -1328,6 +1328,7 @@ __global__ __launch_bounds__(256) void k_block_boxes(BlockBoxJobs J) {
 // nodes' load chains overlap; a quarter issues all its window loads before
 // it reduces. Blocks [0, split) serve tree 0.
 constexpr uint32_t TOP_GROUPS = 4, TOP_LANES = 16;
+constexpr uint32_t kTopLevelBoxes = 256;  // boxes of the highest block level k_top_boxes reads whole
 
 struct TopBoxJobs {
     BlockLevels lv0, lv1;
@@ -1335,6 +1336,7 @@ struct TopBoxJobs {
     const uint64_t* topmask0; const uint64_t* topmask1;
     float4* nodes0; float4* nodes1;
     uint32_t n0, n1;
+    int top0, top1;  // the highest block level built (at most kTopLevelBoxes boxes)
     uint32_t split;
 };
 
@@ -1345,6 +1347,7 @@ __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J, uint32_t chunk)
     const uint2* __restrict__ range = second ? J.range1 : J.range0;
     float4* nodes = second ? J.nodes1 : J.nodes0;
     const BlockLevels& lv = second ? J.lv1 : J.lv0;  // read in place from the kernel arguments
+    const int top = second ? J.top1 : J.top0;
     const uint32_t groups = n > 1 ? (n - 1 + 63) / 64 : 0u;  // groups of the n - 1 internal nodes
     const uint32_t g0 = ((blk - (second ? J.split : 0u)) * 4 + (threadIdx.x >> 6)) * TOP_GROUPS;
     if (g0 >= groups) return;
@@ -1384,7 +1387,7 @@ __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J, uint32_t chunk)
             wS[2 * lvl] = wS[2 * lvl + 1] = 0;
             wC[2 * lvl] = wC[2 * lvl + 1] = 0;
             if (done) continue;
-            if (R - L <= 64) {
+            if (R - L <= 64 || lvl == top) {  // the whole rest at this level (at most kTopLevelBoxes at the top)
                 wS[2 * lvl] = L;
                 wC[2 * lvl] = R - L;
                 done = true;
@@ -1401,10 +1404,13 @@ __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J, uint32_t chunk)
         }
         float b[6] = {part_lo.x, part_lo.y, part_lo.z, part_hi.x, part_hi.y, part_hi.z};
 #pragma unroll
-        for (int w = 0; w < 2 * kMaxBlockLevels; ++w)
+        for (int w = 0; w < 2 * kMaxBlockLevels; ++w) {
 #pragma unroll
             for (uint32_t e = 0; e < 64; e += TOP_LANES)
                 if (ql + e < wC[w]) box_absorb(lv.level[w / 2], wS[w] + ql + e, b);
+            for (uint32_t e = 64; e < wC[w]; e += TOP_LANES)  // a top-level window beyond 64 boxes
+                if (ql + e < wC[w]) box_absorb(lv.level[w / 2], wS[w] + ql + e, b);
+        }
 #pragma unroll
         for (int o = TOP_LANES / 2; o > 0; o >>= 1) {  // within the quarter
 #pragma unroll
@@ -1475,15 +1481,18 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
             off += 2 * (size_t)block_level_count(nn[t], k);
         }
     }
-    // level 1 (64-leaf blocks) was written by k_build_chunks. A union reads
-    // level k only where a range covers more than 64 boxes of level k - 1, so
-    // level k is built only when that level has more than 64 boxes.
+    // level 1 (64-leaf blocks) was written by k_build_chunks. Level k is built
+    // only when level k - 1 has more than kTopLevelBoxes boxes: k_top_boxes
+    // takes the rest of a range at the highest level built whole (2^20 leaves:
+    // levels 1 and 2, 16,384 and 256 boxes, one launch).
+    int top[2] = {1, 1};
     for (int k = 2; k < kMaxBlockLevels; ++k) {
         BlockBoxJobs bj;
         uint32_t nin[2], nout[2];
         for (int t = 0; t < 2; ++t) {
             nin[t] = block_level_count(nn[t], k - 1);
-            nout[t] = nin[t] > 64 ? block_level_count(nn[t], k) : 0;
+            nout[t] = nin[t] > kTopLevelBoxes ? block_level_count(nn[t], k) : 0;
+            if (nout[t]) top[t] = k;
         }
         if (nout[0] == 0 && nout[1] == 0) break;
         bj.in0 = lv[0].level[k - 1]; bj.out0 = (float4*)lv[0].level[k]; bj.n_in0 = nin[0]; bj.n_out0 = nout[0];
@@ -1497,6 +1506,7 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     tj.topmask0 = topmask[0]; tj.topmask1 = topmask[1];
     tj.nodes0 = j0->nodes; tj.nodes1 = o->nodes;
     tj.n0 = nn[0]; tj.n1 = nn[1];
+    tj.top0 = top[0]; tj.top1 = top[1];
     // one wave per TOP_GROUPS groups of 64 internal nodes (n - 1 of them); a
     // one-leaf tree has none
     auto top_waves = [](uint32_t m) { return m > 1 ? blocks_for(blocks_for(m - 1, 64), TOP_GROUPS) : 0u; };
