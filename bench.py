@@ -68,11 +68,12 @@ def kernel_bytes(name, st, ntA, ntB, nvA, nvB):
     launch of each per step; A's tree is the only tree built: B's triangles in
     Morton order are the traversal's queries)."""
     if name == "k_stage_mesh":
-        # both meshes in one launch. per vertex: source xyz 24 B read, double
-        # copy 32 B + integer copy 16 B written; per triangle: indices 12 B
-        # read (its vertices were read once above), box 32 B + Morton key and
+        # both meshes in one launch, neither moved (identity transform, source
+        # already on the integer grid): the launch writes no vertex copies and
+        # reads each source vertex (xyz 24 B) once through the triangles that
+        # use it; per triangle: indices 12 B read, box 32 B + Morton key and
         # id 8 B written, first-pass digit counts 1 KiB per 1024 triangles
-        return 72.0 * (nvA + nvB) + 53.0 * (ntA + ntB)
+        return 24.0 * (nvA + nvB) + 53.0 * (ntA + ntB)
     if name == "k_build_chunks":
         # per leaf of A: sorted key 4 B + triangle id 4 B + its FP32 box 32 B
         # read; leaf and internal node 32 B each + leaf range 8 B written

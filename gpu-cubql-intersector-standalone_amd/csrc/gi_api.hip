@@ -85,6 +85,9 @@ struct MeshSlot {
     int morton_axis = 10;      // Morton bits per axis of the kept sorted keys
     double src_lo[3] = {0.0, 0.0, 0.0}, src_hi[3] = {0.0, 0.0, 0.0};  // bounds of the source vertices
     int src_grid = -1;  // the source vertices are all integers within 2^26 (1), not (0), unknown (-1)
+    // the last staging wrote the moved copies (v64, vi); without a move (and
+    // without a snap that changes them) the kernels read the source vertices
+    bool moved_copy = true;
     uint64_t generation = 0;   // bumped whenever the slot's mesh or buffers are replaced
     DevBuf src, tri, v64, vi, tbox, keys, vals, tkeys, tvals, nodes, range, blocks;
     DevBuf samples;  // every 2^key_sample_shift(nt)-th sorted Morton key (multi-tile sorts only)
@@ -100,6 +103,7 @@ struct MeshSlot {
             else b->release();
         }
         if (borrowed) { borrowed = false; set = false; }
+        moved_copy = true;
         on_device = false;
         full_tree = false;
         built = false;
@@ -458,9 +462,14 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild, StageJob* job) {
     }
     const bool hist0 = topology && radix_sort_takes_hist0(m.nt);
     uint32_t* samp = hist0 ? m.samples.ensure<uint32_t>(key_sample_count(m.nt)) : nullptr;
-    *job = StageJob{m.src.get<double>(), m.nv, m.tri.get<uint32_t>(), m.nt, d_xf, v64, vi, grid + slot,
+    // an unmoved mesh (no transform, and no snap that would change a vertex)
+    // needs no moved copies: no vertex blocks, the grid flag is the upload's
+    // classification (set by the query's init), the kernels read m.src
+    m.moved_copy = !(x.identity && (!x.snap || m.src_grid == 1) && m.src_grid >= 0);
+    *job = StageJob{m.src.get<double>(), m.moved_copy ? m.nv : 0u, m.tri.get<uint32_t>(), m.nt, d_xf, v64, vi,
+                    grid + slot,
                     m.morton_axis, tbox, topology ? keys : nullptr, topology ? vals : nullptr,
-                    hist0 ? hist : nullptr};
+                    hist0 ? hist : nullptr, !m.moved_copy && m.src_grid == 0 ? 1 : 0};
     // the Morton sort runs after the staging, together with the other slot's
     if (topology)
         ctx->pending_sort[slot] = SortJobU32{keys, vals, tk, tv, m.nt, 3 * m.morton_axis, hist0, hist, stmp, samp};
@@ -568,8 +577,9 @@ MeshDev mesh_dev(const MeshSlot& m) {
     MeshDev d;
     d.nv = m.nv;
     d.nt = m.nt;
-    d.v64 = m.v64.get<double4>();
-    d.vi = m.vi.get<int4>();
+    d.src = m.src.get<double>();
+    d.v64 = m.moved_copy ? m.v64.get<double4>() : nullptr;
+    d.vi = m.moved_copy ? m.vi.get<int4>() : nullptr;
     d.tri = m.tri.get<uint32_t>();
     d.nodes = m.nodes.get<float4>();
     d.order = m.sorted_in_tmp ? m.tvals.get<uint32_t>() : m.vals.get<uint32_t>();
@@ -765,7 +775,8 @@ std::vector<uint64_t> presync_signature(gi_context* ctx, const gi_options& o) {
         v.push_back(m.nt);
         v.push_back(m.forest);
         v.push_back(((uint64_t)m.set << 0) | ((uint64_t)m.borrowed << 1) | ((uint64_t)m.on_device << 2) |
-                    ((uint64_t)m.built << 3) | ((uint64_t)m.dirty << 4) | ((uint64_t)m.sorted_in_tmp << 5));
+                    ((uint64_t)m.built << 3) | ((uint64_t)m.dirty << 4) | ((uint64_t)m.sorted_in_tmp << 5) |
+                    ((uint64_t)m.moved_copy << 6) | ((uint64_t)(m.src_grid + 1) << 7));
         v.push_back((uint64_t)m.morton_axis);
         v.push_back(m.generation);
         v.push_back(m.h_vtiles.size());
@@ -909,7 +920,7 @@ int finish_with_residual(gi_context* ctx, const gi_options& o, gi_result* out, u
     std::vector<uint32_t> gp(2 * (size_t)R), go((size_t)R + 1), list(2 * (size_t)Q);
     std::vector<uint64_t> gs(words * S);
     std::vector<int32_t> ge(S, 0);
-    std::vector<double> vA(4 * (size_t)MA.nv), vB(4 * (size_t)MB.nv);
+    std::vector<double> vA(4 * (size_t)MA.nv), vB(4 * (size_t)MB.nv), rawA, rawB;
     if (R) HIP_TRY(hipMemcpyAsync(gp.data(), pairs, 8 * (size_t)R, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(go.data(), offs, 4 * ((size_t)R + 1), hipMemcpyDeviceToHost, s));
     if (S) {
@@ -917,9 +928,26 @@ int finish_with_residual(gi_context* ctx, const gi_options& o, gi_result* out, u
         if (!st.grid) HIP_TRY(hipMemcpyAsync(ge.data(), exp2, 4 * (size_t)S, hipMemcpyDeviceToHost, s));
     }
     HIP_TRY(hipMemcpyAsync(list.data(), ctx->resid.get<uint2>(), 8 * (size_t)Q, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(vA.data(), MA.v64.get<double4>(), 32 * (size_t)MA.nv, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(vB.data(), MB.v64.get<double4>(), 32 * (size_t)MB.nv, hipMemcpyDeviceToHost, s));
+    // the vertices as the exact stage saw them: the moved copies, or the
+    // source of an unmoved mesh (widened to the copies' 4-double layout)
+    for (int k = 0; k < 2; ++k) {
+        const MeshSlot& mk = ctx->mesh[k];
+        std::vector<double>& v = k ? vB : vA;
+        std::vector<double>& raw = k ? rawB : rawA;
+        if (mk.moved_copy) {
+            HIP_TRY(hipMemcpyAsync(v.data(), mk.v64.get<double4>(), 32 * (size_t)mk.nv, hipMemcpyDeviceToHost, s));
+        } else {
+            raw.resize(3 * (size_t)mk.nv);
+            HIP_TRY(hipMemcpyAsync(raw.data(), mk.src.get<double>(), 24 * (size_t)mk.nv, hipMemcpyDeviceToHost, s));
+        }
+    }
     HIP_TRY(hipStreamSynchronize(s));
+    for (int k = 0; k < 2; ++k) {
+        const std::vector<double>& raw = k ? rawB : rawA;
+        std::vector<double>& v = k ? vB : vA;
+        for (size_t i = 0; 3 * i < raw.size(); ++i)
+            for (int c = 0; c < 3; ++c) v[4 * i + c] = raw[3 * i + c];
+    }
     ensure_host_copy(ctx, 0);
     ensure_host_copy(ctx, 1);
     std::vector<HostHit> hits;
@@ -1477,6 +1505,10 @@ int gi_get_vertices(gi_context* ctx, int slot, double* xyz, uint32_t nv) {
         HIP_TRY(hipSetDevice(ctx->device));
         build_meshes(ctx, false);
         sync_all(ctx);
+        if (!m.moved_copy) {  // unmoved: the source vertices are the ones the query uses
+            if (nv) HIP_TRY(hipMemcpy(xyz, m.src.get<double>(), 24 * (size_t)nv, hipMemcpyDeviceToHost));
+            return GI_OK;
+        }
         std::vector<double> v4(4 * (size_t)nv);
         if (nv)
             HIP_TRY(hipMemcpy(v4.data(), m.v64.get<double4>(), 32 * (size_t)nv, hipMemcpyDeviceToHost));
@@ -1557,6 +1589,7 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         d.xf = s.xf;
         for (int k = 0; k < 3; ++k) { d.src_lo[k] = s.src_lo[k]; d.src_hi[k] = s.src_hi[k]; }
         d.src_grid = s.src_grid;
+        d.moved_copy = s.moved_copy;
         d.sorted_in_tmp = s.sorted_in_tmp;
         d.morton_axis = s.morton_axis;
         d.full_tree = s.full_tree;

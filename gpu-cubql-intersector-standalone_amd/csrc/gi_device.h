@@ -72,8 +72,9 @@ struct Xform {
 
 struct MeshDev {
     uint32_t nv, nt;
-    const double4* v64;  // vertices used by the predicates
-    const int4* vi;      // integer copy (valid when the mesh is on the grid)
+    const double* src;   // source vertices (xyz), used when v64 is null (an unmoved mesh)
+    const double4* v64;  // moved vertices used by the predicates, or null
+    const int4* vi;      // their integer copy (valid when the mesh is on the grid), null with v64
     const uint32_t* tri; // 3 * nt
     const float4* nodes; // BVH, 2 float4 per node, 2*nt - 1 nodes (a walked mesh only)
     const uint32_t* order;  // triangle ids in Morton order (the sorted values)
@@ -126,6 +127,10 @@ struct StageJob {
     uint32_t* keys;
     uint32_t* vals;
     uint32_t* hist0;
+    // nv = 0 (no vertex blocks, no moved copies: an unmoved mesh) leaves the
+    // grid flag as the query's init set it (1); force_off_grid clears it
+    // (source vertices off the grid)
+    int force_off_grid;
 };
 // Up to two meshes staged in one launch.
 void launch_stage_meshes(const StageJob* jobs, int njobs, hipStream_t s);

@@ -423,6 +423,7 @@ __global__ __launch_bounds__(256) void k_stage_mesh(StageJobs J) {
     const uint32_t span = second ? J.tb1 : J.tb0;  // padded range of this mesh
     const uint32_t tile = xcd_block(b - t0, span);
     if (tile >= ntiles) return;  // padding
+    if (q.force_off_grid && tile == 0 && threadIdx.x == 0) *q.grid_flag = 0u;  // unmoved, source off the grid
     __shared__ uint32_t hs[4][RADIX];
     __shared__ float frame[6];  // Morton frame: lo[3], ext[3]
     if (q.hist0)
@@ -1554,6 +1555,20 @@ void launch_forest_roots(const uint2* range, const float4* nodes, uint32_t n, co
 // --------------------------------------------------------------- traversal
 
 
+// Vertex i of a mesh as the predicates see it: the moved copy, or for an
+// unmoved mesh (no copy written) the source doubles themselves; as integers
+// for grid meshes (exact: integer values within 2^26).
+__device__ __forceinline__ double4 mesh_vertex(const MeshDev& M, uint32_t i) {
+    if (M.v64) return M.v64[i];
+    const double* p = M.src + 3 * (size_t)i;
+    return make_double4(p[0], p[1], p[2], 0.0);
+}
+__device__ __forceinline__ int4 mesh_vertex_int(const MeshDev& M, uint32_t i) {
+    if (M.vi) return M.vi[i];
+    const double* p = M.src + 3 * (size_t)i;
+    return make_int4((int)p[0], (int)p[1], (int)p[2], 0);
+}
+
 __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t fill, uint2* cand, uint32_t cap,
                                             uint32_t* counters) {
     wave_sync_lds();
@@ -1902,13 +1917,13 @@ __global__ __launch_bounds__(256, 2) void k_filter(MeshDev A, MeshDev B, const u
             d3 PA[3], PB[3];
             if (grid) {
                 for (int v = 0; v < 3; ++v) {
-                    int4 p = A.vi[ta[v]], q = B.vi[tb[v]];
+                    int4 p = mesh_vertex_int(A, ta[v]), q = mesh_vertex_int(B, tb[v]);
                     PA[v] = d3{(double)p.x, (double)p.y, (double)p.z};
                     PB[v] = d3{(double)q.x, (double)q.y, (double)q.z};
                 }
             } else {
                 for (int v = 0; v < 3; ++v) {
-                    double4 p = A.v64[ta[v]], q = B.v64[tb[v]];
+                    double4 p = mesh_vertex(A, ta[v]), q = mesh_vertex(B, tb[v]);
                     PA[v] = d3{p.x, p.y, p.z};
                     PB[v] = d3{q.x, q.y, q.z};
                 }
@@ -2001,7 +2016,7 @@ __global__ __launch_bounds__(256) void k_exact(MeshDev A, MeshDev B, const uint2
             const uint32_t* ta = A.tri + 3 * (size_t)a;
             const uint32_t* tb = B.tri + 3 * (size_t)b;
             for (int v = 0; v < 3; ++v) {
-                int4 p = A.vi[ta[v]], q = B.vi[tb[v]];
+                int4 p = mesh_vertex_int(A, ta[v]), q = mesh_vertex_int(B, tb[v]);
                 PA[v].v[0] = p.x; PA[v].v[1] = p.y; PA[v].v[2] = p.z;
                 PB[v].v[0] = q.x; PB[v].v[1] = q.y; PB[v].v[2] = q.z;
             }
@@ -2219,7 +2234,7 @@ __global__ __launch_bounds__(256) void k_exact_grid_fast(MeshDev A, MeshDev B, c
             V3 PA[3], PB[3];
 #pragma unroll
             for (int v = 0; v < 3; ++v) {
-                const int4 p = A.vi[ta[v]], q = B.vi[tb[v]];
+                const int4 p = mesh_vertex_int(A, ta[v]), q = mesh_vertex_int(B, tb[v]);
                 PA[v].v[0] = p.x; PA[v].v[1] = p.y; PA[v].v[2] = p.z;
                 PB[v].v[0] = q.x; PB[v].v[1] = q.y; PB[v].v[2] = q.z;
             }
@@ -2308,7 +2323,7 @@ __global__ __launch_bounds__(256) void k_exact_grid_lanes(MeshDev A, MeshDev B, 
             const uint32_t* tb = B.tri + 3 * (size_t)b;
 #pragma unroll
             for (int v = 0; v < 3; ++v) {
-                const int4 p = A.vi[ta[v]], q = B.vi[tb[v]];
+                const int4 p = mesh_vertex_int(A, ta[v]), q = mesh_vertex_int(B, tb[v]);
                 PA[v].v[0] = p.x; PA[v].v[1] = p.y; PA[v].v[2] = p.z;
                 PB[v].v[0] = q.x; PB[v].v[1] = q.y; PB[v].v[2] = q.z;
             }
@@ -2480,7 +2495,7 @@ __global__ __launch_bounds__(256) void k_exact_wide(MeshDev A, MeshDev B, const 
             double pa[9], pb[9];
 #pragma unroll
             for (int v = 0; v < 3; ++v) {
-                const double4 p = A.v64[ta[v]], q = B.v64[tb[v]];
+                const double4 p = mesh_vertex(A, ta[v]), q = mesh_vertex(B, tb[v]);
                 pa[3 * v] = p.x; pa[3 * v + 1] = p.y; pa[3 * v + 2] = p.z;
                 pb[3 * v] = q.x; pb[3 * v + 1] = q.y; pb[3 * v + 2] = q.z;
             }
@@ -2578,7 +2593,7 @@ __global__ __launch_bounds__(256) void k_exact_wide_fast(MeshDev A, MeshDev B, c
             double pa[9], pb[9];
 #pragma unroll
             for (int v = 0; v < 3; ++v) {
-                const double4 p = A.v64[ta[v]], q = B.v64[tb[v]];
+                const double4 p = mesh_vertex(A, ta[v]), q = mesh_vertex(B, tb[v]);
                 pa[3 * v] = p.x; pa[3 * v + 1] = p.y; pa[3 * v + 2] = p.z;
                 pb[3 * v] = q.x; pb[3 * v + 1] = q.y; pb[3 * v + 2] = q.z;
             }

@@ -256,6 +256,7 @@ extern "C" int gi_intersect_noalloc(int32_t device, const double* xyzA, uint32_t
     for (int k = 0; k < 2; ++k) {
         D[k]->nv = nv[k];
         D[k]->nt = nt[k];
+        D[k]->src = M[k].src;
         D[k]->v64 = M[k].v64;
         D[k]->vi = M[k].vi;
         D[k]->tri = M[k].tri;
