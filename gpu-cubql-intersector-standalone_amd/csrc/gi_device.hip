@@ -2205,6 +2205,105 @@ __device__ __forceinline__ int transversal_segment(const typename E::V3 (&PA)[3]
     return cmp(lo, la ? xa0 : xb0, hi, ua ? xa1 : xb1) <= 0 ? 1 : 0;
 }
 
+// Lane (lane ^ 1)'s copy of a value through DPP quad permutes (no LDS
+// round trip): the two lanes of a pair trade what each computed. Both lanes
+// of every pair must be active at the call.
+__device__ __forceinline__ int pair_swap(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1, 0, 3, 2]
+}
+__device__ __forceinline__ uint64_t pair_swap(uint64_t x) {
+    const uint32_t lo = (uint32_t)pair_swap((int)(uint32_t)x), hi = (uint32_t)pair_swap((int)(uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ int64_t pair_swap(int64_t x) { return (int64_t)pair_swap((uint64_t)x); }
+__device__ __forceinline__ double pair_swap(double x) {
+    return __builtin_bit_cast(double, pair_swap(__builtin_bit_cast(uint64_t, x)));
+}
+__device__ __forceinline__ i128 pair_swap(i128 x) {
+    const u128 u = (u128)x;
+    return (i128)(((u128)pair_swap((uint64_t)(u >> 64)) << 64) | (u128)pair_swap((uint64_t)u));
+}
+__device__ __forceinline__ W256 pair_swap(const W256& x) {
+    W256 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r.w[i] = pair_swap(x.w[i]);
+    return r;
+}
+template <class E>
+__device__ __forceinline__ typename E::V3 pair_swap_v3(const typename E::V3& x) {
+    typename E::V3 r;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r.v[c] = pair_swap(x.v[c]);
+    return r;
+}
+template <class E>
+__device__ __forceinline__ typename E::HP pair_swap_hp(const typename E::HP& x) {
+    typename E::HP r;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r.h[c] = pair_swap(x.h[c]);
+    r.w = pair_swap(x.w);
+    return r;
+}
+
+// transversal_segment on two lanes: the even lane of the pair holds
+// triangle A (T), the odd lane triangle B. Each lane orients its own
+// vertices against the other triangle's plane (normal and first vertex
+// traded), builds its own triangle's segment on that plane and puts it in
+// order along L's axis (nA x nB and nB x nA give the same axis); the odd
+// lane's segment then goes to the even lane, whose return value, lo and hi
+// are those of transversal_segment for (A, B). The return value is -1 or 0
+// on both lanes alike when the early tests decide the pair; otherwise only
+// the even lane's result holds. npred as there.
+template <class E, class P>
+__device__ __forceinline__ int transversal_segment_pair(const typename E::V3 (&T)[3], typename E::HP& lo,
+                                                        typename E::HP& hi, uint32_t& npred) {
+    typedef typename E::V3 V3;
+    typedef typename E::HP HP;
+    typedef typename E::B B;
+    const V3 n = E::cross(E::sub(T[1], T[0]), E::sub(T[2], T[0]));
+    const V3 nO = pair_swap_v3<E>(n);
+    const V3 O0 = pair_swap_v3<E>(T[0]);
+    B o[3];
+    int s[3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        o[v] = E::dotB(nO, E::sub(T[v], O0));  // own vertices against the other plane
+        s[v] = P::sgnB(o[v]);
+    }
+    npred += 6;
+    const int code = (s[0] == 0 && s[1] == 0 && s[2] == 0) ? 2 : ((s[0] == s[1] && s[1] == s[2]) ? 1 : 0);
+    const int codeO = pair_swap(code);
+    if (code == 2 || codeO == 2) return -1;
+    if (code == 1 || codeO == 1) return 0;  // all on one side
+    HP p, q;
+    tri_plane_segment<E>(T[0], T[1], T[2], o[0], o[1], o[2], s[0], s[1], s[2], p, q);
+    // axis of the largest component of L's direction (n x nO = +-nA x nB)
+    const B d0 = P::mulCC(n.v[1], nO.v[2]) - P::mulCC(n.v[2], nO.v[1]);
+    const B d1 = P::mulCC(n.v[2], nO.v[0]) - P::mulCC(n.v[0], nO.v[2]);
+    const B d2 = P::mulCC(n.v[0], nO.v[1]) - P::mulCC(n.v[1], nO.v[0]);
+    int k = P::cmpAbsB(d1, d0) > 0 ? 1 : 0;
+    if (P::cmpAbsB(d2, k ? d1 : d0) > 0) k = 2;
+    auto xk = [&](const HP& Q) { return exact_to_f64(E::hcomp(Q, k)) / exact_to_f64(Q.w); };
+    auto cmp = [&](const HP& Q, double xq, const HP& R, double xr) {
+        int c = cmp_f64_certified(xq, xr);
+        if (c == 0) c = E::hpCmpAxis(Q, R, k);
+        return c;
+    };
+    npred += 5;
+    double xp = xk(p), xq = xk(q);
+    if (cmp(p, xp, q, xq) > 0) {  // own pair in order along k
+        const HP t = p; p = q; q = t;
+        const double x = xp; xp = xq; xq = x;
+    }
+    // the other lane's ordered pair (on the even lane: B's)
+    const HP pO = pair_swap_hp<E>(p), qO = pair_swap_hp<E>(q);
+    const double xpO = pair_swap(xp), xqO = pair_swap(xq);
+    const bool la = cmp(p, xp, pO, xpO) >= 0, ua = cmp(q, xq, qO, xqO) <= 0;
+    lo = la ? p : pO;
+    hi = ua ? q : qO;
+    return cmp(lo, la ? xp : xpO, hi, ua ? xq : xqO) <= 0 ? 1 : 0;
+}
+
 __global__ __launch_bounds__(256) void k_exact_grid_fast(MeshDev A, MeshDev B, const uint2* __restrict__ list,
                                                          uint32_t cap, const uint32_t* __restrict__ grid_flags,
                                                          int bitsB, uint64_t* __restrict__ rec_key,
@@ -2219,36 +2318,45 @@ __global__ __launch_bounds__(256) void k_exact_grid_fast(MeshDev A, MeshDev B, c
     uint32_t n = counters[CNT_EXACT_LIST];
     if (n > cap) n = cap;
     ExactTally tally;
-    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
-        const uint32_t i = base + threadIdx.x;
+    // two lanes per pair: the even lane holds triangle A and writes the results
+    const bool sideA = (threadIdx.x & 1u) == 0u;
+    for (uint32_t base = blockIdx.x * 128; base < n; base += gridDim.x * 128) {
+        const uint32_t i = base + (threadIdx.x >> 1);
         bool to_slow = false, hit = false;
         uint2 en = make_uint2(0, 0);
-        uint32_t a = 0, b = 0;
         HP lo, hi;
         if (i < n) {
             en = list[i];
-            a = en.x;
-            b = en.y & kIndexMask;
-            const uint32_t* ta = A.tri + 3 * (size_t)a;
-            const uint32_t* tb = B.tri + 3 * (size_t)b;
-            V3 PA[3], PB[3];
+            const uint32_t* t = sideA ? A.tri + 3 * (size_t)en.x : B.tri + 3 * (size_t)(en.y & kIndexMask);
+            const int4* vi = sideA ? A.vi : B.vi;
+            const double* src = sideA ? A.src : B.src;
+            V3 T[3];
 #pragma unroll
             for (int v = 0; v < 3; ++v) {
-                const int4 p = mesh_vertex_int(A, ta[v]), q = mesh_vertex_int(B, tb[v]);
-                PA[v].v[0] = p.x; PA[v].v[1] = p.y; PA[v].v[2] = p.z;
-                PB[v].v[0] = q.x; PB[v].v[1] = q.y; PB[v].v[2] = q.z;
+                const uint32_t j = t[v];
+                if (vi) {
+                    const int4 p = vi[j];
+                    T[v].v[0] = p.x; T[v].v[1] = p.y; T[v].v[2] = p.z;
+                } else {
+                    const double* p = src + 3 * (size_t)j;
+                    T[v].v[0] = (int64_t)p[0]; T[v].v[1] = (int64_t)p[1]; T[v].v[2] = (int64_t)p[2];
+                }
             }
-            const int r = transversal_segment<GridExact, GridPolicy>(PA, PB, lo, hi, tally.pred);
-            to_slow = r < 0;
-            hit = r > 0;
-            if (!to_slow) tally.add((int)(en.y >> kStatusShift), hit, 0, 0);
+            uint32_t np = 0;
+            const int r = transversal_segment_pair<GridExact, GridPolicy>(T, lo, hi, np);
+            to_slow = sideA && r < 0;
+            hit = sideA && r > 0;
+            if (sideA) {
+                tally.pred += np;
+                if (!to_slow) tally.add((int)(en.y >> kStatusShift), hit, 0, 0);
+            }
         }
         const uint32_t want[3] = {to_slow ? 1u : 0u, hit ? 1u : 0u, hit ? 1u : 0u};
         uint32_t got[3];
         block_reserve3(want, counters, CNT_SLOW, CNT_RESULTS, CNT_SEGMENTS, got);
         if (to_slow && got[0] < slow_cap) slow[got[0]] = en;
         if (hit && got[1] < rec_cap) {
-            rec_key[got[1]] = ((uint64_t)a << bitsB) | (uint64_t)b;
+            rec_key[got[1]] = ((uint64_t)en.x << bitsB) | (uint64_t)(en.y & kIndexMask);
             rec_seg[got[1]] = make_uint2(got[2], 1u);
             if (got[2] < seg_cap) {
                 __int128* o = seg_pool + 8 * (size_t)got[2];
@@ -2563,6 +2671,61 @@ __global__ __launch_bounds__(256) void k_exact_wide(MeshDev A, MeshDev B, const 
 // points moved back out of the frame. Pairs whose frame or end points do not
 // fit go to the host residual list; coplanar and degenerate pairs to the slow
 // list (k_exact_wide<true> on it).
+// wide_frame on the two lanes of a pair, each with its own triangle's 9
+// doubles (t, vertex-major; the even lane triangle A): the lanes trade the
+// lowest exponent, the widest coordinate, A's vertex 0 (the translation)
+// and the widest difference, so both come to the frame wide_frame gives for
+// all 18 coordinates, and T holds the lane's own triangle in it.
+__device__ __forceinline__ bool wide_frame_pair(const double (&t)[9], bool sideA, WideExact::V3 (&T)[3],
+                                                WideFrame& f) {
+    int64_t m[9];
+    int q[9];
+    int ok = 1, emin = 0x7FFFFFFF;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        if (!dyadic_parts(t[i], m[i], q[i])) {
+            ok = 0;
+            m[i] = 0;
+            q[i] = 0;
+        }
+        if (m[i] != 0 && q[i] < emin) emin = q[i];
+    }
+    emin = min(emin, pair_swap(emin));
+    if (emin == 0x7FFFFFFF) emin = 0;  // all zero
+    i128 X[9];
+    int kmax = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        X[i] = 0;
+        if (m[i] == 0) continue;
+        const uint64_t um = m[i] < 0 ? (uint64_t)0 - (uint64_t)m[i] : (uint64_t)m[i];
+        const int top = 63 - __builtin_clzll(um) + q[i] - emin;  // position of the top bit of |X|
+        if (top > 125) {
+            ok = 0;
+            continue;
+        }
+        X[i] = (i128)m[i] * ((i128)1 << (q[i] - emin));
+        kmax = top + 1 > kmax ? top + 1 : kmax;
+    }
+    kmax = max(kmax, pair_swap(kmax));
+    f.exp2 = emin;
+    f.shifted = kmax > kWideCoordBits;
+    const i128 x0O[3] = {pair_swap(X[0]), pair_swap(X[1]), pair_swap(X[2])};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) f.x0[k] = f.shifted ? (sideA ? X[k] : x0O[k]) : (i128)0;
+    int k1 = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const i128 d = X[i] - f.x0[i % 3];
+        const int bt = bits128(d);
+        k1 = bt > k1 ? bt : k1;
+        T[i / 3].v[i % 3] = d;
+    }
+    k1 = max(k1, pair_swap(k1));
+    ok = ok & pair_swap(ok);
+    return ok && k1 <= kWideCoordBits;
+}
+
 __global__ __launch_bounds__(256) void k_exact_wide_fast(MeshDev A, MeshDev B, const uint2* __restrict__ list,
                                                          uint32_t cap, const uint32_t* __restrict__ grid_flags,
                                                          int bitsB, uint64_t* __restrict__ rec_key,
@@ -2575,43 +2738,51 @@ __global__ __launch_bounds__(256) void k_exact_wide_fast(MeshDev A, MeshDev B, c
     uint32_t n = counters[CNT_EXACT_LIST];
     if (n > cap) n = cap;
     ExactTally tally;
-    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
-        const uint32_t i = base + threadIdx.x;
+    // two lanes per pair: the even lane holds triangle A and writes the results
+    const bool sideA = (threadIdx.x & 1u) == 0u;
+    for (uint32_t base = blockIdx.x * 128; base < n; base += gridDim.x * 128) {
+        const uint32_t i = base + (threadIdx.x >> 1);
         bool residual = false, to_slow = false, hit = false;
         uint2 en = make_uint2(0, 0);
-        uint32_t a = 0, b = 0;
         WideExact::HP lo, hi;
         WideFrame fr;
         fr.exp2 = 0;
         fr.shifted = 0;
         if (i < n) {
             en = list[i];
-            a = en.x;
-            b = en.y & kIndexMask;
-            const uint32_t* ta = A.tri + 3 * (size_t)a;
-            const uint32_t* tb = B.tri + 3 * (size_t)b;
-            double pa[9], pb[9];
+            const uint32_t* tr = sideA ? A.tri + 3 * (size_t)en.x : B.tri + 3 * (size_t)(en.y & kIndexMask);
+            const double4* v64 = sideA ? A.v64 : B.v64;
+            const double* src = sideA ? A.src : B.src;
+            double t[9];
 #pragma unroll
             for (int v = 0; v < 3; ++v) {
-                const double4 p = mesh_vertex(A, ta[v]), q = mesh_vertex(B, tb[v]);
-                pa[3 * v] = p.x; pa[3 * v + 1] = p.y; pa[3 * v + 2] = p.z;
-                pb[3 * v] = q.x; pb[3 * v + 1] = q.y; pb[3 * v + 2] = q.z;
+                const uint32_t j = tr[v];
+                if (v64) {
+                    const double4 p = v64[j];
+                    t[3 * v] = p.x; t[3 * v + 1] = p.y; t[3 * v + 2] = p.z;
+                } else {
+                    const double* p = src + 3 * (size_t)j;
+                    t[3 * v] = p[0]; t[3 * v + 1] = p[1]; t[3 * v + 2] = p[2];
+                }
             }
-            WideExact::V3 PA[3], PB[3];
-            residual = !wide_frame(pa, pb, PA, PB, fr);
+            WideExact::V3 T[3];
+            residual = !wide_frame_pair(t, sideA, T, fr);
             if (!residual) {
                 uint32_t np = 0;
-                const int r = transversal_segment<WideExact, WidePolicy>(PA, PB, lo, hi, np);
+                const int r = transversal_segment_pair<WideExact, WidePolicy>(T, lo, hi, np);
                 to_slow = r < 0;
                 hit = r > 0;
-                if (hit) residual = !wide_absolute(fr, lo) || !wide_absolute(fr, hi);
+                if (hit && sideA) residual = !wide_absolute(fr, lo) || !wide_absolute(fr, hi);
                 if (residual) {
                     hit = false;
-                } else {
+                } else if (sideA) {
                     tally.pred += np;
                     if (!to_slow) tally.add((int)(en.y >> kStatusShift), hit, 0, 0);
                 }
             }
+            residual = residual && sideA;
+            to_slow = to_slow && sideA;
+            hit = hit && sideA;
         }
         const uint32_t rslot = block_reserve(residual ? 1u : 0u, counters, CNT_RESIDUAL);
         if (residual && rslot < resid_cap) resid[rslot] = en;
@@ -2620,7 +2791,7 @@ __global__ __launch_bounds__(256) void k_exact_wide_fast(MeshDev A, MeshDev B, c
         block_reserve3(want, counters, CNT_SLOW, CNT_RESULTS, CNT_SEGMENTS, got);
         if (to_slow && got[0] < slow_cap) slow[got[0]] = en;
         if (hit && got[1] < rec_cap) {
-            rec_key[got[1]] = ((uint64_t)a << bitsB) | (uint64_t)b;
+            rec_key[got[1]] = ((uint64_t)en.x << bitsB) | (uint64_t)(en.y & kIndexMask);
             rec_seg[got[1]] = make_uint2(got[2], 1u);
             if (got[2] < seg_cap) {
                 uint64_t* o = seg_pool + (size_t)8 * kSegLimbsWide * got[2];
