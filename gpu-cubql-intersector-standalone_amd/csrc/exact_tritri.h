@@ -44,6 +44,7 @@ struct GridPolicy {
     static GI_HD int cmpH(const H& a, const H& b) { return cmp256(a, b); }
     static GI_HD int sgnC(C x) { return sgn64(x); }
     static GI_HD int sgnB(B x) { return sgn128(x); }
+    static GI_HD B pick3(int k, const B& a, const B& b, const B& c) { return k == 0 ? a : (k == 1 ? b : c); }
     static GI_HD int cmpAbsC(C a, C b) {
         uint64_t ua = a < 0 ? (uint64_t)0 - (uint64_t)a : (uint64_t)a;
         uint64_t ub = b < 0 ? (uint64_t)0 - (uint64_t)b : (uint64_t)b;
@@ -68,6 +69,16 @@ struct WidePolicy {
     static GI_HD int cmpH(const H& a, const H& b) { return m512_cmp(a, b); }
     static GI_HD int sgnC(C x) { return sgn128(x); }
     static GI_HD int sgnB(const B& x) { return w256_sgn(x); }
+    // one of three values by a run-time index, limb by limb through masks: a
+    // select between whole structs is folded into an indexed access of a
+    // per-lane array in scratch memory
+    static GI_HD B pick3(int k, const B& a, const B& b, const B& c) {
+        const uint64_t ma = k == 0 ? ~0ull : 0ull, mb = k == 1 ? ~0ull : 0ull, mc = k == 2 ? ~0ull : 0ull;
+        B r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r.w[i] = (a.w[i] & ma) | (b.w[i] & mb) | (c.w[i] & mc);
+        return r;
+    }
     static GI_HD int cmpAbsC(C a, C b) {
         u128 ua = uabs128(a), ub = uabs128(b);
         return (ua > ub) - (ua < ub);
@@ -171,7 +182,7 @@ struct ExactTT {
         }
         return r;
     }
-    static GI_HD B hcomp(const HP& a, int k) { return k == 0 ? a.h[0] : (k == 1 ? a.h[1] : a.h[2]); }
+    static GI_HD B hcomp(const HP& a, int k) { return P::pick3(k, a.h[0], a.h[1], a.h[2]); }
     static GI_HD int hpCmpAxis(const HP& a, const HP& b, int k) {
         return P::cmpH(P::mulBB(hcomp(a, k), b.w), P::mulBB(hcomp(b, k), a.w));
     }

@@ -162,6 +162,7 @@ struct BigPolicy {
     static int sgnB(const N& x) { return x.sign; }
     static int cmpAbsC(const N& a, const N& b) { return N::cmpMag(a, b); }
     static int cmpAbsB(const N& a, const N& b) { return N::cmpMag(a, b); }
+    static const N& pick3(int k, const N& a, const N& b, const N& c) { return k == 0 ? a : (k == 1 ? b : c); }
 };
 
 // Capacities: 64 limbs serve every pair whose coordinates span up to ~560
