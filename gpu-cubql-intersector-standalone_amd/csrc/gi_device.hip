@@ -2982,9 +2982,11 @@ void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const u
 // path after the read-back). The multi-tile path spends 20-odd launches on a
 // few thousand records; a single workgroup is bound by the VALU of one CU
 // (the ballot-match ranking costs ~8 wave64 instructions per key bit).
-//   k_output_runs   kRuns workgroups, each an LSD radix sort of kRunLen
-//                   records in LDS (ranking as in k_sort_small: stable,
-//                   wave ballot match, per-wave digit counts);
+//   k_output_runs   kRuns workgroups, each a bitonic sort of kRunLen
+//                   records, two per thread: 56 of its 66 compare steps
+//                   between lanes of one wave (DPP, swizzles), 10 through
+//                   LDS (an LSD radix sort with ballot-match ranking took
+//                   ~2 us per 8-bit pass, 10.4 us for 40-bit keys);
 //                   and, per run position, the exclusive prefix of the
 //                   segment counts in run order (cpre, run totals after it);
 //   k_output_place  every record's place is its position in its run plus,
@@ -2992,16 +2994,16 @@ void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const u
 //                   smaller (binary searches in LDS, the runs' searches
 //                   interleaved), and its segment offset the sum of the
 //                   runs' count prefixes at those positions: one thread per
-//                   record, 16 workgroups (a single merging workgroup took
+//                   record, 32 workgroups (a single merging workgroup took
 //                   23 us for 10^4 records); writes per place the pair, the
 //                   record index (order) and the offset (total at offs[R]).
 // Each key (faceA << bitsB | faceB) carries its record index in the low
-// idx_bits, so the passes move no separate values (keys are distinct).
+// idx_bits, so the sort moves no separate values (keys are distinct).
 constexpr int RUN_THREADS = 1024, RUN_ITEMS = 2;
 constexpr uint32_t kRunLen = RUN_THREADS * RUN_ITEMS;
 constexpr uint32_t kRuns = 8;
 constexpr uint32_t kOutSmallMax = kRunLen * kRuns;
-constexpr int PLACE_THREADS = 1024;
+constexpr int PLACE_THREADS = 512;
 
 __host__ __device__ __forceinline__ int bits_of_count(uint32_t n) { return n <= 1 ? 1 : 32 - __builtin_clz(n - 1); }
 
@@ -3010,8 +3012,91 @@ __device__ __forceinline__ bool output_small_ok(const uint32_t* counters, uint32
                                                 int key_bits, uint32_t& R) {
     R = counters[CNT_RESULTS];
     return R <= cap_rec && counters[CNT_SEGMENTS] <= cap_seg && R <= kOutSmallMax &&
-           key_bits + bits_of_count(R) <= 64;
+           key_bits + bits_of_count(R) <= 63;
 }
+
+// 64-bit value of lane (lane ^ M) of the wave: DPP quad permutes for
+// M = 1, 2, LDS-free swizzles within 32 lanes for M = 4 .. 16, a permute
+// through the LDS crossbar for 32.
+template <int M>
+__device__ __forceinline__ uint64_t lane_xor_u64(uint64_t x) {
+    int lo = (int)(uint32_t)x, hi = (int)(uint32_t)(x >> 32);
+    if constexpr (M == 1) {
+        lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);  // quad_perm [1, 0, 3, 2]
+        hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
+    } else if constexpr (M == 2) {
+        lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);  // quad_perm [2, 3, 0, 1]
+        hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
+    } else if constexpr (M < 32) {
+        lo = __builtin_amdgcn_ds_swizzle(lo, (M << 10) | 0x1F);  // bit mode: lane ^ M within 32
+        hi = __builtin_amdgcn_ds_swizzle(hi, (M << 10) | 0x1F);
+    } else {
+        lo = __shfl_xor(lo, M);
+        hi = __shfl_xor(hi, M);
+    }
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Bitonic sort of a run of kRunLen keys, thread t holding positions 2t and
+// 2t + 1 (k0, k1). Merge K, step J compares positions i and i ^ J, keeping
+// the smaller one at the lower position where i & K is zero (ascending
+// blocks) and the larger one elsewhere. J = 1 is within a thread, J = 2 ..
+// 64 between lanes of a wave (lane distance J / 2), J >= 128 between waves
+// through LDS (s), one pair per thread.
+template <int K, int J>
+__device__ __forceinline__ void run_bitonic_step(uint64_t& k0, uint64_t& k1, uint64_t* s, uint32_t t) {
+    if constexpr (J >= 128) {
+        const uint32_t i = ((t & ~(uint32_t)(J - 1)) << 1) | (t & (uint32_t)(J - 1));
+        const bool asc = (i & (uint32_t)K) == 0;
+        const uint64_t a = s[i], b = s[i + J];
+        const bool swap = asc ? a > b : a < b;
+        s[i] = swap ? b : a;
+        s[i + J] = swap ? a : b;
+        __syncthreads();
+    } else if constexpr (J >= 2) {
+        const uint64_t p0 = lane_xor_u64<J / 2>(k0), p1 = lane_xor_u64<J / 2>(k1);
+        const bool keep_min = ((t & (uint32_t)(J / 2)) == 0) == ((t & (uint32_t)(K / 2)) == 0);
+        k0 = keep_min ? (k0 < p0 ? k0 : p0) : (k0 < p0 ? p0 : k0);
+        k1 = keep_min ? (k1 < p1 ? k1 : p1) : (k1 < p1 ? p1 : k1);
+    } else {
+        const bool asc = (t & (uint32_t)(K / 2)) == 0;
+        const uint64_t lo = k0 < k1 ? k0 : k1, hi = k0 < k1 ? k1 : k0;
+        k0 = asc ? lo : hi;
+        k1 = asc ? hi : lo;
+    }
+}
+template <int K, int J>
+struct RunBitonicMerge {
+    __device__ static __forceinline__ void run(uint64_t& k0, uint64_t& k1, uint64_t* s, uint32_t t) {
+        if constexpr (J == K / 2 && J >= 128) {  // into LDS for the cross-wave steps
+            s[2 * t] = k0;
+            s[2 * t + 1] = k1;
+            __syncthreads();
+        }
+        run_bitonic_step<K, J>(k0, k1, s, t);
+        if constexpr (J == 128) {  // back to registers
+            k0 = s[2 * t];
+            k1 = s[2 * t + 1];
+            __syncthreads();
+        }
+        RunBitonicMerge<K, J / 2>::run(k0, k1, s, t);
+    }
+};
+template <int K>
+struct RunBitonicMerge<K, 0> {
+    __device__ static __forceinline__ void run(uint64_t&, uint64_t&, uint64_t*, uint32_t) {}
+};
+template <int K>
+struct RunBitonicSort {
+    __device__ static __forceinline__ void run(uint64_t& k0, uint64_t& k1, uint64_t* s, uint32_t t) {
+        RunBitonicSort<K / 2>::run(k0, k1, s, t);
+        RunBitonicMerge<K, K / 2>::run(k0, k1, s, t);
+    }
+};
+template <>
+struct RunBitonicSort<1> {
+    __device__ static __forceinline__ void run(uint64_t&, uint64_t&, uint64_t*, uint32_t) {}
+};
 
 __global__ __launch_bounds__(RUN_THREADS) void k_output_runs(const uint64_t* __restrict__ rec_key,
                                                              const uint2* __restrict__ rec_seg,
@@ -3019,6 +3104,7 @@ __global__ __launch_bounds__(RUN_THREADS) void k_output_runs(const uint64_t* __r
                                                              uint32_t cap_seg, int key_bits, int want_segments,
                                                              uint64_t* __restrict__ runs,
                                                              uint32_t* __restrict__ cpre) {
+    static_assert(RUN_ITEMS == 2 && kRunLen == 2 * RUN_THREADS, "two keys per thread");
     uint32_t R;
     if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R)) return;
     const uint32_t r0 = blockIdx.x * kRunLen;
@@ -3026,92 +3112,22 @@ __global__ __launch_bounds__(RUN_THREADS) void k_output_runs(const uint64_t* __r
     const uint32_t n = min(kRunLen, R - r0);
     const int idx_bits = bits_of_count(R);
     __shared__ uint64_t skey[kRunLen];
-    __shared__ uint32_t wc[RUN_THREADS / 64][RADIX];
-    __shared__ uint32_t base[RADIX];
-    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    const uint64_t lt = lanemask_lt();
-    const uint32_t seg = wave * 64 * RUN_ITEMS + lane;
-    uint64_t key[RUN_ITEMS];
-    uint32_t rank[RUN_ITEMS];
-#pragma unroll
-    for (int it = 0; it < RUN_ITEMS; ++it) {
-        const uint32_t i = seg + 64 * it;
-        key[it] = i < n ? (rec_key[r0 + i] << idx_bits) | (r0 + i) : 0ull;
-    }
-    const int passes = (key_bits + 7) / 8;
-    for (int p = 0; p < passes; ++p) {
-        const int shift = idx_bits + 8 * p;
-        for (int d = threadIdx.x; d < (RUN_THREADS / 64) * RADIX; d += RUN_THREADS) (&wc[0][0])[d] = 0;
-        __syncthreads();
-#pragma unroll
-        for (int it = 0; it < RUN_ITEMS; ++it) {
-            const bool valid = seg + 64 * it < n;
-            const uint32_t d = (uint32_t)(key[it] >> shift) & (RADIX - 1);
-            uint64_t match = __ballot(valid);
-            for (int b = 0; b < 8; ++b) {
-                uint64_t bb = __ballot((d >> b) & 1u);
-                match &= ((d >> b) & 1u) ? bb : ~bb;
-            }
-            const uint32_t peers = (uint32_t)__popcll(match & lt);
-            const uint32_t cur = wc[wave][d];
-            rank[it] = cur + peers;
-            wave_sync_lds();
-            if (valid && peers == 0) wc[wave][d] = cur + (uint32_t)__popcll(match);
-            wave_sync_lds();
-        }
-        __syncthreads();
-        if (threadIdx.x < RADIX) {  // per digit: exclusive prefix over the waves, total to base
-            uint32_t run = 0;
-            for (int w = 0; w < RUN_THREADS / 64; ++w) {
-                const uint32_t t = wc[w][threadIdx.x];
-                wc[w][threadIdx.x] = run;
-                run += t;
-            }
-            base[threadIdx.x] = run;
-        }
-        __syncthreads();
-        if (threadIdx.x < 64) {  // exclusive scan of the 256 digit totals by one wave
-            uint32_t c[4], sum = 0;
-            for (int q = 0; q < 4; ++q) { c[q] = base[4 * threadIdx.x + q]; sum += c[q]; }
-            uint32_t x = sum;
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(x, o);
-                if (lane >= (uint32_t)o) x += y;
-            }
-            uint32_t off = x - sum;
-            for (int q = 0; q < 4; ++q) { base[4 * threadIdx.x + q] = off; off += c[q]; }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int it = 0; it < RUN_ITEMS; ++it) {
-            if (seg + 64 * it >= n) continue;
-            const uint32_t d = (uint32_t)(key[it] >> shift) & (RADIX - 1);
-            skey[base[d] + wc[wave][d] + rank[it]] = key[it];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int it = 0; it < RUN_ITEMS; ++it)
-            if (seg + 64 * it < n) key[it] = skey[seg + 64 * it];
-    }
-#pragma unroll
-    for (int it = 0; it < RUN_ITEMS; ++it)
-        if (seg + 64 * it < n) runs[r0 + seg + 64 * it] = key[it];
-    // segment counts in run order, then their exclusive prefix: thread t
-    // scans positions RUN_ITEMS t .. RUN_ITEMS t + RUN_ITEMS - 1
-    __shared__ uint32_t cnt[kRunLen];
     __shared__ uint32_t wsum[RUN_THREADS / 64];
+    const uint32_t t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    // positions past n hold the largest key, which sorts them last (real
+    // keys stay below 2^63: output_small_ok)
+    const uint32_t q0 = 2 * t;
+    uint64_t k0 = q0 < n ? (rec_key[r0 + q0] << idx_bits) | (r0 + q0) : ~0ull;
+    uint64_t k1 = q0 + 1 < n ? (rec_key[r0 + q0 + 1] << idx_bits) | (r0 + q0 + 1) : ~0ull;
+    RunBitonicSort<kRunLen>::run(k0, k1, skey, t);
+    if (q0 < n) runs[r0 + q0] = k0;
+    if (q0 + 1 < n) runs[r0 + q0 + 1] = k1;
+    // segment counts in run order, then their exclusive prefix
     const uint64_t idx_mask = (1ull << idx_bits) - 1ull;
-#pragma unroll
-    for (int it = 0; it < RUN_ITEMS; ++it)
-        if (seg + 64 * it < n) cnt[seg + 64 * it] = want_segments ? rec_seg[(uint32_t)(key[it] & idx_mask)].y : 0u;
-    __syncthreads();
-    const uint32_t q0 = threadIdx.x * RUN_ITEMS;
-    uint32_t c[RUN_ITEMS], sum = 0;
-#pragma unroll
-    for (int e = 0; e < RUN_ITEMS; ++e) {
-        c[e] = q0 + e < n ? cnt[q0 + e] : 0u;
-        sum += c[e];
-    }
+    uint32_t c[RUN_ITEMS];
+    c[0] = want_segments && q0 < n ? rec_seg[(uint32_t)(k0 & idx_mask)].y : 0u;
+    c[1] = want_segments && q0 + 1 < n ? rec_seg[(uint32_t)(k1 & idx_mask)].y : 0u;
+    const uint32_t sum = c[0] + c[1];
     uint32_t x = sum;
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t y = __shfl_up(x, o);
@@ -3124,12 +3140,9 @@ __global__ __launch_bounds__(RUN_THREADS) void k_output_runs(const uint64_t* __r
         before += w < wave ? wsum[w] : 0u;
         all += wsum[w];
     }
-#pragma unroll
-    for (int e = 0; e < RUN_ITEMS; ++e) {
-        if (q0 + e < n) cpre[r0 + q0 + e] = before;
-        before += c[e];
-    }
-    if (threadIdx.x == 0) cpre[kOutSmallMax + blockIdx.x] = all;
+    if (q0 < n) cpre[r0 + q0] = before;
+    if (q0 + 1 < n) cpre[r0 + q0 + 1] = before + c[0];
+    if (t == 0) cpre[kOutSmallMax + blockIdx.x] = all;
 }
 
 __global__ __launch_bounds__(PLACE_THREADS) void k_output_place(const uint64_t* __restrict__ runs,
@@ -3154,7 +3167,14 @@ __global__ __launch_bounds__(PLACE_THREADS) void k_output_place(const uint64_t* 
     if (blockIdx.x * PLACE_THREADS >= R) return;
     const int idx_bits = bits_of_count(R);
     __shared__ uint64_t sk[kOutSmallMax];
-    for (uint32_t i = threadIdx.x; i < R; i += PLACE_THREADS) sk[i] = runs[i];
+    {  // two keys per 16-byte load (runs is 256-byte aligned; R <= kOutSmallMax)
+        const uint4* __restrict__ runs4 = (const uint4*)runs;
+        for (uint32_t i = threadIdx.x; 2 * i < R; i += PLACE_THREADS) {
+            const uint4 v = runs4[i];
+            sk[2 * i] = ((uint64_t)v.y << 32) | v.x;
+            if (2 * i + 1 < R) sk[2 * i + 1] = ((uint64_t)v.w << 32) | v.z;
+        }
+    }
     __syncthreads();
     if (g >= R) return;
     const uint64_t x = sk[g];
@@ -3208,7 +3228,7 @@ uint32_t output_small_max() { return kOutSmallMax; }
 
 size_t output_small_runs_elems() { return kOutSmallMax + (kOutSmallMax + kRuns + 1) / 2; }
 
-bool output_small_fits(uint32_t R, int key_bits) { return R <= kOutSmallMax && key_bits + bits_of_count(R) <= 64; }
+bool output_small_fits(uint32_t R, int key_bits) { return R <= kOutSmallMax && key_bits + bits_of_count(R) <= 63; }
 
 void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const uint32_t* counters, uint32_t cap_rec,
                          uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint64_t* runs, uint2* pairs,
