@@ -90,14 +90,12 @@ struct MeshSlot {
     bool moved_copy = true;
     uint64_t generation = 0;   // bumped whenever the slot's mesh or buffers are replaced
     DevBuf src, tri, v64, vi, tbox, keys, vals, tkeys, tvals, nodes, range, blocks;
-    DevBuf samples;  // every 2^key_sample_shift(nt)-th sorted Morton key (multi-tile sorts only)
     DevBuf voff, toff, vtiles, ttiles, vtile0, fbounds, roots;
     void release() {
         ++generation;
         DevBuf* forest_bufs[] = {&voff, &toff, &vtiles, &ttiles, &vtile0, &fbounds, &roots};
         for (DevBuf* b : forest_bufs) b->release();
-        DevBuf* all[] = {&src, &tri, &v64, &vi, &tbox, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks,
-                         &samples};
+        DevBuf* all[] = {&src, &tri, &v64, &vi, &tbox, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks};
         for (DevBuf* b : all) {
             if (borrowed) { b->p = nullptr; b->bytes = 0; }
             else b->release();
@@ -406,10 +404,8 @@ void stage_forest(gi_context* ctx, int slot) {
                             grid + slot, bounds, m.morton_axis, tbox, keys, vals, s);
     uint32_t* hist = ctx->bhist[slot].ensure<uint32_t>(sort_hist_elems(nt));
     uint32_t* stmp = ctx->bscan[slot].ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
-    uint32_t* samp = radix_sort_takes_hist0(m.nt) ? m.samples.ensure<uint32_t>(key_sample_count(m.nt)) : nullptr;
     // the sort runs after the join, together with the other slot's
-    ctx->pending_sort[slot] =
-        SortJobU32{keys, vals, tk, tv, m.nt, 3 * m.morton_axis + bits_for(M), false, hist, stmp, samp};
+    ctx->pending_sort[slot] = SortJobU32{keys, vals, tk, tv, m.nt, 3 * m.morton_axis + bits_for(M), false, hist, stmp};
 }
 
 // The part of a mesh's build before the sorts: vertices (transform) ->
@@ -461,7 +457,6 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild, StageJob* job) {
         stmp = ctx->bscan[slot].ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
     }
     const bool hist0 = topology && radix_sort_takes_hist0(m.nt);
-    uint32_t* samp = hist0 ? m.samples.ensure<uint32_t>(key_sample_count(m.nt)) : nullptr;
     // an unmoved mesh (no transform, and no snap that would change a vertex)
     // needs no moved copies: no vertex blocks, the grid flag is the upload's
     // classification (set by the query's init), the kernels read m.src
@@ -472,7 +467,7 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild, StageJob* job) {
                     hist0 ? hist : nullptr, !m.moved_copy && m.src_grid == 0 ? 1 : 0};
     // the Morton sort runs after the staging, together with the other slot's
     if (topology)
-        ctx->pending_sort[slot] = SortJobU32{keys, vals, tk, tv, m.nt, 3 * m.morton_axis, hist0, hist, stmp, samp};
+        ctx->pending_sort[slot] = SortJobU32{keys, vals, tk, tv, m.nt, 3 * m.morton_axis, hist0, hist, stmp};
     return true;
 }
 
@@ -486,8 +481,6 @@ BuildJob<K> slot_job(MeshSlot& m) {
     j.nodes = m.nodes.get<float4>();
     j.range = m.range.get<uint2>();
     j.blocks = m.blocks.get<float4>();
-    // the samples the last topology sort wrote stay valid for a refit (same keys)
-    j.samples = radix_sort_takes_hist0(m.nt) ? m.samples.get<K>() : nullptr;
     return j;
 }
 
@@ -1595,9 +1588,9 @@ int gi_share_mesh(gi_context* dst, int dst_slot, gi_context* src, int src_slot) 
         d.full_tree = s.full_tree;
         d.borrowed = true;
         DevBuf* dd[] = {&d.src, &d.tri, &d.v64, &d.vi, &d.tbox, &d.keys, &d.vals, &d.tkeys, &d.tvals, &d.nodes,
-                        &d.range, &d.blocks, &d.samples};
+                        &d.range, &d.blocks};
         const DevBuf* ss[] = {&s.src, &s.tri, &s.v64, &s.vi, &s.tbox, &s.keys, &s.vals, &s.tkeys, &s.tvals,
-                              &s.nodes, &s.range, &s.blocks, &s.samples};
+                              &s.nodes, &s.range, &s.blocks};
         static_assert(sizeof(dd) / sizeof(dd[0]) == sizeof(ss) / sizeof(ss[0]), "buffer lists differ");
         for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); ++i) { dd[i]->p = ss[i]->p; dd[i]->bytes = ss[i]->bytes; }
         // the owner's grid classification of this mesh

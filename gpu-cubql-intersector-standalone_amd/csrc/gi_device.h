@@ -157,16 +157,8 @@ void launch_tri_boxes_forest(const uint32_t* tri, const uint2* ttiles, uint32_t 
 size_t sort_hist_elems(size_t n);
 size_t scan_tmp_elems(size_t n);
 // whether a sort of n uint32 keys runs the multi-tile passes (and so can take
-// the first pass's counts from launch_stage_mesh, and writes key samples)
+// the first pass's counts from launch_stage_mesh)
 bool radix_sort_takes_hist0(uint32_t n);
-// Key samples: the multi-tile sort's last pass also writes every
-// 2^key_sample_shift(n)-th sorted key, key_sample_count(n) <= kMaxKeySamples
-// of them; the BVH build searches them before the keys themselves when a
-// node's leaf range leaves its workgroup's chunk. Sorts that run in one
-// workgroup (radix_sort_takes_hist0(n) false) write none.
-constexpr uint32_t kMaxKeySamples = 4096;
-int key_sample_shift(uint32_t n);
-uint32_t key_sample_count(uint32_t n);
 // Two meshes' Morton sorts at once: each multi-tile pass of both runs in the
 // same launches. in_tmp[j] = 1 when sort j's result ends in (tkeys, tvals).
 struct SortJobU32 {
@@ -176,7 +168,6 @@ struct SortJobU32 {
     bool hist0_ready;
     uint32_t* hist;      // sort_hist_elems(n)
     uint32_t* scan_tmp;  // scan_tmp_elems(sort_hist_elems(n))
-    uint32_t* samples;   // key_sample_count(n) key samples, or null
 };
 void radix_sort_u32_pair(const SortJobU32* jobs, int njobs, hipStream_t s, int* in_tmp);
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
@@ -209,7 +200,6 @@ struct BuildJob {
     float4* nodes;
     uint2* range;
     float4* blocks;
-    const K* samples;  // the sort's key samples (key_sample_count(n)), or null
 };
 // Builds one or two trees with one launch per kernel (the blocks of a launch
 // are split between the trees); chunk_begin/chunk_end (may be null) bracket

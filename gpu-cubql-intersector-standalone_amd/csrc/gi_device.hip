@@ -733,8 +733,6 @@ struct SortPass {
     uint32_t n_a, n_b, ntiles_a, ntiles_b;
     uint32_t split, scan_split;
     int shift;
-    K* samp_a; K* samp_b;  // last pass: every 2^samp_shift-th output key goes here (null: none)
-    int samp_shift_a, samp_shift_b;
 };
 
 // The tile loops take XCD-contiguous tile numbers (xcd_block): the count words
@@ -851,8 +849,7 @@ template <class K>
 __device__ __forceinline__ void tile_rank_scatter(const K (&k)[SORT_ITEMS], const uint32_t (&v)[SORT_ITEMS],
                                                   K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
                                                   int shift, uint32_t tile, uint32_t (*wc)[RADIX],
-                                                  const uint32_t* gbase, uint32_t* lstart, K* lk, uint32_t* lv,
-                                                  K* __restrict__ samp = nullptr, int samp_shift = 0) {
+                                                  const uint32_t* gbase, uint32_t* lstart, K* lk, uint32_t* lv) {
     static_assert(RADIX == SORT_THREADS, "one digit per thread");
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint32_t base = tile * SORT_TILE + wave * (64 * SORT_ITEMS);
@@ -898,7 +895,6 @@ __device__ __forceinline__ void tile_rank_scatter(const K (&k)[SORT_ITEMS], cons
         const uint32_t pos = gbase[d] + (j - lstart[d]);
         kout[pos] = key;
         vout[pos] = lv[j];
-        if (samp != nullptr && (pos & ((1u << samp_shift) - 1u)) == 0u) samp[pos >> samp_shift] = key;
     }
 }
 
@@ -931,8 +927,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_downsweep(SortPass<K> P) 
         gbase[threadIdx.x] = ex + hist[(size_t)threadIdx.x * ntiles + tile];
     }
     __syncthreads();
-    tile_rank_scatter<K>(k, v, kout, vout, n, P.shift, tile, wc, gbase, lstart, lk, lv, b ? P.samp_b : P.samp_a,
-                         b ? P.samp_shift_b : P.samp_shift_a);
+    tile_rank_scatter<K>(k, v, kout, vout, n, P.shift, tile, wc, gbase, lstart, lk, lv);
 }
 
 // Whole LSD radix sort in one workgroup (one launch) for inputs up to
@@ -1039,7 +1034,6 @@ struct SortRun {
     int passes;
     bool hist0_ready;  // the first pass's per-tile counts are already in hist
     uint32_t* hist; uint32_t* totals;
-    K* samples;        // key samples written by the last pass (null: none)
 };
 
 // The multi-tile passes of one or two sorts; pass p of both runs in the same
@@ -1061,16 +1055,12 @@ static void multi_tile_passes(SortRun<K>* r, int nr, hipStream_t s, int* parity)
             const uint32_t* vi = odd ? q.tv : q.vals;
             K* ko = odd ? q.keys : q.tk;
             uint32_t* vo = odd ? q.vals : q.tv;
-            K* samp = p == q.passes - 1 ? q.samples : nullptr;
-            const int ssh = key_sample_shift(q.n);
             if (!second) {
                 P.kin_a = ki; P.vin_a = vi; P.kout_a = ko; P.vout_a = vo;
                 P.hist_a = q.hist; P.totals_a = q.totals; P.n_a = q.n; P.ntiles_a = q.ntiles;
-                P.samp_a = samp; P.samp_shift_a = ssh;
             } else {
                 P.kin_b = ki; P.vin_b = vi; P.kout_b = ko; P.vout_b = vo;
                 P.hist_b = q.hist; P.totals_b = q.totals; P.n_b = q.n; P.ntiles_b = q.ntiles;
-                P.samp_b = samp; P.samp_shift_b = ssh;
             }
         };
         // slot a: the first active sort, slot b: the second (if any)
@@ -1118,8 +1108,7 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32_
         GI_LAUNCH_CHECK();
         return passes & 1;
     }
-    SortRun<K> r{keys, vals, tk, tv, n, (n + SORT_TILE - 1) / SORT_TILE, (bits + 7) / 8, hist0_ready, hist, scan_tmp,
-                 nullptr};
+    SortRun<K> r{keys, vals, tk, tv, n, (n + SORT_TILE - 1) / SORT_TILE, (bits + 7) / 8, hist0_ready, hist, scan_tmp};
     int parity = 0;
     multi_tile_passes<K>(&r, 1, s, &parity);
     return parity;
@@ -1127,16 +1116,6 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* tk, uint32_t* tv, uint32_
 
 
 bool radix_sort_takes_hist0(uint32_t n) { return n > small_sort_max<uint32_t>(); }
-
-int key_sample_shift(uint32_t n) {
-    int sh = 8;
-    while (((uint64_t)n + (1ull << sh) - 1) >> sh > kMaxKeySamples) ++sh;
-    return sh;
-}
-uint32_t key_sample_count(uint32_t n) {
-    const int sh = key_sample_shift(n);
-    return (uint32_t)(((uint64_t)n + (1ull << sh) - 1) >> sh);
-}
 
 void radix_sort_u32_pair(const SortJobU32* jobs, int njobs, hipStream_t s, int* in_tmp) {
     SortRun<uint32_t> r[2];
@@ -1150,7 +1129,7 @@ void radix_sort_u32_pair(const SortJobU32* jobs, int njobs, hipStream_t s, int* 
             continue;
         }
         r[nr] = SortRun<uint32_t>{q.keys, q.vals, q.tkeys, q.tvals, q.n, (q.n + SORT_TILE - 1) / SORT_TILE,
-                                  (q.bits + 7) / 8, q.hist0_ready, q.hist, q.scan_tmp, q.samples};
+                                  (q.bits + 7) / 8, q.hist0_ready, q.hist, q.scan_tmp};
         map[nr++] = j;
     }
     if (nr == 0) return;
@@ -1166,18 +1145,21 @@ int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t*
 // --------------------------------------------------------- BVH (Karras)
 //
 // Build and refit are one kernel per chunk of CHUNK (256 or 512) sorted
-// leaves, one thread per leaf and per internal node: the chunk's Morton keys
-// sit in LDS for Karras' prefix searches, and every internal node whose leaf
-// range lies inside the chunk takes the union of its leaves from a table of
-// aligned block boxes (1, 2, 4, ... 64 leaves) built in LDS by wave
-// butterflies. No bottom-up merge: no parents, flags or atomics, one barrier
-// (a global-atomic bottom-up refit, with the device-scope fences it needs on
+// leaves, one thread per leaf and per internal node: the chunk keeps in LDS
+// its keys' adjacent common-prefix lengths and their window minima, from
+// which each internal node whose leaf range lies inside the chunk gets its
+// range and split by binary lifting (karras_table_node: about 20 byte reads
+// instead of Karras' two searches, whose ~1,000 VALU instructions per node
+// made this kernel VALU-bound), and takes the union of its leaves from a
+// table of aligned block boxes (1, 2, 4, ... 64 leaves) built in LDS by wave
+// butterflies. No bottom-up merge: no parents, flags or atomics (a
+// global-atomic bottom-up refit, with the device-scope fences it needs on
 // gfx950's per-XCD L2s, cost 6 ms for 2^20 triangles; an LDS-atomic one
 // still a quarter of this kernel). The few "top" nodes whose range crosses a
-// chunk boundary search the sort's key samples and then one stride of keys
-// (gi_karras.h), keep the box of their leaves inside the chunk, and get the
-// rest of their leaf range from a 64-ary hierarchy of block boxes
-// (k_top_boxes).
+// chunk boundary keep the box of their leaves up to the chunk edge; their
+// searches over the whole key array (whose dependent key loads held a wave
+// for ~10 us here) and the rest of their box, from a 64-ary hierarchy of
+// block boxes, are k_top_boxes' work.
 // A refit after a rigid motion reruns the same kernels on the same sorted
 // keys: the topology comes out identical, only the boxes change.
 
@@ -1193,9 +1175,7 @@ struct ChunkJobs {
     uint2* range0; uint2* range1;
     float4* level1_0; float4* level1_1;  // boxes of the 64-leaf blocks (first level of the block hierarchy)
     uint64_t* topmask0; uint64_t* topmask1;  // per 64 internal nodes: which are top nodes
-    const K* samp0; const K* samp1;      // key samples (null: none)
     int n0, n1;
-    int samp_shift0, samp_shift1;
     uint32_t split;
 };
 
@@ -1215,15 +1195,22 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
     float4* __restrict__ level1 = second ? J.level1_1 : J.level1_0;
     uint64_t* __restrict__ topmask = second ? J.topmask1 : J.topmask0;
     const int n = second ? J.n1 : J.n0;
-    const K* __restrict__ samp = second ? J.samp1 : J.samp0;
+    constexpr int kLevels = DeltaLevels<CHUNK>::kLevels;
     __shared__ K skey[CHUNK];
     __shared__ float4 sbox[2 * block_table_offset(CHUNK, 7)];  // (lo, hi) pairs of levels 0..6
+    __shared__ int8_t dtab[kLevels][CHUNK + 1];                // adjacent prefixes and their window minima
     const int t = (int)threadIdx.x;
     const int cs = (int)chunk * CHUNK;
     const int ce = min(n, cs + CHUNK);
     const int jl = cs + t;  // this thread's leaf, and its internal node
     float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
     uint32_t prim = 0;
+    {  // A[jl] = delta(jl, jl + 1) at position t + 1; thread 0 also A[cs - 1] at 0
+        int a = -1;
+        if (jl + 1 < ce || (jl + 1 == ce && ce < n)) a = delta_keys(keys[jl], jl, keys[jl + 1], jl + 1);
+        dtab[0][t + 1] = (int8_t)a;
+        if (t == 0) dtab[0][0] = (int8_t)(cs > 0 ? delta_keys(keys[cs - 1], cs - 1, keys[cs], cs) : -1);
+    }
     if (jl < ce) {
         skey[t] = keys[jl];
         prim = prims[jl];
@@ -1255,6 +1242,16 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
         level1[2 * (size_t)(jl >> 6) + 1] = make_float4(b[3], b[4], b[5], 0.f);
     }
     __syncthreads();
+    // window minima of the adjacent prefixes, level by level (windows that
+    // pass position CHUNK are never read)
+#pragma unroll
+    for (int L = 1; L < kLevels; ++L) {
+        if (t + (1 << L) - 1 <= CHUNK) {
+            const int8_t u = dtab[L - 1][t], v = dtab[L - 1][t + (1 << (L - 1))];
+            dtab[L][t] = u < v ? u : v;
+        }
+        __syncthreads();
+    }
     const KeySrc<K> KS{keys, skey, cs, ce, n};
     if (jl < ce) {
         const uint32_t rope = jl == n - 1 ? kSentinel : chain_top(KS, jl + 1);
@@ -1263,10 +1260,18 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
     }
     if (jl >= min(ce, n - 1)) return;
     int lo_i, hi_i;
-    uint32_t left, right, rope;
-    const bool top = karras_chunk_node(KS, samp, second ? J.samp_shift1 : J.samp_shift0, jl, skey[t], lo_i, hi_i,
-                                       left, right, rope);
-    range[jl] = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
+    uint32_t left = 0, right = 0, rope = 0;
+    // in-chunk nodes from the table; a top node, whose range leaves the
+    // chunk on its side d, keeps here the box of its leaves up to the chunk
+    // edge and gets its range, children and rope from k_top_boxes
+    const bool top = !karras_table_node<CHUNK, K>(KS, dtab, jl, lo_i, hi_i, left, right, rope);
+    if (top) {
+        const bool up = dtab[0][t + 1] > dtab[0][t];  // d = +1: delta(i, i + 1) > delta(i, i - 1)
+        lo_i = up ? jl : cs;
+        hi_i = up ? ce - 1 : jl;
+    } else {
+        range[jl] = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
+    }
     // which of the wave's 64 internal nodes are top nodes (lane 0 holds the
     // first; a wave with none left has no internal node)
     const uint64_t tops = __ballot(top);
@@ -1289,7 +1294,7 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
         if (L < R && ((R >> k) & 1)) { absorb(k, (R >> k) - 1); R -= 1 << k; }
     }
     for (int m = L >> 6; m < (R >> 6); ++m) absorb(6, m);
-    const uint32_t lw = left | (right >= (uint32_t)(n - 1) ? kRightLeaf : 0u);
+    const uint32_t lw = left | (right >= (uint32_t)(n - 1) ? kRightLeaf : 0u);  // top nodes: k_top_boxes
     nodes[2 * (size_t)jl] = make_float4(q[0], q[1], q[2], __uint_as_float(lw));
     nodes[2 * (size_t)jl + 1] = make_float4(q[3], q[4], q[5], __uint_as_float(rope));
 }
@@ -1320,9 +1325,11 @@ __global__ __launch_bounds__(256) void k_block_boxes(BlockBoxJobs J) {
     }
 }
 
-// Boxes of the top nodes. The chunk kernel left in each the box of its
-// range's leaves inside its chunk; the leaves beyond the chunk edge (one side
-// only: the range holds the node's own leaf) come from the block hierarchy as
+// The top nodes: range, children and rope, then the box. The chunk kernel
+// left in each the box of its leaves up to its chunk's edge (on the side
+// its range leaves the chunk). Karras' two searches run over the whole key
+// array, 16 probes per round on a quarter-wave's lanes (karras_top_node16);
+// the leaves beyond the chunk edge then come from the block hierarchy as
 // disjoint aligned windows, at most two per level. A wave takes TOP_GROUPS
 // groups of 64 internal nodes (the chunk kernel's masks) and gives each of
 // their top nodes a quarter of the wave, four nodes at a time, so that the
@@ -1333,7 +1340,8 @@ constexpr uint32_t kTopLevelBoxes = 256;  // boxes of the highest block level k_
 
 struct TopBoxJobs {
     BlockLevels lv0, lv1;
-    const uint2* range0; const uint2* range1;
+    const uint32_t* keys0; const uint32_t* keys1;  // the sorted keys
+    uint2* range0; uint2* range1;
     const uint64_t* topmask0; const uint64_t* topmask1;
     float4* nodes0; float4* nodes1;
     uint32_t n0, n1;
@@ -1345,7 +1353,8 @@ __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J, uint32_t chunk)
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const bool second = blk >= J.split;
     const uint32_t n = second ? J.n1 : J.n0;
-    const uint2* __restrict__ range = second ? J.range1 : J.range0;
+    uint2* __restrict__ range = second ? J.range1 : J.range0;
+    const uint32_t* __restrict__ keys = second ? J.keys1 : J.keys0;
     float4* nodes = second ? J.nodes1 : J.nodes0;
     const BlockLevels& lv = second ? J.lv1 : J.lv0;  // read in place from the kernel arguments
     const int top = second ? J.top1 : J.top0;
@@ -1376,9 +1385,31 @@ __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J, uint32_t chunk)
             }
         for (; kk > 0; --kk) mm &= mm - 1;
         const uint32_t node = (g0 + g) * 64 + (uint32_t)__ffsll((long long)mm) - 1;
-        const uint2 r = range[node];
         const float4 part_lo = nodes[2 * (size_t)node], part_hi = nodes[2 * (size_t)node + 1];
         const uint32_t cs = node / chunk * chunk, ce = min(cs + chunk, n);
+        // range, children and rope: Karras' searches over the whole key
+        // array, 16 probes per round on the quarter's lanes
+        uint2 r;
+        uint32_t lw, rope;
+        {
+            const int i = (int)node, nn = (int)n;
+            const uint32_t ki = keys[i];
+            const int dnext = delta_keys(ki, i, keys[i + 1], i + 1);
+            const int dprev = i > 0 ? delta_keys(ki, i, keys[i - 1], i - 1) : -1;
+            const int d = dnext > dprev ? 1 : -1, dmin = d > 0 ? dprev : dnext;
+            const int lmin = d > 0 ? (int)ce - i : i - (int)cs + 1;
+            const KeySrc<uint32_t> KS{keys, nullptr, 0, 0, nn};
+            auto count = [&](int a, int b, auto pred) {
+                const bool ok = pred(probe16(a, b, (int)ql));
+                return (int)__popcll((__ballot(ok) >> (TOP_LANES * quarter)) & 0xFFFFull);
+            };
+            int lo_i, hi_i;
+            uint32_t left, right;
+            karras_top_node16(KS, i, ki, d, dmin, lmin, count, lo_i, hi_i, left, right, rope);
+            r = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
+            lw = left | (right >= n - 1 ? kRightLeaf : 0u);
+            if (ql == 0) range[node] = r;
+        }
         uint32_t L = r.x < cs ? r.x : ce, R = r.x < cs ? cs : r.y + 1;  // the leaves [L, R) beyond the chunk
         // windows (first element, count) per level and side
         uint32_t wS[2 * kMaxBlockLevels], wC[2 * kMaxBlockLevels];
@@ -1419,9 +1450,9 @@ __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J, uint32_t chunk)
 #pragma unroll
             for (int q = 3; q < 6; ++q) b[q] = fmaxf(b[q], __shfl_xor(b[q], o));
         }
-        if (ql == 0) {  // .w: the left child and the rope, as the chunk kernel wrote them
-            nodes[2 * (size_t)node] = make_float4(b[0], b[1], b[2], part_lo.w);
-            nodes[2 * (size_t)node + 1] = make_float4(b[3], b[4], b[5], part_hi.w);
+        if (ql == 0) {  // .w: the left child (right-leaf flag) and the rope
+            nodes[2 * (size_t)node] = make_float4(b[0], b[1], b[2], __uint_as_float(lw));
+            nodes[2 * (size_t)node + 1] = make_float4(b[3], b[4], b[5], __uint_as_float(rope));
         }
     }
 }
@@ -1455,11 +1486,6 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
     const BuildJob<K>* o = j1 ? j1 : j0;
     cj.keys1 = o->keys; cj.prims1 = o->prims; cj.box1 = o->boxes;
     cj.nodes1 = o->nodes; cj.range1 = o->range; cj.level1_1 = o->blocks; cj.n1 = (int)n1;
-    // key samples: every 2^key_sample_shift(n)-th sorted key
-    cj.samp0 = j0->samples;
-    cj.samp_shift0 = key_sample_shift(n0);
-    cj.samp1 = j1 ? j1->samples : nullptr;
-    cj.samp_shift1 = key_sample_shift(n1);
     uint64_t* topmask[2] = {(uint64_t*)(j0->blocks + top_mask_offset(n0)),
                             (uint64_t*)(o->blocks + top_mask_offset(n1))};
     cj.topmask0 = topmask[0];
@@ -1501,8 +1527,10 @@ static void build_bvh_impl(const BuildJob<K>* jobs, int njobs, uint32_t chunk, h
         bj.split = blocks_for(nout[0], 4);
         k_block_boxes<<<bj.split + blocks_for(nout[1], 4), 256, 0, s>>>(bj);
     }
+    static_assert(sizeof(K) == 4, "top-node searches read 32-bit keys");
     TopBoxJobs tj;
     tj.lv0 = lv[0]; tj.lv1 = lv[1];
+    tj.keys0 = (const uint32_t*)j0->keys; tj.keys1 = (const uint32_t*)o->keys;
     tj.range0 = j0->range; tj.range1 = o->range;
     tj.topmask0 = topmask[0]; tj.topmask1 = topmask[1];
     tj.nodes0 = j0->nodes; tj.nodes1 = o->nodes;

@@ -63,74 +63,128 @@ GI_HD void karras_links(const KeySrc<K>& KS, int i, int j, int gamma, int& lo_i,
     rope = hi_i == n - 1 ? kSentinel : chain_top(KS, hi_i + 1);
 }
 
-// Karras' two searches from leaf i in direction d, both "the largest l in
-// [0, L] with delta(i, i + l d) > thr" (true at l = 0, and monotone: sorted
-// keys). One code path for every lane and both directions, so the probe
-// chains of a wave's lanes overlap. The search first runs over every
-// 2^sh-th key (sh > 0: the sort's key samples samp[q] = key at q << sh; sh
-// = 0: the keys themselves), by galloping (the range end, usually near i) or
-// by bisection (the split, inside a known range), and then bisects the keys
-// inside the one stride left.
-template <class K>
-GI_HD int karras_extent(const KeySrc<K>& KS, const K* __restrict__ samp, int sh, int i, K ki, int d, int L, int thr,
-                        bool gallop) {
-    const int S = 1 << sh;
-    const int q0 = d > 0 ? i >> sh : (i + S - 1) >> sh;  // stride point at or behind i
-    const int e = i + L * d;
-    const int kmax = d > 0 ? (e >> sh) - q0 : q0 - ((e + S - 1) >> sh);  // stride points past i, up to e
-    auto above = [&](int k) {
-        const int q = q0 + k * d;
-        return delta_keys(ki, i, sh ? samp[q] : KS.at(q), q << sh) > thr;
-    };
-    int lo = 0, hi = kmax;
-    if (gallop) {
-        int g = 1;
-        while (g <= kmax && above(g)) g <<= 1;
-        lo = g >> 1;
-        hi = g - 1 < kmax ? g - 1 : kmax;
+// ------------------------------------------------- in-chunk nodes by table
+//
+// For sorted keys the common prefix of keys i < j is the least of the
+// adjacent ones between them: delta(i, j) = min A[i .. j - 1] with A[k] =
+// delta(k, k + 1). A chunk [cs, ce) keeps A[cs - 1 .. ce - 1] at local
+// positions q = 0 .. ce - cs (A[k] at q = k - cs + 1; -1 before the first
+// key and from the last key on) and its minima over windows of 2^L
+// positions, M[L][q] = min A-local[q .. q + 2^L - 1] (entries whose window
+// passes the end are never read). Karras' two searches then become binary
+// lifting over M: the range end is the first adjacent prefix at or below
+// delta(i, i - d) on the node's side, and the split the position of the
+// least adjacent prefix inside the range (unique for distinct composite
+// keys). Prefix lengths fit a byte (-1 .. 96 for 64-bit keys).
+
+template <int C>
+struct DeltaLevels {
+    static constexpr int kLevels = C == 64 ? 7 : (C == 128 ? 8 : (C == 256 ? 9 : (C == 512 ? 10 : 11)));
+};
+
+// Internal node i of the chunk from the table M (rows of C + 1 entries):
+// returns false when its range leaves the chunk (a top node, for
+// karras_top_node16); else its range, children and rope, those of Karras'
+// searches.
+template <int C, class K>
+GI_HD bool karras_table_node(const KeySrc<K>& KS, const int8_t (*M)[C + 1], int i, int& lo_i, int& hi_i,
+                             uint32_t& left, uint32_t& right, uint32_t& rope) {
+    constexpr int top_level = DeltaLevels<C>::kLevels - 1;
+    const int cs = KS.cs, qmax = KS.ce - cs;  // local position of A[ce - 1]
+    const int q0 = i - cs + 1;                // A[i]; A[i - 1] at q0 - 1
+    const int ai = M[0][q0], aim = M[0][q0 - 1];
+    const int d = ai > aim ? 1 : -1;
+    const int dmin = d > 0 ? aim : ai;
+    int p = q0, dn = 127, j;
+    if (d > 0) {  // first A[k], k >= i, at or below dmin: the range is [i, k]
+        for (int L = top_level; L >= 0; --L) {
+            const int w = 1 << L;
+            if (p + w - 1 <= qmax) {
+                const int v = M[L][p];
+                if (v > dmin) {
+                    dn = v < dn ? v : dn;
+                    p += w;
+                }
+            }
+        }
+        if (p > qmax) return false;
+        j = cs - 1 + p;
+    } else {  // last A[k], k < i, at or below dmin: the range is [k + 1, i]
+        for (int L = top_level; L >= 0; --L) {
+            const int w = 1 << L;
+            if (p - w >= 0) {
+                const int v = M[L][p - w];
+                if (v > dmin) {
+                    dn = v < dn ? v : dn;
+                    p -= w;
+                }
+            }
+        }
+        if (p == 0) return false;
+        j = cs + p - 1;
     }
-    while (lo < hi) {
-        const int m = (lo + hi + 1) >> 1;
-        if (above(m)) lo = m;
-        else hi = m - 1;
+    // dn = delta(i, j), the least adjacent prefix of the range; the split
+    // is where it sits (first A at or below it from the range's start)
+    const int lo = i < j ? i : j;
+    p = lo - cs + 1;
+    for (int L = top_level; L >= 0; --L) {
+        const int w = 1 << L;
+        if (p + w - 1 <= qmax && M[L][p] > dn) p += w;
     }
-    // the answer lies between stride point lo and the next one
-    int llo = lo > 0 ? (q0 + lo * d) * S - i : 0;
-    llo = llo < 0 ? -llo : llo;
-    int next = (q0 + d) * S - i;
-    next = (next < 0 ? -next : next) - 1;
-    int lhi = lo > 0 ? llo + S - 1 : next;
-    lhi = lhi < L ? lhi : L;
-    while (llo < lhi) {
-        const int m = (llo + lhi + 1) >> 1;
-        const int p = i + m * d;
-        if (delta_keys(ki, i, KS.at(p), p) > thr) llo = m;
-        else lhi = m - 1;
-    }
-    return llo;
+    karras_links(KS, i, j, cs - 1 + p, lo_i, hi_i, left, right, rope);
+    return true;
 }
 
-// Internal node i of a build chunk (the keys [KS.cs, KS.ce) of KS in LDS, ki
-// = key i): range, children and rope. Returns whether it is a "top" node,
-// one whose range leaves the chunk: one probe past the chunk edge tells
-// (sorted keys: the common prefix only shrinks with distance). An in-chunk
-// node searches the chunk's keys up to its edge; a top node the key samples
-// (samp: every 2^samp_shift-th sorted key; null: none) and then the keys of
-// one stride.
-template <class K>
-GI_HD bool karras_chunk_node(const KeySrc<K>& KS, const K* __restrict__ samp, int samp_shift, int i, K ki, int& lo_i,
+// ------------------------------------------------ top nodes, 16-way search
+//
+// A top node's range leaves its chunk, so its two searches run over the
+// whole key array. They take 16 probes per round (the 16 lanes of a
+// quarter-wave on the GPU, one key load each), which cuts the candidate
+// interval 16-fold per round: a few dependent rounds instead of a binary
+// search's ~40 dependent loads.
+
+// probe k (0 .. 15) of a round over [lo, hi], lo < hi: nondecreasing in k,
+// the last one at hi
+GI_HD int probe16(int lo, int hi, int k) { return lo + (int)(((int64_t)(hi - lo) * (k + 1) + 15) / 16); }
+
+// the interval left after a round in which the first c probes held (a
+// monotone predicate, true at lo): from the last probe that held to just
+// before the first that did not
+GI_HD void narrow16(int& lo, int& hi, int c) {
+    const int l2 = c ? probe16(lo, hi, c - 1) : lo;
+    const int h2 = c < 16 ? probe16(lo, hi, c) - 1 : hi;
+    lo = l2;
+    hi = h2;
+}
+
+// Top node i (key ki) whose range runs from i in direction d past its
+// chunk's edge, at least lmin leaves away (delta(i, i + lmin d) > dmin =
+// delta(i, i - d)): range, children and rope, those of Karras' searches.
+// count(lo, hi, pred) returns how many of the round's 16 probes
+// probe16(lo, hi, k) satisfy pred (a prefix of them).
+template <class K, class Count>
+GI_HD void karras_top_node16(const KeySrc<K>& KS, int i, K ki, int d, int dmin, int lmin, Count count, int& lo_i,
                              int& hi_i, uint32_t& left, uint32_t& right, uint32_t& rope) {
-    const int d = delta(KS, i, i + 1) > delta(KS, i, i - 1) ? 1 : -1;
-    const int dmin = delta(KS, i, i - d);
-    const int edge = d > 0 ? KS.ce : KS.cs - 1;
-    const bool top = edge >= 0 && edge < KS.n && delta(KS, i, edge) > dmin;
-    const int sh = top && samp != nullptr ? samp_shift : 0;
-    const int reach = top ? (d > 0 ? KS.n - 1 - i : i) : (d > 0 ? KS.ce - 1 - i : i - KS.cs);
-    const int l = karras_extent(KS, samp, sh, i, ki, d, reach, dmin, true);
-    const int j = i + l * d;
-    const int s = karras_extent(KS, samp, sh, i, ki, d, l - 1, delta_keys(ki, i, KS.at(j), j), false);
-    karras_links(KS, i, j, i + s * d + (d < 0 ? -1 : 0), lo_i, hi_i, left, right, rope);
-    return top;
+    const int n = KS.n;
+    int lo = lmin, hi = d > 0 ? n - 1 - i : i;  // the range's length l: delta(i, i + l d) > dmin
+    while (lo < hi) {
+        const int c = count(lo, hi, [&](int l) {
+            const int p = i + l * d;
+            return delta_keys(ki, i, KS.at(p), p) > dmin;
+        });
+        narrow16(lo, hi, c);
+    }
+    const int l = lo, j = i + l * d;
+    const int dn = delta_keys(ki, i, KS.at(j), j);
+    int s_lo = 0, s_hi = l - 1;  // the split offset s: delta(i, i + s d) > dn
+    while (s_lo < s_hi) {
+        const int c = count(s_lo, s_hi, [&](int s) {
+            const int p = i + s * d;
+            return delta_keys(ki, i, KS.at(p), p) > dn;
+        });
+        narrow16(s_lo, s_hi, c);
+    }
+    karras_links(KS, i, j, i + s_lo * d + (d < 0 ? -1 : 0), lo_i, hi_i, left, right, rope);
 }
 
 }  // namespace gi

@@ -97,7 +97,7 @@ struct MeshBufs {
     double4* v64;
     int4* vi;
     float4* tbox;
-    uint32_t *keys, *vals, *tk, *tv, *hist, *stmp, *samples;
+    uint32_t *keys, *vals, *tk, *tv, *hist, *stmp;
     float4* nodes;  // A only
     uint2* range;
     float4* blocks;
@@ -118,7 +118,6 @@ void carve_mesh(Carve& c, uint32_t nv, uint32_t nt, bool tree, MeshBufs& m) {
     m.hist = c.take<uint32_t>(sort_hist_elems(nt));
     m.stmp = c.take<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
     m.multi = radix_sort_takes_hist0(nt);
-    m.samples = m.multi ? c.take<uint32_t>(key_sample_count(nt)) : nullptr;
     m.nodes = tree ? c.take<float4>(2 * (size_t)(nt ? 2 * nt - 1 : 1)) : nullptr;
     m.range = tree ? c.take<uint2>(nt) : nullptr;
     m.blocks = tree ? c.take<float4>(block_levels_elems(nt)) : nullptr;
@@ -236,7 +235,7 @@ extern "C" int gi_intersect_noalloc(int32_t device, const double* xyzA, uint32_t
         MeshBufs& m = M[k];
         sj[k] = StageJob{m.src, nv[k], m.tri, nt[k], d_xf + k, m.v64, m.vi, grid + k, m.axis, m.tbox, m.keys, m.vals,
                          m.multi ? m.hist : nullptr};
-        jobs[k] = SortJobU32{m.keys, m.vals, m.tk, m.tv, nt[k], 3 * m.axis, m.multi, m.hist, m.stmp, m.samples};
+        jobs[k] = SortJobU32{m.keys, m.vals, m.tk, m.tv, nt[k], 3 * m.axis, m.multi, m.hist, m.stmp};
     }
     launch_stage_meshes(sj, 2, s);
     int in_tmp[2] = {0, 0};
@@ -249,7 +248,6 @@ extern "C" int gi_intersect_noalloc(int32_t device, const double* xyzA, uint32_t
     bj.nodes = M[0].nodes;
     bj.range = M[0].range;
     bj.blocks = M[0].blocks;
-    bj.samples = M[0].samples;
     launch_build_bvh(&bj, 1, kBuildChunk, s, nullptr, nullptr, 0);
     MeshDev A, B;
     MeshDev* D[2] = {&A, &B};

@@ -1,16 +1,18 @@
-"""BVH topology on the CPU: the per-node routine of k_build_chunks
-(csrc/gi_karras.h, host and device code) compiled for the host and run over
-every node of Morton, uniform, duplicate-heavy, clustered, tiny and batch
-(mesh id above the Morton code, 32 and 64 bits) key sets, at chunk sizes
-64/256/512 and several key-sample strides.
+"""BVH topology on the CPU: the per-node routines of k_build_chunks (in-chunk
+nodes by binary lifting over the chunk's adjacent common-prefix table) and
+k_top_boxes (nodes whose range leaves the chunk, by 16-way searches over all
+keys), csrc/gi_karras.h, host and device code, compiled for the host and run
+over every node of Morton, uniform, duplicate-heavy, clustered, tiny and
+batch (mesh id above the Morton code, 32 and 64 bits) key sets, at chunk
+sizes 64/256/512.
 Each node must equal a plain Karras build (Karras 2012), and the tree must be
 well formed (every node reached once, children splitting their parent's
 range, ropes = pre-order successors). The reference's BVH (cuBQL's builder
 with its level-cut extension, include/third-party/cubql/level_cut_builder.h:
 16-364) is internal state whose shape never reaches the output, so the tree
 is checked against Karras' definition rather than against it. Planting an
-off-by-one in the stride bounds, the split offset or the index tie-break
-fails this test."""
+off-by-one in the lifted range end or in the split position fails this
+test."""
 import os
 import subprocess
 
