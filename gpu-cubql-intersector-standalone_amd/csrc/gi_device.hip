@@ -1388,28 +1388,27 @@ __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J, uint32_t chunk)
         const float4 part_lo = nodes[2 * (size_t)node], part_hi = nodes[2 * (size_t)node + 1];
         const uint32_t cs = node / chunk * chunk, ce = min(cs + chunk, n);
         // range, children and rope: Karras' searches over the whole key
-        // array, 16 probes per round on the quarter's lanes
-        uint2 r;
-        uint32_t lw, rope;
-        {
-            const int i = (int)node, nn = (int)n;
-            const uint32_t ki = keys[i];
-            const int dnext = delta_keys(ki, i, keys[i + 1], i + 1);
-            const int dprev = i > 0 ? delta_keys(ki, i, keys[i - 1], i - 1) : -1;
-            const int d = dnext > dprev ? 1 : -1, dmin = d > 0 ? dprev : dnext;
-            const int lmin = d > 0 ? (int)ce - i : i - (int)cs + 1;
-            const KeySrc<uint32_t> KS{keys, nullptr, 0, 0, nn};
-            auto count = [&](int a, int b, auto pred) {
-                const bool ok = pred(probe16(a, b, (int)ql));
-                return (int)__popcll((__ballot(ok) >> (TOP_LANES * quarter)) & 0xFFFFull);
-            };
-            int lo_i, hi_i;
-            uint32_t left, right;
-            karras_top_node16(KS, i, ki, d, dmin, lmin, count, lo_i, hi_i, left, right, rope);
-            r = make_uint2((uint32_t)lo_i, (uint32_t)hi_i);
-            lw = left | (right >= n - 1 ? kRightLeaf : 0u);
-            if (ql == 0) range[node] = r;
-        }
+        // array, 16 probes per round on the quarter's lanes. The rope's keys
+        // and the box windows need only the range: they are loaded before
+        // the split search, off its chain of dependent rounds.
+        const int i = (int)node, nn = (int)n;
+        const uint32_t ki = keys[i];
+        const int dnext = delta_keys(ki, i, keys[i + 1], i + 1);
+        const int dprev = i > 0 ? delta_keys(ki, i, keys[i - 1], i - 1) : -1;
+        const int d = dnext > dprev ? 1 : -1, dmin = d > 0 ? dprev : dnext;
+        const KeySrc<uint32_t> KS{keys, nullptr, 0, 0, nn};
+        auto count = [&](auto pos, auto pred) {
+            const bool ok = pred(pos((int)ql));
+            return (int)__popcll((__ballot(ok) >> (TOP_LANES * quarter)) & 0xFFFFull);
+        };
+        const int l = karras_top_range16(KS, i, ki, d, dmin, d > 0 ? (int)ce - i : i - (int)cs + 1, count);
+        const int j = i + l * d;
+        const uint2 r = make_uint2((uint32_t)min(i, j), (uint32_t)max(i, j));
+        if (ql == 0) range[node] = r;
+        // the keys the split search starts from and the rope needs, loaded
+        // together before the windows
+        const uint32_t kj = keys[j], k_hi = keys[r.y];
+        const uint32_t k_hi1 = r.y + 1 < n ? keys[r.y + 1] : 0u, k_hi2 = r.y + 2 < n ? keys[r.y + 2] : 0u;
         uint32_t L = r.x < cs ? r.x : ce, R = r.x < cs ? cs : r.y + 1;  // the leaves [L, R) beyond the chunk
         // windows (first element, count) per level and side
         uint32_t wS[2 * kMaxBlockLevels], wC[2 * kMaxBlockLevels];
@@ -1450,6 +1449,11 @@ __global__ __launch_bounds__(256) void k_top_boxes(TopBoxJobs J, uint32_t chunk)
 #pragma unroll
             for (int q = 3; q < 6; ++q) b[q] = fmaxf(b[q], __shfl_xor(b[q], o));
         }
+        const uint32_t rope = r.y == n - 1 ? kSentinel : chain_top_of(nn, (int)r.y + 1, k_hi, k_hi1, k_hi2);
+        uint32_t left, right;
+        karras_children(nn, (int)r.x, (int)r.y, karras_top_split16(KS, i, ki, d, l, delta_keys(ki, i, kj, j), count),
+                        left, right);
+        const uint32_t lw = left | (right >= n - 1 ? kRightLeaf : 0u);
         if (ql == 0) {  // .w: the left child (right-leaf flag) and the rope
             nodes[2 * (size_t)node] = make_float4(b[0], b[1], b[2], __uint_as_float(lw));
             nodes[2 * (size_t)node + 1] = make_float4(b[3], b[4], b[5], __uint_as_float(rope));

@@ -50,16 +50,26 @@ GI_HD uint32_t chain_top(const KeySrc<K>& k, int m) {
     if (m < k.n - 1 && delta(k, m, m + 1) > delta(k, m, m - 1)) return (uint32_t)m;
     return (uint32_t)(k.n - 1 + m);
 }
+// the same for m >= 1 from the keys m - 1, m and m + 1 already at hand
+// (kp1 is used only when m < n - 1)
+template <class K>
+GI_HD uint32_t chain_top_of(int n, int m, K km1, K km, K kp1) {
+    if (m < n - 1 && delta_keys(km, m, kp1, m + 1) > delta_keys(km, m, km1, m - 1)) return (uint32_t)m;
+    return (uint32_t)(n - 1 + m);
+}
 
 // Children and rope of internal node i from its range end j and split gamma.
+GI_HD void karras_children(int n, int lo_i, int hi_i, int gamma, uint32_t& left, uint32_t& right) {
+    left = lo_i == gamma ? (uint32_t)(n - 1 + gamma) : (uint32_t)gamma;
+    right = hi_i == gamma + 1 ? (uint32_t)(n - 1 + gamma + 1) : (uint32_t)(gamma + 1);
+}
 template <class K>
 GI_HD void karras_links(const KeySrc<K>& KS, int i, int j, int gamma, int& lo_i, int& hi_i, uint32_t& left,
                         uint32_t& right, uint32_t& rope) {
     const int n = KS.n;
     lo_i = i < j ? i : j;
     hi_i = i < j ? j : i;
-    left = lo_i == gamma ? (uint32_t)(n - 1 + gamma) : (uint32_t)gamma;
-    right = hi_i == gamma + 1 ? (uint32_t)(n - 1 + gamma + 1) : (uint32_t)(gamma + 1);
+    karras_children(n, lo_i, hi_i, gamma, left, right);
     rope = hi_i == n - 1 ? kSentinel : chain_top(KS, hi_i + 1);
 }
 
@@ -141,7 +151,9 @@ GI_HD bool karras_table_node(const KeySrc<K>& KS, const int8_t (*M)[C + 1], int 
 // whole key array. They take 16 probes per round (the 16 lanes of a
 // quarter-wave on the GPU, one key load each), which cuts the candidate
 // interval 16-fold per round: a few dependent rounds instead of a binary
-// search's ~40 dependent loads.
+// search's ~40 dependent loads. The range search and the split search are
+// separate calls, so that a kernel can issue the loads that need only the
+// range (the rope's keys, the box windows) before the split search.
 
 // probe k (0 .. 15) of a round over [lo, hi], lo < hi: nondecreasing in k,
 // the last one at hi
@@ -157,34 +169,63 @@ GI_HD void narrow16(int& lo, int& hi, int c) {
     hi = h2;
 }
 
+// probe k of a range search's first round: exponential steps lo + 4^k - 1
+// (at most hi), since most ranges that leave a chunk end near its edge
+GI_HD int probe_exp16(int lo, int hi, int k) {
+    const int64_t x = (int64_t)lo + ((int64_t)1 << (2 * k)) - 1;
+    return x < hi ? (int)x : hi;
+}
+
 // Top node i (key ki) whose range runs from i in direction d past its
 // chunk's edge, at least lmin leaves away (delta(i, i + lmin d) > dmin =
-// delta(i, i - d)): range, children and rope, those of Karras' searches.
-// count(lo, hi, pred) returns how many of the round's 16 probes
-// probe16(lo, hi, k) satisfy pred (a prefix of them).
+// delta(i, i - d)): the range's length l, Karras' largest l with
+// delta(i, i + l d) > dmin. count(pos, pred) returns how many of a round's
+// 16 probes pos(0 .. 15) satisfy pred (a prefix of them: pos and pred are
+// monotone).
+template <class K, class Count>
+GI_HD int karras_top_range16(const KeySrc<K>& KS, int i, K ki, int d, int dmin, int lmin, Count count) {
+    int lo = lmin, hi = d > 0 ? KS.n - 1 - i : i;
+    auto above = [&](int l) {
+        const int p = i + l * d;
+        return delta_keys(ki, i, KS.at(p), p) > dmin;
+    };
+    if (lo < hi) {  // probe 0 is lmin, which holds
+        const int a = lo, b = hi;
+        const int c = count([&](int k) { return probe_exp16(a, b, k); }, above);
+        lo = probe_exp16(a, b, c - 1);
+        hi = c < 16 ? probe_exp16(a, b, c) - 1 : b;
+    }
+    while (lo < hi) {
+        const int a = lo, b = hi;
+        narrow16(lo, hi, count([&](int k) { return probe16(a, b, k); }, above));
+    }
+    return lo;
+}
+
+// Its split: Karras' largest s in [0, l - 1] with delta(i, i + s d) > dn =
+// delta(i, i + l d); returns the split position gamma.
+template <class K, class Count>
+GI_HD int karras_top_split16(const KeySrc<K>& KS, int i, K ki, int d, int l, int dn, Count count) {
+    auto above = [&](int s) {
+        const int p = i + s * d;
+        return delta_keys(ki, i, KS.at(p), p) > dn;
+    };
+    int lo = 0, hi = l - 1;
+    while (lo < hi) {
+        const int a = lo, b = hi;
+        narrow16(lo, hi, count([&](int k) { return probe16(a, b, k); }, above));
+    }
+    return i + lo * d + (d < 0 ? -1 : 0);
+}
+
+// Both searches and the links: range, children and rope of the top node.
 template <class K, class Count>
 GI_HD void karras_top_node16(const KeySrc<K>& KS, int i, K ki, int d, int dmin, int lmin, Count count, int& lo_i,
                              int& hi_i, uint32_t& left, uint32_t& right, uint32_t& rope) {
-    const int n = KS.n;
-    int lo = lmin, hi = d > 0 ? n - 1 - i : i;  // the range's length l: delta(i, i + l d) > dmin
-    while (lo < hi) {
-        const int c = count(lo, hi, [&](int l) {
-            const int p = i + l * d;
-            return delta_keys(ki, i, KS.at(p), p) > dmin;
-        });
-        narrow16(lo, hi, c);
-    }
-    const int l = lo, j = i + l * d;
-    const int dn = delta_keys(ki, i, KS.at(j), j);
-    int s_lo = 0, s_hi = l - 1;  // the split offset s: delta(i, i + s d) > dn
-    while (s_lo < s_hi) {
-        const int c = count(s_lo, s_hi, [&](int s) {
-            const int p = i + s * d;
-            return delta_keys(ki, i, KS.at(p), p) > dn;
-        });
-        narrow16(s_lo, s_hi, c);
-    }
-    karras_links(KS, i, j, i + s_lo * d + (d < 0 ? -1 : 0), lo_i, hi_i, left, right, rope);
+    const int l = karras_top_range16(KS, i, ki, d, dmin, lmin, count);
+    const int j = i + l * d;
+    karras_links(KS, i, j, karras_top_split16(KS, i, ki, d, l, delta_keys(ki, i, KS.at(j), j), count), lo_i, hi_i,
+                 left, right, rope);
 }
 
 }  // namespace gi

@@ -96,9 +96,9 @@ int check(const char* set, const std::vector<K>& keys, long& top_nodes) {
     std::vector<int8_t> buf((size_t)DeltaLevels<C>::kLevels * (C + 1));
     int8_t (*M)[C + 1] = reinterpret_cast<int8_t (*)[C + 1]>(buf.data());
     const KeySrc<K> G{keys.data(), nullptr, 0, 0, n};  // all keys, read as the top-node kernel reads them
-    auto count = [](int a, int b, auto pred) {
+    auto count = [](auto pos, auto pred) {
         int c = 0;
-        for (int k = 0; k < 16; ++k) c += pred(probe16(a, b, k)) ? 1 : 0;
+        for (int k = 0; k < 16; ++k) c += pred(pos(k)) ? 1 : 0;
         return c;
     };
     for (int cs = 0; cs < n; cs += C) {
@@ -107,6 +107,9 @@ int check(const char* set, const std::vector<K>& keys, long& top_nodes) {
         const KeySrc<K> KS{keys.data(), keys.data() + cs, cs, ce, n};
         for (int i = cs; i < ce; ++i) {
             T.rope[n - 1 + i] = i == n - 1 ? kSentinel : chain_top(KS, i + 1);  // leaf rope, as the kernel writes it
+            if (i + 1 < n &&  // the top-node kernel's form, from three keys
+                chain_top_of(n, i + 1, keys[i], keys[i + 1], i + 2 < n ? keys[i + 2] : (K)0) != T.rope[n - 1 + i])
+                return fail("chain_top_of differs from chain_top", set, C, i);
             if (i == n - 1) break;
             int lo, hi;
             uint32_t l, r, rope;
