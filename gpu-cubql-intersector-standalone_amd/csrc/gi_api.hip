@@ -172,6 +172,8 @@ struct gi_context {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DevBuf bhist[2], bscan[2], bpart[2];  // per-slot build scratch
     SortJobU32 pending_sort[2] = {};       // Morton sorts queued by the slot stages (n = 0: none)
+    MsdSortJob pending_msd[2] = {};        // the same as bucket sorts (keys null: none)
+    DevBuf msd;                            // bucket-sort words of both slots (2 kMsdWords)
     std::string err;
     int host_threads = 0;
     uint32_t cand_reserve = 0;
@@ -456,7 +458,12 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild, StageJob* job) {
         hist = ctx->bhist[slot].ensure<uint32_t>(sort_hist_elems(nt));
         stmp = ctx->bscan[slot].ensure<uint32_t>(scan_tmp_elems(sort_hist_elems(nt)));
     }
-    const bool hist0 = topology && radix_sort_takes_hist0(m.nt);
+    // a bucket sort where the key width allows it (its counts come from the
+    // staging launch), else the LSD passes (their first counts from there)
+    const int bits = 3 * m.morton_axis;
+    const bool msd = topology && msd_sort_fits(m.nt, bits);
+    const bool hist0 = topology && !msd && radix_sort_takes_hist0(m.nt);
+    uint32_t* words = msd ? ctx->msd.get<uint32_t>() + (size_t)slot * kMsdWords : nullptr;
     // an unmoved mesh (no transform, and no snap that would change a vertex)
     // needs no moved copies: no vertex blocks, the grid flag is the upload's
     // classification (set by the query's init), the kernels read m.src
@@ -464,10 +471,13 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild, StageJob* job) {
     *job = StageJob{m.src.get<double>(), m.moved_copy ? m.nv : 0u, m.tri.get<uint32_t>(), m.nt, d_xf, v64, vi,
                     grid + slot,
                     m.morton_axis, tbox, topology ? keys : nullptr, topology ? vals : nullptr,
-                    hist0 ? hist : nullptr, !m.moved_copy && m.src_grid == 0 ? 1 : 0};
+                    hist0 ? hist : nullptr, !m.moved_copy && m.src_grid == 0 ? 1 : 0, words,
+                    msd ? bits - kMsdBits : 0};
     // the Morton sort runs after the staging, together with the other slot's
-    if (topology)
-        ctx->pending_sort[slot] = SortJobU32{keys, vals, tk, tv, m.nt, 3 * m.morton_axis, hist0, hist, stmp};
+    if (msd)
+        ctx->pending_msd[slot] = MsdSortJob{keys, vals, tk, tv, m.nt, bits, words};
+    else if (topology)
+        ctx->pending_sort[slot] = SortJobU32{keys, vals, tk, tv, m.nt, bits, hist0, hist, stmp};
     return true;
 }
 
@@ -502,8 +512,12 @@ void build_meshes(gi_context* ctx, bool rebuild, int time_mask = 0, bool zero_co
         const MeshSlot& m = ctx->mesh[sl];
         if (!m.forest && slot_needs_stage(m, rebuild)) stage_mask |= 1u << sl;
     }
+    // the bucket sorts' totals and cursors start at zero (both slots' words,
+    // whichever sort a staged slot takes)
+    uint32_t* msd_words = stage_mask ? ctx->msd.ensure<uint32_t>(2 * kMsdWords) : nullptr;
     if (stage_mask || zero_counters)
-        launch_query_init(counters, zero_counters ? CNT_COUNT : 0u, counters + CNT_COUNT, stage_mask, ctx->stream);
+        launch_query_init(counters, zero_counters ? CNT_COUNT : 0u, counters + CNT_COUNT, stage_mask, msd_words,
+                          msd_words ? 2 * kMsdWords : 0u, ctx->stream);
     // batch sides stage on their own streams (fork/join); single meshes share
     // one staging launch on the context's stream
     const bool forest = ctx->mesh[0].forest || ctx->mesh[1].forest;
@@ -513,6 +527,7 @@ void build_meshes(gi_context* ctx, bool rebuild, int time_mask = 0, bool zero_co
     int nsj = 0;
     for (int sl = 0; sl < 2; ++sl) {
         ctx->pending_sort[sl] = SortJobU32{};
+        ctx->pending_msd[sl] = MsdSortJob{};
         sj[nsj].nv = sj[nsj].nt = 0;
         if (ctx->mesh[sl].set) need[sl] = stage_mesh(ctx, sl, rebuild, &sj[nsj]);
         if (need[sl] && !ctx->mesh[sl].forest) ++nsj;
@@ -535,6 +550,15 @@ void build_meshes(gi_context* ctx, bool rebuild, int time_mask = 0, bool zero_co
             }
         radix_sort_u32_pair(jobs, nj, ctx->stream, in_tmp);
         for (int j = 0; j < nj; ++j) ctx->mesh[slot_of[j]].sorted_in_tmp = in_tmp[j];
+        // the bucket sorts, both in the same two launches; they end in (keys, vals)
+        MsdSortJob mj[2];
+        int nm = 0;
+        for (int sl = 0; sl < 2; ++sl)
+            if (ctx->pending_msd[sl].keys) {
+                mj[nm++] = ctx->pending_msd[sl];
+                ctx->mesh[sl].sorted_in_tmp = 0;
+            }
+        msd_sort_pair(mj, nm, ctx->stream);
     }
     BuildJob<uint32_t> jobs[2];
     int nj = 0;
@@ -684,7 +708,7 @@ void enqueue_presync(gi_context* ctx, const gi_options& o, bool build, bool time
         build_meshes(ctx, o.rebuild != 0, mask, true);
         if (timed) record(ctx, ctx->ev[1], s);
     } else {
-        launch_query_init(counters, CNT_COUNT, counters + CNT_COUNT, 0u, s);
+        launch_query_init(counters, CNT_COUNT, counters + CNT_COUNT, 0u, nullptr, 0u, s);
     }
     uint32_t* grid = counters + CNT_COUNT;
     MeshDev A = mesh_dev(MA), B = mesh_dev(MB);

@@ -102,8 +102,10 @@ void launch_inspect_mesh(const double* xyz, uint32_t nv, const uint32_t* tri, ui
 void launch_forest_globalize(uint32_t* tri, const uint2* ttiles, uint32_t n_ttiles, const uint32_t* toff,
                              const uint32_t* voff, MeshInspect* out, hipStream_t s);
 // grid flag = 1, bounds6 = empty box (order-preserving keys)
-// zero counters[0, nzero) and set grid_flags[slot] = 1 for the slots in stage_mask
-void launch_query_init(uint32_t* counters, uint32_t nzero, uint32_t* grid_flags, uint32_t stage_mask, hipStream_t s);
+// zero counters[0, nzero) (one word per kCntStride) and words[0, nwords),
+// and set grid_flags[slot] = 1 for the slots in stage_mask
+void launch_query_init(uint32_t* counters, uint32_t nzero, uint32_t* grid_flags, uint32_t stage_mask,
+                       uint32_t* words, uint32_t nwords, hipStream_t s);
 // Grid-stride width of the vertex part of a mesh's stage launch.
 constexpr uint32_t kPrepareMaxBlocks = 4096;
 // One mesh's staging: its vertices (moved by *xf, in device memory so that a
@@ -131,6 +133,11 @@ struct StageJob {
     // grid flag as the query's init set it (1); force_off_grid clears it
     // (source vertices off the grid)
     int force_off_grid;
+    // bucket sort (msd_sort_pair): the tiles add their counts of the top
+    // kMsdBits key bits (key >> msd_shift) to msd_total (zeroed by the
+    // query's init); null: none
+    uint32_t* msd_total;
+    int msd_shift;
 };
 // Up to two meshes staged in one launch.
 void launch_stage_meshes(const StageJob* jobs, int njobs, hipStream_t s);
@@ -170,6 +177,26 @@ struct SortJobU32 {
     uint32_t* scan_tmp;  // scan_tmp_elems(sort_hist_elems(n))
 };
 void radix_sort_u32_pair(const SortJobU32* jobs, int njobs, hipStream_t s, int* in_tmp);
+// Morton sorts of single meshes (13 to 24 key bits, more than one workgroup's
+// keys) by buckets of the top kMsdBits bits, two launches for one or two
+// sorts: launch_stage_mesh counts the buckets (StageJob.msd_total), then
+// every key goes to its bucket (one reservation per bucket and tile) and
+// each bucket is sorted by its low bits in LDS. Within a bucket equal low
+// bits are equal keys, so no step needs a stable rank: the order among equal
+// keys is whatever the atomics give (the tree over them is a valid Karras
+// tree either way, and every result is ordered afterwards). The sorted keys
+// end in (keys, vals).
+constexpr int kMsdBits = 12;
+constexpr uint32_t kMsdBuckets = 1u << kMsdBits;
+constexpr uint32_t kMsdWords = 3 * kMsdBuckets;  // per sort: totals, cursors (both zeroed by the query's init), starts
+bool msd_sort_fits(uint32_t n, int bits);
+struct MsdSortJob {
+    uint32_t* keys; uint32_t* vals; uint32_t* tkeys; uint32_t* tvals;
+    uint32_t n;
+    int bits;
+    uint32_t* words;  // kMsdWords
+};
+void msd_sort_pair(const MsdSortJob* jobs, int njobs, hipStream_t s);
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals, uint32_t n, int bits,
                    uint32_t* hist, uint32_t* scan_tmp, hipStream_t s);
 // exclusive scan of n uint32 (in place allowed); writes the total to *total

@@ -228,7 +228,7 @@ extern "C" int gi_intersect_noalloc(int32_t device, const double* xyzA, uint32_t
     }
     NA_TRY(hipMemcpy(d_xf, xf, sizeof(xf), hipMemcpyHostToDevice));
     uint32_t* grid = counters + CNT_COUNT;
-    launch_query_init(counters, CNT_COUNT, grid, 3u, s);
+    launch_query_init(counters, CNT_COUNT, grid, 3u, nullptr, 0u, s);
     SortJobU32 jobs[2];
     StageJob sj[2];
     for (int k = 0; k < 2; ++k) {
@@ -267,7 +267,7 @@ extern "C" int gi_intersect_noalloc(int32_t device, const double* xyzA, uint32_t
     int kinds = (src_grid[0] && src_grid[1]) ? kExactGrid : kExactWide;  // unmoved vertices: known on the host
     uint32_t hc[CNT_COUNT + 2];
     for (int pass = 0;; ++pass) {
-        if (pass) launch_query_init(counters, CNT_COUNT, grid, 0u, s);  // queries only; the trees are built
+        if (pass) launch_query_init(counters, CNT_COUNT, grid, 0u, nullptr, 0u, s);  // queries only; the trees are built
         launch_traverse(A, B, nullptr, 0, nullptr, cand, cap_cand, counters, 2, s);
         launch_filter(A, B, cand, cap_cand, grid, exact, cap_exact, counters, 2048, s);
         launch_exact(A, B, exact, cap_exact, grid, bitsB, rec_key, rec_seg, cap_rec, seg_pool, seg_exp, cap_seg, cand,
