@@ -1537,6 +1537,50 @@ int gi_get_vertices(gi_context* ctx, int slot, double* xyz, uint32_t nv) {
     }
 }
 
+int gi_get_tree(gi_context* ctx, int slot, uint32_t n, uint32_t* keys, uint32_t* ids, uint32_t* left,
+                uint32_t* rope, uint32_t* range, uint32_t* leaf_rope) {
+    if (!ctx || slot < 0 || slot > 1) return GI_ERR_INVALID_ARGUMENT;
+    MeshSlot& m = ctx->mesh[slot];
+    if (!m.set) return GI_ERR_NOT_READY;
+    if (n != m.nt || m.forest) {
+        ctx->err = m.forest ? "the slot holds a batch" : "triangle count differs from the slot's";
+        return GI_ERR_INVALID_ARGUMENT;
+    }
+    const bool tree = left || rope || range || leaf_rope;
+    if (tree && slot == GI_MESH_B && !m.full_tree) {
+        ctx->err = "mesh B builds no tree";
+        return GI_ERR_NOT_READY;
+    }
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        build_meshes(ctx, false);
+        sync_all(ctx);
+        if (n == 0) return GI_OK;
+        const uint32_t* dk = m.sorted_in_tmp ? m.tkeys.get<uint32_t>() : m.keys.get<uint32_t>();
+        const uint32_t* dv = m.sorted_in_tmp ? m.tvals.get<uint32_t>() : m.vals.get<uint32_t>();
+        if (keys) HIP_TRY(hipMemcpy(keys, dk, 4 * (size_t)n, hipMemcpyDeviceToHost));
+        if (ids) HIP_TRY(hipMemcpy(ids, dv, 4 * (size_t)n, hipMemcpyDeviceToHost));
+        if (!tree) return GI_OK;
+        std::vector<float4> nodes(2 * (2 * (size_t)n - 1));
+        HIP_TRY(hipMemcpy(nodes.data(), m.nodes.get<float4>(), nodes.size() * sizeof(float4), hipMemcpyDeviceToHost));
+        auto word = [](float w) {
+            uint32_t u;
+            memcpy(&u, &w, 4);
+            return u;
+        };
+        for (size_t i = 0; i + 1 < n; ++i) {
+            if (left) left[i] = word(nodes[2 * i].w);
+            if (rope) rope[i] = word(nodes[2 * i + 1].w);
+        }
+        if (leaf_rope)
+            for (size_t k = 0; k < n; ++k) leaf_rope[k] = word(nodes[2 * (n - 1 + k) + 1].w);
+        if (range && n > 1) HIP_TRY(hipMemcpy(range, m.range.get<uint2>(), 8 * (size_t)(n - 1), hipMemcpyDeviceToHost));
+        return GI_OK;
+    } catch (const GiError& e) {
+        return fail(ctx, e);
+    }
+}
+
 int gi_intersect(gi_context* ctx, const gi_options* opt, gi_result* out) {
     if (!ctx || !out) return GI_ERR_INVALID_ARGUMENT;
     if (ctx->mesh[0].forest || ctx->mesh[1].forest) {

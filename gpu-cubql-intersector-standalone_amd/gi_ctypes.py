@@ -101,7 +101,7 @@ class GiError(RuntimeError):
 _lib = None
 
 SYMBOLS = ["gi_create", "gi_destroy", "gi_last_error", "gi_status_string", "gi_set_mesh", "gi_set_mesh_device",
-           "gi_set_transform", "gi_set_transform_matrix", "gi_build", "gi_get_vertices", "gi_intersect",
+           "gi_set_transform", "gi_set_transform_matrix", "gi_build", "gi_get_vertices", "gi_get_tree", "gi_intersect",
            "gi_result_free",
            "gi_intersect_oneshot", "gi_intersect_noalloc", "gi_set_batch", "gi_intersect_batch", "gi_share_mesh", "gi_get_tuning",
            "gi_set_tuning", "gi_autotune", "gi_clear_device",
@@ -130,6 +130,8 @@ def lib():
         L.gi_set_transform_matrix.argtypes = [vp, C.c_int, dp, C.c_int]
         L.gi_build.argtypes = [vp]
         L.gi_get_vertices.argtypes = [vp, C.c_int, dp, C.c_uint32]
+        u32p = C.POINTER(C.c_uint32)
+        L.gi_get_tree.argtypes = [vp, C.c_int, C.c_uint32, u32p, u32p, u32p, u32p, u32p, u32p]
         L.gi_intersect.argtypes = [vp, C.POINTER(GiOptions), C.POINTER(GiResult)]
         L.gi_result_free.argtypes = [C.POINTER(GiResult)]
         L.gi_intersect_oneshot.argtypes = [C.POINTER(GiConfig), dp, C.c_uint32, up, C.c_uint32, dp, C.c_uint32, up,
@@ -250,6 +252,19 @@ class Intersector:
         """The slot's vertices as the predicates see them (moved, snapped)."""
         out = np.zeros((nv, 3), dtype=np.float64)
         _check(lib().gi_get_vertices(self._ctx, slot, out.ctypes.data_as(C.POINTER(C.c_double)), nv), self._ctx)
+        return out
+
+    def get_tree(self, slot: int, nt: int, tree: bool = True) -> dict:
+        """The slot's sorted Morton keys and ids and, with tree, each internal
+        node's left-child word, rope and leaf range and each leaf's rope."""
+        out = {"keys": np.zeros(nt, np.uint32), "ids": np.zeros(nt, np.uint32)}
+        if tree:
+            m = max(nt - 1, 0)
+            out.update(left=np.zeros(m, np.uint32), rope=np.zeros(m, np.uint32), range=np.zeros((m, 2), np.uint32),
+                       leaf_rope=np.zeros(nt, np.uint32))
+        ptr = lambda k: out[k].ctypes.data_as(C.POINTER(C.c_uint32)) if k in out else None
+        _check(lib().gi_get_tree(self._ctx, slot, nt, ptr("keys"), ptr("ids"), ptr("left"), ptr("rope"),
+                                 ptr("range"), ptr("leaf_rope")), self._ctx)
         return out
 
     def clear_device(self):

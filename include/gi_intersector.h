@@ -172,6 +172,21 @@ int gi_build(gi_context* ctx);
  * createRigidTransformation (src/CPU/RotationTools.h:76-104) so that callers
  * can check them. */
 int gi_get_vertices(gi_context* ctx, int slot, double* xyz, uint32_t nv);
+/* Copies the slot's Morton order and BVH as the traversal reads them, for
+ * callers that check them; builds first when stale. n is the slot's triangle
+ * count. keys, ids: the sorted Morton keys and the triangle ids in that order
+ * (n each). The tree (mesh A, or a B slot that builds one): per internal node
+ * i < n - 1, left[i] its left-child word (bit 31 set: the right child is a
+ * leaf; low 31 bits: the left child, internal node j or leaf k as node
+ * n - 1 + k; the right child is the node after the left child's range),
+ * rope[i] the node after its subtree in pre-order (0xFFFFFFFF at the end) and
+ * range[2 i], range[2 i + 1] its first and last leaf; leaf_rope[k] (n) the
+ * same for leaf k. Any pointer may be null; a tree pointer for a slot that
+ * builds no tree is GI_ERR_NOT_READY. No reference counterpart: the
+ * reference's cuBQL tree (include/third-party/cubql/level_cut_builder.h) is
+ * internal state; this lets tests hold the tree to Karras' definition. */
+int gi_get_tree(gi_context* ctx, int slot, uint32_t n, uint32_t* keys, uint32_t* ids, uint32_t* left,
+                uint32_t* rope, uint32_t* range, uint32_t* leaf_rope);
 int gi_intersect(gi_context* ctx, const gi_options* opt, gi_result* out);
 int gi_result_free(gi_result* res);
 
