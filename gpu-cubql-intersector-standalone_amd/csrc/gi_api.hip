@@ -513,11 +513,18 @@ void build_meshes(gi_context* ctx, bool rebuild, int time_mask = 0, bool zero_co
         if (!m.forest && slot_needs_stage(m, rebuild)) stage_mask |= 1u << sl;
     }
     // the bucket sorts' totals and cursors start at zero (both slots' words,
-    // whichever sort a staged slot takes)
-    uint32_t* msd_words = stage_mask ? ctx->msd.ensure<uint32_t>(2 * kMsdWords) : nullptr;
+    // whichever sort a staged slot takes); the split buckets' words are
+    // zeroed once, when allocated (every sort leaves them zero)
+    uint32_t* msd_words = nullptr;
+    if (stage_mask) {
+        const bool fresh = ctx->msd.bytes < sizeof(uint32_t) * 2 * (size_t)kMsdWords;
+        if (fresh && ctx->capturing) throw GiError{GI_ERR_INTERNAL, "bucket-sort words allocated during a capture"};
+        msd_words = ctx->msd.ensure<uint32_t>(2 * (size_t)kMsdWords);
+        if (fresh) HIP_TRY(hipMemsetAsync(msd_words, 0, sizeof(uint32_t) * 2 * (size_t)kMsdWords, ctx->stream));
+    }
     if (stage_mask || zero_counters)
         launch_query_init(counters, zero_counters ? CNT_COUNT : 0u, counters + CNT_COUNT, stage_mask, msd_words,
-                          msd_words ? 2 * kMsdWords : 0u, ctx->stream);
+                          msd_words ? 2u : 0u, ctx->stream);
     // batch sides stage on their own streams (fork/join); single meshes share
     // one staging launch on the context's stream
     const bool forest = ctx->mesh[0].forest || ctx->mesh[1].forest;

@@ -102,10 +102,11 @@ void launch_inspect_mesh(const double* xyz, uint32_t nv, const uint32_t* tri, ui
 void launch_forest_globalize(uint32_t* tri, const uint2* ttiles, uint32_t n_ttiles, const uint32_t* toff,
                              const uint32_t* voff, MeshInspect* out, hipStream_t s);
 // grid flag = 1, bounds6 = empty box (order-preserving keys)
-// zero counters[0, nzero) (one word per kCntStride) and words[0, nwords),
-// and set grid_flags[slot] = 1 for the slots in stage_mask
+// zero counters[0, nzero) (one word per kCntStride) and the first
+// kMsdZeroWords of each of msd_sorts bucket-sort word sets (msd_words +
+// k kMsdWords), and set grid_flags[slot] = 1 for the slots in stage_mask
 void launch_query_init(uint32_t* counters, uint32_t nzero, uint32_t* grid_flags, uint32_t stage_mask,
-                       uint32_t* words, uint32_t nwords, hipStream_t s);
+                       uint32_t* msd_words, uint32_t msd_sorts, hipStream_t s);
 // Grid-stride width of the vertex part of a mesh's stage launch.
 constexpr uint32_t kPrepareMaxBlocks = 4096;
 // One mesh's staging: its vertices (moved by *xf, in device memory so that a
@@ -186,9 +187,27 @@ void radix_sort_u32_pair(const SortJobU32* jobs, int njobs, hipStream_t s, int* 
 // keys is whatever the atomics give (the tree over them is a valid Karras
 // tree either way, and every result is ordered afterwards). The sorted keys
 // end in (keys, vals).
+//
+// A bucket of more than kMsdBig keys (the first kMsdBigSlots of them per
+// sort) is split: the scatter also counts its keys' low parts in a global
+// histogram, and workgroups of kMsdSlice keys each reserve their places per
+// low part with global atomics, instead of one workgroup walking the whole
+// bucket (a UV sphere's pole cells hold ~15K keys).
 constexpr int kMsdBits = 12;
 constexpr uint32_t kMsdBuckets = 1u << kMsdBits;
-constexpr uint32_t kMsdWords = 3 * kMsdBuckets;  // per sort: totals, cursors (both zeroed by the query's init), starts
+constexpr uint32_t kMsdBig = 4096, kMsdSlice = 2048;
+constexpr int kMsdBigSlots = 32;
+// Words per sort: the totals and cursors (zeroed by the query's init), the
+// starts and the buckets' slots (written by the scatter), the slot table
+// (bucket of each slot, then the slots' first slice: 33 words), then the
+// split buckets' done counters, low-part histograms and low-part cursors:
+// zero when allocated, and every sort leaves them zero (the last slice of a
+// bucket clears its slot).
+constexpr uint32_t kMsdOffCursor = kMsdBuckets, kMsdOffStart = 2 * kMsdBuckets, kMsdOffSlot = 3 * kMsdBuckets,
+                   kMsdOffTab = 4 * kMsdBuckets, kMsdOffDone = kMsdOffTab + 128, kMsdOffHist = kMsdOffDone + 128,
+                   kMsdOffBigCursor = kMsdOffHist + kMsdBigSlots * kMsdBuckets,
+                   kMsdWords = kMsdOffBigCursor + kMsdBigSlots * kMsdBuckets;
+constexpr uint32_t kMsdZeroWords = 2 * kMsdBuckets;  // zeroed by every query's init
 bool msd_sort_fits(uint32_t n, int bits);
 struct MsdSortJob {
     uint32_t* keys; uint32_t* vals; uint32_t* tkeys; uint32_t* tvals;

@@ -23,6 +23,11 @@ pytestmark = pytest.mark.gpu
 
 SENTINEL = 0xFFFFFFFF
 RIGHT_LEAF = 0x80000000
+QUERY_AXIS_BITS = 5  # B's Morton keys (queries only): 5 bits per axis
+
+
+def _axis_bits(nt):  # a tree's Morton keys: 8 bits per axis up to 2^21 triangles, else 10
+    return 8 if nt <= (1 << 21) else 10
 
 
 def _bit_length(x):
@@ -99,6 +104,29 @@ def _check_order(t, nt):
     assert np.array_equal(np.sort(ids), np.arange(nt, dtype=np.uint32)), "ids are not a permutation"
 
 
+def _compact(v):
+    v = v & 0x09249249
+    v = (v | (v >> 2)) & 0x030C30C3
+    v = (v | (v >> 4)) & 0x0300F00F
+    v = (v | (v >> 8)) & 0x030000FF
+    return (v | (v >> 16)) & 0x3FF
+
+
+def _check_keys(x, t, keys, ids, axis):
+    """Every sorted key belongs to its triangle: the Morton cell it names
+    is, within one cell per axis (the GPU's keys come from FP32 boxes), that
+    of the triangle's box centre in the mesh's bounds."""
+    k = keys.astype(np.int64)
+    q = np.stack([_compact(k >> 2), _compact(k >> 1), _compact(k)], axis=1)
+    p = x[t[ids.astype(np.int64)]]
+    c = 0.5 * p.min(axis=1) + 0.5 * p.max(axis=1)
+    lo, ext = x.min(axis=0), x.max(axis=0) - x.min(axis=0)
+    cells = 1 << axis
+    want = np.clip(np.floor((c - lo) / np.where(ext > 0, ext, 1.0) * cells), 0, cells - 1).astype(np.int64)
+    bad = np.nonzero(np.abs(q - want).max(axis=1) > 1)[0]
+    assert len(bad) == 0, f"{len(bad)} keys do not belong to their triangles, first at {bad[:5]}"
+
+
 def _check(xa, ta, xb, tb, chunk=None, transform=None):
     eng = gi.Intersector(device=0)
     try:
@@ -117,6 +145,9 @@ def _check(xa, ta, xb, tb, chunk=None, transform=None):
             assert len(bad) == 0, f"{name} differs at {len(bad)} nodes, first {bad[:5]}"
         tb_ = eng.get_tree(1, len(tb), tree=False)  # B: its query order only
         _check_order(tb_, len(tb))
+        if not transform:
+            _check_keys(xa, ta, t["keys"], t["ids"], _axis_bits(len(ta)))
+        _check_keys(xb, tb, tb_["keys"], tb_["ids"], QUERY_AXIS_BITS)
         return t
     finally:
         eng.close()
@@ -152,3 +183,51 @@ def test_small_torus_tree_is_karras():
     xs, ts = mg.torus(37, 23)
     (xb, tb), _ = mg.icosphere_pair(3)
     _check(mg.snap(xs), ts, xb, tb)
+
+
+# Buckets (the top 4 key bits per axis) of more than 4,096 keys are split
+# over several workgroups, up to 32 such buckets per sort, through a global
+# low-part histogram that each sort leaves cleared.
+
+def test_query_side_split_buckets():
+    # the sphere as B: 24 split buckets of 15-bit keys (3 low bits)
+    (xs, ts), (xt, tt) = mg.sphere_torus(1024, 512)
+    _check(xt, tt, xs, ts)
+
+
+def test_more_split_buckets_than_slots():
+    # 2^21 triangles: 66 buckets over 4,096 keys, the 34 past the slots sorted
+    # by one workgroup each
+    xs, ts = mg.uv_sphere(2048, 512, 3)
+    (xb, tb), _ = mg.icosphere_pair(3)
+    _check(mg.snap(xs), ts, xb, tb)
+
+
+def test_split_buckets_repeated_sorts():
+    # one context sorting meshes with and without split buckets in turn (by
+    # gi_build after new meshes, then by a query that rebuilds): the slots'
+    # histograms and cursors must start every sort at zero
+    (xs, ts), (xt, tt) = mg.sphere_torus(1024, 512)
+    xs2, ts2 = mg.uv_sphere(2048, 512, 3)
+    xs2 = mg.snap(xs2)
+    eng = gi.Intersector(device=0)
+
+    def check(xa, ta, xb, tb):
+        t = eng.get_tree(0, len(ta), tree=False)
+        _check_order(t, len(ta))
+        _check_keys(xa, ta, t["keys"], t["ids"], _axis_bits(len(ta)))
+        u = eng.get_tree(1, len(tb), tree=False)
+        _check_order(u, len(tb))
+        _check_keys(xb, tb, u["keys"], u["ids"], QUERY_AXIS_BITS)
+
+    try:
+        for xa, ta, xb, tb in ((xs, ts, xt, tt), (xt, tt, xs, ts), (xs2, ts2, xt, tt), (xs, ts, xt, tt)):
+            eng.set_mesh(0, xa, ta)
+            eng.set_mesh(1, xb, tb)
+            eng.build()
+            check(xa, ta, xb, tb)
+            res = eng.intersect_raw(output_on_device=True, want_segments=False, rebuild=True)
+            gi.lib().gi_result_free(gi.C.byref(res))
+            check(xa, ta, xb, tb)
+    finally:
+        eng.close()
