@@ -499,12 +499,12 @@ def main():
 
     tuning = eng.get_tuning() if args.no_autotune else eng.autotune(reps=3, rebuild=True)
 
-    def step(timing):
+    def step(timing, convert=True):
         res = eng.intersect_raw(output_on_device=True, want_segments=True, rebuild=True, use_graph=args.graph,
                                 stage_timing=timing)
-        st = res.stats.as_dict()
+        st = type(res.stats).from_buffer_copy(res.stats)  # the C struct, copied
         gi.lib().gi_result_free(gi.C.byref(res))
-        return st
+        return st.as_dict() if convert else st
 
     # a step returns once its results are complete in device memory (the
     # library synchronises its streams), so the clock brackets finished work.
@@ -520,11 +520,12 @@ def main():
     warm = [step(0) for _ in range(max(args.warmup, 1))]
     warm_ms = {k: float(np.mean([w[stat_of[k]] for w in warm])) for k in flags}
     dom = max(warm_ms, key=warm_ms.get)
-    stats = []
+    raw = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        stats.append(step(flags[dom]))
+        raw.append(step(flags[dom], convert=False))  # the stats struct; its Python dict after the clock stops
     t1 = time.perf_counter()
+    stats = [r.as_dict() for r in raw]
     ms = (t1 - t0) * 1000.0 / args.steps
     st = stats[-1]
     for k in ("candidates", "pairs", "segments"):

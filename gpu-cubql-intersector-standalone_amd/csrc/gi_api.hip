@@ -695,8 +695,8 @@ void enqueue_output_small(gi_context* ctx, const gi_options& o) {
     uint32_t* counters = ctx->counters.get<uint32_t>();
     const int bitsB = bits_for(ctx->mesh[1].nt);
     launch_output_small(ctx->rec_key.get<uint64_t>(), ctx->rec_seg.get<uint2>(), counters, ctx->cap_rec, ctx->cap_seg,
-                        bits_for(ctx->mesh[0].nt) + bitsB, bitsB, o.want_segments != 0, ctx->tkeys.get<uint64_t>(),
-                        ctx->pairs.get<uint2>(), ctx->order.get<uint32_t>(), ctx->offs.get<uint32_t>(),
+                        bits_for(ctx->mesh[0].nt) + bitsB, bitsB, o.want_segments != 0, ctx->pairs.get<uint2>(),
+                        ctx->order.get<uint32_t>(), ctx->offs.get<uint32_t>(),
                         ctx->segs.get<uint64_t>(), ctx->exp2.get<int32_t>(), ctx->f64.get<double>(),
                         (uint32_t)output_seg_slots(ctx), ctx->seg_pool.get<uint64_t>(), ctx->seg_exp.get<int32_t>(),
                         counters + CNT_COUNT, ctx->d_counters_host, ctx->stream);
@@ -771,7 +771,6 @@ void run_exact_grid_slow(gi_context* ctx, const gi_options& o, bool timed) {
 
 void ensure_output_bufs(gi_context* ctx, const gi_options& o) {
     const uint32_t cap = std::min(output_small_max(), ctx->cap_rec);
-    ctx->tkeys.ensure<uint64_t>(output_small_runs_elems());  // sorted runs of the small output
     ctx->pairs.ensure<uint2>(std::max(cap, 1u));
     ctx->order.ensure<uint32_t>(std::max(cap, 1u));
     ctx->offs.ensure<uint32_t>((size_t)cap + 1);

@@ -295,13 +295,11 @@ void launch_exact_grid_slow(MeshDev A, MeshDev B, const uint32_t* grid_flags, in
                             uint32_t defer_cap, uint2* slow, uint32_t slow_cap, uint32_t* counters, int blocks,
                             hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
-// Sorted output of up to output_small_max() result records in one workgroup,
+// Sorted output of up to output_small_max() result records in one launch,
 // then their segments; the record count is read on the device, so this is
 // queued before the counters reach the host. Does nothing unless
 // output_small_fits(R, key_bits) and nothing overflowed.
 uint32_t output_small_max();
-// runs: the sorted runs, their segment-count prefixes and totals (uint64 elements)
-size_t output_small_runs_elems();
 bool output_small_fits(uint32_t R, int key_bits);
 // Segments are gathered at the width the exact stage wrote (grid flags on
 // the device: 2 limbs, else 4 with scales copied to seg_exp_out); seg_out
@@ -310,7 +308,7 @@ bool output_small_fits(uint32_t R, int key_bits);
 // (counters[k * kCntStride]) and the two grid flags (counters[CNT_COUNT],
 // [CNT_COUNT + 1]) at the same offsets, in place of a copy after the launches.
 void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const uint32_t* counters, uint32_t cap_rec,
-                         uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint64_t* runs, uint2* pairs,
+                         uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint2* pairs,
                          uint32_t* order, uint32_t* offs, uint64_t* seg_out, int32_t* seg_exp_out, double* seg_f64,
                          uint32_t out_slots, const uint64_t* seg_pool, const int32_t* seg_exp,
                          const uint32_t* grid_flags, uint32_t* counters_out, hipStream_t s);

@@ -3446,34 +3446,13 @@ void launch_gather_segments(const uint32_t* order, const uint2* rec_seg, const u
     GI_LAUNCH_CHECK();
 }
 
-// Sorted output of up to 16,384 result records in two launches, queued
+// Sorted output of up to kOutSmallMax result records in one launch, queued
 // before the host reads the counters: the record count R comes from device
 // memory, and nothing runs unless R fits (the host then takes the multi-tile
-// path after the read-back). The multi-tile path spends 20-odd launches on a
-// few thousand records; a single workgroup is bound by the VALU of one CU
-// (the ballot-match ranking costs ~8 wave64 instructions per key bit).
-//   k_output_runs   kRuns workgroups, each a bitonic sort of kRunLen
-//                   records, two per thread: 56 of its 66 compare steps
-//                   between lanes of one wave (DPP, swizzles), 10 through
-//                   LDS (an LSD radix sort with ballot-match ranking took
-//                   ~2 us per 8-bit pass, 10.4 us for 40-bit keys);
-//                   and, per run position, the exclusive prefix of the
-//                   segment counts in run order (cpre, run totals after it);
-//   k_output_place  every record's place is its position in its run plus,
-//                   for each other run, how many of that run's keys are
-//                   smaller (binary searches in LDS, the runs' searches
-//                   interleaved), and its segment offset the sum of the
-//                   runs' count prefixes at those positions: one thread per
-//                   record, 32 workgroups (a single merging workgroup took
-//                   23 us for 10^4 records); writes per place the pair, the
-//                   record index (order) and the offset (total at offs[R]).
-// Each key (faceA << bitsB | faceB) carries its record index in the low
-// idx_bits, so the sort moves no separate values (keys are distinct).
-constexpr int RUN_THREADS = 1024, RUN_ITEMS = 2;
-constexpr uint32_t kRunLen = RUN_THREADS * RUN_ITEMS;
-constexpr uint32_t kRuns = 8;
-constexpr uint32_t kOutSmallMax = kRunLen * kRuns;
-constexpr int PLACE_THREADS = 512;
+// path after the read-back, 20-odd launches). Each key (faceA << bitsB |
+// faceB) carries its record index in the low idx_bits, so the sort moves no
+// separate values (keys are distinct).
+constexpr uint32_t kOutSmallMax = 16384;
 
 __host__ __device__ __forceinline__ int bits_of_count(uint32_t n) { return n <= 1 ? 1 : 32 - __builtin_clz(n - 1); }
 
@@ -3507,14 +3486,14 @@ __device__ __forceinline__ uint64_t lane_xor_u64(uint64_t x) {
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
-// Bitonic sort of a run of kRunLen keys, thread t holding positions 2t and
-// 2t + 1 (k0, k1). Merge K, step J compares positions i and i ^ J, keeping
+// Bitonic sort of a block's keys, thread t holding positions 2t and 2t + 1
+// (k0, k1). Merge K, step J compares positions i and i ^ J, keeping
 // the smaller one at the lower position where i & K is zero (ascending
 // blocks) and the larger one elsewhere. J = 1 is within a thread, J = 2 ..
 // 64 between lanes of a wave (lane distance J / 2), J >= 128 between waves
 // through LDS (s), one pair per thread.
 template <int K, int J>
-__device__ __forceinline__ void run_bitonic_step(uint64_t& k0, uint64_t& k1, uint64_t* s, uint32_t t) {
+__device__ __forceinline__ void run_bitonic_step(uint64_t& k0, uint64_t& k1, uint64_t* s, uint32_t t, bool active) {
     if constexpr (J >= 128) {
         const uint32_t i = ((t & ~(uint32_t)(J - 1)) << 1) | (t & (uint32_t)(J - 1));
         const bool asc = (i & (uint32_t)K) == 0;
@@ -3524,11 +3503,13 @@ __device__ __forceinline__ void run_bitonic_step(uint64_t& k0, uint64_t& k1, uin
         s[i + J] = swap ? a : b;
         __syncthreads();
     } else if constexpr (J >= 2) {
+        if (!active) return;  // the whole wave (active is the same on its lanes)
         const uint64_t p0 = lane_xor_u64<J / 2>(k0), p1 = lane_xor_u64<J / 2>(k1);
         const bool keep_min = ((t & (uint32_t)(J / 2)) == 0) == ((t & (uint32_t)(K / 2)) == 0);
         k0 = keep_min ? (k0 < p0 ? k0 : p0) : (k0 < p0 ? p0 : k0);
         k1 = keep_min ? (k1 < p1 ? k1 : p1) : (k1 < p1 ? p1 : k1);
     } else {
+        if (!active) return;
         const bool asc = (t & (uint32_t)(K / 2)) == 0;
         const uint64_t lo = k0 < k1 ? k0 : k1, hi = k0 < k1 ? k1 : k0;
         k0 = asc ? lo : hi;
@@ -3537,142 +3518,39 @@ __device__ __forceinline__ void run_bitonic_step(uint64_t& k0, uint64_t& k1, uin
 }
 template <int K, int J>
 struct RunBitonicMerge {
-    __device__ static __forceinline__ void run(uint64_t& k0, uint64_t& k1, uint64_t* s, uint32_t t) {
+    __device__ static __forceinline__ void run(uint64_t& k0, uint64_t& k1, uint64_t* s, uint32_t t, bool active) {
         if constexpr (J == K / 2 && J >= 128) {  // into LDS for the cross-wave steps
             s[2 * t] = k0;
             s[2 * t + 1] = k1;
             __syncthreads();
         }
-        run_bitonic_step<K, J>(k0, k1, s, t);
+        run_bitonic_step<K, J>(k0, k1, s, t, active);
         if constexpr (J == 128) {  // back to registers
             k0 = s[2 * t];
             k1 = s[2 * t + 1];
             __syncthreads();
         }
-        RunBitonicMerge<K, J / 2>::run(k0, k1, s, t);
+        RunBitonicMerge<K, J / 2>::run(k0, k1, s, t, active);
     }
 };
 template <int K>
 struct RunBitonicMerge<K, 0> {
-    __device__ static __forceinline__ void run(uint64_t&, uint64_t&, uint64_t*, uint32_t) {}
+    __device__ static __forceinline__ void run(uint64_t&, uint64_t&, uint64_t*, uint32_t, bool) {}
 };
+// Every aligned block of K positions of the block's 2 x blockDim keys ends
+// sorted (the first one ascending). Waves with active false skip the
+// steps within registers; all threads take part in the LDS steps (K >= 256).
 template <int K>
 struct RunBitonicSort {
-    __device__ static __forceinline__ void run(uint64_t& k0, uint64_t& k1, uint64_t* s, uint32_t t) {
-        RunBitonicSort<K / 2>::run(k0, k1, s, t);
-        RunBitonicMerge<K, K / 2>::run(k0, k1, s, t);
+    __device__ static __forceinline__ void run(uint64_t& k0, uint64_t& k1, uint64_t* s, uint32_t t, bool active) {
+        RunBitonicSort<K / 2>::run(k0, k1, s, t, active);
+        RunBitonicMerge<K, K / 2>::run(k0, k1, s, t, active);
     }
 };
 template <>
 struct RunBitonicSort<1> {
-    __device__ static __forceinline__ void run(uint64_t&, uint64_t&, uint64_t*, uint32_t) {}
+    __device__ static __forceinline__ void run(uint64_t&, uint64_t&, uint64_t*, uint32_t, bool) {}
 };
-
-__global__ __launch_bounds__(RUN_THREADS) void k_output_runs(const uint64_t* __restrict__ rec_key,
-                                                             const uint2* __restrict__ rec_seg,
-                                                             const uint32_t* __restrict__ counters, uint32_t cap_rec,
-                                                             uint32_t cap_seg, int key_bits, int want_segments,
-                                                             uint64_t* __restrict__ runs,
-                                                             uint32_t* __restrict__ cpre) {
-    static_assert(RUN_ITEMS == 2 && kRunLen == 2 * RUN_THREADS, "two keys per thread");
-    uint32_t R;
-    if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R)) return;
-    const uint32_t r0 = blockIdx.x * kRunLen;
-    if (r0 >= R) return;
-    const uint32_t n = min(kRunLen, R - r0);
-    const int idx_bits = bits_of_count(R);
-    __shared__ uint64_t skey[kRunLen];
-    __shared__ uint32_t wsum[RUN_THREADS / 64];
-    const uint32_t t = threadIdx.x, lane = lane_id(), wave = t >> 6;
-    // positions past n hold the largest key, which sorts them last (real
-    // keys stay below 2^63: output_small_ok)
-    const uint32_t q0 = 2 * t;
-    uint64_t k0 = q0 < n ? (rec_key[r0 + q0] << idx_bits) | (r0 + q0) : ~0ull;
-    uint64_t k1 = q0 + 1 < n ? (rec_key[r0 + q0 + 1] << idx_bits) | (r0 + q0 + 1) : ~0ull;
-    RunBitonicSort<kRunLen>::run(k0, k1, skey, t);
-    if (q0 < n) runs[r0 + q0] = k0;
-    if (q0 + 1 < n) runs[r0 + q0 + 1] = k1;
-    // segment counts in run order, then their exclusive prefix
-    const uint64_t idx_mask = (1ull << idx_bits) - 1ull;
-    uint32_t c[RUN_ITEMS];
-    c[0] = want_segments && q0 < n ? rec_seg[(uint32_t)(k0 & idx_mask)].y : 0u;
-    c[1] = want_segments && q0 + 1 < n ? rec_seg[(uint32_t)(k1 & idx_mask)].y : 0u;
-    const uint32_t sum = c[0] + c[1];
-    uint32_t x = sum;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (lane >= (uint32_t)o) x += y;
-    }
-    if (lane == 63) wsum[wave] = x;
-    __syncthreads();
-    uint32_t before = x - sum, all = 0;
-    for (uint32_t w = 0; w < RUN_THREADS / 64; ++w) {
-        before += w < wave ? wsum[w] : 0u;
-        all += wsum[w];
-    }
-    if (q0 < n) cpre[r0 + q0] = before;
-    if (q0 + 1 < n) cpre[r0 + q0 + 1] = before + c[0];
-    if (t == 0) cpre[kOutSmallMax + blockIdx.x] = all;
-}
-
-__global__ __launch_bounds__(PLACE_THREADS) void k_output_place(const uint64_t* __restrict__ runs,
-                                                                const uint32_t* __restrict__ cpre,
-                                                                const uint32_t* __restrict__ counters, uint32_t cap_rec,
-                                                                uint32_t cap_seg, int key_bits, int bitsB,
-                                                                uint2* __restrict__ pairs,
-                                                                uint32_t* __restrict__ order,
-                                                                uint32_t* __restrict__ offs,
-                                                                uint32_t* __restrict__ counters_out) {
-    // the counters are final here (every stage that counts has run): the
-    // first workgroup copies the counter words (one per kCntStride) and the
-    // two grid flags to the host's pinned read-back buffer
-    if (blockIdx.x == 0 && counters_out && threadIdx.x < CNT_COUNT / kCntStride + 2) {
-        const uint32_t i = threadIdx.x < CNT_COUNT / kCntStride ? threadIdx.x * kCntStride
-                                                                 : CNT_COUNT + threadIdx.x - CNT_COUNT / kCntStride;
-        counters_out[i] = counters[i];
-    }
-    uint32_t R;
-    if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R)) return;
-    const uint32_t g = blockIdx.x * PLACE_THREADS + threadIdx.x;  // this thread's record, in run order
-    if (blockIdx.x * PLACE_THREADS >= R) return;
-    const int idx_bits = bits_of_count(R);
-    __shared__ uint64_t sk[kOutSmallMax];
-    {  // two keys per 16-byte load (runs is 256-byte aligned; R <= kOutSmallMax)
-        const uint4* __restrict__ runs4 = (const uint4*)runs;
-        for (uint32_t i = threadIdx.x; 2 * i < R; i += PLACE_THREADS) {
-            const uint4 v = runs4[i];
-            sk[2 * i] = ((uint64_t)v.y << 32) | v.x;
-            if (2 * i + 1 < R) sk[2 * i + 1] = ((uint64_t)v.w << 32) | v.z;
-        }
-    }
-    __syncthreads();
-    if (g >= R) return;
-    const uint64_t x = sk[g];
-    const uint32_t own = g / kRunLen;
-    uint32_t pos = 0, off = 0;
-#pragma unroll
-    for (uint32_t r = 0; r < kRuns; ++r) {
-        const uint32_t r0 = r * kRunLen, len = r0 < R ? min(kRunLen, R - r0) : 0u;
-        // keys of run r below x (keys are distinct: each carries its record index)
-        uint32_t p = 0;
-#pragma unroll
-        for (uint32_t step = kRunLen; step > 0; step >>= 1)
-            if (p + step <= len && sk[r0 + p + step - 1] < x) p += step;
-        p = r == own ? g - r0 : p;
-        pos += p;
-        off += p < len ? cpre[r0 + p] : (len ? cpre[kOutSmallMax + r] : 0u);
-    }
-    const uint64_t pk = x >> idx_bits;
-    const uint64_t maskB = (1ull << bitsB) - 1ull;
-    pairs[pos] = make_uint2((uint32_t)(pk >> bitsB), (uint32_t)(pk & maskB));
-    order[pos] = (uint32_t)(x & ((1ull << idx_bits) - 1ull));
-    offs[pos] = off;
-    if (g == 0) {
-        uint32_t all = 0;
-        for (uint32_t r = 0; r * kRunLen < R; ++r) all += cpre[kOutSmallMax + r];
-        offs[R] = all;
-    }
-}
 
 // segments of the sorted records, for the records k_output_place ordered
 __global__ __launch_bounds__(256) void k_gather_segments_small(const uint32_t* __restrict__ order,
@@ -3694,23 +3572,191 @@ __global__ __launch_bounds__(256) void k_gather_segments_small(const uint32_t* _
         else copy_record_segments<kSegLimbsWide>(rec_seg[order[i]], seg_offsets[i], pool, pool_exp, out, out_exp, f64);
 }
 
-uint32_t output_small_max() { return kOutSmallMax; }
+// kOutBuckets workgroups, each owning the records whose key has its index
+// in the key's top eight bits (a range of A's faces). Every workgroup reads
+// all R records: it counts those of lower ranges and their segments (its
+// output starts there) and packs its own into LDS in record order; a
+// bitonic sort in registers orders them (up to 2 x OB_THREADS keys; the
+// waves past the keys skip the in-register steps, and up to 128 keys take
+// no barrier), or in LDS for a larger range (any size up to R); then each
+// goes straight to its place with its pair, record index and segment offset
+// (a block scan of the segment counts in sorted order). Measured against
+// 64 ranges sorted in LDS with a block barrier per step (19.7 us for 10^4
+// records) and two launches (2,048-record runs, then every record's rank in
+// the other runs by binary searches: 24 us).
+constexpr uint32_t kOutBuckets = 256;
+constexpr int OB_THREADS = 1024, OB_ITEMS = (int)(kOutSmallMax / OB_THREADS);
+static_assert(kOutSmallMax % OB_THREADS == 0, "whole items per thread");
 
-size_t output_small_runs_elems() { return kOutSmallMax + (kOutSmallMax + kRuns + 1) / 2; }
+// Block-wide sums over the 1024-thread block of four counts at once: the
+// exclusive prefix of a and the totals of all four.
+__device__ __forceinline__ uint32_t ob_scan4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t (&tot)[4],
+                                            uint4* ws) {
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    uint32_t x = a;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        b += __shfl_xor(b, o);
+        c += __shfl_xor(c, o);
+        d += __shfl_xor(d, o);
+    }
+    if (lane == 63) ws[wave] = make_uint4(x, b, c, d);
+    __syncthreads();
+    uint32_t off = 0;
+    tot[0] = tot[1] = tot[2] = tot[3] = 0;
+    for (uint32_t w = 0; w < OB_THREADS / 64; ++w) {
+        const uint4 q = ws[w];
+        off += w < wave ? q.x : 0u;
+        tot[0] += q.x;
+        tot[1] += q.y;
+        tot[2] += q.z;
+        tot[3] += q.w;
+    }
+    __syncthreads();
+    return off + x - a;
+}
+
+// exclusive sum over the 1024-thread block; total to tot
+__device__ __forceinline__ uint32_t ob_exclusive_sum(uint32_t v, uint32_t& tot, uint4* ws) {
+    uint32_t t4[4];
+    const uint32_t e = ob_scan4(v, 0u, 0u, 0u, t4, ws);
+    tot = t4[0];
+    return e;
+}
+
+template <int K>
+__device__ __forceinline__ void ob_sort_regs(uint64_t* sk, uint32_t t) {
+    uint64_t k0 = sk[2 * t], k1 = sk[2 * t + 1];
+    if constexpr (K >= 256) __syncthreads();  // every key read before the LDS steps write sk
+    RunBitonicSort<K>::run(k0, k1, sk, t, (t & ~63u) * 2 < (uint32_t)K);  // the wave holds keys of the block
+    if (2 * t < (uint32_t)K) {
+        sk[2 * t] = k0;
+        sk[2 * t + 1] = k1;
+    }
+}
+
+__global__ __launch_bounds__(OB_THREADS) void k_output_buckets(const uint64_t* __restrict__ rec_key,
+                                                               const uint2* __restrict__ rec_seg,
+                                                               const uint32_t* __restrict__ counters, uint32_t cap_rec,
+                                                               uint32_t cap_seg, int key_bits, int bitsB,
+                                                               int want_segments, uint2* __restrict__ pairs,
+                                                               uint32_t* __restrict__ order,
+                                                               uint32_t* __restrict__ offs,
+                                                               uint32_t* __restrict__ counters_out) {
+    // the counters are final here (every stage that counts has run): the
+    // first workgroup copies the counter words (one per kCntStride) and the
+    // two grid flags to the host's pinned read-back buffer
+    if (blockIdx.x == 0 && counters_out && threadIdx.x < CNT_COUNT / kCntStride + 2) {
+        const uint32_t i = threadIdx.x < CNT_COUNT / kCntStride ? threadIdx.x * kCntStride
+                                                                 : CNT_COUNT + threadIdx.x - CNT_COUNT / kCntStride;
+        counters_out[i] = counters[i];
+    }
+    uint32_t R;
+    if (!output_small_ok(counters, cap_rec, cap_seg, key_bits, R)) return;
+    __shared__ uint64_t sk[kOutSmallMax];
+    __shared__ uint4 ws[OB_THREADS / 64];
+    const uint32_t t = threadIdx.x, mine = blockIdx.x;
+    const int idx_bits = bits_of_count(R);
+    const int sh = key_bits > 8 ? key_bits - 8 : 0;  // the record keys are below 2^key_bits
+    // this thread's records: t, t + OB_THREADS, ... (all loaded before any
+    // is looked at)
+    uint64_t rk[OB_ITEMS];
+    uint32_t ns[OB_ITEMS];
+#pragma unroll
+    for (int it = 0; it < OB_ITEMS; ++it) {
+        const uint32_t i = it * OB_THREADS + t;
+        rk[it] = i < R ? rec_key[i] : 0ull;
+        ns[it] = (i < R && want_segments) ? rec_seg[i].y : 0u;
+    }
+    uint32_t own = 0, below = 0, seg_below = 0, seg_all = 0, own_mask = 0;
+#pragma unroll
+    for (int it = 0; it < OB_ITEMS; ++it) {
+        if (it * OB_THREADS + t >= R) continue;
+        const uint32_t b = (uint32_t)(rk[it] >> sh);
+        seg_all += ns[it];
+        below += b < mine ? 1u : 0u;
+        seg_below += b < mine ? ns[it] : 0u;
+        if (b == mine) {
+            own_mask |= 1u << it;
+            ++own;
+        }
+    }
+    uint32_t tot[4];
+    uint32_t pos = ob_scan4(own, below, seg_below, seg_all, tot, ws);
+    const uint32_t n = tot[0];
+    below = tot[1];
+    seg_below = tot[2];
+    if (mine == 0 && t == 0) offs[R] = tot[3];
+    if (n == 0) return;
+#pragma unroll
+    for (int it = 0; it < OB_ITEMS; ++it)
+        if ((own_mask >> it) & 1u) sk[pos++] = (rk[it] << idx_bits) | (it * OB_THREADS + t);
+    uint32_t m = 2;
+    while (m < n) m <<= 1;
+    for (uint32_t i = n + t; i < m; i += OB_THREADS) sk[i] = ~0ull;  // real keys stay below 2^63
+    __syncthreads();
+    if (m <= 2u * OB_THREADS) {
+        switch (m) {  // the same on every thread
+            case 2: if (t < 64) ob_sort_regs<2>(sk, t); break;
+            case 4: if (t < 64) ob_sort_regs<4>(sk, t); break;
+            case 8: if (t < 64) ob_sort_regs<8>(sk, t); break;
+            case 16: if (t < 64) ob_sort_regs<16>(sk, t); break;
+            case 32: if (t < 64) ob_sort_regs<32>(sk, t); break;
+            case 64: if (t < 64) ob_sort_regs<64>(sk, t); break;
+            case 128: if (t < 64) ob_sort_regs<128>(sk, t); break;
+            case 256: ob_sort_regs<256>(sk, t); break;
+            case 512: ob_sort_regs<512>(sk, t); break;
+            case 1024: ob_sort_regs<1024>(sk, t); break;
+            default: ob_sort_regs<2048>(sk, t); break;
+        }
+    } else {  // in LDS, a block barrier per step
+        for (uint32_t k = 2; k <= m; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = t; i < m / 2; i += OB_THREADS) {
+                    const uint32_t lo = ((i & ~(j - 1u)) << 1) | (i & (j - 1u)), hi = lo + j;
+                    const uint64_t a = sk[lo], b = sk[hi];
+                    if ((a > b) == ((lo & k) == 0u)) {
+                        sk[lo] = b;
+                        sk[hi] = a;
+                    }
+                }
+                __syncthreads();
+            }
+    }
+    __syncthreads();
+    const uint64_t idx_mask = (1ull << idx_bits) - 1ull, maskB = (1ull << bitsB) - 1ull;
+    uint32_t carry = seg_below;
+    for (uint32_t p0 = 0; p0 < n; p0 += OB_THREADS) {  // the same rounds on every thread
+        const uint32_t p = p0 + t;
+        const uint64_t x = p < n ? sk[p] : 0ull;
+        const uint32_t rec = (uint32_t)(x & idx_mask);
+        const uint32_t nsp = (p < n && want_segments) ? rec_seg[rec].y : 0u;
+        uint32_t all;
+        const uint32_t e = ob_exclusive_sum(nsp, all, ws);
+        if (p < n) {
+            const uint64_t pk = x >> idx_bits;
+            pairs[below + p] = make_uint2((uint32_t)(pk >> bitsB), (uint32_t)(pk & maskB));
+            order[below + p] = rec;
+            offs[below + p] = carry + e;
+        }
+        carry += all;
+    }
+}
+
+uint32_t output_small_max() { return kOutSmallMax; }
 
 bool output_small_fits(uint32_t R, int key_bits) { return R <= kOutSmallMax && key_bits + bits_of_count(R) <= 63; }
 
 void launch_output_small(const uint64_t* rec_key, const uint2* rec_seg, const uint32_t* counters, uint32_t cap_rec,
-                         uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint64_t* runs, uint2* pairs,
+                         uint32_t cap_seg, int key_bits, int bitsB, bool want_segments, uint2* pairs,
                          uint32_t* order, uint32_t* offs, uint64_t* seg_out, int32_t* seg_exp_out, double* seg_f64,
                          uint32_t out_slots, const uint64_t* seg_pool, const int32_t* seg_exp,
                          const uint32_t* grid_flags, uint32_t* counters_out, hipStream_t s) {
-    uint32_t* cpre = (uint32_t*)(runs + kOutSmallMax);
-    k_output_runs<<<kRuns, RUN_THREADS, 0, s>>>(rec_key, rec_seg, counters, cap_rec, cap_seg, key_bits,
-                                                want_segments ? 1 : 0, runs, cpre);
-    k_output_place<<<kOutSmallMax / PLACE_THREADS, PLACE_THREADS, 0, s>>>(runs, cpre, counters, cap_rec, cap_seg,
-                                                                        key_bits, bitsB, pairs, order, offs,
-                                                                        counters_out);
+    k_output_buckets<<<kOutBuckets, OB_THREADS, 0, s>>>(rec_key, rec_seg, counters, cap_rec, cap_seg, key_bits, bitsB,
+                                                        want_segments ? 1 : 0, pairs, order, offs, counters_out);
     if (want_segments)
         k_gather_segments_small<<<kOutSmallMax / 256, 256, 0, s>>>(order, rec_seg, offs, counters, cap_rec, cap_seg,
                                                                    key_bits, out_slots, seg_pool, seg_exp, grid_flags,

@@ -207,7 +207,6 @@ extern "C" int gi_intersect_noalloc(int32_t device, const double* xyzA, uint32_t
     uint2* rec_seg = c.take<uint2>(cap_rec);
     uint64_t* seg_pool = c.take<uint64_t>(kSegPoolWords * (size_t)cap_seg);
     int32_t* seg_exp = c.take<int32_t>(cap_seg);
-    uint64_t* runs = c.take<uint64_t>(output_small_runs_elems());
     uint2* out_pairs = c.take<uint2>(cap_rec);
     uint32_t* order = c.take<uint32_t>(cap_rec);
     uint32_t* offs = c.take<uint32_t>((size_t)cap_rec + 1);
@@ -273,7 +272,7 @@ extern "C" int gi_intersect_noalloc(int32_t device, const double* xyzA, uint32_t
         launch_exact(A, B, exact, cap_exact, grid, bitsB, rec_key, rec_seg, cap_rec, seg_pool, seg_exp, cap_seg, cand,
                      cap_cand, slow, cap_exact, resid, cap_exact, counters, 1024, kinds, s);
         launch_output_small(rec_key, rec_seg, counters, cap_rec, cap_seg, bitsA + bitsB, bitsB, want_segments != 0,
-                            runs, out_pairs, order, offs, segs, sexp, f64, out_slots, seg_pool, seg_exp, grid, nullptr, s);
+                            out_pairs, order, offs, segs, sexp, f64, out_slots, seg_pool, seg_exp, grid, nullptr, s);
         NA_TRY(hipGetLastError());
         NA_TRY(hipMemcpy(hc, counters, sizeof(hc), hipMemcpyDeviceToHost));
         const bool is_grid = hc[CNT_COUNT] && hc[CNT_COUNT + 1];

@@ -33,12 +33,36 @@ def test_sphere_torus_small_grid():
 
 def test_result_output_one_workgroup():
     # 409,600 triangles per mesh give between 4,096 and 16,384 intersecting
-    # pairs: the small-result output (several 2,048-record runs sorted in LDS,
-    # records placed by their ranks in the other runs) sorts them and derives
-    # their segment offsets from the runs' count prefixes
+    # pairs: the small-result output (one launch: a workgroup per range of
+    # keys counts the records below its range, sorts its own in LDS and
+    # places them) sorts them and derives their segment offsets
     (xa, ta), (xb, tb) = mg.sphere_torus(640, 320)
     res, ref = _check(xa, ta, xb, tb, True)
     assert 4096 < len(res["pairs"]) <= 16384
+
+
+def test_result_output_skewed_to_one_key_range():
+    # every pair on face 0 of A (a large triangle in the plane z = 0; A's
+    # other 1,023 faces are small and far away) against a wall of 2 x 2,500
+    # triangles in the plane x = 1 crossing it: 5,000 pairs whose keys all
+    # share their top bits, so the small-result output sorts them all in one
+    # of its key ranges
+    n = 2500
+    big = np.array([[-8 * n, -8 * n, 0], [8 * n, -8 * n, 0], [0, 8 * n, 0]], dtype=np.float64)
+    k = np.arange(1023, dtype=np.float64)
+    far = np.stack([np.stack([1000 + 4 * k, np.zeros_like(k), np.full_like(k, 1000)], 1),
+                    np.stack([1002 + 4 * k, np.zeros_like(k), np.full_like(k, 1000)], 1),
+                    np.stack([1000 + 4 * k, np.full_like(k, 2), np.full_like(k, 1000)], 1)], 1).reshape(-1, 3)
+    xa = np.concatenate([big, far])
+    ta = np.arange(len(xa), dtype=np.uint32).reshape(-1, 3)
+    y = np.arange(n + 1, dtype=np.float64)
+    xb = np.concatenate([np.stack([np.ones_like(y), y, -np.ones_like(y)], 1),
+                         np.stack([np.ones_like(y), y, np.ones_like(y)], 1)])
+    j = np.arange(n, dtype=np.uint32)
+    lo0, lo1, hi0, hi1 = j, j + 1, j + n + 1, j + n + 2
+    tb = np.concatenate([np.stack([lo0, lo1, hi1], 1), np.stack([lo0, hi1, hi0], 1)]).astype(np.uint32)
+    res, ref = _check(xa, ta, xb, tb, True)
+    assert len(res["pairs"]) == 2 * n and np.all(res["pairs"][:, 0] == 0)
 
 
 def test_result_sort_multi_tile():
