@@ -36,6 +36,28 @@ namespace gi {
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
+// Lane (lane ^ M)'s copy of a float, every lane of the wave active: DPP
+// quad permutes for M = 1, 2, LDS-free swizzles within 32 lanes for M = 4 ..
+// 16, a permute through the LDS crossbar for 32.
+template <int M>
+__device__ __forceinline__ float lane_xor_f32(float x) {
+    int v = __float_as_int(x);
+    if constexpr (M == 1) v = __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad_perm [1, 0, 3, 2]
+    else if constexpr (M == 2) v = __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // [2, 3, 0, 1]
+    else if constexpr (M < 32) v = __builtin_amdgcn_ds_swizzle(v, (M << 10) | 0x1F);  // bit mode: lane ^ M
+    else v = __shfl_xor(v, M);
+    return __int_as_float(v);
+}
+// one butterfly step of a box (lo[3], hi[3]) at lane distance M: each lane
+// then holds the union of its aligned block of 2M lanes
+template <int M>
+__device__ __forceinline__ void box_xor_step(float* b) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) b[c] = fminf(b[c], lane_xor_f32<M>(b[c]));
+#pragma unroll
+    for (int c = 3; c < 6; ++c) b[c] = fmaxf(b[c], lane_xor_f32<M>(b[c]));
+}
+
 // Runs of consecutive valid lanes of a wave with the same slot: one LDS
 // atomic per run instead of one per lane (spatially coherent keys put long
 // runs of one slot in a wave, and same-address LDS atomics serialise). A run
@@ -575,11 +597,14 @@ __device__ __forceinline__ void box_absorb(const float4* __restrict__ e, size_t 
     b[3] = fmaxf(b[3], h.x); b[4] = fmaxf(b[4], h.y); b[5] = fmaxf(b[5], h.z);
 }
 
+// the union of the wave's boxes on every lane (every lane active)
 __device__ __forceinline__ void wave_box_reduce(float* b) {
-    for (int o = 32; o > 0; o >>= 1) {
-        for (int k = 0; k < 3; ++k) b[k] = fminf(b[k], __shfl_xor(b[k], o));
-        for (int k = 3; k < 6; ++k) b[k] = fmaxf(b[k], __shfl_xor(b[k], o));
-    }
+    box_xor_step<32>(b);
+    box_xor_step<16>(b);
+    box_xor_step<8>(b);
+    box_xor_step<4>(b);
+    box_xor_step<2>(b);
+    box_xor_step<1>(b);
 }
 
 // ------------------------------------------------------ forests (batches)
@@ -1659,21 +1684,28 @@ __global__ __launch_bounds__(CHUNK) void k_build_chunks(ChunkJobs<K> J) {
     const float leaf[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
     sbox[2 * t] = make_float4(b[0], b[1], b[2], 0.f);
     sbox[2 * t + 1] = make_float4(b[3], b[4], b[5], 0.f);
-    // aligned 2^k-leaf blocks by butterfly reduction within the wave: after
-    // the exchange at distance 2^(k-1) a lane holds the union of its block
-#pragma unroll
-    for (int k = 1; k <= 6; ++k) {
-        const int o = 1 << (k - 1);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) b[c] = fminf(b[c], __shfl_xor(b[c], o));
-#pragma unroll
-        for (int c = 3; c < 6; ++c) b[c] = fmaxf(b[c], __shfl_xor(b[c], o));
+    // aligned 2^k-leaf blocks by butterfly reduction within the wave (all
+    // lanes active): after the exchange at distance 2^(k-1) a lane holds the
+    // union of its block
+    auto level_box = [&](int k) {
         if ((t & ((1 << k) - 1)) == 0) {
             const int e = 2 * (block_table_offset(CHUNK, k) + (t >> k));
             sbox[e] = make_float4(b[0], b[1], b[2], 0.f);
             sbox[e + 1] = make_float4(b[3], b[4], b[5], 0.f);
         }
-    }
+    };
+    box_xor_step<1>(b);
+    level_box(1);
+    box_xor_step<2>(b);
+    level_box(2);
+    box_xor_step<4>(b);
+    level_box(3);
+    box_xor_step<8>(b);
+    level_box(4);
+    box_xor_step<16>(b);
+    level_box(5);
+    box_xor_step<32>(b);
+    level_box(6);
     // the 64-leaf block is the first level of the block hierarchy
     if ((t & 63) == 0 && jl < ce) {
         level1[2 * (size_t)(jl >> 6)] = make_float4(b[0], b[1], b[2], 0.f);
