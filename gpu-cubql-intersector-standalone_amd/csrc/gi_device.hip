@@ -1275,7 +1275,8 @@ constexpr int MSD_THREADS = 256, MSD_ITEMS = 8, MSDB_ITEMS = 16;
 constexpr uint32_t MSD_TILE = MSD_THREADS * MSD_ITEMS;
 static_assert(MSD_THREADS == SCAN_THREADS, "block_exclusive_sum's block");
 static_assert(kMsdBuckets == 16 * MSD_THREADS, "16 buckets per thread in the scans");
-static_assert(kMsdSlice == MSD_TILE, "a slice is one tile of items");
+constexpr int SLICE_ITEMS = (int)(kMsdSlice / MSD_THREADS);  // a slice's keys per thread
+static_assert(kMsdSlice % MSD_THREADS == 0, "whole items per thread in a slice");
 static_assert(kMsdBigSlots < 255 && 33 <= kMsdOffDone - kMsdOffTab, "slot table");
 constexpr uint32_t kMsdNoSlot = 0xFFFFFFFFu;
 
@@ -1454,15 +1455,15 @@ __device__ void msd_slice(const MsdSort& S, uint32_t e, uint32_t* cnt) {
         for (uint32_t d = t; d < kMsdBuckets; d += MSD_THREADS) cnt[msd_pad(d)] = 0u;
         __syncthreads();
         const uint32_t j0 = g * kMsdSlice, j1 = j0 + kMsdSlice < size ? j0 + kMsdSlice : size;
-        uint32_t k[MSD_ITEMS], v[MSD_ITEMS];
+        uint32_t k[SLICE_ITEMS], v[SLICE_ITEMS];
 #pragma unroll
-        for (int it = 0; it < MSD_ITEMS; ++it) {
+        for (int it = 0; it < SLICE_ITEMS; ++it) {
             const uint32_t j = j0 + it * MSD_THREADS + t;
             k[it] = j < j1 ? S.tk[start + j] : 0u;
             v[it] = j < j1 ? S.tv[start + j] : 0u;
         }
 #pragma unroll
-        for (int it = 0; it < MSD_ITEMS; ++it)
+        for (int it = 0; it < SLICE_ITEMS; ++it)
             if (j0 + it * MSD_THREADS + t < j1) atomicAdd(&cnt[msd_pad(k[it] & mask)], 1u);
         __syncthreads();
         {  // thread t: low parts 16t .. 16t + 15 of the bucket's histogram
@@ -1482,7 +1483,7 @@ __device__ void msd_slice(const MsdSort& S, uint32_t e, uint32_t* cnt) {
         }
         __syncthreads();
 #pragma unroll
-        for (int it = 0; it < MSD_ITEMS; ++it)
+        for (int it = 0; it < SLICE_ITEMS; ++it)
             if (j0 + it * MSD_THREADS + t < j1) {
                 const uint32_t p = atomicAdd(&cnt[msd_pad(k[it] & mask)], 1u);
                 if (p < S.n) {
