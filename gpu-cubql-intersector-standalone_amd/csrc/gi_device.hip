@@ -3585,7 +3585,7 @@ struct RunBitonicSort<1> {
     __device__ static __forceinline__ void run(uint64_t&, uint64_t&, uint64_t*, uint32_t, bool) {}
 };
 
-// segments of the sorted records, for the records k_output_place ordered
+// segments of the sorted records, for the records k_output_buckets ordered
 __global__ __launch_bounds__(256) void k_gather_segments_small(const uint32_t* __restrict__ order,
                                                                const uint2* __restrict__ rec_seg,
                                                                const uint32_t* __restrict__ seg_offsets,
