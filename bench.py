@@ -70,10 +70,11 @@ def kernel_bytes(name, st, ntA, ntB, nvA, nvB):
     if name == "k_stage_mesh":
         # both meshes in one launch, neither moved (identity transform, source
         # already on the integer grid): the launch writes no vertex copies and
-        # reads each source vertex (xyz 24 B) once through the triangles that
-        # use it; per triangle: indices 12 B read, box 32 B + Morton key and
-        # id 8 B written (the sort's bucket counts are atomics on 16 KB)
-        return 24.0 * (nvA + nvB) + 52.0 * (ntA + ntB)
+        # reads each source vertex once through the triangles that use it, as
+        # the 16-byte integer copy made when the mesh was first staged; per
+        # triangle: indices 12 B read, box 32 B + Morton key and id 8 B written
+        # (the sort's bucket counts are atomics on 16 KB)
+        return 16.0 * (nvA + nvB) + 52.0 * (ntA + ntB)
     if name == "k_build_chunks":
         # per leaf of A: sorted key 4 B + triangle id 4 B + its FP32 box 32 B
         # read; leaf and internal node 32 B each + leaf range 8 B written

@@ -91,10 +91,17 @@ struct MeshSlot {
     uint64_t generation = 0;   // bumped whenever the slot's mesh or buffers are replaced
     DevBuf src, tri, v64, vi, tbox, keys, vals, tkeys, tvals, nodes, range, blocks;
     DevBuf voff, toff, vtiles, ttiles, vtile0, fbounds, roots;
+    // the source vertices as integers (an owned slot whose source is on the
+    // grid): made on the first staging after an upload, dropped with the
+    // source; the unmoved mesh's kernels read it instead of the doubles
+    DevBuf vsrc;
+    bool vsrc_ok = false;
     void release() {
         ++generation;
         DevBuf* forest_bufs[] = {&voff, &toff, &vtiles, &ttiles, &vtile0, &fbounds, &roots};
         for (DevBuf* b : forest_bufs) b->release();
+        vsrc.release();
+        vsrc_ok = false;
         DevBuf* all[] = {&src, &tri, &v64, &vi, &tbox, &keys, &vals, &tkeys, &tvals, &nodes, &range, &blocks};
         for (DevBuf* b : all) {
             if (borrowed) { b->p = nullptr; b->bytes = 0; }
@@ -237,7 +244,7 @@ struct gi_context {
         for (auto& m : mesh) m.release();
         DevBuf* all[] = {&counters, &hist, &scan_tmp, &partials, &inspect, &cand, &exact, &rec_key, &rec_seg, &seg_pool,
                          &seg_exp, &resid, &order, &tkeys, &tvals, &pairs, &nseg, &offs, &segs, &f64, &exp2, &pair_off,
-                         &bhist[0], &bhist[1], &bscan[0], &bscan[1], &bpart[0], &bpart[1]};
+                         &bhist[0], &bhist[1], &bscan[0], &bscan[1], &bpart[0], &bpart[1], &msd};
         for (DevBuf* b : all) b->release();
         cap_cand = cap_exact = cap_rec = cap_seg = 0;
     }
@@ -366,9 +373,15 @@ void inspect_upload(gi_context* ctx, int slot, bool tri_local) {
         m.src_hi[k] = h.hi[k];
     }
     m.src_grid = h.off_grid ? 0 : 1;
+    m.vsrc_ok = false;  // new source vertices
     m.on_device = true;
     m.built = false;
     m.dirty = true;
+}
+
+// the integer source copy the unmoved kernels read, when it applies
+const int4* source_ints(const MeshSlot& m) {
+    return (!m.moved_copy && m.vsrc_ok && !m.borrowed && !m.forest) ? m.vsrc.get<int4>() : nullptr;
 }
 
 // A batch: the same stages over all its meshes at once, one launch each.
@@ -468,11 +481,16 @@ bool stage_mesh(gi_context* ctx, int slot, bool rebuild, StageJob* job) {
     // needs no moved copies: no vertex blocks, the grid flag is the upload's
     // classification (set by the query's init), the kernels read m.src
     m.moved_copy = !(x.identity && (!x.snap || m.src_grid == 1) && m.src_grid >= 0);
+    if (!m.moved_copy && m.src_grid == 1 && !m.borrowed && !m.forest && !m.vsrc_ok) {
+        if (ctx->capturing) throw GiError{GI_ERR_INTERNAL, "integer source copy made during a capture"};
+        launch_src_int4(m.src.get<double>(), m.nv, m.vsrc.ensure<int4>(m.nv), s);
+        m.vsrc_ok = true;
+    }
     *job = StageJob{m.src.get<double>(), m.moved_copy ? m.nv : 0u, m.tri.get<uint32_t>(), m.nt, d_xf, v64, vi,
                     grid + slot,
                     m.morton_axis, tbox, topology ? keys : nullptr, topology ? vals : nullptr,
                     hist0 ? hist : nullptr, !m.moved_copy && m.src_grid == 0 ? 1 : 0, words,
-                    msd ? bits - kMsdBits : 0};
+                    msd ? bits - kMsdBits : 0, source_ints(m)};
     // the Morton sort runs after the staging, together with the other slot's
     if (msd)
         ctx->pending_msd[slot] = MsdSortJob{keys, vals, tk, tv, m.nt, bits, words};
@@ -603,7 +621,7 @@ MeshDev mesh_dev(const MeshSlot& m) {
     d.nt = m.nt;
     d.src = m.src.get<double>();
     d.v64 = m.moved_copy ? m.v64.get<double4>() : nullptr;
-    d.vi = m.moved_copy ? m.vi.get<int4>() : nullptr;
+    d.vi = m.moved_copy ? m.vi.get<int4>() : source_ints(m);
     d.tri = m.tri.get<uint32_t>();
     d.nodes = m.nodes.get<float4>();
     d.order = m.sorted_in_tmp ? m.tvals.get<uint32_t>() : m.vals.get<uint32_t>();

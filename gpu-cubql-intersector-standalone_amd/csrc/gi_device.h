@@ -74,7 +74,8 @@ struct MeshDev {
     uint32_t nv, nt;
     const double* src;   // source vertices (xyz), used when v64 is null (an unmoved mesh)
     const double4* v64;  // moved vertices used by the predicates, or null
-    const int4* vi;      // their integer copy (valid when the mesh is on the grid), null with v64
+    const int4* vi;      // their integer copy (valid when the mesh is on the grid); with v64 null, the source
+                         // vertices' integer copy of an unmoved mesh on the grid, or null
     const uint32_t* tri; // 3 * nt
     const float4* nodes; // BVH, 2 float4 per node, 2*nt - 1 nodes (a walked mesh only)
     const uint32_t* order;  // triangle ids in Morton order (the sorted values)
@@ -139,7 +140,13 @@ struct StageJob {
     // query's init); null: none
     uint32_t* msd_total;
     int msd_shift;
+    // an unmoved mesh on the grid: its source vertices as integers (one
+    // 16-byte load per vertex instead of three 8-byte ones), or null
+    const int4* vsrc;
 };
+// vsrc[i] = source vertex i as integers, for a mesh whose source vertices
+// are all on the grid (integers within 2^26)
+void launch_src_int4(const double* src, uint32_t nv, int4* vsrc, hipStream_t s);
 // Up to two meshes staged in one launch.
 void launch_stage_meshes(const StageJob* jobs, int njobs, hipStream_t s);
 

@@ -523,12 +523,24 @@ __global__ __launch_bounds__(256) void k_stage_mesh(StageJobs J) {
         for (int c = 0; c < 3; ++c) id[it][c] = i < q.nt ? tri[3 * (size_t)i + c] : 0u;
     }
     double raw[SORT_ITEMS][3][3];
+    if (q.vsrc) {  // an unmoved mesh on the grid: one 16-byte load per vertex
 #pragma unroll
-    for (int it = 0; it < SORT_ITEMS; ++it)
+        for (int it = 0; it < SORT_ITEMS; ++it)
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+            for (int c = 0; c < 3; ++c) {
+                const int4 v = q.vsrc[id[it][c]];
+                raw[it][c][0] = (double)v.x;
+                raw[it][c][1] = (double)v.y;
+                raw[it][c][2] = (double)v.z;
+            }
+    } else {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) raw[it][c][k] = src[3 * (size_t)id[it][c] + k];
+        for (int it = 0; it < SORT_ITEMS; ++it)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) raw[it][c][k] = src[3 * (size_t)id[it][c] + k];
+    }
     uint32_t bucket[SORT_ITEMS] = {};  // bucket sort: the items' buckets
 #pragma unroll
     for (int it = 0; it < SORT_ITEMS; ++it) {
@@ -568,6 +580,19 @@ __global__ __launch_bounds__(256) void k_stage_mesh(StageJobs J) {
         for (int d = threadIdx.x; d < (int)kMsdBuckets; d += blockDim.x)
             if (mb[d]) atomicAdd(&q.msd_total[d], mb[d]);
     }
+}
+
+__global__ __launch_bounds__(256) void k_src_int4(const double* __restrict__ src, uint32_t nv, int4* __restrict__ vsrc) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
+        const double* p = src + 3 * (size_t)i;
+        vsrc[i] = make_int4((int)p[0], (int)p[1], (int)p[2], 0);
+    }
+}
+
+void launch_src_int4(const double* src, uint32_t nv, int4* vsrc, hipStream_t s) {
+    if (nv == 0) return;
+    k_src_int4<<<std::min<uint32_t>(blocks_for(nv, 256), 4096u), 256, 0, s>>>(src, nv, vsrc);
+    GI_LAUNCH_CHECK();
 }
 
 void launch_stage_meshes(const StageJob* jobs, int njobs, hipStream_t s) {
@@ -2059,10 +2084,15 @@ void launch_forest_roots(const uint2* range, const float4* nodes, uint32_t n, co
 
 
 // Vertex i of a mesh as the predicates see it: the moved copy, or for an
-// unmoved mesh (no copy written) the source doubles themselves; as integers
-// for grid meshes (exact: integer values within 2^26).
+// unmoved mesh (no copy written) the source vertices themselves (their
+// integer copy when they are on the grid); as integers for grid meshes
+// (exact: integer values within 2^26).
 __device__ __forceinline__ double4 mesh_vertex(const MeshDev& M, uint32_t i) {
     if (M.v64) return M.v64[i];
+    if (M.vi) {  // an unmoved mesh on the grid: its integer copy holds the same values
+        const int4 v = M.vi[i];
+        return make_double4((double)v.x, (double)v.y, (double)v.z, 0.0);
+    }
     const double* p = M.src + 3 * (size_t)i;
     return make_double4(p[0], p[1], p[2], 0.0);
 }
