@@ -61,6 +61,32 @@ def test_graph_replay_with_rebuild_every_step():
         eng.close()
 
 
+def test_graph_replays_with_split_bucket_sorts():
+    # 2^20 triangles a side: the bucket sort splits the sphere's pole cells
+    # over slice workgroups, whose slots each sort must leave cleared; graph
+    # replays that rebuild, with eager queries between them, must match the
+    # eager result and leave the same sorted keys
+    (xa, ta), (xb, tb) = mg.sphere_torus(1024, 512)
+    eng = gi.Intersector()
+    try:
+        eng.set_mesh(0, xa, ta)
+        eng.set_mesh(1, xb, tb)
+        ref = eng.intersect(rebuild=True)
+        keys = eng.get_tree(0, len(ta), tree=False)["keys"]
+        replays = 0
+        for i in range(4):
+            r = eng.intersect(rebuild=True, use_graph=True)
+            replays += r["stats"]["graph"]
+            np.testing.assert_array_equal(r["pairs"], ref["pairs"], err_msg=f"replay {i}")
+            assert r["segments"] == ref["segments"], f"replay {i}"
+            np.testing.assert_array_equal(eng.get_tree(0, len(ta), tree=False)["keys"], keys)
+            e = eng.intersect(rebuild=True)
+            np.testing.assert_array_equal(e["pairs"], ref["pairs"], err_msg=f"eager after replay {i}")
+        assert replays >= 2 and len(ref["pairs"]) > 10000
+    finally:
+        eng.close()
+
+
 def test_transform_switched_back_between_eager_queries_and_replays():
     """The device copy of a slot's transform is only refreshed when the staged
     transform changes. Switching between two transforms, eager and replayed
